@@ -1,0 +1,368 @@
+// C-ABI of the gfx950 LAPW H+O setup: context management, SCF-data uploads and the per-k
+// entry points declared in include/exc_lapw.h.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "exc_common.h"
+
+struct exc_ctx {
+  exc_dims dims;
+  int device;
+  hipStream_t stream;
+  exc_dev d;
+  bool have_atoms, have_lo, have_radial, have_mt, have_gvec;
+  int radial_info;         // > 0: first singular matching matrix (1-based), as zgesv's info
+  int mt_variant;          // tile variant of the MT stage (-1: chosen per call)
+  int y_variant;
+  // grow-only workspaces
+  cplx *yt;
+  size_t yt_bytes;
+  // staging for the host-pointer entry points
+  void *stage;
+  size_t stage_bytes;
+  std::vector<void *> owned;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(exc_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return EXC_OK;
+  void *q = nullptr;
+  EXC_CHECK(hipMalloc(&q, n * sizeof(T)));
+  c->owned.push_back(q);
+  *p = (T *)q;
+  return EXC_OK;
+}
+
+template <typename T>
+int dev_upload(exc_ctx *c, T **p, const void *src, size_t n) {
+  if (*p == nullptr) {
+    int rc = dev_alloc(c, p, n);
+    if (rc) return rc;
+  }
+  if (n) EXC_CHECK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return EXC_OK;
+}
+
+int grow(void **p, size_t *have, size_t need) {
+  if (need <= *have) return EXC_OK;
+  if (*p) EXC_CHECK(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  EXC_CHECK(hipMalloc(p, need));
+  *have = need;
+  return EXC_OK;
+}
+
+int use_device(exc_ctx *c) {
+  int cur = -1;
+  EXC_CHECK(hipGetDevice(&cur));
+  if (cur != c->device) EXC_CHECK(hipSetDevice(c->device));
+  return EXC_OK;
+}
+
+bool ready(const exc_ctx *c) { return c && c->have_atoms && c->have_lo && c->have_radial && c->have_mt && c->have_gvec; }
+
+}  // namespace
+
+extern "C" {
+
+int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
+  if (!dims || !out) return EXC_ERR_ARG;
+  *out = nullptr;
+  const exc_dims &m = *dims;
+  if (m.natmtot <= 0 || m.nspecies <= 0 || m.lmaxapw < 0 || m.ngrtot <= 0 || m.ngvec <= 0 ||
+      m.ngvec > m.ngrtot || m.omega <= 0.0 || m.nlomax < 0 || m.maxnlo < 0)
+    return EXC_ERR_ARG;
+  if (m.apwordmax != 2 || m.lmaxmat != m.lmaxapw || m.lmaxapw > 12 || m.lmaxapw < 1) return EXC_ERR_UNSUPPORTED;
+  for (int i = 0; i < 3; ++i)
+    if (m.intgv[3 + i] < m.intgv[i]) return EXC_ERR_ARG;
+  exc_ctx *c = new exc_ctx();
+  c->dims = m;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return EXC_ERR_HIP;
+  }
+  c->stream = nullptr;
+  memset(&c->d, 0, sizeof(c->d));
+  exc_dev &d = c->d;
+  d.natm = m.natmtot;
+  d.nspecies = m.nspecies;
+  d.lmaxapw = m.lmaxapw;
+  d.lmaxmat = m.lmaxmat;
+  d.apwordmax = m.apwordmax;
+  d.lmmaxapw = (m.lmaxapw + 1) * (m.lmaxapw + 1);
+  d.naa = m.apwordmax * (m.lmaxmat + 1) * (m.lmaxmat + 1);
+  d.maxnlo = m.maxnlo;
+  d.nlomax = m.nlomax;
+  d.ngrtot = m.ngrtot;
+  d.ngvec = m.ngvec;
+  for (int i = 0; i < 3; ++i) {
+    d.box_lo[i] = m.intgv[i];
+    d.box_n[i] = m.intgv[3 + i] - m.intgv[i] + 1;
+  }
+  d.omega = m.omega;
+  c->mt_variant = -1;
+  c->y_variant = -1;
+  *out = c;
+  return EXC_OK;
+}
+
+int exc_ctx_destroy(exc_ctx *c) {
+  if (!c) return EXC_OK;
+  use_device(c);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  else hipDeviceSynchronize();
+  for (void *p : c->owned) hipFree(p);
+  if (c->yt) hipFree(c->yt);
+  if (c->stage) hipFree(c->stage);
+  delete c;
+  return EXC_OK;
+}
+
+int exc_ctx_set_stream(exc_ctx *c, void *stream) {
+  if (!c) return EXC_ERR_ARG;
+  c->stream = (hipStream_t)stream;
+  return EXC_OK;
+}
+
+int exc_ctx_nlotot(const exc_ctx *c) { return c ? c->d.nlotot : EXC_ERR_ARG; }
+
+int exc_ctx_set_atoms(exc_ctx *c, const int *natoms, const double *rmt, const double *atposc) {
+  if (!c || !natoms || !rmt || !atposc) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int ns = c->dims.nspecies, na = c->dims.natmtot;
+  std::vector<int> sp;
+  for (int s = 0; s < ns; ++s) {
+    if (natoms[s] < 0 || rmt[s] <= 0.0) return EXC_ERR_ARG;
+    for (int a = 0; a < natoms[s]; ++a) sp.push_back(s);
+  }
+  if ((int)sp.size() != na) return EXC_ERR_ARG;
+  if ((rc = dev_upload(c, &c->d.spidx, sp.data(), na))) return rc;
+  if ((rc = dev_upload(c, &c->d.rmt, rmt, ns))) return rc;
+  if ((rc = dev_upload(c, &c->d.atposc, atposc, 3 * (size_t)na))) return rc;
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->have_atoms = true;
+  return EXC_OK;
+}
+
+int exc_ctx_set_lo(exc_ctx *c, const int *nlorb, const int *lorbl, const int *idxlo) {
+  if (!c || !nlorb || (c->dims.nlomax > 0 && (!lorbl || !idxlo))) return EXC_ERR_ARG;
+  if (!c->have_atoms) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int ns = c->dims.nspecies, na = c->dims.natmtot, nlomax = c->dims.nlomax, lolm = c->dims.lolmmax;
+  std::vector<int> sp(na);
+  EXC_CHECK(hipMemcpy(sp.data(), c->d.spidx, sizeof(int) * na, hipMemcpyDeviceToHost));
+  std::vector<lo_desc> lod;
+  int maxlos = 0;
+  for (int ias = 0; ias < na; ++ias) {
+    const int is = sp[ias];
+    if (nlorb[is] < 0 || nlorb[is] > nlomax) return EXC_ERR_ARG;
+    const int first = (int)lod.size();
+    for (int ilo = 0; ilo < nlorb[is]; ++ilo) {
+      const int l = lorbl[ilo + nlomax * is];
+      if (l < 0 || (l + 1) * (l + 1) > lolm) return EXC_ERR_ARG;
+      for (int m = -l; m <= l; ++m) {
+        lo_desc q;
+        q.ias = ias;
+        q.row = (int)lod.size() - first;
+        q.ilo = ilo;
+        q.l = l;
+        q.m = m;
+        q.lm = l * (l + 1) + m;
+        q.first = first;
+        q.pad = 0;
+        // the setup routines assume the running numbering of genidxlo.f90:24-43
+        if (idxlo[q.lm + lolm * (ilo + nlomax * ias)] != (int)lod.size() + 1) return EXC_ERR_ARG;
+        lod.push_back(q);
+      }
+    }
+    maxlos = (int)lod.size() - first > maxlos ? (int)lod.size() - first : maxlos;
+  }
+  if (maxlos > c->dims.maxnlo) return EXC_ERR_ARG;
+  c->d.nlotot = (int)lod.size();
+  if (c->d.lod == nullptr && !lod.empty()) {
+    if ((rc = dev_alloc(c, &c->d.lod, lod.size()))) return rc;
+  }
+  if (!lod.empty()) EXC_CHECK(hipMemcpy(c->d.lod, lod.data(), sizeof(lo_desc) * lod.size(), hipMemcpyHostToDevice));
+  c->have_lo = true;
+  return EXC_OK;
+}
+
+int exc_ctx_set_radial(exc_ctx *c, const double *dmat) {
+  if (!c || !dmat) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int na = c->dims.natmtot, L = c->dims.lmaxapw, ao = c->dims.apwordmax;
+  std::vector<double> inv((size_t)na * (L + 1) * 4);
+  int info = 0;
+  for (int ias = 0; ias < na; ++ias)
+    for (int l = 0; l <= L; ++l) {
+      const double *m = dmat + (size_t)ao * ao * (l + (L + 1) * ias);
+      // m column-major 2x2: m[0]=D11 m[1]=D21 m[2]=D12 m[3]=D22
+      const double det = m[0] * m[3] - m[2] * m[1];
+      double *o = inv.data() + 4 * (l + (L + 1) * (size_t)ias);
+      if (det == 0.0 || !isfinite(det)) {
+        if (!info) info = 1 + l + (L + 1) * ias;
+        o[0] = o[1] = o[2] = o[3] = 0.0;
+      } else {
+        o[0] = m[3] / det;
+        o[1] = -m[1] / det;
+        o[2] = -m[2] / det;
+        o[3] = m[0] / det;
+      }
+    }
+  if ((rc = dev_upload(c, &c->d.dinv, inv.data(), inv.size()))) return rc;
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->radial_info = info;
+  c->have_radial = true;
+  return info;
+}
+
+int exc_ctx_set_mt(exc_ctx *c, const double *haa, const double *loa, const double *lolo, const double *oalo,
+                   const double *ololo) {
+  if (!c || !haa) return EXC_ERR_ARG;
+  if (c->dims.maxnlo > 0 && (!loa || !lolo || !oalo || !ololo)) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t na = c->dims.natmtot, naa = c->d.naa, mn = c->dims.maxnlo, nl = c->dims.nlomax, ao = c->dims.apwordmax;
+  if ((rc = dev_upload(c, &c->d.haa, haa, naa * naa * na))) return rc;
+  if (mn) {
+    if ((rc = dev_upload(c, &c->d.loa, loa, mn * naa * na))) return rc;
+    if ((rc = dev_upload(c, &c->d.lolo, lolo, mn * mn * na))) return rc;
+  }
+  if (nl) {
+    if ((rc = dev_upload(c, &c->d.oalo, oalo, ao * nl * na))) return rc;
+    if ((rc = dev_upload(c, &c->d.ololo, ololo, nl * nl * na))) return rc;
+  }
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->have_mt = true;
+  return EXC_OK;
+}
+
+int exc_ctx_set_gvec(exc_ctx *c, const int *ivg, const int *ivgig, const double *cfunig, const double *veffig,
+                     const double *ffacg) {
+  if (!c || !ivg || !ivgig || !cfunig || !veffig) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t ng = c->dims.ngrtot, nv = c->dims.ngvec;
+  const size_t box = (size_t)c->d.box_n[0] * c->d.box_n[1] * c->d.box_n[2];
+  for (size_t i = 0; i < box; ++i)
+    if (ivgig[i] < 0 || (size_t)ivgig[i] > ng) return EXC_ERR_ARG;
+  for (size_t g = 0; g < ng; ++g)
+    for (int q = 0; q < 3; ++q) {
+      const int v = ivg[3 * g + q];
+      if (v < c->d.box_lo[q] || v >= c->d.box_lo[q] + c->d.box_n[q]) return EXC_ERR_ARG;
+    }
+  if ((rc = dev_upload(c, &c->d.ivg, ivg, 3 * ng))) return rc;
+  if ((rc = dev_upload(c, &c->d.ivgig, ivgig, box))) return rc;
+  if ((rc = dev_upload(c, &c->d.cfunig, cfunig, ng))) return rc;
+  if ((rc = dev_upload(c, &c->d.veffig, veffig, nv))) return rc;
+  if (ffacg) {
+    // reorder ffacg(ngvec, nspecies) -> contiguous per species (same layout)
+    if ((rc = dev_upload(c, &c->d.ffacg, ffacg, nv * c->dims.nspecies))) return rc;
+  }
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->have_gvec = true;
+  return EXC_OK;
+}
+
+int exc_match_dev(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *tpgpc, const double *sfacgp,
+                  double *apwalm) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (ngp < 0 || ngkmax < ngp || (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  rc = exc_launch_match(c->d, ngp, gpc, tpgpc, (const cplx *)sfacgp, ngkmax, (cplx *)apwalm, ngkmax, c->stream);
+  if (rc) return rc;
+  return c->radial_info;
+}
+
+int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *tpgpc, const double *sfacgp,
+              double *apwalm) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (ngp < 0 || ngkmax < ngp || (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t na = c->d.natm, lmm = c->d.lmmaxapw, ao = c->d.apwordmax, nk = ngkmax;
+  const size_t b_gpc = 8 * nk, b_tp = 16 * nk, b_sf = 16 * nk * na, b_apw = 16 * nk * ao * lmm * na;
+  if ((rc = grow(&c->stage, &c->stage_bytes, b_gpc + b_tp + b_sf + b_apw))) return rc;
+  char *base = (char *)c->stage;
+  double *d_gpc = (double *)base;
+  double *d_tp = (double *)(base + b_gpc);
+  double *d_sf = (double *)(base + b_gpc + b_tp);
+  double *d_apw = (double *)(base + b_gpc + b_tp + b_sf);
+  EXC_CHECK(hipMemcpyAsync(d_gpc, gpc, b_gpc, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(d_tp, tpgpc, b_tp, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(d_sf, sfacgp, b_sf, hipMemcpyHostToDevice, c->stream));
+  // rows ngp+1..ngkmax are not written by match: carry the caller's values through
+  EXC_CHECK(hipMemcpyAsync(d_apw, apwalm, b_apw, hipMemcpyHostToDevice, c->stream));
+  rc = exc_match_dev(c, ngp, ngkmax, d_gpc, d_tp, d_sf, d_apw);
+  if (rc < 0) return rc;
+  EXC_CHECK(hipMemcpyAsync(apwalm, d_apw, b_apw, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                                    const double *vgpc, int ldh, double *h, double *o) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  const int nmat = ngp + c->d.nlotot;
+  if (ngp < 0 || ngkmax < ngp || ldh < nmat || !h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+    return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if (ngp > 0) {
+    const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
+    if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
+    const int yv = c->y_variant >= 0 ? c->y_variant : (ngp >= 2048 ? 0 : 1);
+    const int mv = c->mt_variant >= 0 ? c->mt_variant : (ngp >= 2048 ? 0 : 1);
+    if ((rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream))) return rc;
+    if ((rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o,
+                            ldh, mv, c->stream)))
+      return rc;
+  }
+  if ((rc = exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream))) return rc;
+  return EXC_OK;
+}
+
+int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                                const double *vgpc, int ldh, double *h, double *o) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  const int nmat = ngp + c->d.nlotot;
+  if (ngp < 0 || ngkmax < ngp || ldh < nmat || !h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+    return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t na = c->d.natm, lmm = c->d.lmmaxapw, ao = c->d.apwordmax, nk = ngkmax;
+  const size_t b_apw = 16 * nk * ao * lmm * na, b_ig = 4 * nk, b_vg = 24 * nk, b_h = 16 * (size_t)ldh * nmat;
+  const size_t o_ig = (b_apw + 255) & ~(size_t)255, o_vg = o_ig + ((b_ig + 255) & ~(size_t)255);
+  const size_t o_h = o_vg + ((b_vg + 255) & ~(size_t)255), o_o = o_h + ((b_h + 255) & ~(size_t)255);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_o + b_h))) return rc;
+  char *base = (char *)c->stage;
+  EXC_CHECK(hipMemcpyAsync(base, apwalm, b_apw, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_ig, igpig, b_ig, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_vg, vgpc, b_vg, hipMemcpyHostToDevice, c->stream));
+  if (ldh > nmat) {
+    // rows nmat+1..ldh of the caller's matrices are not part of the result: keep them
+    EXC_CHECK(hipMemcpyAsync(base + o_h, h, b_h, hipMemcpyHostToDevice, c->stream));
+    EXC_CHECK(hipMemcpyAsync(base + o_o, o, b_h, hipMemcpyHostToDevice, c->stream));
+  }
+  rc = exc_hamiltonandoverlapsetup_dev(c, ngp, ngkmax, (const double *)base, (const int *)(base + o_ig),
+                                       (const double *)(base + o_vg), ldh, (double *)(base + o_h),
+                                       (double *)(base + o_o));
+  if (rc) return rc;
+  EXC_CHECK(hipMemcpyAsync(h, base + o_h, b_h, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(o, base + o_o, b_h, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return EXC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Shared definitions of the gfx950 LAPW H+O setup library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/exc_lapw.h"
+
+#define EXC_CHECK(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      fprintf(stderr, "exc_lapw: HIP error %s at %s:%d\n", hipGetErrorString(e_),   \
+              __FILE__, __LINE__);                                                   \
+      return EXC_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+typedef double __attribute__((ext_vector_type(4))) d4_t;
+
+struct cplx {
+  double re, im;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+
+// Per-local-orbital description (one entry per lo column, 0..nlotot-1), built on the
+// host from idxlo/lorbl (genidxlo.f90:24-43).
+struct lo_desc {
+  int ias;     // atom
+  int row;     // row inside the atom's loa/lolo block (0..losize-1)
+  int ilo;     // local-orbital index inside the species (0-based)
+  int l, m;    // angular momentum of the function
+  int lm;      // l(l+1)+m, 0-based
+  int first;   // global lo index of the atom's first lo function
+  int pad;
+};
+
+// Device-resident SCF data of one context.
+struct exc_dev {
+  int natm, nspecies, lmaxapw, lmaxmat, apwordmax, lmmaxapw, naa;
+  int nlotot, maxnlo, nlomax;
+  int ngrtot, ngvec;
+  int box_lo[3], box_n[3];
+  double omega;
+  int *spidx;          // [natm]
+  double *rmt;         // [nspecies]
+  double *atposc;      // [3*natm]
+  double *dinv;        // [natm][lmax+1][2][2]: inverse of the 2x2 matching matrix
+  int *dsing;          // [natm*(lmax+1)]: 1 where the matching matrix is singular
+  cplx *haa;           // [natm][naa][naa] column-major per atom
+  cplx *loa;           // [natm][naa][maxnlo]
+  cplx *lolo;          // [natm][maxnlo][maxnlo]
+  double *oalo;        // [natm][nlomax][apwordmax]
+  double *ololo;       // [natm][nlomax][nlomax]
+  lo_desc *lod;        // [nlotot]
+  int *ivg;            // [3*ngrtot]
+  int *ivgig;          // box
+  cplx *cfunig;        // [ngrtot]
+  cplx *veffig;        // [ngvec]
+  double *ffacg;       // [nspecies][ngvec]
+};
+
+// Launchers (one translation unit each).
+int exc_launch_match(const exc_dev &d, int ngp, const double *gpc, const double *tpgpc,
+                     const cplx *sfacgp, int ldsf, cplx *apwalm, int ldg, hipStream_t s);
+int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy,
+                      int variant, hipStream_t s);
+int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy,
+                  const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
+                  hipStream_t s);
+int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
+                  hipStream_t s);

@@ -1,0 +1,240 @@
+"""Python binding of the gfx950 LAPW H+O setup (ctypes over ``include/exc_lapw.h``).
+
+Mirrors the reference's per-k interface:
+
+* ``Context(crystal)``                 — uploads the SCF state (module data of the reference)
+* ``Context.match(kp)``                — match.f90:13, host arrays in, apwalm out
+* ``Context.hamiltonandoverlapsetup``  — hamiltonandoverlapsetup.f90:8, (H, O) out
+* ``Context.match_dev / setup_dev``    — the same on device tensors (torch, cuda)
+
+The library is loaded from ``exciting_amd/_lib/libexcgpu.so``; there is no fallback: if
+the library is missing, ``load_library`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIBPATH = Path(__file__).resolve().parent / "_lib" / "libexcgpu.so"
+
+EXC_OK = 0
+EXC_ERR_ARG = -1
+EXC_ERR_HIP = -2
+EXC_ERR_UNSUPPORTED = -3
+EXC_ERR_STATE = -4
+
+
+class ExcError(RuntimeError):
+    def __init__(self, where, code):
+        super().__init__(f"{where} returned {code}")
+        self.code = code
+
+
+class ExcDims(C.Structure):
+    _fields_ = [("natmtot", C.c_int), ("nspecies", C.c_int), ("lmaxapw", C.c_int), ("lmaxmat", C.c_int),
+                ("apwordmax", C.c_int), ("nlomax", C.c_int), ("lolmmax", C.c_int), ("maxnlo", C.c_int),
+                ("ngrtot", C.c_int), ("ngvec", C.c_int), ("intgv", C.c_int * 6), ("omega", C.c_double)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_I = C.c_int
+
+# name -> argtypes (restype int)
+SIGNATURES = {
+    "exc_ctx_create": [C.POINTER(ExcDims), C.POINTER(_P)],
+    "exc_ctx_destroy": [_P],
+    "exc_ctx_set_stream": [_P, _P],
+    "exc_ctx_set_atoms": [_P, _P, _P, _P],
+    "exc_ctx_set_lo": [_P, _P, _P, _P],
+    "exc_ctx_set_radial": [_P, _P],
+    "exc_ctx_set_mt": [_P, _P, _P, _P, _P, _P],
+    "exc_ctx_set_gvec": [_P, _P, _P, _P, _P, _P],
+    "exc_match": [_P, _I, _I, _P, _P, _P, _P],
+    "exc_match_dev": [_P, _I, _I, _P, _P, _P, _P],
+    "exc_hamiltonandoverlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
+    "exc_hamiltonandoverlapsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
+    "exc_ctx_nlotot": [_P],
+}
+
+
+def load_library(path: os.PathLike | None = None):
+    """Load libexcgpu.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIBPATH
+    if not p.exists():
+        raise OSError(f"{p} not built: run `python -m exciting_amd.build`")
+    lib = C.CDLL(str(p))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()            # torch tensor
+
+
+def _f(a, dtype):
+    """Fortran-ordered contiguous copy (or the array itself when it already is one)."""
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def dims_of(cr) -> ExcDims:
+    d = ExcDims()
+    d.natmtot = cr.natm
+    d.nspecies = cr.nspecies
+    d.lmaxapw = cr.lmaxapw
+    d.lmaxmat = cr.lmaxmat
+    d.apwordmax = cr.apwordmax
+    d.nlomax = cr.nlomax
+    d.lolmmax = cr.lolmmax
+    d.maxnlo = cr.maxnlo
+    d.ngrtot = cr.ngrtot
+    d.ngvec = cr.ngvec
+    iv = np.asarray(cr.intgv, dtype=np.int32)
+    for i in range(3):
+        d.intgv[i] = int(iv[i, 0])
+        d.intgv[3 + i] = int(iv[i, 1])
+    d.omega = cr.omega
+    return d
+
+
+class Context:
+    """One device context holding the SCF-constant state of a crystal."""
+
+    def __init__(self, cr, stream=None):
+        self.lib = load_library()
+        self.cr = cr
+        h = C.c_void_p()
+        self._keep = []
+        self._call("exc_ctx_create", C.byref(dims_of(cr)), C.byref(h))
+        self.h = h
+        if stream is not None:
+            self.set_stream(stream)
+        self.upload(cr)
+
+    def _call(self, name, *args, ok_positive=False):
+        rc = getattr(self.lib, name)(*args)
+        if rc < 0 or (rc > 0 and not ok_positive):
+            raise ExcError(name, rc)
+        return rc
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream, an int handle, or None."""
+        handle = getattr(stream, "cuda_stream", stream)
+        self._call("exc_ctx_set_stream", self.h, C.c_void_p(handle) if handle else None)
+
+    def upload(self, cr):
+        k = self._keep = []
+
+        def hold(a):
+            k.append(a)
+            return _ptr(a)
+
+        self._call("exc_ctx_set_atoms", self.h, hold(_f(cr.natoms, np.int32)), hold(_f(cr.rmt, np.float64)),
+                   hold(_f(cr.atposc, np.float64)))
+        self._call("exc_ctx_set_lo", self.h, hold(_f(cr.nlorb, np.int32)), hold(_f(cr.lorbl, np.int32)),
+                   hold(_f(cr.idxlo, np.int32)))
+        self.radial_info = self._call("exc_ctx_set_radial", self.h, hold(_f(cr.dmat, np.float64)), ok_positive=True)
+        self._call("exc_ctx_set_mt", self.h, hold(_f(cr.haa, np.complex128)), hold(_f(cr.loa, np.complex128)),
+                   hold(_f(cr.lolo, np.complex128)), hold(_f(cr.oalo, np.float64)), hold(_f(cr.ololo, np.float64)))
+        self._call("exc_ctx_set_gvec", self.h, hold(_f(cr.ivg, np.int32)), hold(_f(cr.ivgig, np.int32)),
+                   hold(_f(cr.cfunig, np.complex128)), hold(_f(cr.veffig, np.complex128)),
+                   hold(_f(cr.ffacg, np.float64)))
+        self._keep = []
+
+    @property
+    def nlotot(self):
+        return self.lib.exc_ctx_nlotot(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.exc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ host-array entry points
+    def match(self, kp, ngkmax=None):
+        """apwalm(ngkmax, apwordmax, lmmaxapw, natm) for k-point kp (match.f90:13)."""
+        cr = self.cr
+        ngkmax = kp.igpig.shape[0] if ngkmax is None else ngkmax
+        apw = np.zeros((ngkmax, cr.apwordmax, cr.lmmaxapw, cr.natm), dtype=np.complex128, order="F")
+        gpc = _f(kp.gpc[:ngkmax], np.float64)
+        tp = _f(kp.tpgpc[:, :ngkmax], np.float64)
+        sf = _f(kp.sfacgp[:ngkmax], np.complex128)
+        self.match_info = self._call("exc_match", self.h, kp.ngp, ngkmax, _ptr(gpc), _ptr(tp), _ptr(sf), _ptr(apw),
+                                     ok_positive=True)
+        return apw
+
+    def hamiltonandoverlapsetup(self, kp, apwalm, ldh=None):
+        """(H, O) of k-point kp (hamiltonandoverlapsetup.f90:8), Fortran-ordered (ldh, nmat)."""
+        nmat = kp.ngp + self.nlotot
+        ldh = nmat if ldh is None else ldh
+        ngkmax = apwalm.shape[0]
+        H = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
+        O = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
+        apw = _f(apwalm, np.complex128)
+        ig = _f(kp.igpig[:ngkmax], np.int32)
+        vg = _f(kp.vgpc[:, :ngkmax], np.float64)
+        self._call("exc_hamiltonandoverlapsetup", self.h, kp.ngp, ngkmax, _ptr(apw), _ptr(ig), _ptr(vg), ldh,
+                   _ptr(H), _ptr(O))
+        return H, O
+
+    # ------------------------------------------------------------------ device entry points
+    def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):
+        return self._call("exc_match_dev", self.h, ngp, ngkmax, _ptr(gpc), _ptr(tpgpc), _ptr(sfacgp), _ptr(apwalm),
+                          ok_positive=True)
+
+    def setup_dev(self, ngp, ngkmax, apwalm, igpig, vgpc, ldh, H, O):
+        return self._call("exc_hamiltonandoverlapsetup_dev", self.h, ngp, ngkmax, _ptr(apwalm), _ptr(igpig),
+                          _ptr(vgpc), ldh, _ptr(H), _ptr(O))
+
+
+class DeviceKPoint:
+    """Per-k inputs resident in device memory (torch tensors in the reference's layouts)."""
+
+    def __init__(self, kp, device="cuda"):
+        import torch
+        n = kp.igpig.shape[0]
+        self.ngp = kp.ngp
+        self.ngkmax = n
+        self.gpc = torch.from_numpy(np.ascontiguousarray(kp.gpc)).to(device)
+        self.tpgpc = torch.from_numpy(np.ascontiguousarray(kp.tpgpc.T)).to(device)          # (ngkmax, 2) == (2, ngkmax) F
+        self.sfacgp = torch.from_numpy(np.ascontiguousarray(kp.sfacgp.T)).to(device)        # (natm, ngkmax) == F
+        self.igpig = torch.from_numpy(np.ascontiguousarray(kp.igpig.astype(np.int32))).to(device)
+        self.vgpc = torch.from_numpy(np.ascontiguousarray(kp.vgpc.T)).to(device)            # (ngkmax, 3) == (3, ngkmax) F
+
+
+def fortran_view(t, shape):
+    """A torch tensor allocated C-contiguous with reversed shape, seen as a Fortran array."""
+    return t.permute(*reversed(range(len(shape))))
+
+
+def alloc_apwalm_dev(cr, ngkmax, device="cuda"):
+    import torch
+    return torch.empty((cr.natm, cr.lmmaxapw, cr.apwordmax, ngkmax), dtype=torch.complex128, device=device)
+
+
+def alloc_matrix_dev(ldh, nmat, device="cuda"):
+    """A (ldh, nmat) Fortran-ordered complex matrix: stored as a (nmat, ldh) C tensor."""
+    import torch
+    return torch.empty((nmat, ldh), dtype=torch.complex128, device=device)

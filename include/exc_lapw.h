@@ -1,0 +1,131 @@
+/*
+ * exc_lapw.h — C-ABI of the gfx950 (MI355X) LAPW Hamiltonian/overlap setup.
+ *
+ * The functions below replace, one for one, the reference routines of exciting's
+ * per-k first-variational setup (paths relative to the reference's src/):
+ *
+ *   exc_match                      match.f90:13            Subroutine match(ngp,gpc,tpgpc,sfacgp,apwalm)
+ *   exc_hamiltonandoverlapsetup    src_eigensystem/hamiltonandoverlapsetup.f90:8
+ *                                  (== hamiltonsetup.f90:8 followed by overlapsetup.f90:8,
+ *                                   including hmlistln.f90:13, olpistln.f90:13 and
+ *                                   hamiltonoverlapocopy_UL.f90:6)
+ *
+ * Module state that the reference routines read implicitly (mt_hscf, oalo, ololo, idxlo,
+ * ivg, ivgig, cfunig, veffig, apwfr-derived boundary values ...) is uploaded once per SCF
+ * iteration into a context with the exc_ctx_set_* calls.
+ *
+ * Conventions
+ *  - Arrays have the reference's Fortran shapes and column-major layout; integer index
+ *    arrays hold the reference's 1-based values.  Complex arrays are interleaved
+ *    (re, im) doubles, i.e. Fortran complex(8) / C double _Complex.
+ *  - Functions whose name ends in _dev take device pointers (memory of the GPU the
+ *    context was created on) and run on the context's stream asynchronously; all other
+ *    pointers are host memory and the call returns when the results are in host memory.
+ *  - H and O are fully overwritten ("=" semantics): the reference adds to a matrix that
+ *    newsystem (modfvsystem.f90:92-101) has just zeroed, so the results are the same.
+ *    Both triangles are written; the strict lower triangle is conj(upper) exactly, as
+ *    after hamiltonoverlapocopy_UL.
+ *  - Return value: 0 = success; > 0 mirrors a LAPACK info (exc_match: 1 + the 0-based
+ *    index (l + (lmaxapw+1)*ias) of the first singular matching matrix, like zgesv's
+ *    info in match.f90:163-175 — the coefficients are still written, as the reference
+ *    continues); < 0 = EXC_ERR_*.
+ *  - One context serves one host thread at a time (the reference calls the setup from one
+ *    thread per MPI rank, scf_cycle.f90:241-276).  Calls on one context are serialised on
+ *    its stream.
+ */
+#ifndef EXC_LAPW_H
+#define EXC_LAPW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXC_OK 0
+#define EXC_ERR_ARG (-1)
+#define EXC_ERR_HIP (-2)
+#define EXC_ERR_UNSUPPORTED (-3)
+#define EXC_ERR_STATE (-4)
+
+typedef struct exc_ctx exc_ctx;
+
+/* Sizes of the module state (mod_atoms, mod_APW_LO, mod_muffin_tin, mod_Gvector). */
+typedef struct exc_dims {
+  int natmtot;    /* number of atoms (mod_atoms natmtot)                              */
+  int nspecies;   /* number of species                                               */
+  int lmaxapw;    /* input%groundstate%lmaxapw (<= 14)                               */
+  int lmaxmat;    /* input%groundstate%lmaxmat (must equal lmaxapw here)             */
+  int apwordmax;  /* APW order; must be 2 with apword(l,is) = 2 for every l          */
+  int nlomax;     /* max nlorb over species (first dimension of lorbl, oalo, ololo)   */
+  int lolmmax;    /* first dimension of idxlo, (lolmax+1)^2                           */
+  int maxnlo;     /* mt_hscf%maxnlo (leading dimension of loa and lolo)              */
+  int ngrtot;     /* number of G vectors in ivg / cfunig                             */
+  int ngvec;      /* G vectors with |G| < gmaxvr (length of veffig)                  */
+  int intgv[6];   /* intgv(3,2) in Fortran order: lower bounds (1..3), upper (1..3)   */
+  double omega;   /* unit-cell volume                                                */
+} exc_dims;
+
+/* Create a context on the current HIP device.  Replaces nothing in the reference: the
+ * reference keeps this state in Fortran modules (mod_atoms, mod_muffin_tin, ...). */
+int exc_ctx_create(const exc_dims *dims, exc_ctx **out);
+int exc_ctx_destroy(exc_ctx *ctx);
+
+/* Run the context's work on this HIP stream (hipStream_t passed as void*; NULL = the
+ * null stream).  The library never creates streams of non-default priority. */
+int exc_ctx_set_stream(exc_ctx *ctx, void *stream);
+
+/* Atoms: natoms(nspecies), rmt(nspecies), atposc(3, natmtot) Cartesian, atoms in the
+ * reference's idxas order (species-major).  mod_atoms / mod_muffin_tin state. */
+int exc_ctx_set_atoms(exc_ctx *ctx, const int *natoms, const double *rmt, const double *atposc);
+
+/* Local orbitals: nlorb(nspecies), lorbl(nlomax, nspecies), idxlo(lolmmax, nlomax, natmtot)
+ * as built by genidxlo.f90:24-43.  Returns EXC_ERR_ARG if idxlo is not the running
+ * species->atom->ilo->m numbering the setup routines assume. */
+int exc_ctx_set_lo(exc_ctx *ctx, const int *nlorb, const int *lorbl, const int *idxlo);
+
+/* Radial boundary values for match: dmat(apwordmax, apwordmax, 0:lmaxapw, natmtot),
+ * dmat(i, j, l, ias) = d^(i-1) u_(j,l) / dr^(i-1) at r = R, i.e. the zd matrix that
+ * match.f90:141-148 builds with polynom from apwfr.  Returns > 0 (as exc_match would) if
+ * a matching matrix is singular. */
+int exc_ctx_set_radial(exc_ctx *ctx, const double *dmat);
+
+/* Muffin-tin matrices of this SCF iteration (mod_eigensystem.F90:60-80, olprad.f90):
+ *   haa(maxaa, maxaa, natmtot)   complex  mt_hscf%main%aa
+ *   loa(maxnlo, maxaa, natmtot)  complex  mt_hscf%main%loa
+ *   lolo(maxnlo, maxnlo, natmtot) complex mt_hscf%main%lolo
+ *   oalo(apwordmax, nlomax, natmtot) real, ololo(nlomax, nlomax, natmtot) real
+ * with maxaa = apwordmax*(lmaxmat+1)^2.  The h1* (IORA) terms are zero in this setup. */
+int exc_ctx_set_mt(exc_ctx *ctx, const double *haa, const double *loa, const double *lolo,
+                   const double *oalo, const double *ololo);
+
+/* G-vector tables (mod_Gvector / mod_potential_and_density):
+ *   ivg(3, ngrtot) integer, ivgig(intgv(1,1):intgv(1,2), ..2.., ..3..) integer (1-based),
+ *   cfunig(ngrtot) complex (gencfun.f90), veffig(ngvec) complex (genveffig.f90),
+ *   ffacg(ngvec, nspecies) real (genffacg.f90; may be NULL, needed only by the VJP). */
+int exc_ctx_set_gvec(exc_ctx *ctx, const int *ivg, const int *ivgig, const double *cfunig,
+                     const double *veffig, const double *ffacg);
+
+/* match.f90:13 — APW matching coefficients of one k-point.
+ *   gpc(ngkmax), tpgpc(2, ngkmax), sfacgp(ngkmax, natmtot) complex,
+ *   apwalm(ngkmax, apwordmax, lmmaxapw, natmtot) complex (output; rows > ngp untouched). */
+int exc_match(exc_ctx *ctx, int ngp, int ngkmax, const double *gpc, const double *tpgpc,
+              const double *sfacgp, double *apwalm);
+int exc_match_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *gpc, const double *tpgpc,
+                  const double *sfacgp, double *apwalm);
+
+/* hamiltonandoverlapsetup.f90:8 (= hamiltonsetup + overlapsetup) — H and O of one k-point.
+ *   apwalm(ngkmax, apwordmax, lmmaxapw, natmtot), igpig(ngkmax), vgpc(3, ngkmax),
+ *   H(ldh, nmat), O(ldh, nmat) complex with nmat = ngp + nlotot <= ldh. */
+int exc_hamiltonandoverlapsetup(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
+                                const int *igpig, const double *vgpc, int ldh, double *h,
+                                double *o);
+int exc_hamiltonandoverlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
+                                    const int *igpig, const double *vgpc, int ldh, double *h,
+                                    double *o);
+
+/* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
+int exc_ctx_nlotot(const exc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EXC_LAPW_H */

@@ -1,0 +1,315 @@
+"""Oracle (test infrastructure only): FP64 CPU restatement of the per-k LAPW setup.
+
+Each function restates one reference routine, in numpy, on the arrays of
+``exciting_amd.synth`` (Fortran shapes, 0-based indices):
+
+* ``match``             — match.f90:83-187 (APW matching coefficients)
+* ``hamiltonsetup``     — src_eigensystem/hamiltonsetup.f90:57-174
+* ``overlapsetup``      — src_eigensystem/overlapsetup.f90:50-166
+* ``hmlistln``          — src_eigensystem/hmlistln.f90:74-84 (ValenceRelativity='none')
+* ``olpistln``          — src_eigensystem/olpistln.f90:69-82 (h1 = 0)
+* ``copy_ul``           — src_eigensystem/hamiltonoverlapocopy_UL.f90:6-30
+* ``eigen``             — the FP64 generalized eigensolve (ZHEGVX range 'I',
+                          modfvsystem.f90:691-693) via LAPACK zhegv
+* ``forcek_ibs``        — forcek.f90:93-383 (IBS force term of one k)
+* ``setup_vjp``         — dE/dtau and dE/dV(G) of (H,O) for given cotangents
+
+Only ``tests/``, ``__graft_entry__.smoke`` and ``bench.py``'s CPU-baseline leg
+may import this module.  The library never calls it.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg
+
+from . import specfun
+
+FOURPI = 4.0 * math.pi
+
+
+def zil(l):
+    return (1j) ** l
+
+
+def match(cr, kp, ngkmax=None):
+    """apwalm(ngkmax, apwordmax, lmmaxapw, natm), match.f90:83-187.
+
+    b_io = (4 pi i^l / sqrt(Omega)) |G+k|^(io-1) j_l^(io-1)(|G+k| R) S_a(G+k) conj(Y_lm),
+    D A = b with D(io1, io2) = d^(io1-1) u_io2,l / dr^(io1-1) at R (solved per (atom, l))."""
+    ngp = kp.ngp
+    if ngkmax is None:
+        ngkmax = kp.igpig.shape[0]
+    lmax = cr.lmaxapw
+    apw = np.zeros((ngkmax, cr.apwordmax, cr.lmmaxapw, cr.natm), dtype=np.complex128, order="F")
+    fpso = FOURPI / math.sqrt(cr.omega)
+    gpc = kp.gpc[:ngp]
+    ylm = specfun.genylm(lmax, kp.tpgpc[0, :ngp], kp.tpgpc[1, :ngp])   # (lmmax, ngp)
+    for is_ in range(cr.nspecies):
+        x = gpc * cr.rmt[is_]
+        djl = np.empty((lmax + 1, 2, ngp))
+        djl[:, 0, :] = specfun.sbessel_array(lmax, x)
+        djl[:, 1, :] = specfun.sbesseld1_array(lmax, x) * gpc[None, :]
+        for ias in np.nonzero(cr.spidx == is_)[0]:
+            for l in range(lmax + 1):
+                zt1 = fpso * zil(l)
+                d = cr.dmat[:, :, l, ias]
+                zt2 = zt1 * kp.sfacgp[:ngp, ias]
+                lms = np.arange(l * l, (l + 1) ** 2)
+                # b(io, igp, m)
+                b = djl[l, :, :, None] * (zt2[None, :, None] * np.conj(ylm[lms, :]).T[None, :, :])
+                sol = np.linalg.solve(d, b.reshape(cr.apwordmax, -1)).reshape(b.shape)
+                apw[:ngp, :, l * l:(l + 1) ** 2, ias] = np.transpose(sol, (1, 0, 2))
+    return apw
+
+
+def _apwi(cr, apwalm, ngp, ias):
+    """apwi(if, igp) with f = (l, m, io), io fastest, for l <= lmaxmat (hamiltonsetup.f90:62-70)."""
+    naa = cr.maxaa
+    a = apwalm[:ngp, :, :(cr.lmaxmat + 1) ** 2, ias]          # (ngp, io, lm)
+    return np.transpose(a, (1, 2, 0)).reshape(naa, ngp, order="F")
+
+
+def hmlistln(cr, kp, H):
+    """Upper-triangle interstitial term H(i,j) += V(G_i-G_j) + 1/2 (G_i+k).(G_j+k) Theta(G_i-G_j), i <= j."""
+    ngp = kp.ngp
+    ivk = cr.ivg[:, kp.igpig[:ngp] - 1]
+    iv = ivk[:, :, None] - ivk[:, None, :]
+    off = cr.intgv[:, 0][:, None, None]
+    idx = iv - off
+    shp = np.array(cr.ivgig.shape)[:, None, None]
+    inb = np.all((idx >= 0) & (idx < shp), axis=0)
+    ig = np.zeros((ngp, ngp), dtype=np.int64)
+    ig[inb] = cr.ivgig[idx[0][inb], idx[1][inb], idx[2][inb]]
+    ok = (ig > 0) & (ig <= cr.ngvec) & np.triu(np.ones((ngp, ngp), dtype=bool))
+    t1 = 0.5 * (kp.vgpc[:, :ngp].T @ kp.vgpc[:, :ngp])
+    igc = np.where(ok, ig - 1, 0)
+    zt = cr.veffig[igc] + t1 * cr.cfunig[igc]
+    H[:ngp, :ngp] += np.where(ok, zt, 0.0)
+    return ok, igc
+
+
+def olpistln(cr, kp, O):
+    ngp = kp.ngp
+    ivk = cr.ivg[:, kp.igpig[:ngp] - 1]
+    iv = ivk[:, :, None] - ivk[:, None, :]
+    idx = iv - cr.intgv[:, 0][:, None, None]
+    shp = np.array(cr.ivgig.shape)[:, None, None]
+    inb = np.all((idx >= 0) & (idx < shp), axis=0)
+    ig = np.zeros((ngp, ngp), dtype=np.int64)
+    ig[inb] = cr.ivgig[idx[0][inb], idx[1][inb], idx[2][inb]]
+    ok = (ig > 0) & (ig <= cr.ngvec) & np.triu(np.ones((ngp, ngp), dtype=bool))
+    O[:ngp, :ngp] += np.where(ok, cr.cfunig[np.where(ok, ig - 1, 0)], 0.0)
+
+
+def hamiltonsetup(cr, kp, apwalm, H):
+    """hamiltonsetup.f90:57-174: per atom zm = h_a apwi, H += apwi^H zm; APW-lo; lo-lo; interstitial."""
+    ngp = kp.ngp
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        apwi = _apwi(cr, apwalm, ngp, ias)
+        zm = cr.haa[:, :, ias] @ apwi
+        H[:ngp, :ngp] += apwi.conj().T @ zm
+        if cr.nlorb[is_] != 0:
+            ns = cr.losize[is_]
+            l3 = cr.lorbl[0, is_]
+            r0 = ngp + cr.idxlo[l3 * l3, 0, ias] - 1         # idxlm(l3,-l3) 0-based = l3*l3
+            H[r0:r0 + ns, :ngp] += cr.loa[:ns, :, ias] @ apwi
+            H[:ngp, r0:r0 + ns] = np.conj(H[r0:r0 + ns, :ngp]).T
+            l1 = cr.lorbl[0, is_]
+            j1 = ngp + cr.idxlo[l1 * l1, 0, ias] - 1
+            nl = cr.nlorb[is_]
+            l3 = cr.lorbl[nl - 1, is_]
+            j3 = ngp + cr.idxlo[l3 * (l3 + 1) + l3, nl - 1, ias] - 1
+            n = j3 - j1 + 1
+            H[j1:j3 + 1, j1:j3 + 1] += cr.lolo[:n, :n, ias]
+    hmlistln(cr, kp, H)
+    return H
+
+
+def overlapsetup(cr, kp, apwalm, O, H=None):
+    """overlapsetup.f90:50-166: O += apwi^H (conj apwi2)^H; APW-lo; lo-lo; interstitial; copy_UL."""
+    ngp = kp.ngp
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        apwi = _apwi(cr, apwalm, ngp, ias)
+        zm = np.conj(apwi.T)                                  # apwi2 = conj(apwalm), h1 = 0
+        O[:ngp, :ngp] += apwi.conj().T @ zm.conj().T
+        if cr.nlorb[is_] != 0:
+            for ilo in range(cr.nlorb[is_]):
+                l = cr.lorbl[ilo, is_]
+                lm1, lm2 = l * l, l * (l + 1) + l
+                j1 = ngp + cr.idxlo[lm1, ilo, ias] - 1
+                j2 = ngp + cr.idxlo[lm2, ilo, ias] - 1
+                for io in range(cr.apwordmax):
+                    O[:ngp, j1:j2 + 1] += np.conj(apwalm[:ngp, io, lm1:lm2 + 1, ias] * cr.oalo[io, ilo, ias])
+                for j in range(j1, j2 + 1):
+                    O[j, :ngp] = np.conj(O[:ngp, j])
+            for ilo1 in range(cr.nlorb[is_]):
+                l = cr.lorbl[ilo1, is_]
+                for ilo2 in range(cr.nlorb[is_]):
+                    if cr.lorbl[ilo2, is_] == l:
+                        lm1 = l * l
+                        j1 = ngp + cr.idxlo[lm1, ilo1, ias] - 1
+                        j2 = ngp + cr.idxlo[lm1, ilo2, ias] - 1
+                        for mm in range(2 * l + 1):
+                            O[j1 + mm, j2 + mm] += cr.ololo[ilo1, ilo2, ias]
+    olpistln(cr, kp, O)
+    if H is not None:
+        copy_ul(H)
+    copy_ul(O)
+    return O
+
+
+def copy_ul(M):
+    """Strict lower triangle := conj(upper), hamiltonoverlapocopy_UL.f90:15-27."""
+    n = M.shape[0]
+    il = np.tril_indices(n, -1)
+    M[il] = np.conj(M.T[il])
+    return M
+
+
+def setup(cr, kp, apwalm=None):
+    """match (if apwalm is None) + hamiltonsetup + overlapsetup, as seceqn/seceqnfv call them."""
+    if apwalm is None:
+        apwalm = match(cr, kp)
+    nmat = kp.ngp + cr.nlotot
+    H = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
+    O = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
+    hamiltonsetup(cr, kp, apwalm, H)
+    overlapsetup(cr, kp, apwalm, O, H)
+    return H, O, apwalm
+
+
+def eigen(H, O, nev=None):
+    """Lowest eigenpairs of H c = e O c via LAPACK zhegv (itype 1, 'V', 'U')."""
+    w, v, info = scipy.linalg.lapack.zhegv(H, O, itype=1, jobz="V", uplo="U")
+    if info != 0:
+        raise np.linalg.LinAlgError(f"zhegv info={info}")
+    if nev is not None:
+        return w[:nev], v[:, :nev]
+    return w, v
+
+
+# ---------------------------------------------------------------------------------------
+# derivatives of (H, O) with respect to atomic positions and V(G)
+# ---------------------------------------------------------------------------------------
+
+def _gdiff_index(cr, kp):
+    """ig(i,j) = index of G_i - G_j with the modulo wrap of forcek.f90:93-97 (0-based)."""
+    ngp = kp.ngp
+    ivk = cr.ivg[:, kp.igpig[:ngp] - 1]
+    iv = ivk[:, :, None] - ivk[:, None, :]
+    lo = cr.intgv[:, 0][:, None, None]
+    n = np.array(cr.ivgig.shape)[:, None, None]
+    iv = np.mod(iv - lo, n) + lo
+    idx = iv - lo
+    return cr.ivgig[idx[0], idx[1], idx[2]].astype(np.int64) - 1
+
+
+def forcek_ibs(cr, kp, evec, evalfv, occ, apwalm=None, wkpt=1.0):
+    """IBS force of one k, forcek.f90:93-383 (spin-unpolarised, first-variational only).
+
+    evec (nmat, nst), evalfv (nst,), occ (nst,).  Returns forceibs (3, natm) = -dE/dtau."""
+    ngp = kp.ngp
+    nmat = ngp + cr.nlotot
+    if apwalm is None:
+        apwalm = match(cr, kp)
+    ijg = _gdiff_index(cr, kp)
+    dp = 0.5 * (kp.vgpc[:, :ngp].T @ kp.vgpc[:, :ngp])
+    ev2 = evec * evalfv[None, :]
+    force = np.zeros((3, cr.natm))
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        Hz = np.zeros((nmat, nmat), dtype=np.complex128)
+        Oz = np.zeros((nmat, nmat), dtype=np.complex128)
+        apwi = _apwi(cr, apwalm, ngp, ias)
+        Hz[:ngp, :ngp] = apwi.conj().T @ (cr.haa[:, :, ias] @ apwi)
+        ns = cr.losize[is_]
+        l3 = cr.lorbl[0, is_]
+        r0 = ngp + cr.idxlo[l3 * l3, 0, ias] - 1
+        Hz[r0:r0 + ns, :ngp] = cr.loa[:ns, :, ias] @ apwi
+        Hz[:ngp, r0:r0 + ns] = np.conj(Hz[r0:r0 + ns, :ngp]).T
+        Oz[:ngp, :ngp] = apwi.conj().T @ apwi
+        for ilo in range(cr.nlorb[is_]):
+            l = cr.lorbl[ilo, is_]
+            lm1, lm2 = l * l, l * (l + 1) + l
+            j1 = ngp + cr.idxlo[lm1, ilo, ias] - 1
+            j2 = ngp + cr.idxlo[lm2, ilo, ias] - 1
+            for io in range(cr.apwordmax):
+                Oz[:ngp, j1:j2 + 1] += np.conj(apwalm[:ngp, io, lm1:lm2 + 1, ias] * cr.oalo[io, ilo, ias])
+            Oz[j1:j2 + 1, :ngp] = np.conj(Oz[:ngp, j1:j2 + 1]).T
+        # zt1 = -ffacg(ig) conj(sfacg(ig, ias)) with sfacg = exp(i G.r)
+        ph = np.exp(-1j * (cr.vgc[:, :].T @ cr.atposc[:, ias]))
+        zt1 = -cr.ffacg[:, is_][np.minimum(ijg, cr.ngvec - 1)] * ph[ijg]
+        zt1 = np.where(ijg < cr.ngvec, zt1, 0.0)
+        for l in range(3):
+            t1 = cr.vgc[l, ijg]
+            dlh = np.zeros((nmat, nmat), dtype=np.complex128)
+            dlo = np.zeros((nmat, nmat), dtype=np.complex128)
+            dlh[:ngp, :ngp] = (dp * zt1 + Hz[:ngp, :ngp]) * t1 * 1j
+            dlo[:ngp, :ngp] = -(zt1 + Oz[:ngp, :ngp]) * t1 * 1j
+            t2 = kp.vgpc[l, :ngp][:, None]
+            dlh[:ngp, ngp:] = Hz[:ngp, ngp:] * t2 * 1j
+            dlh[ngp:, :ngp] = np.conj(dlh[:ngp, ngp:]).T
+            dlo[:ngp, ngp:] = -Oz[:ngp, ngp:] * t2 * 1j
+            dlo[ngp:, :ngp] = np.conj(dlo[:ngp, ngp:]).T
+            zm = dlh @ evec + dlo @ ev2
+            ffv = evec.conj().T @ zm
+            force[l, ias] += wkpt * float(np.sum(occ * np.real(np.diag(ffv))))
+    return force
+
+
+def setup_vjp(cr, kp, D, W, apwalm=None):
+    """Vector-Jacobian product of (H, O) for cotangents D (of H) and W (of O).
+
+    L = Re sum_ij (D_ji H_ij + W_ji O_ij).  Returns
+      dL/dtau (3, natm) real, with H, O rebuilt from tau through A (structure
+      factors) and Theta(G) (gencfun.f90:52-91), V(G) held fixed;
+      dL/dV (ngvec,) complex: the holomorphic partial of sum_ij D_ji V(G_i - G_j)
+      over the full matrix, dL/dV(g) = sum_{ij: G_i-G_j=g} D_ji.
+    Written as the plain product rule on the full-matrix form of H and O."""
+    ngp = kp.ngp
+    nmat = ngp + cr.nlotot
+    if apwalm is None:
+        apwalm = match(cr, kp)
+    ijg = _gdiff_index(cr, kp)
+    dp = 0.5 * (kp.vgpc[:, :ngp].T @ kp.vgpc[:, :ngp])
+    gk = kp.vgpc[:, :ngp]
+    out = np.zeros((3, cr.natm))
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        apwi = _apwi(cr, apwalm, ngp, ias)
+        Hz = np.zeros((nmat, nmat), dtype=np.complex128)
+        Oz = np.zeros((nmat, nmat), dtype=np.complex128)
+        Hz[:ngp, :ngp] = apwi.conj().T @ (cr.haa[:, :, ias] @ apwi)
+        Oz[:ngp, :ngp] = apwi.conj().T @ apwi
+        ns = cr.losize[is_]
+        l3 = cr.lorbl[0, is_]
+        r0 = ngp + cr.idxlo[l3 * l3, 0, ias] - 1
+        Hz[:ngp, r0:r0 + ns] = np.conj(cr.loa[:ns, :, ias] @ apwi).T
+        for ilo in range(cr.nlorb[is_]):
+            l = cr.lorbl[ilo, is_]
+            lm1, lm2 = l * l, l * (l + 1) + l
+            j1 = ngp + cr.idxlo[lm1, ilo, ias] - 1
+            j2 = ngp + cr.idxlo[lm2, ilo, ias] - 1
+            for io in range(cr.apwordmax):
+                Oz[:ngp, j1:j2 + 1] += np.conj(apwalm[:ngp, io, lm1:lm2 + 1, ias] * cr.oalo[io, ilo, ias])
+        ph = np.exp(-1j * (cr.vgc.T @ cr.atposc[:, ias]))
+        T = np.where(ijg < cr.ngvec, -cr.ffacg[:, is_][np.minimum(ijg, cr.ngvec - 1)] * ph[ijg], 0.0)
+        for l in range(3):
+            gd = gk[l][:, None] - gk[l][None, :]
+            dH = np.zeros((nmat, nmat), dtype=np.complex128)
+            dO = np.zeros((nmat, nmat), dtype=np.complex128)
+            dH[:ngp, :ngp] = -1j * gd * (Hz[:ngp, :ngp] + dp * T)
+            dO[:ngp, :ngp] = -1j * gd * (Oz[:ngp, :ngp] + T)
+            dH[:ngp, ngp:] = -1j * gk[l][:, None] * Hz[:ngp, ngp:]
+            dO[:ngp, ngp:] = -1j * gk[l][:, None] * Oz[:ngp, ngp:]
+            dH[ngp:, :ngp] = np.conj(dH[:ngp, ngp:]).T
+            dO[ngp:, :ngp] = np.conj(dO[:ngp, ngp:]).T
+            out[l, ias] = float(np.real(np.sum(D.T * dH) + np.sum(W.T * dO)))
+    dV = np.zeros(cr.ngvec, dtype=np.complex128)
+    ig = _gdiff_index(cr, kp)
+    np.add.at(dV, ig.ravel(), D[:ngp, :ngp].T.ravel())
+    return out, dV

@@ -1,0 +1,102 @@
+"""GPU parity of the plain FP64 path: match and H/O against the CPU oracle.
+
+Tolerances (from the path's contract): H and O relative Frobenius error <= 1e-12,
+eigenvalues (LAPACK zhegv on both) <= 1e-10 Ha, apwalm relative max error <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from conftest import small, workload
+
+pytestmark = pytest.mark.gpu
+
+REL_HO = 1e-12
+EIG_TOL = 1e-10
+
+
+def _ctx(cr):
+    from exciting_amd import lapw
+    return lapw.Context(cr)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _check_k(ctx, cr, kp, nev=None, ldh_extra=0):
+    from oracle import lapw_ref as R
+    A_ref = R.match(cr, kp)
+    A = ctx.match(kp)
+    n = kp.ngp
+    err_a = np.abs(A[:n] - A_ref[:n]).max() / np.abs(A_ref[:n]).max()
+    print(f"  ngp={n} apwalm rel max err {err_a:.2e}", flush=True)
+    assert err_a <= 1e-12
+    H_ref, O_ref, _ = R.setup(cr, kp, A_ref)
+    nmat = n + cr.nlotot
+    H, O = ctx.hamiltonandoverlapsetup(kp, A, ldh=nmat + ldh_extra)
+    H, O = H[:nmat], O[:nmat]
+    eh, eo = _rel(H, H_ref), _rel(O, O_ref)
+    print(f"  |dH|/|H| {eh:.2e}  |dO|/|O| {eo:.2e}", flush=True)
+    assert eh <= REL_HO and eo <= REL_HO
+    # lower triangle is exactly conj(upper)
+    il = np.tril_indices(nmat, -1)
+    assert np.array_equal(H[il], np.conj(H.T[il])) and np.array_equal(O[il], np.conj(O.T[il]))
+    e_ref, _ = R.eigen(H_ref, O_ref)
+    e, _ = R.eigen(np.asfortranarray(H), np.asfortranarray(O))
+    m = nev or len(e)
+    de = np.abs(e[:m] - e_ref[:m]).max()
+    print(f"  max |d eig| (lowest {m}) {de:.2e} Ha", flush=True)
+    assert de <= EIG_TOL
+
+
+@pytest.mark.parametrize("key", ["a", "b", "c", "d"])
+def test_small_workloads(key):
+    w = small(key)
+    ctx = _ctx(w.crystal)
+    for kp in w.kpoints:
+        _check_k(ctx, w.crystal, kp)
+
+
+def test_config0():
+    w = workload(0)
+    ctx = _ctx(w.crystal)
+    _check_k(ctx, w.crystal, w.kpoints[0])
+
+
+def test_config0_padded_leading_dims():
+    """ngkmax > ngp and ldh > nmat: padding rows are left alone, results unchanged."""
+    from exciting_amd import synth
+    w = workload(0)
+    kp = synth.pad_kpoint(w.kpoints[0], w.kpoints[0].ngp + 13)
+    ctx = _ctx(w.crystal)
+    _check_k(ctx, w.crystal, kp, ldh_extra=5)
+
+
+def test_config1_two_kpoints():
+    w = workload(1, 2)
+    ctx = _ctx(w.crystal)
+    for kp in w.kpoints:
+        _check_k(ctx, w.crystal, kp, nev=200)
+
+
+def test_device_entry_points_match_host():
+    """match_dev + setup_dev on torch tensors give the same bits as the host-array path."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 1)
+    cr, kp = w.crystal, w.kpoints[0]
+    ctx = _ctx(cr)
+    A_h = ctx.match(kp)
+    H_h, O_h = ctx.hamiltonandoverlapsetup(kp, A_h)
+    dk = lapw.DeviceKPoint(kp)
+    A = lapw.alloc_apwalm_dev(cr, dk.ngkmax)
+    ctx.match_dev(dk.ngp, dk.ngkmax, dk.gpc, dk.tpgpc, dk.sfacgp, A)
+    nmat = kp.ngp + cr.nlotot
+    H = lapw.alloc_matrix_dev(nmat, nmat)
+    O = lapw.alloc_matrix_dev(nmat, nmat)
+    ctx.setup_dev(dk.ngp, dk.ngkmax, A, dk.igpig, dk.vgpc, nmat, H, O)
+    torch.cuda.synchronize()
+    A_d = A.cpu().numpy().transpose(3, 2, 1, 0)
+    assert np.array_equal(A_d[:kp.ngp], A_h[:kp.ngp])
+    assert np.array_equal(H.cpu().numpy().T, H_h)
+    assert np.array_equal(O.cpu().numpy().T, O_h)
