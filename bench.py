@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark: LAPW H+O setup k-points/s on one MI355X.
+
+One step = match + H/O setup for every k-point of the workload (default: workload 1, the
+orthorhombic 8-atom cell with 16 k-points and N_G+k in [1150, 1250]), with all inputs
+resident in HBM.  Prints one JSON line (the contract of the run driver):
+metric/value/unit, the roofline of the dominant kernel (the muffin-tin MFMA contraction,
+timed with HIP events on its own stream inside the library) and a CPU baseline (the FP64
+oracle, timed on the host's cores on a bounded sample of the same k-points).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H+O setup k-points/sec, 1 MI355X, split-BF16 vs FP64 at <=1e-8 Ha eig error"
+PEAK_FP64 = 78.6e12          # MI355X dense FP64 MFMA, flop/s
+PEAK_HBM = 8.0e12            # bytes/s
+
+WORKLOAD_NAMES = {
+    0: "cubic a=10.26, 2 atoms/1 species, lmax 8, 8 lo/atom, 1 k-point, |G+k|<3.5/R",
+    1: "orthorhombic 8 atoms/2 species (R 2.0,1.8), lmax 10, apword 2, 4 lo/atom, 16 k-points, N_G+k in [1150,1250]",
+    2: "16 atoms/3 species, lmax 10, 4 lo/atom, 64 k-points, N_G+k in [2400,2700]",
+    3: "32 atoms/4 species, lmax 10, 4 lo/atom, 8 k-points, N_G+k ~5000",
+    4: "16 atoms/3 species, lmax 10, 4 lo/atom, 8 k-points, N_G+k ~2500",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def mt_flops(cr, n):
+    """Algorithmic FLOPs of the MT contraction kernel per k: upper triangles of A^H.Y and A^H.A."""
+    return 8.0 * cr.maxaa * cr.natm * n * n
+
+
+def ystage_flops(cr, n):
+    return 8.0 * cr.maxaa * cr.maxaa * cr.natm * n
+
+
+def cpu_baseline(w, budget_s=15.0):
+    """The FP64 oracle (numpy/OpenBLAS on the host cores) on the first k-points of the workload."""
+    from oracle import lapw_ref as R
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    R.setup(w.crystal, w.kpoints[0])            # warm-up (page-in, BLAS threads)
+    t0 = time.perf_counter()
+    done = 0
+    for kp in w.kpoints:
+        R.setup(w.crystal, kp)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "k-points/s", "cores": int(threads), "kind": "port",
+            "sample": f"match+hamiltonsetup+overlapsetup of the first {done} k-point(s) of the same workload "
+                      f"(numpy + host OpenBLAS, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards over "
+              f"k-points across GPUs by design, but no multi-GPU run is part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from exciting_amd import lapw, synth
+
+    w = synth.make_config(args.config, nk=args.nk)
+    cr = w.crystal
+    nk = len(w.kpoints)
+    log(f"workload {args.config}: natm={cr.natm} lmax={cr.lmaxapw} nk={nk} N={[k.ngp for k in w.kpoints]}")
+    stream = torch.cuda.current_stream()
+    ctx = lapw.Context(cr, stream=stream)
+    dks = [lapw.DeviceKPoint(kp) for kp in w.kpoints]
+    apw = [lapw.alloc_apwalm_dev(cr, d.ngkmax) for d in dks]
+    nmats = [kp.ngp + cr.nlotot for kp in w.kpoints]
+    Hs = [lapw.alloc_matrix_dev(n, n) for n in nmats]
+    Os = [lapw.alloc_matrix_dev(n, n) for n in nmats]
+
+    def step():
+        for i, d in enumerate(dks):
+            ctx.match_dev(d.ngp, d.ngkmax, d.gpc, d.tpgpc, d.sfacgp, apw[i])
+            ctx.setup_dev(d.ngp, d.ngkmax, apw[i], d.igpig, d.vgpc, nmats[i], Hs[i], Os[i])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    tim = ctx.read_timing()
+    elapsed = t1 - t0
+    value = nk * args.steps / elapsed
+    ms_step = 1000.0 * elapsed / args.steps
+    for k, (ms, cnt) in tim.items():
+        if cnt:
+            log(f"  stage {k:7s}: {ms / cnt:8.3f} ms/launch over {cnt} launches")
+
+    mt_ms, mt_cnt = tim["mt"]
+    fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints) * args.steps
+    achieved = fl / (mt_ms * 1e-3) if mt_ms > 0 else 0.0
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64, "traffic": None, "kernel": "mt_kernel (A^H.Y and A^H.A + interstitial)",
+            "per_launch_flops": fl / max(mt_cnt, 1)}
+    stages = {k: {"ms_per_launch": (ms / cnt if cnt else None), "launches": cnt} for k, (ms, cnt) in tim.items()}
+
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(w)
+    out = {
+        "metric": METRIC, "value": value, "unit": "k-points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator, no checkpoints)",
+        "config": {"workload": f"config{args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
+                   "path": "plain FP64 (per-k launches)", "parallelism": "single GPU"},
+        "roofline": roof, "stages": stages, "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

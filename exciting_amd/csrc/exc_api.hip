@@ -24,6 +24,14 @@ struct exc_ctx {
   void *stage;
   size_t stage_bytes;
   std::vector<void *> owned;
+  // per-stage HIP-event timers (exc_ctx_set_timing)
+  bool timing;
+  struct mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<mark> marks;
+  std::vector<hipEvent_t> pool;
 };
 
 namespace {
@@ -68,6 +76,37 @@ int use_device(exc_ctx *c) {
 
 bool ready(const exc_ctx *c) { return c && c->have_atoms && c->have_lo && c->have_radial && c->have_mt && c->have_gvec; }
 
+hipEvent_t take_event(exc_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one stage's launches with events when timing is on.
+struct StageTimer {
+  exc_ctx *c;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(exc_ctx *ctx, int st) : c(ctx), stage(st) {
+    if (c->timing && (a = take_event(c))) hipEventRecord(a, c->stream);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) {
+      c->pool.push_back(a);
+      return;
+    }
+    hipEventRecord(b, c->stream);
+    c->marks.push_back({stage, a, b});
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -107,6 +146,27 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
     d.box_n[i] = m.intgv[3 + i] - m.intgv[i] + 1;
   }
   d.omega = m.omega;
+  // Summation order of the muffin-tin sums: l descending (the small high-l terms first),
+  // atoms interleaved inside each l, then m and the APW order io.
+  {
+    const int L = m.lmaxmat, na = m.natmtot, ao = m.apwordmax, naa = d.naa;
+    std::vector<int> kp, fp;
+    for (int l = L; l >= 0; --l) {
+      for (int ias = 0; ias < na; ++ias)
+        for (int mm = -l; mm <= l; ++mm)
+          for (int io = 0; io < ao; ++io) kp.push_back(ias * naa + ao * (l * (l + 1) + mm) + io);
+      for (int mm = -l; mm <= l; ++mm)
+        for (int io = 0; io < ao; ++io) fp.push_back(ao * (l * (l + 1) + mm) + io);
+    }
+    int rc = dev_upload(c, &d.kperm, kp.data(), kp.size());
+    if (!rc) rc = dev_upload(c, &d.fperm, fp.data(), fp.size());
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = EXC_ERR_HIP;
+    if (rc) {
+      for (void *p : c->owned) hipFree(p);
+      delete c;
+      return rc;
+    }
+  }
   c->mt_variant = -1;
   c->y_variant = -1;
   *out = c;
@@ -121,6 +181,11 @@ int exc_ctx_destroy(exc_ctx *c) {
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
   if (c->stage) hipFree(c->stage);
+  for (auto &m : c->marks) {
+    hipEventDestroy(m.a);
+    hipEventDestroy(m.b);
+  }
+  for (hipEvent_t e : c->pool) hipEventDestroy(e);
   delete c;
   return EXC_OK;
 }
@@ -132,6 +197,35 @@ int exc_ctx_set_stream(exc_ctx *c, void *stream) {
 }
 
 int exc_ctx_nlotot(const exc_ctx *c) { return c ? c->d.nlotot : EXC_ERR_ARG; }
+
+int exc_ctx_set_timing(exc_ctx *c, int enable) {
+  if (!c) return EXC_ERR_ARG;
+  c->timing = enable != 0;
+  return EXC_OK;
+}
+
+int exc_ctx_read_timing(exc_ctx *c, double *ms, int *launches) {
+  if (!c || !ms || !launches) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  for (int i = 0; i < EXC_NSTAGE; ++i) {
+    ms[i] = 0.0;
+    launches[i] = 0;
+  }
+  if (!c->marks.empty()) EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (auto &m : c->marks) {
+    float t = 0.f;
+    EXC_CHECK(hipEventElapsedTime(&t, m.a, m.b));
+    if (m.stage >= 0 && m.stage < EXC_NSTAGE) {
+      ms[m.stage] += t;
+      launches[m.stage] += 1;
+    }
+    c->pool.push_back(m.a);
+    c->pool.push_back(m.b);
+  }
+  c->marks.clear();
+  return EXC_OK;
+}
 
 int exc_ctx_set_atoms(exc_ctx *c, const int *natoms, const double *rmt, const double *atposc) {
   if (!c || !natoms || !rmt || !atposc) return EXC_ERR_ARG;
@@ -280,7 +374,10 @@ int exc_match_dev(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const doub
   if (ngp < 0 || ngkmax < ngp || (ngp > 0 && (!gpc || !tpgpc || !sfacgp || !apwalm))) return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
-  rc = exc_launch_match(c->d, ngp, gpc, tpgpc, (const cplx *)sfacgp, ngkmax, (cplx *)apwalm, ngkmax, c->stream);
+  {
+    StageTimer t(c, EXC_STAGE_MATCH);
+    rc = exc_launch_match(c->d, ngp, gpc, tpgpc, (const cplx *)sfacgp, ngkmax, (cplx *)apwalm, ngkmax, c->stream);
+  }
   if (rc) return rc;
   return c->radial_info;
 }
@@ -324,13 +421,23 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
     if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
     const int yv = c->y_variant >= 0 ? c->y_variant : (ngp >= 2048 ? 0 : 1);
     const int mv = c->mt_variant >= 0 ? c->mt_variant : (ngp >= 2048 ? 0 : 1);
-    if ((rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream))) return rc;
-    if ((rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o,
-                            ldh, mv, c->stream)))
-      return rc;
+    {
+      StageTimer t(c, EXC_STAGE_YSTAGE);
+      rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);
+    }
+    if (rc) return rc;
+    {
+      StageTimer t(c, EXC_STAGE_MT);
+      rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o, ldh,
+                         mv, c->stream);
+    }
+    if (rc) return rc;
   }
-  if ((rc = exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream))) return rc;
-  return EXC_OK;
+  {
+    StageTimer t(c, EXC_STAGE_LO);
+    rc = exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream);
+  }
+  return rc;
 }
 
 int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,

@@ -63,6 +63,8 @@ struct exc_dev {
   cplx *cfunig;        // [ngrtot]
   cplx *veffig;        // [ngvec]
   double *ffacg;       // [nspecies][ngvec]
+  int *kperm;          // [natm*naa]: summation order of the MT product (column of apwalm)
+  int *fperm;          // [naa]: summation order inside one atom (Y stage)
 };
 
 // Launchers (one translation unit each).

@@ -21,7 +21,8 @@ using namespace zf64;
 template <int BM, int BN, int BK, int WR, int WC>
 __global__ __launch_bounds__(64 * WR * WC) void ystage_kernel(int ngp, int naa, const cplx *__restrict__ apwalm,
                                                            int ldg, const cplx *__restrict__ haa,
-                                                           cplx *__restrict__ yt, int ldy) {
+                                                           cplx *__restrict__ yt, int ldy,
+                                                           const int *__restrict__ fperm) {
   __shared__ Smem<BM, BN, BK> sm;
   using C = Core<BM, BN, BK, WR, WC, false>;
   C core;
@@ -29,7 +30,7 @@ __global__ __launch_bounds__(64 * WR * WC) void ystage_kernel(int ngp, int naa, 
   const int row0 = blockIdx.x * BM, col0 = blockIdx.y * BN;
   const cplx *A = apwalm + (size_t)ldg * naa * ias;
   const cplx *B = haa + (size_t)naa * naa * ias;
-  core.run(sm, A, ldg, row0, ngp, B, naa, col0, naa, naa);
+  core.run(sm, A, ldg, row0, ngp, B, naa, col0, naa, naa, fperm);
   cplx *Y = yt + (size_t)ldy * naa * ias;
   const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
 #pragma unroll
@@ -77,7 +78,8 @@ template <int BM, int BK, int WR, int WC>
 __global__ __launch_bounds__(64 * WR * WC) void mt_kernel(int ngp, int ktot, int ntile, const cplx *__restrict__ apwalm,
                                                        int ldg, const cplx *__restrict__ yt, int ldy,
                                                        const int *__restrict__ igpig, const double *__restrict__ vgpc,
-                                                       GTab gt, cplx *__restrict__ h, cplx *__restrict__ o, int ldh) {
+                                                       GTab gt, cplx *__restrict__ h, cplx *__restrict__ o, int ldh,
+                                                       const int *__restrict__ kperm) {
   constexpr int BN = BM;
   __shared__ Smem<BM, BN, BK> sm;
   __shared__ int s_gr[BM][3], s_gc[BN][3];
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(64 * WR * WC) void mt_kernel(int ngp, int ktot, int
   C core;
   const cplx *B = which ? apwalm : yt;
   const int ldb = which ? ldg : ldy;
-  core.run(sm, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot);
+  core.run(sm, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm);
 
   cplx *out = which ? o : h;
   const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
@@ -169,7 +171,7 @@ template <int BM, int BN, int BK, int WR, int WC>
 int launch_ystage_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy, hipStream_t s) {
   dim3 grid((ngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm);
   hipLaunchKernelGGL((ystage_kernel<BM, BN, BK, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ngp, d.naa, apwalm, ldg,
-                     d.haa, yt, ldy);
+                     d.haa, yt, ldy, d.fperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -180,7 +182,7 @@ int launch_mt_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cp
   const int ntile = (ngp + BM - 1) / BM;
   const int nitem = ntile * (ntile + 1);
   hipLaunchKernelGGL((mt_kernel<BM, BK, WR, WC>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp, d.natm * d.naa, ntile, apwalm, ldg,
-                     yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh);
+                     yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

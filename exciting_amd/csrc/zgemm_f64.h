@@ -44,24 +44,28 @@ struct Core {
       }
   }
 
-  // Load one BK slab of A (rows row0.., valid rows < mrows) and B into registers.
+  // Load one BK slab of A (rows row0.., valid rows < mrows) and B into registers.  The
+  // k-th step of the sum reads column kperm[k] of both operands: the callers order the
+  // columns so that the small (high-l) terms are accumulated first, which keeps the long
+  // sequential MFMA accumulation as accurate as a blocked BLAS sum.
   __device__ __forceinline__ static void gload(const cplx *__restrict__ A, int lda, int row0, int mrows,
                                                const cplx *__restrict__ B, int ldb, int col0, int nrows,
-                                               int k0, int ktot, cplx (&ra)[LA], cplx (&rb)[LB]) {
+                                               int k0, int ktot, const int *__restrict__ kperm, cplx (&ra)[LA],
+                                               cplx (&rb)[LB]) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < LA; ++r) {
       const int e = tid + kThreads * r;
       const int row = e % BM, k = e / BM;
       const int gr = row0 + row, gk = k0 + k;
-      ra[r] = (gr < mrows && gk < ktot) ? A[gr + (size_t)gk * lda] : cplx{0.0, 0.0};
+      ra[r] = (gr < mrows && gk < ktot) ? A[gr + (size_t)kperm[gk] * lda] : cplx{0.0, 0.0};
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int e = tid + kThreads * r;
       const int row = e % BN, k = e / BN;
       const int gr = col0 + row, gk = k0 + k;
-      rb[r] = (gr < nrows && gk < ktot) ? B[gr + (size_t)gk * ldb] : cplx{0.0, 0.0};
+      rb[r] = (gr < nrows && gk < ktot) ? B[gr + (size_t)kperm[gk] * ldb] : cplx{0.0, 0.0};
     }
   }
 
@@ -133,25 +137,26 @@ struct Core {
   // Full K loop over [0, ktot).
   __device__ __forceinline__ void run(Smem<BM, BN, BK> &sm, const cplx *__restrict__ A, int lda, int row0,
                                       int mrows, const cplx *__restrict__ B, int ldb, int col0, int nrows,
-                                      int ktot) {
+                                      int ktot, const int *__restrict__ kperm) {
     const int wave = threadIdx.x >> 6;
     const int wr = wave / WC, wc = wave % WC;
     zero();
     cplx ra[LA], rb[LB];
-    gload(A, lda, row0, mrows, B, ldb, col0, nrows, 0, ktot, ra, rb);
+    gload(A, lda, row0, mrows, B, ldb, col0, nrows, 0, ktot, kperm, ra, rb);
     for (int k0 = 0; k0 < ktot; k0 += BK) {
       __syncthreads();
       sstore(sm, ra, rb);
       __syncthreads();
-      if (k0 + BK < ktot) gload(A, lda, row0, mrows, B, ldb, col0, nrows, k0 + BK, ktot, ra, rb);
+      if (k0 + BK < ktot) gload(A, lda, row0, mrows, B, ldb, col0, nrows, k0 + BK, ktot, kperm, ra, rb);
       compute(sm, wr, wc);
     }
   }
 
-  // Accumulator element (v) of block (i, j) sits at tile row / column:
-  //   row = wr*WM + i*16 + 4*(lane/16) + v,  col = wc*WN + j*16 + lane%16.
+  // Accumulator element (v) of block (i, j) sits at tile row / column (the output layout of
+  // v_mfma_f64_16x16x4f64, measured on gfx950: register v of lane l holds row 4v + l/16):
+  //   row = wr*WM + i*16 + 4*v + lane/16,  col = wc*WN + j*16 + lane%16.
   __device__ __forceinline__ static int row_of(int wr, int i, int v) {
-    return wr * WM + i * 16 + 4 * ((threadIdx.x & 63) >> 4) + v;
+    return wr * WM + i * 16 + 4 * v + ((threadIdx.x & 63) >> 4);
   }
   __device__ __forceinline__ static int col_of(int wc, int j) { return wc * WN + j * 16 + (threadIdx.x & 15); }
 };

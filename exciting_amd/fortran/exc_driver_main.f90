@@ -1,0 +1,64 @@
+! Test program for the Fortran driver: reads one k-point's inputs from a stream file,
+! runs match and hamiltonandoverlapsetup through the GPU driver as seceqn.f90:57 /
+! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.
+!   exc_driver_main <input.bin> <output.bin>
+program exc_driver_main
+  use, intrinsic :: iso_c_binding
+  use exc_lapw_c, only: exc_dims
+  use exc_evsystem
+  use exc_setup_gpu
+  implicit none
+  type(exc_dims) :: dims
+  integer :: ngp, ngkmax, nmat, lmmax, naa, nbox, u, nlotot
+  integer(c_int), allocatable :: natoms(:), nlorb(:), lorbl(:), idxlo(:), ivg(:), ivgig(:), igpig(:)
+  real(8), allocatable :: rmt(:), atposc(:), dmat(:), oalo(:), ololo(:), ffacg(:), gpc(:), tpgpc(:, :), vgpc(:, :)
+  complex(8), allocatable :: haa(:), loa(:), lolo(:), cfunig(:), veffig(:), sfacgp(:, :), apwalm(:, :, :, :)
+  type(evsystem) :: system
+  character(len=512) :: fin, fout
+  integer :: hdr(18)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access="stream", form="unformatted", status="old")
+  read(u) hdr
+  dims%natmtot = hdr(1); dims%nspecies = hdr(2); dims%lmaxapw = hdr(3); dims%lmaxmat = hdr(4)
+  dims%apwordmax = hdr(5); dims%nlomax = hdr(6); dims%lolmmax = hdr(7); dims%maxnlo = hdr(8)
+  dims%ngrtot = hdr(9); dims%ngvec = hdr(10); dims%intgv = hdr(11:16)
+  ngp = hdr(17); ngkmax = hdr(18)
+  read(u) dims%omega
+  lmmax = (dims%lmaxapw + 1)**2
+  naa = dims%apwordmax * (dims%lmaxmat + 1)**2
+  nbox = product(dims%intgv(4:6) - dims%intgv(1:3) + 1)
+  allocate(natoms(dims%nspecies), rmt(dims%nspecies), atposc(3 * dims%natmtot), nlorb(dims%nspecies))
+  allocate(lorbl(dims%nlomax * dims%nspecies), idxlo(dims%lolmmax * dims%nlomax * dims%natmtot))
+  allocate(dmat(dims%apwordmax**2 * (dims%lmaxapw + 1) * dims%natmtot))
+  allocate(haa(naa * naa * dims%natmtot), loa(dims%maxnlo * naa * dims%natmtot))
+  allocate(lolo(dims%maxnlo**2 * dims%natmtot), oalo(dims%apwordmax * dims%nlomax * dims%natmtot))
+  allocate(ololo(dims%nlomax**2 * dims%natmtot), ivg(3 * dims%ngrtot), ivgig(nbox))
+  allocate(cfunig(dims%ngrtot), veffig(dims%ngvec), ffacg(dims%ngvec * dims%nspecies))
+  allocate(gpc(ngkmax), tpgpc(2, ngkmax), sfacgp(ngkmax, dims%natmtot), igpig(ngkmax), vgpc(3, ngkmax))
+  read(u) natoms, rmt, atposc, nlorb, lorbl, idxlo, dmat, haa, loa, lolo, oalo, ololo
+  read(u) ivg, ivgig, cfunig, veffig, ffacg
+  read(u) gpc, tpgpc, sfacgp, igpig, vgpc
+  close(u)
+
+  call exc_gpu_init(dims, ngkmax, natoms, rmt, atposc, nlorb, lorbl, idxlo, dmat, haa, loa, lolo, &
+                    oalo, ololo, ivg, ivgig, cfunig, veffig, ffacg)
+  nlotot = exc_gpu_nlotot()
+  nmat = ngp + nlotot
+
+  allocate(apwalm(ngkmax, dims%apwordmax, lmmax, dims%natmtot))
+  apwalm = (0.d0, 0.d0)
+  call match(ngp, gpc, tpgpc, sfacgp, apwalm)
+  call newsystem(system, .false., nmat)
+  call hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)
+
+  open(newunit=u, file=trim(fout), access="stream", form="unformatted", status="replace")
+  write(u) nmat
+  write(u) apwalm
+  write(u) system%hamilton%za
+  write(u) system%overlap%za
+  close(u)
+  call deletesystem(system)
+  call exc_gpu_finalize()
+end program exc_driver_main

@@ -1,0 +1,88 @@
+! iso_c_binding interfaces of libexcgpu.so (include/exc_lapw.h).
+! Every routine returns the C-ABI status: 0 ok, > 0 LAPACK-style info, < 0 error.
+module exc_lapw_c
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: EXC_OK = 0, EXC_ERR_ARG = -1, EXC_ERR_HIP = -2, &
+                               EXC_ERR_UNSUPPORTED = -3, EXC_ERR_STATE = -4
+
+  type, bind(c) :: exc_dims
+    integer(c_int) :: natmtot, nspecies, lmaxapw, lmaxmat, apwordmax, nlomax, lolmmax, maxnlo
+    integer(c_int) :: ngrtot, ngvec
+    integer(c_int) :: intgv(6)
+    real(c_double) :: omega
+  end type exc_dims
+
+  interface
+    integer(c_int) function exc_ctx_create(dims, ctx) bind(c, name="exc_ctx_create")
+      import :: c_int, c_ptr, exc_dims
+      type(exc_dims), intent(in) :: dims
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function exc_ctx_destroy(ctx) bind(c, name="exc_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function exc_ctx_set_atoms(ctx, natoms, rmt, atposc) bind(c, name="exc_ctx_set_atoms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: natoms(*)
+      real(c_double), intent(in) :: rmt(*), atposc(*)
+    end function
+
+    integer(c_int) function exc_ctx_set_lo(ctx, nlorb, lorbl, idxlo) bind(c, name="exc_ctx_set_lo")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: nlorb(*), lorbl(*), idxlo(*)
+    end function
+
+    integer(c_int) function exc_ctx_set_radial(ctx, dmat) bind(c, name="exc_ctx_set_radial")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dmat(*)
+    end function
+
+    integer(c_int) function exc_ctx_set_mt(ctx, haa, loa, lolo, oalo, ololo) bind(c, name="exc_ctx_set_mt")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: haa(*), loa(*), lolo(*)
+      real(c_double), intent(in) :: oalo(*), ololo(*)
+    end function
+
+    integer(c_int) function exc_ctx_set_gvec(ctx, ivg, ivgig, cfunig, veffig, ffacg) bind(c, name="exc_ctx_set_gvec")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: ivg(*), ivgig(*)
+      complex(c_double_complex), intent(in) :: cfunig(*), veffig(*)
+      real(c_double), intent(in) :: ffacg(*)
+    end function
+
+    integer(c_int) function exc_match(ctx, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm) bind(c, name="exc_match")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, ngkmax
+      real(c_double), intent(in) :: gpc(*), tpgpc(*)
+      complex(c_double_complex), intent(in) :: sfacgp(*)
+      complex(c_double_complex), intent(inout) :: apwalm(*)
+    end function
+
+    integer(c_int) function exc_hamiltonandoverlapsetup(ctx, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o) &
+        bind(c, name="exc_hamiltonandoverlapsetup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, ngkmax, ldh
+      complex(c_double_complex), intent(in) :: apwalm(*)
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: h(*), o(*)
+    end function
+
+    integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+end module exc_lapw_c

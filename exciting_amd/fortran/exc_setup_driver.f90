@@ -1,0 +1,109 @@
+! Fortran driver of the GPU setup with the reference's call signatures.
+!
+!   match(ngp, gpc, tpgpc, sfacgp, apwalm)                        match.f90:13
+!   hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)     hamiltonandoverlapsetup.f90:8
+!
+! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
+! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
+! iteration by exc_gpu_init, which keeps the device context in this module.  Error
+! handling follows the reference: a singular matching matrix is reported and the run
+! continues (match.f90:166-175); any other failure stops the program.
+module exc_setup_gpu
+  use, intrinsic :: iso_c_binding
+  use exc_lapw_c
+  use exc_evsystem
+  implicit none
+  private
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: ngkmax_ = 0
+  integer, save :: lmaxapw_ = 0, natmtot_ = 0
+
+  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot
+
+contains
+
+  subroutine check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc < 0) then
+      write(*, '("Error(", A, "): GPU setup returned ", I0)') where, rc
+      error stop 1
+    end if
+  end subroutine check
+
+  ! Upload the SCF-constant state.  Shapes as in the reference modules:
+  ! natoms(nspecies), rmt(nspecies), atposc(3,natmtot), nlorb(nspecies), lorbl(nlomax,nspecies),
+  ! idxlo(lolmmax,nlomax,natmtot), dmat(apwordmax,apwordmax,0:lmaxapw,natmtot),
+  ! haa(maxaa,maxaa,natmtot), loa(maxnlo,maxaa,natmtot), lolo(maxnlo,maxnlo,natmtot),
+  ! oalo(apwordmax,nlomax,natmtot), ololo(nlomax,nlomax,natmtot), ivg(3,ngrtot), ivgig(box),
+  ! cfunig(ngrtot), veffig(ngvec), ffacg(ngvec,nspecies)
+  subroutine exc_gpu_init(dims, ngkmax, natoms, rmt, atposc, nlorb, lorbl, idxlo, dmat, haa, loa, lolo, &
+                          oalo, ololo, ivg, ivgig, cfunig, veffig, ffacg)
+    type(exc_dims), intent(in) :: dims
+    integer, intent(in) :: ngkmax
+    integer(c_int), intent(in) :: natoms(*), nlorb(*), lorbl(*), idxlo(*), ivg(*), ivgig(*)
+    real(8), intent(in) :: rmt(*), atposc(*), dmat(*), oalo(*), ololo(*), ffacg(*)
+    complex(8), intent(in) :: haa(*), loa(*), lolo(*), cfunig(*), veffig(*)
+    integer(c_int) :: rc
+    if (c_associated(ctx)) call exc_gpu_finalize()
+    call check(exc_ctx_create(dims, ctx), "exc_ctx_create")
+    ngkmax_ = ngkmax
+    lmaxapw_ = dims%lmaxapw
+    natmtot_ = dims%natmtot
+    call check(exc_ctx_set_atoms(ctx, natoms, rmt, atposc), "exc_ctx_set_atoms")
+    call check(exc_ctx_set_lo(ctx, nlorb, lorbl, idxlo), "exc_ctx_set_lo")
+    rc = exc_ctx_set_radial(ctx, dmat)
+    call check(rc, "exc_ctx_set_radial")
+    if (rc > 0) write(*, '("Warning(exc_gpu_init): singular matching matrix, info = ", I0)') rc
+    call check(exc_ctx_set_mt(ctx, haa, loa, lolo, oalo, ololo), "exc_ctx_set_mt")
+    call check(exc_ctx_set_gvec(ctx, ivg, ivgig, cfunig, veffig, ffacg), "exc_ctx_set_gvec")
+  end subroutine exc_gpu_init
+
+  subroutine exc_gpu_finalize()
+    integer(c_int) :: rc
+    if (c_associated(ctx)) rc = exc_ctx_destroy(ctx)
+    ctx = c_null_ptr
+  end subroutine exc_gpu_finalize
+
+  integer function exc_gpu_nlotot()
+    exc_gpu_nlotot = exc_ctx_nlotot(ctx)
+  end function exc_gpu_nlotot
+
+  ! match.f90:13 — same arguments; ngkmax is the leading dimension set at init
+  subroutine match(ngp, gpc, tpgpc, sfacgp, apwalm)
+    integer, intent(in) :: ngp
+    real(8), intent(in) :: gpc(ngkmax_), tpgpc(2, ngkmax_)
+    complex(8), intent(in) :: sfacgp(ngkmax_, *)
+    complex(8), intent(inout) :: apwalm(ngkmax_, *)
+    integer(c_int) :: rc, l, ias
+    rc = exc_match(ctx, int(ngp, c_int), int(ngkmax_, c_int), gpc, tpgpc, sfacgp, apwalm)
+    call check(rc, "match")
+    if (rc > 0) then
+      l = mod(rc - 1, lmaxapw_ + 1)
+      ias = (rc - 1) / (lmaxapw_ + 1) + 1
+      write(*,*)
+      write(*, '("Error(match): could not find APW matching coefficients")')
+      write(*, '(" for atom ", I4, " and l ", I4)') ias, l
+      write(*, '(" singular matching matrix, INFO = ", I8)') rc
+      write(*,*)
+    end if
+  end subroutine match
+
+  ! hamiltonandoverlapsetup.f90:8 — H and O of one k-point into system%hamilton%za and
+  ! system%overlap%za (rank = nmat).  The matrices are overwritten, which equals the
+  ! reference's accumulation into the zeroed matrices that newsystem provides.
+  subroutine hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)
+    type(evsystem), intent(inout) :: system
+    integer, intent(in) :: ngp
+    complex(8), intent(in) :: apwalm(ngkmax_, *)
+    integer, intent(in) :: igpig(ngkmax_)
+    real(8), intent(in) :: vgpc(3, ngkmax_)
+    integer(c_int) :: ldh
+    ldh = int(system%hamilton%rank, c_int)
+    if (system%overlap%rank /= ldh) error stop "hamiltonandoverlapsetup: H and O ranks differ"
+    call check(exc_hamiltonandoverlapsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, igpig, vgpc, &
+                                           ldh, system%hamilton%za, system%overlap%za), "hamiltonandoverlapsetup")
+  end subroutine hamiltonandoverlapsetup
+
+end module exc_setup_gpu

@@ -59,7 +59,12 @@ SIGNATURES = {
     "exc_hamiltonandoverlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
     "exc_hamiltonandoverlapsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
     "exc_ctx_nlotot": [_P],
+    "exc_ctx_set_timing": [_P, _I],
+    "exc_ctx_read_timing": [_P, _P, _P],
 }
+
+# stage order of exc_ctx_read_timing
+STAGES = ("match", "ystage", "mt", "lo")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -70,6 +75,13 @@ def load_library(path: os.PathLike | None = None):
     p = Path(path) if path is not None else LIBPATH
     if not p.exists():
         raise OSError(f"{p} not built: run `python -m exciting_amd.build`")
+    try:
+        # torch ships its own HIP runtime (loaded as "libamdhip64.so"); importing it first lets
+        # the library's libamdhip64.so.7 dependency resolve to that same runtime instead of a
+        # second copy, so device memory and streams are shared with torch
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -203,6 +215,17 @@ class Context:
     def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):
         return self._call("exc_match_dev", self.h, ngp, ngkmax, _ptr(gpc), _ptr(tpgpc), _ptr(sfacgp), _ptr(apwalm),
                           ok_positive=True)
+
+    def set_timing(self, on=True):
+        self._call("exc_ctx_set_timing", self.h, 1 if on else 0)
+
+    def read_timing(self):
+        """{stage: (total_ms, launches)} recorded since the last read (synchronises the stream)."""
+        n = 8
+        ms = (C.c_double * n)()
+        cnt = (C.c_int * n)()
+        self._call("exc_ctx_read_timing", self.h, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES)}
 
     def setup_dev(self, ngp, ngkmax, apwalm, igpig, vgpc, ldh, H, O):
         return self._call("exc_hamiltonandoverlapsetup_dev", self.h, ngp, ngkmax, _ptr(apwalm), _ptr(igpig),

@@ -125,6 +125,21 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const dou
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
+/* Per-stage kernel timers, the counterpart of the reference's timing accumulators
+ * (mod_timing.F90: timematch, time_hmlaan, time_olpaan, time_hmlalon, time_hmlistln, ...).
+ * When enabled, every launch of a stage is bracketed by HIP events on the context's stream.
+ * exc_ctx_read_timing waits for the stream, returns per stage the summed kernel time (ms)
+ * and launch count since the last read, and clears them.  Stages, in array order:
+ *   EXC_STAGE_MATCH, EXC_STAGE_YSTAGE (h.A^T), EXC_STAGE_MT (A^H.Y, A^H.A + interstitial),
+ *   EXC_STAGE_LO (APW-lo and lo-lo blocks); arrays must hold EXC_NSTAGE entries. */
+#define EXC_STAGE_MATCH 0
+#define EXC_STAGE_YSTAGE 1
+#define EXC_STAGE_MT 2
+#define EXC_STAGE_LO 3
+#define EXC_NSTAGE 8
+int exc_ctx_set_timing(exc_ctx *ctx, int enable);
+int exc_ctx_read_timing(exc_ctx *ctx, double *ms, int *launches);
+
 #ifdef __cplusplus
 }
 #endif

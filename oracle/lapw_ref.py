@@ -103,14 +103,34 @@ def olpistln(cr, kp, O):
     O[:ngp, :ngp] += np.where(ok, cr.cfunig[np.where(ok, ig - 1, 0)], 0.0)
 
 
+def _l_blocks(cr):
+    """Row ranges of apwi (f = (l, m, io)) belonging to each l, largest l first."""
+    ao = cr.apwordmax
+    return [(ao * l * l, ao * (l + 1) ** 2) for l in range(cr.lmaxmat, -1, -1)]
+
+
+def _mt_sum(cr, left, right):
+    """sum_a left_a^H right_a (the per-atom zgemm of hamiltonsetup.f90:90-103 /
+    overlapsetup.f90:95-108 summed over atoms).  The sum over the APW index f is taken in
+    blocks of equal l, from lmax down to 0, so that the small high-l products are added
+    first; BLAS leaves the order of a zgemm's sum open, and this order keeps the oracle's
+    rounding error near 3e-16 relative."""
+    out = None
+    for f0, f1 in _l_blocks(cr):
+        blk = sum(left[a][f0:f1].conj().T @ right[a][f0:f1] for a in range(cr.natm))
+        out = blk if out is None else out + blk
+    return out
+
+
 def hamiltonsetup(cr, kp, apwalm, H):
     """hamiltonsetup.f90:57-174: per atom zm = h_a apwi, H += apwi^H zm; APW-lo; lo-lo; interstitial."""
     ngp = kp.ngp
+    apwis = [_apwi(cr, apwalm, ngp, ias) for ias in range(cr.natm)]
+    zms = [cr.haa[:, :, ias] @ apwis[ias] for ias in range(cr.natm)]
+    H[:ngp, :ngp] += _mt_sum(cr, apwis, zms)
     for ias in range(cr.natm):
         is_ = cr.spidx[ias]
-        apwi = _apwi(cr, apwalm, ngp, ias)
-        zm = cr.haa[:, :, ias] @ apwi
-        H[:ngp, :ngp] += apwi.conj().T @ zm
+        apwi = apwis[ias]
         if cr.nlorb[is_] != 0:
             ns = cr.losize[is_]
             l3 = cr.lorbl[0, is_]
@@ -131,11 +151,11 @@ def hamiltonsetup(cr, kp, apwalm, H):
 def overlapsetup(cr, kp, apwalm, O, H=None):
     """overlapsetup.f90:50-166: O += apwi^H (conj apwi2)^H; APW-lo; lo-lo; interstitial; copy_UL."""
     ngp = kp.ngp
+    apwis = [_apwi(cr, apwalm, ngp, ias) for ias in range(cr.natm)]
+    # zm = conj(apwi)^T (apwi2, h1 = 0); O += apwi^H zm^H = conj(A) A^T
+    O[:ngp, :ngp] += _mt_sum(cr, apwis, apwis)
     for ias in range(cr.natm):
         is_ = cr.spidx[ias]
-        apwi = _apwi(cr, apwalm, ngp, ias)
-        zm = np.conj(apwi.T)                                  # apwi2 = conj(apwalm), h1 = 0
-        O[:ngp, :ngp] += apwi.conj().T @ zm.conj().T
         if cr.nlorb[is_] != 0:
             for ilo in range(cr.nlorb[is_]):
                 l = cr.lorbl[ilo, is_]

@@ -1,0 +1,32 @@
+"""Stream-file exchange with the Fortran driver test program (exc_driver_main.f90)."""
+import numpy as np
+
+
+def _w(f, a, dtype):
+    np.asarray(a, dtype=dtype).ravel(order="F").tofile(f)
+
+
+def write_input(path, cr, kp):
+    hdr = [cr.natm, cr.nspecies, cr.lmaxapw, cr.lmaxmat, cr.apwordmax, cr.nlomax, cr.lolmmax, cr.maxnlo,
+           cr.ngrtot, cr.ngvec, *cr.intgv[:, 0], *cr.intgv[:, 1], kp.ngp, kp.igpig.shape[0]]
+    with open(path, "wb") as f:
+        _w(f, hdr, np.int32)
+        _w(f, [cr.omega], np.float64)
+        _w(f, cr.natoms, np.int32); _w(f, cr.rmt, np.float64); _w(f, cr.atposc, np.float64)
+        _w(f, cr.nlorb, np.int32); _w(f, cr.lorbl, np.int32); _w(f, cr.idxlo, np.int32)
+        _w(f, cr.dmat, np.float64); _w(f, cr.haa, np.complex128); _w(f, cr.loa, np.complex128)
+        _w(f, cr.lolo, np.complex128); _w(f, cr.oalo, np.float64); _w(f, cr.ololo, np.float64)
+        _w(f, cr.ivg, np.int32); _w(f, cr.ivgig, np.int32); _w(f, cr.cfunig, np.complex128)
+        _w(f, cr.veffig, np.complex128); _w(f, cr.ffacg, np.float64)
+        _w(f, kp.gpc, np.float64); _w(f, kp.tpgpc, np.float64); _w(f, kp.sfacgp, np.complex128)
+        _w(f, kp.igpig, np.int32); _w(f, kp.vgpc, np.float64)
+
+
+def read_output(path, cr, ngkmax):
+    with open(path, "rb") as f:
+        nmat = int(np.fromfile(f, np.int32, 1)[0])
+        na = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm
+        apw = np.fromfile(f, np.complex128, na).reshape((ngkmax, cr.apwordmax, cr.lmmaxapw, cr.natm), order="F")
+        H = np.fromfile(f, np.complex128, nmat * nmat).reshape((nmat, nmat), order="F")
+        O = np.fromfile(f, np.complex128, nmat * nmat).reshape((nmat, nmat), order="F")
+    return apw, H, O

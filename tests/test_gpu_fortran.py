@@ -1,0 +1,40 @@
+"""The Fortran driver (amdflang, iso_c_binding) with the reference signatures, on the GPU.
+
+Runs exciting_amd/_lib/exc_driver_main: match(ngp, gpc, tpgpc, sfacgp, apwalm) and
+hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc) for one k-point, and checks
+apwalm, H and O against the oracle (relative Frobenius error <= 1e-12).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import small, workload
+from fortran_io import read_output, write_input
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "exciting_amd", "_lib",
+                   "exc_driver_main")
+
+
+@pytest.mark.parametrize("which", ["small-b", "config0"])
+def test_fortran_driver(tmp_path, which):
+    from oracle import lapw_ref as R
+    if not os.path.exists(EXE):
+        pytest.fail("Fortran driver not built: run python -m exciting_amd.build")
+    w = small("b") if which == "small-b" else workload(0)
+    cr, kp = w.crystal, w.kpoints[0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, cr, kp)
+    print(f"running Fortran driver on {which}", flush=True)
+    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    apw, H, O = read_output(fout, cr, kp.igpig.shape[0])
+    H_ref, O_ref, A_ref = R.setup(cr, kp)
+    n = kp.ngp
+    assert np.abs(apw[:n] - A_ref[:n]).max() <= 1e-12 * np.abs(A_ref[:n]).max()
+    assert np.linalg.norm(H - H_ref) <= 1e-12 * np.linalg.norm(H_ref)
+    assert np.linalg.norm(O - O_ref) <= 1e-12 * np.linalg.norm(O_ref)
