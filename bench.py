@@ -75,6 +75,12 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="autotune tile variants before timing")
+    ap.add_argument("--path", choices=["batched", "perk"], default="batched",
+                    help="batched: all k-points in one grouped launch per stage, replayed as a hipGraph; "
+                         "perk: one launch sequence per k-point")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards over "
@@ -84,23 +90,56 @@ def main():
     import torch
     from exciting_amd import lapw, synth
 
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")     # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 5 copies of 1073741824 bytes")
+        return
+
     w = synth.make_config(args.config, nk=args.nk)
     cr = w.crystal
     nk = len(w.kpoints)
     log(f"workload {args.config}: natm={cr.natm} lmax={cr.lmaxapw} nk={nk} N={[k.ngp for k in w.kpoints]}")
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()          # default priority; a non-null stream can be graph-captured
+    torch.cuda.set_stream(stream)
     ctx = lapw.Context(cr, stream=stream)
-    dks = [lapw.DeviceKPoint(kp) for kp in w.kpoints]
-    apw = [lapw.alloc_apwalm_dev(cr, d.ngkmax) for d in dks]
-    nmats = [kp.ngp + cr.nlotot for kp in w.kpoints]
-    Hs = [lapw.alloc_matrix_dev(n, n) for n in nmats]
-    Os = [lapw.alloc_matrix_dev(n, n) for n in nmats]
+    batch = lapw.DeviceBatch(cr, w.kpoints)
+    dks, apw, nmats, Hs, Os = batch.dks, batch.apw, batch.nmat, batch.H, batch.O
 
-    def step():
+    def step_perk():
         for i, d in enumerate(dks):
             ctx.match_dev(d.ngp, d.ngkmax, d.gpc, d.tpgpc, d.sfacgp, apw[i])
             ctx.setup_dev(d.ngp, d.ngkmax, apw[i], d.igpig, d.vgpc, nmats[i], Hs[i], Os[i])
 
+    def step_batched(graph=True):
+        ctx.setup_batched_dev(batch, lapw.EXC_BATCH_MATCH | (lapw.EXC_BATCH_GRAPH if graph else 0))
+
+    tuned = None
+    if args.tune:
+        from exciting_amd import autotune
+        if args.path == "perk":
+            tuned = autotune.tune(ctx, ["mt", "ystage"], step_perk, reps=2, log=log)
+        else:
+            tuned = autotune.tune(ctx, ["bmt", "ystage"], lambda: step_batched(False), reps=2, log=log)
+    if args.path == "batched":
+        # stage breakdown of one eager (uncaptured) grouped pass, then time the graph replays
+        step_batched(False)
+        torch.cuda.synchronize()
+        ctx.read_timing()
+        ctx.set_timing(True)
+        step_batched(False)
+        ctx.set_timing(False)
+        eager = ctx.read_timing()
+        for k, (ms, cnt) in eager.items():
+            if cnt:
+                log(f"  grouped stage {k:7s}: {ms:8.3f} ms")
+        step = step_batched
+    else:
+        eager = None
+        step = step_perk
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -121,13 +160,21 @@ def main():
         if cnt:
             log(f"  stage {k:7s}: {ms / cnt:8.3f} ms/launch over {cnt} launches")
 
-    mt_ms, mt_cnt = tim["mt"]
-    fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints) * args.steps
+    if args.path == "batched":
+        mt_ms, mt_cnt = eager["bmt"]
+        fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints)
+    else:
+        mt_ms, mt_cnt = tim["mt"]
+        fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints) * args.steps
     achieved = fl / (mt_ms * 1e-3) if mt_ms > 0 else 0.0
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64, "traffic": None, "kernel": "mt_kernel (A^H.Y and A^H.A + interstitial)",
+            "frac": achieved / PEAK_FP64, "traffic": None,
+            "kernel": ("bmt_kernel (grouped, all k)" if args.path == "batched" else "mt_kernel (per k)")
+                      + ": A^H.Y and A^H.A + interstitial",
             "per_launch_flops": fl / max(mt_cnt, 1)}
-    stages = {k: {"ms_per_launch": (ms / cnt if cnt else None), "launches": cnt} for k, (ms, cnt) in tim.items()}
+    stages = {k: {"ms_per_launch": (ms / cnt if cnt else None), "launches": cnt} for k, (ms, cnt) in tim.items() if cnt}
+    if eager:
+        stages["grouped_eager_pass"] = {k: ms for k, (ms, cnt) in eager.items() if cnt}
 
     base = None
     if not args.no_cpu_baseline:
@@ -137,7 +184,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator, no checkpoints)",
         "config": {"workload": f"config{args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
-                   "path": "plain FP64 (per-k launches)", "parallelism": "single GPU"},
+                   "path": ("FP64, all k-points grouped per stage, hipGraph replay" if args.path == "batched"
+                            else "FP64, per-k launches"), "parallelism": "single GPU", "variants": tuned},
         "roofline": roof, "stages": stages, "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

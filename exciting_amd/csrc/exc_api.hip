@@ -32,6 +32,24 @@ struct exc_ctx {
   };
   std::vector<mark> marks;
   std::vector<hipEvent_t> pool;
+  // grouped launches
+  int bmt_variant;          // -1: built-in choice
+  int bsplit_variant;
+  unsigned short *planes;   // split-BF16 slice planes of all k-points (A and Yt)
+  size_t planes_bytes;
+  int *expo;                // per-(row, atom) exponents
+  size_t expo_bytes;
+  cplx *byt;                // Y-stage workspace of all k-points
+  size_t byt_bytes;
+  kdesc *d_ks;
+  size_t ks_cap;
+  mt_item *d_items;
+  size_t items_cap;
+  std::vector<char> batch_sig;   // the batch the descriptors / graph were built for
+  int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
+  hipGraph_t graph;
+  hipGraphExec_t graph_exec;
+  int ncu;
 };
 
 namespace {
@@ -93,7 +111,7 @@ struct StageTimer {
   int stage;
   hipEvent_t a = nullptr;
   StageTimer(exc_ctx *ctx, int st) : c(ctx), stage(st) {
-    if (c->timing && (a = take_event(c))) hipEventRecord(a, c->stream);
+    if (st >= 0 && c->timing && (a = take_event(c))) hipEventRecord(a, c->stream);
   }
   ~StageTimer() {
     if (!a) return;
@@ -169,6 +187,12 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   }
   c->mt_variant = -1;
   c->y_variant = -1;
+  c->bmt_variant = -1;
+  c->bsplit_variant = -1;
+  {
+    hipDeviceProp_t prop;
+    c->ncu = (hipGetDeviceProperties(&prop, c->device) == hipSuccess) ? prop.multiProcessorCount : 256;
+  }
   *out = c;
   return EXC_OK;
 }
@@ -186,6 +210,13 @@ int exc_ctx_destroy(exc_ctx *c) {
     hipEventDestroy(m.b);
   }
   for (hipEvent_t e : c->pool) hipEventDestroy(e);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  if (c->byt) hipFree(c->byt);
+  if (c->d_ks) hipFree(c->d_ks);
+  if (c->d_items) hipFree(c->d_items);
+  if (c->planes) hipFree(c->planes);
+  if (c->expo) hipFree(c->expo);
   delete c;
   return EXC_OK;
 }
@@ -197,6 +228,40 @@ int exc_ctx_set_stream(exc_ctx *c, void *stream) {
 }
 
 int exc_ctx_nlotot(const exc_ctx *c) { return c ? c->d.nlotot : EXC_ERR_ARG; }
+
+int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
+  if (!c) return EXC_ERR_ARG;
+  switch (stage) {
+    case EXC_STAGE_YSTAGE:
+      if (variant < -1 || variant >= kYstageVariants) return EXC_ERR_ARG;
+      c->y_variant = variant;
+      return EXC_OK;
+    case EXC_STAGE_MT:
+      if (variant < -1 || variant >= kMtVariants) return EXC_ERR_ARG;
+      c->mt_variant = variant;
+      return EXC_OK;
+    case EXC_STAGE_BMT:
+      if (variant < -1 || variant >= kBmtVariants) return EXC_ERR_ARG;
+      c->bmt_variant = variant;
+      return EXC_OK;
+    case EXC_STAGE_BSPLIT:
+      if (variant < -1 || variant >= kBsplitVariants) return EXC_ERR_ARG;
+      c->bsplit_variant = variant;
+      return EXC_OK;
+    default:
+      return EXC_ERR_ARG;
+  }
+}
+
+int exc_ctx_num_variants(int stage) {
+  switch (stage) {
+    case EXC_STAGE_YSTAGE: return kYstageVariants;
+    case EXC_STAGE_MT: return kMtVariants;
+    case EXC_STAGE_BMT: return kBmtVariants;
+    case EXC_STAGE_BSPLIT: return kBsplitVariants;
+    default: return 0;
+  }
+}
 
 int exc_ctx_set_timing(exc_ctx *c, int enable) {
   if (!c) return EXC_ERR_ARG;
@@ -419,8 +484,9 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
   if (ngp > 0) {
     const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
     if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
-    const int yv = c->y_variant >= 0 ? c->y_variant : (ngp >= 2048 ? 0 : 1);
-    const int mv = c->mt_variant >= 0 ? c->mt_variant : (ngp >= 2048 ? 0 : 1);
+    // defaults from the autotuner's sweep on MI355X (32x32 MT tiles, 64x64 Y tiles)
+    const int yv = c->y_variant >= 0 ? c->y_variant : 1;
+    const int mv = c->mt_variant >= 0 ? c->mt_variant : 5;
     {
       StageTimer t(c, EXC_STAGE_YSTAGE);
       rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);
@@ -470,6 +536,161 @@ int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *a
   EXC_CHECK(hipMemcpyAsync(o, base + o_o, b_h, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   return EXC_OK;
+}
+
+int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flags) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nk < 0 || (nk > 0 && !ks) || (flags & ~(EXC_BATCH_MATCH | EXC_BATCH_GRAPH | EXC_BATCH_SPLIT_BF16)))
+    return EXC_ERR_ARG;
+  const bool split = flags & EXC_BATCH_SPLIT_BF16;
+  if (split && c->d.naa > kSplitKB) return EXC_ERR_UNSUPPORTED;
+  if (nk == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const bool do_match = flags & EXC_BATCH_MATCH;
+  for (int k = 0; k < nk; ++k) {
+    const exc_kpoint_dev &q = ks[k];
+    if (q.ngp < 0 || q.ngkmax < q.ngp || q.ldh < q.ngp + c->d.nlotot || !q.h || !q.o) return EXC_ERR_ARG;
+    if (q.ngp > 0 && (!q.apwalm || !q.igpig || !q.vgpc)) return EXC_ERR_ARG;
+    if (do_match && q.ngp > 0 && (!q.gpc || !q.tpgpc || !q.sfacgp)) return EXC_ERR_ARG;
+  }
+  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 0) : (c->bmt_variant >= 0 ? c->bmt_variant : 0);
+  const int yv = c->y_variant >= 0 ? c->y_variant : 1;
+  // batch signature: descriptors, flags and variants
+  std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
+  memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
+  int tail[4] = {nk, flags, bv, yv};
+  memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
+  const bool same = (sig == c->batch_sig);
+  if (same && (flags & EXC_BATCH_GRAPH) && c->graph_exec) {
+    StageTimer t(c, EXC_STAGE_GRAPH);
+    EXC_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+    return EXC_OK;
+  }
+  if (!same) {
+    // new batch: lay out the Y workspace, descriptors and the muffin-tin work list
+    EXC_CHECK(hipStreamSynchronize(c->stream));   // earlier launches may still read the old tables
+    if (c->graph_exec) {
+      hipGraphExecDestroy(c->graph_exec);
+      c->graph_exec = nullptr;
+    }
+    if (c->graph) {
+      hipGraphDestroy(c->graph);
+      c->graph = nullptr;
+    }
+    size_t ytot = 0, ptot = 0, etot = 0;
+    for (int k = 0; k < nk; ++k) {
+      ytot += (size_t)ks[k].ngp * c->d.naa * c->d.natm;
+      ptot += 2 * exc_split_plane_elems(ks[k].ngp, c->d.natm);
+      etot += 2 * (size_t)ks[k].ngp * c->d.natm;
+    }
+    if ((rc = grow((void **)&c->byt, &c->byt_bytes, sizeof(cplx) * (ytot ? ytot : 1)))) return rc;
+    if (split) {
+      if ((rc = grow((void **)&c->planes, &c->planes_bytes, 2 * (ptot ? ptot : 1)))) return rc;
+      if ((rc = grow((void **)&c->expo, &c->expo_bytes, sizeof(int) * (etot ? etot : 1)))) return rc;
+    }
+    std::vector<kdesc> kd(nk);
+    std::vector<mt_item> items;
+    const int T = split ? exc_bsplit_tile(bv) : exc_bmt_tile(bv);
+    size_t yoff = 0, poff = 0, eoff = 0;
+    int maxngp = 0;
+    for (int k = 0; k < nk; ++k) {
+      const exc_kpoint_dev &q = ks[k];
+      kdesc &e = kd[k];
+      e.ngp = q.ngp;
+      e.ldg = q.ngkmax;
+      e.ldh = q.ldh;
+      e.ldy = q.ngp;
+      e.gpc = q.gpc;
+      e.tpgpc = q.tpgpc;
+      e.sfacgp = (const cplx *)q.sfacgp;
+      e.apwalm = (cplx *)q.apwalm;
+      e.igpig = q.igpig;
+      e.vgpc = q.vgpc;
+      e.yt = c->byt + yoff;
+      e.h = (cplx *)q.h;
+      e.o = (cplx *)q.o;
+      yoff += (size_t)q.ngp * c->d.naa * c->d.natm;
+      e.sa = e.sy = nullptr;
+      e.ea = e.ey = nullptr;
+      if (split) {
+        const size_t pe = exc_split_plane_elems(q.ngp, c->d.natm);
+        e.sa = c->planes + poff;
+        e.sy = c->planes + poff + pe;
+        poff += 2 * pe;
+        e.ea = c->expo + eoff;
+        e.ey = c->expo + eoff + (size_t)q.ngp * c->d.natm;
+        eoff += 2 * (size_t)q.ngp * c->d.natm;
+      }
+      maxngp = q.ngp > maxngp ? q.ngp : maxngp;
+      const int nt = (q.ngp + T - 1) / T;
+      for (int tj = 0; tj < nt; ++tj)
+        for (int ti = 0; ti <= tj; ++ti) {
+          items.push_back({k, ti, tj, 0});
+          items.push_back({k, ti, tj, 1});
+        }
+    }
+    if ((rc = grow((void **)&c->d_ks, &c->ks_cap, sizeof(kdesc) * nk))) return rc;
+    if ((rc = grow((void **)&c->d_items, &c->items_cap, sizeof(mt_item) * (items.empty() ? 1 : items.size()))))
+      return rc;
+    EXC_CHECK(hipMemcpy(c->d_ks, kd.data(), sizeof(kdesc) * nk, hipMemcpyHostToDevice));
+    if (!items.empty())
+      EXC_CHECK(hipMemcpy(c->d_items, items.data(), sizeof(mt_item) * items.size(), hipMemcpyHostToDevice));
+    c->batch_nitems = (int)items.size();
+    int nb = c->ncu * (split ? exc_bsplit_max_blocks(bv) : exc_bmt_max_blocks(bv));
+    nb = nb < c->batch_nitems ? nb : c->batch_nitems;
+    if (nb >= 8) nb -= nb % 8;
+    c->batch_nblocks = nb > 0 ? nb : 1;
+    c->batch_maxngp = maxngp;
+    c->batch_nk = nk;
+    c->batch_sig = sig;
+  }
+  const bool capture = flags & EXC_BATCH_GRAPH;
+  if (capture && c->stream == nullptr) return EXC_ERR_ARG;   // the null stream cannot be captured
+  if (capture) EXC_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  auto launches = [&]() -> int {
+    int r = EXC_OK;
+    if (do_match) {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_MATCH);
+      if ((r = exc_launch_bmatch(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
+    }
+    {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_YSTAGE);
+      if ((r = exc_launch_bystage(c->d, nk, c->batch_maxngp, c->d_ks, yv, c->stream))) return r;
+    }
+    if (split) {
+      {
+        StageTimer t(c, capture ? -1 : EXC_STAGE_SPLIT);
+        if ((r = exc_launch_split(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
+      }
+      StageTimer t(c, capture ? -1 : EXC_STAGE_BSPLIT);
+      if ((r = exc_launch_bsplit(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream)))
+        return r;
+    } else {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_BMT);
+      if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream))) return r;
+    }
+    {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_LO);
+      if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
+    }
+    return r;
+  };
+  rc = launches();
+  if (capture) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return EXC_ERR_HIP;
+    c->graph = g;
+    EXC_CHECK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    StageTimer t(c, EXC_STAGE_GRAPH);
+    EXC_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  }
+  return rc;
 }
 
 }  // extern "C"

@@ -67,6 +67,26 @@ struct exc_dev {
   int *fperm;          // [naa]: summation order inside one atom (Y stage)
 };
 
+// Device-side description of one k-point of a grouped launch.
+struct kdesc {
+  int ngp, ldg, ldh, ldy;
+  const double *gpc, *tpgpc;
+  const cplx *sfacgp;
+  cplx *apwalm;
+  const int *igpig;
+  const double *vgpc;
+  cplx *yt;
+  cplx *h, *o;
+  // split-BF16 operands (exc_setup_batched_dev with EXC_BATCH_SPLIT_BF16)
+  unsigned short *sa, *sy;   // slice planes of A and of Yt, layout [row][atom][k/16][plane][16]
+  int *ea, *ey;              // per-(row, atom) exponents, [row*natm + atom]
+};
+
+// Work item of the grouped muffin-tin launch: (k, tile row, tile column, H=0 / O=1).
+struct mt_item {
+  int k, ti, tj, which;
+};
+
 // Launchers (one translation unit each).
 int exc_launch_match(const exc_dev &d, int ngp, const double *gpc, const double *tpgpc,
                      const cplx *sfacgp, int ldsf, cplx *apwalm, int ldg, hipStream_t s);
@@ -75,5 +95,26 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
 int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy,
                   const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
                   hipStream_t s);
+constexpr int kYstageVariants = 4;
+constexpr int kMtVariants = 6;
+constexpr int kBmtVariants = 4;
+// Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.
+int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
+int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, int variant, hipStream_t s);
+int exc_bmt_tile(int variant);   // square tile edge of a grouped-MT variant
+int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
+                   hipStream_t s);
+int exc_bmt_max_blocks(int variant);   // resident workgroups per CU of a grouped-MT variant
+int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
+// split-BF16 muffin-tin stage
+constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
+constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
+constexpr int kBsplitVariants = 2;
+size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
+int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
+int exc_bsplit_tile(int variant);
+int exc_bsplit_max_blocks(int variant);
+int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
+                      hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

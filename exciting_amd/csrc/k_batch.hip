@@ -1,0 +1,199 @@
+// Grouped launches over all k-points of a batch (each k with its own basis size N_k).
+//
+// match, Y stage and the lo blocks are grids with a k dimension; the muffin-tin stage is a
+// persistent kernel: a fixed number of workgroups walks a work list of (k, tile row,
+// tile column, H or O) items, so the uneven tile counts of the k-points are load-balanced
+// over the whole chip instead of leaving CUs idle at the end of each k-point.  The device
+// code is the same as the per-k path (match_dev.h, setup_dev.h, lo_dev.h).
+#include "lo_dev.h"
+#include "match_dev.h"
+#include "setup_dev.h"
+
+namespace {
+
+using namespace setup_dev;
+
+constexpr int kMatchThreads = 64;
+
+template <int L>
+__global__ __launch_bounds__(kMatchThreads) void bmatch_kernel(const kdesc *__restrict__ ks, int apwordmax,
+                                                                const int *__restrict__ spidx,
+                                                                const double *__restrict__ rmt, double fpso,
+                                                                const double *__restrict__ dinv) {
+  const kdesc k = ks[blockIdx.z];
+  const int ig = blockIdx.x * kMatchThreads + threadIdx.x;
+  if (ig >= k.ngp) return;
+  match_dev::match_body<L>(ig, blockIdx.y, apwordmax, spidx, rmt, fpso, dinv, k.gpc, k.tpgpc, k.sfacgp, k.ldg,
+                           k.apwalm, k.ldg);
+}
+
+template <int L>
+int launch_bmatch_l(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
+  dim3 grid((maxngp + kMatchThreads - 1) / kMatchThreads, d.natm, nk);
+  hipLaunchKernelGGL(bmatch_kernel<L>, grid, dim3(kMatchThreads), 0, s, ks, d.apwordmax, d.spidx, d.rmt,
+                     4.0 * M_PI / sqrt(d.omega), d.dinv);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BM, int BN, int BK, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void bystage_kernel(const kdesc *__restrict__ ks, int natm, int naa,
+                                                               const cplx *__restrict__ haa,
+                                                               const int *__restrict__ fperm) {
+  __shared__ Smem<BM, BN, BK> sm;
+  const int kk = blockIdx.z / natm, ias = blockIdx.z % natm;
+  const kdesc k = ks[kk];
+  const int row0 = blockIdx.x * BM;
+  if (row0 >= k.ngp) return;
+  YTile<BM, BN, BK, WR, WC>::run(sm, k.ngp, naa, k.apwalm + (size_t)k.ldg * naa * ias, k.ldg,
+                                 haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0,
+                                 blockIdx.y * BN, fperm);
+}
+
+template <int BM, int BN, int BK, int WR, int WC>
+int launch_bystage_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
+  dim3 grid((maxngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm * nk);
+  hipLaunchKernelGGL((bystage_kernel<BM, BN, BK, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa, d.haa,
+                     d.fperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BM, int BK, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void bmt_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                           int nitems, int ktot, GTab gt,
+                                                           const int *__restrict__ kperm) {
+  using T = MtTile<BM, BK, WR, WC>;
+  __shared__ typename T::Shared sh;
+  // Workgroup b runs on XCD b % 8 (round-robin dispatch).  Each XCD walks its own
+  // contiguous eighth of the work list, so the k-points it touches stay in its L2.
+  const int nx = gridDim.x >= 8 ? 8 : 1;
+  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
+  const int chunk = (nitems + nx - 1) / nx;
+  const int end = min(nitems, (x + 1) * chunk);
+  for (int it = x * chunk + lb; it < end; it += per) {
+    const mt_item w = items[it];
+    const kdesc k = ks[w.k];
+    __syncthreads();   // the previous item's epilogue has finished reading the tile tables
+    T::run(sh, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h, k.o, k.ldh, kperm, w.ti, w.tj,
+           w.which);
+  }
+}
+
+template <int BM, int BK, int WR, int WC>
+int launch_bmt_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((bmt_kernel<BM, BK, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
+                     d.natm * d.naa, make_gtab(d), d.kperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BM, int BK, int WR, int WC>
+int max_blocks_t() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_kernel<BM, BK, WR, WC>, 64 * WR * WC, 0) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+__global__ __launch_bounds__(lo_dev::kRows * lo_dev::kSlots) void blo_apw_kernel(const kdesc *__restrict__ ks, int naa,
+                                                                                  int maxnlo, int nlomax, int apwordmax,
+                                                                                  int nlotot,
+                                                                                  const lo_desc *__restrict__ lod,
+                                                                                  const cplx *__restrict__ loa,
+                                                                                  const double *__restrict__ oalo) {
+  __shared__ lo_dev::ApwLoShared sh;
+  const kdesc k = ks[blockIdx.z];
+  if ((int)blockIdx.x * lo_dev::kRows >= k.ngp) return;
+  lo_dev::lo_apw_body(sh, blockIdx.x, blockIdx.y, k.ngp, naa, maxnlo, nlomax, apwordmax, nlotot, lod, k.apwalm, k.ldg,
+                      loa, oalo, k.h, k.o, k.ldh);
+}
+
+__global__ __launch_bounds__(lo_dev::kLoThreads) void blo_lo_kernel(const kdesc *__restrict__ ks, int nlotot, int maxnlo,
+                                                                     int nlomax, const lo_desc *__restrict__ lod,
+                                                                     const cplx *__restrict__ lolo,
+                                                                     const double *__restrict__ ololo) {
+  const kdesc k = ks[blockIdx.z];
+  lo_dev::lo_lo_body(blockIdx.x * lo_dev::kLoThreads + threadIdx.x, blockIdx.y, k.ngp, nlotot, maxnlo, nlomax, lod,
+                     lolo, ololo, k.h, k.o, k.ldh);
+}
+
+}  // namespace
+
+int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
+  if (nk <= 0 || maxngp <= 0) return EXC_OK;
+  switch (d.lmaxapw) {
+    case 1: return launch_bmatch_l<1>(d, nk, maxngp, ks, s);
+    case 2: return launch_bmatch_l<2>(d, nk, maxngp, ks, s);
+    case 3: return launch_bmatch_l<3>(d, nk, maxngp, ks, s);
+    case 4: return launch_bmatch_l<4>(d, nk, maxngp, ks, s);
+    case 5: return launch_bmatch_l<5>(d, nk, maxngp, ks, s);
+    case 6: return launch_bmatch_l<6>(d, nk, maxngp, ks, s);
+    case 7: return launch_bmatch_l<7>(d, nk, maxngp, ks, s);
+    case 8: return launch_bmatch_l<8>(d, nk, maxngp, ks, s);
+    case 9: return launch_bmatch_l<9>(d, nk, maxngp, ks, s);
+    case 10: return launch_bmatch_l<10>(d, nk, maxngp, ks, s);
+    case 11: return launch_bmatch_l<11>(d, nk, maxngp, ks, s);
+    case 12: return launch_bmatch_l<12>(d, nk, maxngp, ks, s);
+    default: return EXC_ERR_UNSUPPORTED;
+  }
+}
+
+int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, int variant, hipStream_t s) {
+  if (nk <= 0 || maxngp <= 0) return EXC_OK;
+  switch (variant) {
+    case 0: return launch_bystage_t<128, 128, 16, 2, 4>(d, nk, maxngp, ks, s);
+    case 1: return launch_bystage_t<64, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
+    case 2: return launch_bystage_t<128, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
+    case 3: return launch_bystage_t<64, 64, 32, 2, 2>(d, nk, maxngp, ks, s);
+    default: return EXC_ERR_ARG;
+  }
+}
+
+int exc_bmt_tile(int variant) {
+  switch (variant) {
+    case 0: return 32;
+    case 1: return 64;
+    case 2: return 128;
+    case 3: return 32;
+    default: return 0;
+  }
+}
+
+int exc_bmt_max_blocks(int variant) {
+  switch (variant) {
+    case 0: return max_blocks_t<32, 16, 2, 2>();
+    case 1: return max_blocks_t<64, 16, 2, 2>();
+    case 2: return max_blocks_t<128, 16, 2, 4>();
+    case 3: return max_blocks_t<32, 32, 2, 2>();
+    default: return 1;
+  }
+}
+
+int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
+                   hipStream_t s) {
+  if (nitems <= 0) return EXC_OK;
+  switch (variant) {
+    case 0: return launch_bmt_t<32, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 1: return launch_bmt_t<64, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 2: return launch_bmt_t<128, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 3: return launch_bmt_t<32, 32, 2, 2>(d, ks, items, nitems, nblocks, s);
+    default: return EXC_ERR_ARG;
+  }
+}
+
+int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
+  if (nk <= 0 || d.nlotot == 0) return EXC_OK;
+  if (d.maxnlo > lo_dev::kMaxLo) return EXC_ERR_UNSUPPORTED;
+  if (maxngp > 0) {
+    dim3 grid((maxngp + lo_dev::kRows - 1) / lo_dev::kRows, d.natm, nk);
+    hipLaunchKernelGGL(blo_apw_kernel, grid, dim3(lo_dev::kRows * lo_dev::kSlots), 0, s, ks, d.naa, d.maxnlo, d.nlomax,
+                       d.apwordmax, d.nlotot, d.lod, d.loa, d.oalo);
+    EXC_CHECK(hipGetLastError());
+  }
+  dim3 grid2((d.nlotot + lo_dev::kLoThreads - 1) / lo_dev::kLoThreads, d.nlotot, nk);
+  hipLaunchKernelGGL(blo_lo_kernel, grid2, dim3(lo_dev::kLoThreads), 0, s, ks, d.nlotot, d.maxnlo, d.nlomax, d.lod,
+                     d.lolo, d.ololo);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

@@ -1,162 +1,19 @@
-// APW matching coefficients on gfx950 (replaces match.f90:83-187).
-//
-// One thread per (G+k vector, atom).  The thread evaluates j_l(|G+k|R) and
-// |G+k| j_l'(|G+k|R) for l <= LMAX, the complex spherical harmonics Y_lm(G+k)
-// (Condon–Shortley phase) by the three-term Legendre recurrence in l at fixed m,
-// and writes
-//   apwalm(ig, io, lm, ias) = (4 pi i^l / sqrt(Omega)) S(ig, ias) conj(Y_lm)
-//                             * (Dinv(io,0) j_l + Dinv(io,1) |G+k| j_l')
-// where Dinv is the inverse of the 2x2 matching matrix of (ias, l) (the zgesv solve of
-// match.f90:163 done once per SCF iteration on the host).  Consecutive threads take
-// consecutive ig, so every store of a fixed (io, lm, ias) column is coalesced.
-#include "exc_common.h"
+// APW matching coefficients of one k-point on gfx950 (match.f90:13); the device code is in
+// match_dev.h.
+#include "match_dev.h"
 
 namespace {
 
 constexpr int kMatchThreads = 64;
 
-// j_l(x), l = 0..L+1, into jl[]: series for x < 1e-8, upward recurrence for x > L,
-// Miller downward recurrence (normalised by j_0 or j_1 in closed form) otherwise.
-template <int L>
-__device__ __forceinline__ void sbessel_seq(double x, double (&jl)[L + 2]) {
-  constexpr int LL = L + 1;
-  if (x < 1e-8) {
-    double t = 1.0, dfac = 1.0;
-    jl[0] = 1.0;
-#pragma unroll
-    for (int l = 1; l <= LL; ++l) {
-      dfac *= (double)(2 * l + 1);
-      t *= x;
-      jl[l] = t / dfac;
-    }
-    return;
-  }
-  const double xi = 1.0 / x;
-  double sn, cs;
-  sincos(x, &sn, &cs);
-  if (x > (double)LL) {
-    double j0 = sn * xi;
-    double j1 = (j0 - cs) * xi;
-    jl[0] = j0;
-    jl[1] = j1;
-#pragma unroll
-    for (int l = 1; l < LL; ++l) {
-      double j2 = (double)(2 * l + 1) * xi * j1 - j0;
-      jl[l + 1] = j2;
-      j0 = j1;
-      j1 = j2;
-    }
-    return;
-  }
-  // downward: start well above L; values grow towards l = 0
-  const int lstart = LL + 20 + (int)ceil(x);
-  double jp1 = 0.0, jc = 1e-300;
-  for (int l = lstart; l > LL; --l) {
-    double jm1 = (double)(2 * l + 1) * xi * jc - jp1;
-    jp1 = jc;
-    jc = jm1;
-    if (fabs(jc) > 1e250) {
-      jc *= 1e-250;
-      jp1 *= 1e-250;
-    }
-  }
-#pragma unroll
-  for (int l = LL; l >= 0; --l) {
-    jl[l] = jc;
-    double jm1 = (double)(2 * l + 1) * xi * jc - jp1;
-    jp1 = jc;
-    jc = jm1;
-    if (fabs(jc) > 1e250) {
-      jc *= 1e-250;
-      jp1 *= 1e-250;
-#pragma unroll
-      for (int q = l; q <= LL; ++q) jl[q] *= 1e-250;
-    }
-  }
-  const double j0t = sn * xi;
-  const double j1t = sn * xi * xi - cs * xi;
-  const double scale = (fabs(j0t) >= fabs(j1t)) ? j0t / jl[0] : j1t / jl[1];
-#pragma unroll
-  for (int l = 0; l <= LL; ++l) jl[l] *= scale;
-}
-
 template <int L>
 __global__ __launch_bounds__(kMatchThreads) void match_kernel(
-    int ngp, int natm, int apwordmax, const int *__restrict__ spidx, const double *__restrict__ rmt,
-    double fpso, const double *__restrict__ dinv, const double *__restrict__ gpc,
-    const double *__restrict__ tpgpc, const cplx *__restrict__ sfacgp, int ldsf,
-    cplx *__restrict__ apwalm, int ldg) {
+    int ngp, int apwordmax, const int *__restrict__ spidx, const double *__restrict__ rmt, double fpso,
+    const double *__restrict__ dinv, const double *__restrict__ gpc, const double *__restrict__ tpgpc,
+    const cplx *__restrict__ sfacgp, int ldsf, cplx *__restrict__ apwalm, int ldg) {
   const int ig = blockIdx.x * kMatchThreads + threadIdx.x;
-  const int ias = blockIdx.y;
   if (ig >= ngp) return;
-  constexpr int lmmax = (L + 1) * (L + 1);
-  const double g = gpc[ig];
-  const double x = g * rmt[spidx[ias]];
-  double jl[L + 2];
-  sbessel_seq<L>(x, jl);
-  double djl[L + 1];
-#pragma unroll
-  for (int l = 0; l <= L; ++l) djl[l] = (x > 0.0) ? g * ((double)l / x * jl[l] - jl[l + 1]) : 0.0;
-
-  double st, ct;
-  sincos(tpgpc[2 * ig], &st, &ct);
-  const double phi = tpgpc[2 * ig + 1];
-  const cplx sf = sfacgp[ig + (size_t)ldsf * ias];
-  const double *di = dinv + (size_t)ias * (L + 1) * 4;
-  const size_t colstride = (size_t)ldg;
-  cplx *out = apwalm + ig + (size_t)ldg * apwordmax * lmmax * ias;
-
-  double pmm = 0.28209479177387814347;  // 1/sqrt(4 pi)
-#pragma unroll
-  for (int m = 0; m <= L; ++m) {
-    if (m > 0) pmm = -sqrt((double)(2 * m + 1) / (2.0 * m)) * st * pmm;
-    double sm, cm;
-    sincos((double)m * phi, &sm, &cm);
-    double p2 = 0.0, p1 = pmm;
-#pragma unroll
-    for (int l = m; l <= L; ++l) {
-      double p;
-      if (l == m) {
-        p = pmm;
-      } else if (l == m + 1) {
-        p = sqrt((double)(2 * m + 3)) * ct * pmm;
-      } else {
-        const double a = sqrt((4.0 * l * l - 1.0) / ((double)(l * l - m * m)));
-        const double b = sqrt(((l - 1.0) * (l - 1.0) - (double)(m * m)) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-        p = a * (ct * p1 - b * p2);
-      }
-      if (l > m) {
-        p2 = p1;
-        p1 = p;
-      }
-      // coefficient (4 pi / sqrt(Omega)) i^l S conj(Y_lm) for +m; -m uses Y_{l,-m} = (-1)^m conj(Y_lm)
-      const cplx ylm = {p * cm, p * sm};
-      cplx il;
-      switch (l & 3) {
-        case 0: il = {fpso, 0.0}; break;
-        case 1: il = {0.0, fpso}; break;
-        case 2: il = {-fpso, 0.0}; break;
-        default: il = {0.0, -fpso}; break;
-      }
-      const cplx c0 = cmul(il, sf);
-      const cplx cp = cmul(c0, cconj(ylm));                      // m >= 0
-      const double sgn = (m & 1) ? -1.0 : 1.0;
-      const cplx cn = {sgn * (c0.re * ylm.re - c0.im * ylm.im),   // conj(Y_{l,-m}) = (-1)^m Y_lm
-                       sgn * (c0.re * ylm.im + c0.im * ylm.re)};
-      const double d00 = di[l * 4 + 0], d01 = di[l * 4 + 2];    // Dinv(1,1), Dinv(1,2) column-major
-      const double d10 = di[l * 4 + 1], d11 = di[l * 4 + 3];
-      const double r0 = d00 * jl[l] + d01 * djl[l];
-      const double r1 = d10 * jl[l] + d11 * djl[l];
-      const int lmp = l * (l + 1) + m;
-      out[colstride * (0 + 2 * lmp)] = {cp.re * r0, cp.im * r0};
-      out[colstride * (1 + 2 * lmp)] = {cp.re * r1, cp.im * r1};
-      if (m > 0) {
-        const int lmn = l * (l + 1) - m;
-        out[colstride * (0 + 2 * lmn)] = {cn.re * r0, cn.im * r0};
-        out[colstride * (1 + 2 * lmn)] = {cn.re * r1, cn.im * r1};
-      }
-    }
-  }
+  match_dev::match_body<L>(ig, blockIdx.y, apwordmax, spidx, rmt, fpso, dinv, gpc, tpgpc, sfacgp, ldsf, apwalm, ldg);
 }
 
 template <int L>
@@ -164,14 +21,13 @@ int launch_match_l(const exc_dev &d, int ngp, const double *gpc, const double *t
                    const cplx *sfacgp, int ldsf, cplx *apwalm, int ldg, hipStream_t s) {
   dim3 grid((ngp + kMatchThreads - 1) / kMatchThreads, d.natm);
   const double fpso = 4.0 * M_PI / sqrt(d.omega);
-  hipLaunchKernelGGL(match_kernel<L>, grid, dim3(kMatchThreads), 0, s, ngp, d.natm, d.apwordmax,
-                     d.spidx, d.rmt, fpso, d.dinv, gpc, tpgpc, sfacgp, ldsf, apwalm, ldg);
+  hipLaunchKernelGGL(match_kernel<L>, grid, dim3(kMatchThreads), 0, s, ngp, d.apwordmax, d.spidx, d.rmt, fpso,
+                     d.dinv, gpc, tpgpc, sfacgp, ldsf, apwalm, ldg);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
 
 }  // namespace
-
 int exc_launch_match(const exc_dev &d, int ngp, const double *gpc, const double *tpgpc,
                      const cplx *sfacgp, int ldsf, cplx *apwalm, int ldg, hipStream_t s) {
   if (ngp <= 0) return EXC_OK;

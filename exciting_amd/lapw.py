@@ -39,6 +39,17 @@ class ExcDims(C.Structure):
                 ("ngrtot", C.c_int), ("ngvec", C.c_int), ("intgv", C.c_int * 6), ("omega", C.c_double)]
 
 
+class KPointDev(C.Structure):
+    """exc_kpoint_dev: one k-point of a grouped launch (device pointers)."""
+    _fields_ = [("ngp", C.c_int), ("ngkmax", C.c_int), ("ldh", C.c_int), ("reserved", C.c_int),
+                ("gpc", C.c_void_p), ("tpgpc", C.c_void_p), ("sfacgp", C.c_void_p), ("igpig", C.c_void_p),
+                ("vgpc", C.c_void_p), ("apwalm", C.c_void_p), ("h", C.c_void_p), ("o", C.c_void_p)]
+
+
+EXC_BATCH_MATCH = 1
+EXC_BATCH_GRAPH = 2
+EXC_BATCH_SPLIT_BF16 = 4
+
 _lib = None
 
 _P = C.c_void_p
@@ -61,10 +72,13 @@ SIGNATURES = {
     "exc_ctx_nlotot": [_P],
     "exc_ctx_set_timing": [_P, _I],
     "exc_ctx_read_timing": [_P, _P, _P],
+    "exc_ctx_set_variant": [_P, _I, _I],
+    "exc_ctx_num_variants": [_I],
+    "exc_setup_batched_dev": [_P, _I, _P, _I],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -219,6 +233,13 @@ class Context:
     def set_timing(self, on=True):
         self._call("exc_ctx_set_timing", self.h, 1 if on else 0)
 
+    def set_variant(self, stage, variant):
+        """Pin the tile variant of a stage ('ystage' or 'mt'); -1 = built-in choice."""
+        self._call("exc_ctx_set_variant", self.h, STAGES.index(stage), int(variant))
+
+    def num_variants(self, stage):
+        return self.lib.exc_ctx_num_variants(STAGES.index(stage))
+
     def read_timing(self):
         """{stage: (total_ms, launches)} recorded since the last read (synchronises the stream)."""
         n = 8
@@ -226,6 +247,10 @@ class Context:
         cnt = (C.c_int * n)()
         self._call("exc_ctx_read_timing", self.h, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))
         return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES)}
+
+    def setup_batched_dev(self, batch, flags=EXC_BATCH_MATCH):
+        """All k-points of `batch` (a DeviceBatch) in one grouped launch per stage."""
+        return self._call("exc_setup_batched_dev", self.h, batch.nk, C.cast(batch.desc, C.c_void_p), flags)
 
     def setup_dev(self, ngp, ngkmax, apwalm, igpig, vgpc, ldh, H, O):
         return self._call("exc_hamiltonandoverlapsetup_dev", self.h, ngp, ngkmax, _ptr(apwalm), _ptr(igpig),
@@ -261,3 +286,22 @@ def alloc_matrix_dev(ldh, nmat, device="cuda"):
     """A (ldh, nmat) Fortran-ordered complex matrix: stored as a (nmat, ldh) C tensor."""
     import torch
     return torch.empty((nmat, ldh), dtype=torch.complex128, device=device)
+
+
+class DeviceBatch:
+    """Device inputs and outputs of many k-points plus the exc_kpoint_dev array describing them."""
+
+    def __init__(self, cr, kpoints, device="cuda"):
+        self.dks = [DeviceKPoint(kp, device) for kp in kpoints]
+        self.nk = len(kpoints)
+        self.nmat = [kp.ngp + cr.nlotot for kp in kpoints]
+        self.apw = [alloc_apwalm_dev(cr, d.ngkmax, device) for d in self.dks]
+        self.H = [alloc_matrix_dev(n, n, device) for n in self.nmat]
+        self.O = [alloc_matrix_dev(n, n, device) for n in self.nmat]
+        self.desc = (KPointDev * self.nk)()
+        for i, d in enumerate(self.dks):
+            e = self.desc[i]
+            e.ngp, e.ngkmax, e.ldh = d.ngp, d.ngkmax, self.nmat[i]
+            e.gpc, e.tpgpc, e.sfacgp = _ptr(d.gpc), _ptr(d.tpgpc), _ptr(d.sfacgp)
+            e.igpig, e.vgpc = _ptr(d.igpig), _ptr(d.vgpc)
+            e.apwalm, e.h, e.o = _ptr(self.apw[i]), _ptr(self.H[i]), _ptr(self.O[i])

@@ -122,6 +122,35 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const dou
                                     const int *igpig, const double *vgpc, int ldh, double *h,
                                     double *o);
 
+/* ---- grouped setup of many k-points (no reference counterpart: the reference loops over
+ * k in scf_cycle.f90:241-276 and calls match + hamiltonsetup + overlapsetup per k). ----
+ * One entry per k-point, all pointers in device memory, layouts as for the per-k calls:
+ *   gpc(ngkmax), tpgpc(2,ngkmax), sfacgp(ngkmax,natmtot)  match inputs (used with EXC_BATCH_MATCH)
+ *   igpig(ngkmax), vgpc(3,ngkmax)                          basis of the k-point
+ *   apwalm(ngkmax,apwordmax,lmmaxapw,natmtot)              output of match / input of the setup
+ *   h(ldh,nmat), o(ldh,nmat)                               outputs, nmat = ngp + nlotot <= ldh */
+typedef struct exc_kpoint_dev {
+  int ngp, ngkmax, ldh, reserved;
+  const double *gpc, *tpgpc, *sfacgp;
+  const int *igpig;
+  const double *vgpc;
+  double *apwalm;
+  double *h, *o;
+} exc_kpoint_dev;
+
+#define EXC_BATCH_MATCH 1  /* compute apwalm of every k first (else apwalm is an input)        */
+#define EXC_BATCH_GRAPH 2  /* record the launches in a hipGraph and replay it on later calls   */
+                           /* with an identical batch (same pointers, sizes, flags, variants); */
+                           /* needs a context stream other than the null stream               */
+#define EXC_BATCH_SPLIT_BF16 4 /* muffin-tin contraction from split-BF16 slices with FP32 MFMA   */
+                               /* accumulation and per-(row, atom) power-of-two scaling (3 slices,*/
+                               /* 6 slice pairs); Y stage, interstitial, lo blocks stay FP64      */
+
+/* match (optional) + H and O of nk k-points, each with its own basis size, in one grouped
+ * launch per stage on the context's stream; the muffin-tin stage is a persistent kernel over
+ * a work list of (k, tile, H/O) items.  Results equal nk calls of the per-k entry points. */
+int exc_setup_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int flags);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
@@ -136,8 +165,19 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_YSTAGE 1
 #define EXC_STAGE_MT 2
 #define EXC_STAGE_LO 3
+#define EXC_STAGE_BMT 4      /* muffin-tin stage of exc_setup_batched_dev                     */
+#define EXC_STAGE_GRAPH 5    /* one replay of a batch's hipGraph                              */
+#define EXC_STAGE_SPLIT 6    /* slicing of A and Yt into BF16 planes (EXC_BATCH_SPLIT_BF16)   */
+#define EXC_STAGE_BSPLIT 7   /* split-BF16 muffin-tin stage of exc_setup_batched_dev          */
 #define EXC_NSTAGE 8
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
+
+/* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,
+ * EXC_STAGE_BSPLIT), as chosen by
+ * the autotuner (exciting_amd/autotune.py); -1 restores the built-in choice by basis size.
+ * exc_ctx_num_variants returns how many variants a stage has.  No reference counterpart. */
+int exc_ctx_set_variant(exc_ctx *ctx, int stage, int variant);
+int exc_ctx_num_variants(int stage);
 int exc_ctx_read_timing(exc_ctx *ctx, double *ms, int *launches);
 
 #ifdef __cplusplus

@@ -74,7 +74,7 @@ def sbessel_array(lmax: int, x: np.ndarray) -> np.ndarray:
         j0t = np.sin(xd) / xd
         j1t = np.sin(xd) / xd ** 2 - np.cos(xd) / xd
         use0 = np.abs(j0t) >= np.abs(j1t)
-        scale = np.where(use0, j0t / vals[0], j1t / np.where(use0, 1.0, vals[1]))
+        scale = np.where(use0, j0t, j1t) / np.where(use0, vals[0], vals[1])
         out[:, down] = vals[:lmax + 1] * scale
     return out
 
