@@ -39,6 +39,8 @@ struct exc_ctx {
   size_t planes_bytes;
   int *expo;                // per-(row, atom) exponents
   size_t expo_bytes;
+  cplx *vjp_work;           // adjoint workspace
+  size_t vjp_bytes;
   cplx *byt;                // Y-stage workspace of all k-points
   size_t byt_bytes;
   kdesc *d_ks;
@@ -217,6 +219,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->d_items) hipFree(c->d_items);
   if (c->planes) hipFree(c->planes);
   if (c->expo) hipFree(c->expo);
+  if (c->vjp_work) hipFree(c->vjp_work);
   delete c;
   return EXC_OK;
 }
@@ -691,6 +694,23 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     EXC_CHECK(hipGraphLaunch(c->graph_exec, c->stream));
   }
   return rc;
+}
+
+int exc_ho_vjp_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+                   const double *sfacgp, int nst, int ldc, const double *evec, const double *wh, const double *wo,
+                   double *dedtau, double *dedv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (!c->d.ffacg) return EXC_ERR_STATE;
+  const int nmat = ngp + c->d.nlotot;
+  if (ngp < 0 || ngkmax < ngp || nst < 0 || ldc < nmat || !dedtau || !dedv) return EXC_ERR_ARG;
+  if (ngp > 0 && nst > 0 && (!apwalm || !igpig || !vgpc || !sfacgp || !evec || !wh || !wo)) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t need = sizeof(cplx) * exc_vjp_work_elems(c->d, ngp, nst);
+  if ((rc = grow((void **)&c->vjp_work, &c->vjp_bytes, need ? need : 1))) return rc;
+  StageTimer t(c, EXC_STAGE_VJP);
+  return exc_launch_vjp(c->d, ngp, nst, (const cplx *)apwalm, ngkmax, igpig, vgpc, (const cplx *)sfacgp,
+                        (const cplx *)evec, ldc, wh, wo, c->vjp_work, dedtau, dedv, c->stream);
 }
 
 }  // extern "C"

@@ -106,6 +106,11 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
                    hipStream_t s);
 int exc_bmt_max_blocks(int variant);   // resident workgroups per CU of a grouped-MT variant
 int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
+// adjoint (vector-Jacobian product) of the setup for rank-nst cotangents
+size_t exc_vjp_work_elems(const exc_dev &d, int ngp, int nst);
+int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int ldg, const int *igpig,
+                   const double *vgpc, const cplx *sfacgp, const cplx *evec, int ldc, const double *wh,
+                   const double *wo, cplx *work, double *dedtau, double *dedv, hipStream_t s);
 // split-BF16 muffin-tin stage
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)

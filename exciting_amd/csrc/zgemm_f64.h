@@ -47,7 +47,7 @@ struct Core {
   // Load one BK slab of A (rows row0.., valid rows < mrows) and B into registers.  The
   // k-th step of the sum reads column kperm[k] of both operands: the callers order the
   // columns so that the small (high-l) terms are accumulated first, which keeps the long
-  // sequential MFMA accumulation as accurate as a blocked BLAS sum.
+  // sequential MFMA accumulation as accurate as a blocked BLAS sum (nullptr: natural order).
   __device__ __forceinline__ static void gload(const cplx *__restrict__ A, int lda, int row0, int mrows,
                                                const cplx *__restrict__ B, int ldb, int col0, int nrows,
                                                int k0, int ktot, const int *__restrict__ kperm, cplx (&ra)[LA],
@@ -58,14 +58,14 @@ struct Core {
       const int e = tid + kThreads * r;
       const int row = e % BM, k = e / BM;
       const int gr = row0 + row, gk = k0 + k;
-      ra[r] = (gr < mrows && gk < ktot) ? A[gr + (size_t)kperm[gk] * lda] : cplx{0.0, 0.0};
+      ra[r] = (gr < mrows && gk < ktot) ? A[gr + (size_t)(kperm ? kperm[gk] : gk) * lda] : cplx{0.0, 0.0};
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int e = tid + kThreads * r;
       const int row = e % BN, k = e / BN;
       const int gr = col0 + row, gk = k0 + k;
-      rb[r] = (gr < nrows && gk < ktot) ? B[gr + (size_t)kperm[gk] * ldb] : cplx{0.0, 0.0};
+      rb[r] = (gr < nrows && gk < ktot) ? B[gr + (size_t)(kperm ? kperm[gk] : gk) * ldb] : cplx{0.0, 0.0};
     }
   }
 

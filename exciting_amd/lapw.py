@@ -75,10 +75,11 @@ SIGNATURES = {
     "exc_ctx_set_variant": [_P, _I, _I],
     "exc_ctx_num_variants": [_I],
     "exc_setup_batched_dev": [_P, _I, _P, _I],
+    "exc_ho_vjp_dev": [_P, _I, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -242,7 +243,7 @@ class Context:
 
     def read_timing(self):
         """{stage: (total_ms, launches)} recorded since the last read (synchronises the stream)."""
-        n = 8
+        n = 12
         ms = (C.c_double * n)()
         cnt = (C.c_int * n)()
         self._call("exc_ctx_read_timing", self.h, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))
@@ -251,6 +252,11 @@ class Context:
     def setup_batched_dev(self, batch, flags=EXC_BATCH_MATCH):
         """All k-points of `batch` (a DeviceBatch) in one grouped launch per stage."""
         return self._call("exc_setup_batched_dev", self.h, batch.nk, C.cast(batch.desc, C.c_void_p), flags)
+
+    def vjp_dev(self, ngp, ngkmax, apwalm, igpig, vgpc, sfacgp, evec, ldc, nst, wh, wo, dedtau, dedv):
+        """Accumulate dL/dtau (3, natm) and dL/dV(G) for D = C diag(wh) C^H, W = C diag(wo) C^H."""
+        return self._call("exc_ho_vjp_dev", self.h, ngp, ngkmax, _ptr(apwalm), _ptr(igpig), _ptr(vgpc),
+                          _ptr(sfacgp), nst, ldc, _ptr(evec), _ptr(wh), _ptr(wo), _ptr(dedtau), _ptr(dedv))
 
     def setup_dev(self, ngp, ngkmax, apwalm, igpig, vgpc, ldh, H, O):
         return self._call("exc_hamiltonandoverlapsetup_dev", self.h, ngp, ngkmax, _ptr(apwalm), _ptr(igpig),

@@ -436,3 +436,22 @@ def make_small(natm_per_species=(1, 1), lmax=3, lorbl=(0, 1), n_target=40, nk=2,
     ngkmax = max(k.ngp for k in kps)
     kps = [pad_kpoint(k, ngkmax) for k in kps]
     return Workload(cfg=key, crystal=cr, kpoints=kps, ngkmax=ngkmax)
+
+
+def with_positions(cr: Crystal, atposc: np.ndarray) -> Crystal:
+    """The crystal with atoms moved to atposc (3, natm): Theta(G) is recomputed
+    (gencfun.f90:52-91); V(G), the muffin-tin matrices and the form factors are kept, as the
+    IBS force term holds them fixed (forcek.f90:267-275)."""
+    pos = np.asfortranarray(np.asarray(atposc, dtype=float))
+    cf, _ = step_function(cr.omega, cr.rmt, cr.natoms, cr.spidx, pos, cr.vgc, cr.gc)
+    return dataclasses.replace(cr, atposc=pos, cfunig=cf)
+
+
+def kpoint_at(kp: KPoint, cr: Crystal) -> KPoint:
+    """The k-point's structure factors for the positions of cr (gensfacgp.f90:38-49)."""
+    out = dataclasses.replace(kp)
+    sf = np.zeros_like(kp.sfacgp)
+    n = kp.ngp
+    sf[:n] = structure_factors(kp.vgpc[:, :n], cr.atposc)
+    out.sfacgp = sf
+    return out

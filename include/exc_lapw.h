@@ -151,6 +151,20 @@ typedef struct exc_kpoint_dev {
  * a work list of (k, tile, H/O) items.  Results equal nk calls of the per-k entry points. */
 int exc_setup_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int flags);
 
+/* Adjoint of the setup (forcek.f90:13, the IBS force term of one k-point).
+ * For the cotangents D = C diag(wh) C^H of H and W = C diag(wo) C^H of O, with
+ * C = evec(ldc, nst) (first-variational eigenvectors, ldc >= nmat), accumulates
+ *   dedtau(3, natmtot) += d/dtau Re sum_ij (D_ji H_ij + W_ji O_ij)
+ *   dedv(ngvec) complex += sum_{ij: G_i - G_j = g} D_ji  (holomorphic partial in V(g))
+ * where H and O depend on the atomic positions through the structure factors in apwalm and
+ * through Theta(G) (gencfun.f90), with V(G), h_a, loa, oalo held fixed as forcek does.
+ * With wh = w_k*occ and wo = -w_k*occ*eval, dedtau is minus forcek's contribution to
+ * forceibs.  sfacgp(ngkmax, natmtot) is the k-point's structure factors (gensfacgp.f90);
+ * ffacg must have been given to exc_ctx_set_gvec.  All pointers are device memory. */
+int exc_ho_vjp_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                   const double *vgpc, const double *sfacgp, int nst, int ldc, const double *evec,
+                   const double *wh, const double *wo, double *dedtau, double *dedv);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
@@ -169,7 +183,8 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_GRAPH 5    /* one replay of a batch's hipGraph                              */
 #define EXC_STAGE_SPLIT 6    /* slicing of A and Yt into BF16 planes (EXC_BATCH_SPLIT_BF16)   */
 #define EXC_STAGE_BSPLIT 7   /* split-BF16 muffin-tin stage of exc_setup_batched_dev          */
-#define EXC_NSTAGE 8
+#define EXC_STAGE_VJP 8      /* exc_ho_vjp_dev                                                */
+#define EXC_NSTAGE 12
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
 /* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,

@@ -203,12 +203,15 @@ def setup(cr, kp, apwalm=None):
 
 
 def eigen(H, O, nev=None):
-    """Lowest eigenpairs of H c = e O c via LAPACK zhegv (itype 1, 'V', 'U')."""
+    """Lowest eigenpairs of H c = e O c: all of them via LAPACK zhegv (itype 1, 'V', 'U'), or
+    the lowest nev via zhegvx with an index range, as solvewithlapack calls it
+    (modfvsystem.f90:691-693, ZHEGVX(1,'V','I','U',...))."""
+    if nev is not None and nev < H.shape[0]:
+        w, v = scipy.linalg.eigh(H, O, lower=False, subset_by_index=[0, nev - 1], driver="gvx")
+        return w, v
     w, v, info = scipy.linalg.lapack.zhegv(H, O, itype=1, jobz="V", uplo="U")
     if info != 0:
         raise np.linalg.LinAlgError(f"zhegv info={info}")
-    if nev is not None:
-        return w[:nev], v[:, :nev]
     return w, v
 
 
