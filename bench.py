@@ -67,6 +67,74 @@ def cpu_baseline(w, budget_s=15.0):
                       f"(numpy + host OpenBLAS, {dt:.1f} s)"}
 
 
+PEAK_BF16 = 2.5e15            # MI355X dense BF16 MFMA, flop/s
+
+
+def bsplit_exec_flops(cr, ngps, tile=64):
+    """BF16 MFMA flops the split kernel executes: every (tile, H|O) item runs 6 slice pairs x 4
+    real products over the padded K = natm*256."""
+    tot = 0.0
+    for n in ngps:
+        t = (n + tile - 1) // tile
+        tot += t * (t + 1) * tile * tile * (cr.natm * 256) * 24 * 2.0
+    return tot
+
+
+def eig_error(Hs, Os, Hf, Of, nev=200):
+    """max |e_split - e_fp64| over the lowest nev eigenvalues (LAPACK zhegvx on the host)."""
+    import scipy.linalg
+    m = min(nev, Hf.shape[0])
+    kw = dict(lower=False, subset_by_index=[0, m - 1], driver="gvx", eigvals_only=True)
+    e0 = scipy.linalg.eigh(Hf, Of, **kw)
+    e1 = scipy.linalg.eigh(Hs, Os, **kw)
+    return float(np.abs(e1 - e0).max()), float(e0[0])
+
+
+def run_split(args, ctx, batch, cr, w, fp64_value, log):
+    import torch
+    from exciting_amd import lapw
+    nk = batch.nk
+    Hf = batch.H[0].cpu().numpy().T.copy()
+    Of = batch.O[0].cpu().numpy().T.copy()
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_BF16
+    ctx.setup_batched_dev(batch, flags)          # eager pass: stage breakdown
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    ctx.setup_batched_dev(batch, flags)
+    ctx.set_timing(False)
+    eager = ctx.read_timing()
+    for k, (ms, cnt) in eager.items():
+        if cnt:
+            log(f"  split stage {k:7s}: {ms:8.3f} ms")
+    gflags = flags | lapw.EXC_BATCH_GRAPH
+    for _ in range(args.warmup):
+        ctx.setup_batched_dev(batch, gflags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.setup_batched_dev(batch, gflags)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = nk * args.steps / dt
+    Hs = batch.H[0].cpu().numpy().T.copy()
+    Os = batch.O[0].cpu().numpy().T.copy()
+    err, e0 = eig_error(Hs, Os, Hf, Of)
+    rel = float(np.linalg.norm(Hs - Hf) / np.linalg.norm(Hf))
+    ms = eager["bsplit"][0]
+    ex = bsplit_exec_flops(cr, [kp.ngp for kp in w.kpoints])
+    fl64 = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints)
+    log(f"  split-BF16: {value:.1f} k-points/s ({value / fp64_value:.2f}x FP64), lowest-200 eig err {err:.2e} Ha, "
+        f"|dH|/|H| {rel:.2e}")
+    return {"value": value, "unit": "k-points/s", "speedup_vs_fp64": value / fp64_value,
+            "ms_per_step": 1000.0 * dt / args.steps, "eig_err_lowest200_Ha_k0": err, "lowest_eig_Ha_k0": e0,
+            "rel_frobenius_dH_k0": rel, "meets_1e-8_Ha": err <= 1e-8,
+            "roofline": {"bound": "mfma", "kernel": "bsplit_kernel (grouped)", "achieved": ex / (ms * 1e-3) / 1e12,
+                         "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": ex / (ms * 1e-3) / PEAK_BF16,
+                         "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12},
+            "stages_ms": {k: v for k, (v, c) in eager.items() if c}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +144,7 @@ def main():
     ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune tile variants before timing")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 measurement")
     ap.add_argument("--path", choices=["batched", "perk"], default="batched",
                     help="batched: all k-points in one grouped launch per stage, replayed as a hipGraph; "
                          "perk: one launch sequence per k-point")
@@ -176,6 +245,10 @@ def main():
     if eager:
         stages["grouped_eager_pass"] = {k: ms for k, (ms, cnt) in eager.items() if cnt}
 
+    split = None
+    if args.path == "batched" and not args.no_split:
+        split = run_split(args, ctx, batch, cr, w, value, log)
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(w)
@@ -186,7 +259,7 @@ def main():
         "config": {"workload": f"config{args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
                    "path": ("FP64, all k-points grouped per stage, hipGraph replay" if args.path == "batched"
                             else "FP64, per-k launches"), "parallelism": "single GPU", "variants": tuned},
-        "roofline": roof, "stages": stages, "cpu_baseline": base,
+        "roofline": roof, "stages": stages, "split_bf16": split, "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:f987dcb8aa5fa01f
+﻿!mod$ v1 sum:fc1730b32ab5e370
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module exc_lapw_c
 use,intrinsic::iso_c_binding,only:c_associated
@@ -189,6 +189,26 @@ integer(4),value::ldh
 complex(8),intent(inout)::h(1_8:*)
 complex(8),intent(inout)::o(1_8:*)
 integer(4)::exc_hamiltonandoverlapsetup
+end
+end interface
+interface
+function exc_ho_vjp(ctx,ngp,ngkmax,apwalm,igpig,vgpc,sfacgp,nst,ldc,evec,wh,wo,dedtau,dedv) bind(c,name="exc_ho_vjp")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(4),value::ngp
+integer(4),value::ngkmax
+complex(8),intent(in)::apwalm(1_8:*)
+integer(4),intent(in)::igpig(1_8:*)
+real(8),intent(in)::vgpc(1_8:*)
+complex(8),intent(in)::sfacgp(1_8:*)
+integer(4),value::nst
+integer(4),value::ldc
+complex(8),intent(in)::evec(1_8:*)
+real(8),intent(in)::wh(1_8:*)
+real(8),intent(in)::wo(1_8:*)
+real(8),intent(inout)::dedtau(1_8:*)
+type(c_ptr),value::dedv
+integer(4)::exc_ho_vjp
 end
 end interface
 interface

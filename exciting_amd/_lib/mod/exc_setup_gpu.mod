@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:ac2d50277657c538
-!need$ f987dcb8aa5fa01f n exc_lapw_c
+﻿!mod$ v1 sum:c5940eccc3ea7dcc
+!need$ fc1730b32ab5e370 n exc_lapw_c
 !need$ 4e11352129309ed7 n exc_evsystem
 !need$ bb381bf46e508468 i __fortran_builtins
 !need$ 0bde2ac47243ead2 i iso_c_binding
@@ -94,6 +94,7 @@ use exc_lapw_c,only:exc_ctx_set_mt
 use exc_lapw_c,only:exc_ctx_set_gvec
 use exc_lapw_c,only:exc_match
 use exc_lapw_c,only:exc_hamiltonandoverlapsetup
+use exc_lapw_c,only:exc_ho_vjp
 use exc_lapw_c,only:exc_ctx_nlotot
 use exc_evsystem,only:hermitianmatrix
 use exc_evsystem,only:null
@@ -191,6 +192,7 @@ private::exc_ctx_set_mt
 private::exc_ctx_set_gvec
 private::exc_match
 private::exc_hamiltonandoverlapsetup
+private::exc_ho_vjp
 private::exc_ctx_nlotot
 private::hermitianmatrix
 private::null
@@ -247,5 +249,19 @@ integer(4),intent(in)::ngp
 complex(8),intent(in)::apwalm(1_8:int(ngkmax_,kind=8),1_8:*)
 integer(4),intent(in)::igpig(1_8:int(ngkmax_,kind=8))
 real(8),intent(in)::vgpc(1_8:3_8,1_8:int(ngkmax_,kind=8))
+end
+subroutine forcek_gpu(ngp,apwalm,igpig,vgpc,sfacgp,nmatmax,nstfv,evecfv,evalfv,occ,wkpt,forceibs)
+integer(4),intent(in)::ngp
+complex(8),intent(in)::apwalm(1_8:int(ngkmax_,kind=8),1_8:*)
+integer(4),intent(in)::igpig(1_8:int(ngkmax_,kind=8))
+real(8),intent(in)::vgpc(1_8:3_8,1_8:int(ngkmax_,kind=8))
+complex(8),intent(in)::sfacgp(1_8:int(ngkmax_,kind=8),1_8:*)
+integer(4),intent(in)::nmatmax
+integer(4),intent(in)::nstfv
+complex(8),intent(in)::evecfv(1_8:int(nmatmax,kind=8),1_8:int(nstfv,kind=8))
+real(8),intent(in)::evalfv(1_8:int(nstfv,kind=8))
+real(8),intent(in)::occ(1_8:int(nstfv,kind=8))
+real(8),intent(in)::wkpt
+real(8),intent(inout)::forceibs(1_8:3_8,1_8:int(natmtot_,kind=8))
 end
 end

@@ -713,4 +713,42 @@ int exc_ho_vjp_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
                         (const cplx *)evec, ldc, wh, wo, c->vjp_work, dedtau, dedv, c->stream);
 }
 
+int exc_ho_vjp(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+               const double *sfacgp, int nst, int ldc, const double *evec, const double *wh, const double *wo,
+               double *dedtau, double *dedv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  const int nmat = ngp + c->d.nlotot;
+  if (ngp < 0 || ngkmax < ngp || nst < 0 || ldc < nmat || !dedtau) return EXC_ERR_ARG;
+  if (ngp > 0 && nst > 0 && (!apwalm || !igpig || !vgpc || !sfacgp || !evec || !wh || !wo)) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t na = c->d.natm, nk = ngkmax;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_apw = 16 * nk * c->d.apwordmax * c->d.lmmaxapw * na, b_ig = 4 * nk, b_vg = 24 * nk;
+  const size_t b_sf = 16 * nk * na, b_ev = 16 * (size_t)ldc * nst, b_w = 8 * (size_t)nst;
+  const size_t b_dt = 8 * 3 * na, b_dv = 16 * (size_t)c->d.ngvec;
+  size_t off[9];
+  off[0] = 0;
+  const size_t sz[9] = {b_apw, b_ig, b_vg, b_sf, b_ev, b_w, b_w, b_dt, b_dv};
+  for (int i = 1; i < 9; ++i) off[i] = off[i - 1] + al(sz[i - 1]);
+  if ((rc = grow(&c->stage, &c->stage_bytes, off[8] + b_dv))) return rc;
+  char *base = (char *)c->stage;
+  const void *src[7] = {apwalm, igpig, vgpc, sfacgp, evec, wh, wo};
+  for (int i = 0; i < 7; ++i)
+    if (sz[i]) EXC_CHECK(hipMemcpyAsync(base + off[i], src[i], sz[i], hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemsetAsync(base + off[7], 0, b_dt + (off[8] - off[7] - b_dt) + b_dv, c->stream));
+  rc = exc_ho_vjp_dev(c, ngp, ngkmax, (const double *)(base + off[0]), (const int *)(base + off[1]),
+                      (const double *)(base + off[2]), (const double *)(base + off[3]), nst, ldc,
+                      (const double *)(base + off[4]), (const double *)(base + off[5]),
+                      (const double *)(base + off[6]), (double *)(base + off[7]), (double *)(base + off[8]));
+  if (rc) return rc;
+  std::vector<double> dt(3 * na), dv(dedv ? 2 * (size_t)c->d.ngvec : 0);
+  EXC_CHECK(hipMemcpyAsync(dt.data(), base + off[7], b_dt, hipMemcpyDeviceToHost, c->stream));
+  if (dedv) EXC_CHECK(hipMemcpyAsync(dv.data(), base + off[8], b_dv, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < dt.size(); ++i) dedtau[i] += dt[i];
+  for (size_t i = 0; i < dv.size(); ++i) dedv[i] += dv[i];
+  return EXC_OK;
+}
+
 }  // extern "C"

@@ -80,6 +80,18 @@ module exc_lapw_c
       complex(c_double_complex), intent(inout) :: h(*), o(*)
     end function
 
+    integer(c_int) function exc_ho_vjp(ctx, ngp, ngkmax, apwalm, igpig, vgpc, sfacgp, nst, ldc, evec, wh, wo, &
+                                       dedtau, dedv) bind(c, name="exc_ho_vjp")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, ngkmax, nst, ldc
+      complex(c_double_complex), intent(in) :: apwalm(*), sfacgp(*), evec(*)
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*), wh(*), wo(*)
+      real(c_double), intent(inout) :: dedtau(*)
+      type(c_ptr), value :: dedv          ! c_loc of a complex(8) dedv(ngvec), or c_null_ptr
+    end function
+
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

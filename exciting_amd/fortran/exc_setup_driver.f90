@@ -19,7 +19,7 @@ module exc_setup_gpu
   integer, save :: ngkmax_ = 0
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
-  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot
+  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu
 
 contains
 
@@ -105,5 +105,23 @@ contains
     call check(exc_hamiltonandoverlapsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, igpig, vgpc, &
                                            ldh, system%hamilton%za, system%overlap%za), "hamiltonandoverlapsetup")
   end subroutine hamiltonandoverlapsetup
+
+  ! forcek.f90:13 — the IBS force contribution of one k-point (spin-unpolarised,
+  ! first-variational states), added to forceibs(3, natmtot) with the k-point weight:
+  ! forceibs -= d/dtau sum_n wkpt occ_n e_n  (F = -dE/dtau, forcek.f90:371-383)
+  subroutine forcek_gpu(ngp, apwalm, igpig, vgpc, sfacgp, nmatmax, nstfv, evecfv, evalfv, occ, wkpt, forceibs)
+    integer, intent(in) :: ngp, nmatmax, nstfv
+    complex(8), intent(in) :: apwalm(ngkmax_, *), sfacgp(ngkmax_, *), evecfv(nmatmax, nstfv)
+    integer, intent(in) :: igpig(ngkmax_)
+    real(8), intent(in) :: vgpc(3, ngkmax_), evalfv(nstfv), occ(nstfv), wkpt
+    real(8), intent(inout) :: forceibs(3, natmtot_)
+    real(8) :: wh(nstfv), wo(nstfv), dedtau(3, natmtot_)
+    wh = wkpt * occ
+    wo = -wkpt * occ * evalfv
+    dedtau = 0.d0
+    call check(exc_ho_vjp(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, igpig, vgpc, sfacgp, &
+                          int(nstfv, c_int), int(nmatmax, c_int), evecfv, wh, wo, dedtau, c_null_ptr), "forcek")
+    forceibs = forceibs - dedtau
+  end subroutine forcek_gpu
 
 end module exc_setup_gpu

@@ -164,6 +164,11 @@ int exc_setup_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int fl
 int exc_ho_vjp_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const int *igpig,
                    const double *vgpc, const double *sfacgp, int nst, int ldc, const double *evec,
                    const double *wh, const double *wo, double *dedtau, double *dedv);
+/* The same with host arrays (for the Fortran driver's forcek); dedtau and dedv are
+ * accumulated in host memory.  dedv may be NULL. */
+int exc_ho_vjp(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+               const double *vgpc, const double *sfacgp, int nst, int ldc, const double *evec,
+               const double *wh, const double *wo, double *dedtau, double *dedv);
 
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);

@@ -30,3 +30,20 @@ def read_output(path, cr, ngkmax):
         H = np.fromfile(f, np.complex128, nmat * nmat).reshape((nmat, nmat), order="F")
         O = np.fromfile(f, np.complex128, nmat * nmat).reshape((nmat, nmat), order="F")
     return apw, H, O
+
+
+def write_eigen(path, C, e, occ, wkpt):
+    with open(path, "wb") as f:
+        np.asarray([C.shape[1]], dtype=np.int32).tofile(f)
+        np.asarray([wkpt], dtype=np.float64).tofile(f)
+        np.asfortranarray(C, dtype=np.complex128).ravel(order="F").tofile(f)
+        np.asarray(e, dtype=np.float64).tofile(f)
+        np.asarray(occ, dtype=np.float64).tofile(f)
+
+
+def read_forces(path, cr, ngkmax):
+    with open(path, "rb") as f:
+        nmat = int(np.fromfile(f, np.int32, 1)[0])
+        skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat
+        np.fromfile(f, np.complex128, skip)
+        return np.fromfile(f, np.float64, 3 * cr.natm).reshape((3, cr.natm), order="F")

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import read_output, write_input
+from fortran_io import read_forces, read_output, write_eigen, write_input
 
 pytestmark = pytest.mark.gpu
 
@@ -38,3 +38,25 @@ def test_fortran_driver(tmp_path, which):
     assert np.abs(apw[:n] - A_ref[:n]).max() <= 1e-12 * np.abs(A_ref[:n]).max()
     assert np.linalg.norm(H - H_ref) <= 1e-12 * np.linalg.norm(H_ref)
     assert np.linalg.norm(O - O_ref) <= 1e-12 * np.linalg.norm(O_ref)
+
+
+def test_fortran_forcek(tmp_path):
+    """forcek_gpu (Fortran, through exc_ho_vjp) against the restated reference forcek."""
+    from oracle import lapw_ref as R
+    w = small("b")
+    cr, kp = w.crystal, w.kpoints[0]
+    H, O, A = R.setup(cr, kp)
+    nst = 10
+    e, C = R.eigen(H, O, nst)
+    occ = np.random.default_rng(2).uniform(0.0, 2.0, nst)
+    wk = 0.25
+    fin, fout, feig = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "eig.bin"
+    write_input(fin, cr, kp)
+    write_eigen(feig, C, e, occ, wk)
+    r = subprocess.run([EXE, str(fin), str(fout), str(feig)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    F = read_forces(fout, cr, kp.igpig.shape[0])
+    F_ref = R.forcek_ibs(cr, kp, C, e, occ, apwalm=A, wkpt=wk)
+    print("forceibs max |diff|", np.abs(F - F_ref).max(), "max |F|", np.abs(F_ref).max())
+    assert np.abs(F - F_ref).max() <= 1e-9 * max(1.0, np.abs(F_ref).max())
