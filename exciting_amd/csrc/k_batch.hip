@@ -59,11 +59,11 @@ int launch_bystage_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
   return EXC_OK;
 }
 
-template <int BM, int BK, int WR, int WC>
+template <int BM, int BK, int WR, int WC, bool DB>
 __global__ __launch_bounds__(64 * WR * WC) void bmt_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
                                                            int nitems, int ktot, GTab gt,
                                                            const int *__restrict__ kperm) {
-  using T = MtTile<BM, BK, WR, WC>;
+  using T = MtTile<BM, BK, WR, WC, DB>;
   __shared__ typename T::Shared sh;
   // Workgroup b runs on XCD b % 8 (round-robin dispatch).  Each XCD walks its own
   // contiguous eighth of the work list, so the k-points it touches stay in its L2.
@@ -80,18 +80,18 @@ __global__ __launch_bounds__(64 * WR * WC) void bmt_kernel(const kdesc *__restri
   }
 }
 
-template <int BM, int BK, int WR, int WC>
+template <int BM, int BK, int WR, int WC, bool DB = false>
 int launch_bmt_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((bmt_kernel<BM, BK, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
+  hipLaunchKernelGGL((bmt_kernel<BM, BK, WR, WC, DB>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
                      d.natm * d.naa, make_gtab(d), d.kperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
 
-template <int BM, int BK, int WR, int WC>
+template <int BM, int BK, int WR, int WC, bool DB = false>
 int max_blocks_t() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_kernel<BM, BK, WR, WC>, 64 * WR * WC, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_kernel<BM, BK, WR, WC, DB>, 64 * WR * WC, 0) != hipSuccess)
     return 1;
   return n > 0 ? n : 1;
 }
@@ -156,6 +156,9 @@ int exc_bmt_tile(int variant) {
     case 1: return 64;
     case 2: return 128;
     case 3: return 32;
+    case 4: return 32;
+    case 5: return 64;
+    case 6: return 64;
     default: return 0;
   }
 }
@@ -166,6 +169,9 @@ int exc_bmt_max_blocks(int variant) {
     case 1: return max_blocks_t<64, 16, 2, 2>();
     case 2: return max_blocks_t<128, 16, 2, 4>();
     case 3: return max_blocks_t<32, 32, 2, 2>();
+    case 4: return max_blocks_t<32, 16, 2, 2, true>();
+    case 5: return max_blocks_t<64, 16, 2, 2, true>();
+    case 6: return max_blocks_t<64, 16, 2, 4, true>();
     default: return 1;
   }
 }
@@ -178,6 +184,9 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 1: return launch_bmt_t<64, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
     case 2: return launch_bmt_t<128, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 3: return launch_bmt_t<32, 32, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 4: return launch_bmt_t<32, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
+    case 5: return launch_bmt_t<64, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
+    case 6: return launch_bmt_t<64, 16, 2, 4, true>(d, ks, items, nitems, nblocks, s);
     default: return EXC_ERR_ARG;
   }
 }

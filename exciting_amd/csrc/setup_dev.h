@@ -79,12 +79,12 @@ struct YTile {
   }
 };
 
-template <int BM, int BK, int WR, int WC>
+template <int BM, int BK, int WR, int WC, bool DB = false>
 struct MtTile {
   static constexpr int BN = BM;
   using C = Core<BM, BN, BK, WR, WC, true>;
   struct Shared {
-    Smem<BM, BN, BK> sm;
+    Smem<BM, BN, BK> sm[DB ? 2 : 1];
     int gr[BM][3], gc[BN][3];
     double vr[BM][3], vc[BN][3];
   };
@@ -119,7 +119,10 @@ struct MtTile {
     C core;
     const cplx *B = which ? apwalm : yt;
     const int ldb = which ? ldg : ldy;
-    core.run(sh.sm, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm);
+    if constexpr (DB)
+      core.run2(sh.sm, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm);
+    else
+      core.run(sh.sm[0], apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm);
 
     cplx *out = which ? o : h;
     const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;

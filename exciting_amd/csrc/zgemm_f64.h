@@ -152,6 +152,30 @@ struct Core {
     }
   }
 
+  // Full K loop with two LDS stages: the next stage is stored while the current one is
+  // multiplied, one barrier per stage.
+  __device__ __forceinline__ void run2(Smem<BM, BN, BK> (&sm)[2], const cplx *__restrict__ A, int lda, int row0,
+                                       int mrows, const cplx *__restrict__ B, int ldb, int col0, int nrows, int ktot,
+                                       const int *__restrict__ kperm) {
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave / WC, wc = wave % WC;
+    zero();
+    cplx ra[LA], rb[LB];
+    gload(A, lda, row0, mrows, B, ldb, col0, nrows, 0, ktot, kperm, ra, rb);
+    __syncthreads();
+    sstore(sm[0], ra, rb);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < ktot; k0 += BK) {
+      const bool more = k0 + BK < ktot;
+      if (more) gload(A, lda, row0, mrows, B, ldb, col0, nrows, k0 + BK, ktot, kperm, ra, rb);
+      compute(sm[cur], wr, wc);
+      if (more) sstore(sm[cur ^ 1], ra, rb);
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+
   // Accumulator element (v) of block (i, j) sits at tile row / column (the output layout of
   // v_mfma_f64_16x16x4f64, measured on gfx950: register v of lane l holds row 4v + l/16):
   //   row = wr*WM + i*16 + 4*v + lane/16,  col = wc*WN + j*16 + lane%16.

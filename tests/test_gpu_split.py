@@ -2,7 +2,8 @@
 
 * against the numpy emulation of the same slicing (oracle/split_ref.py): slice products
   are exact, so H and O may differ only by the FP32 rounding of the kernel's second
-  accumulation level — relative Frobenius error <= 1e-9 of H_MT / O_MT;
+  accumulation level (about 2^-27 sqrt(K) of the block scale) — relative Frobenius error
+  <= 1e-8 of H_MT / O_MT, twenty times below the split's own error against FP64;
 * against the FP64 oracle: relative Frobenius error of H and O <= 1e-6, and the
   eigenvalue error is printed (this seeded spectrum is ill-conditioned; its size is
   reported by bench.py and discussed in DESIGN.md, not asserted at 1e-8 Ha here).
@@ -64,5 +65,5 @@ def test_split_against_emulation_and_fp64(which):
         m = min(200, len(e))
         print(f"{which} k{i} N={n}: vs emulation H {d_em_h:.2e} O {d_em_o:.2e}; vs FP64 H {d64_h:.2e} "
               f"O {d64_o:.2e}; max|d eig| lowest {m}: {np.abs(e[:m] - e_ref[:m]).max():.2e} Ha", flush=True)
-        assert d_em_h <= 1e-9 and d_em_o <= 1e-9
+        assert d_em_h <= 1e-8 and d_em_o <= 1e-8
         assert d64_h <= 1e-6 and d64_o <= 1e-6
