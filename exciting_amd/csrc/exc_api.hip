@@ -467,11 +467,12 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
   EXC_CHECK(hipMemcpyAsync(d_gpc, gpc, b_gpc, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(d_tp, tpgpc, b_tp, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(d_sf, sfacgp, b_sf, hipMemcpyHostToDevice, c->stream));
-  // rows ngp+1..ngkmax are not written by match: carry the caller's values through
-  EXC_CHECK(hipMemcpyAsync(d_apw, apwalm, b_apw, hipMemcpyHostToDevice, c->stream));
   rc = exc_match_dev(c, ngp, ngkmax, d_gpc, d_tp, d_sf, d_apw);
   if (rc < 0) return rc;
-  EXC_CHECK(hipMemcpyAsync(apwalm, d_apw, b_apw, hipMemcpyDeviceToHost, c->stream));
+  // only rows 1..ngp of every column are results; the caller's padding rows stay untouched
+  if (ngp > 0)
+    EXC_CHECK(hipMemcpy2DAsync(apwalm, 16 * nk, d_apw, 16 * nk, 16 * (size_t)ngp, ao * lmm * na,
+                               hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -523,20 +524,19 @@ int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *a
   const size_t o_h = o_vg + ((b_vg + 255) & ~(size_t)255), o_o = o_h + ((b_h + 255) & ~(size_t)255);
   if ((rc = grow(&c->stage, &c->stage_bytes, o_o + b_h))) return rc;
   char *base = (char *)c->stage;
-  EXC_CHECK(hipMemcpyAsync(base, apwalm, b_apw, hipMemcpyHostToDevice, c->stream));
+  if (ngp > 0)   // rows 1..ngp of apwalm are the input; padding rows are never read
+    EXC_CHECK(hipMemcpy2DAsync(base, 16 * nk, apwalm, 16 * nk, 16 * (size_t)ngp, ao * lmm * na,
+                               hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_ig, igpig, b_ig, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_vg, vgpc, b_vg, hipMemcpyHostToDevice, c->stream));
-  if (ldh > nmat) {
-    // rows nmat+1..ldh of the caller's matrices are not part of the result: keep them
-    EXC_CHECK(hipMemcpyAsync(base + o_h, h, b_h, hipMemcpyHostToDevice, c->stream));
-    EXC_CHECK(hipMemcpyAsync(base + o_o, o, b_h, hipMemcpyHostToDevice, c->stream));
-  }
   rc = exc_hamiltonandoverlapsetup_dev(c, ngp, ngkmax, (const double *)base, (const int *)(base + o_ig),
                                        (const double *)(base + o_vg), ldh, (double *)(base + o_h),
                                        (double *)(base + o_o));
   if (rc) return rc;
-  EXC_CHECK(hipMemcpyAsync(h, base + o_h, b_h, hipMemcpyDeviceToHost, c->stream));
-  EXC_CHECK(hipMemcpyAsync(o, base + o_o, b_h, hipMemcpyDeviceToHost, c->stream));
+  // rows nmat+1..ldh of the caller's matrices are not part of the result: copy rows 1..nmat
+  const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
+  EXC_CHECK(hipMemcpy2DAsync(h, pitch, base + o_h, pitch, width, nmat, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpy2DAsync(o, pitch, base + o_o, pitch, width, nmat, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   return EXC_OK;
 }

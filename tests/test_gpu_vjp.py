@@ -9,7 +9,7 @@ W = -C (f e) C^H, as the contract states.
   atomic positions: backward against central finite differences of the forward with step
   1e-4 bohr, Richardson-extrapolated with step 5e-5 (the seeded spectra reach -4000 Ha, so
   |dE/dtau| ~ 1e4 Ha/bohr and the plain central difference's h^2 term alone exceeds the
-  bar); |error| <= 1e-6 Ha/bohr;
+  bar), contracting the differences of H and O with the cotangents; |error| <= 1e-6 Ha/bohr;
 * dE/dV(G) against the exact linear change of L under Hermitian perturbations of V(G).
 """
 import numpy as np
@@ -47,7 +47,7 @@ class Runner:
         self.D = (C * self.wh) @ C.conj().T
         self.W = (C * self.wo) @ C.conj().T
 
-    def forward(self, pos, veff=None):
+    def matrices(self, pos, veff=None):
         cr = synth.with_positions(self.cr, pos)
         if veff is not None:
             cr.veffig = veff
@@ -55,7 +55,14 @@ class Runner:
         self.ctx.upload(cr)
         A = self.ctx.match(kp)
         H, O = self.ctx.hamiltonandoverlapsetup(kp, A)
-        return float(np.real(np.sum(self.D.T * H) + np.sum(self.W.T * O))), cr, kp, A
+        return H, O, cr, kp, A
+
+    def contract(self, H, O):
+        return float(np.real(np.sum(self.D.T * H) + np.sum(self.W.T * O)))
+
+    def forward(self, pos, veff=None):
+        H, O, cr, kp, A = self.matrices(pos, veff)
+        return self.contract(H, O), cr, kp, A
 
     def adjoint(self, cr, kp, A):
         torch = self.torch
@@ -118,9 +125,13 @@ def _check(cr, kp, nst, natoms_fd):
     worst, worst_plain = 0.0, 0.0
 
     def central(a, l, h):
+        # contract the differences H(+h) - H(-h), O(+h) - O(-h): L itself is a near-cancelling
+        # sum of terms up to |e| ~ 4e3 Ha, so L(+h) - L(-h) would lose the digits the check needs
         p = cr.atposc.copy(); p[l, a] += h
         m = cr.atposc.copy(); m[l, a] -= h
-        return (run.forward(p)[0] - run.forward(m)[0]) / (2 * h)
+        Hp, Op = run.matrices(p)[:2]
+        Hm, Om = run.matrices(m)[:2]
+        return run.contract(Hp - Hm, Op - Om) / (2 * h)
 
     for a in range(natoms_fd):
         for l in range(3):
