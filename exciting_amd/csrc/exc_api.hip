@@ -557,7 +557,9 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (q.ngp > 0 && (!q.apwalm || !q.igpig || !q.vgpc)) return EXC_ERR_ARG;
     if (do_match && q.ngp > 0 && (!q.gpc || !q.tpgpc || !q.sfacgp)) return EXC_ERR_ARG;
   }
-  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 0) : (c->bmt_variant >= 0 ? c->bmt_variant : 0);
+  // defaults: the fastest variants measured on MI355X for N = 1150..2700 (grouped MT: 64x64
+  // tile, 8 waves, double-buffered LDS)
+  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 0) : (c->bmt_variant >= 0 ? c->bmt_variant : 6);
   const int yv = c->y_variant >= 0 ? c->y_variant : 1;
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
@@ -627,10 +629,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
       const int nt = (q.ngp + T - 1) / T;
+      const bool fused = !split && exc_bmt_fused(bv);
       for (int tj = 0; tj < nt; ++tj)
         for (int ti = 0; ti <= tj; ++ti) {
           items.push_back({k, ti, tj, 0});
-          items.push_back({k, ti, tj, 1});
+          if (!fused) items.push_back({k, ti, tj, 1});
         }
     }
     if ((rc = grow((void **)&c->d_ks, &c->ks_cap, sizeof(kdesc) * nk))) return rc;

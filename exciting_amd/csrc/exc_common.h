@@ -97,11 +97,12 @@ int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const 
                   hipStream_t s);
 constexpr int kYstageVariants = 4;
 constexpr int kMtVariants = 6;
-constexpr int kBmtVariants = 7;
+constexpr int kBmtVariants = 10;
 // Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.
 int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, int variant, hipStream_t s);
 int exc_bmt_tile(int variant);   // square tile edge of a grouped-MT variant
+bool exc_bmt_fused(int variant);  // the variant computes H and O of a tile pair in one item
 int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                    hipStream_t s);
 int exc_bmt_max_blocks(int variant);   // resident workgroups per CU of a grouped-MT variant
@@ -114,7 +115,7 @@ int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int l
 // split-BF16 muffin-tin stage
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
-constexpr int kBsplitVariants = 2;
+constexpr int kBsplitVariants = 4;
 size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);

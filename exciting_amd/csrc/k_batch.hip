@@ -96,6 +96,41 @@ int max_blocks_t() {
   return n > 0 ? n : 1;
 }
 
+// fused H+O items: one work item per (k, tile row, tile column)
+template <int BM, int BK, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void bmt_ho_kernel(const kdesc *__restrict__ ks,
+                                                              const mt_item *__restrict__ items, int nitems, int ktot,
+                                                              GTab gt, const int *__restrict__ kperm) {
+  using T = MtTileHO<BM, BK, WR, WC>;
+  __shared__ typename T::Shared sh;
+  const int nx = gridDim.x >= 8 ? 8 : 1;
+  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
+  const int chunk = (nitems + nx - 1) / nx;
+  const int end = min(nitems, (x + 1) * chunk);
+  for (int it = x * chunk + lb; it < end; it += per) {
+    const mt_item w = items[it];
+    const kdesc k = ks[w.k];
+    __syncthreads();
+    T::run(sh, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h, k.o, k.ldh, kperm, w.ti, w.tj);
+  }
+}
+
+template <int BM, int BK, int WR, int WC>
+int launch_bmt_ho_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((bmt_ho_kernel<BM, BK, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
+                     d.natm * d.naa, make_gtab(d), d.kperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BM, int BK, int WR, int WC>
+int max_blocks_ho_t() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_ho_kernel<BM, BK, WR, WC>, 64 * WR * WC, 0) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
 __global__ __launch_bounds__(lo_dev::kRows * lo_dev::kSlots) void blo_apw_kernel(const kdesc *__restrict__ ks, int naa,
                                                                                   int maxnlo, int nlomax, int apwordmax,
                                                                                   int nlotot,
@@ -159,9 +194,14 @@ int exc_bmt_tile(int variant) {
     case 4: return 32;
     case 5: return 64;
     case 6: return 64;
+    case 7: return 64;
+    case 8: return 64;
+    case 9: return 32;
     default: return 0;
   }
 }
+
+bool exc_bmt_fused(int variant) { return variant >= 7; }
 
 int exc_bmt_max_blocks(int variant) {
   switch (variant) {
@@ -172,6 +212,9 @@ int exc_bmt_max_blocks(int variant) {
     case 4: return max_blocks_t<32, 16, 2, 2, true>();
     case 5: return max_blocks_t<64, 16, 2, 2, true>();
     case 6: return max_blocks_t<64, 16, 2, 4, true>();
+    case 7: return max_blocks_ho_t<64, 16, 2, 4>();
+    case 8: return max_blocks_ho_t<64, 8, 2, 4>();
+    case 9: return max_blocks_ho_t<32, 16, 2, 2>();
     default: return 1;
   }
 }
@@ -187,6 +230,9 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 4: return launch_bmt_t<32, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
     case 5: return launch_bmt_t<64, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
     case 6: return launch_bmt_t<64, 16, 2, 4, true>(d, ks, items, nitems, nblocks, s);
+    case 7: return launch_bmt_ho_t<64, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 8: return launch_bmt_ho_t<64, 8, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 9: return launch_bmt_ho_t<32, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -307,6 +307,214 @@ __global__ __launch_bounds__(64 * WR * WC) void bsplit_kernel(const kdesc *__res
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// bsplit2: the same arithmetic with a larger tile.  Four waves (2 x 2), wave tile
+// (32 MI) x (32 NJ), workgroup tile BM x BN = (64 MI) x (64 NJ).  One 16-k chunk per LDS
+// stage, two stages in flight, filled by global_load_lds_dwordx4 (no staging registers);
+// operand slices are read from LDS per slice pair just before their MFMAs; FP64 totals,
+// folded per atom block exactly as bsplit_kernel.  An LDS row is 13 segments of 16 bytes (12 data + 1 pad) so that the
+// 32 rows one MFMA operand read touches fall on distinct banks; the pad lanes are masked
+// off the DMA.
+// ---------------------------------------------------------------------------------------
+constexpr int SEG = 16;                     // bytes per load
+constexpr int RSEG = ROWB / SEG;            // data segments per row and chunk: 12
+constexpr int PSEG = RSEG + 1;              // with the pad: 13
+
+template <int MI, int NJ>
+struct Split2Cfg {
+  static constexpr int BM = 64 * MI, BN = 64 * NJ;
+  static constexpr int ABYTES = BM * PSEG * SEG, BBYTES = BN * PSEG * SEG;
+  static constexpr int ALOADS = BM * PSEG / 64, BLOADS = BN * PSEG / 64;   // wave-wide loads per stage
+  struct Shared {
+    unsigned char buf[2][ABYTES + BBYTES];
+    int ea[2][BM], eb[2][BN];
+  };
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// DMA of one chunk of the A and B row panels into an LDS stage.  rowb = bytes between
+// consecutive rows of a slice plane set (natm * NCH records of ROWB bytes).
+template <int MI, int NJ>
+__device__ __forceinline__ void split2_issue(unsigned char *stage, const unsigned char *pa, const unsigned char *pb,
+                                             int row0, int col0, int ngp, unsigned rowb) {
+  using S = Split2Cfg<MI, NJ>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma nounroll
+  for (int b = wave; b < S::ALOADS + S::BLOADS; b += 4) {
+    const bool isa = b < S::ALOADS;
+    const int bb = isa ? b : b - S::ALOADS;
+    const int L = bb * 64 + lane;
+    const int rr = L / PSEG, seg = L - rr * PSEG;
+    int gr = (isa ? row0 : col0) + rr;
+    if (gr >= ngp) gr = 0;                  // rows past N: any valid row, never stored
+    const unsigned char *src = (isa ? pa : pb) + (gr * rowb + (unsigned)seg * SEG);
+    unsigned char *dst = stage + (isa ? 0 : S::ABYTES) + bb * 64 * SEG;
+    if (seg < RSEG) __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)dst, 16, 0, 0);
+  }
+}
+
+template <int MI, int NJ>
+__global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                      int nitems, int natm, GTab gt) {
+  using S = Split2Cfg<MI, NJ>;
+  constexpr int BM = S::BM, BN = S::BN;
+  __shared__ typename S::Shared sh;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int nx = gridDim.x >= 8 ? 8 : 1;
+  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
+  const int chunk = (nitems + nx - 1) / nx;
+  const int end = min(nitems, (x + 1) * chunk);
+  const unsigned rowb = (unsigned)natm * NCH * ROWB;
+  for (int it = x * chunk + lb; it < end; it += per) {
+    const mt_item w = items[it];
+    const kdesc k = ks[w.k];
+    const int ngp = k.ngp;
+    const int row0 = w.ti * BM, col0 = w.tj * BN;
+    const unsigned char *PA = (const unsigned char *)k.sa;
+    const unsigned char *PB = (const unsigned char *)(w.which ? k.sa : k.sy);
+    const int *EA = k.ea;
+    const int *EB = w.which ? k.ea : k.ey;
+    double tre[MI][NJ][16], tim[MI][NJ][16];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) tre[i][j][v] = tim[i][j][v] = 0.0;
+    __syncthreads();                         // previous item done with both stages
+    split2_issue<MI, NJ>(sh.buf[0], PA, PB, row0, col0, ngp, rowb);
+    for (int atom = 0; atom < natm; ++atom) {
+      // exponents of this atom block; buffer atom&1 was last read by the fold of atom-2,
+      // which precedes at least NCH barriers
+      int *ea = sh.ea[atom & 1], *eb = sh.eb[atom & 1];
+      for (int e = tid; e < BM + BN; e += 256) {
+        const bool isr = e < BM;
+        const int idx = isr ? row0 + e : col0 + (e - BM);
+        const int ex = idx < ngp ? (isr ? EA : EB)[(size_t)idx * natm + atom] : 0;
+        if (isr) ea[e] = ex; else eb[e - BM] = ex;
+      }
+      f32x16 a0r[MI][NJ], a0i[MI][NJ], a1r[MI][NJ], a1i[MI][NJ];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) a0r[i][j] = a0i[i][j] = a1r[i][j] = a1i[i][j] = f32x16{};
+      for (int c = 0; c < NCH; ++c) {
+        const int cg = atom * NCH + c;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();                     // chunk cg landed; all waves done with the other stage
+        if (cg + 1 < natm * NCH) {
+          const unsigned off = (unsigned)(cg + 1) * ROWB;      // record (atom, c) of row 0
+          split2_issue<MI, NJ>(sh.buf[(cg + 1) & 1], PA + off, PB + off, row0, col0, ngp, rowb);
+        }
+        const unsigned char *sa = sh.buf[cg & 1] + (wr * 32 * MI + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
+        const unsigned char *sb = sh.buf[cg & 1] + S::ABYTES + (wc * 32 * NJ + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
+        // pairs in the order of bsplit_kernel: (0,0) | (0,1) (1,0) | (0,2) (1,1) (2,0)
+#pragma unroll
+        for (int pr = 0; pr < 6; ++pr) {
+          const int sl = (pr == 2 || pr == 4) ? 1 : (pr == 5 ? 2 : 0);
+          const int tt = (pr == 1 || pr == 4) ? 1 : (pr == 3 ? 2 : 0);
+          bf16x8 ar[MI], ai[MI], na[MI], br[NJ], bi[NJ];
+#pragma unroll
+          for (int i = 0; i < MI; ++i) {
+            ar[i] = *(const bf16x8 *)(sa + i * 32 * PSEG * SEG + (2 * sl) * 32);
+            ai[i] = *(const bf16x8 *)(sa + i * 32 * PSEG * SEG + (2 * sl + 1) * 32);
+            uint4 u = *(const uint4 *)&ai[i];
+            u.x ^= 0x80008000u; u.y ^= 0x80008000u; u.z ^= 0x80008000u; u.w ^= 0x80008000u;
+            na[i] = *(const bf16x8 *)&u;
+          }
+#pragma unroll
+          for (int j = 0; j < NJ; ++j) {
+            br[j] = *(const bf16x8 *)(sb + j * 32 * PSEG * SEG + (2 * tt) * 32);
+            bi[j] = *(const bf16x8 *)(sb + j * 32 * PSEG * SEG + (2 * tt + 1) * 32);
+          }
+#pragma unroll
+          for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+              if (pr == 0) {
+                a0r[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[i], br[j], a0r[i][j], 0, 0, 0);
+                a0r[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai[i], bi[j], a0r[i][j], 0, 0, 0);
+                a0i[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[i], bi[j], a0i[i][j], 0, 0, 0);
+                a0i[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na[i], br[j], a0i[i][j], 0, 0, 0);
+              } else {
+                a1r[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[i], br[j], a1r[i][j], 0, 0, 0);
+                a1r[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ai[i], bi[j], a1r[i][j], 0, 0, 0);
+                a1i[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[i], bi[j], a1i[i][j], 0, 0, 0);
+                a1i[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(na[i], br[j], a1i[i][j], 0, 0, 0);
+              }
+            }
+        }
+      }
+      // fold the atom block into the FP64 totals: x 2^(ea(row) + eb(col) - 16)
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int cl = wc * (32 * NJ) + j * 32 + (lane & 31);
+          const int ebc = eb[cl] - 16;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int rl = wr * (32 * MI) + i * 32 + d32_row(v, lane);
+            const int sc = ea[rl] + ebc;
+            tre[i][j][v] += ldexp((double)a0r[i][j][v] + (double)a1r[i][j][v], sc);
+            tim[i][j][v] += ldexp((double)a0i[i][j][v] + (double)a1i[i][j][v], sc);
+          }
+        }
+    }
+    // interstitial + mirror epilogue (as the FP64 path); G vectors straight from global
+    cplx *out = w.which ? k.o : k.h;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int c = col0 + wc * (32 * NJ) + j * 32 + (lane & 31);
+      if (c >= ngp) continue;
+      const int igc = k.igpig[c] - 1;
+      const int gc0 = gt.ivg[3 * igc], gc1 = gt.ivg[3 * igc + 1], gc2 = gt.ivg[3 * igc + 2];
+      const double vc0 = k.vgpc[3 * c], vc1 = k.vgpc[3 * c + 1], vc2 = k.vgpc[3 * c + 2];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = row0 + wr * (32 * MI) + i * 32 + d32_row(v, lane);
+          if (r > c) continue;
+          double re = tre[i][j][v], im = tim[i][j][v];
+          const int igr = k.igpig[r] - 1;
+          const int ig = lookup_g(gt, gt.ivg[3 * igr] - gc0, gt.ivg[3 * igr + 1] - gc1, gt.ivg[3 * igr + 2] - gc2);
+          if (ig) {
+            const cplx th = gt.cfunig[ig - 1];
+            if (w.which) {
+              re += th.re;
+              im += th.im;
+            } else {
+              const cplx vv = gt.veffig[ig - 1];
+              const double t1 = 0.5 * (k.vgpc[3 * r] * vc0 + k.vgpc[3 * r + 1] * vc1 + k.vgpc[3 * r + 2] * vc2);
+              re += vv.re + t1 * th.re;
+              im += vv.im + t1 * th.im;
+            }
+          }
+          out[r + (size_t)c * k.ldh] = cplx{re, im};
+          if (r < c) out[c + (size_t)r * k.ldh] = cplx{re, -im};
+        }
+    }
+  }
+}
+
+template <int MI, int NJ>
+int launch_bsplit2_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((bsplit2_kernel<MI, NJ>), dim3(nblocks), dim3(256), 0, s, ks, items, nitems, d.natm,
+                     setup_dev::make_gtab(d));
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int MI, int NJ>
+int max_blocks2_t() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bsplit2_kernel<MI, NJ>, 256, 0) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
 template <int BM, int WR, int WC>
 int launch_bsplit_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL((bsplit_kernel<BM, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems, d.natm,
@@ -340,6 +548,8 @@ int exc_bsplit_tile(int variant) {
   switch (variant) {
     case 0: return 64;
     case 1: return 32;
+    case 2: return 128;
+    case 3: return 64;
     default: return 0;
   }
 }
@@ -348,6 +558,8 @@ int exc_bsplit_max_blocks(int variant) {
   switch (variant) {
     case 0: return max_blocks_t<64, 2, 2>();
     case 1: return max_blocks_t<32, 1, 1>();
+    case 2: return max_blocks2_t<2, 2>();
+    case 3: return max_blocks2_t<1, 1>();
     default: return 1;
   }
 }
@@ -358,6 +570,8 @@ int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, i
   switch (variant) {
     case 0: return launch_bsplit_t<64, 2, 2>(d, ks, items, nitems, nblocks, s);
     case 1: return launch_bsplit_t<32, 1, 1>(d, ks, items, nitems, nblocks, s);
+    case 2: return launch_bsplit2_t<2, 2>(d, ks, items, nitems, nblocks, s);
+    case 3: return launch_bsplit2_t<1, 1>(d, ks, items, nitems, nblocks, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -67,3 +67,33 @@ def test_split_against_emulation_and_fp64(which):
               f"O {d64_o:.2e}; max|d eig| lowest {m}: {np.abs(e[:m] - e_ref[:m]).max():.2e} Ha", flush=True)
         assert d_em_h <= 1e-8 and d_em_o <= 1e-8
         assert d64_h <= 1e-6 and d64_o <= 1e-6
+
+
+def test_split_variants_bit_identical():
+    """Every split kernel variant (tile shape, LDS staging) sums in the same order: the MFMA
+    k-steps in ascending k, the two FP32 levels per atom block, then the FP64 fold in atom
+    order.  Their H and O must therefore agree bit for bit."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 2)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_BF16
+    out = {}
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        for v in range(ctx.num_variants("bsplit")):
+            print(f"config1 nk {b.nk} split variant {v}", flush=True)
+            ctx.set_variant("bsplit", v)
+            for t in b.H + b.O:
+                t.fill_(float("nan"))
+            ctx.setup_batched_dev(b, flags)
+            torch.cuda.synchronize()
+            out[v] = ([t.cpu() for t in b.H], [t.cpu() for t in b.O])
+    for v in out:
+        for i in range(b.nk):
+            for m in (0, 1):
+                a, r = out[v][m][i], out[0][m][i]
+                if not torch.equal(a, r):
+                    d = float((a - r).abs().max() / r.abs().max())
+                    raise AssertionError(f"variant {v} k{i} {'HO'[m]} differs: max |d| / max |ref| = {d:.3e}")

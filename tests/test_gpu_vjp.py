@@ -9,7 +9,11 @@ W = -C (f e) C^H, as the contract states.
   atomic positions: backward against central finite differences of the forward with step
   1e-4 bohr, Richardson-extrapolated with step 5e-5 (the seeded spectra reach -4000 Ha, so
   |dE/dtau| ~ 1e4 Ha/bohr and the plain central difference's h^2 term alone exceeds the
-  bar), contracting the differences of H and O with the cotangents; |error| <= 1e-6 Ha/bohr;
+  bar), contracting the differences of H and O with the cotangents; |error| <= 1e-6 Ha/bohr,
+  or 1e-8 of max |dE/dtau| where that is larger: with |dE/dtau| = 1.5e4 Ha/bohr (workload 4)
+  the extrapolated difference still carries ~2e-5 Ha/bohr of h^4 truncation and round-off
+  (1e-9 relative), so the absolute 1e-6 Ha/bohr bar is held by the forcek comparison above,
+  which has no step;
 * dE/dV(G) against the exact linear change of L under Hermitian perturbations of V(G).
 """
 import numpy as np
@@ -115,7 +119,8 @@ def _check(cr, kp, nst, natoms_fd):
     assert np.abs(dt - g_tau).max() <= 1e-9 * max(1.0, np.abs(g_tau).max())
     assert np.abs(dv - g_v).max() <= 1e-9 * max(1.0, np.abs(g_v).max())
     F = R.forcek_ibs(cr, kp, C, e, f, apwalm=A)
-    assert np.abs(F + dt).max() <= 1e-9 * max(1.0, np.abs(F).max())
+    print(f"  forcek: max |F_ref + dE/dtau| {np.abs(F + dt).max():.2e} Ha/bohr", flush=True)
+    assert np.abs(F + dt).max() <= 1e-6                  # Ha/bohr, absolute
     # autograd wrapper vs central finite differences of the GPU forward
     fn = _make_fn(run)
     tau = torch.from_numpy(cr.atposc.copy()).requires_grad_(True)
@@ -142,7 +147,7 @@ def _check(cr, kp, nst, natoms_fd):
             worst_plain = max(worst_plain, abs(d1 - grad[l, a]))
     print(f"  autograd vs finite differences ({natoms_fd} atoms): Richardson max |d| {worst:.2e} Ha/bohr, "
           f"plain central (h={H_STEP}) max |d| {worst_plain:.2e} (max |dE/dtau| {np.abs(grad).max():.2e})", flush=True)
-    assert worst <= 1e-6
+    assert worst <= max(1e-6, 1e-8 * np.abs(grad).max())
     # dE/dV(G): exact linear response to Hermitian perturbations
     key = {tuple(cr.ivg[:, g]): g for g in range(cr.ngvec)}
     for g in [1, 7, 30]:
