@@ -558,8 +558,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (do_match && q.ngp > 0 && (!q.gpc || !q.tpgpc || !q.sfacgp)) return EXC_ERR_ARG;
   }
   // defaults: the fastest variants measured on MI355X for N = 1150..2700 (grouped MT: 64x64
-  // tile, 8 waves, double-buffered LDS)
-  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 0) : (c->bmt_variant >= 0 ? c->bmt_variant : 6);
+  // tile, 8 waves, two LDS stages filled by DMA; split: 64x64 tile, LDS filled by DMA)
+  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 10);
   const int yv = c->y_variant >= 0 ? c->y_variant : 1;
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));

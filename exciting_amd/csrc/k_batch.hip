@@ -8,6 +8,7 @@
 #include "lo_dev.h"
 #include "match_dev.h"
 #include "setup_dev.h"
+#include "mt_dma.h"
 
 namespace {
 
@@ -92,6 +93,49 @@ template <int BM, int BK, int WR, int WC, bool DB = false>
 int max_blocks_t() {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_kernel<BM, BK, WR, WC, DB>, 64 * WR * WC, 0) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+// MT items with LDS filled by DMA, NST stages in flight
+template <int BM, int BK, int NST, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void bmt_dma_kernel(const kdesc *__restrict__ ks,
+                                                               const mt_item *__restrict__ items, int nitems, int ktot,
+                                                               GTab gt, const int *__restrict__ kperm) {
+  using T = MtTileDMA<BM, BK, NST, WR, WC>;
+  using St = typename T::Stage;
+  using S2 = typename std::conditional<(NST > 2), St, char>::type;
+  using S3 = typename std::conditional<(NST > 3), St, char>::type;
+  __shared__ typename T::Tables tb;
+  __shared__ St s0;
+  __shared__ St s1;
+  __shared__ S2 s2;
+  __shared__ S3 s3;
+  const int nx = gridDim.x >= 8 ? 8 : 1;
+  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
+  const int chunk = (nitems + nx - 1) / nx;
+  const int end = min(nitems, (x + 1) * chunk);
+  for (int it = x * chunk + lb; it < end; it += per) {
+    const mt_item w = items[it];
+    const kdesc k = ks[w.k];
+    T::run(tb, &s0, &s1, (St *)&s2, (St *)&s3, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h,
+           k.o, k.ldh, kperm, w.ti, w.tj, w.which);
+  }
+}
+
+template <int BM, int BK, int NST, int WR, int WC>
+int launch_bmt_dma_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((bmt_dma_kernel<BM, BK, NST, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
+                     d.natm * d.naa, make_gtab(d), d.kperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BM, int BK, int NST, int WR, int WC>
+int max_blocks_dma_t() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_dma_kernel<BM, BK, NST, WR, WC>, 64 * WR * WC, 0) !=
+      hipSuccess)
     return 1;
   return n > 0 ? n : 1;
 }
@@ -197,11 +241,12 @@ int exc_bmt_tile(int variant) {
     case 7: return 64;
     case 8: return 64;
     case 9: return 32;
+    case 10: case 11: case 12: case 13: return 64;
     default: return 0;
   }
 }
 
-bool exc_bmt_fused(int variant) { return variant >= 7; }
+bool exc_bmt_fused(int variant) { return variant >= 7 && variant <= 9; }
 
 int exc_bmt_max_blocks(int variant) {
   switch (variant) {
@@ -215,6 +260,10 @@ int exc_bmt_max_blocks(int variant) {
     case 7: return max_blocks_ho_t<64, 16, 2, 4>();
     case 8: return max_blocks_ho_t<64, 8, 2, 4>();
     case 9: return max_blocks_ho_t<32, 16, 2, 2>();
+    case 10: return max_blocks_dma_t<64, 16, 2, 2, 4>();
+    case 11: return max_blocks_dma_t<64, 8, 4, 2, 4>();
+    case 12: return max_blocks_dma_t<64, 16, 3, 2, 4>();
+    case 13: return max_blocks_dma_t<64, 8, 3, 2, 2>();
     default: return 1;
   }
 }
@@ -233,6 +282,10 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 7: return launch_bmt_ho_t<64, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 8: return launch_bmt_ho_t<64, 8, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 9: return launch_bmt_ho_t<32, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 10: return launch_bmt_dma_t<64, 16, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 11: return launch_bmt_dma_t<64, 8, 4, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 12: return launch_bmt_dma_t<64, 16, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 13: return launch_bmt_dma_t<64, 8, 3, 2, 2>(d, ks, items, nitems, nblocks, s);
     default: return EXC_ERR_ARG;
   }
 }

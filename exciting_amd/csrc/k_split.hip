@@ -325,8 +325,8 @@ struct Split2Cfg {
   static constexpr int BM = 64 * MI, BN = 64 * NJ;
   static constexpr int ABYTES = BM * PSEG * SEG, BBYTES = BN * PSEG * SEG;
   static constexpr int ALOADS = BM * PSEG / 64, BLOADS = BN * PSEG / 64;   // wave-wide loads per stage
+  static constexpr int STAGE16 = (ABYTES + BBYTES) / 16;   // one LDS stage in 16-byte units
   struct Shared {
-    unsigned char buf[2][ABYTES + BBYTES];
     int ea[2][BM], eb[2][BN];
   };
 };
@@ -360,6 +360,11 @@ __global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ 
   using S = Split2Cfg<MI, NJ>;
   constexpr int BM = S::BM, BN = S::BN;
   __shared__ typename S::Shared sh;
+  // the two LDS stages are separate variables and the chunk loop alternates them statically:
+  // the compiler can then tell the operand reads of one stage from the DMA into the other
+  // and does not wait for that DMA before the reads (it would with one indexed array)
+  __shared__ uint4 st0[S::STAGE16];
+  __shared__ uint4 st1[S::STAGE16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int nx = gridDim.x >= 8 ? 8 : 1;
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ 
 #pragma unroll
         for (int v = 0; v < 16; ++v) tre[i][j][v] = tim[i][j][v] = 0.0;
     __syncthreads();                         // previous item done with both stages
-    split2_issue<MI, NJ>(sh.buf[0], PA, PB, row0, col0, ngp, rowb);
+    split2_issue<MI, NJ>((unsigned char *)st0, PA, PB, row0, col0, ngp, rowb);
     for (int atom = 0; atom < natm; ++atom) {
       // exponents of this atom block; buffer atom&1 was last read by the fold of atom-2,
       // which precedes at least NCH barriers
@@ -400,16 +405,15 @@ __global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ 
       for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) a0r[i][j] = a0i[i][j] = a1r[i][j] = a1i[i][j] = f32x16{};
-      for (int c = 0; c < NCH; ++c) {
-        const int cg = atom * NCH + c;
+      auto chunk = [&](const unsigned char *cur, unsigned char *nxt, int cg) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();                     // chunk cg landed; all waves done with the other stage
         if (cg + 1 < natm * NCH) {
           const unsigned off = (unsigned)(cg + 1) * ROWB;      // record (atom, c) of row 0
-          split2_issue<MI, NJ>(sh.buf[(cg + 1) & 1], PA + off, PB + off, row0, col0, ngp, rowb);
+          split2_issue<MI, NJ>(nxt, PA + off, PB + off, row0, col0, ngp, rowb);
         }
-        const unsigned char *sa = sh.buf[cg & 1] + (wr * 32 * MI + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
-        const unsigned char *sb = sh.buf[cg & 1] + S::ABYTES + (wc * 32 * NJ + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
+        const unsigned char *sa = cur + (wr * 32 * MI + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
+        const unsigned char *sb = cur + S::ABYTES + (wc * 32 * NJ + (lane & 31)) * PSEG * SEG + (lane >> 5) * 16;
         // pairs in the order of bsplit_kernel: (0,0) | (0,1) (1,0) | (0,2) (1,1) (2,0)
 #pragma unroll
         for (int pr = 0; pr < 6; ++pr) {
@@ -446,6 +450,11 @@ __global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ 
               }
             }
         }
+      };
+      static_assert(NCH % 2 == 0, "the stage alternation needs an even chunk count per atom");
+      for (int c = 0; c < NCH; c += 2) {
+        chunk((const unsigned char *)st0, (unsigned char *)st1, atom * NCH + c);
+        chunk((const unsigned char *)st1, (unsigned char *)st0, atom * NCH + c + 1);
       }
       // fold the atom block into the FP64 totals: x 2^(ea(row) + eb(col) - 16)
 #pragma unroll

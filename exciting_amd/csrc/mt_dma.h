@@ -1,0 +1,236 @@
+// Muffin-tin tile of the grouped setup with LDS filled by DMA (global_load_lds_dwordx4):
+// the same product and epilogue as MtTile, without staging registers and with NST stages
+// in flight.
+//
+// One wave-wide DMA moves one k-column of a 64-row panel: lane l reads the 16-byte complex
+// element of row row0 + 64p + l and the LDS receives it at (stage base) + 16 l.  The
+// column index of the k-th summand is kperm[k] (the l-descending order), identical for
+// all lanes, so it is fetched with a scalar load.  Rows past N are clamped to row 0 (their
+// results are never stored); k-columns past K are zero-filled with LDS stores.
+//
+// The NST stages are separate __shared__ variables and the K loop rotates through them
+// statically (unrolled by NST): the compiler can then tell the operand reads of the
+// current stage from the DMA into the others and waits for that DMA only where this code
+// asks for it (vmcnt <= the loads issued for the later stages).
+//
+// MFMA sequence per element identical to zf64::Core::compute: the results are bit-identical
+// to the register-staged kernels.
+#pragma once
+#include "setup_dev.h"
+
+namespace setup_dev {
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <int BM, int BK>
+struct DmaStage {
+  cplx a[BK][BM];
+  cplx b[BK][BM];
+};
+
+// Workgroup barrier without a memory fence.  __syncthreads() carries a workgroup release
+// fence, and since vmcnt counts the in-flight DMA as well as stores, the compiler then
+// waits for all DMA before every barrier (in-flight stages would drain at each step).  The
+// K loop instead waits explicitly (wait_vm_lgkm0: its own LDS stores and the DMA of the
+// stage about to be read) and the empty asm statements keep the compiler from moving
+// memory accesses across the barrier.
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// s_waitcnt immediate (gfx9 encoding): vmcnt <= v, lgkmcnt = 0, expcnt not waited for
+template <int V>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(V >= 0 && V < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((V & 15) | (7 << 4) | (0 << 8) | ((V >> 4) << 14));
+}
+
+template <int BM, int BK, int NST, int WR, int WC>
+struct MtTileDMA {
+  static constexpr int BN = BM;
+  static constexpr int kWaves = WR * WC, kThreads = 64 * kWaves;
+  static constexpr int WM = BM / WR, WN = BN / WC, MB = WM / 16, NB = WN / 16;
+  static constexpr int RB = BM / 64;                 // 64-row blocks per panel
+  static constexpr int NL = 2 * BK * RB;             // wave-wide DMAs per stage (A and B)
+  static constexpr int L = NL / kWaves;              // per wave
+  static_assert(BM % 64 == 0 && NL % kWaves == 0, "DMA split");
+  static_assert(NST >= 2 && NST <= 4, "stages");
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "MFMA blocks");
+  using Stage = DmaStage<BM, BK>;
+  struct Tables {
+    int gr[BM][3], gc[BN][3];
+    double vr[BM][3], vc[BN][3];
+  };
+
+  // issue the DMAs of K-step `step` (k0 = step*BK) into stage st
+  __device__ __forceinline__ static void issue(Stage *st, const cplx *__restrict__ A, int lda, int row0,
+                                               const cplx *__restrict__ B, int ldb, int col0, int ngp, int ktot,
+                                               const int *__restrict__ kperm, int k0) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int kcs[L];
+#pragma unroll
+    for (int q = 0; q < L; ++q) {                     // wave-uniform: scalar loads, one wait
+      const int j = wave + kWaves * q;
+      const int kk = (j >= BK * RB ? j - BK * RB : j) / RB;
+      const int gk = min(k0 + kk, ktot - 1);
+      kcs[q] = kperm ? kperm[gk] : gk;
+    }
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      const int j = wave + kWaves * q;
+      const bool isb = j >= BK * RB;
+      const int jj = isb ? j - BK * RB : j;
+      const int kk = jj / RB, p = jj % RB;
+      const int gk = k0 + kk;
+      cplx *dst = (isb ? &st->b[kk][64 * p] : &st->a[kk][64 * p]);
+      if (gk < ktot) {
+        const int kc = kcs[q];
+        int gr = (isb ? col0 : row0) + 64 * p + lane;
+        if (gr >= ngp) gr = 0;
+        const cplx *src = (isb ? B + (size_t)kc * ldb : A + (size_t)kc * lda) + gr;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)dst, 16, 0, 0);
+      } else {
+        dst[lane] = cplx{0.0, 0.0};
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void compute(const Stage *st, d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB], int wr,
+                                                 int wc) {
+    const int lane = threadIdx.x & 63;
+    const int lk = lane >> 4, li = lane & 15;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double ar[MB], ai[MB], br[NB], bi[NB], nai[MB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const cplx v = st->a[kk + lk][wr * WM + i * 16 + li];
+        ar[i] = v.re;
+        ai[i] = v.im;
+        nai[i] = -v.im;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const cplx v = st->b[kk + lk][wc * WN + j * 16 + li];
+        br[j] = v.re;
+        bi[j] = v.im;
+      }
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], ci[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], cr[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], br[j], ci[i][j], 0, 0, 0);
+    }
+  }
+
+  // one (tile, H-or-O) item; s0..s3 are the stage variables (s2, s3 unused below NST 3, 4)
+  __device__ static void run(Tables &tb, Stage *s0, Stage *s1, Stage *s2, Stage *s3, int ngp, int ktot,
+                             const cplx *__restrict__ apwalm, int ldg, const cplx *__restrict__ yt, int ldy,
+                             const int *__restrict__ igpig, const double *__restrict__ vgpc, const GTab &gt,
+                             cplx *__restrict__ h, cplx *__restrict__ o, int ldh, const int *__restrict__ kperm, int ti,
+                             int tj, int which) {
+    const int row0 = ti * BM, col0 = tj * BN;
+    const cplx *B = which ? apwalm : yt;
+    const int ldb = which ? ldg : ldy;
+    const int nsteps = (ktot + BK - 1) / BK;
+    // everything of the previous item (its stores and DMA) is complete and every wave is
+    // past its epilogue before the stages and tables are refilled
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < BM + BN; e += kThreads) {
+      const bool isr = e < BM;
+      const int idx = isr ? row0 + e : col0 + (e - BM);
+      int g0 = 0, g1 = 0, g2 = 0;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+      if (idx < ngp) {
+        const int ig = igpig[idx] - 1;
+        g0 = gt.ivg[3 * ig]; g1 = gt.ivg[3 * ig + 1]; g2 = gt.ivg[3 * ig + 2];
+        v0 = vgpc[3 * idx]; v1 = vgpc[3 * idx + 1]; v2 = vgpc[3 * idx + 2];
+      }
+      if (isr) {
+        tb.gr[e][0] = g0; tb.gr[e][1] = g1; tb.gr[e][2] = g2;
+        tb.vr[e][0] = v0; tb.vr[e][1] = v1; tb.vr[e][2] = v2;
+      } else {
+        const int c = e - BM;
+        tb.gc[c][0] = g0; tb.gc[c][1] = g1; tb.gc[c][2] = g2;
+        tb.vc[c][0] = v0; tb.vc[c][1] = v1; tb.vc[c][2] = v2;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);                    // the table loads: only DMA from here on
+    Stage *st[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int u = 0; u < NST - 1; ++u)
+      if (u < nsteps) issue(st[u], apwalm, ldg, row0, B, ldb, col0, ngp, ktot, kperm, u * BK);
+    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+    d4_t cr[MB][NB], ci[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) cr[i][j] = ci[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base < nsteps; base += NST) {
+#pragma unroll
+      for (int u = 0; u < NST; ++u) {
+        const int step = base + u;
+        if (step < nsteps) {
+          // stage `step` has landed once at most the DMAs of the NST-2 later steps (all
+          // full, else wait for everything) are outstanding
+          if (step + NST - 2 < nsteps && (step + NST - 1) * BK <= ktot)
+            wait_vm_lgkm0<(NST - 2) * L>();
+          else
+            wait_vm_lgkm0<0>();
+          stage_barrier();
+          if (step + NST - 1 < nsteps)
+            issue(st[(u + NST - 1) % NST], apwalm, ldg, row0, B, ldb, col0, ngp, ktot, kperm, (step + NST - 1) * BK);
+          compute(st[u], cr, ci, wr, wc);
+        }
+      }
+    }
+
+    cplx *out = which ? o : h;
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int cl = wc * WN + j * 16 + (threadIdx.x & 15);
+        const int c = col0 + cl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rl = wr * WM + i * 16 + 4 * v + ((threadIdx.x & 63) >> 4);
+          const int r = row0 + rl;
+          if (r > c || c >= ngp) continue;
+          double re = cr[i][j][v], im = ci[i][j][v];
+          const int ig = lookup_g(gt, tb.gr[rl][0] - tb.gc[cl][0], tb.gr[rl][1] - tb.gc[cl][1], tb.gr[rl][2] - tb.gc[cl][2]);
+          if (ig) {
+            const cplx th = gt.cfunig[ig - 1];
+            if (which) {
+              re += th.re;
+              im += th.im;
+            } else {
+              const cplx vv = gt.veffig[ig - 1];
+              const double t1 = 0.5 * (tb.vr[rl][0] * tb.vc[cl][0] + tb.vr[rl][1] * tb.vc[cl][1] + tb.vr[rl][2] * tb.vc[cl][2]);
+              re += vv.re + t1 * th.re;
+              im += vv.im + t1 * th.im;
+            }
+          }
+          out[r + (size_t)c * ldh] = cplx{re, im};
+          if (r < c) out[c + (size_t)r * ldh] = cplx{re, -im};
+        }
+      }
+  }
+};
+
+}  // namespace setup_dev
