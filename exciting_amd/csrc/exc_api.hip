@@ -488,9 +488,9 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
   if (ngp > 0) {
     const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
     if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
-    // defaults from the autotuner's sweep on MI355X (32x32 MT tiles, 64x64 Y tiles)
-    const int yv = c->y_variant >= 0 ? c->y_variant : 1;
-    const int mv = c->mt_variant >= 0 ? c->mt_variant : 5;
+    // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA)
+    const int yv = c->y_variant >= 0 ? c->y_variant : 5;
+    const int mv = c->mt_variant >= 0 ? c->mt_variant : 6;
     {
       StageTimer t(c, EXC_STAGE_YSTAGE);
       rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);
@@ -560,7 +560,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // defaults: the fastest variants measured on MI355X for N = 1150..2700 (grouped MT: 64x64
   // tile, 8 waves, two LDS stages filled by DMA; split: 64x64 tile, LDS filled by DMA)
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 10);
-  const int yv = c->y_variant >= 0 ? c->y_variant : 1;
+  const int yv = c->y_variant >= 0 ? c->y_variant : 5;
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);

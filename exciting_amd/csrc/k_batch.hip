@@ -60,6 +60,30 @@ int launch_bystage_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
   return EXC_OK;
 }
 
+template <int BM, int BN, int BK, int NST, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *__restrict__ ks, int natm, int naa,
+                                                                   const cplx *__restrict__ haa,
+                                                                   const int *__restrict__ fperm) {
+  using T = YTileDMA<BM, BN, BK, NST, WR, WC>;
+  using St = typename T::Stage;
+  EXC_DMA_STAGES(St, NST)
+  const int kk = blockIdx.z / natm, ias = blockIdx.z % natm;
+  const kdesc k = ks[kk];
+  const int row0 = blockIdx.x * BM;
+  if (row0 >= k.ngp) return;
+  T::run(EXC_DMA_STAGE_ARGS(St), k.ngp, naa, k.apwalm + (size_t)k.ldg * naa * ias, k.ldg,
+         haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0, blockIdx.y * BN, fperm);
+}
+
+template <int BM, int BN, int BK, int NST, int WR, int WC>
+int launch_bystage_dma_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
+  dim3 grid((maxngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm * nk);
+  hipLaunchKernelGGL((bystage_dma_kernel<BM, BN, BK, NST, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa,
+                     d.haa, d.fperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
 template <int BM, int BK, int WR, int WC, bool DB>
 __global__ __launch_bounds__(64 * WR * WC) void bmt_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
                                                            int nitems, int ktot, GTab gt,
@@ -104,13 +128,8 @@ __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_kernel(const kdesc *__re
                                                                GTab gt, const int *__restrict__ kperm) {
   using T = MtTileDMA<BM, BK, NST, WR, WC>;
   using St = typename T::Stage;
-  using S2 = typename std::conditional<(NST > 2), St, char>::type;
-  using S3 = typename std::conditional<(NST > 3), St, char>::type;
   __shared__ typename T::Tables tb;
-  __shared__ St s0;
-  __shared__ St s1;
-  __shared__ S2 s2;
-  __shared__ S3 s3;
+  EXC_DMA_STAGES(St, NST)
   const int nx = gridDim.x >= 8 ? 8 : 1;
   const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
   const int chunk = (nitems + nx - 1) / nx;
@@ -118,7 +137,7 @@ __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_kernel(const kdesc *__re
   for (int it = x * chunk + lb; it < end; it += per) {
     const mt_item w = items[it];
     const kdesc k = ks[w.k];
-    T::run(tb, &s0, &s1, (St *)&s2, (St *)&s3, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h,
+    T::run(tb, EXC_DMA_STAGE_ARGS(St), k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h,
            k.o, k.ldh, kperm, w.ti, w.tj, w.which);
   }
 }
@@ -225,6 +244,10 @@ int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, in
     case 1: return launch_bystage_t<64, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
     case 2: return launch_bystage_t<128, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
     case 3: return launch_bystage_t<64, 64, 32, 2, 2>(d, nk, maxngp, ks, s);
+    case 4: return launch_bystage_dma_t<64, 64, 16, 2, 2, 2>(d, nk, maxngp, ks, s);
+    case 5: return launch_bystage_dma_t<64, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 6: return launch_bystage_dma_t<128, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 7: return launch_bystage_dma_t<64, 64, 8, 4, 2, 2>(d, nk, maxngp, ks, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -1,6 +1,7 @@
 // Plain FP64 per-k H+O setup on gfx950: the per-k launches of the Y stage and the fused
 // muffin-tin + interstitial + mirror stage (device code in setup_dev.h).
 #include "setup_dev.h"
+#include "mt_dma.h"
 
 namespace {
 
@@ -54,6 +55,61 @@ int launch_mt_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cp
   return EXC_OK;
 }
 
+template <int BM, int BN, int BK, int NST, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void ystage_dma_kernel(int ngp, int naa, const cplx *__restrict__ apwalm,
+                                                                  int ldg, const cplx *__restrict__ haa,
+                                                                  cplx *__restrict__ yt, int ldy,
+                                                                  const int *__restrict__ fperm) {
+  using T = YTileDMA<BM, BN, BK, NST, WR, WC>;
+  using St = typename T::Stage;
+  EXC_DMA_STAGES(St, NST)
+  const int ias = blockIdx.z;
+  T::run(EXC_DMA_STAGE_ARGS(St), ngp, naa, apwalm + (size_t)ldg * naa * ias, ldg, haa + (size_t)naa * naa * ias,
+         yt + (size_t)ldy * naa * ias, ldy, blockIdx.x * BM, blockIdx.y * BN, fperm);
+}
+
+template <int BM, int BN, int BK, int NST, int WR, int WC>
+int launch_ystage_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy, hipStream_t s) {
+  dim3 grid((ngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm);
+  hipLaunchKernelGGL((ystage_dma_kernel<BM, BN, BK, NST, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ngp, d.naa, apwalm,
+                     ldg, d.haa, yt, ldy, d.fperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+// one (tile, H-or-O) item per workgroup, LDS filled by DMA
+template <int BM, int BK, int NST, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void mt_dma_kernel(int ngp, int ktot, int ntile,
+                                                              const cplx *__restrict__ apwalm, int ldg,
+                                                              const cplx *__restrict__ yt, int ldy,
+                                                              const int *__restrict__ igpig,
+                                                              const double *__restrict__ vgpc, GTab gt,
+                                                              cplx *__restrict__ h, cplx *__restrict__ o, int ldh,
+                                                              const int *__restrict__ kperm) {
+  using T = MtTileDMA<BM, BK, NST, WR, WC>;
+  using St = typename T::Stage;
+  __shared__ typename T::Tables tb;
+  EXC_DMA_STAGES(St, NST)
+  const int ntri = ntile * (ntile + 1) / 2;
+  const int item = blockIdx.x;
+  if (item >= 2 * ntri) return;
+  int ti, tj;
+  tri_decode(item >> 1, ti, tj);
+  T::run(tb, EXC_DMA_STAGE_ARGS(St), ngp, ktot, apwalm, ldg, yt, ldy, igpig, vgpc, gt, h, o, ldh, kperm, ti, tj,
+         item & 1);
+}
+
+template <int BM, int BK, int NST, int WR, int WC>
+int launch_mt_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy, const int *igpig,
+                    const double *vgpc, cplx *h, cplx *o, int ldh, hipStream_t s) {
+  const int ntile = (ngp + BM - 1) / BM;
+  const int nitem = ntile * (ntile + 1);
+  hipLaunchKernelGGL((mt_dma_kernel<BM, BK, NST, WR, WC>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp,
+                     d.natm * d.naa, ntile, apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
 }  // namespace
 
 int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy, int variant,
@@ -64,6 +120,10 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
     case 1: return launch_ystage_t<64, 64, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 2: return launch_ystage_t<128, 64, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 3: return launch_ystage_t<64, 64, 32, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 4: return launch_ystage_dma_t<64, 64, 16, 2, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 5: return launch_ystage_dma_t<64, 64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 6: return launch_ystage_dma_t<128, 64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 7: return launch_ystage_dma_t<64, 64, 8, 4, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
     default: return EXC_ERR_ARG;
   }
 }
@@ -78,6 +138,8 @@ int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const 
     case 3: return launch_mt_t<128, 8, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     case 4: return launch_mt_t<96, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     case 5: return launch_mt_t<32, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
+    case 6: return launch_mt_dma_t<64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
+    case 7: return launch_mt_dma_t<64, 8, 4, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     default: return EXC_ERR_ARG;
   }
 }

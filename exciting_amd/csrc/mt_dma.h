@@ -1,32 +1,35 @@
-// Muffin-tin tile of the grouped setup with LDS filled by DMA (global_load_lds_dwordx4):
-// the same product and epilogue as MtTile, without staging registers and with NST stages
-// in flight.
+// Complex FP64 MFMA tiles of the setup with their LDS filled by DMA (global_load_lds_dwordx4):
+// the muffin-tin tile (MtTileDMA, same product and epilogue as MtTile) and the Y-stage tile
+// (YTileDMA, as YTile), without staging registers and with NST stages in flight.
 //
 // One wave-wide DMA moves one k-column of a 64-row panel: lane l reads the 16-byte complex
 // element of row row0 + 64p + l and the LDS receives it at (stage base) + 16 l.  The
 // column index of the k-th summand is kperm[k] (the l-descending order), identical for
-// all lanes, so it is fetched with a scalar load.  Rows past N are clamped to row 0 (their
-// results are never stored); k-columns past K are zero-filled with LDS stores.
+// all lanes, so it is fetched with a scalar load.  Rows past the matrix are clamped to row
+// 0 (their results are never stored); k-columns past K are zero-filled with LDS stores.
 //
-// The NST stages are separate __shared__ variables and the K loop rotates through them
-// statically (unrolled by NST): the compiler can then tell the operand reads of the
-// current stage from the DMA into the others and waits for that DMA only where this code
-// asks for it (vmcnt <= the loads issued for the later stages).
+// The NST stages are separate __shared__ variables (EXC_DMA_STAGES) and the K loop rotates
+// through them statically (unrolled by NST): the compiler can then tell the operand reads
+// of the current stage from the DMA into the others and waits for that DMA only where this
+// code asks for it (vmcnt <= the loads issued for the later stages).
 //
 // MFMA sequence per element identical to zf64::Core::compute: the results are bit-identical
 // to the register-staged kernels.
 #pragma once
+#include <type_traits>
 #include "setup_dev.h"
 
 namespace setup_dev {
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <int BM, int BK>
-struct DmaStage {
-  cplx a[BK][BM];
-  cplx b[BK][BM];
-};
+// the NST stage variables of a kernel: s0_, s1_ (and s2_, s3_ for NST 3, 4; a byte otherwise)
+#define EXC_DMA_STAGES(St, NST)                                              \
+  __shared__ St s0_;                                                         \
+  __shared__ St s1_;                                                         \
+  __shared__ typename std::conditional<((NST) > 2), St, char>::type s2_;     \
+  __shared__ typename std::conditional<((NST) > 3), St, char>::type s3_;
+#define EXC_DMA_STAGE_ARGS(St) &s0_, &s1_, (St *)&s2_, (St *)&s3_
 
 // Workgroup barrier without a memory fence.  __syncthreads() carries a workgroup release
 // fence, and since vmcnt counts the in-flight DMA as well as stores, the compiler then
@@ -47,26 +50,27 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
   __builtin_amdgcn_s_waitcnt((V & 15) | (7 << 4) | (0 << 8) | ((V >> 4) << 14));
 }
 
-template <int BM, int BK, int NST, int WR, int WC>
-struct MtTileDMA {
-  static constexpr int BN = BM;
+// C(i, j) = sum_k op(A(i, k)) B(j, k) over a BM x BN tile, A(i, k) at A[i + kperm[k] lda],
+// B(j, k) at B[j + kperm[k] ldb], op = conj (CONJA) or identity; NST LDS stages filled by DMA.
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool CONJA>
+struct DmaCore {
   static constexpr int kWaves = WR * WC, kThreads = 64 * kWaves;
   static constexpr int WM = BM / WR, WN = BN / WC, MB = WM / 16, NB = WN / 16;
-  static constexpr int RB = BM / 64;                 // 64-row blocks per panel
-  static constexpr int NL = 2 * BK * RB;             // wave-wide DMAs per stage (A and B)
+  static constexpr int RA = BM / 64, RB = BN / 64;   // 64-row blocks of the A and B panels
+  static constexpr int NL = BK * (RA + RB);          // wave-wide DMAs per stage
   static constexpr int L = NL / kWaves;              // per wave
-  static_assert(BM % 64 == 0 && NL % kWaves == 0, "DMA split");
+  static_assert(BM % 64 == 0 && BN % 64 == 0 && NL % kWaves == 0, "DMA split");
   static_assert(NST >= 2 && NST <= 4, "stages");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "MFMA blocks");
-  using Stage = DmaStage<BM, BK>;
-  struct Tables {
-    int gr[BM][3], gc[BN][3];
-    double vr[BM][3], vc[BN][3];
+  struct Stage {
+    cplx a[BK][BM];
+    cplx b[BK][BN];
   };
 
-  // issue the DMAs of K-step `step` (k0 = step*BK) into stage st
-  __device__ __forceinline__ static void issue(Stage *st, const cplx *__restrict__ A, int lda, int row0,
-                                               const cplx *__restrict__ B, int ldb, int col0, int ngp, int ktot,
+  // issue the DMAs of the K-step starting at k0 into stage st; rows past mrows / nrows are
+  // clamped to row 0, k-columns past ktot are zero-filled
+  __device__ __forceinline__ static void issue(Stage *st, const cplx *__restrict__ A, int lda, int row0, int mrows,
+                                               const cplx *__restrict__ B, int ldb, int col0, int nrows, int ktot,
                                                const int *__restrict__ kperm, int k0) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -74,23 +78,22 @@ struct MtTileDMA {
 #pragma unroll
     for (int q = 0; q < L; ++q) {                     // wave-uniform: scalar loads, one wait
       const int j = wave + kWaves * q;
-      const int kk = (j >= BK * RB ? j - BK * RB : j) / RB;
+      const int kk = j < BK * RA ? j / RA : (j - BK * RA) / RB;
       const int gk = min(k0 + kk, ktot - 1);
       kcs[q] = kperm ? kperm[gk] : gk;
     }
 #pragma unroll
     for (int q = 0; q < L; ++q) {
       const int j = wave + kWaves * q;
-      const bool isb = j >= BK * RB;
-      const int jj = isb ? j - BK * RB : j;
-      const int kk = jj / RB, p = jj % RB;
+      const bool isb = j >= BK * RA;
+      const int jj = isb ? j - BK * RA : j;
+      const int kk = isb ? jj / RB : jj / RA, p = isb ? jj % RB : jj % RA;
       const int gk = k0 + kk;
       cplx *dst = (isb ? &st->b[kk][64 * p] : &st->a[kk][64 * p]);
       if (gk < ktot) {
-        const int kc = kcs[q];
         int gr = (isb ? col0 : row0) + 64 * p + lane;
-        if (gr >= ngp) gr = 0;
-        const cplx *src = (isb ? B + (size_t)kc * ldb : A + (size_t)kc * lda) + gr;
+        if (gr >= (isb ? nrows : mrows)) gr = 0;
+        const cplx *src = (isb ? B + (size_t)kcs[q] * ldb : A + (size_t)kcs[q] * lda) + gr;
         __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)dst, 16, 0, 0);
       } else {
         dst[lane] = cplx{0.0, 0.0};
@@ -98,6 +101,7 @@ struct MtTileDMA {
     }
   }
 
+  // the MFMA sequence of zf64::Core::compute, operands from the interleaved stage
   __device__ __forceinline__ static void compute(const Stage *st, d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB], int wr,
                                                  int wc) {
     const int lane = threadIdx.x & 63;
@@ -126,18 +130,83 @@ struct MtTileDMA {
       for (int i = 0; i < MB; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], ci[i][j], 0, 0, 0);
+      if (CONJA) {
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], cr[i][j], 0, 0, 0);
+          for (int j = 0; j < NB; ++j) cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], cr[i][j], 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], br[j], ci[i][j], 0, 0, 0);
+          for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], br[j], ci[i][j], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], cr[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], ci[i][j], 0, 0, 0);
+      }
     }
   }
 
-  // one (tile, H-or-O) item; s0..s3 are the stage variables (s2, s3 unused below NST 3, 4)
+  // the whole K loop.  On entry no DMA of this workgroup is in flight and every wave is done
+  // with the stages; s2, s3 are unused below NST 3, 4.
+  __device__ __forceinline__ static void run(Stage *s0, Stage *s1, Stage *s2, Stage *s3, const cplx *__restrict__ A,
+                                             int lda, int row0, int mrows, const cplx *__restrict__ B, int ldb,
+                                             int col0, int nrows, int ktot, const int *__restrict__ kperm,
+                                             d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB]) {
+    const int nsteps = (ktot + BK - 1) / BK;
+    Stage *st[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int u = 0; u < NST - 1; ++u)
+      if (u < nsteps) issue(st[u], A, lda, row0, mrows, B, ldb, col0, nrows, ktot, kperm, u * BK);
+    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) cr[i][j] = ci[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int base = 0; base < nsteps; base += NST) {
+#pragma unroll
+      for (int u = 0; u < NST; ++u) {
+        const int step = base + u;
+        if (step < nsteps) {
+          // stage `step` has landed once at most the DMAs of the NST-2 later steps (all
+          // full, else wait for everything) are outstanding
+          if (step + NST - 2 < nsteps && (step + NST - 1) * BK <= ktot)
+            wait_vm_lgkm0<(NST - 2) * L>();
+          else
+            wait_vm_lgkm0<0>();
+          stage_barrier();
+          if (step + NST - 1 < nsteps)
+            issue(st[(u + NST - 1) % NST], A, lda, row0, mrows, B, ldb, col0, nrows, ktot, kperm,
+                  (step + NST - 1) * BK);
+          compute(st[u], cr, ci, wr, wc);
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ static int row_of(int wr, int i, int v) {
+    return wr * WM + i * 16 + 4 * v + ((threadIdx.x & 63) >> 4);
+  }
+  __device__ __forceinline__ static int col_of(int wc, int j) { return wc * WN + j * 16 + (threadIdx.x & 15); }
+};
+
+template <int BM, int BK, int NST, int WR, int WC>
+struct MtTileDMA {
+  static constexpr int BN = BM;
+  using C = DmaCore<BM, BN, BK, NST, WR, WC, true>;
+  using Stage = typename C::Stage;
+  static constexpr int kThreads = C::kThreads;
+  struct Tables {
+    int gr[BM][3], gc[BN][3];
+    double vr[BM][3], vc[BN][3];
+  };
+
+  // one (tile, H-or-O) item; s0..s3 are the stage variables
   __device__ static void run(Tables &tb, Stage *s0, Stage *s1, Stage *s2, Stage *s3, int ngp, int ktot,
                              const cplx *__restrict__ apwalm, int ldg, const cplx *__restrict__ yt, int ldy,
                              const int *__restrict__ igpig, const double *__restrict__ vgpc, const GTab &gt,
@@ -146,7 +215,6 @@ struct MtTileDMA {
     const int row0 = ti * BM, col0 = tj * BN;
     const cplx *B = which ? apwalm : yt;
     const int ldb = which ? ldg : ldy;
-    const int nsteps = (ktot + BK - 1) / BK;
     // everything of the previous item (its stores and DMA) is complete and every wave is
     // past its epilogue before the stages and tables are refilled
     __builtin_amdgcn_s_waitcnt(0);
@@ -171,45 +239,20 @@ struct MtTileDMA {
       }
     }
     __builtin_amdgcn_s_waitcnt(0);                    // the table loads: only DMA from here on
-    Stage *st[4] = {s0, s1, s2, s3};
-#pragma unroll
-    for (int u = 0; u < NST - 1; ++u)
-      if (u < nsteps) issue(st[u], apwalm, ldg, row0, B, ldb, col0, ngp, ktot, kperm, u * BK);
-    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
-    d4_t cr[MB][NB], ci[MB][NB];
-#pragma unroll
-    for (int i = 0; i < MB; ++i)
-#pragma unroll
-      for (int j = 0; j < NB; ++j) cr[i][j] = ci[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    for (int base = 0; base < nsteps; base += NST) {
-#pragma unroll
-      for (int u = 0; u < NST; ++u) {
-        const int step = base + u;
-        if (step < nsteps) {
-          // stage `step` has landed once at most the DMAs of the NST-2 later steps (all
-          // full, else wait for everything) are outstanding
-          if (step + NST - 2 < nsteps && (step + NST - 1) * BK <= ktot)
-            wait_vm_lgkm0<(NST - 2) * L>();
-          else
-            wait_vm_lgkm0<0>();
-          stage_barrier();
-          if (step + NST - 1 < nsteps)
-            issue(st[(u + NST - 1) % NST], apwalm, ldg, row0, B, ldb, col0, ngp, ktot, kperm, (step + NST - 1) * BK);
-          compute(st[u], cr, ci, wr, wc);
-        }
-      }
-    }
+    d4_t cr[C::MB][C::NB], ci[C::MB][C::NB];
+    C::run(s0, s1, s2, s3, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm, cr, ci);
 
     cplx *out = which ? o : h;
+    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
 #pragma unroll
-    for (int i = 0; i < MB; ++i)
+    for (int i = 0; i < C::MB; ++i)
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const int cl = wc * WN + j * 16 + (threadIdx.x & 15);
+      for (int j = 0; j < C::NB; ++j) {
+        const int cl = C::col_of(wc, j);
         const int c = col0 + cl;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int rl = wr * WM + i * 16 + 4 * v + ((threadIdx.x & 63) >> 4);
+          const int rl = C::row_of(wr, i, v);
           const int r = row0 + rl;
           if (r > c || c >= ngp) continue;
           double re = cr[i][j][v], im = ci[i][j][v];
@@ -228,6 +271,32 @@ struct MtTileDMA {
           }
           out[r + (size_t)c * ldh] = cplx{re, im};
           if (r < c) out[c + (size_t)r * ldh] = cplx{re, -im};
+        }
+      }
+  }
+};
+
+// One (row tile, column tile) of Yt_a = A_a h_a^T (no conjugation), summed over f in the
+// order fperm; one tile per workgroup.
+template <int BM, int BN, int BK, int NST, int WR, int WC>
+struct YTileDMA {
+  using C = DmaCore<BM, BN, BK, NST, WR, WC, false>;
+  using Stage = typename C::Stage;
+  __device__ static void run(Stage *s0, Stage *s1, Stage *s2, Stage *s3, int ngp, int naa,
+                             const cplx *__restrict__ A, int ldg, const cplx *__restrict__ ha, cplx *__restrict__ Y,
+                             int ldy, int row0, int col0, const int *__restrict__ fperm) {
+    d4_t cr[C::MB][C::NB], ci[C::MB][C::NB];
+    C::run(s0, s1, s2, s3, A, ldg, row0, ngp, ha, naa, col0, naa, naa, fperm, cr, ci);
+    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+#pragma unroll
+    for (int i = 0; i < C::MB; ++i)
+#pragma unroll
+      for (int j = 0; j < C::NB; ++j) {
+        const int c = col0 + C::col_of(wc, j);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = row0 + C::row_of(wr, i, v);
+          if (r < ngp && c < naa) Y[r + (size_t)c * ldy] = cplx{cr[i][j][v], ci[i][j][v]};
         }
       }
   }

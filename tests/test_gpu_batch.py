@@ -57,6 +57,15 @@ def test_batched_equals_per_k(cfg, nk):
             ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH)
             torch.cuda.synchronize()
             _same(b, ref, f"variant {v}")
+        ctx.set_variant("bmt", -1)
+        for v in range(ctx.num_variants("ystage")):
+            print(f"cfg {cfg} nk {nk} grouped Y variant {v}", flush=True)
+            ctx.set_variant("ystage", v)
+            _clear(b)
+            ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH)
+            torch.cuda.synchronize()
+            _same(b, ref, f"Y variant {v}")
+        ctx.set_variant("ystage", -1)
 
 
 def test_graph_replays_match_eager():

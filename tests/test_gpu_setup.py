@@ -100,3 +100,25 @@ def test_device_entry_points_match_host():
     assert np.array_equal(A_d[:kp.ngp], A_h[:kp.ngp])
     assert np.array_equal(H.cpu().numpy().T, H_h)
     assert np.array_equal(O.cpu().numpy().T, O_h)
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_per_k_variants_bit_identical(cfg):
+    """Every tile variant of the per-k Y and MT stages (register- or DMA-staged LDS, any tile
+    shape) sums each element in the same order, so H and O agree bit for bit with the
+    default variants, which test_config0 / test_config1_two_kpoints check against the oracle.
+    Workload 0 (N = 97) exercises partial tiles and a K that is not a multiple of the
+    K-step."""
+    w = workload(cfg, 1)
+    cr, kp = w.crystal, w.kpoints[0]
+    ctx = _ctx(cr)
+    A = ctx.match(kp)
+    H0, O0 = ctx.hamiltonandoverlapsetup(kp, A)
+    for stage in ("mt", "ystage"):
+        for v in range(ctx.num_variants(stage)):
+            print(f"config{cfg} N={kp.ngp} per-k {stage} variant {v}", flush=True)
+            ctx.set_variant(stage, v)
+            H, O = ctx.hamiltonandoverlapsetup(kp, A)
+            assert np.array_equal(H, H0), f"{stage} variant {v}: H differs"
+            assert np.array_equal(O, O0), f"{stage} variant {v}: O differs"
+        ctx.set_variant(stage, -1)
