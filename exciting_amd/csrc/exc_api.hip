@@ -47,6 +47,11 @@ struct exc_ctx {
   size_t ks_cap;
   mt_item *d_items;
   size_t items_cap;
+  double *rfbuf;            // radial factors of all k-points (EXC_BATCH_OLP_LEGENDRE)
+  size_t rf_bytes;
+  mt_item *d_oitems;        // overlap tiles of the addition-theorem kernel
+  size_t oitems_cap;
+  int batch_noitems;
   std::vector<char> batch_sig;   // the batch the descriptors / graph were built for
   int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
   hipGraph_t graph;
@@ -217,6 +222,8 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->byt) hipFree(c->byt);
   if (c->d_ks) hipFree(c->d_ks);
   if (c->d_items) hipFree(c->d_items);
+  if (c->rfbuf) hipFree(c->rfbuf);
+  if (c->d_oitems) hipFree(c->d_oitems);
   if (c->planes) hipFree(c->planes);
   if (c->expo) hipFree(c->expo);
   if (c->vjp_work) hipFree(c->vjp_work);
@@ -543,9 +550,13 @@ int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *a
 
 int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flags) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
-  if (nk < 0 || (nk > 0 && !ks) || (flags & ~(EXC_BATCH_MATCH | EXC_BATCH_GRAPH | EXC_BATCH_SPLIT_BF16)))
+  if (nk < 0 || (nk > 0 && !ks) ||
+      (flags & ~(EXC_BATCH_MATCH | EXC_BATCH_GRAPH | EXC_BATCH_SPLIT_BF16 | EXC_BATCH_OLP_LEGENDRE)))
     return EXC_ERR_ARG;
   const bool split = flags & EXC_BATCH_SPLIT_BF16;
+  // the addition-theorem overlap needs the radial factors that only the grouped match writes
+  const bool olp = flags & EXC_BATCH_OLP_LEGENDRE;
+  if (olp && !(flags & EXC_BATCH_MATCH)) return EXC_ERR_ARG;
   if (split && c->d.naa > kSplitKB) return EXC_ERR_UNSUPPORTED;
   if (nk == 0) return EXC_OK;
   int rc = use_device(c);
@@ -583,9 +594,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       hipGraphDestroy(c->graph);
       c->graph = nullptr;
     }
-    size_t ytot = 0, ptot = 0, etot = 0;
+    size_t ytot = 0, ptot = 0, etot = 0, rtot = 0;
+    const int nl2 = 2 * (c->d.lmaxmat + 1);
     for (int k = 0; k < nk; ++k) {
       ytot += (size_t)ks[k].ngp * c->d.naa * c->d.natm;
+      rtot += (size_t)ks[k].ngp * c->d.natm * nl2;
       ptot += 2 * exc_split_plane_elems(ks[k].ngp, c->d.natm);
       etot += 2 * (size_t)ks[k].ngp * c->d.natm;
     }
@@ -594,10 +607,12 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if ((rc = grow((void **)&c->planes, &c->planes_bytes, 2 * (ptot ? ptot : 1)))) return rc;
       if ((rc = grow((void **)&c->expo, &c->expo_bytes, sizeof(int) * (etot ? etot : 1)))) return rc;
     }
+    if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot ? rtot : 1)))) return rc;
     std::vector<kdesc> kd(nk);
-    std::vector<mt_item> items;
+    std::vector<mt_item> items, oitems;
     const int T = split ? exc_bsplit_tile(bv) : exc_bmt_tile(bv);
-    size_t yoff = 0, poff = 0, eoff = 0;
+    const int TO = exc_olp_tile();
+    size_t yoff = 0, poff = 0, eoff = 0, roff = 0;
     int maxngp = 0;
     for (int k = 0; k < nk; ++k) {
       const exc_kpoint_dev &q = ks[k];
@@ -627,15 +642,34 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         e.ey = c->expo + eoff + (size_t)q.ngp * c->d.natm;
         eoff += 2 * (size_t)q.ngp * c->d.natm;
       }
+      e.rf = nullptr;
+      if (olp) {
+        e.rf = c->rfbuf + roff;
+        roff += (size_t)q.ngp * c->d.natm * nl2;
+      }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
       const int nt = (q.ngp + T - 1) / T;
       const bool fused = !split && exc_bmt_fused(bv);
+      // with the addition-theorem overlap the MT work list holds the H tiles only (a fused
+      // variant still computes both; its O is then overwritten)
       for (int tj = 0; tj < nt; ++tj)
         for (int ti = 0; ti <= tj; ++ti) {
           items.push_back({k, ti, tj, 0});
-          if (!fused) items.push_back({k, ti, tj, 1});
+          if (!fused && !olp) items.push_back({k, ti, tj, 1});
         }
+      if (olp) {
+        const int no = (q.ngp + TO - 1) / TO;
+        for (int tj = 0; tj < no; ++tj)
+          for (int ti = 0; ti <= tj; ++ti) oitems.push_back({k, ti, tj, 1});
+      }
     }
+    if (olp) {
+      if ((rc = grow((void **)&c->d_oitems, &c->oitems_cap, sizeof(mt_item) * (oitems.empty() ? 1 : oitems.size()))))
+        return rc;
+      if (!oitems.empty())
+        EXC_CHECK(hipMemcpy(c->d_oitems, oitems.data(), sizeof(mt_item) * oitems.size(), hipMemcpyHostToDevice));
+    }
+    c->batch_noitems = (int)oitems.size();
     if ((rc = grow((void **)&c->d_ks, &c->ks_cap, sizeof(kdesc) * nk))) return rc;
     if ((rc = grow((void **)&c->d_items, &c->items_cap, sizeof(mt_item) * (items.empty() ? 1 : items.size()))))
       return rc;
@@ -675,6 +709,10 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     } else {
       StageTimer t(c, capture ? -1 : EXC_STAGE_BMT);
       if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream))) return r;
+    }
+    if (olp) {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
+      if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, c->stream))) return r;
     }
     {
       StageTimer t(c, capture ? -1 : EXC_STAGE_LO);

@@ -80,6 +80,9 @@ struct kdesc {
   // split-BF16 operands (exc_setup_batched_dev with EXC_BATCH_SPLIT_BF16)
   unsigned short *sa, *sy;   // slice planes of A and of Yt, layout [row][atom][k/16][plane][16]
   int *ea, *ey;              // per-(row, atom) exponents, [row*natm + atom]
+  // radial factors of apwalm for the overlap by the addition theorem (EXC_BATCH_OLP_LEGENDRE),
+  // written by the grouped match: [(atom*ngp + row)*2(lmaxmat+1) + 2l + io]
+  double *rf;
 };
 
 // Work item of the grouped muffin-tin launch: (k, tile row, tile column, H=0 / O=1).
@@ -120,6 +123,8 @@ size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one opera
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);
 int exc_bsplit_max_blocks(int variant);
+int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
+int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,

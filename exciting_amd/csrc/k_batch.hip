@@ -20,19 +20,19 @@ template <int L>
 __global__ __launch_bounds__(kMatchThreads) void bmatch_kernel(const kdesc *__restrict__ ks, int apwordmax,
                                                                 const int *__restrict__ spidx,
                                                                 const double *__restrict__ rmt, double fpso,
-                                                                const double *__restrict__ dinv) {
+                                                                const double *__restrict__ dinv, int nl) {
   const kdesc k = ks[blockIdx.z];
   const int ig = blockIdx.x * kMatchThreads + threadIdx.x;
   if (ig >= k.ngp) return;
   match_dev::match_body<L>(ig, blockIdx.y, apwordmax, spidx, rmt, fpso, dinv, k.gpc, k.tpgpc, k.sfacgp, k.ldg,
-                           k.apwalm, k.ldg);
+                           k.apwalm, k.ldg, k.rf, nl, k.ngp);
 }
 
 template <int L>
 int launch_bmatch_l(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
   dim3 grid((maxngp + kMatchThreads - 1) / kMatchThreads, d.natm, nk);
   hipLaunchKernelGGL(bmatch_kernel<L>, grid, dim3(kMatchThreads), 0, s, ks, d.apwordmax, d.spidx, d.rmt,
-                     4.0 * M_PI / sqrt(d.omega), d.dinv);
+                     4.0 * M_PI / sqrt(d.omega), d.dinv, d.lmaxmat + 1);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

@@ -1,0 +1,154 @@
+// Muffin-tin overlap of the grouped setup by the addition theorem (EXC_BATCH_OLP_LEGENDRE).
+//
+// apwalm(G, io, lm, a) = (4 pi/sqrt(Omega)) i^l r_{a,io,l}(G) S_a(G) conj(Y_lm(G+k))
+// (match.f90:83-187: r = D^-1 [j_l, |G+k| j_l'] is real, S_a = sfacgp), so the m-sum of the
+// overlap's muffin-tin part collapses by sum_m Y_lm(u) conj(Y_lm(u')) = (2l+1)/(4 pi) P_l(u.u'):
+//
+//   O_MT(G, G') = sum_a conj(S_a(G)) S_a(G') sum_{l <= lmaxmat} P_l(u.u') sum_io rf(G) rf(G')
+//
+// with rf = (4 pi/sqrt(Omega)) sqrt((2l+1)/(4 pi)) r (written by the grouped match) and u the
+// unit vector of G+k from tpgpc.  The l-sum is evaluated per element by Clenshaw's
+// recurrence, l descending (P_{l+1} = (2l+1)/(l+1) x P_l - l/(l+1) P_{l-1}).  This is the
+// same matrix as the reference's conj(A) A^T contraction (olpaa, overlapsetup), a different
+// rounding: about 440 FP64 FMAs per element instead of 8 natm naa flops, on the VALU.
+//
+// Workgroup: one 64x64 tile of the upper triangle (256 threads, 4x4 elements each); per
+// atom the rows' and columns' radial factors are staged in LDS.  Epilogue as the MFMA
+// tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
+#include "setup_dev.h"
+
+namespace {
+
+using namespace setup_dev;
+
+constexpr int kOT = 64;        // tile edge
+constexpr int kNLmax = 16;     // lmaxmat + 1 supported
+constexpr int kRS = 2 * kNLmax + 1;
+
+__global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                  int nitems, int natm, int nl, GTab gt) {
+  __shared__ double rr[kOT][kRS], rc[kOT][kRS];
+  __shared__ cplx sr[kOT], sc[kOT];
+  __shared__ double ur[kOT][3], uc[kOT][3];
+  __shared__ int gr[kOT][3], gc[kOT][3];
+  const int item = blockIdx.x;
+  if (item >= nitems) return;
+  const mt_item w = items[item];
+  const kdesc k = ks[w.k];
+  const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
+  const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
+  const int nl2 = 2 * nl;
+  for (int e = tid; e < 2 * kOT; e += 256) {
+    const bool isr = e < kOT;
+    const int q = isr ? e : e - kOT;
+    const int idx = (isr ? row0 : col0) + q;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    int g0 = 0, g1 = 0, g2 = 0;
+    if (idx < ngp) {
+      double st, ct, sp, cp;
+      sincos(k.tpgpc[2 * idx], &st, &ct);
+      sincos(k.tpgpc[2 * idx + 1], &sp, &cp);
+      u0 = st * cp; u1 = st * sp; u2 = ct;
+      const int ig = k.igpig[idx] - 1;
+      g0 = gt.ivg[3 * ig]; g1 = gt.ivg[3 * ig + 1]; g2 = gt.ivg[3 * ig + 2];
+    }
+    double(*u)[3] = isr ? ur : uc;
+    int(*g)[3] = isr ? gr : gc;
+    u[q][0] = u0; u[q][1] = u1; u[q][2] = u2;
+    g[q][0] = g0; g[q][1] = g1; g[q][2] = g2;
+  }
+  __syncthreads();
+  double x[4][4], are[4][4], aim[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int a = tr + 16 * i, b = tc + 16 * j;
+      x[i][j] = ur[a][0] * uc[b][0] + ur[a][1] * uc[b][1] + ur[a][2] * uc[b][2];
+      are[i][j] = aim[i][j] = 0.0;
+    }
+  for (int at = 0; at < natm; ++at) {
+    __syncthreads();                                  // the previous atom's factors are consumed
+    const double *rfa = k.rf + (size_t)at * ngp * nl2;
+    for (int e = tid; e < 2 * kOT * nl2; e += 256) {
+      const int q = e / nl2, c = e - q * nl2;
+      const bool isr = q < kOT;
+      const int idx = (isr ? row0 + q : col0 + q - kOT);
+      const double v = idx < ngp ? rfa[(size_t)idx * nl2 + c] : 0.0;
+      if (isr) rr[q][c] = v; else rc[q - kOT][c] = v;
+    }
+    for (int e = tid; e < 2 * kOT; e += 256) {
+      const bool isr = e < kOT;
+      const int q = isr ? e : e - kOT;
+      const int idx = (isr ? row0 : col0) + q;
+      const cplx s = idx < ngp ? k.sfacgp[idx + (size_t)k.ldg * at] : cplx{0.0, 0.0};
+      if (isr) sr[q] = s; else sc[q] = s;
+    }
+    __syncthreads();
+    double b1[4][4], b2[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b1[i][j] = b2[i][j] = 0.0;
+    for (int l = nl - 1; l >= 0; --l) {
+      const double al = (double)(2 * l + 1) / (double)(l + 1), bl = -(double)(l + 1) / (double)(l + 2);
+      double p0[4], p1[4], q0[4], q1[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        p0[i] = rr[tr + 16 * i][2 * l];
+        p1[i] = rr[tr + 16 * i][2 * l + 1];
+        q0[i] = rc[tc + 16 * i][2 * l];
+        q1[i] = rc[tc + 16 * i][2 * l + 1];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double a = p0[i] * q0[j] + p1[i] * q1[j];
+          const double b0 = a + al * x[i][j] * b1[i][j] + bl * b2[i][j];
+          b2[i][j] = b1[i][j];
+          b1[i][j] = b0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const cplx s = sr[tr + 16 * i], t = sc[tc + 16 * j];
+        are[i][j] += (s.re * t.re + s.im * t.im) * b1[i][j];     // conj(S(G)) S(G')
+        aim[i][j] += (s.re * t.im - s.im * t.re) * b1[i][j];
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int cl = tc + 16 * j, c = col0 + cl;
+    if (c >= ngp) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = tr + 16 * i, r = row0 + rl;
+      if (r > c) continue;
+      double re = are[i][j], im = aim[i][j];
+      const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
+      if (ig) {
+        const cplx th = gt.cfunig[ig - 1];
+        re += th.re;
+        im += th.im;
+      }
+      k.o[r + (size_t)c * k.ldh] = cplx{re, im};
+      if (r < c) k.o[c + (size_t)r * k.ldh] = cplx{re, -im};
+    }
+  }
+}
+
+}  // namespace
+
+int exc_olp_tile() { return kOT; }
+
+int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return EXC_OK;
+  if (d.lmaxmat + 1 > kNLmax || d.apwordmax != 2) return EXC_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(olp_kernel, dim3(nitems), dim3(256), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
+                     make_gtab(d));
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

@@ -19,6 +19,9 @@ namespace match_dev {
 // Miller downward recurrence (normalised by j_0 or j_1 in closed form) otherwise.
 template <int L>
 __device__ __forceinline__ void sbessel_seq(double x, double (&jl)[L + 2]) {
+  // no FMA contraction: the per-k and the grouped match kernels inline this code into
+  // different surroundings and must still round identically
+#pragma clang fp contract(off)
   constexpr int LL = L + 1;
   if (x < 1e-8) {
     double t = 1.0, dfac = 1.0;
@@ -85,7 +88,9 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
                                            const double *__restrict__ rmt, double fpso,
                                            const double *__restrict__ dinv, const double *__restrict__ gpc,
                                            const double *__restrict__ tpgpc, const cplx *__restrict__ sfacgp,
-                                           int ldsf, cplx *__restrict__ apwalm, int ldg) {
+                                           int ldsf, cplx *__restrict__ apwalm, int ldg, double *__restrict__ rf = nullptr,
+                                           int nl = 0, int ngp = 0) {
+#pragma clang fp contract(off)   // see sbessel_seq
   constexpr int lmmax = (L + 1) * (L + 1);
   const double g = gpc[ig];
   const double x = g * rmt[spidx[ias]];
@@ -135,8 +140,8 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
         case 2: il = {-fpso, 0.0}; break;
         default: il = {0.0, -fpso}; break;
       }
-      const cplx c0 = cmul(il, sf);
-      const cplx cp = cmul(c0, cconj(ylm));                      // m >= 0
+      const cplx c0 = {il.re * sf.re - il.im * sf.im, il.re * sf.im + il.im * sf.re};
+      const cplx cp = {c0.re * ylm.re + c0.im * ylm.im, c0.im * ylm.re - c0.re * ylm.im};   // c0 conj(Y), m >= 0
       const double sgn = (m & 1) ? -1.0 : 1.0;
       const cplx cn = {sgn * (c0.re * ylm.re - c0.im * ylm.im),   // conj(Y_{l,-m}) = (-1)^m Y_lm
                        sgn * (c0.re * ylm.im + c0.im * ylm.re)};
@@ -144,6 +149,14 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
       const double d10 = di[l * 4 + 1], d11 = di[l * 4 + 3];
       const double r0 = d00 * jl[l] + d01 * djl[l];
       const double r1 = d10 * jl[l] + d11 * djl[l];
+      if (rf && m == 0 && l < nl) {
+        // radial factors for the overlap by the addition theorem, weighted by
+        // (4 pi / sqrt(Omega)) sqrt((2l+1) / 4 pi): rf[(ias*ngp + ig)*2nl + 2l + io]
+        const double wl = fpso * sqrt((double)(2 * l + 1) * 0.079577471545947667884);   // 1/(4 pi)
+        double *q = rf + ((size_t)ias * ngp + ig) * 2 * nl + 2 * l;
+        q[0] = wl * r0;
+        q[1] = wl * r1;
+      }
       const int lmp = l * (l + 1) + m;
       out[colstride * (0 + 2 * lmp)] = {cp.re * r0, cp.im * r0};
       out[colstride * (1 + 2 * lmp)] = {cp.re * r1, cp.im * r1};

@@ -49,6 +49,7 @@ class KPointDev(C.Structure):
 EXC_BATCH_MATCH = 1
 EXC_BATCH_GRAPH = 2
 EXC_BATCH_SPLIT_BF16 = 4
+EXC_BATCH_OLP_LEGENDRE = 8
 
 _lib = None
 
@@ -79,7 +80,7 @@ SIGNATURES = {
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp")
 
 
 def load_library(path: os.PathLike | None = None):

@@ -145,6 +145,11 @@ typedef struct exc_kpoint_dev {
 #define EXC_BATCH_SPLIT_BF16 4 /* muffin-tin contraction from split-BF16 slices with FP32 MFMA   */
                                /* accumulation and per-(row, atom) power-of-two scaling (3 slices,*/
                                /* 6 slice pairs); Y stage, interstitial, lo blocks stay FP64      */
+#define EXC_BATCH_OLP_LEGENDRE 8 /* needs EXC_BATCH_MATCH: the muffin-tin part of O from the      */
+                                 /* radial factors of match by the spherical-harmonic addition   */
+                                 /* theorem, sum_a conj(S_a) S_a' sum_l P_l(u.u') sum_io r r',   */
+                                 /* FP64 on the vector units (the same matrix as conj(A) A^T,    */
+                                 /* rounded differently); H is unchanged                         */
 
 /* match (optional) + H and O of nk k-points, each with its own basis size, in one grouped
  * launch per stage on the context's stream; the muffin-tin stage is a persistent kernel over
@@ -189,6 +194,7 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_SPLIT 6    /* slicing of A and Yt into BF16 planes (EXC_BATCH_SPLIT_BF16)   */
 #define EXC_STAGE_BSPLIT 7   /* split-BF16 muffin-tin stage of exc_setup_batched_dev          */
 #define EXC_STAGE_VJP 8      /* exc_ho_vjp_dev                                                */
+#define EXC_STAGE_OLP 9      /* addition-theorem overlap (EXC_BATCH_OLP_LEGENDRE)             */
 #define EXC_NSTAGE 12
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 

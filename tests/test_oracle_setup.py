@@ -209,3 +209,48 @@ def test_config_sizes():
     assert w1.crystal.natm == 8 and w1.crystal.nlotot == 32 and len(w1.kpoints) == 16
     assert all(1150 <= k.ngp <= 1250 for k in w1.kpoints)
     assert len({k.ngp for k in w1.kpoints}) > 4
+
+
+def _radial_factors(cr, kp):
+    """rf(atom, G, l, io) = (4 pi/sqrt(Omega)) sqrt((2l+1)/(4 pi)) r_{io,l}(|G+k|), r = D^-1
+    [j_l(|G+k| R), |G+k| j_l'(|G+k| R)]: the real radial part of apwalm (match.f90:83-187)."""
+    from oracle import specfun
+    ngp, nl = kp.ngp, cr.lmaxmat + 1
+    fpso = 4 * math.pi / math.sqrt(cr.omega)
+    g = kp.gpc[:ngp]
+    rf = np.zeros((cr.natm, ngp, nl, cr.apwordmax))
+    for a in range(cr.natm):
+        x = g * cr.rmt[cr.spidx[a]]
+        j = specfun.sbessel_array(cr.lmaxapw, x)
+        dj = specfun.sbesseld1_array(cr.lmaxapw, x) * g[None, :]
+        for l in range(nl):
+            r = np.linalg.solve(cr.dmat[:, :, l, a], np.stack([j[l], dj[l]]))
+            rf[a, :, l, :] = fpso * math.sqrt((2 * l + 1) / (4 * math.pi)) * r.T
+    return rf
+
+
+@pytest.mark.parametrize("key", ["a", "d"])
+def test_overlap_addition_theorem(key):
+    """The identity the GPU's EXC_BATCH_OLP_LEGENDRE overlap rests on: with apwalm = i^l r S
+    conj(Y_lm), the reference's muffin-tin overlap conj(A) A^T equals
+    sum_a conj(S_a(G)) S_a(G') sum_l P_l(u.u') sum_io rf rf' (addition theorem), here with
+    P_l from scipy rather than the kernel's Clenshaw recurrence.  A dropped (2l+1) weight,
+    a missing conjugate or a wrong l range fails it by O(1)."""
+    w = small(key)
+    cr = w.crystal
+    for kp in w.kpoints:
+        n = kp.ngp
+        A = R.match(cr, kp)
+        apwi = [R._apwi(cr, A, n, a) for a in range(cr.natm)]
+        O_mt = R._mt_sum(cr, apwi, apwi)
+        rf = _radial_factors(cr, kp)
+        th, ph = kp.tpgpc[0, :n], kp.tpgpc[1, :n]
+        u = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+        x = np.clip(u @ u.T, -1.0, 1.0)
+        O = np.zeros((n, n), complex)
+        for a in range(cr.natm):
+            t = sum(sp.eval_legendre(l, x) * (rf[a, :, l, :] @ rf[a, :, l, :].T) for l in range(cr.lmaxmat + 1))
+            S = kp.sfacgp[:n, a]
+            O += np.conj(S)[:, None] * S[None, :] * t
+        err = np.linalg.norm(O - O_mt) / np.linalg.norm(O_mt)
+        assert err < 1e-13, err
