@@ -248,6 +248,10 @@ int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, in
     case 5: return launch_bystage_dma_t<64, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 6: return launch_bystage_dma_t<128, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 7: return launch_bystage_dma_t<64, 64, 8, 4, 2, 2>(d, nk, maxngp, ks, s);
+    case 8: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 9: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 10: return launch_bystage_dma_t<64, 64, 32, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 11: return launch_bystage_dma_t<64, 128, 8, 3, 2, 4>(d, nk, maxngp, ks, s);
     default: return EXC_ERR_ARG;
   }
 }

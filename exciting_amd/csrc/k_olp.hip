@@ -12,8 +12,8 @@
 // same matrix as the reference's conj(A) A^T contraction (olpaa, overlapsetup), a different
 // rounding: about 440 FP64 FMAs per element instead of 8 natm naa flops, on the VALU.
 //
-// Workgroup: one 64x64 tile of the upper triangle (256 threads, 4x4 elements each); per
-// atom the rows' and columns' radial factors are staged in LDS.  Epilogue as the MFMA
+// Workgroup: one 64x64 tile of the upper triangle (512 threads, 4x2 elements each); per
+// atom the rows' and columns' radial factors are staged in LDS as (io0, io1) pairs.  Epilogue as the MFMA
 // tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
 #include "setup_dev.h"
 
@@ -23,11 +23,16 @@ using namespace setup_dev;
 
 constexpr int kOT = 64;        // tile edge
 constexpr int kNLmax = 16;     // lmaxmat + 1 supported
-constexpr int kRS = 2 * kNLmax + 1;
+constexpr int kRS = kNLmax + 1;  // LDS row stride in (io0, io1) pairs: 34 doubles, conflict-free b128 reads
+constexpr int kThreads = 512;  // 8 waves; thread (tr, tc) owns rows tr + 16 i (i < 4), cols tc + 32 j (j < 2)
 
-__global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
-                                                  int nitems, int natm, int nl, GTab gt) {
-  __shared__ double rr[kOT][kRS], rc[kOT][kRS];
+struct d2 {
+  double a, b;
+};
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void olp_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                       int nitems, int natm, int nl, GTab gt) {
+  __shared__ d2 rr[kOT][kRS], rc[kOT][kRS];
   __shared__ cplx sr[kOT], sc[kOT];
   __shared__ double ur[kOT][3], uc[kOT][3];
   __shared__ int gr[kOT][3], gc[kOT][3];
@@ -37,8 +42,7 @@ __global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, 
   const kdesc k = ks[w.k];
   const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
   const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
-  const int nl2 = 2 * nl;
-  for (int e = tid; e < 2 * kOT; e += 256) {
+  for (int e = tid; e < 2 * kOT; e += kThreads) {
     const bool isr = e < kOT;
     const int q = isr ? e : e - kOT;
     const int idx = (isr ? row0 : col0) + q;
@@ -58,26 +62,26 @@ __global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, 
     g[q][0] = g0; g[q][1] = g1; g[q][2] = g2;
   }
   __syncthreads();
-  double x[4][4], are[4][4], aim[4][4];
+  double x[4][2], are[4][2], aim[4][2];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int a = tr + 16 * i, b = tc + 16 * j;
+    for (int j = 0; j < 2; ++j) {
+      const int a = tr + 16 * i, b = tc + 32 * j;
       x[i][j] = ur[a][0] * uc[b][0] + ur[a][1] * uc[b][1] + ur[a][2] * uc[b][2];
       are[i][j] = aim[i][j] = 0.0;
     }
   for (int at = 0; at < natm; ++at) {
     __syncthreads();                                  // the previous atom's factors are consumed
-    const double *rfa = k.rf + (size_t)at * ngp * nl2;
-    for (int e = tid; e < 2 * kOT * nl2; e += 256) {
-      const int q = e / nl2, c = e - q * nl2;
+    const d2 *rfa = (const d2 *)(k.rf + (size_t)at * ngp * 2 * nl);
+    for (int e = tid; e < 2 * kOT * nl; e += kThreads) {
+      const int q = e / nl, l = e - q * nl;
       const bool isr = q < kOT;
       const int idx = (isr ? row0 + q : col0 + q - kOT);
-      const double v = idx < ngp ? rfa[(size_t)idx * nl2 + c] : 0.0;
-      if (isr) rr[q][c] = v; else rc[q - kOT][c] = v;
+      const d2 v = idx < ngp ? rfa[(size_t)idx * nl + l] : d2{0.0, 0.0};
+      if (isr) rr[q][l] = v; else rc[q - kOT][l] = v;
     }
-    for (int e = tid; e < 2 * kOT; e += 256) {
+    for (int e = tid; e < 2 * kOT; e += kThreads) {
       const bool isr = e < kOT;
       const int q = isr ? e : e - kOT;
       const int idx = (isr ? row0 : col0) + q;
@@ -85,26 +89,24 @@ __global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, 
       if (isr) sr[q] = s; else sc[q] = s;
     }
     __syncthreads();
-    double b1[4][4], b2[4][4];
+    double b1[4][2], b2[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b1[i][j] = b2[i][j] = 0.0;
+      for (int j = 0; j < 2; ++j) b1[i][j] = b2[i][j] = 0.0;
+#pragma unroll 1
     for (int l = nl - 1; l >= 0; --l) {
       const double al = (double)(2 * l + 1) / (double)(l + 1), bl = -(double)(l + 1) / (double)(l + 2);
-      double p0[4], p1[4], q0[4], q1[4];
+      d2 p[4], q[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        p0[i] = rr[tr + 16 * i][2 * l];
-        p1[i] = rr[tr + 16 * i][2 * l + 1];
-        q0[i] = rc[tc + 16 * i][2 * l];
-        q1[i] = rc[tc + 16 * i][2 * l + 1];
-      }
+      for (int i = 0; i < 4; ++i) p[i] = rr[tr + 16 * i][l];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) q[j] = rc[tc + 32 * j][l];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double a = p0[i] * q0[j] + p1[i] * q1[j];
+        for (int j = 0; j < 2; ++j) {
+          const double a = p[i].a * q[j].a + p[i].b * q[j].b;
           const double b0 = a + al * x[i][j] * b1[i][j] + bl * b2[i][j];
           b2[i][j] = b1[i][j];
           b1[i][j] = b0;
@@ -113,15 +115,15 @@ __global__ __launch_bounds__(256) void olp_kernel(const kdesc *__restrict__ ks, 
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const cplx s = sr[tr + 16 * i], t = sc[tc + 16 * j];
+      for (int j = 0; j < 2; ++j) {
+        const cplx s = sr[tr + 16 * i], t = sc[tc + 32 * j];
         are[i][j] += (s.re * t.re + s.im * t.im) * b1[i][j];     // conj(S(G)) S(G')
         aim[i][j] += (s.re * t.im - s.im * t.re) * b1[i][j];
       }
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int cl = tc + 16 * j, c = col0 + cl;
+  for (int j = 0; j < 2; ++j) {
+    const int cl = tc + 32 * j, c = col0 + cl;
     if (c >= ngp) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -147,7 +149,7 @@ int exc_olp_tile() { return kOT; }
 int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
   if (d.lmaxmat + 1 > kNLmax || d.apwordmax != 2) return EXC_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(olp_kernel, dim3(nitems), dim3(256), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
+  hipLaunchKernelGGL(olp_kernel, dim3(nitems), dim3(kThreads), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
                      make_gtab(d));
   EXC_CHECK(hipGetLastError());
   return EXC_OK;

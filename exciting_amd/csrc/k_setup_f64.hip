@@ -124,6 +124,10 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
     case 5: return launch_ystage_dma_t<64, 64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 6: return launch_ystage_dma_t<128, 64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 7: return launch_ystage_dma_t<64, 64, 8, 4, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 8: return launch_ystage_dma_t<64, 128, 8, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 9: return launch_ystage_dma_t<64, 128, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 10: return launch_ystage_dma_t<64, 64, 32, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 11: return launch_ystage_dma_t<64, 128, 8, 3, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     default: return EXC_ERR_ARG;
   }
 }
