@@ -11,10 +11,13 @@ oracle, timed on the host's cores on a bounded sample of the same k-points).
 import argparse
 import json
 import os
+import pathlib
 import sys
 import time
 
 import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -250,6 +253,17 @@ def main():
             "kernel": ("bmt_dma_kernel (grouped, all k)" if args.path == "batched" else "mt_dma_kernel (per k)")
                       + (": A^H.Y + interstitial (H)" if olpflag else ": A^H.Y and A^H.A + interstitial"),
             "per_launch_flops": fl / max(mt_cnt, 1)}
+    # HBM traffic of the dominant kernel: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this same
+    # default command, scaled by a 1 GiB copy (committed summary); only for the profiled setup
+    prof = HERE / "profiles" / "setup_kernels_workload1_16k.json"
+    if (args.path == "batched" and args.config == 1 and args.nk is None and not args.tune and olpflag
+            and prof.exists()):
+        rec = json.loads(prof.read_text()).get("bmt_dma_kernel")
+        if rec:
+            roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
+            roof["traffic_source"] = str(prof.relative_to(HERE))
+            roof["algorithmic_bytes"] = sum(32.0 * kp.ngp * cr.natm * cr.maxaa + 16.0 * kp.ngp ** 2
+                                            for kp in w.kpoints)
     stages = {k: {"ms_per_launch": (ms / cnt if cnt else None), "launches": cnt} for k, (ms, cnt) in tim.items() if cnt}
     if eager:
         stages["grouped_eager_pass"] = {k: ms for k, (ms, cnt) in eager.items() if cnt}
