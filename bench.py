@@ -152,9 +152,10 @@ def main():
     ap.add_argument("--no-olp", action="store_true",
                     help="grouped path: overlap muffin-tin part by the MFMA contraction instead of the "
                          "addition theorem (EXC_BATCH_OLP_LEGENDRE)")
-    ap.add_argument("--path", choices=["batched", "perk"], default="batched",
+    ap.add_argument("--path", choices=["batched", "perk", "host"], default="batched",
                     help="batched: all k-points in one grouped launch per stage, replayed as a hipGraph; "
-                         "perk: one launch sequence per k-point")
+                         "perk: one launch sequence per k-point on device arrays; host: the host-array "
+                         "entry points the Fortran driver calls (PCIe copies of apwalm, H and O included)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -195,10 +196,22 @@ def main():
     def step_batched(graph=True):
         ctx.setup_batched_dev(batch, lapw.EXC_BATCH_MATCH | olpflag | (lapw.EXC_BATCH_GRAPH if graph else 0))
 
+    if args.path == "host":
+        # preallocated host arrays, as a Fortran caller holds them (pageable memory)
+        h_apw = [np.zeros((kp.igpig.shape[0], cr.apwordmax, cr.lmmaxapw, cr.natm), np.complex128, order="F")
+                 for kp in w.kpoints]
+        h_ho = [(np.zeros((kp.ngp + cr.nlotot,) * 2, np.complex128, order="F"),
+                 np.zeros((kp.ngp + cr.nlotot,) * 2, np.complex128, order="F")) for kp in w.kpoints]
+
+    def step_host():
+        for i, kp in enumerate(w.kpoints):
+            ctx.match(kp, out=h_apw[i])
+            ctx.hamiltonandoverlapsetup(kp, h_apw[i], out=h_ho[i])
+
     tuned = None
     if args.tune:
         from exciting_amd import autotune
-        if args.path == "perk":
+        if args.path in ("perk", "host"):
             tuned = autotune.tune(ctx, ["mt", "ystage"], step_perk, reps=2, log=log)
         else:
             tuned = autotune.tune(ctx, ["bmt", "ystage"], lambda: step_batched(False), reps=2, log=log)
@@ -217,7 +230,7 @@ def main():
         step = step_batched
     else:
         eager = None
-        step = step_perk
+        step = step_perk if args.path == "perk" else step_host
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -282,7 +295,8 @@ def main():
         "config": {"workload": f"config{args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
                    "path": ("FP64, all k-points grouped per stage, hipGraph replay"
                             + (", overlap MT part by the addition theorem" if olpflag else "")
-                            if args.path == "batched" else "FP64, per-k launches"),
+                            if args.path == "batched" else "FP64, per-k launches"
+                            + (", host arrays (PCIe copies included)" if args.path == "host" else "")),
                    "parallelism": "single GPU", "variants": tuned},
         "roofline": roof, "stages": stages, "split_bf16": split, "cpu_baseline": base,
     }

@@ -201,11 +201,18 @@ class Context:
             pass
 
     # ------------------------------------------------------------------ host-array entry points
-    def match(self, kp, ngkmax=None):
-        """apwalm(ngkmax, apwordmax, lmmaxapw, natm) for k-point kp (match.f90:13)."""
+    def match(self, kp, ngkmax=None, out=None):
+        """apwalm(ngkmax, apwordmax, lmmaxapw, natm) for k-point kp (match.f90:13); `out`: a
+        preallocated Fortran-ordered complex128 array of that shape."""
         cr = self.cr
         ngkmax = kp.igpig.shape[0] if ngkmax is None else ngkmax
-        apw = np.zeros((ngkmax, cr.apwordmax, cr.lmmaxapw, cr.natm), dtype=np.complex128, order="F")
+        shape = (ngkmax, cr.apwordmax, cr.lmmaxapw, cr.natm)
+        if out is None:
+            apw = np.zeros(shape, dtype=np.complex128, order="F")
+        else:
+            if out.shape != shape or out.dtype != np.complex128 or not out.flags.f_contiguous:
+                raise ValueError("match: out must be a Fortran-ordered complex128 array of shape %s" % (shape,))
+            apw = out
         gpc = _f(kp.gpc[:ngkmax], np.float64)
         tp = _f(kp.tpgpc[:, :ngkmax], np.float64)
         sf = _f(kp.sfacgp[:ngkmax], np.complex128)
@@ -213,13 +220,20 @@ class Context:
                                      ok_positive=True)
         return apw
 
-    def hamiltonandoverlapsetup(self, kp, apwalm, ldh=None):
-        """(H, O) of k-point kp (hamiltonandoverlapsetup.f90:8), Fortran-ordered (ldh, nmat)."""
+    def hamiltonandoverlapsetup(self, kp, apwalm, ldh=None, out=None):
+        """(H, O) of k-point kp (hamiltonandoverlapsetup.f90:8), Fortran-ordered (ldh, nmat);
+        `out`: a preallocated pair of such arrays."""
         nmat = kp.ngp + self.nlotot
         ldh = nmat if ldh is None else ldh
         ngkmax = apwalm.shape[0]
-        H = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
-        O = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
+        if out is None:
+            H = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
+            O = np.zeros((ldh, nmat), dtype=np.complex128, order="F")
+        else:
+            H, O = out
+            for m in (H, O):
+                if m.shape != (ldh, nmat) or m.dtype != np.complex128 or not m.flags.f_contiguous:
+                    raise ValueError("hamiltonandoverlapsetup: out must be Fortran-ordered complex128 (ldh, nmat)")
         apw = _f(apwalm, np.complex128)
         ig = _f(kp.igpig[:ngkmax], np.int32)
         vg = _f(kp.vgpc[:, :ngkmax], np.float64)
