@@ -191,7 +191,7 @@ def main():
             ctx.match_dev(d.ngp, d.ngkmax, d.gpc, d.tpgpc, d.sfacgp, apw[i])
             ctx.setup_dev(d.ngp, d.ngkmax, apw[i], d.igpig, d.vgpc, nmats[i], Hs[i], Os[i])
 
-    olpflag = 0 if (args.no_olp or args.path == "perk") else lapw.EXC_BATCH_OLP_LEGENDRE
+    olpflag = lapw.EXC_BATCH_OLP_LEGENDRE if (args.path == "batched" and not args.no_olp) else 0
 
     def step_batched(graph=True):
         ctx.setup_batched_dev(batch, lapw.EXC_BATCH_MATCH | olpflag | (lapw.EXC_BATCH_GRAPH if graph else 0))

@@ -4,6 +4,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "exc_common.h"
@@ -20,9 +22,11 @@ struct exc_ctx {
   // grow-only workspaces
   cplx *yt;
   size_t yt_bytes;
-  // staging for the host-pointer entry points
+  // staging for the host-pointer entry points: device buffers and a pinned host buffer
   void *stage;
   size_t stage_bytes;
+  void *hstage;
+  size_t hstage_bytes;
   std::vector<void *> owned;
   // per-stage HIP-event timers (exc_ctx_set_timing)
   bool timing;
@@ -80,6 +84,47 @@ int dev_upload(exc_ctx *c, T **p, const void *src, size_t n) {
   }
   if (n) EXC_CHECK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
   return EXC_OK;
+}
+
+// pinned host staging (grow-only); every host entry synchronises before it returns, so no
+// copy into the old buffer is in flight when it is replaced
+int grow_host(void **p, size_t *have, size_t need) {
+  if (*have >= need) return EXC_OK;
+  if (*p) hipHostFree(*p);
+  *p = nullptr;
+  *have = 0;
+  EXC_CHECK(hipHostMalloc(p, need, hipHostMallocDefault));
+  *have = need;
+  return EXC_OK;
+}
+
+// ncols columns of rowbytes bytes between column-major host arrays (pitches dld, sld);
+// large transfers are split over host threads (the pageable side is the caller's array)
+void copy_cols(char *dst, size_t dld, const char *src, size_t sld, size_t rowbytes, size_t ncols) {
+  auto work = [=](size_t j0, size_t j1) {
+    for (size_t j = j0; j < j1; ++j) memcpy(dst + j * dld, src + j * sld, rowbytes);
+  };
+  const unsigned hc = std::thread::hardware_concurrency();
+  const unsigned nt = (rowbytes * ncols >= ((size_t)8 << 20)) ? std::min(8u, std::max(1u, hc)) : 1u;
+  if (nt <= 1 || ncols < nt) {
+    work(0, ncols);
+    return;
+  }
+  const size_t per = (ncols + nt - 1) / nt;
+  std::vector<std::thread> th;
+  size_t done = per;                      // columns [0, per) run on this thread
+  try {
+    for (unsigned t = 1; t < nt; ++t) {
+      const size_t a = t * per, b = std::min(ncols, a + per);
+      if (a >= b) break;
+      th.emplace_back(work, a, b);
+      done = b;
+    }
+  } catch (...) {                         // no threads available: copy the rest here
+  }
+  work(0, std::min(ncols, per));
+  if (done < ncols) work(done, ncols);
+  for (auto &x : th) x.join();
 }
 
 int grow(void **p, size_t *have, size_t need) {
@@ -212,6 +257,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
   if (c->stage) hipFree(c->stage);
+  if (c->hstage) hipHostFree(c->hstage);
   for (auto &m : c->marks) {
     hipEventDestroy(m.a);
     hipEventDestroy(m.b);
@@ -477,9 +523,13 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
   rc = exc_match_dev(c, ngp, ngkmax, d_gpc, d_tp, d_sf, d_apw);
   if (rc < 0) return rc;
   // only rows 1..ngp of every column are results; the caller's padding rows stay untouched
-  if (ngp > 0)
-    EXC_CHECK(hipMemcpy2DAsync(apwalm, 16 * nk, d_apw, 16 * nk, 16 * (size_t)ngp, ao * lmm * na,
-                               hipMemcpyDeviceToHost, c->stream));
+  if (ngp > 0) {
+    int r2 = grow_host(&c->hstage, &c->hstage_bytes, b_apw);
+    if (r2) return r2;
+    EXC_CHECK(hipMemcpyAsync(c->hstage, d_apw, b_apw, hipMemcpyDeviceToHost, c->stream));
+    EXC_CHECK(hipStreamSynchronize(c->stream));
+    copy_cols((char *)apwalm, 16 * nk, (const char *)c->hstage, 16 * nk, 16 * (size_t)ngp, ao * lmm * na);
+  }
   EXC_CHECK(hipStreamSynchronize(c->stream));
   return rc;
 }
@@ -530,10 +580,13 @@ int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *a
   const size_t o_ig = (b_apw + 255) & ~(size_t)255, o_vg = o_ig + ((b_ig + 255) & ~(size_t)255);
   const size_t o_h = o_vg + ((b_vg + 255) & ~(size_t)255), o_o = o_h + ((b_h + 255) & ~(size_t)255);
   if ((rc = grow(&c->stage, &c->stage_bytes, o_o + b_h))) return rc;
+  if ((rc = grow_host(&c->hstage, &c->hstage_bytes, std::max(b_apw, 2 * b_h)))) return rc;
   char *base = (char *)c->stage;
-  if (ngp > 0)   // rows 1..ngp of apwalm are the input; padding rows are never read
-    EXC_CHECK(hipMemcpy2DAsync(base, 16 * nk, apwalm, 16 * nk, 16 * (size_t)ngp, ao * lmm * na,
-                               hipMemcpyHostToDevice, c->stream));
+  char *hs = (char *)c->hstage;
+  if (ngp > 0) {   // rows 1..ngp of apwalm are the input; the caller's padding rows are never read
+    copy_cols(hs, 16 * nk, (const char *)apwalm, 16 * nk, 16 * (size_t)ngp, ao * lmm * na);
+    EXC_CHECK(hipMemcpyAsync(base, hs, b_apw, hipMemcpyHostToDevice, c->stream));
+  }
   EXC_CHECK(hipMemcpyAsync(base + o_ig, igpig, b_ig, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_vg, vgpc, b_vg, hipMemcpyHostToDevice, c->stream));
   rc = exc_hamiltonandoverlapsetup_dev(c, ngp, ngkmax, (const double *)base, (const int *)(base + o_ig),
@@ -542,9 +595,11 @@ int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *a
   if (rc) return rc;
   // rows nmat+1..ldh of the caller's matrices are not part of the result: copy rows 1..nmat
   const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
-  EXC_CHECK(hipMemcpy2DAsync(h, pitch, base + o_h, pitch, width, nmat, hipMemcpyDeviceToHost, c->stream));
-  EXC_CHECK(hipMemcpy2DAsync(o, pitch, base + o_o, pitch, width, nmat, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(hs, base + o_h, b_h, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(hs + b_h, base + o_o, b_h, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
+  copy_cols((char *)h, pitch, hs, pitch, width, nmat);
+  copy_cols((char *)o, pitch, hs + b_h, pitch, width, nmat);
   return EXC_OK;
 }
 
