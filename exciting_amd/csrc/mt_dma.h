@@ -62,10 +62,14 @@ struct DmaCore {
   static_assert(BM % 64 == 0 && BN % 64 == 0 && NL % kWaves == 0, "DMA split");
   static_assert(NST >= 2 && NST <= 4, "stages");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "MFMA blocks");
-  struct Stage {
+  struct __attribute__((aligned(16))) Stage {
     cplx a[BK][BM];
     cplx b[BK][BN];
   };
+  // one complex operand as a 16-byte vector: a single ds_read_b128 (16 lanes per LDS pass,
+  // contiguous), where two 8-byte reads of 32 lanes would collide on the banks of the k-rows
+  // 1 KiB apart
+  typedef double dv2 __attribute__((ext_vector_type(2)));
 
   // issue the DMAs of the K-step starting at k0 into stage st; rows past mrows / nrows are
   // clamped to row 0, k-columns past ktot are zero-filled
@@ -111,16 +115,16 @@ struct DmaCore {
       double ar[MB], ai[MB], br[NB], bi[NB], nai[MB];
 #pragma unroll
       for (int i = 0; i < MB; ++i) {
-        const cplx v = st->a[kk + lk][wr * WM + i * 16 + li];
-        ar[i] = v.re;
-        ai[i] = v.im;
-        nai[i] = -v.im;
+        const dv2 v = *reinterpret_cast<const dv2 *>(&st->a[kk + lk][wr * WM + i * 16 + li]);
+        ar[i] = v.x;
+        ai[i] = v.y;
+        nai[i] = -v.y;
       }
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        const cplx v = st->b[kk + lk][wc * WN + j * 16 + li];
-        br[j] = v.re;
-        bi[j] = v.im;
+        const dv2 v = *reinterpret_cast<const dv2 *>(&st->b[kk + lk][wc * WN + j * 16 + li]);
+        br[j] = v.x;
+        bi[j] = v.y;
       }
 #pragma unroll
       for (int i = 0; i < MB; ++i)
