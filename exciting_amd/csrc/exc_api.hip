@@ -60,6 +60,9 @@ struct exc_ctx {
   int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
   hipGraph_t graph;
   hipGraphExec_t graph_exec;
+  // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
   int ncu;
 };
 
@@ -256,6 +259,9 @@ int exc_ctx_destroy(exc_ctx *c) {
   else hipDeviceSynchronize();
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
+  if (c->side) hipStreamDestroy(c->side);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->stage) hipFree(c->stage);
   if (c->hstage) hipHostFree(c->hstage);
   for (auto &m : c->marks) {
@@ -742,12 +748,31 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   }
   const bool capture = flags & EXC_BATCH_GRAPH;
   if (capture && c->stream == nullptr) return EXC_ERR_ARG;   // the null stream cannot be captured
+  // inside a graph the overlap and lo kernels (they read only A, the radial factors and the
+  // structure factors, and write parts of H and O the MT stage does not) form a branch of
+  // their own beside the Y stage (and the slicing); the branches join before the persistent
+  // MT kernel, whose static work split assumes all its workgroups resident at once.  Eager
+  // calls keep one stream so the stage timers stay exact
+  const bool fork = capture;
+  if (fork && !c->side) {
+    EXC_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    EXC_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    EXC_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  hipStream_t s2 = fork ? c->side : c->stream;
   if (capture) EXC_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   auto launches = [&]() -> int {
     int r = EXC_OK;
     if (do_match) {
       StageTimer t(c, capture ? -1 : EXC_STAGE_MATCH);
       if ((r = exc_launch_bmatch(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
+    }
+    if (fork) {
+      if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return EXC_ERR_HIP;
+      if (hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) return EXC_ERR_HIP;
+      if (olp && (r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, s2))) return r;
+      if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, s2))) return r;
+      if (hipEventRecord(c->ev_join, c->side) != hipSuccess) return EXC_ERR_HIP;
     }
     {
       StageTimer t(c, capture ? -1 : EXC_STAGE_YSTAGE);
@@ -758,19 +783,22 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         StageTimer t(c, capture ? -1 : EXC_STAGE_SPLIT);
         if ((r = exc_launch_split(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
       }
+      if (fork && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
       StageTimer t(c, capture ? -1 : EXC_STAGE_BSPLIT);
       if ((r = exc_launch_bsplit(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream)))
         return r;
     } else {
+      if (fork && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
       StageTimer t(c, capture ? -1 : EXC_STAGE_BMT);
       if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream))) return r;
     }
+    if (fork) return r;
     if (olp) {
-      StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
+      StageTimer t(c, EXC_STAGE_OLP);
       if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, c->stream))) return r;
     }
     {
-      StageTimer t(c, capture ? -1 : EXC_STAGE_LO);
+      StageTimer t(c, EXC_STAGE_LO);
       if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
     }
     return r;
