@@ -268,7 +268,8 @@ int exc_bmt_tile(int variant) {
     case 7: return 64;
     case 8: return 64;
     case 9: return 32;
-    case 10: case 11: case 12: case 13: return 64;
+    case 10: case 11: case 12: case 13: case 14: return 64;
+    case 15: case 16: case 17: return 128;
     default: return 0;
   }
 }
@@ -291,6 +292,10 @@ int exc_bmt_max_blocks(int variant) {
     case 11: return max_blocks_dma_t<64, 8, 4, 2, 4>();
     case 12: return max_blocks_dma_t<64, 16, 3, 2, 4>();
     case 13: return max_blocks_dma_t<64, 8, 3, 2, 2>();
+    case 14: return max_blocks_dma_t<64, 16, 2, 2, 2>();
+    case 15: return max_blocks_dma_t<128, 16, 2, 2, 4>();
+    case 16: return max_blocks_dma_t<128, 8, 2, 2, 4>();
+    case 17: return max_blocks_dma_t<128, 8, 3, 2, 4>();
     default: return 1;
   }
 }
@@ -313,6 +318,10 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 11: return launch_bmt_dma_t<64, 8, 4, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 12: return launch_bmt_dma_t<64, 16, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 13: return launch_bmt_dma_t<64, 8, 3, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 14: return launch_bmt_dma_t<64, 16, 2, 2, 2>(d, ks, items, nitems, nblocks, s);
+    case 15: return launch_bmt_dma_t<128, 16, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 16: return launch_bmt_dma_t<128, 8, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
+    case 17: return launch_bmt_dma_t<128, 8, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
     default: return EXC_ERR_ARG;
   }
 }

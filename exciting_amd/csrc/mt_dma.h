@@ -106,12 +106,15 @@ struct DmaCore {
   }
 
   // the MFMA sequence of zf64::Core::compute, operands from the interleaved stage
+  // kvalid: k-columns of this step inside [0, K); the 4-wide sub-steps past it hold only the
+  // zero fill and are skipped (adding exact zeros would not change the sums)
   __device__ __forceinline__ static void compute(const Stage *st, d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB], int wr,
-                                                 int wc) {
+                                                 int wc, int kvalid) {
     const int lane = threadIdx.x & 63;
     const int lk = lane >> 4, li = lane & 15;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
+      if (kk >= kvalid) break;
       double ar[MB], ai[MB], br[NB], bi[NB], nai[MB];
 #pragma unroll
       for (int i = 0; i < MB; ++i) {
@@ -157,11 +160,13 @@ struct DmaCore {
   }
 
   // the whole K loop.  On entry no DMA of this workgroup is in flight and every wave is done
-  // with the stages; s2, s3 are unused below NST 3, 4.
+  // with the stages; s2, s3 are unused below NST 3, 4.  A wave with active = false (its whole
+  // sub-tile below the diagonal of a diagonal tile, never stored) issues its share of the DMA
+  // and meets every barrier but skips the MFMAs.
   __device__ __forceinline__ static void run(Stage *s0, Stage *s1, Stage *s2, Stage *s3, const cplx *__restrict__ A,
                                              int lda, int row0, int mrows, const cplx *__restrict__ B, int ldb,
                                              int col0, int nrows, int ktot, const int *__restrict__ kperm,
-                                             d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB]) {
+                                             d4_t (&cr)[MB][NB], d4_t (&ci)[MB][NB], bool active = true) {
     const int nsteps = (ktot + BK - 1) / BK;
     Stage *st[4] = {s0, s1, s2, s3};
 #pragma unroll
@@ -187,7 +192,7 @@ struct DmaCore {
           if (step + NST - 1 < nsteps)
             issue(st[(u + NST - 1) % NST], A, lda, row0, mrows, B, ldb, col0, nrows, ktot, kperm,
                   (step + NST - 1) * BK);
-          compute(st[u], cr, ci, wr, wc);
+          if (active) compute(st[u], cr, ci, wr, wc, ktot - step * BK);
         }
       }
     }
@@ -244,10 +249,11 @@ struct MtTileDMA {
     }
     __builtin_amdgcn_s_waitcnt(0);                    // the table loads: only DMA from here on
     d4_t cr[C::MB][C::NB], ci[C::MB][C::NB];
-    C::run(s0, s1, s2, s3, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm, cr, ci);
+    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+    const bool below = ti == tj && wr * C::WM > wc * C::WN + C::WN - 1;   // wave sub-tile entirely r > c
+    C::run(s0, s1, s2, s3, apwalm, ldg, row0, ngp, B, ldb, col0, ngp, ktot, kperm, cr, ci, !below);
 
     cplx *out = which ? o : h;
-    const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
 #pragma unroll
     for (int i = 0; i < C::MB; ++i)
 #pragma unroll
