@@ -97,6 +97,10 @@ def run_split(args, ctx, batch, cr, w, fp64_value, log, olpflag=0):
     Of = batch.O[0].cpu().numpy().T.copy()
     flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_BF16 | olpflag
     tuned = None
+    for kv in args.set:
+        st, v = kv.split("=")
+        ctx.set_variant(st, int(v))
+        tuned = dict(tuned or {}, **{st: int(v)})
     if args.tune:
         from exciting_amd import autotune
         tuned = autotune.tune(ctx, ["bsplit"], lambda: ctx.setup_batched_dev(batch, flags), reps=2, log=log)
@@ -148,6 +152,8 @@ def main():
     ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune tile variants before timing")
+    ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
+                    help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 measurement")
     ap.add_argument("--no-olp", action="store_true",
                     help="grouped path: overlap muffin-tin part by the MFMA contraction instead of the "
@@ -209,6 +215,10 @@ def main():
             ctx.hamiltonandoverlapsetup(kp, h_apw[i], out=h_ho[i])
 
     tuned = None
+    for kv in args.set:
+        st, v = kv.split("=")
+        ctx.set_variant(st, int(v))
+        tuned = dict(tuned or {}, **{st: int(v)})
     if args.tune:
         from exciting_amd import autotune
         if args.path in ("perk", "host"):

@@ -629,9 +629,10 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (q.ngp > 0 && (!q.apwalm || !q.igpig || !q.vgpc)) return EXC_ERR_ARG;
     if (do_match && q.ngp > 0 && (!q.gpc || !q.tpgpc || !q.sfacgp)) return EXC_ERR_ARG;
   }
-  // defaults: the fastest variants measured on MI355X for N = 1150..2700 (grouped MT: 64x64
-  // tile, 8 waves, two LDS stages filled by DMA; split: 64x64 tile, LDS filled by DMA)
-  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 10);
+  // defaults: the fastest variants measured on MI355X for N = 1150..5000 (grouped MT: 64x64
+  // tile, 8 waves, two LDS stages filled by DMA, one item per workgroup; split: 64x64 tile,
+  // LDS filled by DMA)
+  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 18);
   const int yv = c->y_variant >= 0 ? c->y_variant : 8;     // 64x128 Y tiles, DMA-staged
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
@@ -750,10 +751,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   if (capture && c->stream == nullptr) return EXC_ERR_ARG;   // the null stream cannot be captured
   // inside a graph the overlap and lo kernels (they read only A, the radial factors and the
   // structure factors, and write parts of H and O the MT stage does not) form a branch of
-  // their own beside the Y stage (and the slicing); the branches join before the persistent
-  // MT kernel, whose static work split assumes all its workgroups resident at once.  Eager
-  // calls keep one stream so the stage timers stay exact
+  // their own beside the Y and MT stages; with a persistent MT kernel (static work split,
+  // all workgroups assumed resident) the branches join before it.  Eager calls keep one
+  // stream so the stage timers stay exact
   const bool fork = capture;
+  const bool join_early = split || exc_bmt_persistent(bv);
   if (fork && !c->side) {
     EXC_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     EXC_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -788,10 +790,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if ((r = exc_launch_bsplit(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream)))
         return r;
     } else {
-      if (fork && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
+      if (fork && join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
       StageTimer t(c, capture ? -1 : EXC_STAGE_BMT);
       if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream))) return r;
     }
+    if (fork && !join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
     if (fork) return r;
     if (olp) {
       StageTimer t(c, EXC_STAGE_OLP);
