@@ -16,16 +16,14 @@ namespace lo_dev {
 
 constexpr int kLoThreads = 128;
 constexpr int kMaxLo = 16;          // lo functions per atom supported
-constexpr int kRows = 64;           // APW rows (G+k) per workgroup of the APW-lo kernel
-constexpr int kSlots = 4;           // lo functions handled in parallel per row
-constexpr int kPerSlot = kMaxLo / kSlots;
+constexpr int kRows = 256;          // APW rows (G+k) per workgroup of the APW-lo kernel
+constexpr int kSlots = 1;           // threads per row (the workgroup is kRows * kSlots threads)
 
-// One workgroup = 64 APW rows x 4 lo slots of one atom.  Thread (r, s) accumulates the lo
-// functions s, s+4, s+8, s+12 of its row; results go through LDS so that both the
-// (row, lo) and the (lo, row) blocks are written in contiguous column segments.
+// One workgroup = 256 APW rows of one atom, one thread per row accumulating all the atom's
+// lo functions: each element of A is read once, the loa coefficients are wave-uniform
+// (scalar loads).  Products are written as explicit FMAs so that the per-k and the grouped
+// kernels, which inline this body, round identically.
 struct ApwLoShared {
-  cplx h[kMaxLo][kRows];
-  cplx o[kMaxLo][kRows];
   int first, cnt;
 };
 
@@ -34,10 +32,6 @@ __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, in
                                             const cplx *__restrict__ apwalm, int ldg, const cplx *__restrict__ loa,
                                             const double *__restrict__ oalo, cplx *__restrict__ h,
                                             cplx *__restrict__ o, int ldh) {
-  cplx(&s_h)[kMaxLo][kRows] = sh.h;
-  cplx(&s_o)[kMaxLo][kRows] = sh.o;
-  int &s_first = sh.first;
-  int &s_cnt = sh.cnt;
   const int tid = threadIdx.x;
   if (tid == 0) {
     int first = -1, cnt = 0;
@@ -46,71 +40,53 @@ __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, in
         if (first < 0) first = q;
         ++cnt;
       }
-    s_first = first;
-    s_cnt = cnt;
+    sh.first = first;
+    sh.cnt = cnt;
   }
   __syncthreads();
-  const int first = s_first, cnt = s_cnt;
-  if (cnt == 0) return;
-  const int r = tid % kRows, slot = tid / kRows;
-  const int i0 = bx * kRows;
-  const int i = i0 + r;
+  const int first = sh.first, cnt = sh.cnt;
+  const int i = bx * kRows + tid;
+  if (cnt == 0 || i >= ngp) return;
   const cplx *A = apwalm + (size_t)ldg * naa * ias;
   const cplx *la = loa + (size_t)maxnlo * naa * ias;
-  cplx acc[kPerSlot];
+  double hr[kMaxLo], hi[kMaxLo];
 #pragma unroll
-  for (int u = 0; u < kPerSlot; ++u) acc[u] = cplx{0.0, 0.0};
-  if (i < ngp) {
-    for (int f = 0; f < naa; ++f) {
-      const cplx a = A[i + (size_t)f * ldg];
+  for (int q = 0; q < kMaxLo; ++q) hr[q] = hi[q] = 0.0;
+#pragma unroll 4
+  for (int f = 0; f < naa; ++f) {
+    const cplx a = A[i + (size_t)f * ldg];
 #pragma unroll
-      for (int u = 0; u < kPerSlot; ++u) {
-        const int q = slot + kSlots * u;
-        if (q < cnt) {
-          const cplx w = la[q + (size_t)maxnlo * f];   // loa row q of the atom = its q-th lo function
-          acc[u].re += w.re * a.re - w.im * a.im;
-          acc[u].im += w.re * a.im + w.im * a.re;
-        }
+    for (int q = 0; q < kMaxLo; ++q) {
+      if (q < cnt) {
+        const cplx w = la[q + (size_t)maxnlo * f];   // loa row q of the atom = its q-th lo function
+        hr[q] = fma(w.re, a.re, hr[q]);
+        hr[q] = fma(-w.im, a.im, hr[q]);
+        hi[q] = fma(w.re, a.im, hi[q]);
+        hi[q] = fma(w.im, a.re, hi[q]);
       }
     }
   }
+  const size_t nlo0 = (size_t)ngp + first;
 #pragma unroll
-  for (int u = 0; u < kPerSlot; ++u) {
-    const int q = slot + kSlots * u;
+  for (int q = 0; q < kMaxLo; ++q) {
     if (q < cnt) {
-      s_h[q][r] = acc[u];
-      cplx ov = {0.0, 0.0};
-      if (i < ngp) {
-        const lo_desc dq = lod[first + q];
-        for (int io = 0; io < apwordmax; ++io) {
-          const cplx a = A[i + (size_t)(io + apwordmax * dq.lm) * ldg];
-          const double w = oalo[io + apwordmax * (dq.ilo + nlomax * ias)];
-          ov.re += a.re * w;
-          ov.im -= a.im * w;
-        }
+      const lo_desc dq = lod[first + q];
+      double orr = 0.0, oi = 0.0;
+      for (int io = 0; io < apwordmax; ++io) {
+        const cplx a = A[i + (size_t)(io + apwordmax * dq.lm) * ldg];
+        const double w = oalo[io + apwordmax * (dq.ilo + nlomax * ias)];
+        orr = fma(a.re, w, orr);
+        oi = fma(-a.im, w, oi);
       }
-      s_o[q][r] = ov;
+      // (row, lo): O primary, H = conj(H(lo, row)) — contiguous over the workgroup's rows
+      const size_t rl = (size_t)i + (nlo0 + q) * ldh;
+      h[rl] = cplx{hr[q], -hi[q]};
+      o[rl] = cplx{orr, oi};
+      // (lo, row): H primary (loa . A^T), O = conj(O(row, lo))
+      const size_t lr = (nlo0 + q) + (size_t)i * ldh;
+      h[lr] = cplx{hr[q], hi[q]};
+      o[lr] = cplx{orr, -oi};
     }
-  }
-  __syncthreads();
-  const int nrow = min(kRows, ngp - i0);
-  // (row, lo) blocks: O is the primary, H the conjugate — contiguous over rows
-  for (int e = tid; e < cnt * kRows; e += kRows * kSlots) {
-    const int q = e / kRows, rr = e % kRows;
-    if (rr >= nrow) continue;
-    const size_t idx = (size_t)(i0 + rr) + (size_t)(ngp + first + q) * ldh;
-    const cplx hv = s_h[q][rr];
-    h[idx] = cplx{hv.re, -hv.im};
-    o[idx] = s_o[q][rr];
-  }
-  // (lo, row) blocks: H is the primary (loa . A^T), O the conjugate — contiguous over lo
-  for (int e = tid; e < cnt * kRows; e += kRows * kSlots) {
-    const int rr = e / cnt, q = e % cnt;
-    if (rr >= nrow) continue;
-    const size_t idx = (size_t)(ngp + first + q) + (size_t)(i0 + rr) * ldh;
-    const cplx ov = s_o[q][rr];
-    h[idx] = s_h[q][rr];
-    o[idx] = cplx{ov.re, -ov.im};
   }
 }
 
