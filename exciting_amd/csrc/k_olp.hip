@@ -13,9 +13,11 @@
 // rounding: about 440 FP64 FMAs per element instead of 8 natm naa flops, on the VALU.
 //
 // Workgroup: one 64x64 tile of the upper triangle (512 threads, 4x2 elements each); per
-// atom the rows' and columns' radial factors are staged in LDS as (io0, io1) pairs.  Epilogue as the MFMA
+// atom the rows' and columns' radial factors and structure factors are brought to LDS by
+// DMA, the next atom's while the current one is evaluated (two stages).  Epilogue as the MFMA
 // tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
 #include "setup_dev.h"
+#include "mt_dma.h"
 
 namespace {
 
@@ -23,17 +25,68 @@ using namespace setup_dev;
 
 constexpr int kOT = 64;        // tile edge
 constexpr int kNLmax = 16;     // lmaxmat + 1 supported
-constexpr int kRS = kNLmax + 1;  // LDS row stride in (io0, io1) pairs: 34 doubles, conflict-free b128 reads
 constexpr int kThreads = 512;  // 8 waves; thread (tr, tc) owns rows tr + 16 i (i < 4), cols tc + 32 j (j < 2)
 
 struct d2 {
   double a, b;
 };
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void olp_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
-                                                       int nitems, int natm, int nl, GTab gt) {
-  __shared__ d2 rr[kOT][kRS], rc[kOT][kRS];
-  __shared__ cplx sr[kOT], sc[kOT];
+// Clenshaw coefficients of the Legendre recurrence, P_{l+1} = al(l) x P_l + bl(l+1) P_{l-1}:
+// al(l) = (2l+1)/(l+1), bl(l+1) = -(l+1)/(l+2) (correctly rounded at compile time, as the
+// division at run time would be); read with scalar loads instead of two FP64 divisions per l
+struct ClenshawTab {
+  double al[kNLmax], bl[kNLmax];
+};
+constexpr ClenshawTab make_clenshaw() {
+  ClenshawTab t{};
+  for (int l = 0; l < kNLmax; ++l) {
+    t.al[l] = (double)(2 * l + 1) / (double)(l + 1);
+    t.bl[l] = -(double)(l + 1) / (double)(l + 2);
+  }
+  return t;
+}
+__constant__ ClenshawTab kClenshaw = make_clenshaw();
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+// One atom's operands in LDS, filled by DMA in lane order: the rows' and the columns'
+// radial factors as (io0, io1) pairs, [row][l] with row stride nl pairs (for odd nl the
+// 16-lane reads of 16 rows fall on distinct banks), then their structure factors.
+struct __attribute__((aligned(16))) OlpStage {
+  d2 rr[kOT * kNLmax], rc[kOT * kNLmax];
+  cplx sr[kOT], sc[kOT];
+};
+
+// DMA of atom `at` into a stage: 2 nl + 2 wave-wide loads of 64 x 16 bytes, rows past N
+// clamped to row N-1 (their products are never stored)
+__device__ __forceinline__ void olp_issue(OlpStage *st, const kdesc &k, int at, int nl, int row0, int col0) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ngp = k.ngp;
+  const d2 *rfa = (const d2 *)(k.rf + (size_t)at * ngp * 2 * nl);
+  for (int j = wave; j < 2 * nl + 2; j += kThreads / 64) {
+    const void *src;
+    void *dst;
+    if (j < 2 * nl) {
+      const bool isr = j < nl;
+      const int e = (isr ? j : j - nl) * 64 + lane;       // element of the [64][nl] block
+      const int q = e / nl, l = e - q * nl;
+      const int idx = min((isr ? row0 : col0) + q, ngp - 1);
+      src = rfa + (size_t)idx * nl + l;
+      dst = (isr ? st->rr : st->rc) + (isr ? j : j - nl) * 64;
+    } else {
+      const bool isr = j == 2 * nl;
+      const int idx = min((isr ? row0 : col0) + lane, ngp - 1);
+      src = k.sfacgp + idx + (size_t)k.ldg * at;
+      dst = isr ? st->sr : st->sc;
+    }
+    __builtin_amdgcn_global_load_lds(src, (lds_void_t *)dst, 16, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void olp_kernel(
+    const kdesc *__restrict__ ks, const mt_item *__restrict__ items, int nitems, int natm, int nl, GTab gt) {
+  __shared__ OlpStage st0;
+  __shared__ OlpStage st1;
   __shared__ double ur[kOT][3], uc[kOT][3];
   __shared__ int gr[kOT][3], gc[kOT][3];
   const int item = blockIdx.x;
@@ -42,6 +95,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const kdesc k = ks[w.k];
   const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
   const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
+  olp_issue(&st0, k, 0, nl, row0, col0);
   for (int e = tid; e < 2 * kOT; e += kThreads) {
     const bool isr = e < kOT;
     const int q = isr ? e : e - kOT;
@@ -71,24 +125,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       x[i][j] = ur[a][0] * uc[b][0] + ur[a][1] * uc[b][1] + ur[a][2] * uc[b][2];
       are[i][j] = aim[i][j] = 0.0;
     }
-  for (int at = 0; at < natm; ++at) {
-    __syncthreads();                                  // the previous atom's factors are consumed
-    const d2 *rfa = (const d2 *)(k.rf + (size_t)at * ngp * 2 * nl);
-    for (int e = tid; e < 2 * kOT * nl; e += kThreads) {
-      const int q = e / nl, l = e - q * nl;
-      const bool isr = q < kOT;
-      const int idx = (isr ? row0 + q : col0 + q - kOT);
-      const d2 v = idx < ngp ? rfa[(size_t)idx * nl + l] : d2{0.0, 0.0};
-      if (isr) rr[q][l] = v; else rc[q - kOT][l] = v;
-    }
-    for (int e = tid; e < 2 * kOT; e += kThreads) {
-      const bool isr = e < kOT;
-      const int q = isr ? e : e - kOT;
-      const int idx = (isr ? row0 : col0) + q;
-      const cplx s = idx < ngp ? k.sfacgp[idx + (size_t)k.ldg * at] : cplx{0.0, 0.0};
-      if (isr) sr[q] = s; else sc[q] = s;
-    }
-    __syncthreads();
+  // one atom from stage cur while the next atom's DMA fills nxt
+  auto atom = [&](const OlpStage *cur, OlpStage *nxt, int at) {
+    wait_vm_lgkm0<0>();
+    stage_barrier();                                  // atom `at` landed; nxt is no longer read
+    if (at + 1 < natm) olp_issue(nxt, k, at + 1, nl, row0, col0);
     double b1[4][2], b2[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -96,17 +137,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       for (int j = 0; j < 2; ++j) b1[i][j] = b2[i][j] = 0.0;
 #pragma unroll 1
     for (int l = nl - 1; l >= 0; --l) {
-      const double al = (double)(2 * l + 1) / (double)(l + 1), bl = -(double)(l + 1) / (double)(l + 2);
-      d2 p[4], q[2];
+      const double al = kClenshaw.al[l], bl = kClenshaw.bl[l];
+      dv2 p[4], q[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) p[i] = rr[tr + 16 * i][l];
+      for (int i = 0; i < 4; ++i) p[i] = *reinterpret_cast<const dv2 *>(&cur->rr[(tr + 16 * i) * nl + l]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) q[j] = rc[tc + 32 * j][l];
+      for (int j = 0; j < 2; ++j) q[j] = *reinterpret_cast<const dv2 *>(&cur->rc[(tc + 32 * j) * nl + l]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const double a = p[i].a * q[j].a + p[i].b * q[j].b;
+          const double a = p[i].x * q[j].x + p[i].y * q[j].y;
           const double b0 = a + al * x[i][j] * b1[i][j] + bl * b2[i][j];
           b2[i][j] = b1[i][j];
           b1[i][j] = b0;
@@ -116,10 +157,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const cplx s = sr[tr + 16 * i], t = sc[tc + 32 * j];
+        const cplx s = cur->sr[tr + 16 * i], t = cur->sc[tc + 32 * j];
         are[i][j] += (s.re * t.re + s.im * t.im) * b1[i][j];     // conj(S(G)) S(G')
         aim[i][j] += (s.re * t.im - s.im * t.re) * b1[i][j];
       }
+  };
+  for (int at = 0; at < natm; at += 2) {
+    atom(&st0, &st1, at);
+    if (at + 1 < natm) atom(&st1, &st0, at + 1);
   }
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
