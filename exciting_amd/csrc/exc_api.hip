@@ -8,6 +8,8 @@
 #include <thread>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "exc_common.h"
 
 struct exc_ctx {
@@ -60,6 +62,13 @@ struct exc_ctx {
   int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
   hipGraph_t graph;
   hipGraphExec_t graph_exec;
+  // generalized eigensolve (rocSOLVER on the context's stream)
+  rocblas_handle blas;
+  double *eig_w;            // all n eigenvalue slots rocSOLVER writes
+  size_t eig_w_bytes;
+  cplx *eig_z;              // n x n eigenvector workspace (rocSOLVER may use all n columns)
+  size_t eig_z_bytes;
+  int *eig_int;             // nev found, info (device)
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
   hipEvent_t ev_fork, ev_join;
@@ -259,6 +268,10 @@ int exc_ctx_destroy(exc_ctx *c) {
   else hipDeviceSynchronize();
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->eig_w) hipFree(c->eig_w);
+  if (c->eig_int) hipFree(c->eig_int);
+  if (c->eig_z) hipFree(c->eig_z);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -286,6 +299,79 @@ int exc_ctx_destroy(exc_ctx *c) {
 int exc_ctx_set_stream(exc_ctx *c, void *stream) {
   if (!c) return EXC_ERR_ARG;
   c->stream = (hipStream_t)stream;
+  return EXC_OK;
+}
+
+int exc_eigen_dev(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec,
+                  int ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nmat < 0 || ldh < (nmat > 1 ? nmat : 1) || nev < 0 || nev > nmat || ldv < (nmat > 1 ? nmat : 1)) return EXC_ERR_ARG;
+  if (nmat == 0 || nev == 0) return EXC_OK;
+  if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if (!c->blas) {
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+      c->blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+  }
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) return EXC_ERR_HIP;
+  // W has n slots and Z n columns: rocSOLVER's divide-and-conquer path uses the columns past
+  // nev as workspace, so the caller's nev columns are filled from an internal n x n buffer
+  if ((rc = grow((void **)&c->eig_w, &c->eig_w_bytes, sizeof(double) * nmat))) return rc;
+  if ((rc = grow((void **)&c->eig_z, &c->eig_z_bytes, sizeof(cplx) * (size_t)nmat * nmat))) return rc;
+  if (!c->eig_int) EXC_CHECK(hipMalloc((void **)&c->eig_int, 2 * sizeof(int)));
+  rocblas_status st;
+  {
+    StageTimer t(c, EXC_STAGE_EIGEN);
+    st = rocsolver_zhegvdx(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_erange_index,
+                           rocblas_fill_upper, nmat, (rocblas_double_complex *)h, ldh, (rocblas_double_complex *)o,
+                           ldh, 0.0, 0.0, 1, nev, c->eig_int, c->eig_w, (rocblas_double_complex *)c->eig_z, nmat,
+                           c->eig_int + 1);
+  }
+  if (st != rocblas_status_success) return EXC_ERR_HIP;
+  EXC_CHECK(hipMemcpyAsync(eval, c->eig_w, sizeof(double) * nev, hipMemcpyDeviceToDevice, c->stream));
+  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, c->eig_z, 16 * (size_t)nmat, 16 * (size_t)nmat, nev,
+                             hipMemcpyDeviceToDevice, c->stream));
+  int hi[2] = {0, 0};
+  EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  if (hi[1] != 0) return hi[1];
+  return hi[0] == nev ? EXC_OK : EXC_ERR_HIP;
+}
+
+int exc_eigen(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec, int ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nmat < 0 || ldh < (nmat > 1 ? nmat : 1) || nev < 0 || nev > nmat || ldv < (nmat > 1 ? nmat : 1)) return EXC_ERR_ARG;
+  if (nmat == 0 || nev == 0) return EXC_OK;
+  if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t b_m = 16 * (size_t)ldh * nmat, b_v = 16 * (size_t)ldv * nev, b_e = 8 * (size_t)nev;
+  const size_t o_o = (b_m + 255) & ~(size_t)255, o_v = o_o + ((b_m + 255) & ~(size_t)255);
+  const size_t o_e = o_v + ((b_v + 255) & ~(size_t)255);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_e + b_e))) return rc;
+  if ((rc = grow_host(&c->hstage, &c->hstage_bytes, std::max(2 * b_m, b_v)))) return rc;
+  char *base = (char *)c->stage, *hs = (char *)c->hstage;
+  const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
+  copy_cols(hs, pitch, (const char *)h, pitch, width, nmat);
+  copy_cols(hs + b_m, pitch, (const char *)o, pitch, width, nmat);
+  EXC_CHECK(hipMemcpyAsync(base, hs, b_m, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_o, hs + b_m, b_m, hipMemcpyHostToDevice, c->stream));
+  rc = exc_eigen_dev(c, nmat, ldh, (double *)base, (double *)(base + o_o), nev, (double *)(base + o_e),
+                     (double *)(base + o_v), ldv);
+  if (rc) return rc;
+  // H and O come back overwritten as ZHEGVX leaves them (O: its Cholesky factor)
+  EXC_CHECK(hipMemcpyAsync(hs, base, b_m, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(hs + b_m, base + o_o, b_m, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  copy_cols((char *)h, pitch, hs, pitch, width, nmat);
+  copy_cols((char *)o, pitch, hs + b_m, pitch, width, nmat);
+  EXC_CHECK(hipMemcpyAsync(hs, base + o_v, b_v, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(eval, base + o_e, b_e, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  copy_cols((char *)evec, 16 * (size_t)ldv, hs, 16 * (size_t)ldv, width, nev);
   return EXC_OK;
 }
 

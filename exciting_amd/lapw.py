@@ -77,10 +77,13 @@ SIGNATURES = {
     "exc_ctx_num_variants": [_I],
     "exc_setup_batched_dev": [_P, _I, _P, _I],
     "exc_ho_vjp_dev": [_P, _I, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
+    "exc_ho_vjp": [_P, _I, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
+    "exc_eigen_dev": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
+    "exc_eigen": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -240,6 +243,25 @@ class Context:
         self._call("exc_hamiltonandoverlapsetup", self.h, kp.ngp, ngkmax, _ptr(apw), _ptr(ig), _ptr(vg), ldh,
                    _ptr(H), _ptr(O))
         return H, O
+
+    def eigen(self, H, O, nev):
+        """Lowest nev eigenpairs of H c = e O c (solvewithlapack, ZHEGVX range 'I') from host
+        arrays; H and O are overwritten as ZHEGVX leaves them.  Returns (eval, evec(nmat, nev))."""
+        nmat = H.shape[1]
+        ldh = H.shape[0]
+        for m in (H, O):
+            if m.shape != (ldh, nmat) or m.dtype != np.complex128 or not m.flags.f_contiguous:
+                raise ValueError("eigen: H, O must be Fortran-ordered complex128 (ldh, nmat)")
+        ev = np.zeros(nev)
+        vec = np.zeros((nmat, nev), dtype=np.complex128, order="F")
+        self._call("exc_eigen", self.h, nmat, ldh, _ptr(H), _ptr(O), nev, _ptr(ev), _ptr(vec), nmat,
+                   ok_positive=False)
+        return ev, vec
+
+    def eigen_dev(self, nmat, ldh, h, o, nev, eval_, evec, ldv):
+        """Device version: torch tensors h, o (stored as (nmat, ldh) row-major = Fortran (ldh, nmat)),
+        eval_(nev) float64, evec (nev, ldv) complex128."""
+        return self._call("exc_eigen_dev", self.h, nmat, ldh, _ptr(h), _ptr(o), nev, _ptr(eval_), _ptr(evec), ldv)
 
     # ------------------------------------------------------------------ device entry points
     def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):

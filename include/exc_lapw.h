@@ -175,6 +175,19 @@ int exc_ho_vjp(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const in
                const double *vgpc, const double *sfacgp, int nst, int ldc, const double *evec,
                const double *wh, const double *wo, double *dedtau, double *dedv);
 
+/* Generalized Hermitian eigenproblem of one k-point after the setup, the counterpart of
+ * solvewithlapack (modfvsystem.f90:585-720), which calls ZHEGVX(itype 1, 'V', range 'I',
+ * 'U', il = 1, iu = nev) (:691-693): H c = e O c for the lowest nev eigenpairs.
+ * h, o: ldh x nmat (upper triangles used; both overwritten, O by its Cholesky factor);
+ * eval(nev) ascending, evec(ldv, nev) O-orthonormal (ldv >= nmat).  Returns 0, or LAPACK's
+ * info > 0 (nmat + i: the leading minor of order i of O is not positive definite; i <= nmat:
+ * i eigenvectors did not converge), or < 0 on an argument/HIP error.
+ * exc_eigen_dev: device arrays, on the context's stream.  exc_eigen: host arrays. */
+int exc_eigen_dev(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, double *eval,
+                  double *evec, int ldv);
+int exc_eigen(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, double *eval,
+              double *evec, int ldv);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
@@ -195,6 +208,7 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_BSPLIT 7   /* split-BF16 muffin-tin stage of exc_setup_batched_dev          */
 #define EXC_STAGE_VJP 8      /* exc_ho_vjp_dev                                                */
 #define EXC_STAGE_OLP 9      /* addition-theorem overlap (EXC_BATCH_OLP_LEGENDRE)             */
+#define EXC_STAGE_EIGEN 10   /* exc_eigen_dev                                                 */
 #define EXC_NSTAGE 12
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
