@@ -143,6 +143,53 @@ def run_split(args, ctx, batch, cr, w, fp64_value, log, olpflag=0):
             "stages_ms": {k: v for k, (v, c) in eager.items() if c}, "variants": tuned}
 
 
+def run_eigen(args, ctx, batch, w, olpflag, nev, log):
+    """Setup + generalized eigensolve of every k on the device (the seceqnfv sequence), H and O
+    consumed in place; the CPU reference is LAPACK zhegvx (scipy) on the same H, O."""
+    import scipy.linalg
+    import torch
+    from exciting_amd import lapw
+    flags = lapw.EXC_BATCH_MATCH | olpflag | lapw.EXC_BATCH_GRAPH
+    evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in range(batch.nk)]
+    vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in batch.nmat]
+
+    def step(nk=batch.nk):
+        ctx.setup_batched_dev(batch, flags)
+        ctx.eigen_batched_dev(batch.nmat[:nk], batch.H[:nk], batch.O[:nk], nev, evs[:nk], vecs[:nk])
+
+    tuned = None
+    if args.tune:
+        from exciting_amd import autotune
+        tuned = autotune.tune(ctx, ["eigen"], lambda: step(min(2, batch.nk)), reps=1, log=log)
+    step()
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_timing(False)
+    tim = ctx.read_timing()
+    ms_eig, cnt = tim["eigen"]
+    value = batch.nk * args.steps / dt
+    # host LAPACK on the first k (bounded: one solve)
+    ctx.setup_batched_dev(batch, flags)
+    torch.cuda.synchronize()
+    H = batch.H[0].cpu().numpy().T.copy()
+    O = batch.O[0].cpu().numpy().T.copy()
+    t1 = time.perf_counter()
+    scipy.linalg.eigh(H, O, lower=False, subset_by_index=[0, nev - 1], driver="gvx")
+    cpu_ms = 1000.0 * (time.perf_counter() - t1)
+    ms_eig /= batch.nk     # one timed launch covers all k-points
+    log(f"  setup + eigensolve (lowest {nev}): {value:.1f} k-points/s, eigensolve {ms_eig / max(cnt, 1):.2f} ms/k "
+        f"(host LAPACK zhegvx {cpu_ms:.0f} ms/k)")
+    return {"value": value, "unit": "k-points/s", "nev": nev, "ms_per_step": 1000.0 * dt / args.steps,
+            "eigensolve_ms_per_k": ms_eig / max(cnt, 1), "solver": "rocSOLVER, k-points on 4 concurrent streams (device, after the setup)",
+            "variants": tuned, "cpu_lapack_zhegvx_ms_per_k": cpu_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +202,9 @@ def main():
     ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
                     help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 measurement")
+    ap.add_argument("--eigen", type=int, default=200, metavar="NEV",
+                    help="grouped path: also time setup + device eigensolve of the lowest NEV states "
+                         "(0: skip)")
     ap.add_argument("--no-olp", action="store_true",
                     help="grouped path: overlap muffin-tin part by the MFMA contraction instead of the "
                          "addition theorem (EXC_BATCH_OLP_LEGENDRE)")
@@ -273,7 +323,7 @@ def main():
     achieved = fl / (mt_ms * 1e-3) if mt_ms > 0 else 0.0
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64, "traffic": None,
-            "kernel": ("bmt_dma_kernel (grouped, all k)" if args.path == "batched" else "mt_dma_kernel (per k)")
+            "kernel": ("bmt_dma_grid_kernel (grouped, all k)" if args.path == "batched" else "mt_dma_kernel (per k)")
                       + (": A^H.Y + interstitial (H)" if olpflag else ": A^H.Y and A^H.A + interstitial"),
             "per_launch_flops": fl / max(mt_cnt, 1)}
     # HBM traffic of the dominant kernel: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this same
@@ -281,7 +331,7 @@ def main():
     prof = HERE / "profiles" / "setup_kernels_workload1_16k.json"
     if (args.path == "batched" and args.config == 1 and args.nk is None and not args.tune and olpflag
             and prof.exists()):
-        rec = json.loads(prof.read_text()).get("bmt_dma_kernel")
+        rec = json.loads(prof.read_text()).get("bmt_dma_grid_kernel")
         if rec:
             roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
             roof["traffic_source"] = str(prof.relative_to(HERE))
@@ -294,6 +344,9 @@ def main():
     split = None
     if args.path == "batched" and not args.no_split:
         split = run_split(args, ctx, batch, cr, w, value, log, olpflag)
+    eig = None
+    if args.path == "batched" and args.eigen > 0:
+        eig = run_eigen(args, ctx, batch, w, olpflag, min(args.eigen, min(batch.nmat)), log)
 
     base = None
     if not args.no_cpu_baseline:
@@ -308,7 +361,8 @@ def main():
                             if args.path == "batched" else "FP64, per-k launches"
                             + (", host arrays (PCIe copies included)" if args.path == "host" else "")),
                    "parallelism": "single GPU", "variants": tuned},
-        "roofline": roof, "stages": stages, "split_bf16": split, "cpu_baseline": base,
+        "roofline": roof, "stages": stages, "split_bf16": split, "setup_and_eigensolve": eig,
+        "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
 

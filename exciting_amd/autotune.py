@@ -17,7 +17,7 @@ def bucket(n: int) -> int:
     return ((n + 255) // 256) * 256
 
 
-def sweep(ctx, stage: str, run: Callable[[], None], reps: int = 3) -> Dict[int, float]:
+def sweep(ctx, stage: str, run: Callable[[], None], reps: int = 3, log=None) -> Dict[int, float]:
     """ms per launch of every variant of `stage` for the work `run()` launches."""
     import torch
     out = {}
@@ -32,6 +32,8 @@ def sweep(ctx, stage: str, run: Callable[[], None], reps: int = 3) -> Dict[int, 
         ctx.set_timing(False)
         ms, cnt = ctx.read_timing()[stage]
         out[v] = ms / max(cnt, 1)
+        if log:
+            log(f"    {stage} v{v}: {out[v]:.3f} ms")
     return out
 
 
@@ -39,7 +41,7 @@ def tune(ctx, stages, run: Callable[[], None], reps: int = 3, log=None) -> Dict[
     """Sweep each stage in turn (others at their current setting) and pin the fastest variant."""
     chosen = {}
     for st in stages:
-        res = sweep(ctx, st, run, reps)
+        res = sweep(ctx, st, run, reps, log)
         best = min(res, key=res.get)
         ctx.set_variant(st, best)
         chosen[st] = best

@@ -62,13 +62,25 @@ struct exc_ctx {
   int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
   hipGraph_t graph;
   hipGraphExec_t graph_exec;
-  // generalized eigensolve (rocSOLVER on the context's stream)
-  rocblas_handle blas;
-  double *eig_w;            // all n eigenvalue slots rocSOLVER writes
-  size_t eig_w_bytes;
-  cplx *eig_z;              // n x n eigenvector workspace (rocSOLVER may use all n columns)
-  size_t eig_z_bytes;
-  int *eig_int;             // nev found, info (device)
+  // generalized eigensolve: worker 0 runs on the context's stream (exc_eigen_dev), workers
+  // 1..kEigStreams on streams of their own (exc_eigen_batched_dev)
+  struct EigWorker {
+    hipStream_t s;
+    rocblas_handle blas;
+    double *w;              // all n eigenvalue slots rocSOLVER writes
+    size_t wb;
+    cplx *z;                // n x n eigenvector workspace (rocSOLVER may use all n columns)
+    size_t zb;
+    double *e;              // n-length workspace (tridiagonal off-diagonal, ifail)
+    size_t eb;
+    hipEvent_t done;
+  };
+  static constexpr int kEigStreams = 4;
+  EigWorker ew[kEigStreams + 1];
+  int eig_variant;          // -1: built-in choice
+  int *eig_int;             // per solve: nev found, info (device), 2 per k-point
+  size_t eig_int_bytes;
+  hipEvent_t eig_fork;
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
   hipEvent_t ev_fork, ev_join;
@@ -253,6 +265,7 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->y_variant = -1;
   c->bmt_variant = -1;
   c->bsplit_variant = -1;
+  c->eig_variant = -1;
   {
     hipDeviceProp_t prop;
     c->ncu = (hipGetDeviceProperties(&prop, c->device) == hipSuccess) ? prop.multiProcessorCount : 256;
@@ -268,10 +281,17 @@ int exc_ctx_destroy(exc_ctx *c) {
   else hipDeviceSynchronize();
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
-  if (c->blas) rocblas_destroy_handle(c->blas);
-  if (c->eig_w) hipFree(c->eig_w);
+  for (int i = 0; i <= exc_ctx::kEigStreams; ++i) {
+    exc_ctx::EigWorker &w = c->ew[i];
+    if (w.blas) rocblas_destroy_handle(w.blas);
+    if (w.w) hipFree(w.w);
+    if (w.z) hipFree(w.z);
+    if (w.e) hipFree(w.e);
+    if (w.done) hipEventDestroy(w.done);
+    if (i > 0 && w.s) hipStreamDestroy(w.s);
+  }
   if (c->eig_int) hipFree(c->eig_int);
-  if (c->eig_z) hipFree(c->eig_z);
+  if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -302,43 +322,136 @@ int exc_ctx_set_stream(exc_ctx *c, void *stream) {
   return EXC_OK;
 }
 
+namespace {
+
+bool eig_args_ok(int nmat, int ldh, int nev, int ldv) {
+  return nmat >= 0 && ldh >= (nmat > 1 ? nmat : 1) && nev >= 0 && nev <= nmat && ldv >= (nmat > 1 ? nmat : 1);
+}
+
+// Enqueue one generalized eigensolve and the copies of its first nev eigenpairs on the
+// worker's stream; d_ints receives (nev found, info).  Solver variants (all: Cholesky of O,
+// reduction to standard form, then)
+//  0 zhegvdx: tridiagonal divide and conquer, lowest nev     1 zhegvx: bisection + inverse iteration
+//  2 zhegvd: divide and conquer, all (default)               3 zhegv: implicit QL/QR, all
+// (rocSOLVER's Jacobi solver, zhegvj, misses the 1e-10 Ha eigenvalue bar on these spectra)
+int eig_enqueue(exc_ctx::EigWorker &W, int v, int nmat, int ldh, double *h, double *o, int nev, double *eval,
+                double *evec, int ldv, int *d_ints) {
+  if (!W.blas) {
+    if (rocblas_create_handle(&W.blas) != rocblas_status_success) {
+      W.blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+  // W has n slots and Z n columns: rocSOLVER's divide-and-conquer path uses the columns past
+  // nev as workspace, so the caller's nev columns are filled from an internal n x n buffer
+  int rc;
+  if ((rc = grow((void **)&W.w, &W.wb, sizeof(double) * nmat))) return rc;
+  if ((rc = grow((void **)&W.z, &W.zb, sizeof(cplx) * (size_t)nmat * nmat))) return rc;
+  if ((rc = grow((void **)&W.e, &W.eb, sizeof(double) * (nmat + 1)))) return rc;
+  const auto itype = rocblas_eform_ax;
+  const auto ev = rocblas_evect_original;
+  const auto up = rocblas_fill_upper;
+  rocblas_double_complex *A = (rocblas_double_complex *)h, *B = (rocblas_double_complex *)o;
+  rocblas_double_complex *Z = (rocblas_double_complex *)W.z;
+  rocblas_status st;
+  bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z
+  switch (v) {
+    case 0:
+      st = rocsolver_zhegvdx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev,
+                             d_ints, W.w, Z, nmat, d_ints + 1);
+      break;
+    case 1:
+      st = rocsolver_zhegvx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev, 0.0,
+                            d_ints, W.w, Z, nmat, (rocblas_int *)W.e, d_ints + 1);
+      break;
+    case 2:
+      st = rocsolver_zhegvd(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+      in_a = true;
+      break;
+    default:
+      st = rocsolver_zhegv(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+      in_a = true;
+      break;
+  }
+  if (st != rocblas_status_success) return EXC_ERR_HIP;
+  EXC_CHECK(hipMemcpyAsync(eval, W.w, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
+  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, in_a ? (const void *)A : (const void *)Z,
+                             16 * (size_t)(in_a ? ldh : nmat), 16 * (size_t)nmat, nev, hipMemcpyDeviceToDevice, W.s));
+  return EXC_OK;
+}
+
+// (nev found, info) of a solve; the full-spectrum solvers (v >= 2) report no count
+int eig_status(const int *hi, int nev, int v) {
+  if (hi[1] != 0) return hi[1];
+  return (v >= 2 || hi[0] == nev) ? EXC_OK : EXC_ERR_HIP;
+}
+
+}  // namespace
+
 int exc_eigen_dev(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec,
                   int ldv) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
-  if (nmat < 0 || ldh < (nmat > 1 ? nmat : 1) || nev < 0 || nev > nmat || ldv < (nmat > 1 ? nmat : 1)) return EXC_ERR_ARG;
+  if (!eig_args_ok(nmat, ldh, nev, ldv)) return EXC_ERR_ARG;
   if (nmat == 0 || nev == 0) return EXC_OK;
   if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
-  if (!c->blas) {
-    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
-      c->blas = nullptr;
-      return EXC_ERR_HIP;
-    }
-  }
-  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) return EXC_ERR_HIP;
-  // W has n slots and Z n columns: rocSOLVER's divide-and-conquer path uses the columns past
-  // nev as workspace, so the caller's nev columns are filled from an internal n x n buffer
-  if ((rc = grow((void **)&c->eig_w, &c->eig_w_bytes, sizeof(double) * nmat))) return rc;
-  if ((rc = grow((void **)&c->eig_z, &c->eig_z_bytes, sizeof(cplx) * (size_t)nmat * nmat))) return rc;
-  if (!c->eig_int) EXC_CHECK(hipMalloc((void **)&c->eig_int, 2 * sizeof(int)));
-  rocblas_status st;
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 2 * sizeof(int)))) return rc;
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 2;   // zhegvd: fastest on MI355X at nmat ~ 1200
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
-    st = rocsolver_zhegvdx(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_erange_index,
-                           rocblas_fill_upper, nmat, (rocblas_double_complex *)h, ldh, (rocblas_double_complex *)o,
-                           ldh, 0.0, 0.0, 1, nev, c->eig_int, c->eig_w, (rocblas_double_complex *)c->eig_z, nmat,
-                           c->eig_int + 1);
+    if ((rc = eig_enqueue(W, v, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
   }
-  if (st != rocblas_status_success) return EXC_ERR_HIP;
-  EXC_CHECK(hipMemcpyAsync(eval, c->eig_w, sizeof(double) * nev, hipMemcpyDeviceToDevice, c->stream));
-  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, c->eig_z, 16 * (size_t)nmat, 16 * (size_t)nmat, nev,
-                             hipMemcpyDeviceToDevice, c->stream));
   int hi[2] = {0, 0};
   EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
-  if (hi[1] != 0) return hi[1];
-  return hi[0] == nev ? EXC_OK : EXC_ERR_HIP;
+  return eig_status(hi, nev, v);
+}
+
+int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o,
+                          int nev, double *const *eval, double *const *evec, const int *ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nk < 0 || (nk > 0 && (!nmat || !ldh || !h || !o || !eval || !evec || !ldv))) return EXC_ERR_ARG;
+  for (int k = 0; k < nk; ++k) {
+    if (!eig_args_ok(nmat[k], ldh[k], nev, ldv[k])) return EXC_ERR_ARG;
+    if (nmat[k] > 0 && nev > 0 && (!h[k] || !o[k] || !eval[k] || !evec[k])) return EXC_ERR_ARG;
+  }
+  if (nk == 0 || nev == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 2 * sizeof(int) * nk))) return rc;
+  if (!c->eig_fork) EXC_CHECK(hipEventCreateWithFlags(&c->eig_fork, hipEventDisableTiming));
+  const int nw = nk < exc_ctx::kEigStreams ? nk : exc_ctx::kEigStreams;
+  for (int i = 1; i <= nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[i];
+    if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
+    if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
+  }
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 2;
+  {
+    StageTimer t(c, EXC_STAGE_EIGEN);
+    EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
+    for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
+    for (int k = 0; k < nk; ++k) {
+      if (nmat[k] == 0) continue;
+      exc_ctx::EigWorker &W = c->ew[1 + k % nw];
+      if ((rc = eig_enqueue(W, v, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k], c->eig_int + 2 * k)))
+        return rc;
+    }
+    for (int i = 1; i <= nw; ++i) {
+      EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
+      EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
+    }
+  }
+  std::vector<int> hi(2 * (size_t)nk, 0);
+  EXC_CHECK(hipMemcpyAsync(hi.data(), c->eig_int, sizeof(int) * 2 * nk, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nk; ++k)
+    if (nmat[k] > 0 && (rc = eig_status(&hi[2 * (size_t)k], nev, v))) return rc;
+  return EXC_OK;
 }
 
 int exc_eigen(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec, int ldv) {
@@ -396,6 +509,10 @@ int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
       if (variant < -1 || variant >= kBsplitVariants) return EXC_ERR_ARG;
       c->bsplit_variant = variant;
       return EXC_OK;
+    case EXC_STAGE_EIGEN:
+      if (variant < -1 || variant >= kEigenVariants) return EXC_ERR_ARG;
+      c->eig_variant = variant;
+      return EXC_OK;
     default:
       return EXC_ERR_ARG;
   }
@@ -407,6 +524,7 @@ int exc_ctx_num_variants(int stage) {
     case EXC_STAGE_MT: return kMtVariants;
     case EXC_STAGE_BMT: return kBmtVariants;
     case EXC_STAGE_BSPLIT: return kBsplitVariants;
+    case EXC_STAGE_EIGEN: return kEigenVariants;
     default: return 0;
   }
 }

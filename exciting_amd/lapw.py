@@ -80,6 +80,7 @@ SIGNATURES = {
     "exc_ho_vjp": [_P, _I, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
     "exc_eigen_dev": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
     "exc_eigen": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
+    "exc_eigen_batched_dev": [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
@@ -262,6 +263,15 @@ class Context:
         """Device version: torch tensors h, o (stored as (nmat, ldh) row-major = Fortran (ldh, nmat)),
         eval_(nev) float64, evec (nev, ldv) complex128."""
         return self._call("exc_eigen_dev", self.h, nmat, ldh, _ptr(h), _ptr(o), nev, _ptr(eval_), _ptr(evec), ldv)
+
+    def eigen_batched_dev(self, nmats, hs, os_, nev, evals, evecs):
+        """exc_eigen_batched_dev over lists of torch tensors (per k as for eigen_dev, ld = nmat)."""
+        nk = len(nmats)
+        ints = (C.c_int * nk)(*nmats)
+        ptrs = [(C.c_void_p * nk)(*[_ptr(t) for t in lst]) for lst in (hs, os_, evals, evecs)]
+        return self._call("exc_eigen_batched_dev", self.h, nk, C.cast(ints, _P), C.cast(ints, _P),
+                          C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
+                          C.cast(ints, _P))
 
     # ------------------------------------------------------------------ device entry points
     def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):

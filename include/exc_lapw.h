@@ -187,6 +187,14 @@ int exc_eigen_dev(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev
                   double *evec, int ldv);
 int exc_eigen(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, double *eval,
               double *evec, int ldv);
+/* The lowest nev eigenpairs of nk k-points at once (device arrays; per k: nmat[k], ldh[k],
+ * h[k], o[k], eval[k], evec[k], ldv[k] as for exc_eigen_dev, the pointer arrays themselves
+ * in host memory).  The solves run concurrently on worker streams forked from and joined
+ * back into the context's stream (the reference solves the k-points one after another in
+ * the loop of scf_cycle.f90:241-276).  Returns 0, or the first failing k-point's status. */
+int exc_eigen_batched_dev(exc_ctx *ctx, int nk, const int *nmat, const int *ldh, double *const *h,
+                          double *const *o, int nev, double *const *eval, double *const *evec,
+                          const int *ldv);
 
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);

@@ -78,3 +78,48 @@ def test_eigen_reports_indefinite_overlap():
     with pytest.raises(lapw.ExcError) as ei:
         ctx.eigen(H, O, 3)
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
+
+
+@pytest.mark.parametrize("variant", range(4))
+def test_eigen_solver_variants(variant):
+    """Every rocSOLVER path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv) to the same bars."""
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 1)
+    ctx = lapw.Context(w.crystal)
+    ctx.set_variant("eigen", variant)
+    kp = w.kpoints[0]
+    H_ref, O_ref, _ = R.setup(w.crystal, kp)
+    nev = 200
+    e_ref, _ = R.eigen(H_ref, O_ref, nev)
+    e, C = ctx.eigen(np.asfortranarray(H_ref.copy()), np.asfortranarray(O_ref.copy()), nev)
+    _check(H_ref, O_ref, e, C, e_ref, f"solver variant {variant}")
+
+
+def test_eigen_batched_equals_single():
+    """The multi-k entry (concurrent worker streams) gives the single-k results bit for bit."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 6)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    nev = 200
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE
+    res = {}
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        for mode in ("single", "batched"):
+            ctx.setup_batched_dev(b, flags)
+            evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+            vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in b.nmat]
+            if mode == "single":
+                for i in range(b.nk):
+                    ctx.eigen_dev(b.nmat[i], b.nmat[i], b.H[i], b.O[i], nev, evs[i], vecs[i], b.nmat[i])
+            else:
+                ctx.eigen_batched_dev(b.nmat, b.H, b.O, nev, evs, vecs)
+            torch.cuda.synchronize()
+            res[mode] = ([e.cpu() for e in evs], [v.cpu() for v in vecs])
+            print(f"eigen {mode}: {b.nk} k-points", flush=True)
+    for i in range(len(w.kpoints)):
+        assert torch.equal(res["single"][0][i], res["batched"][0][i])
+        assert torch.equal(res["single"][1][i], res["batched"][1][i])
