@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:fc1730b32ab5e370
+﻿!mod$ v1 sum:f71f194d1991c68b
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module exc_lapw_c
 use,intrinsic::iso_c_binding,only:c_associated
@@ -209,6 +209,21 @@ real(8),intent(in)::wo(1_8:*)
 real(8),intent(inout)::dedtau(1_8:*)
 type(c_ptr),value::dedv
 integer(4)::exc_ho_vjp
+end
+end interface
+interface
+function exc_eigen(ctx,nmat,ldh,h,o,nev,eval,evec,ldv) bind(c,name="exc_eigen")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(4),value::nmat
+integer(4),value::ldh
+complex(8),intent(inout)::h(1_8:*)
+complex(8),intent(inout)::o(1_8:*)
+integer(4),value::nev
+real(8),intent(inout)::eval(1_8:*)
+complex(8),intent(inout)::evec(1_8:*)
+integer(4),value::ldv
+integer(4)::exc_eigen
 end
 end interface
 interface

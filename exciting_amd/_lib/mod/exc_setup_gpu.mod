@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:c5940eccc3ea7dcc
-!need$ fc1730b32ab5e370 n exc_lapw_c
+﻿!mod$ v1 sum:afc1cc0deae3bf01
+!need$ f71f194d1991c68b n exc_lapw_c
 !need$ 4e11352129309ed7 n exc_evsystem
 !need$ bb381bf46e508468 i __fortran_builtins
 !need$ 0bde2ac47243ead2 i iso_c_binding
@@ -95,6 +95,7 @@ use exc_lapw_c,only:exc_ctx_set_gvec
 use exc_lapw_c,only:exc_match
 use exc_lapw_c,only:exc_hamiltonandoverlapsetup
 use exc_lapw_c,only:exc_ho_vjp
+use exc_lapw_c,only:exc_eigen
 use exc_lapw_c,only:exc_ctx_nlotot
 use exc_evsystem,only:hermitianmatrix
 use exc_evsystem,only:null
@@ -193,6 +194,7 @@ private::exc_ctx_set_gvec
 private::exc_match
 private::exc_hamiltonandoverlapsetup
 private::exc_ho_vjp
+private::exc_eigen
 private::exc_ctx_nlotot
 private::hermitianmatrix
 private::null
@@ -263,5 +265,11 @@ real(8),intent(in)::evalfv(1_8:int(nstfv,kind=8))
 real(8),intent(in)::occ(1_8:int(nstfv,kind=8))
 real(8),intent(in)::wkpt
 real(8),intent(inout)::forceibs(1_8:3_8,1_8:int(natmtot_,kind=8))
+end
+subroutine solvewithgpu(system,nstfv,evecfv,evalfv)
+type(evsystem),intent(inout)::system
+integer(4),intent(in)::nstfv
+complex(8),intent(out)::evecfv(:,:)
+real(8),intent(out)::evalfv(1_8:int(nstfv,kind=8))
 end
 end

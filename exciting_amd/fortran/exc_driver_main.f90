@@ -1,8 +1,10 @@
 ! Test program for the Fortran driver: reads one k-point's inputs from a stream file,
 ! runs match and hamiltonandoverlapsetup through the GPU driver as seceqn.f90:57 /
 ! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
-! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot).
-!   exc_driver_main <input.bin> <output.bin> [<eigen.bin>]
+! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot); with a fourth
+! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
+! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).
+!   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev]
 program exc_driver_main
   use, intrinsic :: iso_c_binding
   use exc_lapw_c, only: exc_dims
@@ -15,16 +17,22 @@ program exc_driver_main
   real(8), allocatable :: rmt(:), atposc(:), dmat(:), oalo(:), ololo(:), ffacg(:), gpc(:), tpgpc(:, :), vgpc(:, :)
   complex(8), allocatable :: haa(:), loa(:), lolo(:), cfunig(:), veffig(:), sfacgp(:, :), apwalm(:, :, :, :)
   type(evsystem) :: system
-  character(len=512) :: fin, fout, feig
-  integer :: hdr(18), nst, u2
+  character(len=512) :: fin, fout, feig, fnev
+  integer :: hdr(18), nst, u2, nev_solve
   complex(8), allocatable :: evec(:, :)
-  real(8), allocatable :: evalfv(:), occ(:), forceibs(:, :)
+  real(8), allocatable :: evalfv(:), occ(:), forceibs(:, :), evs(:)
+  complex(8), allocatable :: vecs(:, :)
   real(8) :: wkpt
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   feig = ""
   if (command_argument_count() >= 3) call get_command_argument(3, feig)
+  nev_solve = 0
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, fnev)
+    read(fnev, *) nev_solve
+  end if
   open(newunit=u, file=trim(fin), access="stream", form="unformatted", status="old")
   read(u) hdr
   dims%natmtot = hdr(1); dims%nspecies = hdr(2); dims%lmaxapw = hdr(3); dims%lmaxmat = hdr(4)
@@ -64,7 +72,7 @@ program exc_driver_main
   write(u) apwalm
   write(u) system%hamilton%za
   write(u) system%overlap%za
-  if (len_trim(feig) > 0) then
+  if (len_trim(feig) > 0 .and. trim(feig) /= "-") then
     open(newunit=u2, file=trim(feig), access="stream", form="unformatted", status="old")
     read(u2) nst, wkpt
     allocate(evec(nmat, nst), evalfv(nst), occ(nst), forceibs(3, dims%natmtot))
@@ -73,6 +81,12 @@ program exc_driver_main
     forceibs = 0.d0
     call forcek_gpu(ngp, apwalm, igpig, vgpc, sfacgp, nmat, nst, evec, evalfv, occ, wkpt, forceibs)
     write(u) forceibs
+  end if
+  if (nev_solve > 0) then
+    allocate(evs(nev_solve), vecs(nmat, nev_solve))
+    call solvewithgpu(system, nev_solve, vecs, evs)
+    write(u) evs
+    write(u) vecs
   end if
   close(u)
   call deletesystem(system)

@@ -92,6 +92,14 @@ module exc_lapw_c
       type(c_ptr), value :: dedv          ! c_loc of a complex(8) dedv(ngvec), or c_null_ptr
     end function
 
+    integer(c_int) function exc_eigen(ctx, nmat, ldh, h, o, nev, eval, evec, ldv) bind(c, name="exc_eigen")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nmat, ldh, nev, ldv
+      complex(c_double_complex), intent(inout) :: h(*), o(*), evec(*)
+      real(c_double), intent(inout) :: eval(*)
+    end function
+
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

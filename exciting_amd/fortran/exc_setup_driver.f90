@@ -2,6 +2,7 @@
 !
 !   match(ngp, gpc, tpgpc, sfacgp, apwalm)                        match.f90:13
 !   hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)     hamiltonandoverlapsetup.f90:8
+!   solvewithgpu(system, nstfv, evecfv, evalfv)                   modfvsystem.f90:585 (solvewithlapack)
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -19,7 +20,8 @@ module exc_setup_gpu
   integer, save :: ngkmax_ = 0
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
-  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu
+  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
+            solvewithgpu
 
 contains
 
@@ -123,5 +125,28 @@ contains
                           int(nstfv, c_int), int(nmatmax, c_int), evecfv, wh, wo, dedtau, c_null_ptr), "forcek")
     forceibs = forceibs - dedtau
   end subroutine forcek_gpu
+
+  ! modfvsystem.f90:585 (solvewithlapack) — the lowest nstfv eigenpairs of the first-variational
+  ! system, the counterpart of its ZHEGVX(1, 'V', 'I', 'U', ..., 1, nstfv, ...) (:691-693), on the
+  ! GPU.  H and O are overwritten as ZHEGVX leaves them; a failure stops the program as the
+  ! reference does (:698-711).
+  subroutine solvewithgpu(system, nstfv, evecfv, evalfv)
+    type(evsystem), intent(inout) :: system
+    integer, intent(in) :: nstfv
+    complex(8), intent(out) :: evecfv(:, :)
+    real(8), intent(out) :: evalfv(nstfv)
+    integer(c_int) :: rc, n
+    n = int(system%hamilton%rank, c_int)
+    if (size(evecfv, 1) < n .or. size(evecfv, 2) < nstfv) error stop "solvewithgpu: evecfv is too small"
+    rc = exc_eigen(ctx, n, n, system%hamilton%za, system%overlap%za, int(nstfv, c_int), evalfv, evecfv, &
+                   int(size(evecfv, 1), c_int))
+    if (rc /= 0) then
+      write(*,*)
+      write(*, '("Error(solvewithgpu): diagonalisation failed")')
+      write(*, '(" ZHEGVX-style INFO = ", I8)') rc
+      write(*,*)
+      error stop 1
+    end if
+  end subroutine solvewithgpu
 
 end module exc_setup_gpu

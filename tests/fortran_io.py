@@ -47,3 +47,14 @@ def read_forces(path, cr, ngkmax):
         skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat
         np.fromfile(f, np.complex128, skip)
         return np.fromfile(f, np.float64, 3 * cr.natm).reshape((3, cr.natm), order="F")
+
+
+def read_eigen_output(path, cr, ngkmax, nev):
+    """evalfv(nev), evecfv(nmat, nev) appended by the driver's solve step (no forces)."""
+    with open(path, "rb") as f:
+        nmat = int(np.fromfile(f, np.int32, 1)[0])
+        skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat
+        np.fromfile(f, np.complex128, skip)
+        e = np.fromfile(f, np.float64, nev)
+        C = np.fromfile(f, np.complex128, nmat * nev).reshape((nmat, nev), order="F")
+    return e, C

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import read_forces, read_output, write_eigen, write_input
+from fortran_io import read_eigen_output, read_forces, read_output, write_eigen, write_input
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +60,26 @@ def test_fortran_forcek(tmp_path):
     F_ref = R.forcek_ibs(cr, kp, C, e, occ, apwalm=A, wkpt=wk)
     print("forceibs max |diff|", np.abs(F - F_ref).max(), "max |F|", np.abs(F_ref).max())
     assert np.abs(F - F_ref).max() <= 1e-9 * max(1.0, np.abs(F_ref).max())
+
+
+@pytest.mark.parametrize("which", ["small-b", "config0"])
+def test_fortran_solvewithgpu(tmp_path, which):
+    """solvewithgpu (Fortran, through exc_eigen) after the setup: eigenvalues against the
+    oracle's LAPACK solve of the oracle's H, O within 1e-10 Ha, residual and O-orthonormality."""
+    from oracle import lapw_ref as R
+    w = small("b") if which == "small-b" else workload(0)
+    cr, kp = w.crystal, w.kpoints[0]
+    nev = 20
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, cr, kp)
+    r = subprocess.run([EXE, str(fin), str(fout), "-", str(nev)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    e, C = read_eigen_output(fout, cr, kp.igpig.shape[0], nev)
+    H, O, _ = R.setup(cr, kp)
+    e_ref, _ = R.eigen(H, O, nev)
+    res = np.linalg.norm(H @ C - (O @ C) * e[None, :], axis=0).max() / (np.linalg.norm(H, 2) + np.abs(e).max() * np.linalg.norm(O, 2))
+    orth = np.abs(C.conj().T @ O @ C - np.eye(nev)).max()
+    print(f"{which}: max |d eig| {np.abs(e - e_ref).max():.2e} Ha, residual {res:.2e}, orthonormality {orth:.2e}")
+    assert np.abs(e - e_ref).max() <= 1e-10
+    assert res <= 1e-12 and orth <= 1e-10
