@@ -332,10 +332,17 @@ bool eig_args_ok(int nmat, int ldh, int nev, int ldv) {
 // worker's stream; d_ints receives (nev found, info).  Solver variants (all: Cholesky of O,
 // reduction to standard form, then)
 //  0 zhegvdx: tridiagonal divide and conquer, lowest nev     1 zhegvx: bisection + inverse iteration
-//  2 zhegvd: divide and conquer, all (default)               3 zhegv: implicit QL/QR, all
+//  2 zhegvd: divide and conquer, all                         3 zhegv: implicit QL/QR, all
+//  4 (default) the reduction C = U^-H H U^-1 with level-3 operations (potrf, trtri, two trmm:
+//    rocSOLVER's zhegst runs column by column), zheevd on C, and X = U^-1 Y for the nev
+//    wanted columns only
 // (rocSOLVER's Jacobi solver, zhegvj, misses the 1e-10 Ha eigenvalue bar on these spectra)
-int eig_enqueue(exc_ctx::EigWorker &W, int v, int nmat, int ldh, double *h, double *o, int nev, double *eval,
-                double *evec, int ldv, int *d_ints) {
+// d_ints: nev found, info of the solver / of potrf, info of trtri, info of heevd
+// Variant 4 runs in two phases: phase 1 factors O (the caller checks its info on the host
+// before phase 2, so a non-positive-definite O never feeds the later steps); the other
+// variants do all their work in phase 2.
+int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, double *h, double *o, int nev,
+                double *eval, double *evec, int ldv, int *d_ints) {
   if (!W.blas) {
     if (rocblas_create_handle(&W.blas) != rocblas_status_success) {
       W.blas = nullptr;
@@ -355,7 +362,12 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int nmat, int ldh, double *h, doub
   rocblas_double_complex *A = (rocblas_double_complex *)h, *B = (rocblas_double_complex *)o;
   rocblas_double_complex *Z = (rocblas_double_complex *)W.z;
   rocblas_status st;
-  bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z
+  bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z (v 0, 1, 4)
+  if (phase == 1) {
+    if (v != 4) return EXC_OK;
+    return rocsolver_zpotrf(W.blas, up, nmat, B, ldh, d_ints + 1) == rocblas_status_success ? EXC_OK
+                                                                                           : EXC_ERR_HIP;   // O = U^H U
+  }
   switch (v) {
     case 0:
       st = rocsolver_zhegvdx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev,
@@ -369,10 +381,26 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int nmat, int ldh, double *h, doub
       st = rocsolver_zhegvd(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
       in_a = true;
       break;
-    default:
+    case 3:
       st = rocsolver_zhegv(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
       in_a = true;
       break;
+    default: {
+      const rocblas_double_complex one = {1.0, 0.0};
+      st = rocsolver_ztrtri(W.blas, up, rocblas_diagonal_non_unit, nmat, B, ldh, d_ints + 2);   // B = U^-1
+      if (st == rocblas_status_success)                                                 // Z = H U^-1
+        st = rocblas_ztrmm(W.blas, rocblas_side_right, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nmat, &one, B, ldh, A, ldh, Z, nmat);
+      if (st == rocblas_status_success)                                                 // H = U^-H Z
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_conjugate_transpose,
+                           rocblas_diagonal_non_unit, nmat, nmat, &one, B, ldh, Z, nmat, A, ldh);
+      if (st == rocblas_status_success)                                                 // C Y = Y diag(w)
+        st = rocsolver_zheevd(W.blas, ev, up, nmat, A, ldh, W.w, W.e, d_ints + 3);
+      if (st == rocblas_status_success)                                                 // Z = U^-1 Y(:, 1:nev)
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nev, &one, B, ldh, A, ldh, Z, nmat);
+      break;
+    }
   }
   if (st != rocblas_status_success) return EXC_ERR_HIP;
   EXC_CHECK(hipMemcpyAsync(eval, W.w, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
@@ -381,8 +409,14 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int nmat, int ldh, double *h, doub
   return EXC_OK;
 }
 
-// (nev found, info) of a solve; the full-spectrum solvers (v >= 2) report no count
-int eig_status(const int *hi, int nev, int v) {
+// status of a solve from its four ints (the full-spectrum solvers, v >= 2, report no count);
+// a failed Cholesky of O is nmat + i as in LAPACK's zhegv
+int eig_status(const int *hi, int nmat, int nev, int v) {
+  if (v == 4) {
+    if (hi[1] > 0) return nmat + hi[1];
+    if (hi[2] != 0) return EXC_ERR_HIP;
+    return hi[3];
+  }
   if (hi[1] != 0) return hi[1];
   return (v >= 2 || hi[0] == nev) ? EXC_OK : EXC_ERR_HIP;
 }
@@ -397,18 +431,24 @@ int exc_eigen_dev(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, 
   if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
-  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 2 * sizeof(int)))) return rc;
-  const int v = c->eig_variant >= 0 ? c->eig_variant : 2;   // zhegvd: fastest on MI355X at nmat ~ 1200
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 4 * sizeof(int)))) return rc;
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 4;
   exc_ctx::EigWorker &W = c->ew[0];
   W.s = c->stream;
+  int hi[4] = {0, 0, 0, 0};
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
-    if ((rc = eig_enqueue(W, v, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
+    if ((rc = eig_enqueue(W, v, 1, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
+    if (v == 4) {
+      EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
+      EXC_CHECK(hipStreamSynchronize(c->stream));
+      if (hi[1] > 0) return nmat + hi[1];
+    }
+    if ((rc = eig_enqueue(W, v, 2, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
   }
-  int hi[2] = {0, 0};
   EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
-  return eig_status(hi, nev, v);
+  return eig_status(hi, nmat, nev, v);
 }
 
 int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o,
@@ -422,7 +462,7 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
   if (nk == 0 || nev == 0) return EXC_OK;
   int rc = use_device(c);
   if (rc) return rc;
-  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 2 * sizeof(int) * nk))) return rc;
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 4 * sizeof(int) * nk))) return rc;
   if (!c->eig_fork) EXC_CHECK(hipEventCreateWithFlags(&c->eig_fork, hipEventDisableTiming));
   const int nw = nk < exc_ctx::kEigStreams ? nk : exc_ctx::kEigStreams;
   for (int i = 1; i <= nw; ++i) {
@@ -430,27 +470,38 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
     if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
     if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
   }
-  const int v = c->eig_variant >= 0 ? c->eig_variant : 2;
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 4;
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
     for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
-    for (int k = 0; k < nk; ++k) {
-      if (nmat[k] == 0) continue;
-      exc_ctx::EigWorker &W = c->ew[1 + k % nw];
-      if ((rc = eig_enqueue(W, v, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k], c->eig_int + 2 * k)))
-        return rc;
+    for (int phase = 1; phase <= 2; ++phase) {
+      if (phase == 2 && v == 4) {
+        // every factorization done and positive definite before the rest runs
+        for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamSynchronize(c->ew[i].s));
+        std::vector<int> hp(4 * (size_t)nk, 0);
+        EXC_CHECK(hipMemcpy(hp.data(), c->eig_int, sizeof(int) * 4 * nk, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nk; ++k)
+          if (nmat[k] > 0 && hp[4 * (size_t)k + 1] > 0) return nmat[k] + hp[4 * (size_t)k + 1];
+      }
+      for (int k = 0; k < nk; ++k) {
+        if (nmat[k] == 0) continue;
+        exc_ctx::EigWorker &W = c->ew[1 + k % nw];
+        if ((rc = eig_enqueue(W, v, phase, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k],
+                              c->eig_int + 4 * k)))
+          return rc;
+      }
     }
     for (int i = 1; i <= nw; ++i) {
       EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
       EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
     }
   }
-  std::vector<int> hi(2 * (size_t)nk, 0);
-  EXC_CHECK(hipMemcpyAsync(hi.data(), c->eig_int, sizeof(int) * 2 * nk, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> hi(4 * (size_t)nk, 0);
+  EXC_CHECK(hipMemcpyAsync(hi.data(), c->eig_int, sizeof(int) * 4 * nk, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   for (int k = 0; k < nk; ++k)
-    if (nmat[k] > 0 && (rc = eig_status(&hi[2 * (size_t)k], nev, v))) return rc;
+    if (nmat[k] > 0 && (rc = eig_status(&hi[4 * (size_t)k], nmat[k], nev, v))) return rc;
   return EXC_OK;
 }
 

@@ -120,7 +120,7 @@ int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int l
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
 constexpr int kBsplitVariants = 4;
-constexpr int kEigenVariants = 4;   // generalized eigensolver paths (exc_eigen_dev)
+constexpr int kEigenVariants = 5;   // generalized eigensolver paths (exc_eigen_dev)
 size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);

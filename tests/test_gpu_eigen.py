@@ -80,9 +80,10 @@ def test_eigen_reports_indefinite_overlap():
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
 
 
-@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("variant", range(5))
 def test_eigen_solver_variants(variant):
-    """Every rocSOLVER path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv) to the same bars."""
+    """Every solver path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv, 4 level-3 reduction +
+    zheevd) to the same bars."""
     from exciting_amd import lapw
     from oracle import lapw_ref as R
     w = workload(1, 1)
