@@ -71,6 +71,8 @@ struct exc_ctx {
     size_t wb;
     cplx *z;                // n x n eigenvector workspace (rocSOLVER may use all n columns)
     size_t zb;
+    cplx *zc;               // n x nev eigenvectors of the grouped path's back-transform
+    size_t zcb;
     double *e;              // n-length workspace (tridiagonal off-diagonal, ifail)
     size_t eb;
     hipEvent_t done;
@@ -78,8 +80,10 @@ struct exc_ctx {
   static constexpr int kEigStreams = 4;
   EigWorker ew[kEigStreams + 1];
   int eig_variant;          // -1: built-in choice
-  int *eig_int;             // per solve: nev found, info (device), 2 per k-point
+  int *eig_int;             // per solve: 4 ints (device)
   size_t eig_int_bytes;
+  void *eg;                 // grouped solve: padded batch, eigenvalues, infos, selection, descriptors
+  size_t eg_bytes;
   hipEvent_t eig_fork;
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
@@ -286,11 +290,13 @@ int exc_ctx_destroy(exc_ctx *c) {
     if (w.blas) rocblas_destroy_handle(w.blas);
     if (w.w) hipFree(w.w);
     if (w.z) hipFree(w.z);
+    if (w.zc) hipFree(w.zc);
     if (w.e) hipFree(w.e);
     if (w.done) hipEventDestroy(w.done);
     if (i > 0 && w.s) hipStreamDestroy(w.s);
   }
   if (c->eig_int) hipFree(c->eig_int);
+  if (c->eg) hipFree(c->eg);
   if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -364,7 +370,11 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, doub
   rocblas_status st;
   bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z (v 0, 1, 4)
   if (phase == 1) {
-    if (v != 4) return EXC_OK;
+    // ZHEGVX('U') reads the upper triangles and the real parts of the diagonals only;
+    // rocSOLVER's solvers and the products of variant 4 read more: lower := conj(upper)
+    if ((rc = exc_launch_eig_hermfill((cplx *)h, nmat, ldh, W.s))) return rc;
+    if ((rc = exc_launch_eig_hermfill((cplx *)o, nmat, ldh, W.s))) return rc;
+    if (v < 4) return EXC_OK;
     return rocsolver_zpotrf(W.blas, up, nmat, B, ldh, d_ints + 1) == rocblas_status_success ? EXC_OK
                                                                                            : EXC_ERR_HIP;   // O = U^H U
   }
@@ -412,13 +422,122 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, doub
 // status of a solve from its four ints (the full-spectrum solvers, v >= 2, report no count);
 // a failed Cholesky of O is nmat + i as in LAPACK's zhegv
 int eig_status(const int *hi, int nmat, int nev, int v) {
-  if (v == 4) {
+  if (v >= 4) {
     if (hi[1] > 0) return nmat + hi[1];
     if (hi[2] != 0) return EXC_ERR_HIP;
     return hi[3];
   }
   if (hi[1] != 0) return hi[1];
   return (v >= 2 || hi[0] == nev) ? EXC_OK : EXC_ERR_HIP;
+}
+
+// Variant 5 of the multi-k entry: every k-point's H, O padded to the largest nmat n and
+// reduced as one strided batch -- Cholesky, U^-1, C = U^-H H U^-1 and the tridiagonal
+// reduction of C, one launch per step for all k-points (the per-k-point reduction is
+// thousands of small launches, one column at a time).  The pad stays out of the leading
+// nmat_k block of every factor (k_eigen.hip).  Then per k-point, on the worker streams: the
+// tridiagonal divide and conquer of the leading nmat_k part (real arithmetic), its first nev
+// eigenvectors through the Householder reflectors (zunmtr) and U^-1 (trmm) into the caller's
+// evec.  H and O are read, not modified.
+int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o, int nev,
+                double *const *eval, double *const *evec, const int *ldv) {
+  std::vector<eig_kdesc> kd;
+  int n = 0;
+  for (int k = 0; k < nk; ++k) {
+    if (nmat[k] == 0) continue;
+    kd.push_back({(const cplx *)h[k], (const cplx *)o[k], eval[k], (cplx *)evec[k], nmat[k], ldh[k], ldv[k], k});
+    n = std::max(n, nmat[k]);
+  }
+  const int nb = (int)kd.size();
+  if (nb == 0) return EXC_OK;
+  if (nb > 65535) return EXC_ERR_ARG;
+  const int nw = std::min(nb, (int)exc_ctx::kEigStreams);
+  const size_t mat = (size_t)n * n, a256 = 255;
+  auto al = [&](size_t b) { return (b + a256) & ~a256; };
+  const size_t o_po = al(16 * mat * nb), o_pz = o_po + al(16 * mat * nb), o_d = o_pz + al(16 * mat * nb);
+  const size_t o_e = o_d + al(8 * (size_t)n * nb);
+  const size_t o_tau = o_e + al(8 * (size_t)n * nb), o_i = o_tau + al(16 * (size_t)n * nb);
+  const size_t o_kd = o_i + al(12 * (size_t)nb);
+  const size_t total = o_kd + sizeof(eig_kdesc) * nb;
+  int rc;
+  if ((rc = grow(&c->eg, &c->eg_bytes, total))) return rc;
+  char *base = (char *)c->eg;
+  cplx *ph = (cplx *)base, *po = (cplx *)(base + o_po), *pz = (cplx *)(base + o_pz), *tau = (cplx *)(base + o_tau);
+  double *d = (double *)(base + o_d), *e = (double *)(base + o_e);
+  int *info = (int *)(base + o_i);
+  eig_kdesc *dkd = (eig_kdesc *)(base + o_kd);
+  for (int i = 0; i <= nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[i];
+    if (i == 0) W.s = c->stream;
+    if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+      W.blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+    if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+    if (i > 0) {   // per worker: real eigenvectors of T (n x n), their first nev columns complex
+      if ((rc = grow((void **)&W.z, &W.zb, sizeof(double) * mat))) return rc;
+      if ((rc = grow((void **)&W.zc, &W.zcb, sizeof(cplx) * (size_t)n * nev))) return rc;
+    }
+  }
+  auto fail = [](const char *what, int a, int b) {
+    fprintf(stderr, "exc_eigen_batched_dev (grouped): %s (%d, %d)\n", what, a, b);
+    return EXC_ERR_HIP;
+  };
+  const auto up = rocblas_fill_upper;
+  const auto nu = rocblas_diagonal_non_unit;
+  const rocblas_double_complex one = {1.0, 0.0};
+  rocblas_handle hb = c->ew[0].blas;
+  auto *A = (rocblas_double_complex *)ph, *B = (rocblas_double_complex *)po, *Z = (rocblas_double_complex *)pz;
+  const rocblas_stride sm = (rocblas_stride)mat;
+  std::vector<int> hi(3 * (size_t)nb, 0);
+  StageTimer t(c, EXC_STAGE_EIGEN);
+  EXC_CHECK(hipMemcpyAsync(dkd, kd.data(), sizeof(eig_kdesc) * nb, hipMemcpyHostToDevice, c->stream));
+  if ((rc = exc_launch_eig_pad(dkd, nb, n, ph, po, 1.0, c->stream))) return rc;
+  // O = U^H U for every k-point; checked before anything reads the factors
+  rocblas_status st = rocsolver_zpotrf_strided_batched(hb, up, n, B, n, sm, info, nb);
+  if (st != rocblas_status_success) return fail("zpotrf_strided_batched status", (int)st, n);
+  EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b)
+    if (hi[b] > 0) return hi[b] <= kd[b].n ? kd[b].n + hi[b] : fail("pad not positive definite", b, hi[b]);
+  st = rocsolver_ztrtri_strided_batched(hb, up, nu, n, B, n, sm, info + nb, nb);   // U^-1
+  if (st == rocblas_status_success)   // Z = H U^-1
+    st = rocblas_ztrmm_strided_batched(hb, rocblas_side_right, up, rocblas_operation_none, nu, n, n, &one, B, n, sm,
+                                       A, n, sm, Z, n, sm, nb);
+  if (st == rocblas_status_success)   // H = U^-H Z
+    st = rocblas_ztrmm_strided_batched(hb, rocblas_side_left, up, rocblas_operation_conjugate_transpose, nu, n, n,
+                                       &one, B, n, sm, Z, n, sm, A, n, sm, nb);
+  if (st == rocblas_status_success)   // C = Q T Q^H
+    st = rocsolver_zhetrd_strided_batched(hb, up, n, A, n, sm, d, n, e, n, (rocblas_double_complex *)tau, n, nb);
+  if (st != rocblas_status_success) return fail("trtri / trmm / zhetrd strided_batched status", (int)st, n);
+  EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
+  for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
+  for (int b = 0; b < nb; ++b) {
+    exc_ctx::EigWorker &W = c->ew[1 + b % nw];
+    const int m = kd[b].n;
+    double *db = d + (size_t)b * n, *eb = e + (size_t)b * n;
+    st = rocsolver_dstedc(W.blas, rocblas_evect_tridiagonal, m, db, eb, (double *)W.z, m, info + 2 * nb + b);
+    if (st != rocblas_status_success) return fail("dstedc status", (int)st, b);
+    if ((rc = exc_launch_eig_r2c((const double *)W.z, m, nev, m, W.zc, m, W.s))) return rc;
+    st = rocsolver_zunmtr(W.blas, rocblas_side_left, up, rocblas_operation_none, m, nev, A + (size_t)b * mat, n,
+                          (rocblas_double_complex *)(tau + (size_t)b * n), (rocblas_double_complex *)W.zc, m);
+    if (st == rocblas_status_success)
+      st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, nu, m, nev, &one, B + (size_t)b * mat,
+                         n, (rocblas_double_complex *)W.zc, m, (rocblas_double_complex *)kd[b].evec, kd[b].ldv);
+    if (st != rocblas_status_success) return fail("zunmtr / trmm status", (int)st, b);
+    EXC_CHECK(hipMemcpyAsync(kd[b].eval, db, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
+  }
+  for (int i = 1; i <= nw; ++i) {
+    EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
+    EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
+  }
+  EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b) {
+    if (hi[nb + b] != 0) return fail("ztrtri info", b, hi[nb + b]);   // of a positive definite factor
+    if (hi[2 * nb + b] != 0) return hi[2 * nb + b];                 // eigenvalues that did not converge
+  }
+  return EXC_OK;
 }
 
 }  // namespace
@@ -439,7 +558,7 @@ int exc_eigen_dev(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, 
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     if ((rc = eig_enqueue(W, v, 1, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
-    if (v == 4) {
+    if (v >= 4) {
       EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
       EXC_CHECK(hipStreamSynchronize(c->stream));
       if (hi[1] > 0) return nmat + hi[1];
@@ -470,13 +589,14 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
     if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
     if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
   }
-  const int v = c->eig_variant >= 0 ? c->eig_variant : 4;
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 5;
+  if (v == 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv);
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
     for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
     for (int phase = 1; phase <= 2; ++phase) {
-      if (phase == 2 && v == 4) {
+      if (phase == 2 && v >= 4) {
         // every factorization done and positive definite before the rest runs
         for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamSynchronize(c->ew[i].s));
         std::vector<int> hp(4 * (size_t)nk, 0);

@@ -120,7 +120,7 @@ int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int l
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
 constexpr int kBsplitVariants = 4;
-constexpr int kEigenVariants = 5;   // generalized eigensolver paths (exc_eigen_dev)
+constexpr int kEigenVariants = 6;   // generalized eigensolver paths (exc_eigen_dev)
 size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);
@@ -129,5 +129,15 @@ int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
 int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
+// generalized eigensolve helpers (k_eigen.hip)
+struct eig_kdesc {   // one k-point of a grouped solve: nmat n, device arrays
+  const cplx *h, *o;
+  double *eval;
+  cplx *evec;
+  int n, ldh, ldv, pad;
+};
+int exc_launch_eig_hermfill(cplx *h, int n, int ldh, hipStream_t s);
+int exc_launch_eig_pad(const eig_kdesc *ks, int nb, int n, cplx *ph, cplx *po, double sigma, hipStream_t s);
+int exc_launch_eig_r2c(const double *zr, int m, int ncols, int ldr, cplx *zc, int ldc, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

@@ -1,0 +1,128 @@
+// Device helpers of the generalized eigensolve (exc_eigen_dev, exc_eigen_batched_dev).
+//
+// solvewithlapack (modfvsystem.f90:585-720) hands ZHEGVX the upper triangles of H and O.
+// The level-3 reduction C = U^-H H U^-1 multiplies by the whole of H, so the lower triangle
+// is first made the conjugate of the upper one (and the diagonal real, as ZHETRD assumes).
+//
+// The grouped solve of many k-points copies every k-point's pair into a strided batch of
+// one common order n = max nmat_k, padded as
+//     H_pad = [H_k 0; 0 sigma I],   O_pad = [O_k 0; 0 I],
+// for the strided-batched Cholesky, triangular inverse, trmm and tridiagonal reduction (one
+// launch per step for all k-points; the per-k-point reduction is thousands of small
+// launches, one column at a time).  The pad stays out of the leading nmat_k block of every
+// result: the Cholesky factor and its inverse are block diagonal with an identity block, the
+// reduced matrix is [C_k 0; 0 sigma I], the Householder reflectors of the columns of C_k touch
+// only rows above their column, and those of the pad columns annihilate vectors that are
+// already zero (tau = 0).  The leading nmat_k part of the tridiagonal matrix, the reflectors
+// and U^-1 are therefore those of the k-point alone (to rounding: the blocking of the
+// batched routines follows n), and the rest of the solve runs per k-point on that part only.
+#include "exc_common.h"
+
+namespace {
+
+constexpr int kET = 32;          // tile edge of the fill and pad kernels
+constexpr int kEThreads = 256;   // 32 rows x 8 column groups
+constexpr int kSelThreads = 1024;
+
+__device__ __forceinline__ cplx czero() { return {0.0, 0.0}; }
+
+// In place: h(r, c) = conj(h(c, r)) for r > c, Im h(c, c) = 0.  Block (ti, tj), ti <= tj,
+// reads tile (ti, tj) of the upper triangle and writes its conjugate transpose at (tj, ti).
+__global__ __launch_bounds__(kEThreads) void herm_fill_kernel(cplx *h, int n, int ldh) {
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (ti > tj) return;
+  __shared__ cplx s[kET][kET + 1];   // s[c][r] = h(row0 + r, col0 + c)
+  const int r = threadIdx.x % kET, cg = threadIdx.x / kET;
+  const int row0 = ti * kET, col0 = tj * kET;
+  for (int c = cg; c < kET; c += kEThreads / kET) {
+    const int R = row0 + r, C = col0 + c;
+    s[c][r] = (R < n && C < n) ? h[R + (size_t)C * ldh] : czero();
+  }
+  __syncthreads();
+  // destination (col0 + r, row0 + c) = conj(h(row0 + c, col0 + r)) = conj(s[r][c]),
+  // for destination rows strictly below the diagonal
+  for (int c = cg; c < kET; c += kEThreads / kET) {
+    const int R = col0 + r, C = row0 + c;
+    if (R >= n || C >= n) continue;
+    if (R > C) {
+      h[R + (size_t)C * ldh] = cconj(s[r][c]);
+    } else if (R == C) {
+      h[R + (size_t)C * ldh].im = 0.0;
+    }
+  }
+}
+
+// Padded copy of matrix `which` (0: H with pad sigma, 1: O with pad 1) of every k-point of
+// the batch into p + k * n * n (leading dimension n), both triangles from the upper one.
+__global__ __launch_bounds__(kEThreads) void eig_pad_kernel(const eig_kdesc *ks, int n, cplx *p, int which,
+                                                            double sigma) {
+  const int ti = blockIdx.x, tj = blockIdx.y, k = blockIdx.z;
+  if (ti > tj) return;
+  const eig_kdesc kd = ks[k];
+  const cplx *src = which ? kd.o : kd.h;
+  const double dpad = which ? 1.0 : sigma;
+  cplx *dst = p + (size_t)k * n * n;
+  __shared__ cplx s[kET][kET + 1];
+  const int r = threadIdx.x % kET, cg = threadIdx.x / kET;
+  const int row0 = ti * kET, col0 = tj * kET;
+  for (int c = cg; c < kET; c += kEThreads / kET) {
+    const int R = row0 + r, C = col0 + c;
+    if (R >= n || C >= n) continue;
+    cplx v;
+    if (R > C) {
+      v = czero();   // below the diagonal of a diagonal tile: written from s below
+    } else if (C < kd.n) {
+      v = src[R + (size_t)C * kd.ldh];
+      if (R == C) v.im = 0.0;
+    } else {
+      v = R == C ? cplx{dpad, 0.0} : czero();
+    }
+    s[c][r] = v;
+    if (R <= C) dst[R + (size_t)C * n] = v;
+  }
+  __syncthreads();
+  for (int c = cg; c < kET; c += kEThreads / kET) {
+    const int R = col0 + r, C = row0 + c;
+    if (R >= n || C >= n || R <= C) continue;
+    dst[R + (size_t)C * n] = cconj(s[r][c]);
+  }
+}
+
+// zc(:, j) = zr(:, j) for j < ncols (m rows): real eigenvectors of the tridiagonal matrix
+// into the complex workspace of the back-transform
+__global__ __launch_bounds__(256) void eig_r2c_kernel(const double *zr, int m, int ldr, cplx *zc, int ldc) {
+  const int r = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (r >= m) return;
+  zc[r + (size_t)j * ldc] = {zr[r + (size_t)j * ldr], 0.0};
+}
+
+inline int tiles(int n) { return (n + kET - 1) / kET; }
+
+}  // namespace
+
+int exc_launch_eig_hermfill(cplx *h, int n, int ldh, hipStream_t s) {
+  if (n <= 0) return EXC_OK;
+  if (ldh < n || tiles(n) > 65535) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(herm_fill_kernel, dim3(tiles(n), tiles(n)), dim3(kEThreads), 0, s, h, n, ldh);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_pad(const eig_kdesc *ks, int nb, int n, cplx *ph, cplx *po, double sigma, hipStream_t s) {
+  if (nb <= 0 || n <= 0) return EXC_OK;
+  if (nb > 65535 || tiles(n) > 65535) return EXC_ERR_ARG;
+  const dim3 grid(tiles(n), tiles(n), nb);
+  hipLaunchKernelGGL(eig_pad_kernel, grid, dim3(kEThreads), 0, s, ks, n, ph, 0, sigma);
+  EXC_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(eig_pad_kernel, grid, dim3(kEThreads), 0, s, ks, n, po, 1, sigma);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_r2c(const double *zr, int m, int ncols, int ldr, cplx *zc, int ldc, hipStream_t s) {
+  if (m <= 0 || ncols <= 0) return EXC_OK;
+  if (ncols > 65535 || ldr < m || ldc < m) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(eig_r2c_kernel, dim3((m + 255) / 256, ncols), dim3(256), 0, s, zr, m, ldr, zc, ldc);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

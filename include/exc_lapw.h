@@ -178,7 +178,9 @@ int exc_ho_vjp(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const in
 /* Generalized Hermitian eigenproblem of one k-point after the setup, the counterpart of
  * solvewithlapack (modfvsystem.f90:585-720), which calls ZHEGVX(itype 1, 'V', range 'I',
  * 'U', il = 1, iu = nev) (:691-693): H c = e O c for the lowest nev eigenpairs.
- * h, o: ldh x nmat (upper triangles used; both overwritten, O by its Cholesky factor);
+ * h, o: ldh x nmat (upper triangles and real parts of the diagonals used, as ZHEGVX('U');
+ * both overwritten, O's upper triangle by its Cholesky factor, the lower triangles by the
+ * conjugates of the upper ones);
  * eval(nev) ascending, evec(ldv, nev) O-orthonormal (ldv >= nmat).  Returns 0, or LAPACK's
  * info > 0 (nmat + i: the leading minor of order i of O is not positive definite; i <= nmat:
  * i eigenvectors did not converge), or < 0 on an argument/HIP error.
@@ -189,9 +191,12 @@ int exc_eigen(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, do
               double *evec, int ldv);
 /* The lowest nev eigenpairs of nk k-points at once (device arrays; per k: nmat[k], ldh[k],
  * h[k], o[k], eval[k], evec[k], ldv[k] as for exc_eigen_dev, the pointer arrays themselves
- * in host memory).  The solves run concurrently on worker streams forked from and joined
- * back into the context's stream (the reference solves the k-points one after another in
- * the loop of scf_cycle.f90:241-276).  Returns 0, or the first failing k-point's status. */
+ * in host memory).  The reference solves the k-points one after another in the loop of
+ * scf_cycle.f90:241-276.  Default: all k-points padded to the largest nmat and solved as one
+ * strided batch, one launch per step for all of them (h, o left unchanged).  Solver variants
+ * 0-4 (exc_ctx_set_variant, EXC_STAGE_EIGEN) instead run the single-k solves concurrently
+ * on worker streams forked from and joined back into the context's stream (h, o
+ * overwritten).  Returns 0, or the first failing k-point's status. */
 int exc_eigen_batched_dev(exc_ctx *ctx, int nk, const int *nmat, const int *ldh, double *const *h,
                           double *const *o, int nev, double *const *eval, double *const *evec,
                           const int *ldv);

@@ -80,10 +80,10 @@ def test_eigen_reports_indefinite_overlap():
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(6))
 def test_eigen_solver_variants(variant):
     """Every solver path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv, 4 level-3 reduction +
-    zheevd) to the same bars."""
+    zheevd; 5 is the grouped multi-k path, the same as 4 for one k-point) to the same bars."""
     from exciting_amd import lapw
     from oracle import lapw_ref as R
     w = workload(1, 1)
@@ -98,12 +98,14 @@ def test_eigen_solver_variants(variant):
 
 
 def test_eigen_batched_equals_single():
-    """The multi-k entry (concurrent worker streams) gives the single-k results bit for bit."""
+    """The multi-k entry on concurrent worker streams (variant 4) gives the single-k results
+    bit for bit."""
     import torch
     from exciting_amd import lapw
     w = workload(1, 6)
     s = torch.cuda.Stream()
     ctx = lapw.Context(w.crystal, stream=s)
+    ctx.set_variant("eigen", 4)
     nev = 200
     flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE
     res = {}
@@ -124,3 +126,98 @@ def test_eigen_batched_equals_single():
     for i in range(len(w.kpoints)):
         assert torch.equal(res["single"][0][i], res["batched"][0][i])
         assert torch.equal(res["single"][1][i], res["batched"][1][i])
+
+
+def _random_pair(rng, n, lower_garbage):
+    """Hermitian H, Hermitian positive definite O (Fortran order); with lower_garbage the
+    strictly lower triangles are overwritten by unrelated values, which ZHEGVX('U') ignores."""
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = (X + X.conj().T) / 2
+    Y = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    O = Y @ Y.conj().T / n + np.eye(n)
+    Hs, Os = H.copy(), O.copy()
+    if lower_garbage:
+        il = np.tril_indices(n, -1)
+        Hs[il] = rng.standard_normal(len(il[0])) * 7 + 3j
+        Os[il] = rng.standard_normal(len(il[0])) * 5 - 1j
+        Hs[np.diag_indices(n)] += 0.25j      # imaginary diagonal parts are ignored as well
+        Os[np.diag_indices(n)] -= 0.5j
+    return H, O, np.asfortranarray(Hs), np.asfortranarray(Os)
+
+
+@pytest.mark.parametrize("variant", range(6))
+def test_eigen_reads_upper_triangles_only(variant):
+    """Like ZHEGVX(uplo 'U'): the lower triangles and the imaginary diagonal of H, O are not
+    part of the input."""
+    import scipy.linalg
+    from exciting_amd import lapw
+    w = small("a")
+    ctx = lapw.Context(w.crystal)
+    ctx.set_variant("eigen", variant)
+    rng = np.random.default_rng(7)
+    n, nev = 96, 20
+    H, O, Hs, Os = _random_pair(rng, n, True)
+    e_ref = scipy.linalg.eigh(H, O, eigvals_only=True, subset_by_index=[0, nev - 1])
+    e, C = ctx.eigen(Hs, Os, nev)
+    _check(H, O, e, C, e_ref, f"upper triangles only, variant {variant}")
+
+
+def _to_dev(M):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(M.T)).cuda()   # (nmat, ld) row-major = Fortran (ld, nmat)
+
+
+def test_eigen_grouped_ragged():
+    """The grouped multi-k solve (variant 5: every k-point padded to the largest nmat, one
+    strided-batched launch per step) on k-points of very different orders, lower triangles
+    filled with unrelated values: every k-point to the LAPACK bars, inputs left unchanged."""
+    import scipy.linalg
+    import torch
+    from exciting_amd import lapw
+    w = small("a")
+    ctx = lapw.Context(w.crystal)
+    ctx.set_variant("eigen", 5)
+    rng = np.random.default_rng(11)
+    sizes, nev = [64, 33, 47, 64, 20], 20
+    pairs = [_random_pair(rng, n, True) for n in sizes]
+    hs = [_to_dev(p[2]) for p in pairs]
+    os_ = [_to_dev(p[3]) for p in pairs]
+    h_in = [t.clone() for t in hs]
+    o_in = [t.clone() for t in os_]
+    evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in sizes]
+    vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in sizes]
+    ctx.eigen_batched_dev(sizes, hs, os_, nev, evs, vecs)
+    torch.cuda.synchronize()
+    for i, n in enumerate(sizes):
+        assert torch.equal(hs[i], h_in[i]) and torch.equal(os_[i], o_in[i])
+        H, O = pairs[i][0], pairs[i][1]
+        e_ref = scipy.linalg.eigh(H, O, eigvals_only=True, subset_by_index=[0, nev - 1])
+        _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"grouped k{i} n={n}")
+
+
+def test_eigen_grouped_after_setup():
+    """Variant 5 after the grouped setup of workload 1 (6 k-points, nmat 1182-1282): each
+    k-point against LAPACK on the same GPU-built H and O."""
+    import scipy.linalg
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 6)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    ctx.set_variant("eigen", 5)
+    nev = 200
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE)
+        torch.cuda.synchronize()
+        Hh = [t.cpu().numpy().T.copy() for t in b.H]
+        Oh = [t.cpu().numpy().T.copy() for t in b.O]
+        evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+        vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in b.nmat]
+        ctx.eigen_batched_dev(b.nmat, b.H, b.O, nev, evs, vecs)
+        torch.cuda.synchronize()
+    for i in range(b.nk):
+        e_ref = scipy.linalg.eigh(Hh[i], Oh[i], lower=False, eigvals_only=True, subset_by_index=[0, nev - 1],
+                                  driver="gvx")
+        _check(Hh[i], Oh[i], evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref,
+               f"grouped after setup k{i} n={b.nmat[i]}")
