@@ -82,8 +82,10 @@ struct exc_ctx {
   int eig_variant;          // -1: built-in choice
   int *eig_int;             // per solve: 4 ints (device)
   size_t eig_int_bytes;
-  void *eg;                 // grouped solve: padded batch, eigenvalues, infos, selection, descriptors
+  void *eg;                 // grouped solve: padded batch, tridiagonal factors, infos, descriptors
   size_t eg_bytes;
+  void *dmw;                // density matrix: A^T C, packed and weighted wavefunction blocks
+  size_t dmw_bytes;
   hipEvent_t eig_fork;
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
@@ -297,6 +299,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   }
   if (c->eig_int) hipFree(c->eig_int);
   if (c->eg) hipFree(c->eg);
+  if (c->dmw) hipFree(c->dmw);
   if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -625,6 +628,148 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
   return EXC_OK;
 }
 
+namespace {
+
+// dm(ws, ws, natm) += sum over the nk k-points of w_k wf1 diag(occ) wf1^H (gendmatmt.f90:59-172):
+// per k one strided batch over atoms of A_a^T C (rows in apwalm's storage order), packed into
+// the reference's row order with the lo rows appended (times evecsv when given), and the
+// occupation-weighted copy; then per atom one GEMM whose contraction runs over the nst
+// columns of every k-point, accumulated into dm.  All arrays are device memory.
+int dmat_run(exc_ctx *c, int nk, const int *ngp, const int *ldg, const double *const *apwalm, int nst,
+             const int *ldv, const double *const *evec, const double *const *evecsv, const double *const *occ,
+             const double *wkpt, double *dm) {
+  const exc_dev &d = c->d;
+  const int ws = d.naa + d.dm_maxlo, na = d.natm, naa = d.naa;
+  const size_t nt = (size_t)nk * nst;
+  if (nt > (size_t)0x7fffffff) return EXC_ERR_ARG;
+  bool any_sv = false;
+  for (int k = 0; k < nk; ++k) any_sv = any_sv || (evecsv && evecsv[k]);
+  const size_t b_w = 16 * (size_t)na * naa * nst, b_x = 16 * (size_t)na * ws * nt;
+  const size_t b_t = any_sv ? 16 * (size_t)na * ws * nst : 0;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  int rc;
+  if ((rc = grow(&c->dmw, &c->dmw_bytes, al(b_w) + 2 * al(b_x) + b_t))) return rc;
+  char *base = (char *)c->dmw;
+  cplx *w = (cplx *)base, *x = (cplx *)(base + al(b_w)), *x2 = (cplx *)(base + al(b_w) + al(b_x));
+  cplx *xt = (cplx *)(base + al(b_w) + 2 * al(b_x));
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
+  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+    W.blas = nullptr;
+    return EXC_ERR_HIP;
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  const size_t xs = (size_t)ws * nt;
+  StageTimer t(c, EXC_STAGE_DMAT);
+  for (int k = 0; k < nk; ++k) {
+    // W_a = A_a^T C_k (naa x nst per atom, rows io + apwordmax lm)
+    if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_transpose, rocblas_operation_none, naa, nst, ngp[k],
+                                      &one, (const rocblas_double_complex *)apwalm[k], ldg[k],
+                                      (rocblas_stride)ldg[k] * naa, (const rocblas_double_complex *)evec[k], ldv[k], 0,
+                                      &zero, (rocblas_double_complex *)w, naa, (rocblas_stride)naa * nst,
+                                      na) != rocblas_status_success)
+      return EXC_ERR_HIP;
+    const bool sv = evecsv && evecsv[k];
+    if (sv) {
+      if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], xt, ws,
+                                     (size_t)ws * nst, 0, c->stream)))
+        return rc;
+      if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_none, ws, nst, nst, &one,
+                                        (const rocblas_double_complex *)xt, ws, (rocblas_stride)ws * nst,
+                                        (const rocblas_double_complex *)evecsv[k], nst, 0, &zero,
+                                        (rocblas_double_complex *)(x + (size_t)k * nst * ws), ws, (rocblas_stride)xs,
+                                        na) != rocblas_status_success)
+        return EXC_ERR_HIP;
+    } else if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], x, ws, xs,
+                                          k * nst, c->stream))) {
+      return rc;
+    }
+    if ((rc = exc_launch_dmat_scale(d, ws, nst, x, x2, ws, xs, k * nst, occ[k], wkpt[k], c->stream))) return rc;
+  }
+  // dm_a += X_a X2_a^H
+  if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, ws, ws,
+                                    (int)nt, &one, (const rocblas_double_complex *)x, ws, (rocblas_stride)xs,
+                                    (const rocblas_double_complex *)x2, ws, (rocblas_stride)xs, &one,
+                                    (rocblas_double_complex *)dm, ws, (rocblas_stride)ws * ws,
+                                    na) != rocblas_status_success)
+    return EXC_ERR_HIP;
+  return EXC_OK;
+}
+
+bool dmat_args_ok(const exc_ctx *c, int ngp, int ngkmax, int nst, int ldv) {
+  return ngp >= 0 && ngkmax >= (ngp > 1 ? ngp : 1) && nst >= 0 && nst <= 65535 && ldv >= ngp + c->d.nlotot &&
+         ldv >= 1;
+}
+
+}  // namespace
+
+int exc_dmat_wfsize(const exc_ctx *c) {
+  if (!c) return EXC_ERR_ARG;
+  if (!c->have_lo) return EXC_ERR_STATE;
+  return c->d.naa + c->d.dm_maxlo;
+}
+
+int exc_gendmatmt_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
+                      const double *evecsv, const double *occ, double wkpt, double *dm) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (!dmat_args_ok(c, ngp, ngkmax, nst, ldv)) return EXC_ERR_ARG;
+  if (nst == 0) return EXC_OK;
+  if (!apwalm || !evecfv || !occ || !dm) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  return dmat_run(c, 1, &ngp, &ngkmax, &apwalm, nst, &ldv, &evecfv, &evecsv, &occ, &wkpt, dm);
+}
+
+int exc_gendmatmt_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int nst, const int *ldv,
+                              const double *const *evecfv, const double *const *evecsv, const double *const *occ,
+                              const double *wkpt, double *dm) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nk < 0 || (nk > 0 && (!ks || !ldv || !evecfv || !occ || !wkpt || !dm))) return EXC_ERR_ARG;
+  std::vector<int> ngp(nk), ldg(nk);
+  std::vector<const double *> apw(nk);
+  for (int k = 0; k < nk; ++k) {
+    if (!dmat_args_ok(c, ks[k].ngp, ks[k].ngkmax, nst, ldv[k])) return EXC_ERR_ARG;
+    if (nst > 0 && (!ks[k].apwalm || !evecfv[k] || !occ[k])) return EXC_ERR_ARG;
+    ngp[k] = ks[k].ngp;
+    ldg[k] = ks[k].ngkmax;
+    apw[k] = ks[k].apwalm;
+  }
+  if (nk == 0 || nst == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  return dmat_run(c, nk, ngp.data(), ldg.data(), apw.data(), nst, ldv, evecfv, evecsv, occ, wkpt, dm);
+}
+
+int exc_gendmatmt(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
+                  const double *evecsv, const double *occ, double wkpt, double *dm) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (!dmat_args_ok(c, ngp, ngkmax, nst, ldv)) return EXC_ERR_ARG;
+  if (nst == 0) return EXC_OK;
+  if (!apwalm || !evecfv || !occ || !dm) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int ws = c->d.naa + c->d.dm_maxlo;
+  const size_t b_a = 16 * (size_t)ngkmax * c->d.naa * c->d.natm, b_v = 16 * (size_t)ldv * nst;
+  const size_t b_s = evecsv ? 16 * (size_t)nst * nst : 0, b_o = 8 * (size_t)nst;
+  const size_t b_d = 16 * (size_t)ws * ws * c->d.natm;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_v = al(b_a), o_s = o_v + al(b_v), o_o = o_s + al(b_s), o_d = o_o + al(b_o);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_d + b_d))) return rc;
+  char *base = (char *)c->stage;
+  EXC_CHECK(hipMemcpyAsync(base, apwalm, b_a, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_v, evecfv, b_v, hipMemcpyHostToDevice, c->stream));
+  if (evecsv) EXC_CHECK(hipMemcpyAsync(base + o_s, evecsv, b_s, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_o, occ, b_o, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_d, dm, b_d, hipMemcpyHostToDevice, c->stream));
+  const double *pa = (const double *)base, *pv = (const double *)(base + o_v);
+  const double *ps = evecsv ? (const double *)(base + o_s) : nullptr, *po = (const double *)(base + o_o);
+  if ((rc = dmat_run(c, 1, &ngp, &ngkmax, &pa, nst, &ldv, &pv, &ps, &po, &wkpt, (double *)(base + o_d)))) return rc;
+  EXC_CHECK(hipMemcpyAsync(dm, base + o_d, b_d, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return EXC_OK;
+}
+
 int exc_eigen(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec, int ldv) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (nmat < 0 || ldh < (nmat > 1 ? nmat : 1) || nev < 0 || nev > nmat || ldv < (nmat > 1 ? nmat : 1)) return EXC_ERR_ARG;
@@ -784,6 +929,22 @@ int exc_ctx_set_lo(exc_ctx *c, const int *nlorb, const int *lorbl, const int *id
   }
   if (maxlos > c->dims.maxnlo) return EXC_ERR_ARG;
   c->d.nlotot = (int)lod.size();
+  // density-matrix tables: per atom (first lo, number of lo), and the row order f = (l, io, m)
+  std::vector<int> dlo(2 * (size_t)na, 0);
+  for (const lo_desc &q : lod) {
+    dlo[2 * q.ias] = q.first;
+    dlo[2 * q.ias + 1] += 1;
+  }
+  std::vector<int> dord;
+  for (int l = 0; l <= c->d.lmaxmat; ++l)
+    for (int io = 0; io < c->d.apwordmax; ++io)
+      for (int m = -l; m <= l; ++m) dord.push_back(io + c->d.apwordmax * (l * l + l + m));
+  if ((int)dord.size() != c->d.naa) return EXC_ERR_ARG;
+  if (!c->d.dm_lo && (rc = dev_alloc(c, &c->d.dm_lo, dlo.size()))) return rc;
+  if (!c->d.dm_order && (rc = dev_alloc(c, &c->d.dm_order, dord.size()))) return rc;
+  EXC_CHECK(hipMemcpy(c->d.dm_lo, dlo.data(), sizeof(int) * dlo.size(), hipMemcpyHostToDevice));
+  EXC_CHECK(hipMemcpy(c->d.dm_order, dord.data(), sizeof(int) * dord.size(), hipMemcpyHostToDevice));
+  c->d.dm_maxlo = maxlos;
   if (c->d.lod == nullptr && !lod.empty()) {
     if ((rc = dev_alloc(c, &c->d.lod, lod.size()))) return rc;
   }

@@ -65,6 +65,10 @@ struct exc_dev {
   double *ffacg;       // [nspecies][ngvec]
   int *kperm;          // [natm*naa]: summation order of the MT product (column of apwalm)
   int *fperm;          // [naa]: summation order inside one atom (Y stage)
+  // muffin-tin density matrix (gendmatmt.f90)
+  int dm_maxlo;        // largest number of lo functions of an atom (mt_dm%maxnlo)
+  int *dm_lo;          // [natm][2]: global index of the atom's first lo function, their number
+  int *dm_order;       // [naa]: apwalm storage column (io + apwordmax lm) of density-matrix row f
 };
 
 // Device-side description of one k-point of a grouped launch.
@@ -139,5 +143,10 @@ struct eig_kdesc {   // one k-point of a grouped solve: nmat n, device arrays
 int exc_launch_eig_hermfill(cplx *h, int n, int ldh, hipStream_t s);
 int exc_launch_eig_pad(const eig_kdesc *ks, int nb, int n, cplx *ph, cplx *po, double sigma, hipStream_t s);
 int exc_launch_eig_r2c(const double *zr, int m, int ncols, int ldr, cplx *zc, int ldc, hipStream_t s);
+// muffin-tin density matrix helpers (k_dmat.hip)
+int exc_launch_dmat_pack(const exc_dev &d, int ws, int nst, const cplx *w, const cplx *evec, int ldv, int ngp,
+                         cplx *x, int ldx, size_t xs, int col0, hipStream_t s);
+int exc_launch_dmat_scale(const exc_dev &d, int ws, int nst, const cplx *x, cplx *x2, int ldx, size_t xs, int col0,
+                          const double *occ, double wk, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

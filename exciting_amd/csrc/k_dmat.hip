@@ -1,0 +1,61 @@
+// Muffin-tin density matrix of the first-variational states (gendmatmt.f90:59-172, spinless):
+// per atom a and k-point, wf1 = [A_a^T C; C(lo rows of a)] and
+//     dm_a += w_k wf1 diag(occ) wf1^H.
+// The library GEMMs do the products (A_a^T C for all atoms in one strided batch per k-point;
+// the k-sum as one GEMM per atom whose contraction runs over the columns of all k-points);
+// these kernels only move data between them:
+//  * pack: rows of A_a^T C from apwalm's storage order (io, lm) into the reference's row order
+//    f = (l, io, m), m fastest (gendmatmt.f90:92-101), then the atom's local-orbital
+//    coefficients (:123-131), into column block col0.. of X;
+//  * scale: X2 = X diag(w_k occ) for one k-point's columns (wf2prime, :151-153).
+#include "exc_common.h"
+
+namespace {
+
+constexpr int kDT = 256;
+
+__global__ __launch_bounds__(kDT) void dmat_pack_kernel(int naa, int ws, const int *order, const int *lo,
+                                                        const cplx *w, int nst, const cplx *evec, int ldv,
+                                                        int ngp, cplx *x, int ldx, size_t xs, int col0) {
+  const int f = blockIdx.x * kDT + threadIdx.x, j = blockIdx.y, a = blockIdx.z;
+  if (f >= ws) return;
+  cplx v = {0.0, 0.0};
+  if (f < naa) {
+    v = w[(size_t)a * naa * nst + order[f] + (size_t)j * naa];
+  } else {
+    const int r = f - naa;
+    if (r < lo[2 * a + 1]) v = evec[ngp + lo[2 * a] + r + (size_t)j * ldv];
+  }
+  x[a * xs + f + (size_t)(col0 + j) * ldx] = v;
+}
+
+__global__ __launch_bounds__(kDT) void dmat_scale_kernel(int ws, const cplx *x, cplx *x2, int ldx, size_t xs,
+                                                         int col0, const double *occ, double wk) {
+  const int f = blockIdx.x * kDT + threadIdx.x, j = blockIdx.y, a = blockIdx.z;
+  if (f >= ws) return;
+  const double s = wk * occ[j];
+  const size_t i = a * xs + f + (size_t)(col0 + j) * ldx;
+  x2[i] = {x[i].re * s, x[i].im * s};
+}
+
+}  // namespace
+
+int exc_launch_dmat_pack(const exc_dev &d, int ws, int nst, const cplx *w, const cplx *evec, int ldv, int ngp,
+                         cplx *x, int ldx, size_t xs, int col0, hipStream_t s) {
+  if (nst <= 0) return EXC_OK;
+  if (nst > 65535 || d.natm > 65535 || ws < d.naa || ldx < ws) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(dmat_pack_kernel, dim3((ws + kDT - 1) / kDT, nst, d.natm), dim3(kDT), 0, s, d.naa, ws,
+                     d.dm_order, d.dm_lo, w, nst, evec, ldv, ngp, x, ldx, xs, col0);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_dmat_scale(const exc_dev &d, int ws, int nst, const cplx *x, cplx *x2, int ldx, size_t xs, int col0,
+                          const double *occ, double wk, hipStream_t s) {
+  if (nst <= 0) return EXC_OK;
+  if (nst > 65535 || d.natm > 65535 || ldx < ws) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(dmat_scale_kernel, dim3((ws + kDT - 1) / kDT, nst, d.natm), dim3(kDT), 0, s, ws, x, x2, ldx,
+                     xs, col0, occ, wk);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

@@ -81,10 +81,14 @@ SIGNATURES = {
     "exc_eigen_dev": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
     "exc_eigen": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
     "exc_eigen_batched_dev": [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
+    "exc_dmat_wfsize": [_P],
+    "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
+    "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
+    "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -272,6 +276,48 @@ class Context:
         return self._call("exc_eigen_batched_dev", self.h, nk, C.cast(ints, _P), C.cast(ints, _P),
                           C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
                           C.cast(ints, _P))
+
+    def dmat_wfsize(self):
+        """Row count of the muffin-tin density matrix (maxaa + largest lo count of an atom)."""
+        rc = self.lib.exc_dmat_wfsize(self.h)
+        if rc < 0:
+            raise ExcError("exc_dmat_wfsize", rc)
+        return rc
+
+    def gendmatmt(self, apwalm, ngp, evec, occ, wkpt, evecsv=None, dm=None):
+        """gendmatmt from host arrays: apwalm (ngkmax, apwordmax, lmmaxapw, natm) and evec
+        (ldv, nst) Fortran-ordered complex128, occ (nst,), evecsv (nst, nst) or None; dm
+        (wfsize, wfsize, natm) Fortran-ordered complex128 is accumulated (new when None)."""
+        ws = self.dmat_wfsize()
+        if dm is None:
+            dm = np.zeros((ws, ws, self.cr.natm), dtype=np.complex128, order="F")
+        for a in (apwalm, evec, dm) + ((evecsv,) if evecsv is not None else ()):
+            if a.dtype != np.complex128 or not a.flags.f_contiguous:
+                raise ValueError("gendmatmt: complex arrays must be Fortran-ordered complex128")
+        if dm.shape != (ws, ws, self.cr.natm):
+            raise ValueError("gendmatmt: dm must be (wfsize, wfsize, natm)")
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        nst = evec.shape[1]
+        self._call("exc_gendmatmt", self.h, ngp, apwalm.shape[0], _ptr(apwalm), nst, evec.shape[0], _ptr(evec),
+                   _ptr(evecsv) if evecsv is not None else None, _ptr(occ), float(wkpt), _ptr(dm))
+        return dm
+
+    def gendmatmt_dev(self, ngp, ngkmax, apwalm, nst, ldv, evec, occ, wkpt, dm, evecsv=None):
+        """Device version (torch tensors; dm (natm, wfsize, wfsize) row-major = Fortran (wfsize, wfsize, natm))."""
+        return self._call("exc_gendmatmt_dev", self.h, ngp, ngkmax, _ptr(apwalm), nst, ldv, _ptr(evec),
+                          _ptr(evecsv) if evecsv is not None else None, _ptr(occ), float(wkpt), _ptr(dm))
+
+    def gendmatmt_batched_dev(self, batch, nst, evecs, occs, wkpts, dm, evecsvs=None):
+        """exc_gendmatmt_batched_dev over a DeviceBatch: evecs[k] (nst, nmat_k) tensors (ldv = nmat_k)."""
+        nk = batch.nk
+        ldv = (C.c_int * nk)(*batch.nmat)
+        ev = (C.c_void_p * nk)(*[_ptr(t) for t in evecs])
+        oc = (C.c_void_p * nk)(*[_ptr(t) for t in occs])
+        wk = (C.c_double * nk)(*[float(x) for x in wkpts])
+        sv = (C.c_void_p * nk)(*[_ptr(t) if t is not None else None for t in evecsvs]) if evecsvs else None
+        return self._call("exc_gendmatmt_batched_dev", self.h, nk, C.cast(batch.desc, _P), nst, C.cast(ldv, _P),
+                          C.cast(ev, _P), C.cast(sv, _P) if sv is not None else None, C.cast(oc, _P),
+                          C.cast(wk, _P), _ptr(dm))
 
     # ------------------------------------------------------------------ device entry points
     def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):

@@ -201,6 +201,32 @@ int exc_eigen_batched_dev(exc_ctx *ctx, int nk, const int *nmat, const int *ldh,
                           double *const *o, int nev, double *const *eval, double *const *evec,
                           const int *ldv);
 
+/* Muffin-tin density matrix of the first-variational states, the counterpart of
+ * gendmatmt(ik, evecfv, evecsv) (gendmatmt.f90:13, called per k at scf_cycle.f90:351; the
+ * spinless part, ValenceRelativity-independent).  Accumulates, per atom a,
+ *   dm(:, :, a) += wkpt * wf diag(occ) wf^H,   wf = wf1 evecsv,
+ *   wf1(1:naa, j) = A_a^T evecfv(1:ngp, j), rows f = (l, io, m), m fastest (:92-121),
+ *   wf1(naa+1 : naa+losize, j) = evecfv(lo rows of a, j) (:123-131),
+ * dm(wfsize, wfsize, natmtot) as mt_dm%alpha%ff, wfsize = exc_dmat_wfsize(ctx) = maxaa +
+ * the largest number of lo functions of an atom (mod_potential_and_density.F90:186-230).
+ * apwalm(ngkmax, apwordmax, lmmaxapw, natmtot) as returned by exc_match; evecfv(ldv, nst)
+ * with ldv >= ngp + nlotot; evecsv(nst, nst) or NULL for the identity (first-variational
+ * states, tevecsv false); occ(nst) = occsv(:, ik).  exc_gendmatmt_dev: device arrays, on the
+ * context's stream.  exc_gendmatmt: host arrays (dm accumulated in host memory). */
+int exc_dmat_wfsize(const exc_ctx *ctx);
+int exc_gendmatmt_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, int nst, int ldv,
+                      const double *evecfv, const double *evecsv, const double *occ, double wkpt,
+                      double *dm);
+int exc_gendmatmt(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, int nst, int ldv,
+                  const double *evecfv, const double *evecsv, const double *occ, double wkpt, double *dm);
+/* The same for the nk k-points of a batch (apwalm, ngp, ngkmax from ks[k]; per k evecfv[k]
+ * with ldv[k], evecsv[k] (the array or its entries may be NULL), occ[k], wkpt[k]; pointer
+ * arrays in host memory, the arrays they point to on the device).  The sum over k-points
+ * runs as one product per atom over the columns of all k-points. */
+int exc_gendmatmt_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int nst, const int *ldv,
+                              const double *const *evecfv, const double *const *evecsv,
+                              const double *const *occ, const double *wkpt, double *dm);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
@@ -222,6 +248,7 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_VJP 8      /* exc_ho_vjp_dev                                                */
 #define EXC_STAGE_OLP 9      /* addition-theorem overlap (EXC_BATCH_OLP_LEGENDRE)             */
 #define EXC_STAGE_EIGEN 10   /* exc_eigen_dev                                                 */
+#define EXC_STAGE_DMAT 11    /* exc_gendmatmt_dev (muffin-tin density matrix)                 */
 #define EXC_NSTAGE 12
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 

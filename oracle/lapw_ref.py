@@ -13,6 +13,8 @@ Each function restates one reference routine, in numpy, on the arrays of
                           modfvsystem.f90:691-693) via LAPACK zhegv
 * ``forcek_ibs``        — forcek.f90:93-383 (IBS force term of one k)
 * ``setup_vjp``         — dE/dtau and dE/dV(G) of (H,O) for given cotangents
+* ``gendmatmt``         — gendmatmt.f90:59-172 (muffin-tin density matrix of one k,
+                          spinless first-variational part)
 
 Only ``tests/``, ``__graft_entry__.smoke`` and ``bench.py``'s CPU-baseline leg
 may import this module.  The library never calls it.
@@ -336,3 +338,48 @@ def setup_vjp(cr, kp, D, W, apwalm=None):
     ig = _gdiff_index(cr, kp)
     np.add.at(dV, ig.ravel(), D[:ngp, :ngp].T.ravel())
     return out, dV
+
+
+def dmat_wfsize(cr):
+    """wfsize = maxaa + maxnlo of the density matrix (mod_potential_and_density.F90:186-230:
+    maxaa counts l <= lmaxapw, maxnlo the largest losize)."""
+    return cr.apwordmax * cr.lmmaxapw + cr.maxnlo
+
+
+def dmat_order(cr):
+    """Storage index io + apwordmax*lm of apwalm's (io, lm) for each APW row f of the density
+    matrix, f = (l, io, m) with m fastest, l <= lmaxmat (gendmatmt.f90:92-101)."""
+    out = []
+    for l in range(cr.lmaxmat + 1):
+        for io in range(cr.apwordmax):
+            for m in range(-l, l + 1):
+                out.append(io + cr.apwordmax * (l * l + l + m))
+    return np.array(out, dtype=np.int64)
+
+
+def gendmatmt(cr, apwalm, ngp, evec, occ, wkpt, evecsv=None, dm=None):
+    """gendmatmt.f90:59-172, spinless: per atom
+        wf1(1:naa, :) = apwi evecfv(1:ngp, :)          (:92-121, apwi rows f = (l, io, m))
+        wf1(maxaa+1 : maxaa+losize, :) = evecfv(lo rows) (:123-131)
+        wfalpha = wf1 evecsv                            (:134-147; identity when None)
+        dm(:, :, a) += wkpt wfalpha diag(occ) wfalpha^H (:151-172)
+    Rows naa+1..maxaa (l > lmaxmat) are zero.  evec: (>= nmat, nst); occ: (nst,)."""
+    ws = dmat_wfsize(cr)
+    maxaa = cr.apwordmax * cr.lmmaxapw
+    if dm is None:
+        dm = np.zeros((ws, ws, cr.natm), dtype=np.complex128, order="F")
+    order = dmat_order(cr)
+    nst = evec.shape[1]
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        a = apwalm[:ngp, :, :, ias].reshape(ngp, -1, order="F")      # (G, io + apwordmax lm)
+        wf1 = np.zeros((ws, nst), dtype=np.complex128)
+        wf1[:len(order)] = a[:, order].T @ evec[:ngp]
+        if cr.nlorb[is_] != 0:
+            ns = cr.losize[is_]
+            l1 = cr.lorbl[0, is_]
+            j1 = ngp + cr.idxlo[l1 * l1, 0, ias] - 1                    # idxlm(l1, -l1)
+            wf1[maxaa:maxaa + ns] = evec[j1:j1 + ns]
+        wfa = wf1 if evecsv is None else wf1 @ evecsv
+        dm[:, :, ias] += wkpt * (wfa * occ[None, :]) @ wfa.conj().T
+    return dm
