@@ -630,11 +630,13 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
 
 namespace {
 
-// dm(ws, ws, natm) += sum over the nk k-points of w_k wf1 diag(occ) wf1^H (gendmatmt.f90:59-172):
-// per k one strided batch over atoms of A_a^T C (rows in apwalm's storage order), packed into
-// the reference's row order with the lo rows appended (times evecsv when given), and the
-// occupation-weighted copy; then per atom one GEMM whose contraction runs over the nst
-// columns of every k-point, accumulated into dm.  All arrays are device memory.
+// dm(ws, ws, natm) += sum over the nk k-points of w_k wf1 diag(occ) wf1^H (gendmatmt.f90:59-172).
+// The k-points are split into contiguous ranges, one per worker stream (the products of one
+// k-point fill only about half of the GPU: nst columns).  Per k: one product A^T C for all
+// atoms (rows in apwalm's storage order), packed into the reference's row order with the lo
+// rows appended (times evecsv when given) and the occupation-weighted copy.  Per worker: one
+// product per atom whose contraction runs over the columns of its k-points, into a partial
+// sum; the partials are added into dm on the context's stream.  All arrays are device memory.
 int dmat_run(exc_ctx *c, int nk, const int *ngp, const int *ldg, const double *const *apwalm, int nst,
              const int *ldv, const double *const *evec, const double *const *evecsv, const double *const *occ,
              const double *wkpt, double *dm) {
@@ -642,59 +644,74 @@ int dmat_run(exc_ctx *c, int nk, const int *ngp, const int *ldg, const double *c
   const int ws = d.naa + d.dm_maxlo, na = d.natm, naa = d.naa;
   const size_t nt = (size_t)nk * nst;
   if (nt > (size_t)0x7fffffff) return EXC_ERR_ARG;
+  const int nw = std::min(nk, (int)exc_ctx::kEigStreams);
   bool any_sv = false;
   for (int k = 0; k < nk; ++k) any_sv = any_sv || (evecsv && evecsv[k]);
-  const size_t b_w = 16 * (size_t)na * naa * nst, b_x = 16 * (size_t)na * ws * nt;
-  const size_t b_t = any_sv ? 16 * (size_t)na * ws * nst : 0;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_w = al(16 * (size_t)na * naa * nst), b_x = al(16 * (size_t)na * ws * nt);
+  const size_t b_t = any_sv ? al(16 * (size_t)na * ws * nst) : 0, b_p = al(16 * (size_t)na * ws * ws);
   int rc;
-  if ((rc = grow(&c->dmw, &c->dmw_bytes, al(b_w) + 2 * al(b_x) + b_t))) return rc;
+  if ((rc = grow(&c->dmw, &c->dmw_bytes, nw * (b_w + b_t + b_p) + 2 * b_x))) return rc;
   char *base = (char *)c->dmw;
-  cplx *w = (cplx *)base, *x = (cplx *)(base + al(b_w)), *x2 = (cplx *)(base + al(b_w) + al(b_x));
-  cplx *xt = (cplx *)(base + al(b_w) + 2 * al(b_x));
-  exc_ctx::EigWorker &W = c->ew[0];
-  W.s = c->stream;
-  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
-    W.blas = nullptr;
-    return EXC_ERR_HIP;
+  cplx *x = (cplx *)base, *x2 = (cplx *)(base + b_x);
+  char *per = base + 2 * b_x;   // per worker: W, Xt, partial
+  if (!c->eig_fork) EXC_CHECK(hipEventCreateWithFlags(&c->eig_fork, hipEventDisableTiming));
+  for (int i = 1; i <= nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[i];
+    if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
+    if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
+    if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+      W.blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+    if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
   }
-  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
   const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
   const size_t xs = (size_t)ws * nt;
   StageTimer t(c, EXC_STAGE_DMAT);
-  for (int k = 0; k < nk; ++k) {
-    // W_a = A_a^T C_k (naa x nst per atom, rows io + apwordmax lm)
-    if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_transpose, rocblas_operation_none, naa, nst, ngp[k],
-                                      &one, (const rocblas_double_complex *)apwalm[k], ldg[k],
-                                      (rocblas_stride)ldg[k] * naa, (const rocblas_double_complex *)evec[k], ldv[k], 0,
-                                      &zero, (rocblas_double_complex *)w, naa, (rocblas_stride)naa * nst,
-                                      na) != rocblas_status_success)
-      return EXC_ERR_HIP;
-    const bool sv = evecsv && evecsv[k];
-    if (sv) {
-      if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], xt, ws,
-                                     (size_t)ws * nst, 0, c->stream)))
-        return rc;
-      if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_none, ws, nst, nst, &one,
-                                        (const rocblas_double_complex *)xt, ws, (rocblas_stride)ws * nst,
-                                        (const rocblas_double_complex *)evecsv[k], nst, 0, &zero,
-                                        (rocblas_double_complex *)(x + (size_t)k * nst * ws), ws, (rocblas_stride)xs,
-                                        na) != rocblas_status_success)
+  EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
+  for (int i = 0; i < nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[1 + i];
+    EXC_CHECK(hipStreamWaitEvent(W.s, c->eig_fork, 0));
+    cplx *w = (cplx *)(per + i * (b_w + b_t + b_p)), *xt = (cplx *)((char *)w + b_w);
+    cplx *part = (cplx *)((char *)w + b_w + b_t);
+    const int k0 = (int)((long)nk * i / nw), k1 = (int)((long)nk * (i + 1) / nw);
+    for (int k = k0; k < k1; ++k) {
+      // W = A^T C_k for all atoms at once (the atoms' columns of apwalm are contiguous):
+      // (natm naa) x nst, row a naa + io + apwordmax lm
+      if (rocblas_zgemm(W.blas, rocblas_operation_transpose, rocblas_operation_none, na * naa, nst, ngp[k], &one,
+                        (const rocblas_double_complex *)apwalm[k], ldg[k], (const rocblas_double_complex *)evec[k],
+                        ldv[k], &zero, (rocblas_double_complex *)w, na * naa) != rocblas_status_success)
         return EXC_ERR_HIP;
-    } else if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], x, ws, xs,
-                                          k * nst, c->stream))) {
-      return rc;
+      if (evecsv && evecsv[k]) {
+        if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], xt, ws,
+                                       (size_t)ws * nst, 0, W.s)))
+          return rc;
+        if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_none, ws, nst, nst, &one,
+                                          (const rocblas_double_complex *)xt, ws, (rocblas_stride)ws * nst,
+                                          (const rocblas_double_complex *)evecsv[k], nst, 0, &zero,
+                                          (rocblas_double_complex *)(x + (size_t)k * nst * ws), ws,
+                                          (rocblas_stride)xs, na) != rocblas_status_success)
+          return EXC_ERR_HIP;
+      } else if ((rc = exc_launch_dmat_pack(d, ws, nst, w, (const cplx *)evec[k], ldv[k], ngp[k], x, ws, xs,
+                                            k * nst, W.s))) {
+        return rc;
+      }
+      if ((rc = exc_launch_dmat_scale(d, ws, nst, x, x2, ws, xs, k * nst, occ[k], wkpt[k], W.s))) return rc;
     }
-    if ((rc = exc_launch_dmat_scale(d, ws, nst, x, x2, ws, xs, k * nst, occ[k], wkpt[k], c->stream))) return rc;
+    // part_a = X_a X2_a^H over this worker's columns
+    const size_t c0 = (size_t)k0 * nst * ws;
+    if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, ws, ws,
+                                      (k1 - k0) * nst, &one, (const rocblas_double_complex *)(x + c0), ws,
+                                      (rocblas_stride)xs, (const rocblas_double_complex *)(x2 + c0), ws,
+                                      (rocblas_stride)xs, &zero, (rocblas_double_complex *)part, ws,
+                                      (rocblas_stride)ws * ws, na) != rocblas_status_success)
+      return EXC_ERR_HIP;
+    EXC_CHECK(hipEventRecord(W.done, W.s));
+    EXC_CHECK(hipStreamWaitEvent(c->stream, W.done, 0));
   }
-  // dm_a += X_a X2_a^H
-  if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_none, rocblas_operation_conjugate_transpose, ws, ws,
-                                    (int)nt, &one, (const rocblas_double_complex *)x, ws, (rocblas_stride)xs,
-                                    (const rocblas_double_complex *)x2, ws, (rocblas_stride)xs, &one,
-                                    (rocblas_double_complex *)dm, ws, (rocblas_stride)ws * ws,
-                                    na) != rocblas_status_success)
-    return EXC_ERR_HIP;
-  return EXC_OK;
+  return exc_launch_dmat_reduce((cplx *)dm, (const cplx *)per, nw, (b_w + b_t + b_p) / sizeof(cplx),
+                                (b_w + b_t) / sizeof(cplx), (size_t)na * ws * ws, c->stream);
 }
 
 bool dmat_args_ok(const exc_ctx *c, int ngp, int ngkmax, int nst, int ldv) {

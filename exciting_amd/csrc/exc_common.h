@@ -148,5 +148,7 @@ int exc_launch_dmat_pack(const exc_dev &d, int ws, int nst, const cplx *w, const
                          cplx *x, int ldx, size_t xs, int col0, hipStream_t s);
 int exc_launch_dmat_scale(const exc_dev &d, int ws, int nst, const cplx *x, cplx *x2, int ldx, size_t xs, int col0,
                           const double *occ, double wk, hipStream_t s);
+int exc_launch_dmat_reduce(cplx *dm, const cplx *per, int np, size_t pstride, size_t poff, size_t n,
+                           hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);
