@@ -1,7 +1,8 @@
 ! Test program for the Fortran driver: reads one k-point's inputs from a stream file,
 ! runs match and hamiltonandoverlapsetup through the GPU driver as seceqn.f90:57 /
 ! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
-! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot); with a fourth
+! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot), then
+! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot); with a fourth
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
 ! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).
 !   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev]
@@ -21,8 +22,9 @@ program exc_driver_main
   integer :: hdr(18), nst, u2, nev_solve
   complex(8), allocatable :: evec(:, :)
   real(8), allocatable :: evalfv(:), occ(:), forceibs(:, :), evs(:)
-  complex(8), allocatable :: vecs(:, :)
+  complex(8), allocatable :: vecs(:, :), dm(:, :, :)
   real(8) :: wkpt
+  integer :: ws
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -81,6 +83,11 @@ program exc_driver_main
     forceibs = 0.d0
     call forcek_gpu(ngp, apwalm, igpig, vgpc, sfacgp, nmat, nst, evec, evalfv, occ, wkpt, forceibs)
     write(u) forceibs
+    ws = exc_gpu_dmat_wfsize()
+    allocate(dm(ws, ws, dims%natmtot))
+    dm = (0.d0, 0.d0)
+    call gendmatmt_gpu(ngp, apwalm, nmat, nst, evec, occ, wkpt, dm)
+    write(u) dm
   end if
   if (nev_solve > 0) then
     allocate(evs(nev_solve), vecs(nmat, nev_solve))

@@ -100,6 +100,22 @@ module exc_lapw_c
       real(c_double), intent(inout) :: eval(*)
     end function
 
+    integer(c_int) function exc_dmat_wfsize(ctx) bind(c, name="exc_dmat_wfsize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function exc_gendmatmt(ctx, ngp, ngkmax, apwalm, nst, ldv, evecfv, evecsv, occ, wkpt, dm) &
+        bind(c, name="exc_gendmatmt")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx, evecsv
+      integer(c_int), value :: ngp, ngkmax, nst, ldv
+      complex(c_double_complex), intent(in) :: apwalm(*), evecfv(*)
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), value :: wkpt
+      complex(c_double_complex), intent(inout) :: dm(*)
+    end function
+
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

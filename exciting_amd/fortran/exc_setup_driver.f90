@@ -3,6 +3,8 @@
 !   match(ngp, gpc, tpgpc, sfacgp, apwalm)                        match.f90:13
 !   hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)     hamiltonandoverlapsetup.f90:8
 !   solvewithgpu(system, nstfv, evecfv, evalfv)                   modfvsystem.f90:585 (solvewithlapack)
+!   gendmatmt_gpu(ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, dm [, evecsv])
+!                                                                 gendmatmt.f90:13
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -21,7 +23,7 @@ module exc_setup_gpu
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
-            solvewithgpu
+            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize
 
 contains
 
@@ -148,5 +150,31 @@ contains
       error stop 1
     end if
   end subroutine solvewithgpu
+
+  ! Rows (and columns) of the muffin-tin density matrix, mt_dm%maxaa + mt_dm%maxnlo
+  integer function exc_gpu_dmat_wfsize()
+    exc_gpu_dmat_wfsize = int(exc_dmat_wfsize(ctx))
+    if (exc_gpu_dmat_wfsize < 0) call check(int(exc_gpu_dmat_wfsize, c_int), "exc_dmat_wfsize")
+  end function exc_gpu_dmat_wfsize
+
+  ! gendmatmt.f90:13 (gendmatmt(ik, evecfv, evecsv), the spinless part) with the module state it
+  ! reads passed in: apwalm of the k-point (match), wkpt(ik), occsv(:, ik).  Adds to
+  ! dm(wfsize, wfsize, natmtot) (mt_dm%alpha%ff), per atom a,
+  !   dm(:, :, a) += wkpt * wf diag(occ) wf^H,  wf = [A_a^T evecfv; evecfv(lo rows of a)] evecsv
+  ! with evecsv(nstfv, nstfv) the identity when absent.
+  subroutine gendmatmt_gpu(ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, dm, evecsv)
+    integer, intent(in) :: ngp, nmatmax, nstfv
+    complex(8), intent(in) :: apwalm(ngkmax_, *), evecfv(nmatmax, nstfv)
+    real(8), intent(in) :: occ(nstfv), wkpt
+    complex(8), intent(inout) :: dm(:, :, :)
+    complex(8), intent(in), target, optional :: evecsv(nstfv, nstfv)
+    type(c_ptr) :: psv
+    if (size(dm, 1) /= exc_gpu_dmat_wfsize() .or. size(dm, 2) /= size(dm, 1) .or. size(dm, 3) /= natmtot_) &
+      error stop "gendmatmt_gpu: dm must be (wfsize, wfsize, natmtot)"
+    psv = c_null_ptr
+    if (present(evecsv)) psv = c_loc(evecsv)
+    call check(exc_gendmatmt(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, int(nstfv, c_int), &
+                             int(nmatmax, c_int), evecfv, psv, occ, wkpt, dm), "gendmatmt")
+  end subroutine gendmatmt_gpu
 
 end module exc_setup_gpu

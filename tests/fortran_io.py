@@ -58,3 +58,13 @@ def read_eigen_output(path, cr, ngkmax, nev):
         e = np.fromfile(f, np.float64, nev)
         C = np.fromfile(f, np.complex128, nmat * nev).reshape((nmat, nev), order="F")
     return e, C
+
+
+def read_dmat(path, cr, ngkmax, ws):
+    """dm(wfsize, wfsize, natm) appended by the driver after forceibs (gendmatmt_gpu)."""
+    with open(path, "rb") as f:
+        nmat = int(np.fromfile(f, np.int32, 1)[0])
+        skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat
+        np.fromfile(f, np.complex128, skip)
+        np.fromfile(f, np.float64, 3 * cr.natm)
+        return np.fromfile(f, np.complex128, ws * ws * cr.natm).reshape((ws, ws, cr.natm), order="F")

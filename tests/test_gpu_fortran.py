@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import read_eigen_output, read_forces, read_output, write_eigen, write_input
+from fortran_io import read_dmat, read_eigen_output, read_forces, read_output, write_eigen, write_input
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,8 @@ def test_fortran_driver(tmp_path, which):
 
 
 def test_fortran_forcek(tmp_path):
-    """forcek_gpu (Fortran, through exc_ho_vjp) against the restated reference forcek."""
+    """forcek_gpu (Fortran, through exc_ho_vjp) against the restated reference forcek, and
+    gendmatmt_gpu (through exc_gendmatmt) against the restated gendmatmt."""
     from oracle import lapw_ref as R
     w = small("b")
     cr, kp = w.crystal, w.kpoints[0]
@@ -60,6 +61,13 @@ def test_fortran_forcek(tmp_path):
     F_ref = R.forcek_ibs(cr, kp, C, e, occ, apwalm=A, wkpt=wk)
     print("forceibs max |diff|", np.abs(F - F_ref).max(), "max |F|", np.abs(F_ref).max())
     assert np.abs(F - F_ref).max() <= 1e-9 * max(1.0, np.abs(F_ref).max())
+    # gendmatmt_gpu of the same states, on the driver's own apwalm
+    apw, _, _ = read_output(fout, cr, kp.igpig.shape[0])
+    dm = read_dmat(fout, cr, kp.igpig.shape[0], R.dmat_wfsize(cr))
+    dm_ref = R.gendmatmt(cr, apw, kp.ngp, C, occ, wk)
+    rel = np.linalg.norm((dm - dm_ref).ravel()) / np.linalg.norm(dm_ref.ravel())
+    print("gendmatmt_gpu rel diff", rel)
+    assert rel <= 1e-12
 
 
 @pytest.mark.parametrize("which", ["small-b", "config0"])
