@@ -515,25 +515,35 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   if (st != rocblas_status_success) return fail("trtri / trmm / zhetrd strided_batched status", (int)st, n);
   EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
   for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
-  for (int b = 0; b < nb; ++b) {
+  // per k-point on the workers; every worker is joined below, also after an error
+  int err = EXC_OK;
+  for (int b = 0; b < nb && !err; ++b) {
     exc_ctx::EigWorker &W = c->ew[1 + b % nw];
     const int m = kd[b].n;
     double *db = d + (size_t)b * n, *eb = e + (size_t)b * n;
     st = rocsolver_dstedc(W.blas, rocblas_evect_tridiagonal, m, db, eb, (double *)W.z, m, info + 2 * nb + b);
-    if (st != rocblas_status_success) return fail("dstedc status", (int)st, b);
-    if ((rc = exc_launch_eig_r2c((const double *)W.z, m, nev, m, W.zc, m, W.s))) return rc;
+    if (st != rocblas_status_success) {
+      err = fail("dstedc status", (int)st, b);
+      break;
+    }
+    if ((err = exc_launch_eig_r2c((const double *)W.z, m, nev, m, W.zc, m, W.s))) break;
     st = rocsolver_zunmtr(W.blas, rocblas_side_left, up, rocblas_operation_none, m, nev, A + (size_t)b * mat, n,
                           (rocblas_double_complex *)(tau + (size_t)b * n), (rocblas_double_complex *)W.zc, m);
     if (st == rocblas_status_success)
       st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, nu, m, nev, &one, B + (size_t)b * mat,
                          n, (rocblas_double_complex *)W.zc, m, (rocblas_double_complex *)kd[b].evec, kd[b].ldv);
-    if (st != rocblas_status_success) return fail("zunmtr / trmm status", (int)st, b);
-    EXC_CHECK(hipMemcpyAsync(kd[b].eval, db, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
+    if (st != rocblas_status_success) {
+      err = fail("zunmtr / trmm status", (int)st, b);
+      break;
+    }
+    if (hipMemcpyAsync(kd[b].eval, db, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s) != hipSuccess)
+      err = EXC_ERR_HIP;
   }
   for (int i = 1; i <= nw; ++i) {
     EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
     EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
   }
+  if (err) return err;
   EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   for (int b = 0; b < nb; ++b) {
@@ -607,18 +617,18 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
         for (int k = 0; k < nk; ++k)
           if (nmat[k] > 0 && hp[4 * (size_t)k + 1] > 0) return nmat[k] + hp[4 * (size_t)k + 1];
       }
-      for (int k = 0; k < nk; ++k) {
+      for (int k = 0; k < nk && !rc; ++k) {
         if (nmat[k] == 0) continue;
         exc_ctx::EigWorker &W = c->ew[1 + k % nw];
-        if ((rc = eig_enqueue(W, v, phase, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k],
-                              c->eig_int + 4 * k)))
-          return rc;
+        rc = eig_enqueue(W, v, phase, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k], c->eig_int + 4 * k);
       }
+      if (rc) break;
     }
-    for (int i = 1; i <= nw; ++i) {
+    for (int i = 1; i <= nw; ++i) {   // joined also after an error
       EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
       EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
     }
+    if (rc) return rc;
   }
   std::vector<int> hi(4 * (size_t)nk, 0);
   EXC_CHECK(hipMemcpyAsync(hi.data(), c->eig_int, sizeof(int) * 4 * nk, hipMemcpyDeviceToHost, c->stream));
@@ -670,12 +680,12 @@ int dmat_run(exc_ctx *c, int nk, const int *ngp, const int *ldg, const double *c
   const size_t xs = (size_t)ws * nt;
   StageTimer t(c, EXC_STAGE_DMAT);
   EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
-  for (int i = 0; i < nw; ++i) {
-    exc_ctx::EigWorker &W = c->ew[1 + i];
-    EXC_CHECK(hipStreamWaitEvent(W.s, c->eig_fork, 0));
+  // a worker's enqueue; every forked worker is joined below, also after an error
+  auto work = [&](exc_ctx::EigWorker &W, int i) -> int {
     cplx *w = (cplx *)(per + i * (b_w + b_t + b_p)), *xt = (cplx *)((char *)w + b_w);
     cplx *part = (cplx *)((char *)w + b_w + b_t);
     const int k0 = (int)((long)nk * i / nw), k1 = (int)((long)nk * (i + 1) / nw);
+    int rc;
     for (int k = k0; k < k1; ++k) {
       // W = A^T C_k for all atoms at once (the atoms' columns of apwalm are contiguous):
       // (natm naa) x nst, row a naa + io + apwordmax lm
@@ -707,9 +717,17 @@ int dmat_run(exc_ctx *c, int nk, const int *ngp, const int *ldg, const double *c
                                       (rocblas_stride)xs, &zero, (rocblas_double_complex *)part, ws,
                                       (rocblas_stride)ws * ws, na) != rocblas_status_success)
       return EXC_ERR_HIP;
+    return EXC_OK;
+  };
+  int err = EXC_OK;
+  for (int i = 0; i < nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[1 + i];
+    EXC_CHECK(hipStreamWaitEvent(W.s, c->eig_fork, 0));
+    if (!err) err = work(W, i);
     EXC_CHECK(hipEventRecord(W.done, W.s));
     EXC_CHECK(hipStreamWaitEvent(c->stream, W.done, 0));
   }
+  if (err) return err;
   return exc_launch_dmat_reduce((cplx *)dm, (const cplx *)per, nw, (b_w + b_t + b_p) / sizeof(cplx),
                                 (b_w + b_t) / sizeof(cplx), (size_t)na * ws * ws, c->stream);
 }
