@@ -455,3 +455,61 @@ def kpoint_at(kp: KPoint, cr: Crystal) -> KPoint:
     sf[:n] = structure_factors(kp.vgpc[:, :n], cr.atposc)
     out.sfacgp = sf
     return out
+
+
+@dataclasses.dataclass
+class RadialData:
+    """Muffin-tin radial data of one SCF iteration (the inputs of hmlint and olprad)."""
+    lmaxvr: int
+    nrmt: np.ndarray          # (nspecies,) int32
+    spr: np.ndarray           # (nrmtmax, nspecies) radial mesh, Fortran order
+    apwfr: np.ndarray         # (nrmtmax, 2, apwordmax, lmaxapw+1, natm): u, du/dr
+    lofr: np.ndarray          # (nrmtmax, 2, nlomax, natm): u, du/dr
+    veffmt: np.ndarray        # (lmmaxvr, nrmtmax, natm): coefficients of the real harmonics
+
+
+def make_radial(cr: Crystal, nr=(300,), lmaxvr=6, seed=0, rmin=1e-5) -> RadialData:
+    """Seeded radial data: per species a logarithmic mesh of nr[is] points from rmin to R
+    (the form of genrmesh.f90), radial functions u = r^l (1 + a r + b r^2) e^(-c r) with
+    a, b in U(-0.5, 0.5), c in U(0.5, 1.5) per function and atom (derivative analytic), and a
+    potential whose spherical part is (-Z/r + d e^(-r)) sqrt(4 pi) (Z in U(5, 30) per species)
+    and whose lm > 0 coefficients are N(0, 0.3) r^2 e^(-r)."""
+    rng = np.random.default_rng(20261202 + 77 + seed)
+    ns = cr.nspecies
+    nrs = np.array([nr[min(i, len(nr) - 1)] for i in range(ns)], dtype=np.int32)
+    nmax = int(nrs.max())
+    spr = np.zeros((nmax, ns), order="F")
+    for i in range(ns):
+        n = int(nrs[i])
+        t = math.log(cr.rmt[i] / rmin) / (n - 1)
+        spr[:n, i] = rmin * np.exp(np.arange(n) * t)
+        spr[n - 1, i] = cr.rmt[i]
+
+    def fn(r, l):
+        a, b = rng.uniform(-0.5, 0.5, 2)
+        c = rng.uniform(0.5, 1.5)
+        p = 1.0 + a * r + b * r * r
+        dp = a + 2.0 * b * r
+        e = np.exp(-c * r)
+        u = r ** l * p * e
+        du = (l * r ** (l - 1) if l > 0 else 0.0) * p * e + r ** l * (dp - c * p) * e
+        return u, du
+
+    apwfr = np.zeros((nmax, 2, cr.apwordmax, cr.lmaxapw + 1, cr.natm), order="F")
+    lofr = np.zeros((nmax, 2, max(cr.nlomax, 1), cr.natm), order="F")
+    lmv = (lmaxvr + 1) ** 2
+    veffmt = np.zeros((lmv, nmax, cr.natm), order="F")
+    zs = rng.uniform(5.0, 30.0, ns)
+    for a in range(cr.natm):
+        is_ = int(cr.spidx[a])
+        n = int(nrs[is_])
+        r = spr[:n, is_]
+        for l in range(cr.lmaxapw + 1):
+            for io in range(cr.apwordmax):
+                apwfr[:n, 0, io, l, a], apwfr[:n, 1, io, l, a] = fn(r, l)
+        for ilo in range(int(cr.nlorb[is_])):
+            lofr[:n, 0, ilo, a], lofr[:n, 1, ilo, a] = fn(r, int(cr.lorbl[ilo, is_]))
+        d = rng.uniform(-2.0, 2.0)
+        veffmt[0, :n, a] = (-zs[is_] / r + d * np.exp(-r)) * math.sqrt(4.0 * math.pi)
+        veffmt[1:, :n, a] = rng.normal(0.0, 0.3, (lmv - 1, 1)) * (r * r * np.exp(-r))[None, :]
+    return RadialData(lmaxvr=lmaxvr, nrmt=nrs, spr=spr, apwfr=apwfr, lofr=lofr, veffmt=veffmt)

@@ -1,9 +1,11 @@
-"""Writes tests/golden/specfun_ref.npz from the reference's own Fortran special functions.
+"""Writes tests/golden/specfun_ref.npz and hmlint_ref.npz from the reference's own Fortran routines.
 
 The reference's src/src_lib/{sbessel,sbesseldm,factnm,factr,genylm,sphcrd}.f90 are compiled
 (where they lie in the reference checkout) by oracle/Makefile into oracle/_ref/libexcref.so;
 this script evaluates j_l, j_l' (sbesseldm with m = 0, 1) and Y_lm (genylm) on fixed inputs
-and stores inputs and outputs.  Run from the repository root:
+and stores inputs and outputs; likewise the spline integral (fderiv, m = -1), Gaunt
+coefficients (gaunt) and real spherical harmonics (genrlm) on the inputs of
+test_oracle_hmlint.fixture_inputs.  Run from the repository root:
 
     make -C oracle && python tests/golden/make_golden.py
 """
@@ -18,6 +20,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 sys.path.insert(0, os.path.dirname(HERE))
 
 from test_oracle_specfun import XS, _reflib, ref_genylm, ref_sbesseldm  # noqa: E402
+import test_oracle_hmlint as H  # noqa: E402
 
 
 def main():
@@ -33,6 +36,13 @@ def main():
     ylm = np.array([ref_genylm(lib, lmax, t, p) for t, p in tp])
     np.savez_compressed(os.path.join(HERE, "specfun_ref.npz"), x=xs, tp=tp, lmax=lmax, jl=jl, djl=djl, ylm=ylm)
     print("wrote", os.path.join(HERE, "specfun_ref.npz"))
+    hl = H._reflib()
+    meshes, funcs, args, tps = H.fixture_inputs()
+    ints = np.array([H.ref_fderiv_int(hl, x, f) for x, f in zip(meshes, funcs)])
+    gaunts = np.array([H.ref_gaunt(hl, *map(int, a)) for a in args])
+    rlms = np.array([H.ref_genrlm(hl, 8, t, p) for t, p in tps])
+    np.savez_compressed(os.path.join(HERE, "hmlint_ref.npz"), ints=ints, gaunt=gaunts, rlm=rlms)
+    print("wrote", os.path.join(HERE, "hmlint_ref.npz"))
 
 
 if __name__ == "__main__":
