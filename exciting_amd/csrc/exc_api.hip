@@ -86,6 +86,14 @@ struct exc_ctx {
   size_t eg_bytes;
   void *dmw;                // density matrix: A^T C, packed and weighted wavefunction blocks
   size_t dmw_bytes;
+  // muffin-tin integrals (exc_ctx_hmlint): host copies of nlorb / lorbl, the Gaunt table of
+  // gnt_lmaxvr (device), radial inputs + integrals (device)
+  std::vector<int> h_nlorb, h_lorbl;
+  cplx *gnt;
+  size_t gnt_bytes;
+  int gnt_lmaxvr;
+  void *hml;
+  size_t hml_bytes;
   hipEvent_t eig_fork;
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
@@ -272,6 +280,7 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->bmt_variant = -1;
   c->bsplit_variant = -1;
   c->eig_variant = -1;
+  c->gnt_lmaxvr = -1;
   {
     hipDeviceProp_t prop;
     c->ncu = (hipGetDeviceProperties(&prop, c->device) == hipSuccess) ? prop.multiProcessorCount : 256;
@@ -300,6 +309,8 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->eig_int) hipFree(c->eig_int);
   if (c->eg) hipFree(c->eg);
   if (c->dmw) hipFree(c->dmw);
+  if (c->gnt) hipFree(c->gnt);
+  if (c->hml) hipFree(c->hml);
   if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -980,6 +991,10 @@ int exc_ctx_set_lo(exc_ctx *c, const int *nlorb, const int *lorbl, const int *id
   EXC_CHECK(hipMemcpy(c->d.dm_lo, dlo.data(), sizeof(int) * dlo.size(), hipMemcpyHostToDevice));
   EXC_CHECK(hipMemcpy(c->d.dm_order, dord.data(), sizeof(int) * dord.size(), hipMemcpyHostToDevice));
   c->d.dm_maxlo = maxlos;
+  c->h_nlorb.assign(nlorb, nlorb + ns);
+  c->h_lorbl.assign((size_t)nlomax * ns, 0);
+  for (int is = 0; is < ns; ++is)
+    for (int ilo = 0; ilo < nlorb[is]; ++ilo) c->h_lorbl[ilo + (size_t)nlomax * is] = lorbl[ilo + nlomax * is];
   if (c->d.lod == nullptr && !lod.empty()) {
     if ((rc = dev_alloc(c, &c->d.lod, lod.size()))) return rc;
   }
@@ -1034,6 +1049,83 @@ int exc_ctx_set_mt(exc_ctx *c, const double *haa, const double *loa, const doubl
     if ((rc = dev_upload(c, &c->d.oalo, oalo, ao * nl * na))) return rc;
     if ((rc = dev_upload(c, &c->d.ololo, ololo, nl * nl * na))) return rc;
   }
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->have_mt = true;
+  return EXC_OK;
+}
+
+int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const double *spr, const double *apwfr,
+                   const double *lofr, const double *veffmt, double *haa, double *loa, double *lolo, double *oalo,
+                   double *ololo) {
+  if (!c || !nrmt || !spr || !apwfr || !veffmt) return EXC_ERR_ARG;
+  if (!c->have_atoms || !c->have_lo) return EXC_ERR_STATE;
+  const exc_dev &d = c->d;
+  const int ns = c->dims.nspecies, na = d.natm, nlomax = c->dims.nlomax, ao = d.apwordmax;
+  if (nlomax > 0 && !lofr) return EXC_ERR_ARG;
+  if (lmaxvr < 0 || lmaxvr > 2 * d.lmaxmat || nrmtmax < 4 || nrmtmax > 8192) return EXC_ERR_ARG;
+  for (int is = 0; is < ns; ++is) {
+    if (nrmt[is] < 4 || nrmt[is] > nrmtmax) return EXC_ERR_ARG;
+    const double *x = spr + (size_t)nrmtmax * is;
+    if (!(x[0] > 0.0)) return EXC_ERR_ARG;
+    for (int i = 1; i < nrmt[is]; ++i)
+      if (!(x[i] > x[i - 1])) return EXC_ERR_ARG;
+    for (int ilo = 0; ilo < c->h_nlorb[is]; ++ilo)
+      if (c->h_lorbl[ilo + (size_t)nlomax * is] > d.lmaxmat) return EXC_ERR_UNSUPPORTED;
+  }
+  int rc = use_device(c);
+  if (rc) return rc;
+  // integration weights of each species' mesh, the Gaunt table (per lmaxvr)
+  std::vector<double> wt((size_t)nrmtmax * ns, 0.0);
+  for (int is = 0; is < ns; ++is) exc_hml_spline_weights(nrmt[is], spr + (size_t)nrmtmax * is, &wt[(size_t)nrmtmax * is]);
+  const int lmm = (d.lmaxmat + 1) * (d.lmaxmat + 1), lmv = (lmaxvr + 1) * (lmaxvr + 1);
+  if (c->gnt_lmaxvr != lmaxvr) {
+    std::vector<cplx> g((size_t)lmm * lmm * (lmaxvr + 1) * 2);
+    exc_hml_gaunt_table(d.lmaxmat, lmaxvr, g.data());
+    if ((rc = grow((void **)&c->gnt, &c->gnt_bytes, sizeof(cplx) * g.size()))) return rc;
+    EXC_CHECK(hipMemcpy(c->gnt, g.data(), sizeof(cplx) * g.size(), hipMemcpyHostToDevice));
+    c->gnt_lmaxvr = lmaxvr;
+  }
+  // inputs and the radial integrals in one device buffer
+  const int nf = ao * (d.lmaxmat + 1) + (nlomax > 0 ? nlomax : 0);
+  const size_t n_w = (size_t)nrmtmax * ns, n_apw = (size_t)nrmtmax * 2 * ao * (d.lmaxapw + 1) * na;
+  const size_t n_lo = (size_t)nrmtmax * 2 * (nlomax > 0 ? nlomax : 0) * na, n_v = (size_t)lmv * nrmtmax * na;
+  const size_t n_rad = (size_t)na * (lmv + 2) * nf * nf;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_spr = al(8 * n_w), o_apw = o_spr + al(8 * n_w), o_lo = o_apw + al(8 * n_apw), o_v = o_lo + al(8 * n_lo);
+  const size_t o_rad = o_v + al(8 * n_v), o_i = o_rad + al(8 * n_rad);
+  const size_t total = o_i + sizeof(int) * (ns + ns + (size_t)(nlomax > 0 ? nlomax : 1) * ns);
+  if ((rc = grow(&c->hml, &c->hml_bytes, total))) return rc;
+  char *b = (char *)c->hml;
+  std::vector<int> ints(nrmt, nrmt + ns);
+  ints.insert(ints.end(), c->h_nlorb.begin(), c->h_nlorb.end());
+  if (nlomax > 0) ints.insert(ints.end(), c->h_lorbl.begin(), c->h_lorbl.end());
+  else ints.insert(ints.end(), (size_t)ns, 0);
+  EXC_CHECK(hipMemcpy(b, wt.data(), 8 * n_w, hipMemcpyHostToDevice));
+  EXC_CHECK(hipMemcpy(b + o_spr, spr, 8 * n_w, hipMemcpyHostToDevice));
+  EXC_CHECK(hipMemcpy(b + o_apw, apwfr, 8 * n_apw, hipMemcpyHostToDevice));
+  if (n_lo) EXC_CHECK(hipMemcpy(b + o_lo, lofr, 8 * n_lo, hipMemcpyHostToDevice));
+  EXC_CHECK(hipMemcpy(b + o_v, veffmt, 8 * n_v, hipMemcpyHostToDevice));
+  EXC_CHECK(hipMemcpy(b + o_i, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice));
+  // output arrays of the context (what exc_ctx_set_mt uploads)
+  const size_t mn = d.maxnlo, nl = nlomax > 0 ? nlomax : 0;
+  if (!c->d.haa && (rc = dev_alloc(c, &c->d.haa, (size_t)d.naa * d.naa * na))) return rc;
+  if (mn && !c->d.loa && (rc = dev_alloc(c, &c->d.loa, mn * d.naa * na))) return rc;
+  if (mn && !c->d.lolo && (rc = dev_alloc(c, &c->d.lolo, mn * mn * na))) return rc;
+  if (nl && !c->d.oalo && (rc = dev_alloc(c, &c->d.oalo, (size_t)ao * nl * na))) return rc;
+  if (nl && !c->d.ololo && (rc = dev_alloc(c, &c->d.ololo, nl * nl * na))) return rc;
+  const int *di = (const int *)(b + o_i);
+  {
+    StageTimer t(c, EXC_STAGE_HMLINT);
+    if ((rc = exc_launch_hmlint(c->d, lmv, nrmtmax, di, di + ns, di + 2 * ns, (const double *)b,
+                                (const double *)(b + o_spr), (const double *)(b + o_apw), (const double *)(b + o_lo),
+                                (const double *)(b + o_v), (double *)(b + o_rad), c->gnt, c->stream)))
+      return rc;
+  }
+  if (haa) EXC_CHECK(hipMemcpyAsync(haa, c->d.haa, 16 * (size_t)d.naa * d.naa * na, hipMemcpyDeviceToHost, c->stream));
+  if (mn && loa) EXC_CHECK(hipMemcpyAsync(loa, c->d.loa, 16 * mn * d.naa * na, hipMemcpyDeviceToHost, c->stream));
+  if (mn && lolo) EXC_CHECK(hipMemcpyAsync(lolo, c->d.lolo, 16 * mn * mn * na, hipMemcpyDeviceToHost, c->stream));
+  if (nl && oalo) EXC_CHECK(hipMemcpyAsync(oalo, c->d.oalo, 8 * (size_t)ao * nl * na, hipMemcpyDeviceToHost, c->stream));
+  if (nl && ololo) EXC_CHECK(hipMemcpyAsync(ololo, c->d.ololo, 8 * nl * nl * na, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_mt = true;
   return EXC_OK;

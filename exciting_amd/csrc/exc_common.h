@@ -150,5 +150,11 @@ int exc_launch_dmat_scale(const exc_dev &d, int ws, int nst, const cplx *x, cplx
                           const double *occ, double wk, hipStream_t s);
 int exc_launch_dmat_reduce(cplx *dm, const cplx *per, int np, size_t pstride, size_t poff, size_t n,
                            hipStream_t s);
+// muffin-tin integrals (k_hmlint.hip)
+void exc_hml_spline_weights(int n, const double *x, double *wt);
+void exc_hml_gaunt_table(int lmaxmat, int lmaxvr, cplx *g);
+int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
+                      const double *wt, const double *spr, const double *apwfr, const double *lofr,
+                      const double *veffmt, double *rad, const cplx *gnt, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

@@ -82,13 +82,14 @@ SIGNATURES = {
     "exc_eigen": [_P, _I, _I, _P, _P, _I, _P, _P, _I],
     "exc_eigen_batched_dev": [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
     "exc_dmat_wfsize": [_P],
+    "exc_ctx_hmlint": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat", "hmlint")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -277,6 +278,29 @@ class Context:
                           C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
                           C.cast(ints, _P))
 
+    def hmlint(self, rd, out=True):
+        """exc_ctx_hmlint from synth.RadialData: computes h_a, loa, lolo, oalo, ololo on the GPU
+        and installs them in the context; with out, also returns them as host arrays in the
+        shapes of Crystal.haa / loa / lolo / oalo / ololo."""
+        cr = self.cr
+        nrmtmax = rd.spr.shape[0]
+        spr = np.asfortranarray(rd.spr, dtype=np.float64)
+        apw = np.asfortranarray(rd.apwfr, dtype=np.float64)
+        lofr = np.asfortranarray(rd.lofr, dtype=np.float64)
+        v = np.asfortranarray(rd.veffmt, dtype=np.float64)
+        nrmt = np.ascontiguousarray(rd.nrmt, dtype=np.int32)
+        res = None
+        if out:
+            mn, nl = max(cr.maxnlo, 1), max(cr.nlomax, 1)
+            res = (np.zeros((cr.maxaa, cr.maxaa, cr.natm), dtype=np.complex128, order="F"),
+                   np.zeros((mn, cr.maxaa, cr.natm), dtype=np.complex128, order="F"),
+                   np.zeros((mn, mn, cr.natm), dtype=np.complex128, order="F"),
+                   np.zeros((cr.apwordmax, nl, cr.natm), order="F"),
+                   np.zeros((nl, nl, cr.natm), order="F"))
+        self._call("exc_ctx_hmlint", self.h, rd.lmaxvr, nrmtmax, _ptr(nrmt), _ptr(spr), _ptr(apw), _ptr(lofr),
+                   _ptr(v), *[_ptr(a) if res else None for a in (res or (None,) * 5)])
+        return res
+
     def dmat_wfsize(self):
         """Row count of the muffin-tin density matrix (maxaa + largest lo count of an atom)."""
         rc = self.lib.exc_dmat_wfsize(self.h)
@@ -336,7 +360,7 @@ class Context:
 
     def read_timing(self):
         """{stage: (total_ms, launches)} recorded since the last read (synchronises the stream)."""
-        n = 12
+        n = len(STAGES)             # EXC_NSTAGE
         ms = (C.c_double * n)()
         cnt = (C.c_int * n)()
         self._call("exc_ctx_read_timing", self.h, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))

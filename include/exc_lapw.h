@@ -94,6 +94,21 @@ int exc_ctx_set_radial(exc_ctx *ctx, const double *dmat);
  *   lolo(maxnlo, maxnlo, natmtot) complex mt_hscf%main%lolo
  *   oalo(apwordmax, nlomax, natmtot) real, ololo(nlomax, nlomax, natmtot) real
  * with maxaa = apwordmax*(lmaxmat+1)^2.  The h1* (IORA) terms are zero in this setup. */
+/* The same matrices computed on the GPU from this SCF iteration's radial data, the
+ * counterpart of hmlint (hmlint.f90:14, mt_kin.f90 + mt_pot.f90 with ValenceRelativity
+ * 'none', spinless) and olprad (olprad.f90:13): every radial integral is the reference's
+ * cubic-spline integral (fderiv.f90 m = -1, spline.f90) on the species' mesh, the angular
+ * part the Gaunt coefficients int conj(Y_lm1) R_lm2 Y_lm3 (gauntyry.f90).  Inputs (host):
+ *   nrmt(nspecies), spr(nrmtmax, nspecies)                      the radial meshes
+ *   apwfr(nrmtmax, 2, apwordmax, 0:lmaxapw, natmtot)              u and du/dr of the APW functions
+ *   lofr(nrmtmax, 2, nlomax, natmtot)                              the same of the local orbitals
+ *   veffmt((lmaxvr+1)^2, nrmtmax, natmtot)                         V in real spherical harmonics
+ * The results replace the context's muffin-tin matrices (as exc_ctx_set_mt would) and are
+ * also copied to the host arrays given (each may be NULL), in exc_ctx_set_mt's shapes.
+ * Needs exc_ctx_set_atoms and exc_ctx_set_lo first; lo angular momenta <= lmaxmat. */
+int exc_ctx_hmlint(exc_ctx *ctx, int lmaxvr, int nrmtmax, const int *nrmt, const double *spr,
+                   const double *apwfr, const double *lofr, const double *veffmt, double *haa,
+                   double *loa, double *lolo, double *oalo, double *ololo);
 int exc_ctx_set_mt(exc_ctx *ctx, const double *haa, const double *loa, const double *lolo,
                    const double *oalo, const double *ololo);
 
@@ -249,7 +264,8 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_OLP 9      /* addition-theorem overlap (EXC_BATCH_OLP_LEGENDRE)             */
 #define EXC_STAGE_EIGEN 10   /* exc_eigen_dev                                                 */
 #define EXC_STAGE_DMAT 11    /* exc_gendmatmt_dev (muffin-tin density matrix)                 */
-#define EXC_NSTAGE 12
+#define EXC_STAGE_HMLINT 12  /* exc_ctx_hmlint (muffin-tin radial integrals and Gaunt sums)   */
+#define EXC_NSTAGE 13
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
 /* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,

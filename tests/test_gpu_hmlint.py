@@ -1,0 +1,64 @@
+"""Muffin-tin integrals on the device (exc_ctx_hmlint: hmlint = mt_kin + mt_pot, olprad) against
+the oracle (oracle/hmlint_ref.py, pinned in test_oracle_hmlint.py), and the setup consuming
+the matrices the call installs.
+
+Bar: max |difference| <= 1e-12 of the largest element (FP64; the device sums the spline
+integral as a weight vector, the oracle interval by interval)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import small, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("which", ["small-d", "config1"])
+def test_hmlint_against_oracle(which):
+    from exciting_amd import lapw, synth
+    from oracle import hmlint_ref as M
+    w = small("d") if which == "small-d" else workload(1, 1)
+    cr = w.crystal
+    rd = synth.make_radial(cr, nr=(150, 120) if which == "small-d" else (260, 230),
+                           lmaxvr=4 if which == "small-d" else 6)
+    ctx = lapw.Context(cr)
+    haa, loa, lolo, oalo, ololo = ctx.hmlint(rd)
+    rh, rl, rll = M.hmlint(cr, rd)
+    ro, roo = M.olprad(cr, rd)
+    mn, nl = cr.maxnlo, cr.nlomax
+    errs = [_rel(haa, rh), _rel(loa[:mn], rl[:mn]), _rel(lolo[:mn, :mn], rll[:mn, :mn]),
+            _rel(oalo[:, :nl], ro[:, :nl]), _rel(ololo[:nl, :nl], roo[:nl, :nl])]
+    print(f"{which}: hmlint/olprad max rel diff haa {errs[0]:.2e} loa {errs[1]:.2e} lolo {errs[2]:.2e} "
+          f"oalo {errs[3]:.2e} ololo {errs[4]:.2e}", flush=True)
+    assert max(errs) <= 1e-12
+
+
+def test_setup_uses_hmlint_matrices():
+    """After exc_ctx_hmlint the per-k setup uses the matrices it computed: H and O against the
+    oracle setup with the oracle's hmlint/olprad matrices in place of the seeded ones."""
+    from exciting_amd import lapw, synth
+    from oracle import hmlint_ref as M
+    from oracle import lapw_ref as R
+    w = small("d")
+    cr = w.crystal
+    rd = synth.make_radial(cr, nr=(150, 120), lmaxvr=4)
+    ctx = lapw.Context(cr)
+    ctx.hmlint(rd, out=False)
+    rh, rl, rll = M.hmlint(cr, rd)
+    ro, roo = M.olprad(cr, rd)
+    mn, nl = cr.maxnlo, cr.nlomax
+    cr2 = dataclasses.replace(cr, haa=rh, loa=np.asfortranarray(rl[:mn]), lolo=np.asfortranarray(rll[:mn, :mn]),
+                              oalo=np.asfortranarray(ro[:, :nl]), ololo=np.asfortranarray(roo[:nl, :nl]))
+    kp = w.kpoints[0]
+    apw = ctx.match(kp)
+    H, O = ctx.hamiltonandoverlapsetup(kp, apw)
+    H_ref, O_ref, _ = R.setup(cr2, kp)
+    eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
+    eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
+    print(f"setup after hmlint: H rel {eh:.2e}, O rel {eo:.2e}", flush=True)
+    assert eh <= 1e-12 and eo <= 1e-12
