@@ -193,6 +193,40 @@ def run_eigen(args, ctx, batch, w, olpflag, nev, log):
             "variants": tuned, "cpu_lapack_zhegvx_ms_per_k": cpu_ms, "density_matrix": dmat}
 
 
+def run_hmlint(args, w, log):
+    """The muffin-tin integrals of one SCF iteration (exc_ctx_hmlint: hmlint + olprad) on seeded
+    radial data of the workload's atoms (meshes of 500 / 460 points, lmaxvr 6), on a context of
+    its own; with the CPU baseline on, the numpy oracle's time for the same call beside it."""
+    import torch
+    from exciting_amd import lapw, synth
+    cr = w.crystal
+    rd = synth.make_radial(cr, nr=(500, 460), lmaxvr=6)
+    ctx = lapw.Context(cr)
+    ctx.hmlint(rd, out=False)
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    reps = max(args.steps, 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.hmlint(rd, out=False)
+    dt = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    ms, cnt = ctx.read_timing()["hmlint"]
+    res = {"ms_per_call": 1000.0 * dt, "kernel_ms": ms / max(cnt, 1), "natm": cr.natm, "lmaxvr": rd.lmaxvr,
+           "nrmt": [int(x) for x in rd.nrmt], "cpu_oracle_ms": None}
+    if not args.no_cpu_baseline:
+        from oracle import hmlint_ref as M
+        t1 = time.perf_counter()
+        M.hmlint(cr, rd)
+        M.olprad(cr, rd)
+        res["cpu_oracle_ms"] = 1000.0 * (time.perf_counter() - t1)
+    log(f"  muffin-tin integrals (hmlint + olprad, {cr.natm} atoms): {res['ms_per_call']:.2f} ms per call, "
+        f"kernels {res['kernel_ms']:.3f} ms" + (f", numpy oracle {res['cpu_oracle_ms']:.0f} ms"
+                                                 if res["cpu_oracle_ms"] else ""))
+    return res
+
+
 def run_dmat(args, ctx, batch, w, nst, vecs, log):
     """Muffin-tin density matrix of all k-points (gendmatmt) from the eigenvectors of the
     eigensolve, seeded occupations in [0, 2] and equal k-point weights."""
@@ -236,6 +270,7 @@ def main():
     ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
                     help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 measurement")
+    ap.add_argument("--no-hmlint", action="store_true", help="skip the muffin-tin integrals (hmlint) measurement")
     ap.add_argument("--eigen", type=int, default=200, metavar="NEV",
                     help="grouped path: also time setup + device eigensolve of the lowest NEV states "
                          "(0: skip)")
@@ -382,6 +417,9 @@ def main():
     if args.path == "batched" and args.eigen > 0:
         eig = run_eigen(args, ctx, batch, w, olpflag, min(args.eigen, min(batch.nmat)), log)
 
+    hml = None
+    if args.path == "batched" and not args.no_hmlint:
+        hml = run_hmlint(args, w, log)
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(w)
@@ -396,7 +434,7 @@ def main():
                             + (", host arrays (PCIe copies included)" if args.path == "host" else "")),
                    "parallelism": "single GPU", "variants": tuned},
         "roofline": roof, "stages": stages, "split_bf16": split, "setup_and_eigensolve": eig,
-        "cpu_baseline": base,
+        "muffin_tin_integrals": hml, "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
 
