@@ -4,8 +4,9 @@
 ! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot), then
 ! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot); with a fourth
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
-! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).
-!   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev]
+! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).  With a fifth file of radial
+! data it first runs hmlint_gpu (hmlint + olprad), whose matrices replace the input's.
+!   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev] [<radial.bin>]
 program exc_driver_main
   use, intrinsic :: iso_c_binding
   use exc_lapw_c, only: exc_dims
@@ -24,7 +25,10 @@ program exc_driver_main
   real(8), allocatable :: evalfv(:), occ(:), forceibs(:, :), evs(:)
   complex(8), allocatable :: vecs(:, :), dm(:, :, :)
   real(8) :: wkpt
-  integer :: ws
+  integer :: ws, lmaxvr, nrmtmax
+  character(len=512) :: frad
+  integer(c_int), allocatable :: nrmt(:)
+  real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -60,6 +64,19 @@ program exc_driver_main
 
   call exc_gpu_init(dims, ngkmax, natoms, rmt, atposc, nlorb, lorbl, idxlo, dmat, haa, loa, lolo, &
                     oalo, ololo, ivg, ivgig, cfunig, veffig, ffacg)
+  if (command_argument_count() >= 5) then
+    ! radial data file: lmaxvr, nrmtmax, nrmt(nspecies), spr, apwfr, lofr, veffmt
+    call get_command_argument(5, frad)
+    open(newunit=u, file=trim(frad), access="stream", form="unformatted", status="old")
+    read(u) lmaxvr, nrmtmax
+    allocate(nrmt(dims%nspecies), spr(nrmtmax, dims%nspecies))
+    allocate(apwfr(nrmtmax * 2 * dims%apwordmax * (dims%lmaxapw + 1) * dims%natmtot))
+    allocate(lofr(nrmtmax * 2 * max(dims%nlomax, 1) * dims%natmtot))
+    allocate(veffmt((lmaxvr + 1)**2 * nrmtmax * dims%natmtot))
+    read(u) nrmt, spr, apwfr, lofr, veffmt
+    close(u)
+    call hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
+  end if
   nlotot = exc_gpu_nlotot()
   nmat = ngp + nlotot
 

@@ -52,6 +52,17 @@ module exc_lapw_c
       real(c_double), intent(in) :: oalo(*), ololo(*)
     end function
 
+    integer(c_int) function exc_ctx_hmlint(ctx, lmaxvr, nrmtmax, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, &
+                                           oalo, ololo) bind(c, name="exc_ctx_hmlint")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lmaxvr, nrmtmax
+      integer(c_int), intent(in) :: nrmt(*)
+      real(c_double), intent(in) :: spr(*), apwfr(*), lofr(*), veffmt(*)
+      complex(c_double_complex), intent(inout) :: haa(*), loa(*), lolo(*)
+      real(c_double), intent(inout) :: oalo(*), ololo(*)
+    end function
+
     integer(c_int) function exc_ctx_set_gvec(ctx, ivg, ivgig, cfunig, veffig, ffacg) bind(c, name="exc_ctx_set_gvec")
       import :: c_int, c_ptr, c_double, c_double_complex
       type(c_ptr), value :: ctx

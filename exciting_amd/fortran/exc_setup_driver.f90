@@ -5,6 +5,8 @@
 !   solvewithgpu(system, nstfv, evecfv, evalfv)                   modfvsystem.f90:585 (solvewithlapack)
 !   gendmatmt_gpu(ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, dm [, evecsv])
 !                                                                 gendmatmt.f90:13
+!   hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
+!                                                                 hmlint.f90:14 + olprad.f90:13
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -23,7 +25,7 @@ module exc_setup_gpu
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
-            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize
+            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu
 
 contains
 
@@ -176,5 +178,21 @@ contains
     call check(exc_gendmatmt(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, int(nstfv, c_int), &
                              int(nmatmax, c_int), evecfv, psv, occ, wkpt, dm), "gendmatmt")
   end subroutine gendmatmt_gpu
+
+  ! hmlint.f90:14 (mt_kin + mt_pot, ValenceRelativity 'none', spinless) and olprad.f90:13 with
+  ! the module state they read passed in: the radial meshes nrmt(nspecies), spr(nrmtmax,
+  ! nspecies), apwfr(nrmtmax, 2, apwordmax, 0:lmaxapw, natmtot), lofr(nrmtmax, 2, nlomax,
+  ! natmtot), veffmt(lmmaxvr, nrmtmax, natmtot).  The matrices replace the context's (the
+  ! setup uses them from here on) and are returned in mt_hscf%main%aa / loa / lolo and oalo /
+  ! ololo shapes.  Called once per SCF iteration, after the radial functions are updated.
+  subroutine hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
+    integer, intent(in) :: lmaxvr
+    integer(c_int), intent(in) :: nrmt(:)
+    real(8), intent(in) :: spr(:, :), apwfr(*), lofr(*), veffmt(*)
+    complex(8), intent(out) :: haa(*), loa(*), lolo(*)
+    real(8), intent(out) :: oalo(*), ololo(*)
+    call check(exc_ctx_hmlint(ctx, int(lmaxvr, c_int), int(size(spr, 1), c_int), nrmt, spr, apwfr, lofr, veffmt, &
+                              haa, loa, lolo, oalo, ololo), "hmlint")
+  end subroutine hmlint_gpu
 
 end module exc_setup_gpu

@@ -68,3 +68,14 @@ def read_dmat(path, cr, ngkmax, ws):
         np.fromfile(f, np.complex128, skip)
         np.fromfile(f, np.float64, 3 * cr.natm)
         return np.fromfile(f, np.complex128, ws * ws * cr.natm).reshape((ws, ws, cr.natm), order="F")
+
+
+def write_radial(path, cr, rd):
+    """Radial data for the driver's hmlint_gpu step: lmaxvr, nrmtmax, nrmt, spr, apwfr, lofr, veffmt."""
+    with open(path, "wb") as f:
+        _w(f, [rd.lmaxvr, rd.spr.shape[0]], np.int32)
+        _w(f, rd.nrmt, np.int32)
+        _w(f, rd.spr, np.float64)
+        _w(f, rd.apwfr, np.float64)
+        _w(f, rd.lofr, np.float64)
+        _w(f, rd.veffmt, np.float64)

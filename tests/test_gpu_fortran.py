@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import read_dmat, read_eigen_output, read_forces, read_output, write_eigen, write_input
+from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, write_eigen, write_input,
+                        write_radial)
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +92,32 @@ def test_fortran_solvewithgpu(tmp_path, which):
     print(f"{which}: max |d eig| {np.abs(e - e_ref).max():.2e} Ha, residual {res:.2e}, orthonormality {orth:.2e}")
     assert np.abs(e - e_ref).max() <= 1e-10
     assert res <= 1e-12 and orth <= 1e-10
+
+
+def test_fortran_hmlint_then_setup(tmp_path):
+    """hmlint_gpu (Fortran, through exc_ctx_hmlint) before the setup: H and O against the oracle
+    setup with the oracle's hmlint/olprad matrices in place of the seeded ones."""
+    import dataclasses
+    from exciting_amd import synth
+    from oracle import hmlint_ref as M
+    from oracle import lapw_ref as R
+    w = small("b")
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(140, 110), lmaxvr=4)
+    fin, fout, frad = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "radial.bin"
+    write_input(fin, cr, kp)
+    write_radial(frad, cr, rd)
+    r = subprocess.run([EXE, str(fin), str(fout), "-", "0", str(frad)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    _, H, O = read_output(fout, cr, kp.igpig.shape[0])
+    rh, rl, rll = M.hmlint(cr, rd)
+    ro, roo = M.olprad(cr, rd)
+    mn, nl = cr.maxnlo, cr.nlomax
+    cr2 = dataclasses.replace(cr, haa=rh, loa=np.asfortranarray(rl[:mn]), lolo=np.asfortranarray(rll[:mn, :mn]),
+                              oalo=np.asfortranarray(ro[:, :nl]), ololo=np.asfortranarray(roo[:nl, :nl]))
+    H_ref, O_ref, _ = R.setup(cr2, kp)
+    eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
+    eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
+    print(f"Fortran hmlint_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
+    assert eh <= 1e-12 and eo <= 1e-12
