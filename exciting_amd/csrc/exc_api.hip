@@ -1054,6 +1054,51 @@ int exc_ctx_set_mt(exc_ctx *c, const double *haa, const double *loa, const doubl
   return EXC_OK;
 }
 
+namespace {
+// m-th derivative (m = 0, 1) at x of the cubic through four points (polynom.f90 with np = 4):
+// the cubic in t = x - xa[0] from the offsets of the other three points
+double cubic_deriv(int m, const double *xa, const double *ya, double x) {
+  const double x1 = xa[1] - xa[0], x2 = xa[2] - xa[0], x3 = xa[3] - xa[0];
+  const double y1 = ya[1] - ya[0], y2 = ya[2] - ya[0], y3 = ya[3] - ya[0];
+  const double t1 = 1.0 / (x1 * x2 * x3 * (x1 - x2) * (x1 - x3) * (x2 - x3));
+  const double t2 = x1 * x2 * y3, t3 = x2 * x3 * y1, t4 = x3 * x1 * y2;
+  const double t5 = x1 * x1, t6 = x2 * x2, t7 = x3 * x3;
+  const double c2 = t1 * (t3 * (x3 * t6 - x2 * t7) + t4 * (x1 * t7 - x3 * t5) + t2 * (x2 * t5 - x1 * t6));
+  const double c3 = t1 * (t3 * (t7 - t6) + t4 * (t5 - t7) + t2 * (t6 - t5));
+  const double c4 = t1 * (t3 * (x2 - x3) + t4 * (x3 - x1) + t2 * (x1 - x2));
+  const double t = x - xa[0];
+  return m == 0 ? ya[0] + t * (c2 + t * (c3 + c4 * t)) : c2 + t * (2.0 * c3 + 3.0 * c4 * t);
+}
+}  // namespace
+
+int exc_ctx_set_radial_fr(exc_ctx *c, int nrmtmax, const int *nrmt, const double *spr, const double *apwfr) {
+  if (!c || !nrmt || !spr || !apwfr) return EXC_ERR_ARG;
+  if (!c->have_atoms) return EXC_ERR_STATE;
+  const int ns = c->dims.nspecies, na = c->dims.natmtot, L = c->dims.lmaxapw, ao = c->dims.apwordmax;
+  if (std::max(ao + 1, 4) != 4) return EXC_ERR_UNSUPPORTED;   // match.f90:85 uses np = 4 here
+  for (int is = 0; is < ns; ++is)
+    if (nrmt[is] < 4 || nrmt[is] > nrmtmax) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  std::vector<int> sp(na);
+  EXC_CHECK(hipMemcpy(sp.data(), c->d.spidx, sizeof(int) * na, hipMemcpyDeviceToHost));
+  std::vector<double> rmt(ns);
+  EXC_CHECK(hipMemcpy(rmt.data(), c->d.rmt, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  // dmat(io1, io2, l, ias) = d^(io1-1) u_(io2,l) / dr^(io1-1) at R (match.f90:139-148)
+  std::vector<double> dmat((size_t)ao * ao * (L + 1) * na);
+  for (int ias = 0; ias < na; ++ias) {
+    const int is = sp[ias], n = nrmt[is];
+    const double *xa = spr + (size_t)nrmtmax * is + (n - 4);
+    for (int l = 0; l <= L; ++l)
+      for (int io2 = 0; io2 < ao; ++io2) {
+        const double *ya = apwfr + (size_t)nrmtmax * 2 * (io2 + (size_t)ao * (l + (size_t)(L + 1) * ias)) + (n - 4);
+        for (int io1 = 0; io1 < ao; ++io1)
+          dmat[io1 + ao * (io2 + (size_t)ao * (l + (size_t)(L + 1) * ias))] = cubic_deriv(io1, xa, ya, rmt[is]);
+      }
+  }
+  return exc_ctx_set_radial(c, dmat.data());
+}
+
 int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const double *spr, const double *apwfr,
                    const double *lofr, const double *veffmt, double *haa, double *loa, double *lolo, double *oalo,
                    double *ololo) {

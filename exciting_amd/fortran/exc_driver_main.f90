@@ -5,7 +5,8 @@
 ! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot); with a fourth
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
 ! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).  With a fifth file of radial
-! data it first runs hmlint_gpu (hmlint + olprad), whose matrices replace the input's.
+! data it first runs set_radial_gpu (match's boundary values from apwfr) and hmlint_gpu (hmlint +
+! olprad), whose results replace the input's dmat and muffin-tin matrices.
 !   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev] [<radial.bin>]
 program exc_driver_main
   use, intrinsic :: iso_c_binding
@@ -75,6 +76,7 @@ program exc_driver_main
     allocate(veffmt((lmaxvr + 1)**2 * nrmtmax * dims%natmtot))
     read(u) nrmt, spr, apwfr, lofr, veffmt
     close(u)
+    call set_radial_gpu(nrmt, spr, apwfr)
     call hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
   end if
   nlotot = exc_gpu_nlotot()

@@ -52,6 +52,14 @@ module exc_lapw_c
       real(c_double), intent(in) :: oalo(*), ololo(*)
     end function
 
+    integer(c_int) function exc_ctx_set_radial_fr(ctx, nrmtmax, nrmt, spr, apwfr) bind(c, name="exc_ctx_set_radial_fr")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrmtmax
+      integer(c_int), intent(in) :: nrmt(*)
+      real(c_double), intent(in) :: spr(*), apwfr(*)
+    end function
+
     integer(c_int) function exc_ctx_hmlint(ctx, lmaxvr, nrmtmax, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, &
                                            oalo, ololo) bind(c, name="exc_ctx_hmlint")
       import :: c_int, c_ptr, c_double, c_double_complex

@@ -7,6 +7,7 @@
 !                                                                 gendmatmt.f90:13
 !   hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
 !                                                                 hmlint.f90:14 + olprad.f90:13
+!   set_radial_gpu(nrmt, spr, apwfr)                             match.f90:139-148 (zd by polynom)
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -25,7 +26,7 @@ module exc_setup_gpu
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
-            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu
+            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu
 
 contains
 
@@ -194,5 +195,17 @@ contains
     call check(exc_ctx_hmlint(ctx, int(lmaxvr, c_int), int(size(spr, 1), c_int), nrmt, spr, apwfr, lofr, veffmt, &
                               haa, loa, lolo, oalo, ololo), "hmlint")
   end subroutine hmlint_gpu
+
+  ! match.f90:139-148 — the boundary values zd that match builds with polynom from apwfr at R,
+  ! installed once per SCF iteration (after the radial functions change) instead of dmat.
+  ! A singular matching matrix is reported as match.f90:166-175 does; the run continues.
+  subroutine set_radial_gpu(nrmt, spr, apwfr)
+    integer(c_int), intent(in) :: nrmt(:)
+    real(8), intent(in) :: spr(:, :), apwfr(*)
+    integer(c_int) :: rc
+    rc = exc_ctx_set_radial_fr(ctx, int(size(spr, 1), c_int), nrmt, spr, apwfr)
+    call check(rc, "set_radial_gpu")
+    if (rc > 0) write(*, '("Warning(set_radial_gpu): singular matching matrix, info = ", I0)') rc
+  end subroutine set_radial_gpu
 
 end module exc_setup_gpu

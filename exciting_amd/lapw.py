@@ -83,6 +83,7 @@ SIGNATURES = {
     "exc_eigen_batched_dev": [_P, _I, _P, _P, _P, _P, _I, _P, _P, _P],
     "exc_dmat_wfsize": [_P],
     "exc_ctx_hmlint": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "exc_ctx_set_radial_fr": [_P, _I, _P, _P, _P],
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
@@ -277,6 +278,15 @@ class Context:
         return self._call("exc_eigen_batched_dev", self.h, nk, C.cast(ints, _P), C.cast(ints, _P),
                           C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
                           C.cast(ints, _P))
+
+    def set_radial_fr(self, rd):
+        """exc_ctx_set_radial_fr: match's boundary values from the radial functions of rd
+        (returns the singular-matching info, 0 when none)."""
+        spr = np.asfortranarray(rd.spr, dtype=np.float64)
+        apw = np.asfortranarray(rd.apwfr, dtype=np.float64)
+        nrmt = np.ascontiguousarray(rd.nrmt, dtype=np.int32)
+        return self._call("exc_ctx_set_radial_fr", self.h, rd.spr.shape[0], _ptr(nrmt), _ptr(spr), _ptr(apw),
+                          ok_positive=True)
 
     def hmlint(self, rd, out=True):
         """exc_ctx_hmlint from synth.RadialData: computes h_a, loa, lolo, oalo, ololo on the GPU

@@ -87,6 +87,12 @@ int exc_ctx_set_lo(exc_ctx *ctx, const int *nlorb, const int *lorbl, const int *
  * match.f90:141-148 builds with polynom from apwfr.  Returns > 0 (as exc_match would) if
  * a matching matrix is singular. */
 int exc_ctx_set_radial(exc_ctx *ctx, const double *dmat);
+/* The same from the radial functions themselves, as match.f90:139-148 builds zd: the value
+ * and first derivative at R of the cubic through the last four mesh points (polynom.f90 with
+ * np = max(apwordmax+1, 4) = 4, match.f90:85) of apwfr(nrmtmax, 2, apwordmax, 0:lmaxapw,
+ * natmtot) (only the u part is read) on the meshes spr(nrmtmax, nspecies), nrmt(nspecies). */
+int exc_ctx_set_radial_fr(exc_ctx *ctx, int nrmtmax, const int *nrmt, const double *spr,
+                          const double *apwfr);
 
 /* Muffin-tin matrices of this SCF iteration (mod_eigensystem.F90:60-80, olprad.f90):
  *   haa(maxaa, maxaa, natmtot)   complex  mt_hscf%main%aa

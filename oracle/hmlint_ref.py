@@ -10,6 +10,9 @@ SCF iteration, the producers of the setup's h_a, oalo and ololo:
 * ``hmlint``        — src_eigensystem/hmlint.f90:33-34 = mt_kin.f90:96-221 + mt_pot.f90:111-307
                       (ValenceRelativity 'none', spinless: no rmtable factor, no bxcmt, no SO)
 * ``olprad``        — src_eigensystem/olprad.f90:90-125 (oalo, ololo; no IORA terms)
+* ``polynom``       — src_lib/polynom.f90 (np = 4: the cubic through four points and its
+                      derivatives at x), and ``dmat_from_radial`` — match.f90:85, 139-148
+                      (the boundary values zd of the radial functions at R)
 
 Arrays follow the reference shapes (Fortran order, 0-based here): apwfr(nrmtmax, 2, apwordmax,
 0:lmaxapw, natm) with [:, 0] = u and [:, 1] = du/dr, lofr(nrmtmax, 2, nlomax, natm),
@@ -296,3 +299,45 @@ def olprad(cr, rd):
                 if int(cr.lorbl[ilo2, is_]) == l:
                     ololo[ilo, ilo2, ias] = S[b, ao * (L + 1) + ilo2]
     return oalo, ololo
+
+
+def polynom(m, xa, ya, x):
+    """polynom.f90, np = 4 (the case match.f90:85 uses for apwordmax <= 3): the m-th derivative
+    at x (m = 0..3) of the cubic through (xa_i, ya_i), in the reference's form: the coefficients
+    of the cubic in t = x - xa_1 from the divided differences of the three offsets."""
+    x1, x2, x3 = xa[1] - xa[0], xa[2] - xa[0], xa[3] - xa[0]
+    y1, y2, y3 = ya[1] - ya[0], ya[2] - ya[0], ya[3] - ya[0]
+    t1 = 1.0 / (x1 * x2 * x3 * (x1 - x2) * (x1 - x3) * (x2 - x3))
+    t2, t3, t4 = x1 * x2 * y3, x2 * x3 * y1, x3 * x1 * y2
+    t5, t6, t7 = x1 ** 2, x2 ** 2, x3 ** 2
+    c2 = t1 * (t3 * (x3 * t6 - x2 * t7) + t4 * (x1 * t7 - x3 * t5) + t2 * (x2 * t5 - x1 * t6))
+    c3 = t1 * (t3 * (t7 - t6) + t4 * (t5 - t7) + t2 * (t6 - t5))
+    c4 = t1 * (t3 * (x2 - x3) + t4 * (x3 - x1) + t2 * (x1 - x2))
+    t = x - xa[0]
+    if m == 0:
+        return ya[0] + t * (c2 + t * (c3 + c4 * t))
+    if m == 1:
+        return c2 + t * (2.0 * c3 + 3.0 * c4 * t)
+    if m == 2:
+        return 6.0 * c4 * t + 2.0 * c3
+    if m == 3:
+        return 6.0 * c4
+    return 0.0
+
+
+def dmat_from_radial(cr, rd):
+    """dmat(io1, io2, l, ias) = d^(io1-1) u_(io2,l) / dr^(io1-1) at R from the last four mesh
+    points of apwfr (match.f90:139-148 with np = max(apwordmax+1, 4) = 4)."""
+    ao = cr.apwordmax
+    assert max(ao + 1, 4) == 4
+    d = np.zeros((ao, ao, cr.lmaxapw + 1, cr.natm), order="F")
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        n = int(rd.nrmt[is_])
+        xa = rd.spr[n - 4:n, is_]
+        for l in range(cr.lmaxapw + 1):
+            for io2 in range(ao):
+                ya = rd.apwfr[n - 4:n, 0, io2, l, ias]
+                for io1 in range(ao):
+                    d[io1, io2, l, ias] = polynom(io1, xa, ya, float(cr.rmt[is_]))
+    return d

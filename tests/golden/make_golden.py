@@ -4,8 +4,8 @@ The reference's src/src_lib/{sbessel,sbesseldm,factnm,factr,genylm,sphcrd}.f90 a
 (where they lie in the reference checkout) by oracle/Makefile into oracle/_ref/libexcref.so;
 this script evaluates j_l, j_l' (sbesseldm with m = 0, 1) and Y_lm (genylm) on fixed inputs
 and stores inputs and outputs; likewise the spline integral (fderiv, m = -1), Gaunt
-coefficients (gaunt) and real spherical harmonics (genrlm) on the inputs of
-test_oracle_hmlint.fixture_inputs.  Run from the repository root:
+coefficients (gaunt), real spherical harmonics (genrlm) and the four-point derivatives
+(polynom) on the inputs of test_oracle_hmlint.fixture_inputs / polynom_inputs.  Run from the repository root:
 
     make -C oracle && python tests/golden/make_golden.py
 """
@@ -41,7 +41,8 @@ def main():
     ints = np.array([H.ref_fderiv_int(hl, x, f) for x, f in zip(meshes, funcs)])
     gaunts = np.array([H.ref_gaunt(hl, *map(int, a)) for a in args])
     rlms = np.array([H.ref_genrlm(hl, 8, t, p) for t, p in tps])
-    np.savez_compressed(os.path.join(HERE, "hmlint_ref.npz"), ints=ints, gaunt=gaunts, rlm=rlms)
+    polys = np.array([[H.ref_polynom(hl, m, xa, ya, x) for m in range(4)] for xa, ya, x in H.polynom_inputs()])
+    np.savez_compressed(os.path.join(HERE, "hmlint_ref.npz"), ints=ints, gaunt=gaunts, rlm=rlms, polynom=polys)
     print("wrote", os.path.join(HERE, "hmlint_ref.npz"))
 
 

@@ -95,8 +95,9 @@ def test_fortran_solvewithgpu(tmp_path, which):
 
 
 def test_fortran_hmlint_then_setup(tmp_path):
-    """hmlint_gpu (Fortran, through exc_ctx_hmlint) before the setup: H and O against the oracle
-    setup with the oracle's hmlint/olprad matrices in place of the seeded ones."""
+    """set_radial_gpu and hmlint_gpu (Fortran, through exc_ctx_set_radial_fr / exc_ctx_hmlint)
+    before the setup: H and O against the oracle setup with the oracle's boundary values and
+    hmlint/olprad matrices in place of the seeded ones."""
     import dataclasses
     from exciting_amd import synth
     from oracle import hmlint_ref as M
@@ -114,10 +115,11 @@ def test_fortran_hmlint_then_setup(tmp_path):
     rh, rl, rll = M.hmlint(cr, rd)
     ro, roo = M.olprad(cr, rd)
     mn, nl = cr.maxnlo, cr.nlomax
-    cr2 = dataclasses.replace(cr, haa=rh, loa=np.asfortranarray(rl[:mn]), lolo=np.asfortranarray(rll[:mn, :mn]),
-                              oalo=np.asfortranarray(ro[:, :nl]), ololo=np.asfortranarray(roo[:nl, :nl]))
+    cr2 = dataclasses.replace(cr, dmat=M.dmat_from_radial(cr, rd), haa=rh, loa=np.asfortranarray(rl[:mn]),
+                              lolo=np.asfortranarray(rll[:mn, :mn]), oalo=np.asfortranarray(ro[:, :nl]),
+                              ololo=np.asfortranarray(roo[:nl, :nl]))
     H_ref, O_ref, _ = R.setup(cr2, kp)
     eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
     eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
-    print(f"Fortran hmlint_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
+    print(f"Fortran set_radial_gpu + hmlint_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
     assert eh <= 1e-12 and eo <= 1e-12

@@ -40,7 +40,16 @@ def _reflib():
     lib.fderiv_.restype = None
     lib.genrlm_.restype = None
     lib.gaunt_.restype = C.c_double
+    lib.polynom_.restype = C.c_double
     return lib
+
+
+def ref_polynom(lib, m, xa, ya, x):
+    xa = np.ascontiguousarray(xa, dtype=np.float64)
+    ya = np.ascontiguousarray(ya, dtype=np.float64)
+    c = np.zeros(len(xa))
+    return lib.polynom_(C.byref(C.c_int(m)), C.byref(C.c_int(len(xa))), xa.ctypes.data_as(C.c_void_p),
+                        ya.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p), C.byref(C.c_double(x)))
 
 
 def ref_fderiv_int(lib, x, f):
@@ -88,8 +97,24 @@ def fixture_inputs():
     return meshes, funcs, np.array(args, dtype=np.int64), tps
 
 
-def _check(ints, gaunts, rlms):
+def polynom_inputs():
+    """Four-point stencils at the end of logarithmic meshes, as match uses them."""
+    rng = np.random.default_rng(23)
+    out = []
+    for n in (60, 300, 700):
+        x = 1e-5 * np.exp(np.arange(n) * math.log(2.2 / 1e-5) / (n - 1))
+        xa = x[-4:]
+        ya = np.sin(rng.uniform(0.5, 2.0) * xa) * np.exp(-xa) + rng.uniform(-1, 1)
+        out.append((xa, ya, x[-1]))
+    return out
+
+
+def _check(ints, gaunts, rlms, polys=None):
     meshes, funcs, args, tps = fixture_inputs()
+    if polys is not None:
+        for (xa, ya, x), ref in zip(polynom_inputs(), polys):
+            for m in range(4):
+                assert abs(M.polynom(m, xa, ya, x) - ref[m]) <= 1e-12 * max(1.0, abs(ref[m]))
     for x, f, g in zip(meshes, funcs, ints):
         assert abs(M.fderiv_int(x, f) - g) <= 1e-13 * max(1.0, abs(g))
     ours = np.array([M.gaunt(*map(int, a)) for a in args])
@@ -107,14 +132,28 @@ def test_against_reference_fortran():
     gaunts = np.array([ref_gaunt(lib, *map(int, a)) for a in args])
     rlms = np.array([ref_genrlm(lib, 8, t, p) for t, p in tps])
     assert np.count_nonzero(gaunts) > 50
-    _check(ints, gaunts, rlms)
+    polys = [[ref_polynom(lib, m, xa, ya, x) for m in range(4)] for xa, ya, x in polynom_inputs()]
+    _check(ints, gaunts, rlms, polys)
 
 
 def test_against_reference_fixtures():
     if not os.path.exists(GOLDEN):
         pytest.skip("golden fixture not generated")
     g = np.load(GOLDEN)
-    _check(list(g["ints"]), g["gaunt"], g["rlm"])
+    _check(list(g["ints"]), g["gaunt"], g["rlm"], g["polynom"] if "polynom" in g else None)
+
+
+def test_polynom_exact_on_cubics():
+    """The cubic through four points reproduces any cubic and its derivatives."""
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        a = rng.normal(size=4)
+        xa = np.sort(rng.uniform(1.0, 2.0, 4))
+        x = rng.uniform(1.0, 2.2)
+        p = np.polynomial.Polynomial(a)
+        for m in range(4):
+            ref = p.deriv(m)(x) if m else p(x)
+            assert abs(M.polynom(m, xa, p(xa), x) - ref) <= 1e-9 * max(1.0, abs(ref))
 
 
 def _sphere_grid(nt=24, nphi=52):
