@@ -15,6 +15,11 @@ Each function restates one reference routine, in numpy, on the arrays of
 * ``setup_vjp``         — dE/dtau and dE/dV(G) of (H,O) for given cotangents
 * ``gendmatmt``         — gendmatmt.f90:59-172 (muffin-tin density matrix of one k,
                           spinless first-variational part)
+* ``igfft``, ``gencfun``, ``genveffig`` — gengvec.f90:100-117 (FFT index of each G),
+                          gencfun.f90:52-96 (Theta(G) and Theta(r), cfdamp = 0),
+                          genveffig.f90:23-33 (V(G) = forward FFT of V(r) Theta(r)), with
+                          zfftifc.f90's convention: forward (sign -1) scaled by 1/N, backward
+                          unscaled; ``dft3`` is the transform by direct summation
 
 Only ``tests/``, ``__graft_entry__.smoke`` and ``bench.py``'s CPU-baseline leg
 may import this module.  The library never calls it.
@@ -383,3 +388,64 @@ def gendmatmt(cr, apwalm, ngp, evec, occ, wkpt, evecsv=None, dm=None):
         wfa = wf1 if evecsv is None else wf1 @ evecsv
         dm[:, :, ias] += wkpt * (wfa * occ[None, :]) @ wfa.conj().T
     return dm
+
+
+def fft_grid(cr):
+    """ngrid(3) of the FFT: the G box (intgv)."""
+    return [int(cr.intgv[i, 1] - cr.intgv[i, 0] + 1) for i in range(3)]
+
+
+def igfft(cr):
+    """0-based position on the FFT grid of every G (gengvec.f90:100-117): negative components
+    wrap to ngrid + i; index j1 + n1 (j2 + n2 j3)."""
+    n = fft_grid(cr)
+    j = [np.where(cr.ivg[i] >= 0, cr.ivg[i], n[i] + cr.ivg[i]) for i in range(3)]
+    return (j[0] + n[0] * (j[1] + n[1] * j[2])).astype(np.int64)
+
+
+def dft3(z, n, sgn):
+    """The transform of zfftifc(3, n, sgn, z) by direct summation (small grids only): forward
+    (sgn = -1) sum_r e^(-i 2 pi k.r / n) z(r) / N, backward (sgn = +1) unscaled."""
+    z = np.asarray(z, dtype=np.complex128).reshape(n, order="F")
+    out = z
+    for ax in range(3):
+        m = n[ax]
+        k = np.arange(m)
+        w = np.exp(sgn * 2j * np.pi * np.outer(k, k) / m)
+        out = np.moveaxis(np.tensordot(w, np.moveaxis(out, ax, 0), axes=(1, 0)), 0, ax)
+    if sgn < 0:
+        out = out / (n[0] * n[1] * n[2])
+    return out.ravel(order="F")
+
+
+def _fft3(z, n, sgn):
+    """zfftifc's convention with numpy's FFT (pinned against dft3)."""
+    z = np.asarray(z, dtype=np.complex128).reshape(n, order="F")
+    out = np.fft.fftn(z) / (n[0] * n[1] * n[2]) if sgn < 0 else np.fft.ifftn(z) * (n[0] * n[1] * n[2])
+    return out.ravel(order="F")
+
+
+def gencfun(cr, fft=_fft3):
+    """gencfun.f90:52-96 (cfdamp = 0): Theta(G) = delta_G0 - sum_a e^(-i G.r_a) Theta_s(G) with
+    Theta_s(G) = (4 pi/Omega)(sin GR - GR cos GR)/G^3, (4 pi/Omega) R^3/3 at G = 0, for every G
+    of the grid, and Theta(r) = the backward transform (real part)."""
+    t1 = 4.0 * np.pi / cr.omega
+    cf = np.zeros(cr.ngrtot, dtype=np.complex128)
+    cf[0] = 1.0
+    for ias in range(cr.natm):
+        R = cr.rmt[cr.spidx[ias]]
+        big = cr.gc > 1e-6
+        ff = np.full(cr.ngrtot, (t1 / 3.0) * R ** 3)
+        t2 = cr.gc[big] * R
+        ff[big] = t1 * (np.sin(t2) - t2 * np.cos(t2)) / cr.gc[big] ** 3
+        ph = -(cr.vgc.T @ cr.atposc[:, ias])
+        cf -= (np.cos(ph) + 1j * np.sin(ph)) * ff
+    z = np.zeros(cr.ngrtot, dtype=np.complex128)
+    z[igfft(cr)] = cf
+    return cf, fft(z, fft_grid(cr), 1).real
+
+
+def genveffig(cr, veffir, cfunir, fft=_fft3):
+    """genveffig.f90:23-33: V(G) = forward transform of V(r) Theta(r) at the first ngvec G."""
+    z = fft(np.asarray(veffir) * np.asarray(cfunir), fft_grid(cr), -1)
+    return z[igfft(cr)[:cr.ngvec]]

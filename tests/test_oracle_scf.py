@@ -1,0 +1,70 @@
+"""Pins the oracle's per-SCF producers of Theta(G) and V(G) (oracle.lapw_ref: igfft, gencfun,
+genveffig, dft3).
+
+* dft3 against the defining sum written out element by element (tiny grid), and the numpy
+  transform the oracle uses against dft3, both directions (zfftifc.f90: forward sign -1 scaled
+  by 1/N, backward +1 unscaled).
+* igfft: a bijection of the G list onto the grid, G = 0 at 0, negative components wrapped.
+* gencfun: Theta(G) equals the configuration's (pinned against a real-space integral in
+  test_oracle_setup.py); the grid mean of Theta(r) is Theta(0); forward(backward) = identity.
+* genveffig: a plane wave cos(G0.r) with Theta(r) = 1 gives 1/2 at +-G0 and nothing else.
+"""
+import math
+
+import numpy as np
+
+from conftest import small
+from oracle import lapw_ref as R
+
+
+def test_dft3_against_the_sum():
+    n = [3, 4, 5]
+    rng = np.random.default_rng(1)
+    z = rng.standard_normal(60) + 1j * rng.standard_normal(60)
+    for sgn in (-1, 1):
+        out = np.zeros(60, dtype=np.complex128)
+        for k3 in range(5):
+            for k2 in range(4):
+                for k1 in range(3):
+                    s = 0j
+                    for r3 in range(5):
+                        for r2 in range(4):
+                            for r1 in range(3):
+                                ph = sgn * 2 * math.pi * (k1 * r1 / 3 + k2 * r2 / 4 + k3 * r3 / 5)
+                                s += z[r1 + 3 * (r2 + 4 * r3)] * complex(math.cos(ph), math.sin(ph))
+                    out[k1 + 3 * (k2 + 4 * k3)] = s / 60 if sgn < 0 else s
+        assert np.abs(R.dft3(z, n, sgn) - out).max() <= 1e-12
+        assert np.abs(R._fft3(z, n, sgn) - out).max() <= 1e-12
+
+
+def test_igfft_is_a_bijection():
+    cr = small("a").crystal
+    ix = R.igfft(cr)
+    n = R.fft_grid(cr)
+    assert ix[0] == 0 and (cr.ivg[:, 0] == 0).all()
+    assert len(set(ix.tolist())) == cr.ngrtot == n[0] * n[1] * n[2]
+    g = np.nonzero((cr.ivg[0] == -1) & (cr.ivg[1] == 2) & (cr.ivg[2] == -3))[0][0]
+    assert ix[g] == (n[0] - 1) + n[0] * (2 + n[1] * (n[2] - 3))
+
+
+def test_gencfun_properties():
+    cr = small("d").crystal
+    cf, cir = R.gencfun(cr)
+    assert np.abs(cf - cr.cfunig).max() <= 1e-14
+    assert abs(cir.mean() - cf[0].real) <= 1e-13
+    back = R._fft3(cir, R.fft_grid(cr), -1)[R.igfft(cr)]
+    assert np.abs(back - cf).max() <= 1e-13
+
+
+def test_genveffig_plane_wave():
+    cr = small("d").crystal
+    n = R.fft_grid(cr)
+    g0 = np.nonzero((cr.ivg[0] == 1) & (cr.ivg[1] == -2) & (cr.ivg[2] == 0))[0][0]
+    assert g0 < cr.ngvec
+    r1, r2, r3 = np.meshgrid(np.arange(n[0]) / n[0], np.arange(n[1]) / n[1], np.arange(n[2]) / n[2], indexing="ij")
+    v = np.cos(2 * math.pi * (1 * r1 - 2 * r2 + 0 * r3)).ravel(order="F")
+    vg = R.genveffig(cr, v, np.ones(cr.ngrtot))
+    gm = np.nonzero((cr.ivg[0] == -1) & (cr.ivg[1] == 2) & (cr.ivg[2] == 0))[0][0]
+    expect = np.zeros(cr.ngvec, dtype=np.complex128)
+    expect[g0] = expect[gm] = 0.5
+    assert np.abs(vg - expect).max() <= 1e-13
