@@ -52,7 +52,7 @@ def build_hip(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         list(ex.map(one, zip(srcs, objs)))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread", "-o", str(tmp)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lhipfft", "-lpthread", "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -8,6 +8,7 @@
 #include <thread>
 #include <vector>
 
+#include <hipfft/hipfft.h>
 #include <rocsolver/rocsolver.h>
 
 #include "exc_common.h"
@@ -94,6 +95,12 @@ struct exc_ctx {
   int gnt_lmaxvr;
   void *hml;
   size_t hml_bytes;
+  // Theta(r) / V(G) (exc_ctx_gencfun, exc_ctx_genveffig): 3-D FFT plan of the G box, the grid
+  // work array, Theta(r) on the grid, a staging array for V(r)
+  hipfftHandle fft;
+  bool have_fft, have_cfunir;
+  cplx *fz;
+  double *cfunir, *fin;
   hipEvent_t eig_fork;
   // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
   hipStream_t side;
@@ -310,6 +317,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->eg) hipFree(c->eg);
   if (c->dmw) hipFree(c->dmw);
   if (c->gnt) hipFree(c->gnt);
+  if (c->have_fft) hipfftDestroy(c->fft);
   if (c->hml) hipFree(c->hml);
   if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
@@ -1173,6 +1181,69 @@ int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const d
   if (nl && ololo) EXC_CHECK(hipMemcpyAsync(ololo, c->d.ololo, 8 * nl * nl * na, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_mt = true;
+  return EXC_OK;
+}
+
+namespace {
+// the FFT of the G box (every grid point one G) and its work arrays, made once per context
+int scf_prepare(exc_ctx *c) {
+  const exc_dev &d = c->d;
+  const size_t box = (size_t)d.box_n[0] * d.box_n[1] * d.box_n[2];
+  if (box != (size_t)c->dims.ngrtot) return EXC_ERR_UNSUPPORTED;   // the G list must fill the box
+  int rc;
+  if (!c->fz && (rc = dev_alloc(c, &c->fz, box))) return rc;
+  if (!c->cfunir && (rc = dev_alloc(c, &c->cfunir, box))) return rc;
+  if (!c->fin && (rc = dev_alloc(c, &c->fin, box))) return rc;
+  if (!c->have_fft) {
+    // Fortran order: the first index runs fastest, so it is hipFFT's last dimension
+    if (hipfftPlan3d(&c->fft, d.box_n[2], d.box_n[1], d.box_n[0], HIPFFT_Z2Z) != HIPFFT_SUCCESS) return EXC_ERR_HIP;
+    c->have_fft = true;
+  }
+  if (hipfftSetStream(c->fft, c->stream) != HIPFFT_SUCCESS) return EXC_ERR_HIP;
+  return EXC_OK;
+}
+}  // namespace
+
+int exc_ctx_gencfun(exc_ctx *c, const double *bvec, double *cfunig, double *cfunir) {
+  if (!c || !bvec) return EXC_ERR_ARG;
+  if (!c->have_atoms || !c->have_gvec) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if ((rc = scf_prepare(c))) return rc;
+  const int n = c->dims.ngrtot;
+  {
+    StageTimer t(c, EXC_STAGE_SCF);
+    if ((rc = exc_launch_cfunig(c->d, bvec, n, c->d.cfunig, c->fz, c->stream))) return rc;
+    if (hipfftExecZ2Z(c->fft, (hipfftDoubleComplex *)c->fz, (hipfftDoubleComplex *)c->fz, HIPFFT_BACKWARD) !=
+        HIPFFT_SUCCESS)
+      return EXC_ERR_HIP;
+    if ((rc = exc_launch_scf_real(n, c->fz, c->cfunir, c->stream))) return rc;
+  }
+  if (cfunig) EXC_CHECK(hipMemcpyAsync(cfunig, c->d.cfunig, 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  if (cfunir) EXC_CHECK(hipMemcpyAsync(cfunir, c->cfunir, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->have_cfunir = true;
+  return EXC_OK;
+}
+
+int exc_ctx_genveffig(exc_ctx *c, const double *veffir, double *veffig) {
+  if (!c || !veffir) return EXC_ERR_ARG;
+  if (!c->have_cfunir) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if ((rc = scf_prepare(c))) return rc;
+  const int n = c->dims.ngrtot, nv = c->dims.ngvec;
+  EXC_CHECK(hipMemcpyAsync(c->fin, veffir, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  {
+    StageTimer t(c, EXC_STAGE_SCF);
+    if ((rc = exc_launch_scf_product(n, c->fin, c->cfunir, c->fz, c->stream))) return rc;
+    if (hipfftExecZ2Z(c->fft, (hipfftDoubleComplex *)c->fz, (hipfftDoubleComplex *)c->fz, HIPFFT_FORWARD) !=
+        HIPFFT_SUCCESS)
+      return EXC_ERR_HIP;
+    if ((rc = exc_launch_scf_gather(c->d, nv, 1.0 / (double)n, c->fz, c->d.veffig, c->stream))) return rc;
+  }
+  if (veffig) EXC_CHECK(hipMemcpyAsync(veffig, c->d.veffig, 16 * (size_t)nv, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
   return EXC_OK;
 }
 

@@ -156,5 +156,10 @@ void exc_hml_gaunt_table(int lmaxmat, int lmaxvr, cplx *g);
 int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
                       const double *wt, const double *spr, const double *apwfr, const double *lofr,
                       const double *veffmt, double *rad, const cplx *gnt, hipStream_t s);
+// Theta(G), Theta(r), V(G) (k_scf.hip)
+int exc_launch_cfunig(const exc_dev &d, const double *bvec, int ngrtot, cplx *cf, cplx *z, hipStream_t s);
+int exc_launch_scf_real(int n, const cplx *z, double *out, hipStream_t s);
+int exc_launch_scf_product(int n, const double *a, const double *b, cplx *z, hipStream_t s);
+int exc_launch_scf_gather(const exc_dev &d, int ngvec, double scale, const cplx *z, cplx *out, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

@@ -1,0 +1,102 @@
+// Per-SCF producers of the interstitial tables the setup reads: Theta(G) and Theta(r)
+// (gencfun.f90:52-96, cfdamp = 0) and V(G) (genveffig.f90:23-33).  The FFT grid is the G box
+// of the context (ngrid(i) = box extent, every grid point one G); position of G on the grid
+// j1 + n1 (j2 + n2 j3) with negative components wrapped to n + i (gengvec.f90:100-117).  The
+// transforms are hipFFT's double-complex 3-D ones with zfftifc.f90's convention: forward
+// (sign -1) scaled by 1/N, backward (+1) unscaled.
+#include <math.h>
+
+#include "exc_common.h"
+
+namespace {
+
+constexpr int kST = 256;
+
+struct Bvec {
+  double b[9];   // reciprocal lattice vectors as columns: b[i + 3 j] = component i of b_j
+};
+
+__device__ __forceinline__ int grid_index(const exc_dev &d, const int *ivg, int g) {
+  int j[3];
+  for (int q = 0; q < 3; ++q) {
+    const int v = ivg[3 * g + q];
+    j[q] = v >= 0 ? v : d.box_n[q] + v;
+  }
+  return j[0] + d.box_n[0] * (j[1] + d.box_n[1] * j[2]);
+}
+
+// Theta(G) of every G, written at its grid position for the backward transform
+__global__ __launch_bounds__(kST) void scf_cfunig_kernel(exc_dev d, Bvec B, int ngrtot, cplx *cf, cplx *z) {
+  const int g = blockIdx.x * kST + threadIdx.x;
+  if (g >= ngrtot) return;
+  const int *iv = d.ivg + 3 * g;
+  double G[3];
+  for (int i = 0; i < 3; ++i) G[i] = B.b[i] * iv[0] + B.b[i + 3] * iv[1] + B.b[i + 6] * iv[2];
+  const double gc = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+  const double t1 = 4.0 * M_PI / d.omega;
+  cplx v = {g == 0 ? 1.0 : 0.0, 0.0};
+  for (int a = 0; a < d.natm; ++a) {
+    const double R = d.rmt[d.spidx[a]];
+    double ff;
+    if (gc > 1e-6) {
+      const double t2 = gc * R;
+      ff = t1 * (sin(t2) - t2 * cos(t2)) / (gc * gc * gc);
+    } else {
+      ff = (t1 / 3.0) * R * R * R;
+    }
+    const double *r = d.atposc + 3 * a;
+    const double ph = -(G[0] * r[0] + G[1] * r[1] + G[2] * r[2]);
+    v.re -= cos(ph) * ff;
+    v.im -= sin(ph) * ff;
+  }
+  cf[g] = v;
+  z[grid_index(d, d.ivg, g)] = v;
+}
+
+__global__ __launch_bounds__(kST) void scf_real_kernel(int n, const cplx *z, double *out) {
+  const int i = blockIdx.x * kST + threadIdx.x;
+  if (i < n) out[i] = z[i].re;
+}
+
+__global__ __launch_bounds__(kST) void scf_product_kernel(int n, const double *a, const double *b, cplx *z) {
+  const int i = blockIdx.x * kST + threadIdx.x;
+  if (i < n) z[i] = {a[i] * b[i], 0.0};
+}
+
+__global__ __launch_bounds__(kST) void scf_gather_kernel(exc_dev d, int ngvec, double scale, const cplx *z,
+                                                         cplx *out) {
+  const int g = blockIdx.x * kST + threadIdx.x;
+  if (g >= ngvec) return;
+  const cplx v = z[grid_index(d, d.ivg, g)];
+  out[g] = {v.re * scale, v.im * scale};
+}
+
+inline unsigned blocks(int n) { return (unsigned)((n + kST - 1) / kST); }
+
+}  // namespace
+
+int exc_launch_cfunig(const exc_dev &d, const double *bvec, int ngrtot, cplx *cf, cplx *z, hipStream_t s) {
+  Bvec B;
+  for (int i = 0; i < 9; ++i) B.b[i] = bvec[i];
+  hipLaunchKernelGGL(scf_cfunig_kernel, dim3(blocks(ngrtot)), dim3(kST), 0, s, d, B, ngrtot, cf, z);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_scf_real(int n, const cplx *z, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(scf_real_kernel, dim3(blocks(n)), dim3(kST), 0, s, n, z, out);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_scf_product(int n, const double *a, const double *b, cplx *z, hipStream_t s) {
+  hipLaunchKernelGGL(scf_product_kernel, dim3(blocks(n)), dim3(kST), 0, s, n, a, b, z);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_scf_gather(const exc_dev &d, int ngvec, double scale, const cplx *z, cplx *out, hipStream_t s) {
+  hipLaunchKernelGGL(scf_gather_kernel, dim3(blocks(ngvec)), dim3(kST), 0, s, d, ngvec, scale, z, out);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

@@ -84,13 +84,16 @@ SIGNATURES = {
     "exc_dmat_wfsize": [_P],
     "exc_ctx_hmlint": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "exc_ctx_set_radial_fr": [_P, _I, _P, _P, _P],
+    "exc_ctx_gencfun": [_P, _P, _P, _P],
+    "exc_ctx_genveffig": [_P, _P, _P],
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
-STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat", "hmlint")
+STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat", "hmlint",
+          "scf")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -278,6 +281,22 @@ class Context:
         return self._call("exc_eigen_batched_dev", self.h, nk, C.cast(ints, _P), C.cast(ints, _P),
                           C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
                           C.cast(ints, _P))
+
+    def gencfun(self):
+        """exc_ctx_gencfun with the crystal's bvec: (cfunig(ngrtot), cfunir(ngrtot)) as host arrays."""
+        cr = self.cr
+        bvec = np.asfortranarray(cr.bvec, dtype=np.float64)
+        cf = np.zeros(cr.ngrtot, dtype=np.complex128)
+        cir = np.zeros(cr.ngrtot)
+        self._call("exc_ctx_gencfun", self.h, _ptr(bvec), _ptr(cf), _ptr(cir))
+        return cf, cir
+
+    def genveffig(self, veffir):
+        """exc_ctx_genveffig: V(G) (ngvec,) from V(r) on the grid (ngrtot,)."""
+        v = np.ascontiguousarray(veffir, dtype=np.float64)
+        out = np.zeros(self.cr.ngvec, dtype=np.complex128)
+        self._call("exc_ctx_genveffig", self.h, _ptr(v), _ptr(out))
+        return out
 
     def set_radial_fr(self, rd):
         """exc_ctx_set_radial_fr: match's boundary values from the radial functions of rd

@@ -248,6 +248,21 @@ int exc_gendmatmt_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, in
                               const double *const *evecfv, const double *const *evecsv,
                               const double *const *occ, const double *wkpt, double *dm);
 
+/* Per-SCF producers of the interstitial tables, on the GPU.  The FFT grid is the G box of
+ * exc_ctx_set_gvec (ngrid(i) = intgv(2,i) - intgv(1,i) + 1, every grid point one of the ngrtot
+ * G-vectors; EXC_ERR_UNSUPPORTED otherwise), arrays on it in Fortran order, positions as
+ * igfft (gengvec.f90:100-117); transforms as zfftifc.f90: forward scaled by 1/N.
+ * exc_ctx_gencfun — gencfun.f90:13 with cfdamp = 0: Theta(G) = delta_G0 - sum_a
+ *   e^(-i G.r_a) (4 pi/Omega)(sin GR - GR cos GR)/G^3 for every G (bvec(3,3): reciprocal
+ *   lattice vectors as columns, mod_lattice), which replaces the context's cfunig, and
+ *   Theta(r) = its backward transform (real part), kept for exc_ctx_genveffig.  Optional host
+ *   outputs cfunig(ngrtot) complex, cfunir(ngrtot) real.
+ * exc_ctx_genveffig — genveffig.f90:13: V(G) = forward transform of veffir(ngrtot) * Theta(r)
+ *   at the first ngvec G, which replaces the context's veffig; optional host output
+ *   veffig(ngvec) complex.  Needs exc_ctx_gencfun first. */
+int exc_ctx_gencfun(exc_ctx *ctx, const double *bvec, double *cfunig, double *cfunir);
+int exc_ctx_genveffig(exc_ctx *ctx, const double *veffir, double *veffig);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
@@ -271,7 +286,8 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_EIGEN 10   /* exc_eigen_dev                                                 */
 #define EXC_STAGE_DMAT 11    /* exc_gendmatmt_dev (muffin-tin density matrix)                 */
 #define EXC_STAGE_HMLINT 12  /* exc_ctx_hmlint (muffin-tin radial integrals and Gaunt sums)   */
-#define EXC_NSTAGE 13
+#define EXC_STAGE_SCF 13     /* exc_ctx_gencfun, exc_ctx_genveffig (Theta, V(G), FFTs)        */
+#define EXC_NSTAGE 14
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
 /* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,
