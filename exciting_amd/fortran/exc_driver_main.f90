@@ -6,8 +6,9 @@
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
 ! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).  With a fifth file of radial
 ! data it first runs set_radial_gpu (match's boundary values from apwfr) and hmlint_gpu (hmlint +
-! olprad), whose results replace the input's dmat and muffin-tin matrices.
-!   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev] [<radial.bin>]
+! olprad), whose results replace the input's dmat and muffin-tin matrices; with a sixth file
+! (bvec, V(r)) it runs gencfun_gpu and genveffig_gpu, whose Theta(G) and V(G) replace the input's.
+!   exc_driver_main <input.bin> <output.bin> [<eigen.bin> | -] [nev] [<radial.bin> | -] [<scf.bin>]
 program exc_driver_main
   use, intrinsic :: iso_c_binding
   use exc_lapw_c, only: exc_dims
@@ -27,7 +28,9 @@ program exc_driver_main
   complex(8), allocatable :: vecs(:, :), dm(:, :, :)
   real(8) :: wkpt
   integer :: ws, lmaxvr, nrmtmax
-  character(len=512) :: frad
+  character(len=512) :: frad, fscf
+  real(8) :: bvec(3, 3)
+  real(8), allocatable :: veffir(:), cfunir(:)
   integer(c_int), allocatable :: nrmt(:)
   real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:)
 
@@ -65,9 +68,10 @@ program exc_driver_main
 
   call exc_gpu_init(dims, ngkmax, natoms, rmt, atposc, nlorb, lorbl, idxlo, dmat, haa, loa, lolo, &
                     oalo, ololo, ivg, ivgig, cfunig, veffig, ffacg)
-  if (command_argument_count() >= 5) then
+  frad = "-"
+  if (command_argument_count() >= 5) call get_command_argument(5, frad)
+  if (trim(frad) /= "-") then
     ! radial data file: lmaxvr, nrmtmax, nrmt(nspecies), spr, apwfr, lofr, veffmt
-    call get_command_argument(5, frad)
     open(newunit=u, file=trim(frad), access="stream", form="unformatted", status="old")
     read(u) lmaxvr, nrmtmax
     allocate(nrmt(dims%nspecies), spr(nrmtmax, dims%nspecies))
@@ -78,6 +82,16 @@ program exc_driver_main
     close(u)
     call set_radial_gpu(nrmt, spr, apwfr)
     call hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
+  end if
+  if (command_argument_count() >= 6) then
+    ! interstitial file: bvec(3,3), veffir(ngrtot) -> Theta(G), Theta(r), V(G) on the GPU
+    call get_command_argument(6, fscf)
+    open(newunit=u, file=trim(fscf), access="stream", form="unformatted", status="old")
+    allocate(veffir(dims%ngrtot), cfunir(dims%ngrtot))
+    read(u) bvec, veffir
+    close(u)
+    call gencfun_gpu(bvec, cfunig, cfunir)
+    call genveffig_gpu(veffir, veffig)
   end if
   nlotot = exc_gpu_nlotot()
   nmat = ngp + nlotot

@@ -52,6 +52,21 @@ module exc_lapw_c
       real(c_double), intent(in) :: oalo(*), ololo(*)
     end function
 
+    integer(c_int) function exc_ctx_gencfun(ctx, bvec, cfunig, cfunir) bind(c, name="exc_ctx_gencfun")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: bvec(*)
+      complex(c_double_complex), intent(inout) :: cfunig(*)
+      real(c_double), intent(inout) :: cfunir(*)
+    end function
+
+    integer(c_int) function exc_ctx_genveffig(ctx, veffir, veffig) bind(c, name="exc_ctx_genveffig")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: veffir(*)
+      complex(c_double_complex), intent(inout) :: veffig(*)
+    end function
+
     integer(c_int) function exc_ctx_set_radial_fr(ctx, nrmtmax, nrmt, spr, apwfr) bind(c, name="exc_ctx_set_radial_fr")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

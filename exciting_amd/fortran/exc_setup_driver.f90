@@ -8,6 +8,8 @@
 !   hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
 !                                                                 hmlint.f90:14 + olprad.f90:13
 !   set_radial_gpu(nrmt, spr, apwfr)                             match.f90:139-148 (zd by polynom)
+!   gencfun_gpu(bvec, cfunig, cfunir)                             gencfun.f90:13 (cfdamp = 0)
+!   genveffig_gpu(veffir, veffig)                                 genveffig.f90:13
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -26,7 +28,8 @@ module exc_setup_gpu
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
-            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu
+            solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu, gencfun_gpu, &
+            genveffig_gpu
 
 contains
 
@@ -207,5 +210,23 @@ contains
     call check(rc, "set_radial_gpu")
     if (rc > 0) write(*, '("Warning(set_radial_gpu): singular matching matrix, info = ", I0)') rc
   end subroutine set_radial_gpu
+
+  ! gencfun.f90:13 (cfdamp = 0): Theta(G) (cfunig(ngrtot)) and Theta(r) (cfunir(ngrtot)) on the
+  ! GPU from the atoms of the context; bvec(3,3) as mod_lattice.  Theta(G) replaces the
+  ! context's cfunig; Theta(r) stays on the device for genveffig_gpu.
+  subroutine gencfun_gpu(bvec, cfunig, cfunir)
+    real(8), intent(in) :: bvec(3, 3)
+    complex(8), intent(out) :: cfunig(*)
+    real(8), intent(out) :: cfunir(*)
+    call check(exc_ctx_gencfun(ctx, bvec, cfunig, cfunir), "gencfun")
+  end subroutine gencfun_gpu
+
+  ! genveffig.f90:13: V(G) = forward FFT of veffir(ngrtot) Theta(r) (veffig(ngvec)), which
+  ! replaces the context's veffig.  After gencfun_gpu.
+  subroutine genveffig_gpu(veffir, veffig)
+    real(8), intent(in) :: veffir(*)
+    complex(8), intent(out) :: veffig(*)
+    call check(exc_ctx_genveffig(ctx, veffir, veffig), "genveffig")
+  end subroutine genveffig_gpu
 
 end module exc_setup_gpu

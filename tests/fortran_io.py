@@ -79,3 +79,10 @@ def write_radial(path, cr, rd):
         _w(f, rd.apwfr, np.float64)
         _w(f, rd.lofr, np.float64)
         _w(f, rd.veffmt, np.float64)
+
+
+def write_scf(path, cr, veffir):
+    """bvec(3,3) and V(r) on the FFT grid for the driver's gencfun_gpu / genveffig_gpu step."""
+    with open(path, "wb") as f:
+        _w(f, cr.bvec, np.float64)
+        _w(f, veffir, np.float64)

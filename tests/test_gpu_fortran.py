@@ -12,7 +12,7 @@ import pytest
 
 from conftest import small, workload
 from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, write_eigen, write_input,
-                        write_radial)
+                        write_radial, write_scf)
 
 pytestmark = pytest.mark.gpu
 
@@ -122,4 +122,30 @@ def test_fortran_hmlint_then_setup(tmp_path):
     eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
     eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
     print(f"Fortran set_radial_gpu + hmlint_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
+    assert eh <= 1e-12 and eo <= 1e-12
+
+
+def test_fortran_gencfun_genveffig_then_setup(tmp_path):
+    """gencfun_gpu and genveffig_gpu (Fortran) before the setup: H and O against the oracle
+    setup with the oracle's Theta(G) and V(G)."""
+    import dataclasses
+    from oracle import lapw_ref as R
+    from test_gpu_scf import _veffir
+    w = small("b")
+    cr, kp = w.crystal, w.kpoints[0]
+    v = _veffir(cr, 7)
+    fin, fout, fscf = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "scf.bin"
+    write_input(fin, cr, kp)
+    write_scf(fscf, cr, v)
+    r = subprocess.run([EXE, str(fin), str(fout), "-", "0", "-", str(fscf)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    _, H, O = read_output(fout, cr, kp.igpig.shape[0])
+    cf, cir = R.gencfun(cr)
+    cr2 = dataclasses.replace(cr, cfunig=cf, veffig=R.genveffig(cr, v, cir))
+    H_ref, O_ref, _ = R.setup(cr2, kp)
+    eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
+    eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
+    print(f"Fortran gencfun_gpu + genveffig_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
     assert eh <= 1e-12 and eo <= 1e-12
