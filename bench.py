@@ -194,9 +194,11 @@ def run_eigen(args, ctx, batch, w, olpflag, nev, log):
 
 
 def run_hmlint(args, w, log):
-    """The muffin-tin integrals of one SCF iteration (exc_ctx_hmlint: hmlint + olprad) on seeded
-    radial data of the workload's atoms (meshes of 500 / 460 points, lmaxvr 6), on a context of
-    its own; with the CPU baseline on, the numpy oracle's time for the same call beside it."""
+    """The per-SCF producers on a context of their own: the muffin-tin integrals
+    (exc_ctx_hmlint: hmlint + olprad) on seeded radial data of the workload's atoms (meshes of
+    500 / 460 points, lmaxvr 6), and Theta(G), Theta(r), V(G) (exc_ctx_gencfun + genveffig) on
+    the workload's G box with a seeded V(r); with the CPU baseline on, the numpy oracle's time
+    for the same calls beside them."""
     import torch
     from exciting_amd import lapw, synth
     cr = w.crystal
@@ -224,6 +226,34 @@ def run_hmlint(args, w, log):
     log(f"  muffin-tin integrals (hmlint + olprad, {cr.natm} atoms): {res['ms_per_call']:.2f} ms per call, "
         f"kernels {res['kernel_ms']:.3f} ms" + (f", numpy oracle {res['cpu_oracle_ms']:.0f} ms"
                                                  if res["cpu_oracle_ms"] else ""))
+    # Theta(G), Theta(r) and V(G) of the G box (gencfun + genveffig, hipFFT)
+    rng = np.random.default_rng(20261202)
+    v = rng.standard_normal(cr.ngrtot)
+    ctx.gencfun()
+    ctx.genveffig(v)
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.gencfun()
+        ctx.genveffig(v)
+    dt2 = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    ms2, cnt2 = ctx.read_timing()["scf"]
+    grid = [int(cr.intgv[i, 1] - cr.intgv[i, 0] + 1) for i in range(3)]
+    res["interstitial_tables"] = {"ms_per_call": 1000.0 * dt2, "kernel_ms": ms2 / reps,
+                                  "fft_grid": grid, "ngvec": int(cr.ngvec), "cpu_oracle_ms": None}
+    if not args.no_cpu_baseline:
+        from oracle import lapw_ref as R
+        t1 = time.perf_counter()
+        cf, cir = R.gencfun(cr)
+        R.genveffig(cr, v, cir)
+        res["interstitial_tables"]["cpu_oracle_ms"] = 1000.0 * (time.perf_counter() - t1)
+    it = res["interstitial_tables"]
+    log(f"  Theta(G), Theta(r), V(G) (gencfun + genveffig, grid {grid}): {it['ms_per_call']:.2f} ms per call, "
+        f"kernels + FFTs {it['kernel_ms']:.3f} ms" + (f", numpy oracle {it['cpu_oracle_ms']:.0f} ms"
+                                                     if it["cpu_oracle_ms"] else ""))
     return res
 
 
