@@ -10,6 +10,8 @@ SCF iteration, the producers of the setup's h_a, oalo and ololo:
 * ``hmlint``        — src_eigensystem/hmlint.f90:33-34 = mt_kin.f90:96-221 + mt_pot.f90:111-307
                       (ValenceRelativity 'none', spinless: no rmtable factor, no bxcmt, no SO)
 * ``olprad``        — src_eigensystem/olprad.f90:90-125 (oalo, ololo; no IORA terms)
+* ``genrhomt``      — genrhomt.f90:34-150 (the muffin-tin density rho_lm(r) in real harmonics
+                      from the density matrix of gendmatmt, listgnt as init1.f90:604-621)
 * ``polynom``       — src_lib/polynom.f90 (np = 4: the cubic through four points and its
                       derivatives at x), and ``dmat_from_radial`` — match.f90:85, 139-148
                       (the boundary values zd of the radial functions at R)
@@ -341,3 +343,47 @@ def dmat_from_radial(cr, rd):
                 for io1 in range(ao):
                     d[io1, io2, l, ias] = polynom(io1, xa, ya, float(cr.rmt[is_]))
     return d
+
+
+def genrhomt(cr, rd, dm):
+    """genrhomt.f90:34-150: densitymt(lmmaxvr, nrmtmax, natm) from dm(wfsize, wfsize, natm)
+    (rows f = (l, io, m) m fastest, then the atom's lo functions (ilo, m)):
+        rho_lm2(r) = sum_{a, b} u_a(r) u_b(r) sum_{m1, m3} Re(dm(f1, f3) conj(G(lm1, lm2, lm3)))
+    over APW-APW pairs, twice the APW-lo ones, and the lo-lo ones, with G(lm1, lm2, lm3) =
+    int conj(Y_lm1) R_lm2 Y_lm3 (listgnt(i, lm3, lm1), init1.f90:569, 619)."""
+    ao, L = cr.apwordmax, cr.lmaxmat
+    maxaa = ao * cr.lmmaxapw
+    lmv = (rd.lmaxvr + 1) ** 2
+    G = gaunt_table(max(L, int(cr.lorbl.max()) if cr.nlotot else 0), rd.lmaxvr)
+    nrmax = rd.spr.shape[0]
+    out = np.zeros((lmv, nrmax, cr.natm), order="F")
+    for ias in range(cr.natm):
+        is_ = cr.spidx[ias]
+        n = int(rd.nrmt[is_])
+        rho = np.zeros((lmv, n))
+        # functions: (row offset in dm, l, u) — APW in dm order (l, io, m), then lo (ilo, m)
+        funcs = []
+        off = 0
+        for l in range(L + 1):
+            for io in range(ao):
+                funcs.append((off, l, rd.apwfr[:n, 0, io, l, ias], False))
+                off += 2 * l + 1
+        off = maxaa
+        for ilo in range(cr.nlorb[is_]):
+            l = int(cr.lorbl[ilo, is_])
+            funcs.append((off, l, rd.lofr[:n, 0, ilo, ias], True))
+            off += 2 * l + 1
+        for (o1, l1, u1, lo1) in funcs:
+            for (o3, l3, u3, lo3) in funcs:
+                if lo1 and not lo3:
+                    continue                      # the lo-APW block enters as twice APW-lo
+                w = 2.0 if (lo3 and not lo1) else 1.0
+                fac = np.zeros(lmv)
+                for m1 in range(-l1, l1 + 1):
+                    for m3 in range(-l3, l3 + 1):
+                        z = dm[o1 + m1 + l1, o3 + m3 + l3, ias]
+                        g = G[l1 * l1 + l1 + m1, :lmv, l3 * l3 + l3 + m3]
+                        fac += w * (z * np.conj(g)).real
+                rho += fac[:, None] * (u1 * u3)[None, :]
+        out[:, :n, ias] = rho
+    return out

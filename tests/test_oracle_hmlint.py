@@ -247,3 +247,40 @@ def test_hmlint_elements_by_quadrature():
                         assert abs(lolo[r, r3, ias] - z) <= 1e-11 * scale
                         r3 += 1
                 r += 1
+
+
+def test_genrhomt_reproduces_potential_energy():
+    """The muffin-tin density from the density matrix (genrhomt) integrated against the
+    potential equals tr(dm h_pot), with h_pot the potential part of the oracle's hmlint (its
+    matrix minus the one at V = 0) in the density matrix's order: sum_lm int rho_lm V_lm r^2 dr
+    = sum_ff' dm(f, f') h_pot(f', f) for any coefficients, by the same radial rule."""
+    import dataclasses
+    from oracle import lapw_ref as R
+    w = small("d")
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(120, 100), lmaxvr=4)
+    apw = R.match(cr, kp)
+    rng = np.random.default_rng(31)
+    nmat, nst = kp.ngp + cr.nlotot, 7
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    dm = R.gendmatmt(cr, apw, kp.ngp, C, rng.uniform(0, 2, nst), 0.7)
+    rho = M.genrhomt(cr, rd, dm)
+    h1 = M.hmlint(cr, rd)
+    h0 = M.hmlint(cr, dataclasses.replace(rd, veffmt=np.zeros_like(rd.veffmt)))
+    order = R.dmat_order(cr)
+    naa, maxaa, ws = len(order), cr.apwordmax * cr.lmmaxapw, R.dmat_wfsize(cr)
+    e_rho = e_dm = 0.0
+    for a in range(cr.natm):
+        is_ = cr.spidx[a]
+        n = int(rd.nrmt[is_])
+        x = rd.spr[:n, is_]
+        e_rho += sum(M.fderiv_int(x, rho[q, :n, a] * rd.veffmt[q, :n, a] * x * x) for q in range(rho.shape[0]))
+        hp = [h1[i][..., a] - h0[i][..., a] for i in range(3)]
+        Hd = np.zeros((ws, ws), dtype=np.complex128)
+        Hd[:naa, :naa] = hp[0][np.ix_(order, order)]
+        ns = cr.losize[is_] if cr.nlorb[is_] else 0
+        Hd[maxaa:maxaa + ns, :naa] = hp[1][:ns][:, order]
+        Hd[:naa, maxaa:maxaa + ns] = hp[1][:ns][:, order].conj().T
+        Hd[maxaa:maxaa + ns, maxaa:maxaa + ns] = hp[2][:ns, :ns]
+        e_dm += np.trace(dm[:, :, a] @ Hd).real
+    assert abs(e_rho - e_dm) <= 1e-11 * abs(e_dm), (e_rho, e_dm)
