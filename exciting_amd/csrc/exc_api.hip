@@ -95,6 +95,7 @@ struct exc_ctx {
   int gnt_lmaxvr;
   void *hml;
   size_t hml_bytes;
+  int hml_lmaxvr, hml_nrmtmax;   // of the last exc_ctx_hmlint (-1: none)
   // Theta(r) / V(G) (exc_ctx_gencfun, exc_ctx_genveffig): 3-D FFT plan of the G box, the grid
   // work array, Theta(r) on the grid, a staging array for V(r)
   hipfftHandle fft;
@@ -288,6 +289,7 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->bsplit_variant = -1;
   c->eig_variant = -1;
   c->gnt_lmaxvr = -1;
+  c->hml_lmaxvr = -1;
   {
     hipDeviceProp_t prop;
     c->ncu = (hipGetDeviceProperties(&prop, c->device) == hipSuccess) ? prop.multiProcessorCount : 256;
@@ -1107,6 +1109,35 @@ int exc_ctx_set_radial_fr(exc_ctx *c, int nrmtmax, const int *nrmt, const double
   return exc_ctx_set_radial(c, dmat.data());
 }
 
+namespace {
+// device buffer of exc_ctx_hmlint: weights, meshes, apwfr, lofr, veffmt, radial integrals, ints
+struct HmlLayout {
+  int nf;
+  size_t n_w, n_apw, n_lo, n_v, n_rad, o_spr, o_apw, o_lo, o_v, o_rad, o_i, total;
+};
+HmlLayout hml_layout(const exc_ctx *c, int lmaxvr, int nrmtmax) {
+  const exc_dev &d = c->d;
+  const int ns = c->dims.nspecies, na = d.natm, nlomax = c->dims.nlomax, ao = d.apwordmax;
+  const int lmv = (lmaxvr + 1) * (lmaxvr + 1);
+  HmlLayout L;
+  L.nf = ao * (d.lmaxmat + 1) + (nlomax > 0 ? nlomax : 0);
+  L.n_w = (size_t)nrmtmax * ns;
+  L.n_apw = (size_t)nrmtmax * 2 * ao * (d.lmaxapw + 1) * na;
+  L.n_lo = (size_t)nrmtmax * 2 * (nlomax > 0 ? nlomax : 0) * na;
+  L.n_v = (size_t)lmv * nrmtmax * na;
+  L.n_rad = (size_t)na * (lmv + 2) * L.nf * L.nf;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  L.o_spr = al(8 * L.n_w);
+  L.o_apw = L.o_spr + al(8 * L.n_w);
+  L.o_lo = L.o_apw + al(8 * L.n_apw);
+  L.o_v = L.o_lo + al(8 * L.n_lo);
+  L.o_rad = L.o_v + al(8 * L.n_v);
+  L.o_i = L.o_rad + al(8 * L.n_rad);
+  L.total = L.o_i + sizeof(int) * (ns + ns + (size_t)(nlomax > 0 ? nlomax : 1) * ns);
+  return L;
+}
+}  // namespace
+
 int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const double *spr, const double *apwfr,
                    const double *lofr, const double *veffmt, double *haa, double *loa, double *lolo, double *oalo,
                    double *ololo) {
@@ -1139,15 +1170,11 @@ int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const d
     c->gnt_lmaxvr = lmaxvr;
   }
   // inputs and the radial integrals in one device buffer
-  const int nf = ao * (d.lmaxmat + 1) + (nlomax > 0 ? nlomax : 0);
-  const size_t n_w = (size_t)nrmtmax * ns, n_apw = (size_t)nrmtmax * 2 * ao * (d.lmaxapw + 1) * na;
-  const size_t n_lo = (size_t)nrmtmax * 2 * (nlomax > 0 ? nlomax : 0) * na, n_v = (size_t)lmv * nrmtmax * na;
-  const size_t n_rad = (size_t)na * (lmv + 2) * nf * nf;
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t o_spr = al(8 * n_w), o_apw = o_spr + al(8 * n_w), o_lo = o_apw + al(8 * n_apw), o_v = o_lo + al(8 * n_lo);
-  const size_t o_rad = o_v + al(8 * n_v), o_i = o_rad + al(8 * n_rad);
-  const size_t total = o_i + sizeof(int) * (ns + ns + (size_t)(nlomax > 0 ? nlomax : 1) * ns);
-  if ((rc = grow(&c->hml, &c->hml_bytes, total))) return rc;
+  const HmlLayout H = hml_layout(c, lmaxvr, nrmtmax);
+  const size_t n_w = H.n_w, n_apw = H.n_apw, n_lo = H.n_lo, n_v = H.n_v;
+  const size_t o_spr = H.o_spr, o_apw = H.o_apw, o_lo = H.o_lo, o_v = H.o_v, o_rad = H.o_rad, o_i = H.o_i;
+  c->hml_lmaxvr = -1;
+  if ((rc = grow(&c->hml, &c->hml_bytes, H.total))) return rc;
   char *b = (char *)c->hml;
   std::vector<int> ints(nrmt, nrmt + ns);
   ints.insert(ints.end(), c->h_nlorb.begin(), c->h_nlorb.end());
@@ -1181,6 +1208,40 @@ int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const d
   if (nl && ololo) EXC_CHECK(hipMemcpyAsync(ololo, c->d.ololo, 8 * nl * nl * na, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_mt = true;
+  c->hml_lmaxvr = lmaxvr;
+  c->hml_nrmtmax = nrmtmax;
+  return EXC_OK;
+}
+
+int exc_genrhomt_dev(exc_ctx *c, const double *dm, double *rhomt) {
+  if (!c || !dm || !rhomt) return EXC_ERR_ARG;
+  if (c->hml_lmaxvr < 0 || c->gnt_lmaxvr != c->hml_lmaxvr) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int lmaxvr = c->hml_lmaxvr, nrmtmax = c->hml_nrmtmax, ns = c->dims.nspecies;
+  const HmlLayout H = hml_layout(c, lmaxvr, nrmtmax);
+  const char *b = (const char *)c->hml;
+  const int *di = (const int *)(b + H.o_i);
+  StageTimer t(c, EXC_STAGE_HMLINT);
+  return exc_launch_genrhomt(c->d, (lmaxvr + 1) * (lmaxvr + 1), nrmtmax, di, di + ns, di + 2 * ns,
+                             (const double *)(b + H.o_apw), (const double *)(b + H.o_lo), c->gnt, (const cplx *)dm,
+                             c->d.naa + c->d.dm_maxlo, (double *)(b + H.o_rad), rhomt, c->stream);
+}
+
+int exc_genrhomt(exc_ctx *c, const double *dm, double *rhomt) {
+  if (!c || !dm || !rhomt) return EXC_ERR_ARG;
+  if (c->hml_lmaxvr < 0) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t ws = c->d.naa + c->d.dm_maxlo, na = c->d.natm;
+  const size_t b_d = 16 * ws * ws * na, b_r = 8 * (size_t)(c->hml_lmaxvr + 1) * (c->hml_lmaxvr + 1) * c->hml_nrmtmax * na;
+  const size_t o_r = (b_d + 255) & ~(size_t)255;
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_r + b_r))) return rc;
+  char *base = (char *)c->stage;
+  EXC_CHECK(hipMemcpyAsync(base, dm, b_d, hipMemcpyHostToDevice, c->stream));
+  if ((rc = exc_genrhomt_dev(c, (const double *)base, (double *)(base + o_r)))) return rc;
+  EXC_CHECK(hipMemcpyAsync(rhomt, base + o_r, b_r, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
   return EXC_OK;
 }
 

@@ -153,6 +153,9 @@ int exc_launch_dmat_reduce(cplx *dm, const cplx *per, int np, size_t pstride, si
 // muffin-tin integrals (k_hmlint.hip)
 void exc_hml_spline_weights(int n, const double *x, double *wt);
 void exc_hml_gaunt_table(int lmaxmat, int lmaxvr, cplx *g);
+int exc_launch_genrhomt(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
+                        const double *apwfr, const double *lofr, const cplx *gnt, const cplx *dm, int ws, double *fbuf,
+                        double *rhomt, hipStream_t s);
 int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
                       const double *wt, const double *spr, const double *apwfr, const double *lofr,
                       const double *veffmt, double *rad, const cplx *gnt, hipStream_t s);

@@ -18,6 +18,11 @@
 //    with G the Gaunt coefficient int conj(Y_lm1) R_lm2 Y_lm3 (gauntyry.f90, nonzero only for
 //    m2 = +-(m1 - m3)), stored per (lm1, lm3) as [l2][sign of m2]; rows and columns in the
 //    setup's order (APW (l, m, io) io fastest, lo (ilo, m) m fastest); and oalo, ololo from S.
+// And the reverse, the muffin-tin density from a density matrix (genrhomt.f90:34-150):
+//  * rho_factor_kernel: F_lm2(a, b) = w sum_{m1, m3} Re(dm(f1, f3) conj(G(lm1, lm2, lm3)))
+//    (w = 2 for APW-lo pairs, the lo-APW block then 0), one thread per (atom, a, b, l2);
+//  * rho_radial_kernel: rho_lm2(r) = sum_{a, b} F_lm2(a, b) u_a(r) u_b(r), one thread per
+//    (atom, lm2, r), on the radial functions of the last hmlint call.
 #include <math.h>
 
 #include <vector>
@@ -45,6 +50,9 @@ struct HmlArgs {
   const int *dm_lo;                           // [natm][2]: first lo, count
   cplx *haa, *loa, *lolo;
   double *oalo, *ololo;
+  const cplx *dm;                             // genrhomt: dm(ws, ws, natm)
+  int ws;
+  double *rho;                                // genrhomt: rhomt(lmv, nrmtmax, natm)
 };
 
 __device__ __forceinline__ const double *radial_u(const HmlArgs &A, int atom, int a, int *l) {
@@ -178,6 +186,78 @@ __global__ __launch_bounds__(kHT) void hml_assemble_kernel(HmlArgs A) {
   }
 }
 
+// row of function a's m = -l in the density matrix: APW (l, io) at apwordmax l^2 + io (2l+1)
+// (rows (l, io, m) m fastest, gendmatmt.f90:92-101), lo ilo after maxaa (ilo, m)
+__device__ __forceinline__ int dm_row(const HmlArgs &A, int atom, int a) {
+  if (a < A.naf) {
+    const int io = a % A.ao, l = a / A.ao;
+    return A.ao * l * l + io * (2 * l + 1);
+  }
+  const int is = A.spidx[atom];
+  int r = A.naa;
+  for (int i = 0; i < a - A.naf; ++i) r += 2 * A.lorbl[is * A.nlomax + i] + 1;
+  return r;
+}
+
+constexpr int kMaxL2 = 16;   // lmaxvr supported by genrhomt
+
+// F[atom][lm2][a][b] = w sum_{m1, m3} Re(dm(f1, f3) conj(G(lm1, lm2, lm3))) for the l2 of this
+// thread (genrhomt.f90:60-71, 96-107, 131-142; w = 2 for APW-lo, the lo-APW block is 0)
+__global__ __launch_bounds__(kHT) void rho_factor_kernel(HmlArgs A) {
+  const int atom = blockIdx.y, L2 = isqrt_lm(A.lmv - 1) + 1;
+  const int t = blockIdx.x * kHT + threadIdx.x;
+  if (t >= A.nf * A.nf * L2) return;
+  const int l2 = t % L2, ab = t / L2, a = ab / A.nf, b = ab % A.nf;
+  const int lmm = (A.lmaxmat + 1) * (A.lmaxmat + 1);
+  double *F = A.rad + (size_t)atom * A.lmv * A.nf * A.nf + (size_t)a * A.nf + b;
+  const size_t qs = (size_t)A.nf * A.nf;
+  double acc[2 * kMaxL2 + 1];
+  for (int i = 0; i <= 2 * l2; ++i) acc[i] = 0.0;
+  int la = 0, lb = 0;
+  const double *ua = radial_u(A, atom, a, &la), *ub = radial_u(A, atom, b, &lb);
+  if (ua && ub && !(a >= A.naf && b < A.naf)) {
+    const double w = (a < A.naf && b >= A.naf) ? 2.0 : 1.0;
+    const int o1 = dm_row(A, atom, a), o3 = dm_row(A, atom, b);
+    const cplx *dm = A.dm + (size_t)atom * A.ws * A.ws;
+    for (int m1 = -la; m1 <= la; ++m1)
+      for (int m3 = -lb; m3 <= lb; ++m3) {
+        const int d = m1 > m3 ? m1 - m3 : m3 - m1;
+        if (d > l2) continue;
+        const cplx z = dm[(o1 + m1 + la) + (size_t)A.ws * (o3 + m3 + lb)];
+        const cplx *g = A.gnt + (((size_t)(la * la + la + m1) * lmm + (lb * lb + lb + m3)) * L2 + l2) * 2;
+        acc[l2 + d] = fma(z.re, g[0].re, fma(z.im, g[0].im, acc[l2 + d]));
+        if (d > 0) acc[l2 - d] = fma(z.re, g[1].re, fma(z.im, g[1].im, acc[l2 - d]));
+      }
+    for (int i = 0; i <= 2 * l2; ++i) acc[i] *= w;
+  }
+  for (int m2 = -l2; m2 <= l2; ++m2) F[(size_t)(l2 * l2 + l2 + m2) * qs] = acc[l2 + m2];
+}
+
+// rhomt(lm2, r, atom) = sum_{a, b} F[atom][lm2][a][b] u_a(r) u_b(r) (genrhomt.f90:72-77)
+__global__ __launch_bounds__(kHT) void rho_radial_kernel(HmlArgs A) {
+  const int r = blockIdx.x * kHT + threadIdx.x, lm2 = blockIdx.y, atom = blockIdx.z;
+  if (r >= A.nrmtmax) return;
+  const int nr = A.nrmt[A.spidx[atom]];
+  double s = 0.0;
+  if (r < nr) {
+    const double *F = A.rad + ((size_t)atom * A.lmv + lm2) * A.nf * A.nf;
+    for (int a = 0; a < A.nf; ++a) {
+      int la;
+      const double *ua = radial_u(A, atom, a, &la);
+      if (!ua) continue;
+      const double va = ua[r];
+      double t = 0.0;
+      for (int b = 0; b < A.nf; ++b) {
+        int lb;
+        const double *ub = radial_u(A, atom, b, &lb);
+        if (ub) t = fma(F[a * A.nf + b], ub[r], t);
+      }
+      s = fma(va, t, s);
+    }
+  }
+  A.rho[lm2 + (size_t)A.lmv * (r + (size_t)A.nrmtmax * atom)] = s;
+}
+
 // ------------------------------------------------------------------ host helpers
 
 long double lfact(int n) {
@@ -305,9 +385,10 @@ void exc_hml_gaunt_table(int lmaxmat, int lmaxvr, cplx *g) {
         }
 }
 
-int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
-                      const double *wt, const double *spr, const double *apwfr, const double *lofr,
-                      const double *veffmt, double *rad, const cplx *gnt, hipStream_t s) {
+namespace {
+HmlArgs hml_args(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
+                 const double *wt, const double *spr, const double *apwfr, const double *lofr, const double *veffmt,
+                 double *rad, const cplx *gnt) {
   HmlArgs A;
   A.natm = d.natm;
   A.ao = d.apwordmax;
@@ -338,6 +419,17 @@ int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, c
   A.lolo = d.lolo;
   A.oalo = d.oalo;
   A.ololo = d.ololo;
+  A.dm = nullptr;
+  A.ws = 0;
+  A.rho = nullptr;
+  return A;
+}
+}  // namespace
+
+int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
+                      const double *wt, const double *spr, const double *apwfr, const double *lofr,
+                      const double *veffmt, double *rad, const cplx *gnt, hipStream_t s) {
+  HmlArgs A = hml_args(d, lmv, nrmtmax, nrmt, nlorb, lorbl, wt, spr, apwfr, lofr, veffmt, rad, gnt);
   if (d.natm > 65535 || lmv + 2 > 65535) return EXC_ERR_ARG;
   if ((size_t)nrmtmax * sizeof(double) > 64 * 1024) return EXC_ERR_ARG;
   hipLaunchKernelGGL(hml_radial_kernel, dim3(lmv + 2, d.natm), dim3(kHT), nrmtmax * sizeof(double), s, A);
@@ -345,6 +437,23 @@ int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, c
   const size_t n = (size_t)d.naa * d.naa + (size_t)d.maxnlo * d.naa + (size_t)d.maxnlo * d.maxnlo +
                    (size_t)A.ao * A.nlomax + (size_t)A.nlomax * A.nlomax;
   hipLaunchKernelGGL(hml_assemble_kernel, dim3((unsigned)((n + kHT - 1) / kHT), d.natm), dim3(kHT), 0, s, A);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_genrhomt(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, const int *nlorb, const int *lorbl,
+                        const double *apwfr, const double *lofr, const cplx *gnt, const cplx *dm, int ws, double *fbuf,
+                        double *rhomt, hipStream_t s) {
+  HmlArgs A = hml_args(d, lmv, nrmtmax, nrmt, nlorb, lorbl, nullptr, nullptr, apwfr, lofr, nullptr, fbuf, gnt);
+  A.dm = dm;
+  A.ws = ws;
+  A.rho = rhomt;
+  const int L2 = (int)lround(sqrt((double)lmv));
+  if (L2 - 1 > kMaxL2 || d.natm > 65535 || lmv > 65535 || ws < d.naa + A.maxnlo) return EXC_ERR_ARG;
+  const size_t nt = (size_t)A.nf * A.nf * L2;
+  hipLaunchKernelGGL(rho_factor_kernel, dim3((unsigned)((nt + kHT - 1) / kHT), d.natm), dim3(kHT), 0, s, A);
+  EXC_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(rho_radial_kernel, dim3((nrmtmax + kHT - 1) / kHT, lmv, d.natm), dim3(kHT), 0, s, A);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

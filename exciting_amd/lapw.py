@@ -85,6 +85,8 @@ SIGNATURES = {
     "exc_ctx_hmlint": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "exc_ctx_set_radial_fr": [_P, _I, _P, _P, _P],
     "exc_ctx_gencfun": [_P, _P, _P, _P],
+    "exc_genrhomt_dev": [_P, _P, _P],
+    "exc_genrhomt": [_P, _P, _P],
     "exc_ctx_genveffig": [_P, _P, _P],
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
@@ -281,6 +283,14 @@ class Context:
         return self._call("exc_eigen_batched_dev", self.h, nk, C.cast(ints, _P), C.cast(ints, _P),
                           C.cast(ptrs[0], _P), C.cast(ptrs[1], _P), nev, C.cast(ptrs[2], _P), C.cast(ptrs[3], _P),
                           C.cast(ints, _P))
+
+    def genrhomt(self, dm, lmaxvr, nrmtmax):
+        """exc_genrhomt from a host density matrix (wfsize, wfsize, natm) Fortran-ordered:
+        rhomt((lmaxvr+1)^2, nrmtmax, natm) with the radial data of the last hmlint call."""
+        dm = np.asfortranarray(dm, dtype=np.complex128)
+        out = np.zeros(((lmaxvr + 1) ** 2, nrmtmax, self.cr.natm), order="F")
+        self._call("exc_genrhomt", self.h, _ptr(dm), _ptr(out))
+        return out
 
     def gencfun(self):
         """exc_ctx_gencfun with the crystal's bvec: (cfunig(ngrtot), cfunir(ngrtot)) as host arrays."""

@@ -247,6 +247,17 @@ int exc_gendmatmt(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, int n
 int exc_gendmatmt_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int nst, const int *ldv,
                               const double *const *evecfv, const double *const *evecsv,
                               const double *const *occ, const double *wkpt, double *dm);
+/* The muffin-tin density from the density matrix, the counterpart of genrhomt(basis, basis,
+ * rhomt) (genrhomt.f90:13, called after the k loop when useDensityMatrix,
+ * scf_cycle.f90:365-366; the spinless part):
+ *   rhomt(lm2, r, a) = sum u_f(r) u_f'(r) Re(dm(f, f', a) conj(G(lm_f, lm2, lm_f')))
+ * over APW-APW, twice APW-lo and lo-lo pairs (genrhomt.f90:34-150), G = int conj(Y) R Y
+ * (gauntyry.f90), on the radial functions, meshes and lmaxvr of the last exc_ctx_hmlint call.
+ * dm(wfsize, wfsize, natmtot) as exc_gendmatmt accumulates it; rhomt((lmaxvr+1)^2, nrmtmax,
+ * natmtot) real (zero beyond each species' nrmt).  _dev: device arrays on the context's
+ * stream; without: host arrays. */
+int exc_genrhomt_dev(exc_ctx *ctx, const double *dm, double *rhomt);
+int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
 
 /* Per-SCF producers of the interstitial tables, on the GPU.  The FFT grid is the G box of
  * exc_ctx_set_gvec (ngrid(i) = intgv(2,i) - intgv(1,i) + 1, every grid point one of the ngrtot

@@ -93,3 +93,64 @@ def test_match_and_setup_from_radial_data():
     eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
     print(f"from radial data: apwalm {ea:.2e}, H {eh:.2e}, O {eo:.2e}", flush=True)
     assert ea <= 1e-12 and eh <= 1e-12 and eo <= 1e-12
+
+
+@pytest.mark.parametrize("which", ["small-d", "config1"])
+def test_genrhomt_against_oracle(which):
+    """The muffin-tin density from a density matrix (exc_genrhomt) on the radial data of the
+    preceding hmlint call, against the oracle's genrhomt (pinned by the potential-energy
+    identity in test_oracle_hmlint.py)."""
+    from exciting_amd import lapw, synth
+    from oracle import hmlint_ref as M
+    from oracle import lapw_ref as R
+    w = small("d") if which == "small-d" else workload(1, 1)
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(150, 120) if which == "small-d" else (260, 230),
+                           lmaxvr=4 if which == "small-d" else 6)
+    ctx = lapw.Context(cr)
+    ctx.hmlint(rd, out=False)
+    rng = np.random.default_rng(41)
+    nmat, nst = kp.ngp + cr.nlotot, 20
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    dm = R.gendmatmt(cr, R.match(cr, kp), kp.ngp, C, rng.uniform(0, 2, nst), 0.5)
+    rho = ctx.genrhomt(dm, rd.lmaxvr, rd.spr.shape[0])
+    ref = M.genrhomt(cr, rd, dm)
+    e = np.abs(rho - ref).max() / np.abs(ref).max()
+    print(f"{which}: genrhomt max rel diff {e:.2e}", flush=True)
+    assert e <= 1e-12
+
+
+def test_density_chain_on_device():
+    """One SCF pass's device chain at workload 1 (4 k-points): grouped setup, grouped
+    eigensolve, density matrix of all k-points, muffin-tin density; the density against the
+    oracle's genrhomt of the density matrix the device produced."""
+    import torch
+    from exciting_amd import lapw, synth
+    from oracle import hmlint_ref as M
+    w = workload(1, 4)
+    cr = w.crystal
+    rd = synth.make_radial(cr, nr=(260, 230), lmaxvr=6)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(cr, stream=s)
+    nev = 100
+    with torch.cuda.stream(s):
+        ctx.hmlint(rd, out=False)
+        b = lapw.DeviceBatch(cr, w.kpoints)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE)
+        evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+        vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in b.nmat]
+        ctx.eigen_batched_dev(b.nmat, b.H, b.O, nev, evs, vecs)
+        occs = [torch.full((nev,), 2.0, dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+        ws = ctx.dmat_wfsize()
+        dm = torch.zeros((cr.natm, ws, ws), dtype=torch.complex128, device="cuda")
+        ctx.gendmatmt_batched_dev(b, nev, vecs, occs, [1.0 / b.nk] * b.nk, dm)
+        lmv, nrm = (rd.lmaxvr + 1) ** 2, rd.spr.shape[0]
+        rho = torch.zeros((cr.natm, nrm, lmv), dtype=torch.float64, device="cuda")
+        ctx._call("exc_genrhomt_dev", ctx.h, dm.data_ptr(), rho.data_ptr())
+        torch.cuda.synchronize()
+    dmh = np.asfortranarray(np.transpose(dm.cpu().numpy(), (2, 1, 0)))
+    got = np.transpose(rho.cpu().numpy(), (2, 1, 0))
+    ref = M.genrhomt(cr, rd, dmh)
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"device density chain: max rel diff {e:.2e}", flush=True)
+    assert e <= 1e-12
