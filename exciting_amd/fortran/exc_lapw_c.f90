@@ -150,6 +150,13 @@ module exc_lapw_c
       complex(c_double_complex), intent(inout) :: dm(*)
     end function
 
+    integer(c_int) function exc_genrhomt(ctx, dm, rhomt) bind(c, name="exc_genrhomt")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: dm(*)
+      real(c_double), intent(inout) :: rhomt(*)
+    end function
+
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -9,6 +9,7 @@
 !                                                                 hmlint.f90:14 + olprad.f90:13
 !   set_radial_gpu(nrmt, spr, apwfr)                             match.f90:139-148 (zd by polynom)
 !   gencfun_gpu(bvec, cfunig, cfunir)                             gencfun.f90:13 (cfdamp = 0)
+!   genrhomt_gpu(dm, rhomt)                                       genrhomt.f90:13
 !   genveffig_gpu(veffir, veffig)                                 genveffig.f90:13
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
@@ -29,7 +30,7 @@ module exc_setup_gpu
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
             solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu, gencfun_gpu, &
-            genveffig_gpu
+            genveffig_gpu, genrhomt_gpu
 
 contains
 
@@ -228,5 +229,14 @@ contains
     complex(8), intent(out) :: veffig(*)
     call check(exc_ctx_genveffig(ctx, veffir, veffig), "genveffig")
   end subroutine genveffig_gpu
+
+  ! genrhomt.f90:13 (genrhomt(basis, basis, rhomt), scf_cycle.f90:365-366): the muffin-tin
+  ! density rhomt(lmmaxvr, nrmtmax, natmtot) from the density matrix dm (mt_dm%alpha%ff, as
+  ! gendmatmt_gpu accumulates it) and the radial functions of the last hmlint_gpu call.
+  subroutine genrhomt_gpu(dm, rhomt)
+    complex(8), intent(in) :: dm(:, :, :)
+    real(8), intent(out) :: rhomt(*)
+    call check(exc_genrhomt(ctx, dm, rhomt), "genrhomt")
+  end subroutine genrhomt_gpu
 
 end module exc_setup_gpu

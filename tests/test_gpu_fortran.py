@@ -11,8 +11,8 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, write_eigen, write_input,
-                        write_radial, write_scf)
+from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, read_rhomt, write_eigen,
+                        write_input, write_radial, write_scf)
 
 pytestmark = pytest.mark.gpu
 
@@ -149,3 +149,34 @@ def test_fortran_gencfun_genveffig_then_setup(tmp_path):
     eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
     print(f"Fortran gencfun_gpu + genveffig_gpu + setup: H rel {eh:.2e}, O rel {eo:.2e}")
     assert eh <= 1e-12 and eo <= 1e-12
+
+
+def test_fortran_gendmatmt_genrhomt(tmp_path):
+    """gendmatmt_gpu then genrhomt_gpu (Fortran) on radial data: the density against the
+    oracle's genrhomt of the density matrix the driver wrote."""
+    from exciting_amd import synth
+    from oracle import hmlint_ref as M
+    from oracle import lapw_ref as R
+    w = small("b")
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(140, 110), lmaxvr=4)
+    nmat = kp.ngp + cr.nlotot
+    rng = np.random.default_rng(12)
+    nst = 8
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    e, occ = np.sort(rng.standard_normal(nst)), rng.uniform(0, 2, nst)
+    fin, fout, feig, frad = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "eig.bin", tmp_path / "rad.bin"
+    write_input(fin, cr, kp)
+    write_eigen(feig, C, e, occ, 0.3)
+    write_radial(frad, cr, rd)
+    r = subprocess.run([EXE, str(fin), str(fout), str(feig), "0", str(frad)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    ngkmax, ws = kp.igpig.shape[0], R.dmat_wfsize(cr)
+    dm = read_dmat(fout, cr, ngkmax, ws)
+    rho = read_rhomt(fout, cr, ngkmax, ws, (rd.lmaxvr + 1) ** 2, rd.spr.shape[0])
+    ref = M.genrhomt(cr, rd, dm)
+    rel = np.abs(rho - ref).max() / np.abs(ref).max()
+    print("Fortran genrhomt_gpu rel diff", rel)
+    assert rel <= 1e-12
