@@ -196,7 +196,8 @@ def run_eigen(args, ctx, batch, w, olpflag, nev, log):
 def run_hmlint(args, w, log):
     """The per-SCF producers on a context of their own: the muffin-tin integrals
     (exc_ctx_hmlint: hmlint + olprad) on seeded radial data of the workload's atoms (meshes of
-    500 / 460 points, lmaxvr 6), and Theta(G), Theta(r), V(G) (exc_ctx_gencfun + genveffig) on
+    500 / 460 points, lmaxvr 6), the density from a seeded Hermitian density matrix on the same
+    radial data (exc_genrhomt_dev), and Theta(G), Theta(r), V(G) (exc_ctx_gencfun + genveffig) on
     the workload's G box with a seeded V(r); with the CPU baseline on, the numpy oracle's time
     for the same calls beside them."""
     import torch
@@ -226,6 +227,27 @@ def run_hmlint(args, w, log):
     log(f"  muffin-tin integrals (hmlint + olprad, {cr.natm} atoms): {res['ms_per_call']:.2f} ms per call, "
         f"kernels {res['kernel_ms']:.3f} ms" + (f", numpy oracle {res['cpu_oracle_ms']:.0f} ms"
                                                  if res["cpu_oracle_ms"] else ""))
+    # the muffin-tin density from a density matrix on the same radial data (genrhomt)
+    ws = ctx.dmat_wfsize()
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn((cr.natm, ws, ws), dtype=torch.complex128, device="cuda", generator=g)
+    dm = (x + x.transpose(1, 2).conj()).contiguous()
+    lmv, nrm = (rd.lmaxvr + 1) ** 2, rd.spr.shape[0]
+    rho = torch.zeros((cr.natm, nrm, lmv), dtype=torch.float64, device="cuda")
+    ctx._call("exc_genrhomt_dev", ctx.h, dm.data_ptr(), rho.data_ptr())
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx._call("exc_genrhomt_dev", ctx.h, dm.data_ptr(), rho.data_ptr())
+    torch.cuda.synchronize()
+    dt3 = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    ms3, _ = ctx.read_timing()["hmlint"]
+    res["density_from_dm"] = {"ms_per_call": 1000.0 * dt3, "kernel_ms": ms3 / reps, "wfsize": ws}
+    log(f"  muffin-tin density from the density matrix (genrhomt): {1000.0 * dt3:.3f} ms per call, "
+        f"kernels {ms3 / reps:.3f} ms")
     # Theta(G), Theta(r) and V(G) of the G box (gencfun + genveffig, hipFFT)
     rng = np.random.default_rng(20261202)
     v = rng.standard_normal(cr.ngrtot)
