@@ -100,6 +100,10 @@ struct exc_ctx {
   // work array, Theta(r) on the grid, a staging array for V(r)
   hipfftHandle fft;
   bool have_fft, have_cfunir;
+  hipfftHandle fftb;          // batched backward FFT of genrhoir, fftb_n transforms per call
+  int fftb_n;
+  cplx *zb;
+  size_t zb_bytes;
   cplx *fz;
   double *cfunir, *fin;
   hipEvent_t eig_fork;
@@ -320,6 +324,8 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->dmw) hipFree(c->dmw);
   if (c->gnt) hipFree(c->gnt);
   if (c->have_fft) hipfftDestroy(c->fft);
+  if (c->fftb_n > 0) hipfftDestroy(c->fftb);
+  if (c->zb) hipFree(c->zb);
   if (c->hml) hipFree(c->hml);
   if (c->eig_fork) hipEventDestroy(c->eig_fork);
   if (c->side) hipStreamDestroy(c->side);
@@ -1304,6 +1310,99 @@ int exc_ctx_genveffig(exc_ctx *c, const double *veffir, double *veffig) {
     if ((rc = exc_launch_scf_gather(c->d, nv, 1.0 / (double)n, c->fz, c->d.veffig, c->stream))) return rc;
   }
   if (veffig) EXC_CHECK(hipMemcpyAsync(veffig, c->d.veffig, 16 * (size_t)nv, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return EXC_OK;
+}
+
+namespace {
+constexpr int kRhoirChunk = 64;   // states per batched FFT of genrhoir
+
+// rhoir += the interstitial density of the nst states of each of nk k-points (device arrays)
+int rhoir_run(exc_ctx *c, int nk, const int *ngp, const int *const *igpig, int nst, const int *ldv,
+              const double *const *evec, const double *const *occ, const double *wkpt, double epsocc, double *rhoir) {
+  int rc;
+  if ((rc = scf_prepare(c))) return rc;
+  const exc_dev &d = c->d;
+  const int N = c->dims.ngrtot, B = std::min(nst, kRhoirChunk);
+  if ((rc = grow((void **)&c->zb, &c->zb_bytes, sizeof(cplx) * (size_t)B * N))) return rc;
+  if (c->fftb_n != B) {
+    if (c->fftb_n > 0) hipfftDestroy(c->fftb);
+    c->fftb_n = 0;
+    int dims[3] = {d.box_n[2], d.box_n[1], d.box_n[0]};   // Fortran order: first index fastest
+    if (hipfftPlanMany(&c->fftb, 3, dims, nullptr, 1, N, nullptr, 1, N, HIPFFT_Z2Z, B) != HIPFFT_SUCCESS)
+      return EXC_ERR_HIP;
+    c->fftb_n = B;
+  }
+  if (hipfftSetStream(c->fftb, c->stream) != HIPFFT_SUCCESS) return EXC_ERR_HIP;
+  StageTimer t(c, EXC_STAGE_SCF);
+  for (int k = 0; k < nk; ++k)
+    for (int j0 = 0; j0 < nst; j0 += B) {
+      const int nb = std::min(B, nst - j0);
+      EXC_CHECK(hipMemsetAsync(c->zb, 0, sizeof(cplx) * (size_t)B * N, c->stream));
+      if ((rc = exc_launch_rhoir_scatter(d, ngp[k], igpig[k], (const cplx *)evec[k], ldv[k], j0, nb, N, c->zb,
+                                         c->stream)))
+        return rc;
+      if (hipfftExecZ2Z(c->fftb, (hipfftDoubleComplex *)c->zb, (hipfftDoubleComplex *)c->zb, HIPFFT_BACKWARD) !=
+          HIPFFT_SUCCESS)
+        return EXC_ERR_HIP;
+      if ((rc = exc_launch_rhoir_accum(N, nb, c->zb, occ[k], j0, wkpt[k], epsocc, d.omega, rhoir, c->stream)))
+        return rc;
+    }
+  return EXC_OK;
+}
+}  // namespace
+
+int exc_genrhoir_dev(exc_ctx *c, int ngp, const int *igpig, int nst, int ldv, const double *evecfv, const double *occ,
+                     double wkpt, double epsocc, double *rhoir) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (ngp < 0 || nst < 0 || ldv < std::max(ngp, 1)) return EXC_ERR_ARG;
+  if (nst == 0 || ngp == 0) return EXC_OK;
+  if (!igpig || !evecfv || !occ || !rhoir) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  return rhoir_run(c, 1, &ngp, &igpig, nst, &ldv, &evecfv, &occ, &wkpt, epsocc, rhoir);
+}
+
+int exc_genrhoir_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int nst, const int *ldv,
+                             const double *const *evecfv, const double *const *occ, const double *wkpt, double epsocc,
+                             double *rhoir) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nk < 0 || nst < 0 || (nk > 0 && (!ks || !ldv || !evecfv || !occ || !wkpt || !rhoir))) return EXC_ERR_ARG;
+  std::vector<int> ngp(nk);
+  std::vector<const int *> ig(nk);
+  for (int k = 0; k < nk; ++k) {
+    if (ks[k].ngp < 0 || ldv[k] < std::max(ks[k].ngp, 1)) return EXC_ERR_ARG;
+    if (nst > 0 && (!ks[k].igpig || !evecfv[k] || !occ[k])) return EXC_ERR_ARG;
+    ngp[k] = ks[k].ngp;
+    ig[k] = ks[k].igpig;
+  }
+  if (nk == 0 || nst == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  return rhoir_run(c, nk, ngp.data(), ig.data(), nst, ldv, evecfv, occ, wkpt, epsocc, rhoir);
+}
+
+int exc_genrhoir(exc_ctx *c, int ngp, const int *igpig, int nst, int ldv, const double *evecfv, const double *occ,
+                 double wkpt, double epsocc, double *rhoir) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (ngp < 0 || nst < 0 || ldv < std::max(ngp, 1)) return EXC_ERR_ARG;
+  if (nst == 0 || ngp == 0) return EXC_OK;
+  if (!igpig || !evecfv || !occ || !rhoir) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t N = c->dims.ngrtot, b_i = 4 * (size_t)ngp, b_v = 16 * (size_t)ldv * nst, b_o = 8 * (size_t)nst;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_v = al(b_i), o_o = o_v + al(b_v), o_r = o_o + al(b_o);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_r + 8 * N))) return rc;
+  char *base = (char *)c->stage;
+  EXC_CHECK(hipMemcpyAsync(base, igpig, b_i, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_v, evecfv, b_v, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_o, occ, b_o, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_r, rhoir, 8 * N, hipMemcpyHostToDevice, c->stream));
+  const int *pi = (const int *)base;
+  const double *pv = (const double *)(base + o_v), *po = (const double *)(base + o_o);
+  if ((rc = rhoir_run(c, 1, &ngp, &pi, nst, &ldv, &pv, &po, &wkpt, epsocc, (double *)(base + o_r)))) return rc;
+  EXC_CHECK(hipMemcpyAsync(rhoir, base + o_r, 8 * N, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   return EXC_OK;
 }

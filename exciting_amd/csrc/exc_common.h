@@ -164,5 +164,9 @@ int exc_launch_cfunig(const exc_dev &d, const double *bvec, int ngrtot, cplx *cf
 int exc_launch_scf_real(int n, const cplx *z, double *out, hipStream_t s);
 int exc_launch_scf_product(int n, const double *a, const double *b, cplx *z, hipStream_t s);
 int exc_launch_scf_gather(const exc_dev &d, int ngvec, double scale, const cplx *z, cplx *out, hipStream_t s);
+int exc_launch_rhoir_scatter(const exc_dev &d, int ngp, const int *igpig, const cplx *evec, int ldv, int j0, int nb,
+                             int N, cplx *z, hipStream_t s);
+int exc_launch_rhoir_accum(int N, int nb, const cplx *z, const double *occ, int j0, double wk, double eps,
+                           double omega, double *rho, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);

@@ -1,9 +1,11 @@
 // Per-SCF producers of the interstitial tables the setup reads: Theta(G) and Theta(r)
-// (gencfun.f90:52-96, cfdamp = 0) and V(G) (genveffig.f90:23-33).  The FFT grid is the G box
-// of the context (ngrid(i) = box extent, every grid point one G); position of G on the grid
-// j1 + n1 (j2 + n2 j3) with negative components wrapped to n + i (gengvec.f90:100-117).  The
-// transforms are hipFFT's double-complex 3-D ones with zfftifc.f90's convention: forward
-// (sign -1) scaled by 1/N, backward (+1) unscaled.
+// (gencfun.f90:52-96, cfdamp = 0) and V(G) (genveffig.f90:23-33); and the interstitial
+// density of the occupied states (genrhoir.f90:28-89): a chunk of states is scattered to the
+// grid, transformed backward in one batched FFT and |psi|^2 accumulated per grid point.
+// The FFT grid is the G box of the context (ngrid(i) = box extent, every grid point one G);
+// position of G on the grid j1 + n1 (j2 + n2 j3) with negative components wrapped to n + i
+// (gengvec.f90:100-117).  The transforms are hipFFT's double-complex 3-D ones with
+// zfftifc.f90's convention: forward (sign -1) scaled by 1/N, backward (+1) unscaled.
 #include <math.h>
 
 #include "exc_common.h"
@@ -71,6 +73,31 @@ __global__ __launch_bounds__(kST) void scf_gather_kernel(exc_dev d, int ngvec, d
   out[g] = {v.re * scale, v.im * scale};
 }
 
+// z[b N + grid(G_igk)] = evec(igk, j0 + b): the plane-wave coefficients of state j0 + b
+__global__ __launch_bounds__(kST) void rhoir_scatter_kernel(exc_dev d, int ngp, const int *igpig, const cplx *evec,
+                                                            int ldv, int j0, int N, cplx *z) {
+  const int igk = blockIdx.x * kST + threadIdx.x, b = blockIdx.y;
+  if (igk >= ngp) return;
+  const int g = igpig[igk] - 1;
+  if (g < 0 || g >= N) return;
+  z[(size_t)b * N + grid_index(d, d.ivg, g)] = evec[igk + (size_t)ldv * (j0 + b)];
+}
+
+// rho(r) += sum_b t_b |z_b(r)|^2, t_b = wk occ(j0 + b) / Omega, states with |wk occ| <= eps skipped
+__global__ __launch_bounds__(kST) void rhoir_accum_kernel(int N, int nb, const cplx *z, const double *occ, int j0,
+                                                          double wk, double eps, double omega, double *rho) {
+  const int r = blockIdx.x * kST + threadIdx.x;
+  if (r >= N) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double t1 = wk * occ[j0 + b];
+    if (fabs(t1) <= eps) continue;
+    const cplx v = z[(size_t)b * N + r];
+    s += (t1 / omega) * (v.re * v.re + v.im * v.im);
+  }
+  rho[r] += s;
+}
+
 inline unsigned blocks(int n) { return (unsigned)((n + kST - 1) / kST); }
 
 }  // namespace
@@ -97,6 +124,22 @@ int exc_launch_scf_product(int n, const double *a, const double *b, cplx *z, hip
 
 int exc_launch_scf_gather(const exc_dev &d, int ngvec, double scale, const cplx *z, cplx *out, hipStream_t s) {
   hipLaunchKernelGGL(scf_gather_kernel, dim3(blocks(ngvec)), dim3(kST), 0, s, d, ngvec, scale, z, out);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_rhoir_scatter(const exc_dev &d, int ngp, const int *igpig, const cplx *evec, int ldv, int j0, int nb,
+                             int N, cplx *z, hipStream_t s) {
+  if (ngp <= 0 || nb <= 0) return EXC_OK;
+  if (nb > 65535) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(rhoir_scatter_kernel, dim3(blocks(ngp), nb), dim3(kST), 0, s, d, ngp, igpig, evec, ldv, j0, N, z);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_rhoir_accum(int N, int nb, const cplx *z, const double *occ, int j0, double wk, double eps,
+                           double omega, double *rho, hipStream_t s) {
+  hipLaunchKernelGGL(rhoir_accum_kernel, dim3(blocks(N)), dim3(kST), 0, s, N, nb, z, occ, j0, wk, eps, omega, rho);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

@@ -87,6 +87,9 @@ SIGNATURES = {
     "exc_ctx_gencfun": [_P, _P, _P, _P],
     "exc_genrhomt_dev": [_P, _P, _P],
     "exc_genrhomt": [_P, _P, _P],
+    "exc_genrhoir_dev": [_P, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
+    "exc_genrhoir_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, C.c_double, _P],
+    "exc_genrhoir": [_P, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
     "exc_ctx_genveffig": [_P, _P, _P],
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
@@ -291,6 +294,27 @@ class Context:
         out = np.zeros(((lmaxvr + 1) ** 2, nrmtmax, self.cr.natm), order="F")
         self._call("exc_genrhomt", self.h, _ptr(dm), _ptr(out))
         return out
+
+    def genrhoir(self, kp, evec, occ, wkpt, epsocc=1e-8, rhoir=None):
+        """exc_genrhoir from host arrays: evec (ldv, nst) Fortran complex128; rhoir (ngrtot,)
+        accumulated (new when None)."""
+        evec = np.asfortranarray(evec, dtype=np.complex128)
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        ig = np.ascontiguousarray(kp.igpig[:kp.ngp], dtype=np.int32)
+        out = np.zeros(self.cr.ngrtot) if rhoir is None else rhoir
+        self._call("exc_genrhoir", self.h, kp.ngp, _ptr(ig), evec.shape[1], evec.shape[0], _ptr(evec), _ptr(occ),
+                   float(wkpt), float(epsocc), _ptr(out))
+        return out
+
+    def genrhoir_batched_dev(self, batch, nst, evecs, occs, wkpts, rhoir, epsocc=1e-8):
+        """exc_genrhoir_batched_dev over a DeviceBatch: evecs[k] (nst, nmat_k) tensors."""
+        nk = batch.nk
+        ldv = (C.c_int * nk)(*batch.nmat)
+        ev = (C.c_void_p * nk)(*[_ptr(t) for t in evecs])
+        oc = (C.c_void_p * nk)(*[_ptr(t) for t in occs])
+        wk = (C.c_double * nk)(*[float(x) for x in wkpts])
+        return self._call("exc_genrhoir_batched_dev", self.h, nk, C.cast(batch.desc, _P), nst, C.cast(ldv, _P),
+                          C.cast(ev, _P), C.cast(oc, _P), C.cast(wk, _P), float(epsocc), _ptr(rhoir))
 
     def gencfun(self):
         """exc_ctx_gencfun with the crystal's bvec: (cfunig(ngrtot), cfunir(ngrtot)) as host arrays."""

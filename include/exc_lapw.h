@@ -258,6 +258,22 @@ int exc_gendmatmt_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, in
  * stream; without: host arrays. */
 int exc_genrhomt_dev(exc_ctx *ctx, const double *dm, double *rhomt);
 int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
+/* The interstitial density of one k-point, the counterpart of genrhoir(ik, evecfv, evecsv)
+ * (genrhoir.f90:13, called beside gendmatmt at scf_cycle.f90:352; spinless, first-variational
+ * states): for every state j with |wkpt occ(j)| > epsocc, psi_j(r) = the backward FFT (unscaled)
+ * of evecfv(1:ngp, j) placed at the grid positions of G(igpig(1:ngp)), and
+ *   rhoir(r) += wkpt occ(j) / Omega |psi_j(r)|^2
+ * on the FFT grid of exc_ctx_gencfun (the G box), rhoir(ngrtot) accumulated.  _dev: device
+ * arrays (evecfv(ldv, nst), occ(nst), igpig(ngp) 1-based), on the context's stream; the
+ * batched form takes ngp and igpig from ks[k] and one evecfv, occ, wkpt per k-point, with the
+ * states of every k-point transformed in batches of 64; without _dev: host arrays. */
+int exc_genrhoir_dev(exc_ctx *ctx, int ngp, const int *igpig, int nst, int ldv, const double *evecfv,
+                     const double *occ, double wkpt, double epsocc, double *rhoir);
+int exc_genrhoir_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, int nst, const int *ldv,
+                             const double *const *evecfv, const double *const *occ, const double *wkpt,
+                             double epsocc, double *rhoir);
+int exc_genrhoir(exc_ctx *ctx, int ngp, const int *igpig, int nst, int ldv, const double *evecfv,
+                 const double *occ, double wkpt, double epsocc, double *rhoir);
 
 /* Per-SCF producers of the interstitial tables, on the GPU.  The FFT grid is the G box of
  * exc_ctx_set_gvec (ngrid(i) = intgv(2,i) - intgv(1,i) + 1, every grid point one of the ngrtot

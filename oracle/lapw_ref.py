@@ -15,6 +15,8 @@ Each function restates one reference routine, in numpy, on the arrays of
 * ``setup_vjp``         — dE/dtau and dE/dV(G) of (H,O) for given cotangents
 * ``gendmatmt``         — gendmatmt.f90:59-172 (muffin-tin density matrix of one k,
                           spinless first-variational part)
+* ``genrhoir``          — genrhoir.f90:28-89 (interstitial density of one k, spinless
+                          first-variational states)
 * ``igfft``, ``gencfun``, ``genveffig`` — gengvec.f90:100-117 (FFT index of each G),
                           gencfun.f90:52-96 (Theta(G) and Theta(r), cfdamp = 0),
                           genveffig.f90:23-33 (V(G) = forward FFT of V(r) Theta(r)), with
@@ -449,3 +451,21 @@ def genveffig(cr, veffir, cfunir, fft=_fft3):
     """genveffig.f90:23-33: V(G) = forward transform of V(r) Theta(r) at the first ngvec G."""
     z = fft(np.asarray(veffir) * np.asarray(cfunir), fft_grid(cr), -1)
     return z[igfft(cr)[:cr.ngvec]]
+
+
+def genrhoir(cr, kp, evec, occ, wkpt, epsocc=1e-8, fft=_fft3, rhoir=None):
+    """genrhoir.f90:28-89 (spinless, tevecsv false): for each state j with |wkpt occ_j| >
+    epsocc, psi_j(r) = backward transform of evecfv(1:ngp, j) placed at igfft(igkig(igk)), and
+    rhoir(r) += wkpt occ_j / Omega |psi_j(r)|^2 on the FFT grid (ngrtot,)."""
+    n = fft_grid(cr)
+    ix = igfft(cr)[kp.igpig[:kp.ngp] - 1]
+    out = np.zeros(cr.ngrtot) if rhoir is None else rhoir
+    for j in range(evec.shape[1]):
+        t1 = wkpt * occ[j]
+        if abs(t1) <= epsocc:
+            continue
+        z = np.zeros(cr.ngrtot, dtype=np.complex128)
+        z[ix] = evec[:kp.ngp, j]
+        z = fft(z, n, 1)
+        out += (t1 / cr.omega) * (z.real ** 2 + z.imag ** 2)
+    return out

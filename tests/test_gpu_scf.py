@@ -61,3 +61,59 @@ def test_setup_reads_the_new_tables():
     eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
     print(f"setup after gencfun/genveffig: H {eh:.2e}, O {eo:.2e}", flush=True)
     assert eh <= 1e-12 and eo <= 1e-12
+
+
+@pytest.mark.parametrize("which", ["small-d", "config1"])
+def test_genrhoir_against_oracle(which):
+    """The interstitial density of one k-point (exc_genrhoir, batched FFTs of 64 states) against
+    the oracle's genrhoir (pinned by Parseval and direct sums in test_oracle_scf.py); 70 states
+    at workload 1 cross the chunk boundary; a zero and a below-epsocc occupation are skipped."""
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = small("d") if which == "small-d" else workload(1, 1)
+    cr, kp = w.crystal, w.kpoints[0]
+    ctx = lapw.Context(cr)
+    ctx.gencfun()
+    rng = np.random.default_rng(13)
+    nmat, nst = kp.ngp + cr.nlotot, (9 if which == "small-d" else 70)
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    occ = rng.uniform(0.0, 2.0, nst)
+    occ[1], occ[2] = 0.0, 1e-12
+    rho0 = rng.uniform(0, 1, cr.ngrtot)
+    got = ctx.genrhoir(kp, C, occ, 0.35, rhoir=rho0.copy())
+    ref = R.genrhoir(cr, kp, C, occ, 0.35, rhoir=rho0.copy())
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"{which}: genrhoir max rel diff {e:.2e}", flush=True)
+    assert e <= 1e-12
+
+
+def test_genrhoir_batched_after_eigensolve():
+    import torch
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 4)
+    cr = w.crystal
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(cr, stream=s)
+    nev = 100
+    rng = np.random.default_rng(17)
+    occs_h = [rng.uniform(0.0, 2.0, nev) for _ in w.kpoints]
+    wk = list(rng.uniform(0.1, 0.4, len(w.kpoints)))
+    with torch.cuda.stream(s):
+        ctx.gencfun()
+        b = lapw.DeviceBatch(cr, w.kpoints)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE)
+        evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+        vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in b.nmat]
+        ctx.eigen_batched_dev(b.nmat, b.H, b.O, nev, evs, vecs)
+        occs = [torch.from_numpy(o).cuda() for o in occs_h]
+        rho = torch.zeros(cr.ngrtot, dtype=torch.float64, device="cuda")
+        ctx.genrhoir_batched_dev(b, nev, vecs, occs, wk, rho)
+        torch.cuda.synchronize()
+    ref = None
+    for k, kp in enumerate(w.kpoints):
+        ref = R.genrhoir(cr, kp, np.asfortranarray(vecs[k].cpu().numpy().T), occs_h[k], wk[k], rhoir=ref)
+    got = rho.cpu().numpy()
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"genrhoir batched over {b.nk} k-points: max rel diff {e:.2e}", flush=True)
+    assert e <= 1e-12

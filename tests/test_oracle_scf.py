@@ -8,6 +8,8 @@ genveffig, dft3).
 * gencfun: Theta(G) equals the configuration's (pinned against a real-space integral in
   test_oracle_setup.py); the grid mean of Theta(r) is Theta(0); forward(backward) = identity.
 * genveffig: a plane wave cos(G0.r) with Theta(r) = 1 gives 1/2 at +-G0 and nothing else.
+* genrhoir: Parseval, (Omega/N) sum_r rho(r) = w sum_j occ_j sum_G |C_Gj|^2; psi_j at grid
+  points by direct summation over G; states with |w occ| <= epsocc skipped.
 """
 import math
 
@@ -68,3 +70,26 @@ def test_genveffig_plane_wave():
     expect = np.zeros(cr.ngvec, dtype=np.complex128)
     expect[g0] = expect[gm] = 0.5
     assert np.abs(vg - expect).max() <= 1e-13
+
+
+def test_genrhoir_parseval_and_direct_sum():
+    w = small("d")
+    cr, kp = w.crystal, w.kpoints[0]
+    rng = np.random.default_rng(8)
+    nmat, nst = kp.ngp + cr.nlotot, 5
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    occ = np.array([2.0, 1.5, 0.0, 1e-12, 0.7])
+    wk = 0.4
+    rho = R.genrhoir(cr, kp, C, occ, wk)
+    n = R.fft_grid(cr)
+    N = n[0] * n[1] * n[2]
+    kept = np.abs(wk * occ) > 1e-8
+    par = wk * sum(occ[j] * np.sum(np.abs(C[:kp.ngp, j]) ** 2) for j in range(nst) if kept[j])
+    assert abs(cr.omega / N * rho.sum() - par) <= 1e-12 * par
+    # direct sum at a few grid points: psi(r) = sum_G C_G e^{i G.r}, r = sum_i (j_i / n_i) a_i
+    G = cr.ivg[:, kp.igpig[:kp.ngp] - 1]
+    for (j1, j2, j3) in [(0, 0, 0), (3, 5, 7), (n[0] - 1, 1, n[2] - 2)]:
+        frac = np.array([j1 / n[0], j2 / n[1], j3 / n[2]])
+        ph = np.exp(2j * np.pi * (frac @ G))
+        val = sum((wk * occ[j] / cr.omega) * abs(np.dot(C[:kp.ngp, j], ph)) ** 2 for j in range(nst) if kept[j])
+        assert abs(rho[j1 + n[0] * (j2 + n[1] * j3)] - val) <= 1e-12 * max(val, 1e-300) + 1e-14
