@@ -306,8 +306,23 @@ def run_dmat(args, ctx, batch, w, nst, vecs, log):
     tf = flops / (ms * 1e-3) / 1e12
     log(f"  density matrix (gendmatmt, {nst} states): {ms / batch.nk:.3f} ms/k, {tf:.1f} TFLOP/s "
         f"({tf * 1e12 / PEAK_FP64:.2f} of the FP64 MFMA peak)")
+    # the interstitial density of the same states (genrhoir: batched backward FFTs on the G box)
+    rho = torch.zeros(cr.ngrtot, dtype=torch.float64, device="cuda")
+    ctx.genrhoir_batched_dev(batch, nst, vecs, occs, wk, rho)
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    for _ in range(args.steps):
+        ctx.genrhoir_batched_dev(batch, nst, vecs, occs, wk, rho)
+    torch.cuda.synchronize()
+    ctx.set_timing(False)
+    ms_ir, cnt_ir = ctx.read_timing()["scf"]
+    ms_ir /= max(cnt_ir, 1)
+    grid = [int(cr.intgv[i, 1] - cr.intgv[i, 0] + 1) for i in range(3)]
+    log(f"  interstitial density (genrhoir, {nst} states, grid {grid}): {ms_ir / batch.nk:.3f} ms/k")
     return {"ms_per_k": ms / batch.nk, "tflops": tf, "frac_fp64_peak": tf * 1e12 / PEAK_FP64, "flops_per_launch": flops,
-            "k_points": batch.nk, "states": nst, "wfsize": ws}
+            "k_points": batch.nk, "states": nst, "wfsize": ws,
+            "interstitial_density_ms_per_k": ms_ir / batch.nk, "fft_grid": grid}
 
 
 def main():

@@ -2,8 +2,8 @@
 ! runs match and hamiltonandoverlapsetup through the GPU driver as seceqn.f90:57 /
 ! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
 ! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot), then
-! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot) (and, with radial
-! data, genrhomt_gpu's rhomt(lmmaxvr, nrmtmax, natmtot)); with a fourth
+! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot) and genrhoir_gpu's
+! rhoir(ngrtot) (and, with radial data, genrhomt_gpu's rhomt(lmmaxvr, nrmtmax, natmtot)); with a fourth
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
 ! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).  With a fifth file of radial
 ! data it first runs set_radial_gpu (match's boundary values from apwfr) and hmlint_gpu (hmlint +
@@ -33,7 +33,7 @@ program exc_driver_main
   real(8) :: bvec(3, 3)
   real(8), allocatable :: veffir(:), cfunir(:)
   integer(c_int), allocatable :: nrmt(:)
-  real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:), rhomt(:)
+  real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:), rhomt(:), rhoir(:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -122,6 +122,10 @@ program exc_driver_main
     dm = (0.d0, 0.d0)
     call gendmatmt_gpu(ngp, apwalm, nmat, nst, evec, occ, wkpt, dm)
     write(u) dm
+    allocate(rhoir(dims%ngrtot))
+    rhoir = 0.d0
+    call genrhoir_gpu(ngp, igpig, nmat, nst, evec, occ, wkpt, 1.d-8, rhoir)
+    write(u) rhoir
     if (trim(frad) /= "-") then
       ! with radial data: the muffin-tin density of these states
       allocate(rhomt((lmaxvr + 1)**2 * nrmtmax * dims%natmtot))

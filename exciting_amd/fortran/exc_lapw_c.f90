@@ -157,6 +157,18 @@ module exc_lapw_c
       real(c_double), intent(inout) :: rhomt(*)
     end function
 
+    integer(c_int) function exc_genrhoir(ctx, ngp, igpig, nst, ldv, evecfv, occ, wkpt, epsocc, rhoir) &
+        bind(c, name="exc_genrhoir")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ngp, nst, ldv
+      integer(c_int), intent(in) :: igpig(*)
+      complex(c_double_complex), intent(in) :: evecfv(*)
+      real(c_double), intent(in) :: occ(*)
+      real(c_double), value :: wkpt, epsocc
+      real(c_double), intent(inout) :: rhoir(*)
+    end function
+
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -10,6 +10,8 @@
 !   set_radial_gpu(nrmt, spr, apwfr)                             match.f90:139-148 (zd by polynom)
 !   gencfun_gpu(bvec, cfunig, cfunir)                             gencfun.f90:13 (cfdamp = 0)
 !   genrhomt_gpu(dm, rhomt)                                       genrhomt.f90:13
+!   genrhoir_gpu(ngp, igpig, nmatmax, nstfv, evecfv, occ, wkpt, epsocc, rhoir)
+!                                                                 genrhoir.f90:13
 !   genveffig_gpu(veffir, veffig)                                 genveffig.f90:13
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
@@ -30,7 +32,7 @@ module exc_setup_gpu
 
   public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
             solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu, gencfun_gpu, &
-            genveffig_gpu, genrhomt_gpu
+            genveffig_gpu, genrhomt_gpu, genrhoir_gpu
 
 contains
 
@@ -238,5 +240,18 @@ contains
     real(8), intent(out) :: rhomt(*)
     call check(exc_genrhomt(ctx, dm, rhomt), "genrhomt")
   end subroutine genrhomt_gpu
+
+  ! genrhoir.f90:13 (genrhoir(ik, evecfv, evecsv), spinless first-variational states) with the
+  ! module state it reads passed in (igkig(:, 1, ik), occsv(:, ik), wkpt(ik), epsocc): adds
+  ! wkpt occ / Omega |psi|^2 of each state with |wkpt occ| > epsocc to rhoir(ngrtot).
+  subroutine genrhoir_gpu(ngp, igpig, nmatmax, nstfv, evecfv, occ, wkpt, epsocc, rhoir)
+    integer, intent(in) :: ngp, nmatmax, nstfv
+    integer(c_int), intent(in) :: igpig(*)
+    complex(8), intent(in) :: evecfv(nmatmax, nstfv)
+    real(8), intent(in) :: occ(nstfv), wkpt, epsocc
+    real(8), intent(inout) :: rhoir(*)
+    call check(exc_genrhoir(ctx, int(ngp, c_int), igpig, int(nstfv, c_int), int(nmatmax, c_int), evecfv, occ, wkpt, &
+                            epsocc, rhoir), "genrhoir")
+  end subroutine genrhoir_gpu
 
 end module exc_setup_gpu

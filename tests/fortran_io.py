@@ -88,11 +88,21 @@ def write_scf(path, cr, veffir):
         _w(f, veffir, np.float64)
 
 
-def read_rhomt(path, cr, ngkmax, ws, lmv, nrmtmax):
-    """rhomt(lmv, nrmtmax, natm) appended after dm when the driver had radial data."""
+def read_rhoir(path, cr, ngkmax, ws):
+    """rhoir(ngrtot) appended after dm (genrhoir_gpu)."""
     with open(path, "rb") as f:
         nmat = int(np.fromfile(f, np.int32, 1)[0])
         skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat + ws * ws * cr.natm
         np.fromfile(f, np.complex128, skip)
         np.fromfile(f, np.float64, 3 * cr.natm)
+        return np.fromfile(f, np.float64, cr.ngrtot)
+
+
+def read_rhomt(path, cr, ngkmax, ws, lmv, nrmtmax):
+    """rhomt(lmv, nrmtmax, natm) appended after dm and rhoir when the driver had radial data."""
+    with open(path, "rb") as f:
+        nmat = int(np.fromfile(f, np.int32, 1)[0])
+        skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat + ws * ws * cr.natm
+        np.fromfile(f, np.complex128, skip)
+        np.fromfile(f, np.float64, 3 * cr.natm + cr.ngrtot)
         return np.fromfile(f, np.float64, lmv * nrmtmax * cr.natm).reshape((lmv, nrmtmax, cr.natm), order="F")

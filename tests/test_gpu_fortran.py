@@ -11,8 +11,8 @@ import numpy as np
 import pytest
 
 from conftest import small, workload
-from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, read_rhomt, write_eigen,
-                        write_input, write_radial, write_scf)
+from fortran_io import (read_dmat, read_eigen_output, read_forces, read_output, read_rhoir, read_rhomt,
+                        write_eigen, write_input, write_radial, write_scf)
 
 pytestmark = pytest.mark.gpu
 
@@ -153,7 +153,8 @@ def test_fortran_gencfun_genveffig_then_setup(tmp_path):
 
 def test_fortran_gendmatmt_genrhomt(tmp_path):
     """gendmatmt_gpu then genrhomt_gpu (Fortran) on radial data: the density against the
-    oracle's genrhomt of the density matrix the driver wrote."""
+    oracle's genrhomt of the density matrix the driver wrote; genrhoir_gpu's interstitial
+    density against the oracle's genrhoir."""
     from exciting_amd import synth
     from oracle import hmlint_ref as M
     from oracle import lapw_ref as R
@@ -180,3 +181,8 @@ def test_fortran_gendmatmt_genrhomt(tmp_path):
     rel = np.abs(rho - ref).max() / np.abs(ref).max()
     print("Fortran genrhomt_gpu rel diff", rel)
     assert rel <= 1e-12
+    rir = read_rhoir(fout, cr, ngkmax, ws)
+    rir_ref = R.genrhoir(cr, kp, C, occ, 0.3)
+    rel2 = np.abs(rir - rir_ref).max() / np.abs(rir_ref).max()
+    print("Fortran genrhoir_gpu rel diff", rel2)
+    assert rel2 <= 1e-12
