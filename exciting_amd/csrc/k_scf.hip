@@ -83,17 +83,25 @@ __global__ __launch_bounds__(kST) void rhoir_scatter_kernel(exc_dev d, int ngp, 
   z[(size_t)b * N + grid_index(d, d.ivg, g)] = evec[igk + (size_t)ldv * (j0 + b)];
 }
 
-// rho(r) += sum_b t_b |z_b(r)|^2, t_b = wk occ(j0 + b) / Omega, states with |wk occ| <= eps skipped
+// rho(r) += sum_b t_b |z_b(r)|^2, t_b = wk occ(j0 + b) / Omega, states with |wk occ| <= eps
+// skipped; the weights of the chunk (at most kST) are formed once per workgroup
 __global__ __launch_bounds__(kST) void rhoir_accum_kernel(int N, int nb, const cplx *z, const double *occ, int j0,
                                                           double wk, double eps, double omega, double *rho) {
+  __shared__ double t2[kST];
+  __shared__ int keep[kST];
+  if (threadIdx.x < nb) {
+    const double t1 = wk * occ[j0 + threadIdx.x];
+    keep[threadIdx.x] = fabs(t1) > eps;
+    t2[threadIdx.x] = t1 / omega;
+  }
+  __syncthreads();
   const int r = blockIdx.x * kST + threadIdx.x;
   if (r >= N) return;
   double s = 0.0;
   for (int b = 0; b < nb; ++b) {
-    const double t1 = wk * occ[j0 + b];
-    if (fabs(t1) <= eps) continue;
+    if (!keep[b]) continue;
     const cplx v = z[(size_t)b * N + r];
-    s += (t1 / omega) * (v.re * v.re + v.im * v.im);
+    s += t2[b] * (v.re * v.re + v.im * v.im);
   }
   rho[r] += s;
 }
@@ -139,6 +147,8 @@ int exc_launch_rhoir_scatter(const exc_dev &d, int ngp, const int *igpig, const 
 
 int exc_launch_rhoir_accum(int N, int nb, const cplx *z, const double *occ, int j0, double wk, double eps,
                            double omega, double *rho, hipStream_t s) {
+  if (nb <= 0) return EXC_OK;
+  if (nb > kST) return EXC_ERR_ARG;
   hipLaunchKernelGGL(rhoir_accum_kernel, dim3(blocks(N)), dim3(kST), 0, s, N, nb, z, occ, j0, wk, eps, omega, rho);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
