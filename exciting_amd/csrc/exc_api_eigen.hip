@@ -1,0 +1,351 @@
+// C-ABI entry points of the generalized eigensolve after the setup (exc_eigen, exc_eigen_dev,
+// exc_eigen_batched_dev): the counterpart of solvewithlapack (modfvsystem.f90:585-720).
+#include "exc_ctx.h"
+
+using namespace exc_impl;
+
+extern "C" {
+
+namespace {
+
+bool eig_args_ok(int nmat, int ldh, int nev, int ldv) {
+  return nmat >= 0 && ldh >= (nmat > 1 ? nmat : 1) && nev >= 0 && nev <= nmat && ldv >= (nmat > 1 ? nmat : 1);
+}
+
+// Enqueue one generalized eigensolve and the copies of its first nev eigenpairs on the
+// worker's stream; d_ints receives (nev found, info).  Solver variants (all: Cholesky of O,
+// reduction to standard form, then)
+//  0 zhegvdx: tridiagonal divide and conquer, lowest nev     1 zhegvx: bisection + inverse iteration
+//  2 zhegvd: divide and conquer, all                         3 zhegv: implicit QL/QR, all
+//  4 (default) the reduction C = U^-H H U^-1 with level-3 operations (potrf, trtri, two trmm:
+//    rocSOLVER's zhegst runs column by column), zheevd on C, and X = U^-1 Y for the nev
+//    wanted columns only
+// (rocSOLVER's Jacobi solver, zhegvj, misses the 1e-10 Ha eigenvalue bar on these spectra)
+// d_ints: nev found, info of the solver / of potrf, info of trtri, info of heevd
+// Variant 4 runs in two phases: phase 1 factors O (the caller checks its info on the host
+// before phase 2, so a non-positive-definite O never feeds the later steps); the other
+// variants do all their work in phase 2.
+int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, double *h, double *o, int nev,
+                double *eval, double *evec, int ldv, int *d_ints) {
+  if (!W.blas) {
+    if (rocblas_create_handle(&W.blas) != rocblas_status_success) {
+      W.blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+  // W has n slots and Z n columns: rocSOLVER's divide-and-conquer path uses the columns past
+  // nev as workspace, so the caller's nev columns are filled from an internal n x n buffer
+  int rc;
+  if ((rc = grow((void **)&W.w, &W.wb, sizeof(double) * nmat))) return rc;
+  if ((rc = grow((void **)&W.z, &W.zb, sizeof(cplx) * (size_t)nmat * nmat))) return rc;
+  if ((rc = grow((void **)&W.e, &W.eb, sizeof(double) * (nmat + 1)))) return rc;
+  const auto itype = rocblas_eform_ax;
+  const auto ev = rocblas_evect_original;
+  const auto up = rocblas_fill_upper;
+  rocblas_double_complex *A = (rocblas_double_complex *)h, *B = (rocblas_double_complex *)o;
+  rocblas_double_complex *Z = (rocblas_double_complex *)W.z;
+  rocblas_status st;
+  bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z (v 0, 1, 4)
+  if (phase == 1) {
+    // ZHEGVX('U') reads the upper triangles and the real parts of the diagonals only;
+    // rocSOLVER's solvers and the products of variant 4 read more: lower := conj(upper)
+    if ((rc = exc_launch_eig_hermfill((cplx *)h, nmat, ldh, W.s))) return rc;
+    if ((rc = exc_launch_eig_hermfill((cplx *)o, nmat, ldh, W.s))) return rc;
+    if (v < 4) return EXC_OK;
+    return rocsolver_zpotrf(W.blas, up, nmat, B, ldh, d_ints + 1) == rocblas_status_success ? EXC_OK
+                                                                                           : EXC_ERR_HIP;   // O = U^H U
+  }
+  switch (v) {
+    case 0:
+      st = rocsolver_zhegvdx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev,
+                             d_ints, W.w, Z, nmat, d_ints + 1);
+      break;
+    case 1:
+      st = rocsolver_zhegvx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev, 0.0,
+                            d_ints, W.w, Z, nmat, (rocblas_int *)W.e, d_ints + 1);
+      break;
+    case 2:
+      st = rocsolver_zhegvd(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+      in_a = true;
+      break;
+    case 3:
+      st = rocsolver_zhegv(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+      in_a = true;
+      break;
+    default: {
+      const rocblas_double_complex one = {1.0, 0.0};
+      st = rocsolver_ztrtri(W.blas, up, rocblas_diagonal_non_unit, nmat, B, ldh, d_ints + 2);   // B = U^-1
+      if (st == rocblas_status_success)                                                 // Z = H U^-1
+        st = rocblas_ztrmm(W.blas, rocblas_side_right, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nmat, &one, B, ldh, A, ldh, Z, nmat);
+      if (st == rocblas_status_success)                                                 // H = U^-H Z
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_conjugate_transpose,
+                           rocblas_diagonal_non_unit, nmat, nmat, &one, B, ldh, Z, nmat, A, ldh);
+      if (st == rocblas_status_success)                                                 // C Y = Y diag(w)
+        st = rocsolver_zheevd(W.blas, ev, up, nmat, A, ldh, W.w, W.e, d_ints + 3);
+      if (st == rocblas_status_success)                                                 // Z = U^-1 Y(:, 1:nev)
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nev, &one, B, ldh, A, ldh, Z, nmat);
+      break;
+    }
+  }
+  if (st != rocblas_status_success) return EXC_ERR_HIP;
+  EXC_CHECK(hipMemcpyAsync(eval, W.w, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
+  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, in_a ? (const void *)A : (const void *)Z,
+                             16 * (size_t)(in_a ? ldh : nmat), 16 * (size_t)nmat, nev, hipMemcpyDeviceToDevice, W.s));
+  return EXC_OK;
+}
+
+// status of a solve from its four ints (the full-spectrum solvers, v >= 2, report no count);
+// a failed Cholesky of O is nmat + i as in LAPACK's zhegv
+int eig_status(const int *hi, int nmat, int nev, int v) {
+  if (v >= 4) {
+    if (hi[1] > 0) return nmat + hi[1];
+    if (hi[2] != 0) return EXC_ERR_HIP;
+    return hi[3];
+  }
+  if (hi[1] != 0) return hi[1];
+  return (v >= 2 || hi[0] == nev) ? EXC_OK : EXC_ERR_HIP;
+}
+
+// Variant 5 of the multi-k entry: every k-point's H, O padded to the largest nmat n and
+// reduced as one strided batch -- Cholesky, U^-1, C = U^-H H U^-1 and the tridiagonal
+// reduction of C, one launch per step for all k-points (the per-k-point reduction is
+// thousands of small launches, one column at a time).  The pad stays out of the leading
+// nmat_k block of every factor (k_eigen.hip).  Then per k-point, on the worker streams: the
+// tridiagonal divide and conquer of the leading nmat_k part (real arithmetic), its first nev
+// eigenvectors through the Householder reflectors (zunmtr) and U^-1 (trmm) into the caller's
+// evec.  H and O are read, not modified.
+int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o, int nev,
+                double *const *eval, double *const *evec, const int *ldv) {
+  std::vector<eig_kdesc> kd;
+  int n = 0;
+  for (int k = 0; k < nk; ++k) {
+    if (nmat[k] == 0) continue;
+    kd.push_back({(const cplx *)h[k], (const cplx *)o[k], eval[k], (cplx *)evec[k], nmat[k], ldh[k], ldv[k], k});
+    n = std::max(n, nmat[k]);
+  }
+  const int nb = (int)kd.size();
+  if (nb == 0) return EXC_OK;
+  if (nb > 65535) return EXC_ERR_ARG;
+  const int nw = std::min(nb, (int)exc_ctx::kEigStreams);
+  const size_t mat = (size_t)n * n, a256 = 255;
+  auto al = [&](size_t b) { return (b + a256) & ~a256; };
+  const size_t o_po = al(16 * mat * nb), o_pz = o_po + al(16 * mat * nb), o_d = o_pz + al(16 * mat * nb);
+  const size_t o_e = o_d + al(8 * (size_t)n * nb);
+  const size_t o_tau = o_e + al(8 * (size_t)n * nb), o_i = o_tau + al(16 * (size_t)n * nb);
+  const size_t o_kd = o_i + al(12 * (size_t)nb);
+  const size_t total = o_kd + sizeof(eig_kdesc) * nb;
+  int rc;
+  if ((rc = grow(&c->eg, &c->eg_bytes, total))) return rc;
+  char *base = (char *)c->eg;
+  cplx *ph = (cplx *)base, *po = (cplx *)(base + o_po), *pz = (cplx *)(base + o_pz), *tau = (cplx *)(base + o_tau);
+  double *d = (double *)(base + o_d), *e = (double *)(base + o_e);
+  int *info = (int *)(base + o_i);
+  eig_kdesc *dkd = (eig_kdesc *)(base + o_kd);
+  for (int i = 0; i <= nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[i];
+    if (i == 0) W.s = c->stream;
+    if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+      W.blas = nullptr;
+      return EXC_ERR_HIP;
+    }
+    if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+    if (i > 0) {   // per worker: real eigenvectors of T (n x n), their first nev columns complex
+      if ((rc = grow((void **)&W.z, &W.zb, sizeof(double) * mat))) return rc;
+      if ((rc = grow((void **)&W.zc, &W.zcb, sizeof(cplx) * (size_t)n * nev))) return rc;
+    }
+  }
+  auto fail = [](const char *what, int a, int b) {
+    fprintf(stderr, "exc_eigen_batched_dev (grouped): %s (%d, %d)\n", what, a, b);
+    return EXC_ERR_HIP;
+  };
+  const auto up = rocblas_fill_upper;
+  const auto nu = rocblas_diagonal_non_unit;
+  const rocblas_double_complex one = {1.0, 0.0};
+  rocblas_handle hb = c->ew[0].blas;
+  auto *A = (rocblas_double_complex *)ph, *B = (rocblas_double_complex *)po, *Z = (rocblas_double_complex *)pz;
+  const rocblas_stride sm = (rocblas_stride)mat;
+  std::vector<int> hi(3 * (size_t)nb, 0);
+  StageTimer t(c, EXC_STAGE_EIGEN);
+  EXC_CHECK(hipMemcpyAsync(dkd, kd.data(), sizeof(eig_kdesc) * nb, hipMemcpyHostToDevice, c->stream));
+  if ((rc = exc_launch_eig_pad(dkd, nb, n, ph, po, 1.0, c->stream))) return rc;
+  // O = U^H U for every k-point; checked before anything reads the factors
+  rocblas_status st = rocsolver_zpotrf_strided_batched(hb, up, n, B, n, sm, info, nb);
+  if (st != rocblas_status_success) return fail("zpotrf_strided_batched status", (int)st, n);
+  EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b)
+    if (hi[b] > 0) return hi[b] <= kd[b].n ? kd[b].n + hi[b] : fail("pad not positive definite", b, hi[b]);
+  st = rocsolver_ztrtri_strided_batched(hb, up, nu, n, B, n, sm, info + nb, nb);   // U^-1
+  if (st == rocblas_status_success)   // Z = H U^-1
+    st = rocblas_ztrmm_strided_batched(hb, rocblas_side_right, up, rocblas_operation_none, nu, n, n, &one, B, n, sm,
+                                       A, n, sm, Z, n, sm, nb);
+  if (st == rocblas_status_success)   // H = U^-H Z
+    st = rocblas_ztrmm_strided_batched(hb, rocblas_side_left, up, rocblas_operation_conjugate_transpose, nu, n, n,
+                                       &one, B, n, sm, Z, n, sm, A, n, sm, nb);
+  if (st == rocblas_status_success)   // C = Q T Q^H
+    st = rocsolver_zhetrd_strided_batched(hb, up, n, A, n, sm, d, n, e, n, (rocblas_double_complex *)tau, n, nb);
+  if (st != rocblas_status_success) return fail("trtri / trmm / zhetrd strided_batched status", (int)st, n);
+  EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
+  for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
+  // per k-point on the workers; every worker is joined below, also after an error
+  int err = EXC_OK;
+  for (int b = 0; b < nb && !err; ++b) {
+    exc_ctx::EigWorker &W = c->ew[1 + b % nw];
+    const int m = kd[b].n;
+    double *db = d + (size_t)b * n, *eb = e + (size_t)b * n;
+    st = rocsolver_dstedc(W.blas, rocblas_evect_tridiagonal, m, db, eb, (double *)W.z, m, info + 2 * nb + b);
+    if (st != rocblas_status_success) {
+      err = fail("dstedc status", (int)st, b);
+      break;
+    }
+    if ((err = exc_launch_eig_r2c((const double *)W.z, m, nev, m, W.zc, m, W.s))) break;
+    st = rocsolver_zunmtr(W.blas, rocblas_side_left, up, rocblas_operation_none, m, nev, A + (size_t)b * mat, n,
+                          (rocblas_double_complex *)(tau + (size_t)b * n), (rocblas_double_complex *)W.zc, m);
+    if (st == rocblas_status_success)
+      st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, nu, m, nev, &one, B + (size_t)b * mat,
+                         n, (rocblas_double_complex *)W.zc, m, (rocblas_double_complex *)kd[b].evec, kd[b].ldv);
+    if (st != rocblas_status_success) {
+      err = fail("zunmtr / trmm status", (int)st, b);
+      break;
+    }
+    if (hipMemcpyAsync(kd[b].eval, db, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s) != hipSuccess)
+      err = EXC_ERR_HIP;
+  }
+  for (int i = 1; i <= nw; ++i) {
+    EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
+    EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
+  }
+  if (err) return err;
+  EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < nb; ++b) {
+    if (hi[nb + b] != 0) return fail("ztrtri info", b, hi[nb + b]);   // of a positive definite factor
+    if (hi[2 * nb + b] != 0) return hi[2 * nb + b];                 // eigenvalues that did not converge
+  }
+  return EXC_OK;
+}
+
+}  // namespace
+
+int exc_eigen_dev(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec,
+                  int ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (!eig_args_ok(nmat, ldh, nev, ldv)) return EXC_ERR_ARG;
+  if (nmat == 0 || nev == 0) return EXC_OK;
+  if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 4 * sizeof(int)))) return rc;
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 4;
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
+  int hi[4] = {0, 0, 0, 0};
+  {
+    StageTimer t(c, EXC_STAGE_EIGEN);
+    if ((rc = eig_enqueue(W, v, 1, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
+    if (v >= 4) {
+      EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
+      EXC_CHECK(hipStreamSynchronize(c->stream));
+      if (hi[1] > 0) return nmat + hi[1];
+    }
+    if ((rc = eig_enqueue(W, v, 2, nmat, ldh, h, o, nev, eval, evec, ldv, c->eig_int))) return rc;
+  }
+  EXC_CHECK(hipMemcpyAsync(hi, c->eig_int, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return eig_status(hi, nmat, nev, v);
+}
+
+int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o,
+                          int nev, double *const *eval, double *const *evec, const int *ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nk < 0 || (nk > 0 && (!nmat || !ldh || !h || !o || !eval || !evec || !ldv))) return EXC_ERR_ARG;
+  for (int k = 0; k < nk; ++k) {
+    if (!eig_args_ok(nmat[k], ldh[k], nev, ldv[k])) return EXC_ERR_ARG;
+    if (nmat[k] > 0 && nev > 0 && (!h[k] || !o[k] || !eval[k] || !evec[k])) return EXC_ERR_ARG;
+  }
+  if (nk == 0 || nev == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if ((rc = grow((void **)&c->eig_int, &c->eig_int_bytes, 4 * sizeof(int) * nk))) return rc;
+  if (!c->eig_fork) EXC_CHECK(hipEventCreateWithFlags(&c->eig_fork, hipEventDisableTiming));
+  const int nw = nk < exc_ctx::kEigStreams ? nk : exc_ctx::kEigStreams;
+  for (int i = 1; i <= nw; ++i) {
+    exc_ctx::EigWorker &W = c->ew[i];
+    if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
+    if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
+  }
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 5;
+  if (v == 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv);
+  {
+    StageTimer t(c, EXC_STAGE_EIGEN);
+    EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
+    for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
+    for (int phase = 1; phase <= 2; ++phase) {
+      if (phase == 2 && v >= 4) {
+        // every factorization done and positive definite before the rest runs
+        for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamSynchronize(c->ew[i].s));
+        std::vector<int> hp(4 * (size_t)nk, 0);
+        EXC_CHECK(hipMemcpy(hp.data(), c->eig_int, sizeof(int) * 4 * nk, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nk; ++k)
+          if (nmat[k] > 0 && hp[4 * (size_t)k + 1] > 0) return nmat[k] + hp[4 * (size_t)k + 1];
+      }
+      for (int k = 0; k < nk && !rc; ++k) {
+        if (nmat[k] == 0) continue;
+        exc_ctx::EigWorker &W = c->ew[1 + k % nw];
+        rc = eig_enqueue(W, v, phase, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k], c->eig_int + 4 * k);
+      }
+      if (rc) break;
+    }
+    for (int i = 1; i <= nw; ++i) {   // joined also after an error
+      EXC_CHECK(hipEventRecord(c->ew[i].done, c->ew[i].s));
+      EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
+    }
+    if (rc) return rc;
+  }
+  std::vector<int> hi(4 * (size_t)nk, 0);
+  EXC_CHECK(hipMemcpyAsync(hi.data(), c->eig_int, sizeof(int) * 4 * nk, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nk; ++k)
+    if (nmat[k] > 0 && (rc = eig_status(&hi[4 * (size_t)k], nmat[k], nev, v))) return rc;
+  return EXC_OK;
+}
+
+
+int exc_eigen(exc_ctx *c, int nmat, int ldh, double *h, double *o, int nev, double *eval, double *evec, int ldv) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (nmat < 0 || ldh < (nmat > 1 ? nmat : 1) || nev < 0 || nev > nmat || ldv < (nmat > 1 ? nmat : 1)) return EXC_ERR_ARG;
+  if (nmat == 0 || nev == 0) return EXC_OK;
+  if (!h || !o || !eval || !evec) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t b_m = 16 * (size_t)ldh * nmat, b_v = 16 * (size_t)ldv * nev, b_e = 8 * (size_t)nev;
+  const size_t o_o = (b_m + 255) & ~(size_t)255, o_v = o_o + ((b_m + 255) & ~(size_t)255);
+  const size_t o_e = o_v + ((b_v + 255) & ~(size_t)255);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_e + b_e))) return rc;
+  if ((rc = grow_host(&c->hstage, &c->hstage_bytes, std::max(2 * b_m, b_v)))) return rc;
+  char *base = (char *)c->stage, *hs = (char *)c->hstage;
+  const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
+  copy_cols(hs, pitch, (const char *)h, pitch, width, nmat);
+  copy_cols(hs + b_m, pitch, (const char *)o, pitch, width, nmat);
+  EXC_CHECK(hipMemcpyAsync(base, hs, b_m, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_o, hs + b_m, b_m, hipMemcpyHostToDevice, c->stream));
+  rc = exc_eigen_dev(c, nmat, ldh, (double *)base, (double *)(base + o_o), nev, (double *)(base + o_e),
+                     (double *)(base + o_v), ldv);
+  if (rc) return rc;
+  // H and O come back overwritten as ZHEGVX leaves them (O: its Cholesky factor)
+  EXC_CHECK(hipMemcpyAsync(hs, base, b_m, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(hs + b_m, base + o_o, b_m, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  copy_cols((char *)h, pitch, hs, pitch, width, nmat);
+  copy_cols((char *)o, pitch, hs + b_m, pitch, width, nmat);
+  EXC_CHECK(hipMemcpyAsync(hs, base + o_v, b_v, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(eval, base + o_e, b_e, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  copy_cols((char *)evec, 16 * (size_t)ldv, hs, 16 * (size_t)ldv, width, nev);
+  return EXC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+// Private to the library: the context behind the C-ABI handle and the helpers the entry-point
+// files share (device allocation and uploads owned by the context, grow-only buffers, the
+// stage timers).  Not part of the C-ABI.
+#ifndef EXC_CTX_H
+#define EXC_CTX_H
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include <hipfft/hipfft.h>
+#include <rocsolver/rocsolver.h>
+
+#include "exc_common.h"
+
+struct exc_ctx {
+  exc_dims dims;
+  int device;
+  hipStream_t stream;
+  exc_dev d;
+  bool have_atoms, have_lo, have_radial, have_mt, have_gvec;
+  int radial_info;         // > 0: first singular matching matrix (1-based), as zgesv's info
+  int mt_variant;          // tile variant of the MT stage (-1: chosen per call)
+  int y_variant;
+  // grow-only workspaces
+  cplx *yt;
+  size_t yt_bytes;
+  // staging for the host-pointer entry points: device buffers and a pinned host buffer
+  void *stage;
+  size_t stage_bytes;
+  void *hstage;
+  size_t hstage_bytes;
+  std::vector<void *> owned;
+  // per-stage HIP-event timers (exc_ctx_set_timing)
+  bool timing;
+  struct mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<mark> marks;
+  std::vector<hipEvent_t> pool;
+  // grouped launches
+  int bmt_variant;          // -1: built-in choice
+  int bsplit_variant;
+  unsigned short *planes;   // split-BF16 slice planes of all k-points (A and Yt)
+  size_t planes_bytes;
+  int *expo;                // per-(row, atom) exponents
+  size_t expo_bytes;
+  cplx *vjp_work;           // adjoint workspace
+  size_t vjp_bytes;
+  cplx *byt;                // Y-stage workspace of all k-points
+  size_t byt_bytes;
+  kdesc *d_ks;
+  size_t ks_cap;
+  mt_item *d_items;
+  size_t items_cap;
+  double *rfbuf;            // radial factors of all k-points (EXC_BATCH_OLP_LEGENDRE)
+  size_t rf_bytes;
+  mt_item *d_oitems;        // overlap tiles of the addition-theorem kernel
+  size_t oitems_cap;
+  int batch_noitems;
+  std::vector<char> batch_sig;   // the batch the descriptors / graph were built for
+  int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
+  hipGraph_t graph;
+  hipGraphExec_t graph_exec;
+  // generalized eigensolve: worker 0 runs on the context's stream (exc_eigen_dev), workers
+  // 1..kEigStreams on streams of their own (exc_eigen_batched_dev)
+  struct EigWorker {
+    hipStream_t s;
+    rocblas_handle blas;
+    double *w;              // all n eigenvalue slots rocSOLVER writes
+    size_t wb;
+    cplx *z;                // n x n eigenvector workspace (rocSOLVER may use all n columns)
+    size_t zb;
+    cplx *zc;               // n x nev eigenvectors of the grouped path's back-transform
+    size_t zcb;
+    double *e;              // n-length workspace (tridiagonal off-diagonal, ifail)
+    size_t eb;
+    hipEvent_t done;
+  };
+  static constexpr int kEigStreams = 4;
+  EigWorker ew[kEigStreams + 1];
+  int eig_variant;          // -1: built-in choice
+  int *eig_int;             // per solve: 4 ints (device)
+  size_t eig_int_bytes;
+  void *eg;                 // grouped solve: padded batch, tridiagonal factors, infos, descriptors
+  size_t eg_bytes;
+  void *dmw;                // density matrix: A^T C, packed and weighted wavefunction blocks
+  size_t dmw_bytes;
+  // muffin-tin integrals (exc_ctx_hmlint): host copies of nlorb / lorbl, the Gaunt table of
+  // gnt_lmaxvr (device), radial inputs + integrals (device)
+  std::vector<int> h_nlorb, h_lorbl;
+  cplx *gnt;
+  size_t gnt_bytes;
+  int gnt_lmaxvr;
+  void *hml;
+  size_t hml_bytes;
+  int hml_lmaxvr, hml_nrmtmax;   // of the last exc_ctx_hmlint (-1: none)
+  // Theta(r) / V(G) (exc_ctx_gencfun, exc_ctx_genveffig): 3-D FFT plan of the G box, the grid
+  // work array, Theta(r) on the grid, a staging array for V(r)
+  hipfftHandle fft;
+  bool have_fft, have_cfunir;
+  hipfftHandle fftb;          // batched backward FFT of genrhoir, fftb_n transforms per call
+  int fftb_n;
+  cplx *zb;
+  size_t zb_bytes;
+  cplx *fz;
+  double *cfunir, *fin;
+  hipEvent_t eig_fork;
+  // second branch of a captured batch (overlap + lo kernels beside the Y + MT chain)
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
+  int ncu;
+};
+
+namespace exc_impl {
+
+template <typename T>
+inline int dev_alloc(exc_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return EXC_OK;
+  void *q = nullptr;
+  EXC_CHECK(hipMalloc(&q, n * sizeof(T)));
+  c->owned.push_back(q);
+  *p = (T *)q;
+  return EXC_OK;
+}
+
+template <typename T>
+inline int dev_upload(exc_ctx *c, T **p, const void *src, size_t n) {
+  if (*p == nullptr) {
+    int rc = dev_alloc(c, p, n);
+    if (rc) return rc;
+  }
+  if (n) EXC_CHECK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return EXC_OK;
+}
+
+// pinned host staging (grow-only); every host entry synchronises before it returns, so no
+// copy into the old buffer is in flight when it is replaced
+inline int grow_host(void **p, size_t *have, size_t need) {
+  if (*have >= need) return EXC_OK;
+  if (*p) hipHostFree(*p);
+  *p = nullptr;
+  *have = 0;
+  EXC_CHECK(hipHostMalloc(p, need, hipHostMallocDefault));
+  *have = need;
+  return EXC_OK;
+}
+
+// ncols columns of rowbytes bytes between column-major host arrays (pitches dld, sld);
+// large transfers are split over host threads (the pageable side is the caller's array)
+inline void copy_cols(char *dst, size_t dld, const char *src, size_t sld, size_t rowbytes, size_t ncols) {
+  auto work = [=](size_t j0, size_t j1) {
+    for (size_t j = j0; j < j1; ++j) memcpy(dst + j * dld, src + j * sld, rowbytes);
+  };
+  const unsigned hc = std::thread::hardware_concurrency();
+  const unsigned nt = (rowbytes * ncols >= ((size_t)8 << 20)) ? std::min(8u, std::max(1u, hc)) : 1u;
+  if (nt <= 1 || ncols < nt) {
+    work(0, ncols);
+    return;
+  }
+  const size_t per = (ncols + nt - 1) / nt;
+  std::vector<std::thread> th;
+  size_t done = per;                      // columns [0, per) run on this thread
+  try {
+    for (unsigned t = 1; t < nt; ++t) {
+      const size_t a = t * per, b = std::min(ncols, a + per);
+      if (a >= b) break;
+      th.emplace_back(work, a, b);
+      done = b;
+    }
+  } catch (...) {                         // no threads available: copy the rest here
+  }
+  work(0, std::min(ncols, per));
+  if (done < ncols) work(done, ncols);
+  for (auto &x : th) x.join();
+}
+
+inline int grow(void **p, size_t *have, size_t need) {
+  if (need <= *have) return EXC_OK;
+  if (*p) EXC_CHECK(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  EXC_CHECK(hipMalloc(p, need));
+  *have = need;
+  return EXC_OK;
+}
+
+inline int use_device(exc_ctx *c) {
+  int cur = -1;
+  EXC_CHECK(hipGetDevice(&cur));
+  if (cur != c->device) EXC_CHECK(hipSetDevice(c->device));
+  return EXC_OK;
+}
+
+inline bool ready(const exc_ctx *c) { return c && c->have_atoms && c->have_lo && c->have_radial && c->have_mt && c->have_gvec; }
+
+inline hipEvent_t take_event(exc_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one stage's launches with events when timing is on.
+struct StageTimer {
+  exc_ctx *c;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(exc_ctx *ctx, int st) : c(ctx), stage(st) {
+    if (st >= 0 && c->timing && (a = take_event(c))) hipEventRecord(a, c->stream);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) {
+      c->pool.push_back(a);
+      return;
+    }
+    hipEventRecord(b, c->stream);
+    c->marks.push_back({stage, a, b});
+  }
+};
+
+}  // namespace exc_impl
+
+#endif
