@@ -114,11 +114,17 @@ int eig_status(const int *hi, int nmat, int nev, int v) {
 // reduction of C, one launch per step for all k-points (the per-k-point reduction is
 // thousands of small launches, one column at a time).  The pad stays out of the leading
 // nmat_k block of every factor (k_eigen.hip).  Then per k-point, on the worker streams: the
-// tridiagonal divide and conquer of the leading nmat_k part (real arithmetic), its first nev
-// eigenvectors through the Householder reflectors (zunmtr) and U^-1 (trmm) into the caller's
-// evec.  H and O are read, not modified.
+// tridiagonal divide and conquer of the leading nmat_k part (real arithmetic), then its first
+// nev eigenvectors through the Householder reflectors and U^-1 into the caller's evec:
+//  variant 5: per k-point, zunmtr and trmm on the worker streams;
+//  variant 6: for the whole batch, the reflectors applied kBT at a time as block reflectors
+//    I - V T V^H (T from zlarft's recurrence, k_eigen.hip) with strided-batched GEMMs, then
+//    one strided-batched trmm; the eigenvectors of every k-point sit in an n x nev slice
+//    whose rows past nmat_k stay zero, so the pad's rows never mix in.
+// H and O are read, not modified.
+constexpr int kBT = 64;   // reflectors per block of the batched back-transform
 int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o, int nev,
-                double *const *eval, double *const *evec, const int *ldv) {
+                double *const *eval, double *const *evec, const int *ldv, bool blocked) {
   std::vector<eig_kdesc> kd;
   int n = 0;
   for (int k = 0; k < nk; ++k) {
@@ -136,7 +142,12 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   const size_t o_e = o_d + al(8 * (size_t)n * nb);
   const size_t o_tau = o_e + al(8 * (size_t)n * nb), o_i = o_tau + al(16 * (size_t)n * nb);
   const size_t o_kd = o_i + al(12 * (size_t)nb);
-  const size_t total = o_kd + sizeof(eig_kdesc) * nb;
+  // variant 6: V (n x kBT), G and T (kBT x kBT), W1 and W2 (kBT x nev) per k-point
+  const size_t o_v = o_kd + al(sizeof(eig_kdesc) * nb), o_g = o_v + (blocked ? al(16 * (size_t)n * kBT * nb) : 0);
+  const size_t o_t = o_g + (blocked ? al(16 * (size_t)kBT * kBT * nb) : 0);
+  const size_t o_w1 = o_t + (blocked ? al(16 * (size_t)kBT * kBT * nb) : 0);
+  const size_t o_w2 = o_w1 + (blocked ? al(16 * (size_t)kBT * nev * nb) : 0);
+  const size_t total = o_w2 + (blocked ? al(16 * (size_t)kBT * nev * nb) : 0);
   int rc;
   if ((rc = grow(&c->eg, &c->eg_bytes, total))) return rc;
   char *base = (char *)c->eg;
@@ -154,7 +165,7 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
     if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
     if (i > 0) {   // per worker: real eigenvectors of T (n x n), their first nev columns complex
       if ((rc = grow((void **)&W.z, &W.zb, sizeof(double) * mat))) return rc;
-      if ((rc = grow((void **)&W.zc, &W.zcb, sizeof(cplx) * (size_t)n * nev))) return rc;
+      if (!blocked && (rc = grow((void **)&W.zc, &W.zcb, sizeof(cplx) * (size_t)n * nev))) return rc;
     }
   }
   auto fail = [](const char *what, int a, int b) {
@@ -188,6 +199,9 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   if (st == rocblas_status_success)   // C = Q T Q^H
     st = rocsolver_zhetrd_strided_batched(hb, up, n, A, n, sm, d, n, e, n, (rocblas_double_complex *)tau, n, nb);
   if (st != rocblas_status_success) return fail("trtri / trmm / zhetrd strided_batched status", (int)st, n);
+  // variant 6: the eigenvectors of T of k-point b in X = pz + b n nev (rows past nmat_k zero)
+  const size_t sx = (size_t)n * nev;
+  if (blocked) EXC_CHECK(hipMemsetAsync(pz, 0, sizeof(cplx) * sx * nb, c->stream));
   EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));
   for (int i = 1; i <= nw; ++i) EXC_CHECK(hipStreamWaitEvent(c->ew[i].s, c->eig_fork, 0));
   // per k-point on the workers; every worker is joined below, also after an error
@@ -200,6 +214,12 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
     if (st != rocblas_status_success) {
       err = fail("dstedc status", (int)st, b);
       break;
+    }
+    if (blocked) {
+      if ((err = exc_launch_eig_r2c((const double *)W.z, m, nev, m, pz + (size_t)b * sx, n, W.s))) break;
+      if (hipMemcpyAsync(kd[b].eval, db, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s) != hipSuccess)
+        err = EXC_ERR_HIP;
+      continue;
     }
     if ((err = exc_launch_eig_r2c((const double *)W.z, m, nev, m, W.zc, m, W.s))) break;
     st = rocsolver_zunmtr(W.blas, rocblas_side_left, up, rocblas_operation_none, m, nev, A + (size_t)b * mat, n,
@@ -219,6 +239,33 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
     EXC_CHECK(hipStreamWaitEvent(c->stream, c->ew[i].done, 0));
   }
   if (err) return err;
+  if (blocked) {
+    // X := Q X = H(n-2) ... H(0) X, block [i0, i0 + ib) of reflectors at a time, block 0 first
+    auto *V = (rocblas_double_complex *)(base + o_v), *G = (rocblas_double_complex *)(base + o_g);
+    auto *T = (rocblas_double_complex *)(base + o_t), *W1 = (rocblas_double_complex *)(base + o_w1);
+    auto *W2 = (rocblas_double_complex *)(base + o_w2);
+    const rocblas_double_complex zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+    const auto N = rocblas_operation_none, CT = rocblas_operation_conjugate_transpose;
+    const rocblas_stride sv = (rocblas_stride)n * kBT, st2 = (rocblas_stride)kBT * kBT;
+    const rocblas_stride sw = (rocblas_stride)kBT * nev, sxx = (rocblas_stride)sx;
+    for (int i0 = 0; i0 < n - 1 && st == rocblas_status_success; i0 += kBT) {
+      const int ib = std::min(kBT, n - 1 - i0), mb = i0 + ib;   // rows 0 .. mb-1 of the block's vectors
+      if ((rc = exc_launch_eig_vblock(ph, nb, n, i0, ib, sv, (cplx *)V, c->stream))) return rc;
+      st = rocblas_zgemm_strided_batched(hb, CT, N, ib, ib, mb, &one, V, n, sv, V, n, sv, &zero, G, ib, st2, nb);
+      if (st != rocblas_status_success) break;
+      if ((rc = exc_launch_eig_tblock(dkd, (const cplx *)G, tau, nb, n, i0, ib, st2, (cplx *)T, c->stream))) return rc;
+      st = rocblas_zgemm_strided_batched(hb, CT, N, ib, nev, mb, &one, V, n, sv, Z, n, sxx, &zero, W1, ib, sw, nb);
+      if (st == rocblas_status_success)
+        st = rocblas_zgemm_strided_batched(hb, N, N, ib, nev, ib, &one, T, ib, st2, W1, ib, sw, &zero, W2, ib, sw, nb);
+      if (st == rocblas_status_success)
+        st = rocblas_zgemm_strided_batched(hb, N, N, mb, nev, ib, &mone, V, n, sv, W2, ib, sw, &one, Z, n, sxx, nb);
+    }
+    if (st == rocblas_status_success)   // U^-1 X into ph (its reflectors are no longer needed)
+      st = rocblas_ztrmm_strided_batched(hb, rocblas_side_left, up, rocblas_operation_none, nu, n, nev, &one, B, n, sm,
+                                         Z, n, sxx, A, n, sxx, nb);
+    if (st != rocblas_status_success) return fail("blocked back-transform status", (int)st, n);
+    if ((rc = exc_launch_eig_copyout(dkd, nb, n, nev, ph, c->stream))) return rc;
+  }
   EXC_CHECK(hipMemcpyAsync(hi.data(), info, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   for (int b = 0; b < nb; ++b) {
@@ -277,8 +324,8 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
     if (!W.s) EXC_CHECK(hipStreamCreateWithFlags(&W.s, hipStreamNonBlocking));   // default priority
     if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
   }
-  const int v = c->eig_variant >= 0 ? c->eig_variant : 5;
-  if (v == 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv);
+  const int v = c->eig_variant >= 0 ? c->eig_variant : 6;
+  if (v == 5 || v == 6) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv, v == 6);
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));

@@ -96,6 +96,65 @@ __global__ __launch_bounds__(256) void eig_r2c_kernel(const double *zr, int m, i
   zc[r + (size_t)j * ldc] = {zr[r + (size_t)j * ldr], 0.0};
 }
 
+// Block i0 .. i0+ib-1 of the reflectors of the upper tridiagonal reduction (zhetrd 'U':
+// reflector i has v(0:i-1) = A(0:i-1, i+1), v(i) = 1, zero below) as explicit columns,
+// V[k sv + r + n c] for r < m = i0 + ib, of every k-point k of the batch
+__global__ __launch_bounds__(256) void eig_vblock_kernel(const cplx *A, int n, int i0, int ib, size_t sv, cplx *V) {
+  const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y, k = blockIdx.z;
+  const int m = i0 + ib;
+  if (r >= m) return;
+  const int i = i0 + c;
+  cplx v = {0.0, 0.0};
+  if (r < i) v = A[(size_t)k * n * n + r + (size_t)n * (i + 1)];
+  else if (r == i) v.re = 1.0;
+  V[(size_t)k * sv + r + (size_t)n * c] = v;
+}
+
+// T (ib x ib, lower triangular) of the block reflector H(i0+ib-1) ... H(i0) = I - V T V^H from
+// G = V^H V and tau (zlarft, direct 'B', storev 'C'): for i = ib-1 down to 0,
+// T(i, i) = tau_i, T(i+1:, i) = -tau_i T(i+1:, i+1:) G(i+1:, i).  The reflectors of k-point k
+// beyond its own order (i >= nmat_k - 1: the pad columns and the coupling column, zero
+// vectors) are left out (tau taken as 0).  One workgroup of kTB threads per k-point; thread
+// r owns row r of T.
+constexpr int kTB = 64;
+__global__ __launch_bounds__(kTB) void eig_tblock_kernel(const eig_kdesc *ks, const cplx *G, const cplx *tau, int n,
+                                                         int i0, int ib, size_t sgt, cplx *T) {
+  const int k = blockIdx.x, r = threadIdx.x;
+  const int last = ks[k].n - 1;   // reflectors i0 + i < last are the k-point's
+  const cplx *g = G + (size_t)k * sgt;   // G and T of k-point k: leading dimension ib, at k * sgt
+  cplx *t = T + (size_t)k * sgt;
+  __shared__ cplx tmp[kTB];
+  if (r < ib)
+    for (int c = 0; c < ib; ++c) t[r + ib * c] = czero();
+  for (int i = ib - 1; i >= 0; --i) {
+    const cplx ti = i0 + i < last ? tau[(size_t)k * n + i0 + i] : czero();
+    if (r > i && r < ib) {   // tmp(r) = -tau_i G(r, i)
+      const cplx p = cmul(ti, g[r + ib * i]);
+      tmp[r] = {-p.re, -p.im};
+    }
+    __syncthreads();
+    if (r > i && r < ib) {   // T(r, i) = sum_{q = i+1}^{r} T(r, q) tmp(q)
+      cplx acc = czero();
+      for (int q = i + 1; q <= r; ++q) {
+        const cplx p = cmul(t[r + ib * q], tmp[q]);
+        acc.re += p.re;
+        acc.im += p.im;
+      }
+      t[r + ib * i] = acc;
+    }
+    if (r == i) t[i + ib * i] = ti;
+    __syncthreads();
+  }
+}
+
+// evec_k(r, j) = x(r, j) for r < nmat_k, j < nev (x: n x nev per k-point, leading dimension n)
+__global__ __launch_bounds__(256) void eig_copyout_kernel(const eig_kdesc *ks, int n, int nev, const cplx *x) {
+  const int r = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  const eig_kdesc kd = ks[k];
+  if (r >= kd.n) return;
+  kd.evec[r + (size_t)j * kd.ldv] = x[(size_t)k * n * nev + r + (size_t)j * n];
+}
+
 inline int tiles(int n) { return (n + kET - 1) / kET; }
 
 }  // namespace
@@ -123,6 +182,32 @@ int exc_launch_eig_r2c(const double *zr, int m, int ncols, int ldr, cplx *zc, in
   if (m <= 0 || ncols <= 0) return EXC_OK;
   if (ncols > 65535 || ldr < m || ldc < m) return EXC_ERR_ARG;
   hipLaunchKernelGGL(eig_r2c_kernel, dim3((m + 255) / 256, ncols), dim3(256), 0, s, zr, m, ldr, zc, ldc);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_vblock(const cplx *A, int nb, int n, int i0, int ib, size_t sv, cplx *V, hipStream_t s) {
+  if (nb <= 0 || ib <= 0) return EXC_OK;
+  if (sv < (size_t)n * ib) return EXC_ERR_ARG;
+  if (nb > 65535 || ib > 64 || i0 < 0 || i0 + ib > n - 1) return EXC_ERR_ARG;   // reflector i in column i + 1
+  hipLaunchKernelGGL(eig_vblock_kernel, dim3((i0 + ib + 255) / 256, ib, nb), dim3(256), 0, s, A, n, i0, ib, sv, V);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_tblock(const eig_kdesc *ks, const cplx *G, const cplx *tau, int nb, int n, int i0, int ib,
+                          size_t sgt, cplx *T, hipStream_t s) {
+  if (nb <= 0 || ib <= 0) return EXC_OK;
+  if (ib > kTB || i0 + ib > n || sgt < (size_t)ib * ib) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(eig_tblock_kernel, dim3(nb), dim3(kTB), 0, s, ks, G, tau, n, i0, ib, sgt, T);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_copyout(const eig_kdesc *ks, int nb, int n, int nev, const cplx *x, hipStream_t s) {
+  if (nb <= 0 || nev <= 0) return EXC_OK;
+  if (nb > 65535 || nev > 65535) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(eig_copyout_kernel, dim3((n + 255) / 256, nev, nb), dim3(256), 0, s, ks, n, nev, x);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

@@ -80,10 +80,11 @@ def test_eigen_reports_indefinite_overlap():
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(7))
 def test_eigen_solver_variants(variant):
     """Every solver path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv, 4 level-3 reduction +
-    zheevd; 5 is the grouped multi-k path, the same as 4 for one k-point) to the same bars."""
+    zheevd; 5 and 6 are the grouped multi-k paths, the same as 4 for one k-point) to the same
+    bars."""
     from exciting_amd import lapw
     from oracle import lapw_ref as R
     w = workload(1, 1)
@@ -145,7 +146,7 @@ def _random_pair(rng, n, lower_garbage):
     return H, O, np.asfortranarray(Hs), np.asfortranarray(Os)
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(7))
 def test_eigen_reads_upper_triangles_only(variant):
     """Like ZHEGVX(uplo 'U'): the lower triangles and the imaginary diagonal of H, O are not
     part of the input."""
@@ -167,18 +168,21 @@ def _to_dev(M):
     return torch.from_numpy(np.ascontiguousarray(M.T)).cuda()   # (nmat, ld) row-major = Fortran (ld, nmat)
 
 
-def test_eigen_grouped_ragged():
-    """The grouped multi-k solve (variant 5: every k-point padded to the largest nmat, one
-    strided-batched launch per step) on k-points of very different orders, lower triangles
-    filled with unrelated values: every k-point to the LAPACK bars, inputs left unchanged."""
+@pytest.mark.parametrize("variant", [5, 6])
+def test_eigen_grouped_ragged(variant):
+    """The grouped multi-k solve (every k-point padded to the largest nmat, one strided-batched
+    launch per step; variant 6 also back-transforms the eigenvectors as one batch, 64
+    reflectors per block, so orders above 64 cross block boundaries) on k-points of very
+    different orders, lower triangles filled with unrelated values: every k-point to the
+    LAPACK bars, inputs left unchanged."""
     import scipy.linalg
     import torch
     from exciting_amd import lapw
     w = small("a")
     ctx = lapw.Context(w.crystal)
-    ctx.set_variant("eigen", 5)
+    ctx.set_variant("eigen", variant)
     rng = np.random.default_rng(11)
-    sizes, nev = [64, 33, 47, 64, 20], 20
+    sizes, nev = [64, 33, 47, 64, 20, 150, 129], 20
     pairs = [_random_pair(rng, n, True) for n in sizes]
     hs = [_to_dev(p[2]) for p in pairs]
     os_ = [_to_dev(p[3]) for p in pairs]
@@ -192,19 +196,20 @@ def test_eigen_grouped_ragged():
         assert torch.equal(hs[i], h_in[i]) and torch.equal(os_[i], o_in[i])
         H, O = pairs[i][0], pairs[i][1]
         e_ref = scipy.linalg.eigh(H, O, eigvals_only=True, subset_by_index=[0, nev - 1])
-        _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"grouped k{i} n={n}")
+        _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"grouped v{variant} k{i} n={n}")
 
 
-def test_eigen_grouped_after_setup():
-    """Variant 5 after the grouped setup of workload 1 (6 k-points, nmat 1182-1282): each
-    k-point against LAPACK on the same GPU-built H and O."""
+@pytest.mark.parametrize("variant", [5, 6])
+def test_eigen_grouped_after_setup(variant):
+    """The grouped paths after the grouped setup of workload 1 (6 k-points, nmat 1182-1282):
+    each k-point against LAPACK on the same GPU-built H and O."""
     import scipy.linalg
     import torch
     from exciting_amd import lapw
     w = workload(1, 6)
     s = torch.cuda.Stream()
     ctx = lapw.Context(w.crystal, stream=s)
-    ctx.set_variant("eigen", 5)
+    ctx.set_variant("eigen", variant)
     nev = 200
     with torch.cuda.stream(s):
         b = lapw.DeviceBatch(w.crystal, w.kpoints)
@@ -220,4 +225,4 @@ def test_eigen_grouped_after_setup():
         e_ref = scipy.linalg.eigh(Hh[i], Oh[i], lower=False, eigvals_only=True, subset_by_index=[0, nev - 1],
                                   driver="gvx")
         _check(Hh[i], Oh[i], evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref,
-               f"grouped after setup k{i} n={b.nmat[i]}")
+               f"grouped v{variant} after setup k{i} n={b.nmat[i]}")
