@@ -120,11 +120,16 @@ int eig_status(const int *hi, int nmat, int nev, int v) {
 //  variant 6: for the whole batch, the reflectors applied kBT at a time as block reflectors
 //    I - V T V^H (T from zlarft's recurrence, k_eigen.hip) with strided-batched GEMMs, then
 //    one strided-batched trmm; the eigenvectors of every k-point sit in an n x nev slice
-//    whose rows past nmat_k stay zero, so the pad's rows never mix in.
+//    whose rows past nmat_k stay zero, so the pad's rows never mix in;
+//  variant 7: as 6, with the tridiagonal eigenpairs of the lowest nev only, by bisection and
+//    inverse iteration (dstebz + dstein, as ZHEGVX range 'I' with abstol = the reference's
+//    default evaltol) instead of the full divide and conquer.
 // H and O are read, not modified.
-constexpr int kBT = 64;   // reflectors per block of the batched back-transform
+constexpr int kBT = 64;            // reflectors per block of the batched back-transform
+constexpr double kAbsTol = 1e-14;  // bisection tolerance (Ha): evaltol's default, groundstate.xsd:512-513
 int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *const *h, double *const *o, int nev,
-                double *const *eval, double *const *evec, const int *ldv, bool blocked) {
+                double *const *eval, double *const *evec, const int *ldv, int v) {
+  const bool blocked = v >= 6, bisect = v == 7;
   std::vector<eig_kdesc> kd;
   int n = 0;
   for (int k = 0; k < nk; ++k) {
@@ -147,7 +152,13 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   const size_t o_t = o_g + (blocked ? al(16 * (size_t)kBT * kBT * nb) : 0);
   const size_t o_w1 = o_t + (blocked ? al(16 * (size_t)kBT * kBT * nb) : 0);
   const size_t o_w2 = o_w1 + (blocked ? al(16 * (size_t)kBT * nev * nb) : 0);
-  const size_t total = o_w2 + (blocked ? al(16 * (size_t)kBT * nev * nb) : 0);
+  const size_t o_wv = o_w2 + (blocked ? al(16 * (size_t)kBT * nev * nb) : 0);
+  // variant 7: eigenvalues, block and split indices, failure flags (n each) and 4 ints per k-point
+  const size_t o_ib = o_wv + (bisect ? al(8 * (size_t)n * nb) : 0);
+  const size_t o_is = o_ib + (bisect ? al(4 * (size_t)n * nb) : 0);
+  const size_t o_if = o_is + (bisect ? al(4 * (size_t)n * nb) : 0);
+  const size_t o_si = o_if + (bisect ? al(4 * (size_t)n * nb) : 0);
+  const size_t total = o_si + (bisect ? al(16 * (size_t)nb) : 0);
   int rc;
   if ((rc = grow(&c->eg, &c->eg_bytes, total))) return rc;
   char *base = (char *)c->eg;
@@ -210,6 +221,23 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
     exc_ctx::EigWorker &W = c->ew[1 + b % nw];
     const int m = kd[b].n;
     double *db = d + (size_t)b * n, *eb = e + (size_t)b * n;
+    if (bisect) {   // lowest nev of T_k: bisection, inverse iteration, then ordered by value
+      double *wv = (double *)(base + o_wv) + (size_t)b * n;
+      int *ibk = (int *)(base + o_ib) + (size_t)b * n, *isp = (int *)(base + o_is) + (size_t)b * n;
+      int *ifl = (int *)(base + o_if) + (size_t)b * n, *si = (int *)(base + o_si) + 4 * b;
+      st = rocsolver_dstebz(W.blas, rocblas_erange_index, rocblas_eorder_blocks, m, 0.0, 0.0, 1, nev, kAbsTol, db, eb,
+                            si, si + 1, wv, ibk, isp, si + 2);
+      if (st == rocblas_status_success)
+        st = rocsolver_dstein(W.blas, m, db, eb, si, wv, ibk, isp, (double *)W.z, m, ifl, si + 3);
+      if (st != rocblas_status_success) {
+        err = fail("dstebz / dstein status", (int)st, b);
+        break;
+      }
+      if ((err = exc_launch_eig_sort_r2c((const double *)W.z, m, nev, m, si, wv, pz + (size_t)b * sx, n,
+                                         kd[b].eval, W.s)))
+        break;
+      continue;
+    }
     st = rocsolver_dstedc(W.blas, rocblas_evect_tridiagonal, m, db, eb, (double *)W.z, m, info + 2 * nb + b);
     if (st != rocblas_status_success) {
       err = fail("dstedc status", (int)st, b);
@@ -270,7 +298,16 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   EXC_CHECK(hipStreamSynchronize(c->stream));
   for (int b = 0; b < nb; ++b) {
     if (hi[nb + b] != 0) return fail("ztrtri info", b, hi[nb + b]);   // of a positive definite factor
-    if (hi[2 * nb + b] != 0) return hi[2 * nb + b];                 // eigenvalues that did not converge
+    if (!bisect && hi[2 * nb + b] != 0) return hi[2 * nb + b];      // eigenvalues that did not converge
+  }
+  if (bisect) {   // per k-point: eigenvalues found, split blocks, dstebz info, dstein info
+    std::vector<int> si(4 * (size_t)nb);
+    EXC_CHECK(hipMemcpy(si.data(), base + o_si, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb; ++b) {
+      // at least nev (a cluster of equal eigenvalues at the upper end comes whole)
+      if (si[4 * b] < nev || si[4 * b + 2] != 0) return fail("dstebz count / info", si[4 * b], si[4 * b + 2]);
+      if (si[4 * b + 3] != 0) return si[4 * b + 3];   // eigenvectors that did not converge (zhegvx info)
+    }
   }
   return EXC_OK;
 }
@@ -325,7 +362,7 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
     if (!W.done) EXC_CHECK(hipEventCreateWithFlags(&W.done, hipEventDisableTiming));
   }
   const int v = c->eig_variant >= 0 ? c->eig_variant : 6;
-  if (v == 5 || v == 6) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv, v == 6);
+  if (v >= 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv, v);
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));

@@ -124,7 +124,7 @@ int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int l
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
 constexpr int kBsplitVariants = 4;
-constexpr int kEigenVariants = 7;   // generalized eigensolver paths (exc_eigen_dev)
+constexpr int kEigenVariants = 8;   // generalized eigensolver paths (exc_eigen_dev)
 size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);
@@ -146,6 +146,8 @@ int exc_launch_eig_r2c(const double *zr, int m, int ncols, int ldr, cplx *zc, in
 int exc_launch_eig_vblock(const cplx *A, int nb, int n, int i0, int ib, size_t sv, cplx *V, hipStream_t s);
 int exc_launch_eig_tblock(const eig_kdesc *ks, const cplx *G, const cplx *tau, int nb, int n, int i0, int ib,
                           size_t sgt, cplx *T, hipStream_t s);
+int exc_launch_eig_sort_r2c(const double *zr, int m, int nev, int ldr, const int *nfound, const double *w, cplx *x,
+                            int ldx, double *eval, hipStream_t s);
 int exc_launch_eig_copyout(const eig_kdesc *ks, int nb, int n, int nev, const cplx *x, hipStream_t s);
 // muffin-tin density matrix helpers (k_dmat.hip)
 int exc_launch_dmat_pack(const exc_dev &d, int ws, int nst, const cplx *w, const cplx *evec, int ldv, int ngp,

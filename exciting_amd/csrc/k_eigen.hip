@@ -147,6 +147,34 @@ __global__ __launch_bounds__(kTB) void eig_tblock_kernel(const eig_kdesc *ks, co
   }
 }
 
+// Eigenpairs of a tridiagonal matrix from bisection and inverse iteration (dstebz with eorder
+// 'blocks', then dstein), ordered by split-off block; *nfound >= nev of them (the bisection
+// keeps a whole cluster of equal eigenvalues at the upper end of the index range).  Column j
+// moves to its rank among the nfound eigenvalues (ties by index) if that is below nev:
+// x(r, rank_j) = z(r, j) as complex for r < m, eval(rank_j) = w(j).  Every workgroup of
+// column j counts the rank itself (nfound reads); the grid covers m columns.
+__global__ __launch_bounds__(256) void eig_sort_r2c_kernel(const double *zr, int m, int nev, int ldr,
+                                                           const int *nfound, const double *w, cplx *x, int ldx,
+                                                           double *eval) {
+  const int r = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  const int nf = min(*nfound, m);
+  if (j >= nf) return;   // uniform over the workgroup
+  __shared__ int part[256];
+  const double wj = w[j];
+  int cnt = 0;
+  for (int i = threadIdx.x; i < nf; i += 256) cnt += (w[i] < wj) || (w[i] == wj && i < j);
+  part[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  const int rank = part[0];
+  if (rank >= nev) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) eval[rank] = wj;
+  if (r < m) x[r + (size_t)rank * ldx] = {zr[r + (size_t)j * ldr], 0.0};
+}
+
 // evec_k(r, j) = x(r, j) for r < nmat_k, j < nev (x: n x nev per k-point, leading dimension n)
 __global__ __launch_bounds__(256) void eig_copyout_kernel(const eig_kdesc *ks, int n, int nev, const cplx *x) {
   const int r = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
@@ -200,6 +228,16 @@ int exc_launch_eig_tblock(const eig_kdesc *ks, const cplx *G, const cplx *tau, i
   if (nb <= 0 || ib <= 0) return EXC_OK;
   if (ib > kTB || i0 + ib > n || sgt < (size_t)ib * ib) return EXC_ERR_ARG;
   hipLaunchKernelGGL(eig_tblock_kernel, dim3(nb), dim3(kTB), 0, s, ks, G, tau, n, i0, ib, sgt, T);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_eig_sort_r2c(const double *zr, int m, int nev, int ldr, const int *nfound, const double *w, cplx *x,
+                            int ldx, double *eval, hipStream_t s) {
+  if (m <= 0 || nev <= 0) return EXC_OK;
+  if (m > 65535 || nev > m) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(eig_sort_r2c_kernel, dim3((m + 255) / 256, m), dim3(256), 0, s, zr, m, nev, ldr, nfound, w, x,
+                     ldx, eval);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

@@ -80,10 +80,10 @@ def test_eigen_reports_indefinite_overlap():
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
 
 
-@pytest.mark.parametrize("variant", range(7))
+@pytest.mark.parametrize("variant", range(8))
 def test_eigen_solver_variants(variant):
     """Every solver path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv, 4 level-3 reduction +
-    zheevd; 5 and 6 are the grouped multi-k paths, the same as 4 for one k-point) to the same
+    zheevd; 5-7 are the grouped multi-k paths, the same as 4 for one k-point) to the same
     bars."""
     from exciting_amd import lapw
     from oracle import lapw_ref as R
@@ -146,7 +146,7 @@ def _random_pair(rng, n, lower_garbage):
     return H, O, np.asfortranarray(Hs), np.asfortranarray(Os)
 
 
-@pytest.mark.parametrize("variant", range(7))
+@pytest.mark.parametrize("variant", range(8))
 def test_eigen_reads_upper_triangles_only(variant):
     """Like ZHEGVX(uplo 'U'): the lower triangles and the imaginary diagonal of H, O are not
     part of the input."""
@@ -168,11 +168,12 @@ def _to_dev(M):
     return torch.from_numpy(np.ascontiguousarray(M.T)).cuda()   # (nmat, ld) row-major = Fortran (ld, nmat)
 
 
-@pytest.mark.parametrize("variant", [5, 6])
+@pytest.mark.parametrize("variant", [5, 6, 7])
 def test_eigen_grouped_ragged(variant):
     """The grouped multi-k solve (every k-point padded to the largest nmat, one strided-batched
-    launch per step; variant 6 also back-transforms the eigenvectors as one batch, 64
-    reflectors per block, so orders above 64 cross block boundaries) on k-points of very
+    launch per step; variants 6 and 7 also back-transform the eigenvectors as one batch, 64
+    reflectors per block, so orders above 64 cross block boundaries; 7 finds the tridiagonal
+    eigenpairs by bisection and inverse iteration, as ZHEGVX) on k-points of very
     different orders, lower triangles filled with unrelated values: every k-point to the
     LAPACK bars, inputs left unchanged."""
     import scipy.linalg
@@ -199,7 +200,7 @@ def test_eigen_grouped_ragged(variant):
         _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"grouped v{variant} k{i} n={n}")
 
 
-@pytest.mark.parametrize("variant", [5, 6])
+@pytest.mark.parametrize("variant", [5, 6, 7])
 def test_eigen_grouped_after_setup(variant):
     """The grouped paths after the grouped setup of workload 1 (6 k-points, nmat 1182-1282):
     each k-point against LAPACK on the same GPU-built H and O."""
@@ -226,3 +227,43 @@ def test_eigen_grouped_after_setup(variant):
                                   driver="gvx")
         _check(Hh[i], Oh[i], evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref,
                f"grouped v{variant} after setup k{i} n={b.nmat[i]}")
+
+
+@pytest.mark.parametrize("variant", [5, 6, 7])
+def test_eigen_grouped_degenerate(variant):
+    """Exactly degenerate spectra (eigenvalues in pairs and triples, as symmetry makes them):
+    H = L Q diag(lam) Q^H L^H with O = L L^H, so the generalized eigenvalues are lam.  The
+    eigenvectors of a multiple eigenvalue are fixed only as a subspace; the residual and the
+    O-orthonormality check every one of them (inverse iteration must orthogonalize within the
+    clusters)."""
+    import torch
+    from exciting_amd import lapw
+    w = small("a")
+    ctx = lapw.Context(w.crystal)
+    ctx.set_variant("eigen", variant)
+    rng = np.random.default_rng(23)
+    sizes, nev = [150, 97], 40
+    hs, os_, refs = [], [], []
+    for n in sizes:
+        k3 = n // 9                       # triples, then pairs, then singles: n values
+        k2 = (n - 3 * k3) // 2
+        k1 = n - 3 * k3 - 2 * k2
+        vals = rng.uniform(-2.0, 3.0, k3 + k2 + k1)
+        lam = np.sort(np.concatenate([np.repeat(vals[:k3], 3), np.repeat(vals[k3:k3 + k2], 2), vals[k3 + k2:]]))
+        X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        Q, _ = np.linalg.qr(X)
+        Y = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        O = Y @ Y.conj().T / n + np.eye(n)
+        L = np.linalg.cholesky(O)
+        H = L @ (Q * lam[None, :]) @ Q.conj().T @ L.conj().T
+        H = (H + H.conj().T) / 2
+        hs.append(_to_dev(np.asfortranarray(H)))
+        os_.append(_to_dev(np.asfortranarray(O)))
+        refs.append((H, O, lam[:nev]))
+    evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in sizes]
+    vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in sizes]
+    ctx.eigen_batched_dev(sizes, hs, os_, nev, evs, vecs)
+    torch.cuda.synchronize()
+    for i, n in enumerate(sizes):
+        H, O, e_ref = refs[i]
+        _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"degenerate v{variant} k{i} n={n}")
