@@ -123,9 +123,9 @@ __global__ __launch_bounds__(kTB) void eig_tblock_kernel(const eig_kdesc *ks, co
   const int last = ks[k].n - 1;   // reflectors i0 + i < last are the k-point's
   const cplx *g = G + (size_t)k * sgt;   // G and T of k-point k: leading dimension ib, at k * sgt
   cplx *t = T + (size_t)k * sgt;
+  __shared__ double tre[kTB][kTB], tim[kTB][kTB];   // T(r, q) at [q][r]: row r is thread r's
   __shared__ cplx tmp[kTB];
-  if (r < ib)
-    for (int c = 0; c < ib; ++c) t[r + ib * c] = czero();
+  for (int q = 0; q < kTB; ++q) tre[q][r] = tim[q][r] = 0.0;
   for (int i = ib - 1; i >= 0; --i) {
     const cplx ti = i0 + i < last ? tau[(size_t)k * n + i0 + i] : czero();
     if (r > i && r < ib) {   // tmp(r) = -tau_i G(r, i)
@@ -134,17 +134,23 @@ __global__ __launch_bounds__(kTB) void eig_tblock_kernel(const eig_kdesc *ks, co
     }
     __syncthreads();
     if (r > i && r < ib) {   // T(r, i) = sum_{q = i+1}^{r} T(r, q) tmp(q)
-      cplx acc = czero();
+      double are = 0.0, aim = 0.0;
       for (int q = i + 1; q <= r; ++q) {
-        const cplx p = cmul(t[r + ib * q], tmp[q]);
-        acc.re += p.re;
-        acc.im += p.im;
+        const double xr = tre[q][r], xi = tim[q][r], yr = tmp[q].re, yi = tmp[q].im;
+        are += xr * yr - xi * yi;
+        aim += xr * yi + xi * yr;
       }
-      t[r + ib * i] = acc;
+      tre[i][r] = are;
+      tim[i][r] = aim;
     }
-    if (r == i) t[i + ib * i] = ti;
+    if (r == i) {
+      tre[i][i] = ti.re;
+      tim[i][i] = ti.im;
+    }
     __syncthreads();
   }
+  if (r < ib)
+    for (int q = 0; q < ib; ++q) t[r + (size_t)ib * q] = {tre[q][r], tim[q][r]};
 }
 
 // Eigenpairs of a tridiagonal matrix from bisection and inverse iteration (dstebz with eorder
