@@ -1,0 +1,117 @@
+"""Pins the oracle's direct-route muffin-tin density (oracle/rhovalk_ref.py: sphcover,
+genshtmat, wavefmt, rhovalk; rhovalk.f90:99-207, spinless, first-variational states).
+
+* sphcover: against the reference's own routine (src_lib/sphcover.f90, compiled by
+  oracle/Makefile into oracle/_ref) and against the vectors it produced
+  (tests/golden/rhovalk_ref.npz), for point counts of lmaxvr 0..8 and odd counts.
+* genshtmat: rbshtvr rfshtvr = 1 and the points are unisolvent (the reference stops if the
+  covering is improper); zbshtvr and rbshtvr are the oracle's Y_lm and R_lm (pinned in
+  test_oracle_specfun / test_oracle_hmlint) at those points.
+* rhovalk against the density-matrix route: with radial functions zero above l = 2 the
+  wavefunctions have l <= 2, |psi|^2 has l <= 4 <= lmaxvr and the reference's point
+  transform is exact, so rhovalk must equal genrhomt(gendmatmt(...)) (both pinned on their
+  own; independent contractions: wavefunctions squared on the sphere vs. Gaunt sums over the
+  density matrix).  A conjugated coefficient vector (zdotc instead of zdotu) or lo
+  coefficients read one column off fail it.
+* lradstep: only every lradstep-th mesh point is written, with the lradstep = 1 values.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import small
+from oracle import hmlint_ref as M
+from oracle import lapw_ref as R
+from oracle import rhovalk_ref as V
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REFLIB = os.path.join(ROOT, "oracle", "_ref", "libexcref.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "rhovalk_ref.npz")
+COUNTS = [(l + 1) ** 2 for l in range(9)] + [2, 3, 7, 50, 77]
+
+
+def _reflib():
+    if not os.path.exists(REFLIB):
+        return None
+    return C.CDLL(REFLIB)
+
+
+def ref_sphcover(lib, n):
+    tp = np.zeros((2, n), order="F")
+    lib.sphcover_(C.byref(C.c_int(n)), tp.ctypes.data_as(C.c_void_p))
+    return tp
+
+
+def test_sphcover_against_reference_build():
+    lib = _reflib()
+    if lib is None:
+        pytest.skip("oracle/_ref not built (needs the reference checkout; see oracle/Makefile)")
+    for n in COUNTS:
+        assert np.abs(V.sphcover(n) - ref_sphcover(lib, n)).max() <= 1e-15, n
+
+
+def test_sphcover_against_golden():
+    g = np.load(GOLDEN)
+    for n, tp in zip(g["counts"], [g[f"tp{n}"] for n in g["counts"]]):
+        assert np.abs(V.sphcover(int(n)) - tp).max() <= 1e-15
+
+
+def test_sht_pair():
+    for lmaxvr in (2, 4, 6, 8):
+        zb, rb, rf = V.genshtmat(lmaxvr)
+        n = (lmaxvr + 1) ** 2
+        assert np.linalg.cond(rb) < 1e6
+        assert np.abs(rb @ rf - np.eye(n)).max() <= 1e-11
+        tp = V.sphcover(n)
+        assert np.abs(zb[5] - R.specfun.genylm(lmaxvr, tp[0, 5:6], tp[1, 5:6])[:, 0]).max() <= 1e-15
+        assert np.abs(rb[7] - M.rlm(lmaxvr, tp[0, 7], tp[1, 7])[0]).max() <= 1e-15
+
+
+def _band_limited_case(seed=3):
+    from exciting_amd import synth
+    w = small("d")
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(120, 100), lmaxvr=4, seed=seed)
+    rd.apwfr[:, :, :, 3:, :] = 0.0            # wavefunctions of l <= 2 only (lo l <= 2 too)
+    assert int(cr.lorbl.max()) <= 2 and cr.lmaxapw >= 4
+    apw = R.match(cr, kp)
+    rng = np.random.default_rng(seed)
+    nmat, nst = kp.ngp + cr.nlotot, 7
+    Cv = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    occ = rng.uniform(0.2, 2.0, nst)
+    return cr, kp, rd, apw, Cv, occ
+
+
+def test_rhovalk_equals_density_matrix_route():
+    cr, kp, rd, apw, Cv, occ = _band_limited_case()
+    wk = 0.3
+    rho = V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ, wk)
+    rho_dm = M.genrhomt(cr, rd, R.gendmatmt(cr, apw, kp.ngp, Cv, occ, wk))
+    scale = np.abs(rho_dm).max()
+    assert scale > 0
+    assert np.abs(rho - rho_dm).max() <= 1e-12 * scale
+    # planted mistakes fail it: conjugated coefficients, lo coefficients one column off
+    bad = V.rhovalk(cr, rd, apw, kp.ngp, Cv.conj(), occ, wk)
+    assert np.abs(bad - rho_dm).max() > 1e-3 * scale
+    C2 = Cv.copy()
+    C2[kp.ngp:] = np.roll(Cv[kp.ngp:], 1, axis=0)
+    bad = V.rhovalk(cr, rd, apw, kp.ngp, C2, occ, wk)
+    assert np.abs(bad - rho_dm).max() > 1e-3 * scale
+
+
+def test_rhovalk_radial_step_and_epsocc():
+    cr, kp, rd, apw, Cv, occ = _band_limited_case(5)
+    occ[2] = 1e-12                             # below epsocc / wkpt: skipped
+    r1 = V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ, 0.5)
+    r3 = V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ, 0.5, lradstep=3)
+    for ias in range(cr.natm):
+        n = int(rd.nrmt[cr.spidx[ias]])
+        on = np.arange(0, n, 3)
+        off = np.setdiff1d(np.arange(rd.spr.shape[0]), on)
+        assert np.abs(r3[:, on, ias] - r1[:, on, ias]).max() <= 1e-14 * np.abs(r1).max()
+        assert not r3[:, off, ias].any()
+    occ2 = occ.copy()
+    occ2[2] = 0.0
+    assert np.array_equal(V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ2, 0.5), r1)
