@@ -277,6 +277,60 @@ def run_hmlint(args, w, log):
     log(f"  Theta(G), Theta(r), V(G) (gencfun + genveffig, grid {grid}): {it['ms_per_call']:.2f} ms per call, "
         f"kernels + FFTs {it['kernel_ms']:.3f} ms" + (f", numpy oracle {it['cpu_oracle_ms']:.0f} ms"
                                                      if it["cpu_oracle_ms"] else ""))
+    res["direct_density"] = run_rhovalk(args, w, ctx, reps, log)
+    return res
+
+
+def run_rhovalk(args, w, ctx, reps, log):
+    """The direct-route muffin-tin density of one k-point (exc_rhovalk_dev) at the reference's
+    default lmaxvr 8, lradstep 1: radial data re-made at lmaxvr 8 (exc_ctx_hmlint), apwalm of
+    the first k-point, 200 seeded coefficient vectors, occupations in [0, 2].  Flops counted:
+    8 lmmaxvr^2 per (atom, mesh point, state) for the backward transform plus the A^T C
+    product; the CPU baseline is the numpy oracle on the first 20 states, scaled by 10."""
+    import torch
+    from exciting_amd import synth
+    from oracle import rhovalk_ref as V
+    cr, kp = w.crystal, w.kpoints[0]
+    lmaxvr, nst = 8, 200
+    rd = synth.make_radial(cr, nr=(500, 460), lmaxvr=lmaxvr)
+    ctx.hmlint(rd, out=False)
+    apw = ctx.match(kp)
+    rng = np.random.default_rng(20261203)
+    nmat = kp.ngp + cr.nlotot
+    C = np.asfortranarray(rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst)))
+    occ = rng.uniform(0.0, 2.0, nst)
+    zb, _, rf = V.genshtmat(lmaxvr)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x.T)).cuda()   # Fortran order on the device
+    a_d, c_d, zb_d, rf_d, occ_d = t(np.asfortranarray(apw)), t(C), t(zb), t(rf), torch.from_numpy(occ).cuda()
+    nrm, lmv = rd.spr.shape[0], (lmaxvr + 1) ** 2
+    rho = torch.zeros((cr.natm, nrm, lmv), dtype=torch.float64, device="cuda")
+
+    def call():
+        ctx.rhovalk_dev(1, zb_d, rf_d, kp.ngp, apw.shape[0], a_d, nst, nmat, c_d, occ_d, 0.25, rho)
+
+    call()
+    torch.cuda.synchronize()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    ms, _ = ctx.read_timing()["scf"]
+    npts = sum(int(rd.nrmt[cr.spidx[a]]) for a in range(cr.natm))
+    flops = 8.0 * lmv * lmv * npts * nst + 8.0 * cr.natm * cr.apwordmax * cr.lmmaxapw * nst * kp.ngp
+    res = {"ms_per_k": 1000.0 * dt, "kernel_ms": ms / reps, "states": nst, "lmaxvr": lmaxvr,
+           "tflops": flops / (ms / reps * 1e-3) / 1e12, "flops_per_k": flops, "cpu_oracle_ms_per_k": None}
+    if not args.no_cpu_baseline:
+        t1 = time.perf_counter()
+        V.rhovalk(cr, rd, apw, kp.ngp, C[:, :20], occ[:20], 0.25, sht=(zb, None, rf))
+        res["cpu_oracle_ms_per_k"] = 10.0 * 1000.0 * (time.perf_counter() - t1)
+    log(f"  direct muffin-tin density (rhovalk, {nst} states, lmaxvr {lmaxvr}): {res['ms_per_k']:.3f} ms per k-point, "
+        f"kernels {res['kernel_ms']:.3f} ms, {res['tflops']:.1f} TFLOP/s"
+        + (f", numpy oracle {res['cpu_oracle_ms_per_k']:.0f} ms per k-point (20 states x 10)"
+           if res["cpu_oracle_ms_per_k"] else ""))
     return res
 
 

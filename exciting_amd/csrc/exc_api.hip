@@ -99,6 +99,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->eig_int) hipFree(c->eig_int);
   if (c->eg) hipFree(c->eg);
   if (c->dmw) hipFree(c->dmw);
+  if (c->rvw) hipFree(c->rvw);
   if (c->gnt) hipFree(c->gnt);
   if (c->have_fft) hipfftDestroy(c->fft);
   if (c->fftb_n > 0) hipfftDestroy(c->fftb);

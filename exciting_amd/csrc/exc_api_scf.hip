@@ -189,6 +189,85 @@ int exc_genrhomt(exc_ctx *c, const double *dm, double *rhomt) {
   return EXC_OK;
 }
 
+int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp, int ngkmax,
+                    const double *apwalm, int nst, int ldv, const double *evecfv, const double *occ, double wkpt,
+                    double epsocc, double *rhomt) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (c->hml_lmaxvr < 0) return EXC_ERR_STATE;   // the radial functions of exc_ctx_hmlint
+  const exc_dev &d = c->d;
+  const int lmaxvr = c->hml_lmaxvr, nrmtmax = c->hml_nrmtmax, ns = c->dims.nspecies;
+  if (lmaxvr > d.lmaxapw) return EXC_ERR_ARG;   // wavefmt.f90:76-81
+  if (lmaxvr > 8 || d.apwordmax > 4) return EXC_ERR_UNSUPPORTED;
+  if (lradstep < 1 || nst < 0 || nst > 65535 || ngp < 0 || ngkmax < (ngp > 1 ? ngp : 1) || ldv < ngp + d.nlotot ||
+      ldv < 1)
+    return EXC_ERR_ARG;
+  if (nst == 0) return EXC_OK;
+  if (!zbshtvr || !rfshtvr || !apwalm || !evecfv || !occ || !rhomt) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const int ldw = d.natm * d.naa;
+  if ((rc = grow(&c->rvw, &c->rvw_bytes, sizeof(cplx) * (size_t)ldw * nst))) return rc;
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
+  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+    W.blas = nullptr;
+    return EXC_ERR_HIP;
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+  const HmlLayout H = hml_layout(c, lmaxvr, nrmtmax);
+  const char *b = (const char *)c->hml;
+  const int *di = (const int *)(b + H.o_i);
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  StageTimer t(c, EXC_STAGE_SCF);
+  // W = A^T C for all atoms: (natm naa) x nst, row a naa + io + apwordmax lm (the zdotu of
+  // wavefmt.f90:95 for every (atom, io, lm) and state at once)
+  if (ngp > 0) {
+    if (rocblas_zgemm(W.blas, rocblas_operation_transpose, rocblas_operation_none, ldw, nst, ngp, &one,
+                      (const rocblas_double_complex *)apwalm, ngkmax, (const rocblas_double_complex *)evecfv, ldv,
+                      &zero, (rocblas_double_complex *)c->rvw, ldw) != rocblas_status_success)
+      return EXC_ERR_HIP;
+  } else {
+    EXC_CHECK(hipMemsetAsync(c->rvw, 0, sizeof(cplx) * (size_t)ldw * nst, c->stream));
+  }
+  return exc_launch_rhovalk(d, lmaxvr, lradstep, nrmtmax, c->dims.nlomax, di, di + ns, di + 2 * ns,
+                            (const double *)(b + H.o_apw), (const double *)(b + H.o_lo), (const cplx *)c->rvw, ldw,
+                            ngp, (const cplx *)evecfv, ldv, nst, occ, wkpt, epsocc, (const cplx *)zbshtvr, rfshtvr,
+                            rhomt, c->stream);
+}
+
+int exc_rhovalk(exc_ctx *c, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp, int ngkmax,
+                const double *apwalm, int nst, int ldv, const double *evecfv, const double *occ, double wkpt,
+                double epsocc, double *rhomt) {
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (c->hml_lmaxvr < 0) return EXC_ERR_STATE;
+  if (nst < 0 || ldv < 1 || ngkmax < 1) return EXC_ERR_ARG;
+  if (nst == 0) return EXC_OK;
+  if (!zbshtvr || !rfshtvr || !apwalm || !evecfv || !occ || !rhomt) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const exc_dev &d = c->d;
+  const size_t lmv = (size_t)(c->hml_lmaxvr + 1) * (c->hml_lmaxvr + 1);
+  const size_t b_a = 16 * (size_t)ngkmax * d.naa * d.natm, b_v = 16 * (size_t)ldv * nst, b_o = 8 * (size_t)nst;
+  const size_t b_z = 16 * lmv * lmv, b_f = 8 * lmv * lmv, b_r = 8 * lmv * c->hml_nrmtmax * d.natm;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_v = al(b_a), o_o = o_v + al(b_v), o_z = o_o + al(b_o), o_f = o_z + al(b_z), o_r = o_f + al(b_f);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_r + b_r))) return rc;
+  char *base = (char *)c->stage;
+  EXC_CHECK(hipMemcpyAsync(base, apwalm, b_a, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_v, evecfv, b_v, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_o, occ, b_o, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_z, zbshtvr, b_z, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_f, rfshtvr, b_f, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(base + o_r, rhomt, b_r, hipMemcpyHostToDevice, c->stream));
+  if ((rc = exc_rhovalk_dev(c, lradstep, (const double *)(base + o_z), (const double *)(base + o_f), ngp, ngkmax,
+                            (const double *)base, nst, ldv, (const double *)(base + o_v),
+                            (const double *)(base + o_o), wkpt, epsocc, (double *)(base + o_r))))
+    return rc;
+  EXC_CHECK(hipMemcpyAsync(rhomt, base + o_r, b_r, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return EXC_OK;
+}
+
 namespace {
 // the FFT of the G box (every grid point one G) and its work arrays, made once per context
 int scf_prepare(exc_ctx *c) {

@@ -91,6 +91,8 @@ struct exc_ctx {
   size_t eg_bytes;
   void *dmw;                // density matrix: A^T C, packed and weighted wavefunction blocks
   size_t dmw_bytes;
+  void *rvw;                // rhovalk: A^T C of one k-point
+  size_t rvw_bytes;
   // muffin-tin integrals (exc_ctx_hmlint): host copies of nlorb / lorbl, the Gaunt table of
   // gnt_lmaxvr (device), radial inputs + integrals (device)
   std::vector<int> h_nlorb, h_lorbl;

@@ -260,6 +260,25 @@ int exc_gendmatmt_batched_dev(exc_ctx *ctx, int nk, const exc_kpoint_dev *ks, in
  * stream; without: host arrays. */
 int exc_genrhomt_dev(exc_ctx *ctx, const double *dm, double *rhomt);
 int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
+/* The muffin-tin density of one k-point by the direct route, the counterpart of
+ * rhovalk(ik, evecfv, evecsv) (rhovalk.f90:1, called per k when useDensityMatrix is false,
+ * scf_cycle.f90:410; spinless, first-variational states, evecsv the identity): per atom, for
+ * every state j with |wkpt occ(j)| > epsocc, the wavefunction in complex harmonics on every
+ * lradstep-th mesh point (wavefmt.f90:86-124, l <= lmaxvr), transformed to the lmmaxvr points
+ * of the spherical covering by zbshtvr, squared and weighted, and transformed back by rfshtvr:
+ *   rhomt(:, ir, a) += rfshtvr sum_j wkpt occ(j) |zbshtvr wfmt_j(:, ir)|^2,
+ * ir = 1, 1 + lradstep, ... <= nrmt (rhovalk.f90:99-207; the other mesh points are not
+ * touched).  Radial functions, meshes and lmaxvr (<= lmaxapw, <= 8) of the last
+ * exc_ctx_hmlint call; zbshtvr(lmmaxvr, lmmaxvr) complex and rfshtvr(lmmaxvr, lmmaxvr) real
+ * as genshtmat makes them (genshtmat.f90:78-108); apwalm as exc_match returns it, evecfv(ldv,
+ * nst), occ(nst); rhomt((lmaxvr+1)^2, nrmtmax, natmtot) accumulated.  _dev: device arrays on
+ * the context's stream; without: host arrays. */
+int exc_rhovalk_dev(exc_ctx *ctx, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp,
+                    int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
+                    const double *occ, double wkpt, double epsocc, double *rhomt);
+int exc_rhovalk(exc_ctx *ctx, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp,
+                int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
+                const double *occ, double wkpt, double epsocc, double *rhomt);
 /* The interstitial density of one k-point, the counterpart of genrhoir(ik, evecfv, evecsv)
  * (genrhoir.f90:13, called beside gendmatmt at scf_cycle.f90:352; spinless, first-variational
  * states): for every state j with |wkpt occ(j)| > epsocc, psi_j(r) = the backward FFT (unscaled)
