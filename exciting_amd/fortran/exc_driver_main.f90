@@ -3,7 +3,8 @@
 ! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
 ! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot), then
 ! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot) and genrhoir_gpu's
-! rhoir(ngrtot) (and, with radial data, genrhomt_gpu's rhomt(lmmaxvr, nrmtmax, natmtot)); with a fourth
+! rhoir(ngrtot) (and, with radial data, genrhomt_gpu's rhomt(lmmaxvr, nrmtmax, natmtot), then, if
+! lmaxvr <= lmaxapw, rhovalk_gpu's rhomt of the same states by the direct route); with a fourth
 ! argument nev > 0 it then solves the eigenproblem (solvewithgpu, as seceqnfv.f90 calls
 ! solvewithlapack) and appends evalfv(nev) and evecfv(nmat, nev).  With a fifth file of radial
 ! data it first runs set_radial_gpu (match's boundary values from apwfr) and hmlint_gpu (hmlint +
@@ -33,7 +34,8 @@ program exc_driver_main
   real(8) :: bvec(3, 3)
   real(8), allocatable :: veffir(:), cfunir(:)
   integer(c_int), allocatable :: nrmt(:)
-  real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:), rhomt(:), rhoir(:)
+  real(8), allocatable :: spr(:, :), apwfr(:), lofr(:), veffmt(:), rhomt(:), rhoir(:), rfsht(:, :)
+  complex(8), allocatable :: zbsht(:, :)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -72,14 +74,16 @@ program exc_driver_main
   frad = "-"
   if (command_argument_count() >= 5) call get_command_argument(5, frad)
   if (trim(frad) /= "-") then
-    ! radial data file: lmaxvr, nrmtmax, nrmt(nspecies), spr, apwfr, lofr, veffmt
+    ! radial data file: lmaxvr, nrmtmax, nrmt(nspecies), spr, apwfr, lofr, veffmt, and the
+    ! spherical-harmonic transforms zbshtvr, rfshtvr (lmmaxvr x lmmaxvr) of genshtmat
     open(newunit=u, file=trim(frad), access="stream", form="unformatted", status="old")
     read(u) lmaxvr, nrmtmax
     allocate(nrmt(dims%nspecies), spr(nrmtmax, dims%nspecies))
     allocate(apwfr(nrmtmax * 2 * dims%apwordmax * (dims%lmaxapw + 1) * dims%natmtot))
     allocate(lofr(nrmtmax * 2 * max(dims%nlomax, 1) * dims%natmtot))
     allocate(veffmt((lmaxvr + 1)**2 * nrmtmax * dims%natmtot))
-    read(u) nrmt, spr, apwfr, lofr, veffmt
+    allocate(zbsht((lmaxvr + 1)**2, (lmaxvr + 1)**2), rfsht((lmaxvr + 1)**2, (lmaxvr + 1)**2))
+    read(u) nrmt, spr, apwfr, lofr, veffmt, zbsht, rfsht
     close(u)
     call set_radial_gpu(nrmt, spr, apwfr)
     call hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
@@ -131,6 +135,12 @@ program exc_driver_main
       allocate(rhomt((lmaxvr + 1)**2 * nrmtmax * dims%natmtot))
       call genrhomt_gpu(dm, rhomt)
       write(u) rhomt
+      if (lmaxvr <= dims%lmaxapw) then
+        ! and by the direct route (rhovalk, lradstep 1), where wavefmt allows lmaxvr
+        rhomt = 0.d0
+        call rhovalk_gpu(1, zbsht, rfsht, ngp, apwalm, nmat, nst, evec, occ, wkpt, 1.d-8, rhomt)
+        write(u) rhomt
+      end if
     end if
   end if
   if (nev_solve > 0) then

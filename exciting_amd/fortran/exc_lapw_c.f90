@@ -157,6 +157,17 @@ module exc_lapw_c
       real(c_double), intent(inout) :: rhomt(*)
     end function
 
+    integer(c_int) function exc_rhovalk(ctx, lradstep, zbshtvr, rfshtvr, ngp, ngkmax, apwalm, nst, ldv, evecfv, &
+        occ, wkpt, epsocc, rhomt) bind(c, name="exc_rhovalk")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lradstep, ngp, ngkmax, nst, ldv
+      complex(c_double_complex), intent(in) :: zbshtvr(*), apwalm(*), evecfv(*)
+      real(c_double), intent(in) :: rfshtvr(*), occ(*)
+      real(c_double), value :: wkpt, epsocc
+      real(c_double), intent(inout) :: rhomt(*)
+    end function
+
     integer(c_int) function exc_genrhoir(ctx, ngp, igpig, nst, ldv, evecfv, occ, wkpt, epsocc, rhoir) &
         bind(c, name="exc_genrhoir")
       import :: c_int, c_ptr, c_double, c_double_complex

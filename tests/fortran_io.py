@@ -71,7 +71,10 @@ def read_dmat(path, cr, ngkmax, ws):
 
 
 def write_radial(path, cr, rd):
-    """Radial data for the driver's hmlint_gpu step: lmaxvr, nrmtmax, nrmt, spr, apwfr, lofr, veffmt."""
+    """Radial data for the driver's hmlint_gpu step: lmaxvr, nrmtmax, nrmt, spr, apwfr, lofr, veffmt,
+    then the transforms zbshtvr, rfshtvr of the oracle's genshtmat (for rhovalk_gpu)."""
+    from oracle import rhovalk_ref as V
+    zb, _, rf = V.genshtmat(rd.lmaxvr)
     with open(path, "wb") as f:
         _w(f, [rd.lmaxvr, rd.spr.shape[0]], np.int32)
         _w(f, rd.nrmt, np.int32)
@@ -79,6 +82,8 @@ def write_radial(path, cr, rd):
         _w(f, rd.apwfr, np.float64)
         _w(f, rd.lofr, np.float64)
         _w(f, rd.veffmt, np.float64)
+        _w(f, zb, np.complex128)
+        _w(f, rf, np.float64)
 
 
 def write_scf(path, cr, veffir):
@@ -98,11 +103,12 @@ def read_rhoir(path, cr, ngkmax, ws):
         return np.fromfile(f, np.float64, cr.ngrtot)
 
 
-def read_rhomt(path, cr, ngkmax, ws, lmv, nrmtmax):
-    """rhomt(lmv, nrmtmax, natm) appended after dm and rhoir when the driver had radial data."""
+def read_rhomt(path, cr, ngkmax, ws, lmv, nrmtmax, direct=False):
+    """rhomt(lmv, nrmtmax, natm) appended after dm and rhoir when the driver had radial data
+    (genrhomt_gpu); with direct, the one after it (rhovalk_gpu)."""
     with open(path, "rb") as f:
         nmat = int(np.fromfile(f, np.int32, 1)[0])
         skip = ngkmax * cr.apwordmax * cr.lmmaxapw * cr.natm + 2 * nmat * nmat + ws * ws * cr.natm
         np.fromfile(f, np.complex128, skip)
-        np.fromfile(f, np.float64, 3 * cr.natm + cr.ngrtot)
+        np.fromfile(f, np.float64, 3 * cr.natm + cr.ngrtot + (lmv * nrmtmax * cr.natm if direct else 0))
         return np.fromfile(f, np.float64, lmv * nrmtmax * cr.natm).reshape((lmv, nrmtmax, cr.natm), order="F")

@@ -186,3 +186,34 @@ def test_fortran_gendmatmt_genrhomt(tmp_path):
     rel2 = np.abs(rir - rir_ref).max() / np.abs(rir_ref).max()
     print("Fortran genrhoir_gpu rel diff", rel2)
     assert rel2 <= 1e-12
+
+
+def test_fortran_rhovalk(tmp_path):
+    """rhovalk_gpu (Fortran) with lmaxvr 4 <= lmaxapw: the direct-route muffin-tin density of the
+    driver's states against the oracle's rhovalk on the apwalm the driver wrote."""
+    from exciting_amd import synth
+    from oracle import lapw_ref as R
+    from oracle import rhovalk_ref as V
+    w = small("d")
+    cr, kp = w.crystal, w.kpoints[0]
+    rd = synth.make_radial(cr, nr=(120, 100), lmaxvr=4)
+    nmat = kp.ngp + cr.nlotot
+    rng = np.random.default_rng(14)
+    nst = 6
+    C = rng.standard_normal((nmat, nst)) + 1j * rng.standard_normal((nmat, nst))
+    e, occ = np.sort(rng.standard_normal(nst)), rng.uniform(0.1, 2, nst)
+    fin, fout, feig, frad = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "eig.bin", tmp_path / "rad.bin"
+    write_input(fin, cr, kp)
+    write_eigen(feig, C, e, occ, 0.3)
+    write_radial(frad, cr, rd)
+    r = subprocess.run([EXE, str(fin), str(fout), str(feig), "0", str(frad)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    ngkmax, ws = kp.igpig.shape[0], R.dmat_wfsize(cr)
+    apw, _, _ = read_output(fout, cr, ngkmax)
+    rho = read_rhomt(fout, cr, ngkmax, ws, (rd.lmaxvr + 1) ** 2, rd.spr.shape[0], direct=True)
+    ref = V.rhovalk(cr, rd, apw, kp.ngp, C, occ, 0.3)
+    rel = np.abs(rho - ref).max() / np.abs(ref).max()
+    print("Fortran rhovalk_gpu rel diff", rel)
+    assert rel <= 1e-12
