@@ -306,7 +306,7 @@ def run_rhovalk(args, w, ctx, reps, log):
     rho = torch.zeros((cr.natm, nrm, lmv), dtype=torch.float64, device="cuda")
 
     def call():
-        ctx.rhovalk_dev(1, zb_d, rf_d, kp.ngp, apw.shape[0], a_d, nst, nmat, c_d, occ_d, 0.25, rho)
+        ctx.rhovalk_dev(1, zb_d, rf_d, kp.ngp, apw.shape[0], a_d, nst, nmat, c_d, occ_d, 0.25, rho)   # evecsv: the identity
 
     call()
     torch.cuda.synchronize()

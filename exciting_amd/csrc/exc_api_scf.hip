@@ -190,8 +190,8 @@ int exc_genrhomt(exc_ctx *c, const double *dm, double *rhomt) {
 }
 
 int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp, int ngkmax,
-                    const double *apwalm, int nst, int ldv, const double *evecfv, const double *occ, double wkpt,
-                    double epsocc, double *rhomt) {
+                    const double *apwalm, int nst, int ldv, const double *evecfv, const double *evecsv,
+                    const double *occ, double wkpt, double epsocc, double *rhomt) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (c->hml_lmaxvr < 0) return EXC_ERR_STATE;   // the radial functions of exc_ctx_hmlint
   const exc_dev &d = c->d;
@@ -205,8 +205,9 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
   if (!zbshtvr || !rfshtvr || !apwalm || !evecfv || !occ || !rhomt) return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
-  const int ldw = d.natm * d.naa;
-  if ((rc = grow(&c->rvw, &c->rvw_bytes, sizeof(cplx) * (size_t)ldw * nst))) return rc;
+  const int ldw = d.natm * d.naa, nmat = ngp + d.nlotot;
+  const size_t o_c = (sizeof(cplx) * (size_t)ldw * nst + 255) & ~(size_t)255;
+  if ((rc = grow(&c->rvw, &c->rvw_bytes, o_c + (evecsv ? sizeof(cplx) * (size_t)nmat * nst : 0)))) return rc;
   exc_ctx::EigWorker &W = c->ew[0];
   W.s = c->stream;
   if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
@@ -219,6 +220,15 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
   const int *di = (const int *)(b + H.o_i);
   const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
   StageTimer t(c, EXC_STAGE_SCF);
+  if (evecsv) {   // the states evecfv evecsv (wavefmt is linear in the coefficients)
+    cplx *c2 = (cplx *)((char *)c->rvw + o_c);
+    if (rocblas_zgemm(W.blas, rocblas_operation_none, rocblas_operation_none, nmat, nst, nst, &one,
+                      (const rocblas_double_complex *)evecfv, ldv, (const rocblas_double_complex *)evecsv, nst,
+                      &zero, (rocblas_double_complex *)c2, nmat) != rocblas_status_success)
+      return EXC_ERR_HIP;
+    evecfv = (const double *)c2;
+    ldv = nmat;
+  }
   // W = A^T C for all atoms: (natm naa) x nst, row a naa + io + apwordmax lm (the zdotu of
   // wavefmt.f90:95 for every (atom, io, lm) and state at once)
   if (ngp > 0) {
@@ -236,8 +246,8 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
 }
 
 int exc_rhovalk(exc_ctx *c, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp, int ngkmax,
-                const double *apwalm, int nst, int ldv, const double *evecfv, const double *occ, double wkpt,
-                double epsocc, double *rhomt) {
+                const double *apwalm, int nst, int ldv, const double *evecfv, const double *evecsv, const double *occ,
+                double wkpt, double epsocc, double *rhomt) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (c->hml_lmaxvr < 0) return EXC_ERR_STATE;
   if (nst < 0 || ldv < 1 || ngkmax < 1) return EXC_ERR_ARG;
@@ -249,9 +259,11 @@ int exc_rhovalk(exc_ctx *c, int lradstep, const double *zbshtvr, const double *r
   const size_t lmv = (size_t)(c->hml_lmaxvr + 1) * (c->hml_lmaxvr + 1);
   const size_t b_a = 16 * (size_t)ngkmax * d.naa * d.natm, b_v = 16 * (size_t)ldv * nst, b_o = 8 * (size_t)nst;
   const size_t b_z = 16 * lmv * lmv, b_f = 8 * lmv * lmv, b_r = 8 * lmv * c->hml_nrmtmax * d.natm;
+  const size_t b_s = evecsv ? 16 * (size_t)nst * nst : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_v = al(b_a), o_o = o_v + al(b_v), o_z = o_o + al(b_o), o_f = o_z + al(b_z), o_r = o_f + al(b_f);
-  if ((rc = grow(&c->stage, &c->stage_bytes, o_r + b_r))) return rc;
+  const size_t o_s = o_r + al(b_r);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_s + b_s))) return rc;
   char *base = (char *)c->stage;
   EXC_CHECK(hipMemcpyAsync(base, apwalm, b_a, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_v, evecfv, b_v, hipMemcpyHostToDevice, c->stream));
@@ -259,9 +271,11 @@ int exc_rhovalk(exc_ctx *c, int lradstep, const double *zbshtvr, const double *r
   EXC_CHECK(hipMemcpyAsync(base + o_z, zbshtvr, b_z, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_f, rfshtvr, b_f, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_r, rhomt, b_r, hipMemcpyHostToDevice, c->stream));
+  if (evecsv) EXC_CHECK(hipMemcpyAsync(base + o_s, evecsv, b_s, hipMemcpyHostToDevice, c->stream));
   if ((rc = exc_rhovalk_dev(c, lradstep, (const double *)(base + o_z), (const double *)(base + o_f), ngp, ngkmax,
                             (const double *)base, nst, ldv, (const double *)(base + o_v),
-                            (const double *)(base + o_o), wkpt, epsocc, (double *)(base + o_r))))
+                            evecsv ? (const double *)(base + o_s) : nullptr, (const double *)(base + o_o), wkpt,
+                            epsocc, (double *)(base + o_r))))
     return rc;
   EXC_CHECK(hipMemcpyAsync(rhomt, base + o_r, b_r, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));

@@ -158,9 +158,9 @@ module exc_lapw_c
     end function
 
     integer(c_int) function exc_rhovalk(ctx, lradstep, zbshtvr, rfshtvr, ngp, ngkmax, apwalm, nst, ldv, evecfv, &
-        occ, wkpt, epsocc, rhomt) bind(c, name="exc_rhovalk")
+        evecsv, occ, wkpt, epsocc, rhomt) bind(c, name="exc_rhovalk")
       import :: c_int, c_ptr, c_double, c_double_complex
-      type(c_ptr), value :: ctx
+      type(c_ptr), value :: ctx, evecsv
       integer(c_int), value :: lradstep, ngp, ngkmax, nst, ldv
       complex(c_double_complex), intent(in) :: zbshtvr(*), apwalm(*), evecfv(*)
       real(c_double), intent(in) :: rfshtvr(*), occ(*)

@@ -14,7 +14,7 @@
 !                                                                 genrhoir.f90:13
 !   genveffig_gpu(veffir, veffig)                                 genveffig.f90:13
 !   rhovalk_gpu(lradstep, zbshtvr, rfshtvr, ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt,
-!               epsocc, rhomt)                                    rhovalk.f90:1
+!               epsocc, rhomt [, evecsv])                         rhovalk.f90:1
 !
 ! The module state the reference routines read implicitly (mod_atoms, mod_APW_LO,
 ! mod_muffin_tin, mod_eigensystem's mt_hscf, mod_Gvector, ...) is handed over once per SCF
@@ -257,17 +257,23 @@ contains
   end subroutine genrhoir_gpu
 
   ! rhovalk.f90:1 (rhovalk(ik, evecfv, evecsv), useDensityMatrix false, scf_cycle.f90:410;
-  ! spinless first-variational states) with the module state it reads passed in
-  ! (input%groundstate%lradstep, zbshtvr, rfshtvr of genshtmat, occsv(:, ik), wkpt(ik),
-  ! epsocc): adds the k-point's muffin-tin density to rhomt(lmmaxvr, nrmtmax, natmtot) on the
-  ! radial functions and lmaxvr of the last hmlint_gpu call.
-  subroutine rhovalk_gpu(lradstep, zbshtvr, rfshtvr, ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, epsocc, rhomt)
+  ! spinless) with the module state it reads passed in (input%groundstate%lradstep, zbshtvr,
+  ! rfshtvr of genshtmat, occsv(:, ik), wkpt(ik), epsocc): adds the k-point's muffin-tin
+  ! density to rhomt(lmmaxvr, nrmtmax, natmtot) on the radial functions and lmaxvr of the last
+  ! hmlint_gpu call; the states are evecfv evecsv, evecsv(nstfv, nstfv) the identity when absent.
+  subroutine rhovalk_gpu(lradstep, zbshtvr, rfshtvr, ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, epsocc, rhomt, &
+                         evecsv)
     integer, intent(in) :: lradstep, ngp, nmatmax, nstfv
     complex(8), intent(in) :: zbshtvr(:, :), apwalm(:, :, :, :), evecfv(nmatmax, nstfv)
     real(8), intent(in) :: rfshtvr(:, :), occ(nstfv), wkpt, epsocc
     real(8), intent(inout) :: rhomt(*)
+    complex(8), intent(in), target, optional :: evecsv(nstfv, nstfv)
+    type(c_ptr) :: psv
+    psv = c_null_ptr
+    if (present(evecsv)) psv = c_loc(evecsv)
     call check(exc_rhovalk(ctx, int(lradstep, c_int), zbshtvr, rfshtvr, int(ngp, c_int), int(size(apwalm, 1), c_int), &
-                           apwalm, int(nstfv, c_int), int(nmatmax, c_int), evecfv, occ, wkpt, epsocc, rhomt), "rhovalk")
+                           apwalm, int(nstfv, c_int), int(nmatmax, c_int), evecfv, psv, occ, wkpt, epsocc, rhomt), &
+               "rhovalk")
   end subroutine rhovalk_gpu
 
 end module exc_setup_gpu

@@ -87,8 +87,8 @@ SIGNATURES = {
     "exc_ctx_gencfun": [_P, _P, _P, _P],
     "exc_genrhomt_dev": [_P, _P, _P],
     "exc_genrhomt": [_P, _P, _P],
-    "exc_rhovalk_dev": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
-    "exc_rhovalk": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
+    "exc_rhovalk_dev": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, C.c_double, _P],
+    "exc_rhovalk": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, C.c_double, _P],
     "exc_genrhoir_dev": [_P, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
     "exc_genrhoir_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, C.c_double, _P],
     "exc_genrhoir": [_P, _I, _P, _I, _I, _P, _P, C.c_double, C.c_double, _P],
@@ -298,15 +298,16 @@ class Context:
         return out
 
     def rhovalk(self, apwalm, ngp, evec, occ, wkpt, zbshtvr, rfshtvr, lmaxvr, nrmtmax, lradstep=1, epsocc=1e-8,
-                rhomt=None):
+                rhomt=None, evecsv=None):
         """exc_rhovalk from host arrays (the radial data of the last hmlint call): apwalm
         (ngkmax, apwordmax, lmmaxapw, natm) and evec (ldv, nst) Fortran complex128, occ (nst,),
         zbshtvr / rfshtvr (lmmaxvr, lmmaxvr) Fortran complex128 / float64; rhomt (lmmaxvr,
-        nrmtmax, natm) Fortran float64 accumulated (new when None)."""
+        nrmtmax, natm) Fortran float64 accumulated (new when None); evecsv (nst, nst) Fortran
+        complex128 or None for the identity."""
         lmv = (lmaxvr + 1) ** 2
         if rhomt is None:
             rhomt = np.zeros((lmv, nrmtmax, self.cr.natm), order="F")
-        for a in (apwalm, evec, zbshtvr):
+        for a in (apwalm, evec, zbshtvr) + ((evecsv,) if evecsv is not None else ()):
             if a.dtype != np.complex128 or not a.flags.f_contiguous:
                 raise ValueError("rhovalk: complex arrays must be Fortran-ordered complex128")
         if rhomt.shape != (lmv, nrmtmax, self.cr.natm) or not rhomt.flags.f_contiguous:
@@ -314,17 +315,18 @@ class Context:
         rf = np.asfortranarray(rfshtvr, dtype=np.float64)
         occ = np.ascontiguousarray(occ, dtype=np.float64)
         self._call("exc_rhovalk", self.h, int(lradstep), _ptr(zbshtvr), _ptr(rf), int(ngp), apwalm.shape[0],
-                   _ptr(apwalm), evec.shape[1], evec.shape[0], _ptr(evec), _ptr(occ), float(wkpt), float(epsocc),
-                   _ptr(rhomt))
+                   _ptr(apwalm), evec.shape[1], evec.shape[0], _ptr(evec),
+                   _ptr(evecsv) if evecsv is not None else None, _ptr(occ), float(wkpt), float(epsocc), _ptr(rhomt))
         return rhomt
 
     def rhovalk_dev(self, lradstep, zbshtvr, rfshtvr, ngp, ngkmax, apwalm, nst, ldv, evec, occ, wkpt, rhomt,
-                    epsocc=1e-8):
+                    epsocc=1e-8, evecsv=None):
         """Device version (torch tensors; rhomt (natm, nrmtmax, lmmaxvr) row-major = Fortran
         (lmmaxvr, nrmtmax, natm); zbshtvr / rfshtvr transposed likewise)."""
         return self._call("exc_rhovalk_dev", self.h, int(lradstep), _ptr(zbshtvr), _ptr(rfshtvr), int(ngp),
-                          int(ngkmax), _ptr(apwalm), int(nst), int(ldv), _ptr(evec), _ptr(occ), float(wkpt),
-                          float(epsocc), _ptr(rhomt))
+                          int(ngkmax), _ptr(apwalm), int(nst), int(ldv), _ptr(evec),
+                          _ptr(evecsv) if evecsv is not None else None, _ptr(occ), float(wkpt), float(epsocc),
+                          _ptr(rhomt))
 
     def genrhoir(self, kp, evec, occ, wkpt, epsocc=1e-8, rhoir=None):
         """exc_genrhoir from host arrays: evec (ldv, nst) Fortran complex128; rhoir (ngrtot,)

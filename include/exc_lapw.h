@@ -262,7 +262,9 @@ int exc_genrhomt_dev(exc_ctx *ctx, const double *dm, double *rhomt);
 int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
 /* The muffin-tin density of one k-point by the direct route, the counterpart of
  * rhovalk(ik, evecfv, evecsv) (rhovalk.f90:1, called per k when useDensityMatrix is false,
- * scf_cycle.f90:410; spinless, first-variational states, evecsv the identity): per atom, for
+ * scf_cycle.f90:410; spinless): the states are evecfv evecsv (evecsv(nst, nst), or NULL for the
+ * identity: first-variational states, tevecsv false; the reference also leaves out entries of
+ * evecsv with |Re| + |Im| <= epsocc, rhovalk.f90:115-117, which are kept here).  Per atom, for
  * every state j with |wkpt occ(j)| > epsocc, the wavefunction in complex harmonics on every
  * lradstep-th mesh point (wavefmt.f90:86-124, l <= lmaxvr), transformed to the lmmaxvr points
  * of the spherical covering by zbshtvr, squared and weighted, and transformed back by rfshtvr:
@@ -275,10 +277,11 @@ int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
  * the context's stream; without: host arrays. */
 int exc_rhovalk_dev(exc_ctx *ctx, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp,
                     int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
-                    const double *occ, double wkpt, double epsocc, double *rhomt);
+                    const double *evecsv, const double *occ, double wkpt, double epsocc,
+                    double *rhomt);
 int exc_rhovalk(exc_ctx *ctx, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp,
                 int ngkmax, const double *apwalm, int nst, int ldv, const double *evecfv,
-                const double *occ, double wkpt, double epsocc, double *rhomt);
+                const double *evecsv, const double *occ, double wkpt, double epsocc, double *rhomt);
 /* The interstitial density of one k-point, the counterpart of genrhoir(ik, evecfv, evecsv)
  * (genrhoir.f90:13, called beside gendmatmt at scf_cycle.f90:352; spinless, first-variational
  * states): for every state j with |wkpt occ(j)| > epsocc, psi_j(r) = the backward FFT (unscaled)

@@ -2,7 +2,8 @@
 only tests/, __graft_entry__.smoke and bench.py's cpu_baseline leg use it; nothing in the
 library calls it).
 
-The spinless, first-variational part (tevecsv false, spin unassociated):
+The spinless part (spin unassociated; first-variational states, or with evecsv the states of
+tevecsv true with one spinor):
 
 * sphcover (src_lib/sphcover.f90:49-60): n points on the unit sphere, (theta, phi) per point;
 * genshtmat (genshtmat.f90:78-108): zbshtvr(itp, lm) = Y_lm, rbshtvr(itp, lm) = R_lm at the
@@ -12,7 +13,10 @@ The spinless, first-variational part (tevecsv false, spin unassociated):
       wfmt(lm, irc) = sum_io (sum_G evec(G) apwalm(G, io, lm)) apwfr(ir, io, l)
                     + sum_ilo evec(ngp + idxlo(lm, ilo)) lofr(ir, ilo)      (l <= lmaxvr);
 * rhovalk (rhovalk.f90:99-207): per atom, for every state with |wkpt occ| > epsocc,
-  rfmt(itp, irc) += wkpt occ |(zbshtvr wfmt)(itp, irc)|^2, then
+  rfmt(itp, irc) += wkpt occ |(zbshtvr wfmt)(itp, irc)|^2 (with evecsv, tevecsv true and one
+  spinor: wfmt of state j = sum over first-variational states ist with
+  |Re evecsv(ist, j)| + |Im evecsv(ist, j)| > epsocc of evecsv(ist, j) wfmt_ist,
+  rhovalk.f90:108-136), then
   rhomt(:, ir, a) += rfshtvr rfmt(:, irc) at ir = 1, 1 + lradstep, ... (the other points of
   the mesh are not touched).
 
@@ -91,10 +95,10 @@ def wavefmt(cr, rd, apwalm, ngp, evec, ias, lmaxvr, lradstep):
     return wf
 
 
-def rhovalk(cr, rd, apwalm, ngp, evec, occ, wkpt, lradstep=1, epsocc=1e-8, sht=None, rhomt=None):
+def rhovalk(cr, rd, apwalm, ngp, evec, occ, wkpt, lradstep=1, epsocc=1e-8, sht=None, rhomt=None, evecsv=None):
     """rhomt(lmmaxvr, nrmtmax, natm) += the k-point's muffin-tin density (rhovalk.f90:99-207,
-    spinless, evecsv the identity).  evec: (>= nmat, nst); occ: (nst,); sht: genshtmat's
-    (zbshtvr, rbshtvr, rfshtvr) or None to make them."""
+    spinless).  evec: (>= nmat, nst); occ: (nst,); evecsv: (nst, nst) or None for the
+    identity; sht: genshtmat's (zbshtvr, rbshtvr, rfshtvr) or None to make them."""
     lmaxvr = rd.lmaxvr
     lmv = (lmaxvr + 1) ** 2
     zb, _, rf = sht if sht is not None else genshtmat(lmaxvr)
@@ -108,7 +112,14 @@ def rhovalk(cr, rd, apwalm, ngp, evec, occ, wkpt, lradstep=1, epsocc=1e-8, sht=N
             t1 = wkpt * occ[j]
             if abs(t1) <= epsocc:
                 continue
-            w3 = zb @ wavefmt(cr, rd, apwalm, ngp, evec[:, j], ias, lmaxvr, lradstep)
+            if evecsv is None:
+                w3 = zb @ wavefmt(cr, rd, apwalm, ngp, evec[:, j], ias, lmaxvr, lradstep)
+            else:
+                w3 = 0
+                for ist in range(evec.shape[1]):
+                    z = evecsv[ist, j]
+                    if abs(z.real) + abs(z.imag) > epsocc:
+                        w3 = w3 + z * (zb @ wavefmt(cr, rd, apwalm, ngp, evec[:, ist], ias, lmaxvr, lradstep))
             rfmt += t1 * (w3.real ** 2 + w3.imag ** 2)
         rhomt[:, irs, ias] += rf @ rfmt
     return rhomt

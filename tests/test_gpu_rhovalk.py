@@ -88,3 +88,25 @@ def test_rhovalk_dev_equals_host_entry():
     torch.cuda.synchronize()
     dev = rho.cpu().numpy().T
     assert np.array_equal(dev, host)
+
+
+def test_rhovalk_with_evecsv():
+    """The states evecfv evecsv (tevecsv true, one spinor) against the oracle's sum of the
+    first-variational wavefunctions weighted by evecsv (no entry below epsocc in a random
+    unitary evecsv, so the reference's skip leaves nothing out)."""
+    from exciting_amd import lapw
+    from oracle import rhovalk_ref as V
+    cr, kp, rd, apw, C, occ = _case("small-d", 4, 9, 31)
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9))
+    U = np.asfortranarray(np.linalg.qr(X)[0])
+    assert (np.abs(U.real) + np.abs(U.imag)).min() > 1e-8
+    ctx = lapw.Context(cr)
+    ctx.hmlint(rd, out=False)
+    zb, _, rf = V.genshtmat(4)
+    nrmtmax = rd.spr.shape[0]
+    got = ctx.rhovalk(np.asfortranarray(apw), kp.ngp, C, occ, 0.35, zb, rf, 4, nrmtmax, evecsv=U)
+    ref = V.rhovalk(cr, rd, apw, kp.ngp, C, occ, 0.35, sht=(zb, None, rf), evecsv=U)
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"rhovalk with evecsv: max rel diff {e:.2e}", flush=True)
+    assert e <= 1e-12

@@ -115,3 +115,20 @@ def test_rhovalk_radial_step_and_epsocc():
     occ2 = occ.copy()
     occ2[2] = 0.0
     assert np.array_equal(V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ2, 0.5), r1)
+
+
+def test_rhovalk_second_variational_states():
+    """With evecsv: the states evecfv evecsv (wavefmt is linear), and a unitary evecsv with equal
+    occupations leaves the density unchanged (the trace of a density is basis-free)."""
+    cr, kp, rd, apw, Cv, occ = _band_limited_case(7)
+    rng = np.random.default_rng(2)
+    nst = Cv.shape[1]
+    X = rng.standard_normal((nst, nst)) + 1j * rng.standard_normal((nst, nst))
+    U, _ = np.linalg.qr(X)
+    r_sv = V.rhovalk(cr, rd, apw, kp.ngp, Cv, occ, 0.5, evecsv=U)
+    r_mul = V.rhovalk(cr, rd, apw, kp.ngp, Cv @ U, occ, 0.5)
+    assert np.abs(r_sv - r_mul).max() <= 1e-13 * np.abs(r_mul).max()
+    eq = np.full(nst, 1.3)
+    r_u = V.rhovalk(cr, rd, apw, kp.ngp, Cv, eq, 0.5, evecsv=U)
+    r_1 = V.rhovalk(cr, rd, apw, kp.ngp, Cv, eq, 0.5)
+    assert np.abs(r_u - r_1).max() <= 1e-12 * np.abs(r_1).max()
