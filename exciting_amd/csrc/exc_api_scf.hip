@@ -206,8 +206,10 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
   int rc = use_device(c);
   if (rc) return rc;
   const int ldw = d.natm * d.naa, nmat = ngp + d.nlotot;
-  const size_t o_c = (sizeof(cplx) * (size_t)ldw * nst + 255) & ~(size_t)255;
-  if ((rc = grow(&c->rvw, &c->rvw_bytes, o_c + (evecsv ? sizeof(cplx) * (size_t)nmat * nst : 0)))) return rc;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_c = al(sizeof(cplx) * (size_t)ldw * nst);   // W, then evecfv evecsv, then the kernels' work
+  const size_t o_k = o_c + (evecsv ? al(sizeof(cplx) * (size_t)nmat * nst) : 0);
+  if ((rc = grow(&c->rvw, &c->rvw_bytes, o_k + exc_rhovalk_work_bytes(d, lmaxvr, c->dims.nlomax, nst)))) return rc;
   exc_ctx::EigWorker &W = c->ew[0];
   W.s = c->stream;
   if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
@@ -229,12 +231,16 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
     evecfv = (const double *)c2;
     ldv = nmat;
   }
-  // W = A^T C for all atoms: (natm naa) x nst, row a naa + io + apwordmax lm (the zdotu of
-  // wavefmt.f90:95 for every (atom, io, lm) and state at once)
+  // W = A^T C (the zdotu of wavefmt.f90:95 for every (atom, io, lm <= lmaxvr) and state at
+  // once): per atom the first apwordmax lmmaxvr columns of its apwalm block, one strided batch
+  // over the atoms; row a naa + io + apwordmax lm of W (leading dimension natm naa)
   if (ngp > 0) {
-    if (rocblas_zgemm(W.blas, rocblas_operation_transpose, rocblas_operation_none, ldw, nst, ngp, &one,
-                      (const rocblas_double_complex *)apwalm, ngkmax, (const rocblas_double_complex *)evecfv, ldv,
-                      &zero, (rocblas_double_complex *)c->rvw, ldw) != rocblas_status_success)
+    const int mrows = d.apwordmax * (lmaxvr + 1) * (lmaxvr + 1);
+    if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_transpose, rocblas_operation_none, mrows, nst, ngp,
+                                      &one, (const rocblas_double_complex *)apwalm, ngkmax,
+                                      (rocblas_stride)ngkmax * d.naa, (const rocblas_double_complex *)evecfv, ldv, 0,
+                                      &zero, (rocblas_double_complex *)c->rvw, ldw, (rocblas_stride)d.naa,
+                                      d.natm) != rocblas_status_success)
       return EXC_ERR_HIP;
   } else {
     EXC_CHECK(hipMemsetAsync(c->rvw, 0, sizeof(cplx) * (size_t)ldw * nst, c->stream));
@@ -242,7 +248,7 @@ int exc_rhovalk_dev(exc_ctx *c, int lradstep, const double *zbshtvr, const doubl
   return exc_launch_rhovalk(d, lmaxvr, lradstep, nrmtmax, c->dims.nlomax, di, di + ns, di + 2 * ns,
                             (const double *)(b + H.o_apw), (const double *)(b + H.o_lo), (const cplx *)c->rvw, ldw,
                             ngp, (const cplx *)evecfv, ldv, nst, occ, wkpt, epsocc, (const cplx *)zbshtvr, rfshtvr,
-                            rhomt, c->stream);
+                            (char *)c->rvw + o_k, rhomt, c->stream);
 }
 
 int exc_rhovalk(exc_ctx *c, int lradstep, const double *zbshtvr, const double *rfshtvr, int ngp, int ngkmax,

@@ -150,10 +150,11 @@ int exc_launch_eig_sort_r2c(const double *zr, int m, int nev, int ldr, const int
                             int ldx, double *eval, hipStream_t s);
 int exc_launch_eig_copyout(const eig_kdesc *ks, int nb, int n, int nev, const cplx *x, hipStream_t s);
 // direct-route muffin-tin density (k_rhovalk.hip)
+size_t exc_rhovalk_work_bytes(const exc_dev &d, int lmaxvr, int nlomax, int nst);
 int exc_launch_rhovalk(const exc_dev &d, int lmaxvr, int lradstep, int nrmtmax, int nlomax, const int *nrmt,
                        const int *nlorb, const int *lorbl, const double *apwfr, const double *lofr, const cplx *w,
                        int ldw, int ngp, const cplx *evec, int ldv, int nst, const double *occ, double wk,
-                       double eps, const cplx *zb, const double *rf, double *rhomt, hipStream_t s);
+                       double eps, const cplx *zb, const double *rf, void *work, double *rhomt, hipStream_t s);
 // muffin-tin density matrix helpers (k_dmat.hip)
 int exc_launch_dmat_pack(const exc_dev &d, int ws, int nst, const cplx *w, const cplx *evec, int ldv, int ngp,
                          cplx *x, int ldx, size_t xs, int col0, hipStream_t s);
