@@ -1,5 +1,5 @@
-// Muffin-tin density of the direct route, rhovalk (rhovalk.f90:99-207, spinless, first-
-// variational states): per atom a and coarse radial point irc (mesh point ir = irc lradstep),
+// Muffin-tin density of the direct route, rhovalk (rhovalk.f90:99-207, spinless; C holds the
+// states' coefficients): per atom a and coarse radial point irc (mesh point ir = irc lradstep),
 //   psi_j(lm)  = sum_io W(a, io, lm; j) u_{io,l}(ir) + sum_ilo C(lo(a, ilo, m); j) v_ilo(ir)
 //                (wavefmt.f90:91-124; W = A^T C, the zdotu of apwalm with the coefficients),
 //   rfmt(itp)  = sum_j w occ_j |sum_lm zbshtvr(itp, lm) psi_j(lm)|^2   (|w occ_j| > epsocc),
