@@ -191,15 +191,31 @@ def _positions(rng, avec, rmt, spidx):
     return np.array(frac).T  # (3, natm) fractional
 
 
+def nfftifc(n):
+    """The next n' >= n whose prime factors are 2, 3 and 5 only (nfftifc.f90)."""
+    if n <= 0:
+        raise ValueError("nfftifc: n <= 0")
+    while True:
+        i = n
+        for p in (2, 3, 5):
+            while i % p == 0:
+                i //= p
+        if i == 1:
+            return n
+        n += 1
+
+
 def gvector_set(avec, bvec, gmaxvr):
-    """G vectors of an integer box, sorted by length (stable), following gengvec.f90:56-125.
+    """G vectors of an integer box, sorted by length (stable), following gengvec.f90:56-125, on
+    the box of gridsize.f90:31-52: ngrid_i = nfftifc(int(gmaxvr |a_i| / pi) + 1), intgv =
+    [ngrid/2 - ngrid + 1, ngrid/2] (the FFT grid; it holds the sphere |G| < gmaxvr).
 
     Returns intgv (3,2), ivg (3,ngrtot), ivgig (box, 1-based), vgc, gc, ngvec."""
-    nmax = [int(math.ceil(gmaxvr * np.linalg.norm(avec[:, i]) / (2.0 * math.pi))) + 1 for i in range(3)]
-    intgv = np.array([[-n, n] for n in nmax], dtype=np.int32)
-    r1 = np.arange(-nmax[0], nmax[0] + 1)
-    r2 = np.arange(-nmax[1], nmax[1] + 1)
-    r3 = np.arange(-nmax[2], nmax[2] + 1)
+    ngrid = [nfftifc(int(gmaxvr * np.linalg.norm(avec[:, i]) / math.pi) + 1) for i in range(3)]
+    intgv = np.array([[g // 2 - g + 1, g // 2] for g in ngrid], dtype=np.int32)
+    r1 = np.arange(intgv[0, 0], intgv[0, 1] + 1)
+    r2 = np.arange(intgv[1, 0], intgv[1, 1] + 1)
+    r3 = np.arange(intgv[2, 0], intgv[2, 1] + 1)
     # i1 outermost, i3 innermost as in the reference's triple loop
     i1, i2, i3 = np.meshgrid(r1, r2, r3, indexing="ij")
     iv = np.stack([i1.ravel(), i2.ravel(), i3.ravel()]).astype(np.int64)
@@ -209,9 +225,10 @@ def gvector_set(avec, bvec, gmaxvr):
     ivg = np.asfortranarray(iv[:, order].astype(np.int32))
     gc = g[order]
     vgc = np.asfortranarray(bvec @ ivg.astype(float))
-    shape = tuple(int(n) * 2 + 1 for n in nmax)
+    shape = tuple(int(g) for g in ngrid)
     ivgig = np.zeros(shape, dtype=np.int32, order="F")
-    ivgig[ivg[0] + nmax[0], ivg[1] + nmax[1], ivg[2] + nmax[2]] = np.arange(1, ivg.shape[1] + 1, dtype=np.int32)
+    ivgig[ivg[0] - intgv[0, 0], ivg[1] - intgv[1, 0], ivg[2] - intgv[2, 0]] = np.arange(1, ivg.shape[1] + 1,
+                                                                                       dtype=np.int32)
     below = np.nonzero(gc < gmaxvr)[0]
     ngvec = int(below[-1] + 1) if below.size else 1
     return intgv, ivg, ivgig, vgc, gc, ngvec
