@@ -270,7 +270,7 @@ int exc_genrhomt(exc_ctx *ctx, const double *dm, double *rhomt);
  * of the spherical covering by zbshtvr, squared and weighted, and transformed back by rfshtvr:
  *   rhomt(:, ir, a) += rfshtvr sum_j wkpt occ(j) |zbshtvr wfmt_j(:, ir)|^2,
  * ir = 1, 1 + lradstep, ... <= nrmt (rhovalk.f90:99-207; the other mesh points are not
- * touched).  Radial functions, meshes and lmaxvr (<= lmaxapw, <= 8) of the last
+ * touched).  Radial functions, meshes and lmaxvr (<= lmaxapw, <= 8; nlomax <= 8) of the last
  * exc_ctx_hmlint call; zbshtvr(lmmaxvr, lmmaxvr) complex and rfshtvr(lmmaxvr, lmmaxvr) real
  * as genshtmat makes them (genshtmat.f90:78-108); apwalm as exc_match returns it, evecfv(ldv,
  * nst), occ(nst); rhomt((lmaxvr+1)^2, nrmtmax, natmtot) accumulated.  _dev: device arrays on

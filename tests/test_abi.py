@@ -69,8 +69,14 @@ def test_argument_errors_without_gpu():
     assert lib.exc_match(None, 1, 1, None, None, None, None) == lapw.EXC_ERR_ARG
     assert lib.exc_hamiltonandoverlapsetup(None, 1, 1, None, None, None, 1, None, None) == lapw.EXC_ERR_ARG
     assert lib.exc_ctx_set_timing(None, 1) == lapw.EXC_ERR_ARG
+    assert lib.exc_rhovalk(None, 1, None, None, 1, 1, None, 1, 1, None, None, None, C.c_double(1.0),
+                           C.c_double(1e-8), None) == lapw.EXC_ERR_ARG
+    assert lib.exc_rhovalk_dev(None, 1, None, None, 1, 1, None, 1, 1, None, None, None, C.c_double(1.0),
+                               C.c_double(1e-8), None) == lapw.EXC_ERR_ARG
+    assert lib.exc_eigen_batched_dev(None, 1, None, None, None, None, 1, None, None, None) == lapw.EXC_ERR_ARG
     assert lib.exc_ctx_destroy(None) == 0
     assert lib.exc_ctx_num_variants(2) >= 2
+    assert lib.exc_ctx_num_variants(10) == 8          # eigensolve paths 0-7
 
 
 def test_fortran_driver_built():
