@@ -524,11 +524,17 @@ def main():
         mt_ms, mt_cnt = tim["mt"]
         fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints) * args.steps
     achieved = fl / (mt_ms * 1e-3) if mt_ms > 0 else 0.0
+    # the grouped default MT kernel (variant 19) forms each complex product from three real
+    # MFMA products (Gauss's form) instead of four: it executes 6 of the 8 algorithmic flops
+    bmt_v = (tuned or {}).get("bmt", 19)
+    gauss = args.path == "batched" and bmt_v == 19
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64, "traffic": None,
             "kernel": ("bmt_dma_grid_kernel (grouped, all k)" if args.path == "batched" else "mt_dma_kernel (per k)")
-                      + (": A^H.Y + interstitial (H)" if olpflag else ": A^H.Y and A^H.A + interstitial"),
-            "per_launch_flops": fl / max(mt_cnt, 1)}
+                      + (": A^H.Y + interstitial (H)" if olpflag else ": A^H.Y and A^H.A + interstitial")
+                      + (", 3 real MFMA products per complex product" if gauss else ""),
+            "per_launch_flops": fl / max(mt_cnt, 1),
+            "executed_mfma_frac": (0.75 if gauss else 1.0) * achieved / PEAK_FP64}
     # HBM traffic of the dominant kernel: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this same
     # default command, scaled by a 1 GiB copy (committed summary); only for the profiled setup
     prof = HERE / "profiles" / "setup_kernels_workload1_16k.json"

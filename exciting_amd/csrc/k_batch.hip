@@ -145,12 +145,12 @@ __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_kernel(const kdesc *__re
 // the same items on a plain grid, one item per workgroup: workgroup b runs on XCD b % 8 and
 // takes the next item of that XCD's contiguous eighth of the list (L2 locality as in the
 // persistent kernel), while the dispatcher balances the load dynamically
-template <int BM, int BK, int NST, int WR, int WC>
+template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_grid_kernel(const kdesc *__restrict__ ks,
                                                                     const mt_item *__restrict__ items, int nitems,
                                                                     int ktot, GTab gt,
                                                                     const int *__restrict__ kperm) {
-  using T = MtTileDMA<BM, BK, NST, WR, WC>;
+  using T = MtTileDMA<BM, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   __shared__ typename T::Tables tb;
   EXC_DMA_STAGES(St, NST)
@@ -164,10 +164,10 @@ __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_grid_kernel(const kdesc 
          k.ldh, kperm, w.ti, w.tj, w.which);
 }
 
-template <int BM, int BK, int NST, int WR, int WC>
+template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_bmt_dma_grid_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
   const int chunk = (nitems + 7) / 8;
-  hipLaunchKernelGGL((bmt_dma_grid_kernel<BM, BK, NST, WR, WC>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks,
+  hipLaunchKernelGGL((bmt_dma_grid_kernel<BM, BK, NST, WR, WC, G3>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks,
                      items, nitems, d.natm * d.naa, make_gtab(d), d.kperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
@@ -301,14 +301,14 @@ int exc_bmt_tile(int variant) {
     case 9: return 32;
     case 10: case 11: case 12: case 13: case 14: return 64;
     case 15: case 16: case 17: return 128;
-    case 18: return 64;
+    case 18: case 19: return 64;
     default: return 0;
   }
 }
 
 bool exc_bmt_fused(int variant) { return variant >= 7 && variant <= 9; }
 
-bool exc_bmt_persistent(int variant) { return variant != 18; }
+bool exc_bmt_persistent(int variant) { return variant != 18 && variant != 19; }
 
 int exc_bmt_max_blocks(int variant) {
   switch (variant) {
@@ -330,7 +330,7 @@ int exc_bmt_max_blocks(int variant) {
     case 15: return max_blocks_dma_t<128, 16, 2, 2, 4>();
     case 16: return max_blocks_dma_t<128, 8, 2, 2, 4>();
     case 17: return max_blocks_dma_t<128, 8, 3, 2, 4>();
-    case 18: return 1;   // plain grid: the block count is the item count
+    case 18: case 19: return 1;   // plain grid: the block count is the item count
     default: return 1;
   }
 }
@@ -358,6 +358,7 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 16: return launch_bmt_dma_t<128, 8, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 17: return launch_bmt_dma_t<128, 8, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 18: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4>(d, ks, items, nitems, s);
+    case 19: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4, true>(d, ks, items, nitems, s);   // Gauss's 3 products
     default: return EXC_ERR_ARG;
   }
 }

@@ -52,7 +52,9 @@ __device__ __forceinline__ void wait_vm_lgkm0() {
 
 // C(i, j) = sum_k op(A(i, k)) B(j, k) over a BM x BN tile, A(i, k) at A[i + kperm[k] lda],
 // B(j, k) at B[j + kperm[k] ldb], op = conj (CONJA) or identity; NST LDS stages filled by DMA.
-template <int BM, int BN, int BK, int NST, int WR, int WC, bool CONJA>
+// G3: three real products per complex product (Gauss's form, as zgemm3m) instead of four:
+// P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar -+ ai)(br + bi), combined after the K loop.
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool CONJA, bool G3 = false>
 struct DmaCore {
   static constexpr int kWaves = WR * WC, kThreads = 64 * kWaves;
   static constexpr int WM = BM / WR, WN = BN / WC, MB = WM / 16, NB = WN / 16;
@@ -159,6 +161,44 @@ struct DmaCore {
     }
   }
 
+  // Gauss's form: p1 += ar br, p2 += ai bi, p3 += (ar + s ai)(br + bi), s = -1 for CONJA
+  __device__ __forceinline__ static void compute3(const Stage *st, d4_t (&p1)[MB][NB], d4_t (&p2)[MB][NB],
+                                                  d4_t (&p3)[MB][NB], int wr, int wc, int kvalid) {
+    const int lane = threadIdx.x & 63;
+    const int lk = lane >> 4, li = lane & 15;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      if (kk >= kvalid) break;
+      double ar[MB], ai[MB], as[MB], br[NB], bi[NB], bs[NB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const dv2 v = *reinterpret_cast<const dv2 *>(&st->a[kk + lk][wr * WM + i * 16 + li]);
+        ar[i] = v.x;
+        ai[i] = v.y;
+        as[i] = CONJA ? v.x - v.y : v.x + v.y;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const dv2 v = *reinterpret_cast<const dv2 *>(&st->b[kk + lk][wc * WN + j * 16 + li]);
+        br[j] = v.x;
+        bi[j] = v.y;
+        bs[j] = v.x + v.y;
+      }
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], p1[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], p2[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], p3[i][j], 0, 0, 0);
+    }
+  }
+
   // the whole K loop.  On entry no DMA of this workgroup is in flight and every wave is done
   // with the stages; s2, s3 are unused below NST 3, 4.  A wave with active = false (its whole
   // sub-tile below the diagonal of a diagonal tile, never stored) issues its share of the DMA
@@ -173,10 +213,17 @@ struct DmaCore {
     for (int u = 0; u < NST - 1; ++u)
       if (u < nsteps) issue(st[u], A, lda, row0, mrows, B, ldb, col0, nrows, ktot, kperm, u * BK);
     const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+    d4_t c3[G3 ? MB : 1][G3 ? NB : 1];   // P3 of Gauss's form (cr, ci hold P1, P2 meanwhile)
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j) cr[i][j] = ci[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    if constexpr (G3) {
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) c3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
     for (int base = 0; base < nsteps; base += NST) {
 #pragma unroll
       for (int u = 0; u < NST; ++u) {
@@ -192,9 +239,24 @@ struct DmaCore {
           if (step + NST - 1 < nsteps)
             issue(st[(u + NST - 1) % NST], A, lda, row0, mrows, B, ldb, col0, nrows, ktot, kperm,
                   (step + NST - 1) * BK);
-          if (active) compute(st[u], cr, ci, wr, wc, ktot - step * BK);
+          if (active) {
+            if constexpr (G3)
+              compute3(st[u], cr, ci, c3, wr, wc, ktot - step * BK);
+            else
+              compute(st[u], cr, ci, wr, wc, ktot - step * BK);
+          }
         }
       }
+    }
+    if constexpr (G3) {   // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2;  a b: Re = P1 - P2, Im = P3 - P1 - P2
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const d4_t p1 = cr[i][j], p2 = ci[i][j];
+          cr[i][j] = CONJA ? p1 + p2 : p1 - p2;
+          ci[i][j] = CONJA ? (c3[i][j] - p1) + p2 : (c3[i][j] - p1) - p2;
+        }
     }
   }
 
@@ -204,10 +266,10 @@ struct DmaCore {
   __device__ __forceinline__ static int col_of(int wc, int j) { return wc * WN + j * 16 + (threadIdx.x & 15); }
 };
 
-template <int BM, int BK, int NST, int WR, int WC>
+template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 struct MtTileDMA {
   static constexpr int BN = BM;
-  using C = DmaCore<BM, BN, BK, NST, WR, WC, true>;
+  using C = DmaCore<BM, BN, BK, NST, WR, WC, true, G3>;
   using Stage = typename C::Stage;
   static constexpr int kThreads = C::kThreads;
   struct Tables {

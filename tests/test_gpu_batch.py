@@ -2,8 +2,10 @@
 persistent work-list kernel, optionally captured in a hipGraph.
 
 Its arithmetic is the per-k path's (same device code, same summation order), so H, O and
-apwalm must equal the per-k results bit for bit, for every tile variant; the per-k path is
-itself checked against the oracle in test_gpu_setup.py, and one k here is re-checked."""
+apwalm must equal the per-k results bit for bit, for every tile variant but the one with
+Gauss's three real products per complex product (grouped MT variant 19: other roundings,
+H and O within 1e-14 of the per-k results relative to their norm); the per-k path is itself
+checked against the oracle in test_gpu_setup.py, and one k here is re-checked."""
 import numpy as np
 import pytest
 
@@ -31,13 +33,22 @@ def _clear(batch):
         batch.O[i].zero_()
 
 
-def _same(batch, ref, what):
+GAUSS_BMT = (19,)
+
+
+def _same(batch, ref, what, rtol=0.0):
+    import torch
     for i in range(batch.nk):
         a, h, o = ref[i]
         n = batch.dks[i].ngp
         assert bool((batch.apw[i][..., :n] == a[..., :n]).all()), f"{what}: apwalm differs at k={i}"
-        assert bool((batch.H[i] == h).all()), f"{what}: H differs at k={i}"
-        assert bool((batch.O[i] == o).all()), f"{what}: O differs at k={i}"
+        if rtol == 0.0:
+            assert bool((batch.H[i] == h).all()), f"{what}: H differs at k={i}"
+            assert bool((batch.O[i] == o).all()), f"{what}: O differs at k={i}"
+        else:
+            eh = (torch.linalg.norm(batch.H[i] - h) / torch.linalg.norm(h)).item()
+            eo = (torch.linalg.norm(batch.O[i] - o) / torch.linalg.norm(o)).item()
+            assert eh <= rtol and eo <= rtol, f"{what}: H {eh:.2e}, O {eo:.2e} at k={i}"
 
 
 @pytest.mark.parametrize("cfg,nk", [(1, 16), (2, 6)])
@@ -56,8 +67,8 @@ def test_batched_equals_per_k(cfg, nk):
             _clear(b)
             ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH)
             torch.cuda.synchronize()
-            _same(b, ref, f"variant {v}")
-        ctx.set_variant("bmt", -1)
+            _same(b, ref, f"variant {v}", 1e-14 if v in GAUSS_BMT else 0.0)
+        ctx.set_variant("bmt", 18)   # the four-product MT kernel: Y variants bit for bit
         for v in range(ctx.num_variants("ystage")):
             print(f"cfg {cfg} nk {nk} grouped Y variant {v}", flush=True)
             ctx.set_variant("ystage", v)
@@ -69,6 +80,10 @@ def test_batched_equals_per_k(cfg, nk):
 
 
 def test_graph_replays_match_eager():
+    """Replays of the captured grouped launch (default variants) equal the eager grouped
+    launch bit for bit, with the program's own eager work between them; the eager grouped
+    result is within 1e-14 of the per-k path (the default MT kernel uses Gauss's three
+    products)."""
     import torch
     from exciting_amd import lapw
     w = workload(1, 16)
@@ -76,7 +91,12 @@ def test_graph_replays_match_eager():
     ctx = lapw.Context(w.crystal, stream=s)
     with torch.cuda.stream(s):
         b = lapw.DeviceBatch(w.crystal, w.kpoints)
-        ref = _perk(ctx, b)
+        perk = _perk(ctx, b)
+        _clear(b)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH)
+        torch.cuda.synchronize()
+        _same(b, perk, "eager grouped vs per-k", 1e-14)
+        ref = [(b.apw[i].clone(), b.H[i].clone(), b.O[i].clone()) for i in range(b.nk)]
         flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_GRAPH
         for rep in range(4):
             _clear(b)
