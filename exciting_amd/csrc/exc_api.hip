@@ -505,7 +505,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // tile, 8 waves, two LDS stages filled by DMA, one item per workgroup; split: 64x64 tile,
   // LDS filled by DMA)
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 19);
-  const int yv = c->y_variant >= 0 ? c->y_variant : 8;     // 64x128 Y tiles, DMA-staged
+  const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);

@@ -60,11 +60,11 @@ int launch_bystage_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
   return EXC_OK;
 }
 
-template <int BM, int BN, int BK, int NST, int WR, int WC>
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *__restrict__ ks, int natm, int naa,
                                                                    const cplx *__restrict__ haa,
                                                                    const int *__restrict__ fperm) {
-  using T = YTileDMA<BM, BN, BK, NST, WR, WC>;
+  using T = YTileDMA<BM, BN, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   EXC_DMA_STAGES(St, NST)
   const int kk = blockIdx.z / natm, ias = blockIdx.z % natm;
@@ -75,10 +75,10 @@ __global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *
          haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0, blockIdx.y * BN, fperm);
 }
 
-template <int BM, int BN, int BK, int NST, int WR, int WC>
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_bystage_dma_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
   dim3 grid((maxngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm * nk);
-  hipLaunchKernelGGL((bystage_dma_kernel<BM, BN, BK, NST, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa,
+  hipLaunchKernelGGL((bystage_dma_kernel<BM, BN, BK, NST, WR, WC, G3>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa,
                      d.haa, d.fperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
@@ -283,6 +283,7 @@ int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, in
     case 9: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 10: return launch_bystage_dma_t<64, 64, 32, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 11: return launch_bystage_dma_t<64, 128, 8, 3, 2, 4>(d, nk, maxngp, ks, s);
+    case 12: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, nk, maxngp, ks, s);   // Gauss's 3 products
     default: return EXC_ERR_ARG;
   }
 }

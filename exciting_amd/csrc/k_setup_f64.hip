@@ -55,12 +55,12 @@ int launch_mt_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cp
   return EXC_OK;
 }
 
-template <int BM, int BN, int BK, int NST, int WR, int WC>
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void ystage_dma_kernel(int ngp, int naa, const cplx *__restrict__ apwalm,
                                                                   int ldg, const cplx *__restrict__ haa,
                                                                   cplx *__restrict__ yt, int ldy,
                                                                   const int *__restrict__ fperm) {
-  using T = YTileDMA<BM, BN, BK, NST, WR, WC>;
+  using T = YTileDMA<BM, BN, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   EXC_DMA_STAGES(St, NST)
   const int ias = blockIdx.z;
@@ -68,10 +68,10 @@ __global__ __launch_bounds__(64 * WR * WC) void ystage_dma_kernel(int ngp, int n
          yt + (size_t)ldy * naa * ias, ldy, blockIdx.x * BM, blockIdx.y * BN, fperm);
 }
 
-template <int BM, int BN, int BK, int NST, int WR, int WC>
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_ystage_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy, hipStream_t s) {
   dim3 grid((ngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm);
-  hipLaunchKernelGGL((ystage_dma_kernel<BM, BN, BK, NST, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ngp, d.naa, apwalm,
+  hipLaunchKernelGGL((ystage_dma_kernel<BM, BN, BK, NST, WR, WC, G3>), grid, dim3(64 * WR * WC), 0, s, ngp, d.naa, apwalm,
                      ldg, d.haa, yt, ldy, d.fperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
@@ -128,6 +128,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
     case 9: return launch_ystage_dma_t<64, 128, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 10: return launch_ystage_dma_t<64, 64, 32, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 11: return launch_ystage_dma_t<64, 128, 8, 3, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 12: return launch_ystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, s);   // Gauss's 3 products
     default: return EXC_ERR_ARG;
   }
 }

@@ -350,9 +350,9 @@ struct MtTileDMA {
 
 // One (row tile, column tile) of Yt_a = A_a h_a^T (no conjugation), summed over f in the
 // order fperm; one tile per workgroup.
-template <int BM, int BN, int BK, int NST, int WR, int WC>
+template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 struct YTileDMA {
-  using C = DmaCore<BM, BN, BK, NST, WR, WC, false>;
+  using C = DmaCore<BM, BN, BK, NST, WR, WC, false, G3>;
   using Stage = typename C::Stage;
   __device__ static void run(Stage *s0, Stage *s1, Stage *s2, Stage *s3, int ngp, int naa,
                              const cplx *__restrict__ A, int ldg, const cplx *__restrict__ ha, cplx *__restrict__ Y,

@@ -102,13 +102,17 @@ def test_device_entry_points_match_host():
     assert np.array_equal(O.cpu().numpy().T, O_h)
 
 
+GAUSS_Y = (12,)
+
+
 @pytest.mark.parametrize("cfg", [0, 1])
 def test_per_k_variants_bit_identical(cfg):
     """Every tile variant of the per-k Y and MT stages (register- or DMA-staged LDS, any tile
     shape) sums each element in the same order, so H and O agree bit for bit with the
-    default variants, which test_config0 / test_config1_two_kpoints check against the oracle.
-    Workload 0 (N = 97) exercises partial tiles and a K that is not a multiple of the
-    K-step."""
+    default variants, which test_config0 / test_config1_two_kpoints check against the oracle;
+    the Y variant with Gauss's three real products per complex product (12) rounds otherwise
+    and agrees to 1e-14 relative.  Workload 0 (N = 97) exercises partial tiles and a K that is
+    not a multiple of the K-step."""
     w = workload(cfg, 1)
     cr, kp = w.crystal, w.kpoints[0]
     ctx = _ctx(cr)
@@ -119,6 +123,10 @@ def test_per_k_variants_bit_identical(cfg):
             print(f"config{cfg} N={kp.ngp} per-k {stage} variant {v}", flush=True)
             ctx.set_variant(stage, v)
             H, O = ctx.hamiltonandoverlapsetup(kp, A)
+            if stage == "ystage" and v in GAUSS_Y:
+                eh = np.linalg.norm(H - H0) / np.linalg.norm(H0)
+                assert eh <= 1e-14 and np.array_equal(O, O0), f"{stage} variant {v}: H {eh:.2e}"
+                continue
             assert np.array_equal(H, H0), f"{stage} variant {v}: H differs"
             assert np.array_equal(O, O0), f"{stage} variant {v}: O differs"
         ctx.set_variant(stage, -1)
