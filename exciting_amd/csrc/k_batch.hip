@@ -302,14 +302,14 @@ int exc_bmt_tile(int variant) {
     case 9: return 32;
     case 10: case 11: case 12: case 13: case 14: return 64;
     case 15: case 16: case 17: return 128;
-    case 18: case 19: return 64;
+    case 18: case 19: case 20: case 21: return 64;
     default: return 0;
   }
 }
 
 bool exc_bmt_fused(int variant) { return variant >= 7 && variant <= 9; }
 
-bool exc_bmt_persistent(int variant) { return variant != 18 && variant != 19; }
+bool exc_bmt_persistent(int variant) { return variant < 18 || variant > 21; }
 
 int exc_bmt_max_blocks(int variant) {
   switch (variant) {
@@ -331,7 +331,7 @@ int exc_bmt_max_blocks(int variant) {
     case 15: return max_blocks_dma_t<128, 16, 2, 2, 4>();
     case 16: return max_blocks_dma_t<128, 8, 2, 2, 4>();
     case 17: return max_blocks_dma_t<128, 8, 3, 2, 4>();
-    case 18: case 19: return 1;   // plain grid: the block count is the item count
+    case 18: case 19: case 20: case 21: return 1;   // plain grid: the block count is the item count
     default: return 1;
   }
 }
@@ -360,6 +360,8 @@ int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 17: return launch_bmt_dma_t<128, 8, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
     case 18: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4>(d, ks, items, nitems, s);
     case 19: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4, true>(d, ks, items, nitems, s);   // Gauss's 3 products
+    case 20: return launch_bmt_dma_grid_t<64, 8, 4, 2, 4, true>(d, ks, items, nitems, s);
+    case 21: return launch_bmt_dma_grid_t<64, 16, 3, 2, 4, true>(d, ks, items, nitems, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -33,7 +33,7 @@ def _clear(batch):
         batch.O[i].zero_()
 
 
-GAUSS_BMT = (19,)   # grouped MT variants with Gauss's three real products per complex product
+GAUSS_BMT = (19, 20, 21)   # grouped MT variants with Gauss's three real products per complex product
 GAUSS_Y = (12,)     # Y variants likewise
 
 
