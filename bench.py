@@ -387,9 +387,48 @@ def run_dmat(args, ctx, batch, w, nst, vecs, log):
     ms_ir /= max(cnt_ir, 1)
     grid = [int(cr.intgv[i, 1] - cr.intgv[i, 0] + 1) for i in range(3)]
     log(f"  interstitial density (genrhoir, {nst} states, grid {grid}): {ms_ir / batch.nk:.3f} ms/k")
+    vjp = run_vjp(args, ctx, batch, cr, nst, vecs, occs, wk, log)
     return {"ms_per_k": ms / batch.nk, "tflops": tf, "frac_fp64_peak": tf * 1e12 / PEAK_FP64, "flops_per_launch": flops,
             "k_points": batch.nk, "states": nst, "wfsize": ws,
-            "interstitial_density_ms_per_k": ms_ir / batch.nk, "fft_grid": grid}
+            "interstitial_density_ms_per_k": ms_ir / batch.nk, "fft_grid": grid, "ibs_force_adjoint": vjp}
+
+
+def run_vjp(args, ctx, batch, cr, nst, vecs, occs, wk, log):
+    """The adjoint of the setup (exc_ho_vjp_dev: forcek's IBS term, dE/dtau and dE/dV(G)) on
+    the eigenvectors of the eigensolve: wh = w occ, wo = -w occ e with seeded e (the time does
+    not depend on the values), every k-point of the batch per step."""
+    import torch
+    rng = np.random.default_rng(20261204)
+    ev = [np.sort(rng.uniform(-2.0, 2.0, nst)) for _ in range(batch.nk)]
+    wh = [(wk[i] * occs[i]).contiguous() for i in range(batch.nk)]
+    wo = [(-wk[i] * occs[i] * torch.from_numpy(ev[i]).cuda()).contiguous() for i in range(batch.nk)]
+    dt = torch.zeros((cr.natm, 3), dtype=torch.float64, device="cuda")
+    dv = torch.zeros(cr.ngvec, dtype=torch.complex128, device="cuda")
+
+    def step():
+        for i, d in enumerate(batch.dks):
+            ctx.vjp_dev(d.ngp, d.ngkmax, batch.apw[i], d.igpig, d.vgpc, d.sfacgp, vecs[i], batch.nmat[i], nst,
+                        wh[i], wo[i], dt, dv)
+
+    try:
+        step()
+        torch.cuda.synchronize()
+    except Exception as e:   # reported, not fatal: the other legs' figures stand
+        log(f"  IBS force adjoint: not measured ({e})")
+        return None
+    ctx.read_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt_s = (time.perf_counter() - t0) / args.steps
+    ctx.set_timing(False)
+    ms, cnt = ctx.read_timing()["vjp"]
+    res = {"ms_per_k": 1000.0 * dt_s / batch.nk, "kernel_ms_per_k": ms / max(cnt, 1), "states": nst}
+    log(f"  IBS force adjoint (exc_ho_vjp_dev, {nst} states): {res['ms_per_k']:.3f} ms/k, "
+        f"kernels {res['kernel_ms_per_k']:.3f} ms/k")
+    return res
 
 
 def main():
