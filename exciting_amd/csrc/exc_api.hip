@@ -707,7 +707,29 @@ int exc_ho_vjp_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
   if (rc) return rc;
   const size_t need = sizeof(cplx) * exc_vjp_work_elems(c->d, ngp, nst);
   if ((rc = grow((void **)&c->vjp_work, &c->vjp_bytes, need ? need : 1))) return rc;
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
+  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+    W.blas = nullptr;
+    return EXC_ERR_HIP;
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
   StageTimer t(c, EXC_STAGE_VJP);
+  if (ngp > 0 && nst > 0) {
+    // uv(:, l K + c) = (q_l C)^T A(:, c), l = 0..3 (q_0 = 1): the q-scaled copies of C, then
+    // one strided batch of four library products into uv (nst x 4K, leading dimension nst)
+    const int ktot = c->d.natm * c->d.naa;
+    cplx *uv = c->vjp_work;
+    cplx *cs = uv + (size_t)nst * 4 * ktot + 2 * (size_t)ngp * nst;
+    if ((rc = exc_launch_vjp_qscale(ngp, nst, (const cplx *)evec, ldc, vgpc, cs, c->stream))) return rc;
+    const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+    if (rocblas_zgemm_strided_batched(W.blas, rocblas_operation_transpose, rocblas_operation_none, nst, ktot, ngp,
+                                      &one, (const rocblas_double_complex *)cs, ngp, (rocblas_stride)ngp * nst,
+                                      (const rocblas_double_complex *)apwalm, ngkmax, 0, &zero,
+                                      (rocblas_double_complex *)uv, nst, (rocblas_stride)ktot * nst,
+                                      4) != rocblas_status_success)
+      return EXC_ERR_HIP;
+  }
   return exc_launch_vjp(c->d, ngp, nst, (const cplx *)apwalm, ngkmax, igpig, vgpc, (const cplx *)sfacgp,
                         (const cplx *)evec, ldc, wh, wo, c->vjp_work, dedtau, dedv, c->stream);
 }

@@ -119,7 +119,8 @@ int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStr
 size_t exc_vjp_work_elems(const exc_dev &d, int ngp, int nst);
 int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int ldg, const int *igpig,
                    const double *vgpc, const cplx *sfacgp, const cplx *evec, int ldc, const double *wh,
-                   const double *wo, cplx *work, double *dedtau, double *dedv, hipStream_t s);
+                   const double *wo, cplx *work, double *dedtau, double *dedv, hipStream_t s);   // uv in work first
+int exc_launch_vjp_qscale(int ngp, int nst, const cplx *evec, int ldc, const double *vgpc, cplx *cs, hipStream_t s);
 // split-BF16 muffin-tin stage
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)

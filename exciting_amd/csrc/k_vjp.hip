@@ -11,8 +11,9 @@
 //  Theta(G)   :  sum_ij (q_il - q_jl) Im(-ffac_a(G_i-G_j) conj(S_ia) S_ja (dp_ij D_ji + W_ji))
 //                (the tau dependence of Theta(G) = delta_G0 - sum_a exp(-iG.r_a) ffac_a(G))
 //  dL/dV(g)   =  sum_{ij: G_i - G_j = g} D_ji  (holomorphic partial over the full matrix)
-// Kernels: uv (FP64 MFMA, both operands k-contiguous), atom (per (atom, state) traces),
-// dw_theta (FP64 MFMA tiles of D^T, W^T over the APW block, reduced in the epilogue).
+// U and V_l come from library products the entry point runs before exc_launch_vjp (the
+// q-scaled copies of C from q_scale_kernel times A); kernels here: atom (per (atom, state)
+// traces), dw_theta (FP64 MFMA tiles of D^T, W^T over the APW block, reduced in the epilogue).
 #include "setup_dev.h"
 
 namespace {
@@ -20,61 +21,6 @@ namespace {
 using namespace zf64;
 using setup_dev::GTab;
 using setup_dev::lookup_g;
-
-// ---------------------------------------------------------------------------------------
-// uv: out(m, c') = sum_G C(G, m) * B(G, c'), c' in [0, 4K): B = A, q_x A, q_y A, q_z A
-// ---------------------------------------------------------------------------------------
-constexpr int UBM = 64, UBN = 64, UBK = 16;
-using UCore = Core<UBM, UBN, UBK, 2, 2, false>;
-
-__global__ __launch_bounds__(256) void uv_kernel(int ngp, int nst, int ktot, const cplx *__restrict__ evec, int ldc,
-                                                 const cplx *__restrict__ apwalm, int ldg,
-                                                 const double *__restrict__ vgpc, cplx *__restrict__ uv) {
-  __shared__ Smem<UBM, UBN, UBK> sm;
-  UCore core;
-  core.zero();
-  const int m0 = blockIdx.x * UBM, c0 = blockIdx.y * UBN;
-  const int tid = threadIdx.x, wave = tid >> 6, wr = wave >> 1, wc = wave & 1;
-  for (int g0 = 0; g0 < ngp; g0 += UBK) {
-    __syncthreads();
-    for (int e = tid; e < UBM * UBK; e += 256) {
-      const int row = e / UBK, k = e % UBK;
-      const int m = m0 + row, g = g0 + k;
-      const cplx v = (m < nst && g < ngp) ? evec[g + (size_t)m * ldc] : cplx{0.0, 0.0};
-      sm.a_re[k][row] = v.re;
-      sm.a_im[k][row] = v.im;
-    }
-    for (int e = tid; e < UBN * UBK; e += 256) {
-      const int row = e / UBK, k = e % UBK;
-      const int cp = c0 + row, g = g0 + k;
-      cplx v = {0.0, 0.0};
-      if (cp < 4 * ktot && g < ngp) {
-        const int l = cp / ktot, c = cp % ktot;
-        v = apwalm[g + (size_t)c * ldg];
-        if (l) {
-          const double s = vgpc[3 * g + l - 1];
-          v.re *= s;
-          v.im *= s;
-        }
-      }
-      sm.b_re[k][row] = v.re;
-      sm.b_im[k][row] = v.im;
-    }
-    __syncthreads();
-    core.compute(sm, wr, wc);
-  }
-#pragma unroll
-  for (int i = 0; i < UCore::MB; ++i)
-#pragma unroll
-    for (int j = 0; j < UCore::NB; ++j) {
-      const int c = c0 + UCore::col_of(wc, j);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = m0 + UCore::row_of(wr, i, v);
-        if (r < nst && c < 4 * ktot) uv[r + (size_t)c * nst] = cplx{core.cr[i][j][v], core.ci[i][j][v]};
-      }
-    }
-}
 
 // ---------------------------------------------------------------------------------------
 // atom: one workgroup per (atom, state n); MT and APW-lo terms
@@ -270,19 +216,40 @@ __global__ __launch_bounds__(256) void dw_theta_kernel(int ngp, int nst, int nat
   }
 }
 
+// cs_l(G, m) = q_l(G) C(G, m) for l = 1..3 and cs_0 = C: the left operands of
+// uv(:, l K + c) = cs_l^T A(:, c), one strided batch of library products
+__global__ __launch_bounds__(256) void q_scale_kernel(int ngp, int nst, const cplx *__restrict__ evec, int ldc,
+                                                      const double *__restrict__ vgpc, cplx *__restrict__ cs) {
+  const int g = blockIdx.x * 256 + threadIdx.x, m = blockIdx.y;
+  if (g >= ngp) return;
+  const cplx v = evec[g + (size_t)ldc * m];
+  const size_t o = g + (size_t)ngp * m, st = (size_t)ngp * nst;
+  cs[o] = v;
+  for (int l = 1; l <= 3; ++l) {
+    const double q = vgpc[3 * g + l - 1];
+    cs[o + l * st] = cplx{v.re * q, v.im * q};
+  }
+}
+
 }  // namespace
+
+// the four q-scaled copies of the APW rows of C (4 ngp nst elements at cs)
+int exc_launch_vjp_qscale(int ngp, int nst, const cplx *evec, int ldc, const double *vgpc, cplx *cs, hipStream_t s) {
+  if (ngp <= 0 || nst <= 0) return EXC_OK;
+  if (nst > 65535) return EXC_ERR_ARG;
+  hipLaunchKernelGGL(q_scale_kernel, dim3((ngp + 255) / 256, nst), dim3(256), 0, s, ngp, nst, evec, ldc, vgpc, cs);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
 
 int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int ldg, const int *igpig,
                    const double *vgpc, const cplx *sfacgp, const cplx *evec, int ldc, const double *wh,
                    const double *wo, cplx *work, double *dedtau, double *dedv, hipStream_t s) {
   if (ngp <= 0 || nst <= 0) return EXC_OK;
   const int ktot = d.natm * d.naa;
-  cplx *uv = work;                                   // nst x 4K
+  cplx *uv = work;                                   // nst x 4K, filled by the caller
   cplx *cf = uv + (size_t)nst * 4 * ktot;            // ngp x nst
   cplx *cg = cf + (size_t)ngp * nst;
-  dim3 g1((nst + UBM - 1) / UBM, (4 * ktot + UBN - 1) / UBN);
-  hipLaunchKernelGGL(uv_kernel, g1, dim3(256), 0, s, ngp, nst, ktot, evec, ldc, apwalm, ldg, vgpc, uv);
-  EXC_CHECK(hipGetLastError());
   if (d.naa > ATH || d.nlotot > ATH) return EXC_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(atom_kernel, dim3(d.natm, nst), dim3(ATH), 0, s, ngp, nst, d.natm, d.naa, d.maxnlo, d.nlomax,
                      d.apwordmax, d.nlotot, d.lod, d.haa, d.loa, d.oalo, evec, ldc, wh, wo, uv, dedtau);
@@ -297,6 +264,7 @@ int exc_launch_vjp(const exc_dev &d, int ngp, int nst, const cplx *apwalm, int l
   return EXC_OK;
 }
 
+// uv (nst x 4K), cf and cg (ngp x nst each), then the q-scaled copies of C (4 ngp nst)
 size_t exc_vjp_work_elems(const exc_dev &d, int ngp, int nst) {
-  return (size_t)nst * 4 * d.natm * d.naa + 2 * (size_t)ngp * nst;
+  return (size_t)nst * 4 * d.natm * d.naa + 6 * (size_t)ngp * nst;
 }
