@@ -423,9 +423,10 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
   if (ngp > 0) {
     const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
     if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
-    // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA)
+    // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA;
+    // the MT tile with Gauss's three real products per complex product)
     const int yv = c->y_variant >= 0 ? c->y_variant : 5;
-    const int mv = c->mt_variant >= 0 ? c->mt_variant : 6;
+    const int mv = c->mt_variant >= 0 ? c->mt_variant : 8;
     {
       StageTimer t(c, EXC_STAGE_YSTAGE);
       rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);

@@ -103,7 +103,7 @@ int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const 
                   const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
                   hipStream_t s);
 constexpr int kYstageVariants = 13;
-constexpr int kMtVariants = 8;
+constexpr int kMtVariants = 9;
 constexpr int kBmtVariants = 22;
 // Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.
 int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);

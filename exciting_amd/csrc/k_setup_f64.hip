@@ -78,7 +78,7 @@ int launch_ystage_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, 
 }
 
 // one (tile, H-or-O) item per workgroup, LDS filled by DMA
-template <int BM, int BK, int NST, int WR, int WC>
+template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void mt_dma_kernel(int ngp, int ktot, int ntile,
                                                               const cplx *__restrict__ apwalm, int ldg,
                                                               const cplx *__restrict__ yt, int ldy,
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64 * WR * WC) void mt_dma_kernel(int ngp, int ktot,
                                                               const double *__restrict__ vgpc, GTab gt,
                                                               cplx *__restrict__ h, cplx *__restrict__ o, int ldh,
                                                               const int *__restrict__ kperm) {
-  using T = MtTileDMA<BM, BK, NST, WR, WC>;
+  using T = MtTileDMA<BM, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   __shared__ typename T::Tables tb;
   EXC_DMA_STAGES(St, NST)
@@ -99,12 +99,12 @@ __global__ __launch_bounds__(64 * WR * WC) void mt_dma_kernel(int ngp, int ktot,
          item & 1);
 }
 
-template <int BM, int BK, int NST, int WR, int WC>
+template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_mt_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy, const int *igpig,
                     const double *vgpc, cplx *h, cplx *o, int ldh, hipStream_t s) {
   const int ntile = (ngp + BM - 1) / BM;
   const int nitem = ntile * (ntile + 1);
-  hipLaunchKernelGGL((mt_dma_kernel<BM, BK, NST, WR, WC>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp,
+  hipLaunchKernelGGL((mt_dma_kernel<BM, BK, NST, WR, WC, G3>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp,
                      d.natm * d.naa, ntile, apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
@@ -145,6 +145,7 @@ int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const 
     case 5: return launch_mt_t<32, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     case 6: return launch_mt_dma_t<64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     case 7: return launch_mt_dma_t<64, 8, 4, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
+    case 8: return launch_mt_dma_t<64, 16, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -14,15 +14,22 @@ from conftest import workload
 pytestmark = pytest.mark.gpu
 
 
-def _perk(ctx, batch):
+def _perk(ctx, batch, four_product=True):
+    """The per-k results; with four_product, on the four-product per-k variants (MT 6, Y 5)."""
     import torch
     out = []
+    if four_product:
+        ctx.set_variant("mt", 6)
+        ctx.set_variant("ystage", 5)
     for i, d in enumerate(batch.dks):
         ctx.match_dev(d.ngp, d.ngkmax, d.gpc, d.tpgpc, d.sfacgp, batch.apw[i])
         ctx.setup_dev(d.ngp, d.ngkmax, batch.apw[i], d.igpig, d.vgpc, batch.nmat[i], batch.H[i], batch.O[i])
     torch.cuda.synchronize()
     for i in range(batch.nk):
         out.append((batch.apw[i].clone(), batch.H[i].clone(), batch.O[i].clone()))
+    if four_product:
+        ctx.set_variant("mt", -1)
+        ctx.set_variant("ystage", -1)
     return out
 
 
