@@ -102,7 +102,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
 int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy,
                   const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
                   hipStream_t s);
-constexpr int kYstageVariants = 13;
+constexpr int kYstageVariants = 14;
 constexpr int kMtVariants = 9;
 constexpr int kBmtVariants = 22;
 // Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.

@@ -284,6 +284,7 @@ int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, in
     case 10: return launch_bystage_dma_t<64, 64, 32, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 11: return launch_bystage_dma_t<64, 128, 8, 3, 2, 4>(d, nk, maxngp, ks, s);
     case 12: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, nk, maxngp, ks, s);   // Gauss's 3 products
+    case 13: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, nk, maxngp, ks, s);
     default: return EXC_ERR_ARG;
   }
 }

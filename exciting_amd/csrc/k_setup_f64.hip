@@ -129,6 +129,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
     case 10: return launch_ystage_dma_t<64, 64, 32, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 11: return launch_ystage_dma_t<64, 128, 8, 3, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 12: return launch_ystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, s);   // Gauss's 3 products
+    case 13: return launch_ystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, s);
     default: return EXC_ERR_ARG;
   }
 }

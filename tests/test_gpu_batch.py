@@ -41,7 +41,7 @@ def _clear(batch):
 
 
 GAUSS_BMT = (19, 20, 21)   # grouped MT variants with Gauss's three real products per complex product
-GAUSS_Y = (12,)     # Y variants likewise
+GAUSS_Y = (12, 13)  # Y variants likewise
 
 
 def _same(batch, ref, what, rtol=0.0):
