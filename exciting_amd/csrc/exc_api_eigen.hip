@@ -363,6 +363,19 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
   }
   const int v = c->eig_variant >= 0 ? c->eig_variant : 6;
   if (v >= 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv, v);
+  // every worker's buffers sized for the largest nmat before anything is queued: a worker
+  // solves k and k + nw one after another, and eig_enqueue must not reallocate a buffer that
+  // the queued solve of an earlier k-point still uses
+  {
+    int nmax = 1;
+    for (int k = 0; k < nk; ++k) nmax = std::max(nmax, nmat[k]);
+    for (int i = 1; i <= nw; ++i) {
+      exc_ctx::EigWorker &W = c->ew[i];
+      if ((rc = grow((void **)&W.w, &W.wb, sizeof(double) * nmax))) return rc;
+      if ((rc = grow((void **)&W.z, &W.zb, sizeof(cplx) * (size_t)nmax * nmax))) return rc;
+      if ((rc = grow((void **)&W.e, &W.eb, sizeof(double) * (nmax + 1)))) return rc;
+    }
+  }
   {
     StageTimer t(c, EXC_STAGE_EIGEN);
     EXC_CHECK(hipEventRecord(c->eig_fork, c->stream));

@@ -183,9 +183,14 @@ inline void copy_cols(char *dst, size_t dld, const char *src, size_t sld, size_t
   for (auto &x : th) x.join();
 }
 
+// A larger buffer replaces the old one; work already queued on any stream may still read or
+// write the old one, so the device drains before it is freed.
 inline int grow(void **p, size_t *have, size_t need) {
   if (need <= *have) return EXC_OK;
-  if (*p) EXC_CHECK(hipFree(*p));
+  if (*p) {
+    EXC_CHECK(hipDeviceSynchronize());
+    EXC_CHECK(hipFree(*p));
+  }
   *p = nullptr;
   *have = 0;
   EXC_CHECK(hipMalloc(p, need));
