@@ -6,7 +6,8 @@
   <= 1e-8 of H_MT / O_MT, twenty times below the split's own error against FP64;
 * against the FP64 oracle: relative Frobenius error of H and O <= 1e-6, and the
   eigenvalue error is printed (this seeded spectrum is ill-conditioned; its size is
-  reported by bench.py and discussed in DESIGN.md, not asserted at 1e-8 Ha here).
+  reported by bench.py, not asserted at 1e-8 Ha here: a 3-slice split of H moves the lowest
+  eigenvalues of these spectra by about 1e-6 Ha).
 """
 import numpy as np
 import pytest
