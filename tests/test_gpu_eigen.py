@@ -129,6 +129,43 @@ def test_eigen_batched_equals_single():
         assert torch.equal(res["single"][1][i], res["batched"][1][i])
 
 
+def test_eigen_batched_growing_orders():
+    """The multi-k per-k-solver path (variant 4, 4 worker streams) on a fresh context with
+    orders that grow from each worker's first k-point to its second (k and k + 4 share a
+    worker): every k-point to the LAPACK bars and equal bit for bit to the single-k solve.
+    Regression test for worker buffers that were reallocated for k + 4 while k's queued
+    solve still used them."""
+    import scipy.linalg
+    import torch
+    from exciting_amd import lapw
+    w = small("a")
+    rng = np.random.default_rng(19)
+    sizes, nev = [40, 44, 48, 52, 90, 96, 102, 108], 12
+    pairs = [_random_pair(rng, n, False) for n in sizes]
+    res = {}
+    for mode in ("single", "batched"):
+        ctx = lapw.Context(w.crystal)
+        ctx.set_variant("eigen", 4)
+        hs = [_to_dev(np.asfortranarray(p[0])) for p in pairs]
+        os_ = [_to_dev(np.asfortranarray(p[1])) for p in pairs]
+        evs = [torch.zeros(nev, dtype=torch.float64, device="cuda") for _ in sizes]
+        vecs = [torch.zeros((nev, n), dtype=torch.complex128, device="cuda") for n in sizes]
+        if mode == "single":
+            for i, n in enumerate(sizes):
+                ctx.eigen_dev(n, n, hs[i], os_[i], nev, evs[i], vecs[i], n)
+        else:
+            ctx.eigen_batched_dev(sizes, hs, os_, nev, evs, vecs)
+        torch.cuda.synchronize()
+        res[mode] = ([e.cpu() for e in evs], [v.cpu() for v in vecs])
+        print(f"growing orders, {mode}", flush=True)
+    for i, n in enumerate(sizes):
+        H, O = pairs[i][0], pairs[i][1]
+        e_ref = scipy.linalg.eigh(H, O, eigvals_only=True, subset_by_index=[0, nev - 1])
+        _check(H, O, res["batched"][0][i].numpy(), res["batched"][1][i].numpy().T, e_ref, f"growing k{i} n={n}")
+        assert torch.equal(res["single"][0][i], res["batched"][0][i])
+        assert torch.equal(res["single"][1][i], res["batched"][1][i])
+
+
 def _random_pair(rng, n, lower_garbage):
     """Hermitian H, Hermitian positive definite O (Fortran order); with lower_garbage the
     strictly lower triangles are overwritten by unrelated values, which ZHEGVX('U') ignores."""
