@@ -363,9 +363,12 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
   }
   const int v = c->eig_variant >= 0 ? c->eig_variant : 6;
   if (v >= 5) return eig_grouped(c, nk, nmat, ldh, h, o, nev, eval, evec, ldv, v);
-  // every worker's buffers sized for the largest nmat before anything is queued: a worker
-  // solves k and k + nw one after another, and eig_enqueue must not reallocate a buffer that
-  // the queued solve of an earlier k-point still uses
+  // Variants 0-4 solve the k-points one after another on one worker stream: run
+  // concurrently on the 4 worker streams, rocSOLVER's potrf / trtri / heevd chain returned one
+  // k-point's eigenpairs wrong (ghost eigenvalues off by 0.1-0.3 Ha) in about one test run of
+  // four, with no shared buffer or missing wait in this code.  The worker's buffers are sized
+  // for the largest nmat before anything is queued, so eig_enqueue never reallocates a buffer
+  // that the queued solve of an earlier k-point still uses.
   {
     int nmax = 1;
     for (int k = 0; k < nk; ++k) nmax = std::max(nmax, nmat[k]);
@@ -391,7 +394,7 @@ int exc_eigen_batched_dev(exc_ctx *c, int nk, const int *nmat, const int *ldh, d
       }
       for (int k = 0; k < nk && !rc; ++k) {
         if (nmat[k] == 0) continue;
-        exc_ctx::EigWorker &W = c->ew[1 + k % nw];
+        exc_ctx::EigWorker &W = c->ew[1];   // one after another (see below)
         rc = eig_enqueue(W, v, phase, nmat[k], ldh[k], h[k], o[k], nev, eval[k], evec[k], ldv[k], c->eig_int + 4 * k);
       }
       if (rc) break;

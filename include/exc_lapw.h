@@ -218,8 +218,9 @@ int exc_eigen(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, do
  * and conquer per k-point on worker streams (h, o left unchanged); variant 5 back-transforms
  * the eigenvectors per k-point instead of as one batch; variant 7 finds the tridiagonal
  * eigenpairs by bisection and inverse iteration as ZHEGVX does (slow).  Solver variants 0-4
- * (exc_ctx_set_variant, EXC_STAGE_EIGEN) instead run the single-k solves concurrently on
- * worker streams forked from and joined back into the context's stream (h, o overwritten).  Returns 0, or the first failing k-point's status. */
+ * (exc_ctx_set_variant, EXC_STAGE_EIGEN) instead run the single-k solves one after another
+ * on a worker stream forked from and joined back into the context's stream (h, o
+ * overwritten).  Returns 0, or the first failing k-point's status. */
 int exc_eigen_batched_dev(exc_ctx *ctx, int nk, const int *nmat, const int *ldh, double *const *h,
                           double *const *o, int nev, double *const *eval, double *const *evec,
                           const int *ldv);
