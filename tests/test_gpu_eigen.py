@@ -99,8 +99,8 @@ def test_eigen_solver_variants(variant):
 
 
 def test_eigen_batched_equals_single():
-    """The multi-k entry on concurrent worker streams (variant 4) gives the single-k results
-    bit for bit."""
+    """The multi-k entry with the per-k solver (variant 4, on a worker stream forked from the
+    context's stream) gives the single-k results bit for bit."""
     import torch
     from exciting_amd import lapw
     w = workload(1, 6)
@@ -130,11 +130,10 @@ def test_eigen_batched_equals_single():
 
 
 def test_eigen_batched_growing_orders():
-    """The multi-k per-k-solver path (variant 4, 4 worker streams) on a fresh context with
-    orders that grow from each worker's first k-point to its second (k and k + 4 share a
-    worker): every k-point to the LAPACK bars and equal bit for bit to the single-k solve.
-    Regression test for worker buffers that were reallocated for k + 4 while k's queued
-    solve still used them."""
+    """The multi-k per-k-solver path (variant 4) on a fresh context with orders that grow
+    along the worker's queue: every k-point to the LAPACK bars and equal bit for bit to the
+    single-k solve.  Regression test for worker buffers that were reallocated for a larger
+    k-point while an earlier k-point's queued solve still used them."""
     import scipy.linalg
     import torch
     from exciting_amd import lapw
