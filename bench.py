@@ -606,7 +606,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "k-points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator, no checkpoints)",
-        "config": {"workload": f"config{args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
+        "config": {"workload": f"workload {args.config}: " + WORKLOAD_NAMES[args.config], "k_points": nk,
                    "path": ("FP64, all k-points grouped per stage, hipGraph replay"
                             + (", overlap MT part by the addition theorem" if olpflag else "")
                             if args.path == "batched" else "FP64, per-k launches"

@@ -24,7 +24,7 @@ def _inputs(cr, kp, nst, seed):
     return C, occ
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_gendmatmt_host_against_oracle(which):
     from exciting_amd import lapw
     from oracle import lapw_ref as R

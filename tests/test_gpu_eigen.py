@@ -27,7 +27,7 @@ def _check(H_ref, O_ref, e, C, e_ref, tag):
     assert orth <= 1e-10
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_eigen_host_against_lapack(which):
     from exciting_amd import lapw
     from oracle import lapw_ref as R
@@ -65,7 +65,7 @@ def test_eigen_on_device_after_grouped_setup():
         e_ref, _ = R.eigen(H_ref, O_ref, nev)
         e = out[i][0].cpu().numpy()
         C = out[i][1].cpu().numpy().T
-        _check(H_ref, O_ref, e, C, e_ref, f"config1 k{i} N={kp.ngp} device pipeline")
+        _check(H_ref, O_ref, e, C, e_ref, f"workload1 k{i} N={kp.ngp} device pipeline")
 
 
 def test_eigen_reports_indefinite_overlap():

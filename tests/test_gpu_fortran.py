@@ -20,7 +20,7 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
                    "exc_driver_main")
 
 
-@pytest.mark.parametrize("which", ["small-b", "config0"])
+@pytest.mark.parametrize("which", ["small-b", "workload0"])
 def test_fortran_driver(tmp_path, which):
     from oracle import lapw_ref as R
     if not os.path.exists(EXE):
@@ -71,7 +71,7 @@ def test_fortran_forcek(tmp_path):
     assert rel <= 1e-12
 
 
-@pytest.mark.parametrize("which", ["small-b", "config0"])
+@pytest.mark.parametrize("which", ["small-b", "workload0"])
 def test_fortran_solvewithgpu(tmp_path, which):
     """solvewithgpu (Fortran, through exc_eigen) after the setup: eigenvalues against the
     oracle's LAPACK solve of the oracle's H, O within 1e-10 Ha, residual and O-orthonormality."""

@@ -18,7 +18,7 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_hmlint_against_oracle(which):
     from exciting_amd import lapw, synth
     from oracle import hmlint_ref as M
@@ -95,7 +95,7 @@ def test_match_and_setup_from_radial_data():
     assert ea <= 1e-12 and eh <= 1e-12 and eo <= 1e-12
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_genrhomt_against_oracle(which):
     """The muffin-tin density from a density matrix (exc_genrhomt) on the radial data of the
     preceding hmlint call, against the oracle's genrhomt (pinned by the potential-energy

@@ -30,11 +30,11 @@ def _run(w, flags):
     return ctx, s, b, [(b.apw[i].cpu(), b.H[i].cpu(), b.O[i].cpu()) for i in range(b.nk)]
 
 
-@pytest.mark.parametrize("which", ["small-d", "config0", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload0", "workload1"])
 def test_olp_legendre_against_oracle(which):
     from exciting_amd import lapw
     from oracle import lapw_ref as R
-    w = {"small-d": lambda: small("d"), "config0": lambda: workload(0), "config1": lambda: workload(1, 2)}[which]()
+    w = {"small-d": lambda: small("d"), "workload0": lambda: workload(0), "workload1": lambda: workload(1, 2)}[which]()
     base = lapw.EXC_BATCH_MATCH
     ref = _run(w, base)[3]
     res = _run(w, base | lapw.EXC_BATCH_OLP_LEGENDRE)[3]

@@ -29,8 +29,8 @@ def _case(which, lmaxvr, nst, seed):
     return cr, kp, rd, apw, C, occ
 
 
-@pytest.mark.parametrize("which,lmaxvr,lradstep", [("small-d", 4, 1), ("small-d", 4, 3), ("config1", 8, 1),
-                                                   ("config1", 6, 2)])
+@pytest.mark.parametrize("which,lmaxvr,lradstep", [("small-d", 4, 1), ("small-d", 4, 3), ("workload1", 8, 1),
+                                                   ("workload1", 6, 2)])
 def test_rhovalk_against_oracle(which, lmaxvr, lradstep):
     from exciting_amd import lapw
     from oracle import rhovalk_ref as V
@@ -74,7 +74,7 @@ def test_rhovalk_dev_equals_host_entry():
     import torch
     from exciting_amd import lapw
     from oracle import rhovalk_ref as V
-    cr, kp, rd, apw, C, occ = _case("config1", 8, 24, 9)
+    cr, kp, rd, apw, C, occ = _case("workload1", 8, 24, 9)
     ctx = lapw.Context(cr)
     ctx.hmlint(rd, out=False)
     zb, _, rf = V.genshtmat(8)

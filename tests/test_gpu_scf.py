@@ -25,7 +25,7 @@ def _veffir(cr, seed):
     return v / np.abs(v).max() - 0.5
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_gencfun_genveffig_against_oracle(which):
     from exciting_amd import lapw
     from oracle import lapw_ref as R
@@ -63,7 +63,7 @@ def test_setup_reads_the_new_tables():
     assert eh <= 1e-12 and eo <= 1e-12
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_genrhoir_against_oracle(which):
     """The interstitial density of one k-point (exc_genrhoir, batched FFTs of 64 states) against
     the oracle's genrhoir (pinned by Parseval and direct sums in test_oracle_scf.py); 70 states

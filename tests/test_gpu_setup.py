@@ -57,13 +57,13 @@ def test_small_workloads(key):
         _check_k(ctx, w.crystal, kp)
 
 
-def test_config0():
+def test_workload0():
     w = workload(0)
     ctx = _ctx(w.crystal)
     _check_k(ctx, w.crystal, w.kpoints[0])
 
 
-def test_config0_padded_leading_dims():
+def test_workload0_padded_leading_dims():
     """ngkmax > ngp and ldh > nmat: padding rows are left alone, results unchanged."""
     from exciting_amd import synth
     w = workload(0)
@@ -72,7 +72,7 @@ def test_config0_padded_leading_dims():
     _check_k(ctx, w.crystal, kp, ldh_extra=5)
 
 
-def test_config1_two_kpoints():
+def test_workload1_two_kpoints():
     w = workload(1, 2)
     ctx = _ctx(w.crystal)
     for kp in w.kpoints:
@@ -112,7 +112,7 @@ def test_per_k_variants_bit_identical(cfg):
     """Every tile variant of the per-k Y and MT stages (register- or DMA-staged LDS, any tile
     shape) sums each element in the same order, so H and O agree bit for bit with the
     four-product variants MT 6 / Y 5 (checked against the oracle like the defaults in
-    test_config0 / test_config1_two_kpoints); the variants with Gauss's three real products
+    test_workload0 / test_workload1_two_kpoints); the variants with Gauss's three real products
     per complex product (MT 8, Y 12) round otherwise and agree to 1e-14 relative.  Workload 0
     (N = 97) exercises partial tiles and a K that is not a multiple of the K-step."""
     w = workload(cfg, 1)
@@ -124,7 +124,7 @@ def test_per_k_variants_bit_identical(cfg):
     H0, O0 = ctx.hamiltonandoverlapsetup(kp, A)
     for stage in ("mt", "ystage"):
         for v in range(ctx.num_variants(stage)):
-            print(f"config{cfg} N={kp.ngp} per-k {stage} variant {v}", flush=True)
+            print(f"workload{cfg} N={kp.ngp} per-k {stage} variant {v}", flush=True)
             ctx.set_variant(stage, v)
             H, O = ctx.hamiltonandoverlapsetup(kp, A)
             if v in (GAUSS_Y if stage == "ystage" else GAUSS_MT):

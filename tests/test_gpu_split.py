@@ -29,7 +29,7 @@ def _run_split(w, nk):
     return [(b.H[i].cpu().numpy().T, b.O[i].cpu().numpy().T) for i in range(nk)]
 
 
-@pytest.mark.parametrize("which", ["small-d", "config1"])
+@pytest.mark.parametrize("which", ["small-d", "workload1"])
 def test_split_against_emulation_and_fp64(which):
     from oracle import lapw_ref as R
     from oracle import split_ref as S
@@ -84,7 +84,7 @@ def test_split_variants_bit_identical():
     with torch.cuda.stream(s):
         b = lapw.DeviceBatch(w.crystal, w.kpoints)
         for v in range(ctx.num_variants("bsplit")):
-            print(f"config1 nk {b.nk} split variant {v}", flush=True)
+            print(f"workload1 nk {b.nk} split variant {v}", flush=True)
             ctx.set_variant("bsplit", v)
             for t in b.H + b.O:
                 t.fill_(float("nan"))

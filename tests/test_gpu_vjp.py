@@ -166,6 +166,6 @@ def test_vjp_small():
     _check(w.crystal, w.kpoints[0], nst=16, natoms_fd=w.crystal.natm)
 
 
-def test_vjp_config4_one_kpoint():
+def test_vjp_workload4_one_kpoint():
     w = workload(4, 1)
     _check(w.crystal, w.kpoints[0], nst=64, natoms_fd=3)
