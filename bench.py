@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark: LAPW H+O setup k-points/s on one MI355X.
 
-One step = match + H/O setup for every k-point of the workload (default: workload 1, the
-orthorhombic 8-atom cell with 16 k-points and N_G+k in [1150, 1250]), with all inputs
-resident in HBM.  Prints one JSON line (the contract of the run driver):
+One step = match + H/O setup for every k-point of the workload (default: workload 3, the
+32-atom cell with 4 species, lmax 10 and 8 k-points of N_G+k about 5000, on which the
+split-precision contraction is compared with FP64), with all inputs resident in HBM.  Prints one JSON line (the contract of the run driver):
 metric/value/unit, the roofline of the dominant kernel (the muffin-tin MFMA contraction,
 timed with HIP events on its own stream inside the library) and a CPU baseline (the FP64
 oracle, timed on the host's cores on a bounded sample of the same k-points).
@@ -48,26 +48,39 @@ def ystage_flops(cr, n):
     return 8.0 * cr.maxaa * cr.maxaa * cr.natm * n
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, budget_s=15.0):
-    """The FP64 oracle (numpy/OpenBLAS on the host cores) on the first k-points of the workload."""
+    """The FP64 oracle (numpy/OpenBLAS on the host cores) on the first k-points of the workload,
+    with the muffin-tin sums in the reference's own decomposition (one zgemm per atom,
+    hamiltonsetup.f90:75-103, overlapsetup.f90:95-108) and match as the oracle restates it."""
     from oracle import lapw_ref as R
     try:
         from threadpoolctl import threadpool_info
         threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
     except Exception:
         threads = os.cpu_count() or 1
-    R.setup(w.crystal, w.kpoints[0])            # warm-up (page-in, BLAS threads)
     t0 = time.perf_counter()
     done = 0
     for kp in w.kpoints:
-        R.setup(w.crystal, kp)
+        R.setup(w.crystal, kp, per_atom=True)
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "k-points/s", "cores": int(threads), "kind": "port",
-            "sample": f"match+hamiltonsetup+overlapsetup of the first {done} k-point(s) of the same workload "
-                      f"(numpy + host OpenBLAS, {dt:.1f} s)"}
+    return {"value": done / dt, "unit": "k-points/s", "cores": int(threads), "cpu_model": cpu_model(),
+            "kind": "port",
+            "sample": f"match + hamiltonsetup + overlapsetup of the first {done} k-point(s) of the same workload "
+                      f"(numpy + host OpenBLAS, one zgemm per atom as the reference, {dt:.1f} s, no warm-up "
+                      f"pass)"}
 
 
 PEAK_BF16 = 2.5e15            # MI355X dense BF16 MFMA, flop/s
@@ -79,31 +92,46 @@ def bsplit_alg_flops(cr, ngps):
     return sum(6.0 * mt_flops(cr, n) for n in ngps)
 
 
-def eig_error(Hs, Os, Hf, Of, nev=200):
-    """max |e_split - e_fp64| over the lowest nev eigenvalues (LAPACK zhegvx on the host)."""
+PEAK_INT8 = 5.0e15            # MI355X dense INT8 MFMA, op/s
+
+
+def lowest_eigs(H, O, nev=200):
+    """The lowest nev eigenvalues of H c = e O c (LAPACK zhegvx on the host, range 'I', upper
+    triangles, as solvewithlapack calls it)."""
     import scipy.linalg
-    m = min(nev, Hf.shape[0])
-    kw = dict(lower=False, subset_by_index=[0, m - 1], driver="gvx", eigvals_only=True)
-    e0 = scipy.linalg.eigh(Hf, Of, **kw)
-    e1 = scipy.linalg.eigh(Hs, Os, **kw)
-    return float(np.abs(e1 - e0).max()), float(e0[0])
+    m = min(nev, H.shape[0])
+    return scipy.linalg.eigh(H, O, lower=False, subset_by_index=[0, m - 1], driver="gvx", eigvals_only=True)
 
 
-def run_split(args, ctx, batch, cr, w, fp64_value, log, olpflag=0):
+def crt_alg_ops(cr, ngps, nmod, share):
+    """Algorithmic INT8 MFMA ops of the residue products: per modulus three real products
+    (Gauss's form) over the upper triangle (N (N + 1) / 2 elements) and K = natm * maxaa, two
+    ops per multiply-add; share = 0.5 when O comes from the addition theorem (H only)."""
+    K = cr.natm * cr.maxaa
+    return sum(2.0 * 3 * nmod * K * n * (n + 1) / 2 for n in ngps) * (2 * share)
+
+
+def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt"):
+    """The split-precision leg: the muffin-tin contraction from exact integer residue products
+    (kind 'crt', EXC_BATCH_SPLIT_CRT, default 11 moduli) or from three BF16 slices (kind
+    'bf16', EXC_BATCH_SPLIT_BF16); everything else as the FP64 leg.  Eigenvalue error: the
+    lowest 200 of k-point 0 against the FP64 GPU path's (e0)."""
     import torch
     from exciting_amd import lapw
     nk = batch.nk
-    Hf = batch.H[0].cpu().numpy().T.copy()
-    Of = batch.O[0].cpu().numpy().T.copy()
-    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_BF16 | olpflag
+    flags = lapw.EXC_BATCH_MATCH | olpflag | (lapw.EXC_BATCH_SPLIT_CRT if kind == "crt" else lapw.EXC_BATCH_SPLIT_BF16)
     tuned = None
     for kv in args.set:
         st, v = kv.split("=")
         ctx.set_variant(st, int(v))
         tuned = dict(tuned or {}, **{st: int(v)})
+    if kind == "crt" and args.moduli:
+        ctx.set_moduli(args.moduli)
+    nmod = args.moduli or 11
     if args.tune:
         from exciting_amd import autotune
-        tuned = autotune.tune(ctx, ["bsplit"], lambda: ctx.setup_batched_dev(batch, flags), reps=2, log=log)
+        tuned = autotune.tune(ctx, ["crtgemm" if kind == "crt" else "bsplit"], lambda: ctx.setup_batched_dev(batch, flags),
+                              reps=2, log=log)
     ctx.setup_batched_dev(batch, flags)          # eager pass: stage breakdown
     torch.cuda.synchronize()
     ctx.read_timing()
@@ -113,7 +141,7 @@ def run_split(args, ctx, batch, cr, w, fp64_value, log, olpflag=0):
     eager = ctx.read_timing()
     for k, (ms, cnt) in eager.items():
         if cnt:
-            log(f"  split stage {k:7s}: {ms:8.3f} ms")
+            log(f"  {kind} split stage {k:8s}: {ms:8.3f} ms")
     gflags = flags | lapw.EXC_BATCH_GRAPH
     for _ in range(args.warmup):
         ctx.setup_batched_dev(batch, gflags)
@@ -126,20 +154,28 @@ def run_split(args, ctx, batch, cr, w, fp64_value, log, olpflag=0):
     value = nk * args.steps / dt
     Hs = batch.H[0].cpu().numpy().T.copy()
     Os = batch.O[0].cpu().numpy().T.copy()
-    err, e0 = eig_error(Hs, Os, Hf, Of)
-    rel = float(np.linalg.norm(Hs - Hf) / np.linalg.norm(Hf))
-    ms = eager["bsplit"][0]
+    e1 = lowest_eigs(Hs, Os)
+    err = float(np.abs(e1 - e0).max())
     share = 0.5 if olpflag else 1.0           # H only when O comes from the addition theorem
-    ex = share * bsplit_alg_flops(cr, [kp.ngp for kp in w.kpoints])
-    fl64 = share * sum(mt_flops(cr, kp.ngp) for kp in w.kpoints)
-    log(f"  split-BF16: {value:.1f} k-points/s ({value / fp64_value:.2f}x FP64), lowest-200 eig err {err:.2e} Ha, "
-        f"|dH|/|H| {rel:.2e}")
+    ngps = [kp.ngp for kp in w.kpoints]
+    fl64 = share * sum(mt_flops(cr, n) for n in ngps)
+    if kind == "crt":
+        ms = eager["crtgemm"][0]
+        ops = crt_alg_ops(cr, ngps, nmod, share)
+        roof = {"bound": "mfma", "kernel": f"crt_gemm_kernel (residue products, {nmod} moduli, int8 MFMA)",
+                "achieved": ops / (ms * 1e-3) / 1e12, "peak": PEAK_INT8 / 1e12, "unit": "TOP/s",
+                "frac": ops / (ms * 1e-3) / PEAK_INT8, "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12,
+                "per_launch_ops": ops}
+    else:
+        ms = eager["bsplit"][0]
+        ex = share * bsplit_alg_flops(cr, ngps)
+        roof = {"bound": "mfma", "kernel": "bsplit2_kernel (grouped split-BF16 MT)",
+                "achieved": ex / (ms * 1e-3) / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                "frac": ex / (ms * 1e-3) / PEAK_BF16, "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12}
+    log(f"  {kind} split: {value:.1f} k-points/s ({value / fp64_value:.2f}x FP64), lowest-200 eig err {err:.2e} Ha")
     return {"value": value, "unit": "k-points/s", "speedup_vs_fp64": value / fp64_value,
-            "ms_per_step": 1000.0 * dt / args.steps, "eig_err_lowest200_Ha_k0": err, "lowest_eig_Ha_k0": e0,
-            "rel_frobenius_dH_k0": rel, "meets_1e-8_Ha": err <= 1e-8,
-            "roofline": {"bound": "mfma", "kernel": "bsplit2_kernel (grouped split-BF16 MT)", "achieved": ex / (ms * 1e-3) / 1e12,
-                         "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": ex / (ms * 1e-3) / PEAK_BF16,
-                         "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12},
+            "ms_per_step": 1000.0 * dt / args.steps, "eig_err_lowest200_Ha_k0": err, "lowest_eig_Ha_k0": float(e0[0]),
+            "meets_1e-8_Ha": err <= 1e-8, "moduli": nmod if kind == "crt" else None, "roofline": roof,
             "stages_ms": {k: v for k, (v, c) in eager.items() if c}, "variants": tuned}
 
 
@@ -165,28 +201,32 @@ def run_eigen(args, ctx, batch, w, olpflag, nev, log):
     torch.cuda.synchronize()
     ctx.read_timing()
     ctx.set_timing(True)
+    # the solve grows as nmat^3: at nmat ~5000 one step of 8 k-points takes seconds
+    steps = args.steps if max(batch.nmat) <= 3000 else 1
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
     tim = ctx.read_timing()
     ms_eig, cnt = tim["eigen"]
-    value = batch.nk * args.steps / dt
-    # host LAPACK on the first k (bounded: one solve)
-    ctx.setup_batched_dev(batch, flags)
-    torch.cuda.synchronize()
-    H = batch.H[0].cpu().numpy().T.copy()
-    O = batch.O[0].cpu().numpy().T.copy()
-    t1 = time.perf_counter()
-    scipy.linalg.eigh(H, O, lower=False, subset_by_index=[0, nev - 1], driver="gvx")
-    cpu_ms = 1000.0 * (time.perf_counter() - t1)
+    value = batch.nk * steps / dt
+    # host LAPACK on the first k (bounded: one solve, and only below nmat 3000)
+    cpu_ms = None
+    if batch.nmat[0] <= 3000:
+        ctx.setup_batched_dev(batch, flags)
+        torch.cuda.synchronize()
+        H = batch.H[0].cpu().numpy().T.copy()
+        O = batch.O[0].cpu().numpy().T.copy()
+        t1 = time.perf_counter()
+        scipy.linalg.eigh(H, O, lower=False, subset_by_index=[0, nev - 1], driver="gvx")
+        cpu_ms = 1000.0 * (time.perf_counter() - t1)
     ms_eig /= batch.nk     # one timed launch covers all k-points
-    log(f"  setup + eigensolve (lowest {nev}): {value:.1f} k-points/s, eigensolve {ms_eig / max(cnt, 1):.2f} ms/k "
-        f"(host LAPACK zhegvx {cpu_ms:.0f} ms/k)")
+    log(f"  setup + eigensolve (lowest {nev}): {value:.1f} k-points/s, eigensolve {ms_eig / max(cnt, 1):.2f} ms/k"
+        + (f" (host LAPACK zhegvx {cpu_ms:.0f} ms/k)" if cpu_ms else ""))
     dmat = run_dmat(args, ctx, batch, w, nev, vecs, log)
-    return {"value": value, "unit": "k-points/s", "nev": nev, "ms_per_step": 1000.0 * dt / args.steps,
+    return {"value": value, "unit": "k-points/s", "nev": nev, "ms_per_step": 1000.0 * dt / steps, "steps": steps,
             "eigensolve_ms_per_k": ms_eig / max(cnt, 1),
             "solver": "rocSOLVER: k-points padded to one order, strided-batched Cholesky, inverse, reduction and "
                       "tridiagonalization; per-k divide and conquer on 4 streams; back-transform of "
@@ -436,13 +476,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune tile variants before timing")
     ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
                     help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
-    ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 measurement")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision measurements")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the three-slice BF16 split measurement")
+    ap.add_argument("--moduli", type=int, default=None, help="moduli of the modular split (8..12, default 11)")
     ap.add_argument("--no-hmlint", action="store_true", help="skip the muffin-tin integrals (hmlint) measurement")
     ap.add_argument("--eigen", type=int, default=200, metavar="NEV",
                     help="grouped path: also time setup + device eigensolve of the lowest NEV states "
@@ -576,8 +618,8 @@ def main():
             "executed_mfma_frac": (0.75 if gauss else 1.0) * achieved / PEAK_FP64}
     # HBM traffic of the dominant kernel: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this same
     # default command, scaled by a 1 GiB copy (committed summary); only for the profiled setup
-    prof = HERE / "profiles" / "setup_kernels_workload1_16k.json"
-    if (args.path == "batched" and args.config == 1 and args.nk is None and not args.tune and olpflag
+    prof = HERE / "profiles" / f"setup_kernels_workload{args.config}.json"
+    if (args.path == "batched" and args.nk is None and not args.tune and not args.set and olpflag
             and prof.exists()):
         rec = json.loads(prof.read_text()).get("bmt_dma_grid_kernel")
         if rec:
@@ -591,7 +633,13 @@ def main():
 
     split = None
     if args.path == "batched" and not args.no_split:
-        split = run_split(args, ctx, batch, cr, w, value, log, olpflag)
+        # eigenvalues of the FP64 result of k-point 0, the split legs' yardstick
+        step_batched(False)
+        torch.cuda.synchronize()
+        e0 = lowest_eigs(batch.H[0].cpu().numpy().T.copy(), batch.O[0].cpu().numpy().T.copy())
+        split = {"crt": run_split(args, ctx, batch, cr, w, value, e0, log, olpflag, "crt")}
+        if not args.no_bf16:
+            split["bf16"] = run_split(args, ctx, batch, cr, w, value, e0, log, olpflag, "bf16")
     eig = None
     if args.path == "batched" and args.eigen > 0:
         eig = run_eigen(args, ctx, batch, w, olpflag, min(args.eigen, min(batch.nmat)), log)
@@ -612,7 +660,7 @@ def main():
                             if args.path == "batched" else "FP64, per-k launches"
                             + (", host arrays (PCIe copies included)" if args.path == "host" else "")),
                    "parallelism": "single GPU", "variants": tuned},
-        "roofline": roof, "stages": stages, "split_bf16": split, "setup_and_eigensolve": eig,
+        "roofline": roof, "stages": stages, "split": split, "setup_and_eigensolve": eig,
         "muffin_tin_integrals": hml, "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

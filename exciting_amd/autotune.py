@@ -1,20 +1,44 @@
 """Tile-shape autotuner for the setup kernels.
 
-For a stage ("ystage", "mt", and for the grouped launch "bmt"), every compiled variant is
-run on the caller's real inputs, timed with the library's per-stage HIP-event timers, and
-the fastest is pinned in the context with ``exc_ctx_set_variant``.  Results are keyed by
-the basis-size bucket (N rounded up to a multiple of 256) and the APW count naa, and can
-be stored as JSON so a later run starts from the tuned table.
+For a stage ("ystage", "mt", and for the grouped launch "bmt", "crtgemm", "bsplit"), every
+compiled variant is run on the caller's real inputs, timed with the library's per-stage
+HIP-event timers, and the fastest is pinned in the context with ``exc_ctx_set_variant``.
+Results are keyed by the basis-size bucket (the largest N of the batch rounded up to a
+multiple of 256), the APW count naa and the atom count, and are kept in a JSON table;
+``apply`` pins the table's choice for a batch before it is launched.  The library's built-in
+defaults are used for shapes the table does not hold.  ``TABLE`` is the table measured on
+MI355X for the shapes of the four GPU workloads.
 """
 from __future__ import annotations
 
 import json
 import os
-from typing import Callable, Dict
+from pathlib import Path
+from typing import Callable, Dict, Optional
+
+TABLE = Path(__file__).with_name("tuned.json")
 
 
 def bucket(n: int) -> int:
     return ((n + 255) // 256) * 256
+
+
+def key(cr, ngps) -> str:
+    """Table key of a batch: N bucket of its largest basis, naa, natm."""
+    return f"N{bucket(max(ngps))}-naa{cr.maxaa}-natm{cr.natm}"
+
+
+def lookup(cr, ngps, path: os.PathLike = TABLE) -> Optional[Dict[str, int]]:
+    return load(path, key(cr, ngps))
+
+
+def apply(ctx, cr, ngps, path: os.PathLike = TABLE) -> Dict[str, int]:
+    """Pin the tuned variants of this batch shape in ctx (nothing if the table has no entry);
+    returns what was pinned."""
+    chosen = lookup(cr, ngps, path) or {}
+    for st, v in chosen.items():
+        ctx.set_variant(st, int(v))
+    return dict(chosen)
 
 
 def sweep(ctx, stage: str, run: Callable[[], None], reps: int = 3, log=None) -> Dict[int, float]:

@@ -57,6 +57,8 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
     }
     int rc = dev_upload(c, &d.kperm, kp.data(), kp.size());
     if (!rc) rc = dev_upload(c, &d.fperm, fp.data(), fp.size());
+    const unsigned zero = 0u;
+    if (!rc) rc = dev_upload(c, &d.miss, &zero, 1);
     if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = EXC_ERR_HIP;
     if (rc) {
       for (void *p : c->owned) hipFree(p);
@@ -68,6 +70,8 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->y_variant = -1;
   c->bmt_variant = -1;
   c->bsplit_variant = -1;
+  c->crt_variant = -1;
+  c->crt_nmod = 11;
   c->eig_variant = -1;
   c->gnt_lmaxvr = -1;
   c->hml_lmaxvr = -1;
@@ -93,6 +97,7 @@ int exc_ctx_destroy(exc_ctx *c) {
     if (w.z) hipFree(w.z);
     if (w.zc) hipFree(w.zc);
     if (w.e) hipFree(w.e);
+    if (w.ws) hipFree(w.ws);
     if (w.done) hipEventDestroy(w.done);
     if (i > 0 && w.s) hipStreamDestroy(w.s);
   }
@@ -110,6 +115,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->stage) hipFree(c->stage);
+  if (c->apw_res) hipFree(c->apw_res);
   if (c->hstage) hipHostFree(c->hstage);
   for (auto &m : c->marks) {
     hipEventDestroy(m.a);
@@ -124,6 +130,11 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->rfbuf) hipFree(c->rfbuf);
   if (c->d_oitems) hipFree(c->d_oitems);
   if (c->planes) hipFree(c->planes);
+  if (c->crt_q) hipFree(c->crt_q);
+  if (c->crt_x) hipFree(c->crt_x);
+  if (c->crt_res) hipFree(c->crt_res);
+  if (c->d_citems) hipFree(c->d_citems);
+  if (c->d_cwork) hipFree(c->d_cwork);
   if (c->expo) hipFree(c->expo);
   if (c->vjp_work) hipFree(c->vjp_work);
   delete c;
@@ -137,6 +148,22 @@ int exc_ctx_set_stream(exc_ctx *c, void *stream) {
 }
 
 int exc_ctx_nlotot(const exc_ctx *c) { return c ? c->d.nlotot : EXC_ERR_ARG; }
+
+int exc_ctx_status(exc_ctx *c) {
+  if (!c) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  unsigned miss = 0;
+  EXC_CHECK(hipMemcpy(&miss, c->d.miss, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (!miss) return EXC_OK;
+  const unsigned zero = 0u;
+  EXC_CHECK(hipMemcpy(c->d.miss, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+  fprintf(stderr,
+          "exc_lapw: %u pair(s) of basis vectors with G_i - G_j outside the first ngvec G-vectors "
+          "(2 gkmax > gmaxvr, init1.f90:340-347)\n", miss);
+  return EXC_ERR_ARG;
+}
 
 int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
   if (!c) return EXC_ERR_ARG;
@@ -161,6 +188,10 @@ int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
       if (variant < -1 || variant >= kEigenVariants) return EXC_ERR_ARG;
       c->eig_variant = variant;
       return EXC_OK;
+    case EXC_STAGE_CRTGEMM:
+      if (variant < -1 || variant >= kCrtVariants) return EXC_ERR_ARG;
+      c->crt_variant = variant;
+      return EXC_OK;
     default:
       return EXC_ERR_ARG;
   }
@@ -173,8 +204,24 @@ int exc_ctx_num_variants(int stage) {
     case EXC_STAGE_BMT: return kBmtVariants;
     case EXC_STAGE_BSPLIT: return kBsplitVariants;
     case EXC_STAGE_EIGEN: return kEigenVariants;
+    case EXC_STAGE_CRTGEMM: return kCrtVariants;
     default: return 0;
   }
+}
+
+int exc_ctx_set_moduli(exc_ctx *c, int n) {
+  if (!c || n < 8 || n > kCrtMaxUsed) return EXC_ERR_ARG;
+  c->crt_nmod = n;
+  return EXC_OK;
+}
+
+int exc_crt_moduli(int n, int *moduli, int *beta) {
+  if (n < 8 || n > kCrtMaxUsed || !moduli || !beta) return EXC_ERR_ARG;
+  CrtTab t;
+  exc_crt_tables(n, &t);
+  for (int l = 0; l < n; ++l) moduli[l] = t.m[l];
+  *beta = t.beta;
+  return EXC_OK;
 }
 
 int exc_ctx_set_timing(exc_ctx *c, int enable) {
@@ -231,6 +278,9 @@ int exc_ctx_set_lo(exc_ctx *c, const int *nlorb, const int *lorbl, const int *id
   int rc = use_device(c);
   if (rc) return rc;
   const int ns = c->dims.nspecies, na = c->dims.natmtot, nlomax = c->dims.nlomax, lolm = c->dims.lolmmax;
+  // the tables below are replaced by synchronous copies on the null stream, which does not wait
+  // for a non-blocking context stream: queued work that reads them completes first
+  EXC_CHECK(hipStreamSynchronize(c->stream));
   std::vector<int> sp(na);
   EXC_CHECK(hipMemcpy(sp.data(), c->d.spidx, sizeof(int) * na, hipMemcpyDeviceToHost));
   std::vector<lo_desc> lod;
@@ -303,8 +353,13 @@ int exc_ctx_set_radial(exc_ctx *c, const double *dmat) {
       const double det = m[0] * m[3] - m[2] * m[1];
       double *o = inv.data() + 4 * (l + (L + 1) * (size_t)ias);
       if (det == 0.0 || !isfinite(det)) {
+        // zgesv stops after the factorization with info > 0 and leaves the right-hand sides
+        // unsolved; match.f90:166-185 prints the error and stores them as the coefficients,
+        // so the inverse is taken as the identity here
         if (!info) info = 1 + l + (L + 1) * ias;
-        o[0] = o[1] = o[2] = o[3] = 0.0;
+        o[0] = 1.0;
+        o[1] = o[2] = 0.0;
+        o[3] = 1.0;
       } else {
         o[0] = m[3] / det;
         o[1] = -m[1] / det;
@@ -389,12 +444,14 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
   if (rc) return rc;
   const size_t na = c->d.natm, lmm = c->d.lmmaxapw, ao = c->d.apwordmax, nk = ngkmax;
   const size_t b_gpc = 8 * nk, b_tp = 16 * nk, b_sf = 16 * nk * na, b_apw = 16 * nk * ao * lmm * na;
-  if ((rc = grow(&c->stage, &c->stage_bytes, b_gpc + b_tp + b_sf + b_apw))) return rc;
+  c->apw_res_valid = false;
+  if ((rc = grow(&c->stage, &c->stage_bytes, b_gpc + b_tp + b_sf))) return rc;
+  if ((rc = grow((void **)&c->apw_res, &c->apw_res_bytes, b_apw ? b_apw : 1))) return rc;
   char *base = (char *)c->stage;
   double *d_gpc = (double *)base;
   double *d_tp = (double *)(base + b_gpc);
   double *d_sf = (double *)(base + b_gpc + b_tp);
-  double *d_apw = (double *)(base + b_gpc + b_tp + b_sf);
+  double *d_apw = (double *)c->apw_res;
   EXC_CHECK(hipMemcpyAsync(d_gpc, gpc, b_gpc, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(d_tp, tpgpc, b_tp, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(d_sf, sfacgp, b_sf, hipMemcpyHostToDevice, c->stream));
@@ -409,25 +466,33 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
     copy_cols((char *)apwalm, 16 * nk, (const char *)c->hstage, 16 * nk, 16 * (size_t)ngp, ao * lmm * na);
   }
   EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->apw_res_valid = true;
+  c->apw_res_ngp = ngp;
+  c->apw_res_ngkmax = ngkmax;
   return rc;
 }
 
-int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
-                                    const double *vgpc, int ldh, double *h, double *o) {
+}  // extern "C"
+
+namespace {
+
+// H (h given), O (o given) or both of one k-point from device arrays
+int setup_dev_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc, int ldh,
+                  double *h, double *o) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   const int nmat = ngp + c->d.nlotot;
-  if (ngp < 0 || ngkmax < ngp || ldh < nmat || !h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
+  if (ngp < 0 || ngkmax < ngp || ldh < nmat || (!h && !o) || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
     return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
   if (ngp > 0) {
-    const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
-    if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
     // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA;
     // the MT tile with Gauss's three real products per complex product)
     const int yv = c->y_variant >= 0 ? c->y_variant : 5;
     const int mv = c->mt_variant >= 0 ? c->mt_variant : 8;
-    {
+    if (h) {   // the Y stage feeds only H
+      const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
+      if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
       StageTimer t(c, EXC_STAGE_YSTAGE);
       rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);
     }
@@ -439,55 +504,103 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
     }
     if (rc) return rc;
   }
-  {
-    StageTimer t(c, EXC_STAGE_LO);
-    rc = exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream);
-  }
-  return rc;
+  StageTimer t(c, EXC_STAGE_LO);
+  return exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream);
 }
 
-int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
-                                const double *vgpc, int ldh, double *h, double *o) {
+// the same from host arrays; apwalm = NULL takes the device copy of the last exc_match
+int setup_host_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+                   int ldh, double *h, double *o) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   const int nmat = ngp + c->d.nlotot;
-  if (ngp < 0 || ngkmax < ngp || ldh < nmat || !h || !o || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
-    return EXC_ERR_ARG;
+  if (ngp < 0 || ngkmax < ngp || ldh < nmat || (!h && !o) || (ngp > 0 && (!igpig || !vgpc))) return EXC_ERR_ARG;
+  const bool resident = apwalm == nullptr;
+  if (resident && ngp > 0 && !(c->apw_res_valid && c->apw_res_ngp == ngp && c->apw_res_ngkmax == ngkmax))
+    return EXC_ERR_STATE;
   int rc = use_device(c);
   if (rc) return rc;
   const size_t na = c->d.natm, lmm = c->d.lmmaxapw, ao = c->d.apwordmax, nk = ngkmax;
-  const size_t b_apw = 16 * nk * ao * lmm * na, b_ig = 4 * nk, b_vg = 24 * nk, b_h = 16 * (size_t)ldh * nmat;
-  const size_t o_ig = (b_apw + 255) & ~(size_t)255, o_vg = o_ig + ((b_ig + 255) & ~(size_t)255);
-  const size_t o_h = o_vg + ((b_vg + 255) & ~(size_t)255), o_o = o_h + ((b_h + 255) & ~(size_t)255);
-  if ((rc = grow(&c->stage, &c->stage_bytes, o_o + b_h))) return rc;
-  if ((rc = grow_host(&c->hstage, &c->hstage_bytes, std::max(b_apw, 2 * b_h)))) return rc;
+  const size_t b_apw = resident ? 0 : 16 * nk * ao * lmm * na, b_ig = 4 * nk, b_vg = 24 * nk;
+  const size_t b_h = 16 * (size_t)ldh * nmat;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_ig = al(b_apw), o_vg = o_ig + al(b_ig), o_h = o_vg + al(b_vg);
+  const size_t o_o = o_h + (h ? al(b_h) : 0);
+  if ((rc = grow(&c->stage, &c->stage_bytes, o_o + (o ? b_h : 0)))) return rc;
+  const int nout = (h ? 1 : 0) + (o ? 1 : 0);
+  if ((rc = grow_host(&c->hstage, &c->hstage_bytes, std::max(b_apw, nout * b_h)))) return rc;
   char *base = (char *)c->stage;
   char *hs = (char *)c->hstage;
-  if (ngp > 0) {   // rows 1..ngp of apwalm are the input; the caller's padding rows are never read
+  const double *d_apw = resident ? (const double *)c->apw_res : (const double *)base;
+  if (!resident && ngp > 0) {   // rows 1..ngp of apwalm are the input; the caller's padding rows are never read
     copy_cols(hs, 16 * nk, (const char *)apwalm, 16 * nk, 16 * (size_t)ngp, ao * lmm * na);
     EXC_CHECK(hipMemcpyAsync(base, hs, b_apw, hipMemcpyHostToDevice, c->stream));
   }
   EXC_CHECK(hipMemcpyAsync(base + o_ig, igpig, b_ig, hipMemcpyHostToDevice, c->stream));
   EXC_CHECK(hipMemcpyAsync(base + o_vg, vgpc, b_vg, hipMemcpyHostToDevice, c->stream));
-  rc = exc_hamiltonandoverlapsetup_dev(c, ngp, ngkmax, (const double *)base, (const int *)(base + o_ig),
-                                       (const double *)(base + o_vg), ldh, (double *)(base + o_h),
-                                       (double *)(base + o_o));
+  double *dh = h ? (double *)(base + o_h) : nullptr, *dout = o ? (double *)(base + o_o) : nullptr;
+  rc = setup_dev_sel(c, ngp, ngkmax, d_apw, (const int *)(base + o_ig), (const double *)(base + o_vg), ldh, dh, dout);
   if (rc) return rc;
   // rows nmat+1..ldh of the caller's matrices are not part of the result: copy rows 1..nmat
   const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
-  EXC_CHECK(hipMemcpyAsync(hs, base + o_h, b_h, hipMemcpyDeviceToHost, c->stream));
-  EXC_CHECK(hipMemcpyAsync(hs + b_h, base + o_o, b_h, hipMemcpyDeviceToHost, c->stream));
+  if (h) EXC_CHECK(hipMemcpyAsync(hs, dh, b_h, hipMemcpyDeviceToHost, c->stream));
+  if (o) EXC_CHECK(hipMemcpyAsync(hs + (h ? b_h : 0), dout, b_h, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
-  copy_cols((char *)h, pitch, hs, pitch, width, nmat);
-  copy_cols((char *)o, pitch, hs + b_h, pitch, width, nmat);
-  return EXC_OK;
+  if (h) copy_cols((char *)h, pitch, hs, pitch, width, nmat);
+  if (o) copy_cols((char *)o, pitch, hs + (h ? b_h : 0), pitch, width, nmat);
+  return exc_ctx_status(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                                    const double *vgpc, int ldh, double *h, double *o) {
+  if (!h || !o) return EXC_ERR_ARG;
+  return setup_dev_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o);
+}
+
+int exc_hamiltonsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                          const double *vgpc, int ldh, double *h) {
+  if (!h) return EXC_ERR_ARG;
+  return setup_dev_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, nullptr);
+}
+
+int exc_overlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+                         int ldh, double *o) {
+  if (!o) return EXC_ERR_ARG;
+  return setup_dev_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, nullptr, o);
+}
+
+int exc_hamiltonandoverlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                                const double *vgpc, int ldh, double *h, double *o) {
+  if (!h || !o) return EXC_ERR_ARG;
+  return setup_host_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o);
+}
+
+int exc_hamiltonsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+                      int ldh, double *h) {
+  if (!h) return EXC_ERR_ARG;
+  return setup_host_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, nullptr);
+}
+
+int exc_overlapsetup(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc,
+                     int ldh, double *o) {
+  if (!o) return EXC_ERR_ARG;
+  return setup_host_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, nullptr, o);
 }
 
 int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flags) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (nk < 0 || (nk > 0 && !ks) ||
-      (flags & ~(EXC_BATCH_MATCH | EXC_BATCH_GRAPH | EXC_BATCH_SPLIT_BF16 | EXC_BATCH_OLP_LEGENDRE)))
+      (flags & ~(EXC_BATCH_MATCH | EXC_BATCH_GRAPH | EXC_BATCH_SPLIT_BF16 | EXC_BATCH_OLP_LEGENDRE |
+                 EXC_BATCH_SPLIT_CRT)))
     return EXC_ERR_ARG;
   const bool split = flags & EXC_BATCH_SPLIT_BF16;
+  const bool crt = flags & EXC_BATCH_SPLIT_CRT;
+  if (split && crt) return EXC_ERR_ARG;
+  // exact int32 sums of three residue products of K terms: 3 K 128^2 < 2^31
+  if (crt && c->d.natm * c->d.naa > 43000) return EXC_ERR_UNSUPPORTED;
   // the addition-theorem overlap needs the radial factors that only the grouped match writes
   const bool olp = flags & EXC_BATCH_OLP_LEGENDRE;
   if (olp && !(flags & EXC_BATCH_MATCH)) return EXC_ERR_ARG;
@@ -507,10 +620,15 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // LDS filled by DMA)
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 19);
   const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
+  const int cv = c->crt_variant >= 0 ? c->crt_variant : 2;
+  CrtTab ct;
+  if (crt) exc_crt_tables(c->crt_nmod, &ct);
+  const int npa = olp ? 3 : 4;                               // residue planes of A (re + im for O)
+  const int kp = exc_crt_kpad(c->d.natm * c->d.naa);
   // batch signature: descriptors, flags and variants
-  std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 4 * sizeof(int));
+  std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 6 * sizeof(int));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
-  int tail[4] = {nk, flags, bv, yv};
+  int tail[6] = {nk, flags, bv, yv, crt ? cv : -1, crt ? c->crt_nmod : -1};
   memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
   const bool same = (sig == c->batch_sig);
   if (same && (flags & EXC_BATCH_GRAPH) && c->graph_exec) {
@@ -543,6 +661,20 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if ((rc = grow((void **)&c->expo, &c->expo_bytes, sizeof(int) * (etot ? etot : 1)))) return rc;
     }
     if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot ? rtot : 1)))) return rc;
+    size_t qtot = 0, xtot = 0;
+    if (crt) {
+      for (int k = 0; k < nk; ++k) {
+        const size_t cnp = (size_t)(ks[k].ngp + 127) / 128 * 128;
+        qtot += (size_t)(npa + 3) * ct.n * kp * cnp;
+        xtot += 2 * cnp;
+      }
+      if ((rc = grow(&c->crt_q, &c->crt_q_bytes, qtot ? qtot : 1))) return rc;
+      if ((rc = grow((void **)&c->crt_x, &c->crt_x_bytes, sizeof(int) * (xtot ? xtot : 1)))) return rc;
+    }
+    std::vector<mt_item> citems;
+    std::vector<int2> cwork;
+    size_t qoff = 0, xoff = 0;
+    int maxcnp = 0;
     std::vector<kdesc> kd(nk);
     std::vector<mt_item> items, oitems;
     const int T = split ? exc_bsplit_tile(bv) : exc_bmt_tile(bv);
@@ -582,6 +714,30 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         e.rf = c->rfbuf + roff;
         roff += (size_t)q.ngp * c->d.natm * nl2;
       }
+      e.cnp = 0;
+      e.qa = e.qy = nullptr;
+      e.xa = e.xy = nullptr;
+      if (crt) {
+        // planes [modulus][plane][K/16][cnp][16] of A then of Yt; tiles (k, ti <= tj, H | O)
+        // column by column, and the (tile, modulus) work modulus-major within the k-point
+        const int cnp = (q.ngp + 127) / 128 * 128;
+        e.cnp = cnp;
+        e.qa = (char *)c->crt_q + qoff;
+        e.qy = (char *)c->crt_q + qoff + (size_t)npa * ct.n * kp * cnp;
+        qoff += (size_t)(npa + 3) * ct.n * kp * cnp;
+        e.xa = c->crt_x + xoff;
+        e.xy = c->crt_x + xoff + cnp;
+        xoff += 2 * (size_t)cnp;
+        maxcnp = cnp > maxcnp ? cnp : maxcnp;
+        const int nt = cnp / 128, first = (int)citems.size();
+        for (int tj = 0; tj < nt; ++tj)
+          for (int ti = 0; ti <= tj; ++ti) {
+            citems.push_back({k, ti, tj, 0});
+            if (!olp) citems.push_back({k, ti, tj, 1});
+          }
+        for (int l = 0; l < ct.n; ++l)
+          for (int i = first; i < (int)citems.size(); ++i) cwork.push_back(make_int2(i, l));
+      }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
       const int nt = (q.ngp + T - 1) / T;
       const bool fused = !split && exc_bmt_fused(bv);
@@ -605,6 +761,21 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         EXC_CHECK(hipMemcpy(c->d_oitems, oitems.data(), sizeof(mt_item) * oitems.size(), hipMemcpyHostToDevice));
     }
     c->batch_noitems = (int)oitems.size();
+    c->batch_ncitems = (int)citems.size();
+    c->batch_ncwork = (int)cwork.size();
+    c->batch_maxcnp = maxcnp;
+    if (crt) {
+      const size_t rbytes = (size_t)citems.size() * ct.n * 2 * 128 * 128;
+      if ((rc = grow((void **)&c->crt_res, &c->crt_res_bytes, rbytes ? rbytes : 1))) return rc;
+      if ((rc = grow((void **)&c->d_citems, &c->citems_cap, sizeof(mt_item) * (citems.empty() ? 1 : citems.size()))))
+        return rc;
+      if ((rc = grow((void **)&c->d_cwork, &c->cwork_cap, sizeof(int2) * (cwork.empty() ? 1 : cwork.size()))))
+        return rc;
+      if (!citems.empty())
+        EXC_CHECK(hipMemcpy(c->d_citems, citems.data(), sizeof(mt_item) * citems.size(), hipMemcpyHostToDevice));
+      if (!cwork.empty())
+        EXC_CHECK(hipMemcpy(c->d_cwork, cwork.data(), sizeof(int2) * cwork.size(), hipMemcpyHostToDevice));
+    }
     if ((rc = grow((void **)&c->d_ks, &c->ks_cap, sizeof(kdesc) * nk))) return rc;
     if ((rc = grow((void **)&c->d_items, &c->items_cap, sizeof(mt_item) * (items.empty() ? 1 : items.size()))))
       return rc;
@@ -653,7 +824,23 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       StageTimer t(c, capture ? -1 : EXC_STAGE_YSTAGE);
       if ((r = exc_launch_bystage(c->d, nk, c->batch_maxngp, c->d_ks, yv, c->stream))) return r;
     }
-    if (split) {
+    if (crt) {
+      // the residue GEMM and the merge write only H's (and without the addition theorem O's)
+      // APW-APW tiles: no conflict with the side branch
+      {
+        StageTimer t(c, capture ? -1 : EXC_STAGE_CRTPREP);
+        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, npa, c->stream))) return r;
+      }
+      {
+        StageTimer t(c, capture ? -1 : EXC_STAGE_CRTGEMM);
+        if ((r = exc_launch_crt_gemm(c->d, c->d_ks, c->d_citems, c->d_cwork, c->batch_ncwork, ct, npa, c->crt_res, cv,
+                                     c->stream)))
+          return r;
+      }
+      StageTimer t(c, capture ? -1 : EXC_STAGE_CRTMERGE);
+      if ((r = exc_launch_crt_merge(c->d, c->d_ks, c->d_citems, c->batch_ncitems, ct, c->crt_res, cv, c->stream)))
+        return r;
+    } else if (split) {
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_SPLIT);
         if ((r = exc_launch_split(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
@@ -770,7 +957,7 @@ int exc_ho_vjp(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int 
   EXC_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < dt.size(); ++i) dedtau[i] += dt[i];
   for (size_t i = 0; i < dv.size(); ++i) dedv[i] += dv[i];
-  return EXC_OK;
+  return exc_ctx_status(c);
 }
 
 }  // extern "C"

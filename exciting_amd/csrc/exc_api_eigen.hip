@@ -12,6 +12,28 @@ bool eig_args_ok(int nmat, int ldh, int nev, int ldv) {
   return nmat >= 0 && ldh >= (nmat > 1 ? nmat : 1) && nev >= 0 && nev <= nmat && ldv >= (nmat > 1 ? nmat : 1);
 }
 
+// Gives a worker's rocBLAS handle a workspace of its own, sized by a rocBLAS size query over
+// the calls `calls` makes (recorded, not run): no workspace is then allocated or freed by the
+// library while the handle's work is queued.  In its default mode rocBLAS grows a handle's
+// workspace at enqueue time by freeing the old buffer and allocating a larger one, with
+// earlier work of the handle still queued on its stream; with several worker streams another
+// handle's allocation could then take over that memory.  The buffer here only grows through
+// grow(), which drains the device before it frees.
+extern "C++" template <class F>
+int pin_workspace(exc_ctx::EigWorker &W, F &&calls) {
+  if (rocblas_start_device_memory_size_query(W.blas) != rocblas_status_success) return EXC_ERR_HIP;
+  calls();
+  size_t need = 0;
+  if (rocblas_stop_device_memory_size_query(W.blas, &need) != rocblas_status_success) return EXC_ERR_HIP;
+  need = std::max(need, (size_t)256);
+  const bool moved = need > W.wsb;
+  int rc = grow(&W.ws, &W.wsb, need);
+  if (rc) return rc;
+  if ((moved || !W.ws_set) && rocblas_set_workspace(W.blas, W.ws, W.wsb) != rocblas_status_success) return EXC_ERR_HIP;
+  W.ws_set = true;
+  return EXC_OK;
+}
+
 // Enqueue one generalized eigensolve and the copies of its first nev eigenpairs on the
 // worker's stream; d_ints receives (nev found, info).  Solver variants (all: Cholesky of O,
 // reduction to standard form, then)
@@ -25,6 +47,52 @@ bool eig_args_ok(int nmat, int ldh, int nev, int ldv) {
 // Variant 4 runs in two phases: phase 1 factors O (the caller checks its info on the host
 // before phase 2, so a non-positive-definite O never feeds the later steps); the other
 // variants do all their work in phase 2.
+// The rocBLAS / rocSOLVER calls of one solve (phase 1: the Cholesky factor of variant 4;
+// phase 2: the rest), used both to run them and, in a size query, to size the handle's
+// workspace; in_a: eigenvectors returned in A (all n columns) rather than in Z.
+rocblas_status eig_calls(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, double *h, double *o, int nev,
+                         int *d_ints, bool *in_a) {
+  const auto itype = rocblas_eform_ax;
+  const auto ev = rocblas_evect_original;
+  const auto up = rocblas_fill_upper;
+  rocblas_double_complex *A = (rocblas_double_complex *)h, *B = (rocblas_double_complex *)o;
+  rocblas_double_complex *Z = (rocblas_double_complex *)W.z;
+  *in_a = false;
+  if (phase == 1) return v < 4 ? rocblas_status_success : rocsolver_zpotrf(W.blas, up, nmat, B, ldh, d_ints + 1);
+  switch (v) {
+    case 0:
+      return rocsolver_zhegvdx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev,
+                               d_ints, W.w, Z, nmat, d_ints + 1);
+    case 1:
+      return rocsolver_zhegvx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev, 0.0,
+                              d_ints, W.w, Z, nmat, (rocblas_int *)W.e, d_ints + 1);
+    case 2:
+      *in_a = true;
+      return rocsolver_zhegvd(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+    case 3:
+      *in_a = true;
+      return rocsolver_zhegv(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
+    default: {
+      const rocblas_double_complex one = {1.0, 0.0};
+      rocblas_status st = rocsolver_ztrtri(W.blas, up, rocblas_diagonal_non_unit, nmat, B, ldh, d_ints + 2);   // B = U^-1
+      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // Z = H U^-1
+        st = rocblas_ztrmm(W.blas, rocblas_side_right, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nmat, &one, B, ldh, A, ldh, Z, nmat);
+      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // H = U^-H Z
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_conjugate_transpose,
+                           rocblas_diagonal_non_unit, nmat, nmat, &one, B, ldh, Z, nmat, A, ldh);
+      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // C Y = Y diag(w)
+        st = rocsolver_zheevd(W.blas, ev, up, nmat, A, ldh, W.w, W.e, d_ints + 3);
+      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // Z = U^-1 Y(:, 1:nev)
+        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
+                           nev, &one, B, ldh, A, ldh, Z, nmat);
+      return st;
+    }
+  }
+}
+
+inline bool call_ok(rocblas_status st) { return st == rocblas_status_success || st == rocblas_status_size_unchanged; }
+
 int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, double *h, double *o, int nev,
                 double *eval, double *evec, int ldv, int *d_ints) {
   if (!W.blas) {
@@ -40,59 +108,23 @@ int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, doub
   if ((rc = grow((void **)&W.w, &W.wb, sizeof(double) * nmat))) return rc;
   if ((rc = grow((void **)&W.z, &W.zb, sizeof(cplx) * (size_t)nmat * nmat))) return rc;
   if ((rc = grow((void **)&W.e, &W.eb, sizeof(double) * (nmat + 1)))) return rc;
-  const auto itype = rocblas_eform_ax;
-  const auto ev = rocblas_evect_original;
-  const auto up = rocblas_fill_upper;
-  rocblas_double_complex *A = (rocblas_double_complex *)h, *B = (rocblas_double_complex *)o;
-  rocblas_double_complex *Z = (rocblas_double_complex *)W.z;
-  rocblas_status st;
-  bool in_a = false;      // eigenvectors returned in A (all n columns) rather than in Z (v 0, 1, 4)
+  bool in_a = false;
   if (phase == 1) {
+    // the workspace of both phases, before anything of this solve is queued
+    if ((rc = pin_workspace(W, [&]() {
+           eig_calls(W, v, 1, nmat, ldh, h, o, nev, d_ints, &in_a);
+           eig_calls(W, v, 2, nmat, ldh, h, o, nev, d_ints, &in_a);
+         })))
+      return rc;
     // ZHEGVX('U') reads the upper triangles and the real parts of the diagonals only;
     // rocSOLVER's solvers and the products of variant 4 read more: lower := conj(upper)
     if ((rc = exc_launch_eig_hermfill((cplx *)h, nmat, ldh, W.s))) return rc;
     if ((rc = exc_launch_eig_hermfill((cplx *)o, nmat, ldh, W.s))) return rc;
-    if (v < 4) return EXC_OK;
-    return rocsolver_zpotrf(W.blas, up, nmat, B, ldh, d_ints + 1) == rocblas_status_success ? EXC_OK
-                                                                                           : EXC_ERR_HIP;   // O = U^H U
+    return call_ok(eig_calls(W, v, 1, nmat, ldh, h, o, nev, d_ints, &in_a)) ? EXC_OK : EXC_ERR_HIP;   // O = U^H U
   }
-  switch (v) {
-    case 0:
-      st = rocsolver_zhegvdx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev,
-                             d_ints, W.w, Z, nmat, d_ints + 1);
-      break;
-    case 1:
-      st = rocsolver_zhegvx(W.blas, itype, ev, rocblas_erange_index, up, nmat, A, ldh, B, ldh, 0.0, 0.0, 1, nev, 0.0,
-                            d_ints, W.w, Z, nmat, (rocblas_int *)W.e, d_ints + 1);
-      break;
-    case 2:
-      st = rocsolver_zhegvd(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
-      in_a = true;
-      break;
-    case 3:
-      st = rocsolver_zhegv(W.blas, itype, ev, up, nmat, A, ldh, B, ldh, W.w, W.e, d_ints + 1);
-      in_a = true;
-      break;
-    default: {
-      const rocblas_double_complex one = {1.0, 0.0};
-      st = rocsolver_ztrtri(W.blas, up, rocblas_diagonal_non_unit, nmat, B, ldh, d_ints + 2);   // B = U^-1
-      if (st == rocblas_status_success)                                                 // Z = H U^-1
-        st = rocblas_ztrmm(W.blas, rocblas_side_right, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
-                           nmat, &one, B, ldh, A, ldh, Z, nmat);
-      if (st == rocblas_status_success)                                                 // H = U^-H Z
-        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_conjugate_transpose,
-                           rocblas_diagonal_non_unit, nmat, nmat, &one, B, ldh, Z, nmat, A, ldh);
-      if (st == rocblas_status_success)                                                 // C Y = Y diag(w)
-        st = rocsolver_zheevd(W.blas, ev, up, nmat, A, ldh, W.w, W.e, d_ints + 3);
-      if (st == rocblas_status_success)                                                 // Z = U^-1 Y(:, 1:nev)
-        st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
-                           nev, &one, B, ldh, A, ldh, Z, nmat);
-      break;
-    }
-  }
-  if (st != rocblas_status_success) return EXC_ERR_HIP;
+  if (!call_ok(eig_calls(W, v, 2, nmat, ldh, h, o, nev, d_ints, &in_a))) return EXC_ERR_HIP;
   EXC_CHECK(hipMemcpyAsync(eval, W.w, sizeof(double) * nev, hipMemcpyDeviceToDevice, W.s));
-  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, in_a ? (const void *)A : (const void *)Z,
+  EXC_CHECK(hipMemcpy2DAsync(evec, 16 * (size_t)ldv, in_a ? (const void *)h : (const void *)W.z,
                              16 * (size_t)(in_a ? ldh : nmat), 16 * (size_t)nmat, nev, hipMemcpyDeviceToDevice, W.s));
   return EXC_OK;
 }
@@ -123,7 +155,9 @@ int eig_status(const int *hi, int nmat, int nev, int v) {
 //    whose rows past nmat_k stay zero, so the pad's rows never mix in;
 //  variant 7: as 6, with the tridiagonal eigenpairs of the lowest nev only, by bisection and
 //    inverse iteration (dstebz + dstein, as ZHEGVX range 'I' with abstol = the reference's
-//    default evaltol) instead of the full divide and conquer.
+//    default evaltol) instead of the full divide and conquer;
+//  variant 8: as 6, the per-k divide and conquer of all k-points on one worker stream, one
+//    after another (the serialized form of 6: the same calls on the same data).
 // H and O are read, not modified.
 constexpr int kBT = 64;            // reflectors per block of the batched back-transform
 constexpr double kAbsTol = 1e-14;  // bisection tolerance (Ha): evaltol's default, groundstate.xsd:512-513
@@ -140,7 +174,8 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   const int nb = (int)kd.size();
   if (nb == 0) return EXC_OK;
   if (nb > 65535) return EXC_ERR_ARG;
-  const int nw = std::min(nb, (int)exc_ctx::kEigStreams);
+  // variant 8: variant 6 with every k-point's divide and conquer on one worker stream
+  const int nw = v == 8 ? 1 : std::min(nb, (int)exc_ctx::kEigStreams);
   const size_t mat = (size_t)n * n, a256 = 255;
   auto al = [&](size_t b) { return (b + a256) & ~a256; };
   const size_t o_po = al(16 * mat * nb), o_pz = o_po + al(16 * mat * nb), o_d = o_pz + al(16 * mat * nb);
@@ -190,6 +225,55 @@ int eig_grouped(exc_ctx *c, int nk, const int *nmat, const int *ldh, double *con
   auto *A = (rocblas_double_complex *)ph, *B = (rocblas_double_complex *)po, *Z = (rocblas_double_complex *)pz;
   const rocblas_stride sm = (rocblas_stride)mat;
   std::vector<int> hi(3 * (size_t)nb, 0);
+  // every handle's workspace sized for its calls (the largest order n) before anything is queued
+  {
+    auto *V = (rocblas_double_complex *)(base + o_v), *G = (rocblas_double_complex *)(base + o_g);
+    auto *T = (rocblas_double_complex *)(base + o_t), *W1 = (rocblas_double_complex *)(base + o_w1);
+    auto *W2 = (rocblas_double_complex *)(base + o_w2);
+    const rocblas_double_complex zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+    const auto N = rocblas_operation_none, CT = rocblas_operation_conjugate_transpose;
+    const rocblas_stride sv = (rocblas_stride)n * kBT, st2 = (rocblas_stride)kBT * kBT;
+    const rocblas_stride sw = (rocblas_stride)kBT * nev, sxx = (rocblas_stride)n * nev;
+    const int ib = std::min(kBT, std::max(n - 1, 1));
+    if ((rc = pin_workspace(c->ew[0], [&]() {
+           rocsolver_zpotrf_strided_batched(hb, up, n, B, n, sm, info, nb);
+           rocsolver_ztrtri_strided_batched(hb, up, nu, n, B, n, sm, info + nb, nb);
+           rocblas_ztrmm_strided_batched(hb, rocblas_side_right, up, N, nu, n, n, &one, B, n, sm, A, n, sm, Z, n, sm, nb);
+           rocblas_ztrmm_strided_batched(hb, rocblas_side_left, up, CT, nu, n, n, &one, B, n, sm, Z, n, sm, A, n, sm, nb);
+           rocsolver_zhetrd_strided_batched(hb, up, n, A, n, sm, d, n, e, n, (rocblas_double_complex *)tau, n, nb);
+           if (blocked) {
+             rocblas_zgemm_strided_batched(hb, CT, N, ib, ib, n, &one, V, n, sv, V, n, sv, &zero, G, ib, st2, nb);
+             rocblas_zgemm_strided_batched(hb, CT, N, ib, nev, n, &one, V, n, sv, Z, n, sxx, &zero, W1, ib, sw, nb);
+             rocblas_zgemm_strided_batched(hb, N, N, ib, nev, ib, &one, T, ib, st2, W1, ib, sw, &zero, W2, ib, sw, nb);
+             rocblas_zgemm_strided_batched(hb, N, N, n, nev, ib, &mone, V, n, sv, W2, ib, sw, &one, Z, n, sxx, nb);
+             rocblas_ztrmm_strided_batched(hb, rocblas_side_left, up, N, nu, n, nev, &one, B, n, sm, Z, n, sxx, A, n, sxx,
+                                           nb);
+           }
+         })))
+      return rc;
+    for (int i = 1; i <= nw; ++i) {
+      exc_ctx::EigWorker &W = c->ew[i];
+      if ((rc = pin_workspace(W, [&]() {
+             if (bisect) {
+               double *wv = (double *)(base + o_wv);
+               int *ibk = (int *)(base + o_ib), *isp = (int *)(base + o_is), *ifl = (int *)(base + o_if);
+               int *si = (int *)(base + o_si);
+               rocsolver_dstebz(W.blas, rocblas_erange_index, rocblas_eorder_blocks, n, 0.0, 0.0, 1, nev, kAbsTol, d, e,
+                                si, si + 1, wv, ibk, isp, si + 2);
+               rocsolver_dstein(W.blas, n, d, e, si, wv, ibk, isp, (double *)W.z, n, ifl, si + 3);
+               return;
+             }
+             rocsolver_dstedc(W.blas, rocblas_evect_tridiagonal, n, d, e, (double *)W.z, n, info + 2 * nb);
+             if (!blocked) {
+               rocsolver_zunmtr(W.blas, rocblas_side_left, up, N, n, nev, A, n, (rocblas_double_complex *)tau,
+                                (rocblas_double_complex *)W.zc, n);
+               rocblas_ztrmm(W.blas, rocblas_side_left, up, N, nu, n, nev, &one, B, n, (rocblas_double_complex *)W.zc, n,
+                             (rocblas_double_complex *)W.zc, n);
+             }
+           })))
+        return rc;
+    }
+  }
   StageTimer t(c, EXC_STAGE_EIGEN);
   EXC_CHECK(hipMemcpyAsync(dkd, kd.data(), sizeof(eig_kdesc) * nb, hipMemcpyHostToDevice, c->stream));
   if ((rc = exc_launch_eig_pad(dkd, nb, n, ph, po, 1.0, c->stream))) return rc;

@@ -102,6 +102,10 @@ int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const d
   }
   int rc = use_device(c);
   if (rc) return rc;
+  // the uploads below are synchronous copies on the null stream, which does not wait for a
+  // non-blocking context stream: work still queued there (genrhomt, rhovalk) may read the
+  // Gaunt table and the radial buffer they overwrite, so it completes first
+  EXC_CHECK(hipStreamSynchronize(c->stream));
   // integration weights of each species' mesh, the Gaunt table (per lmaxvr)
   std::vector<double> wt((size_t)nrmtmax * ns, 0.0);
   for (int is = 0; is < ns; ++is) exc_hml_spline_weights(nrmt[is], spr + (size_t)nrmtmax * is, &wt[(size_t)nrmtmax * is]);

@@ -69,6 +69,7 @@ struct exc_dev {
   int dm_maxlo;        // largest number of lo functions of an atom (mt_dm%maxnlo)
   int *dm_lo;          // [natm][2]: global index of the atom's first lo function, their number
   int *dm_order;       // [naa]: apwalm storage column (io + apwordmax lm) of density-matrix row f
+  unsigned *miss;      // [1]: basis differences G_i - G_j met outside the first ngvec G-vectors
 };
 
 // Device-side description of one k-point of a grouped launch.
@@ -87,6 +88,11 @@ struct kdesc {
   // radial factors of apwalm for the overlap by the addition theorem (EXC_BATCH_OLP_LEGENDRE),
   // written by the grouped match: [(atom*ngp + row)*2(lmaxmat+1) + 2l + io]
   double *rf;
+  // modular split (EXC_BATCH_SPLIT_CRT): rows padded to cnp (a multiple of 128), int8 residue
+  // planes of A and Yt, layout [modulus][plane][K/16][cnp][16], and per-row scale exponents
+  int cnp, pad_;
+  void *qa, *qy;
+  int *xa, *xy;
 };
 
 // Work item of the grouped muffin-tin launch: (k, tile row, tile column, H=0 / O=1).
@@ -125,7 +131,7 @@ int exc_launch_vjp_qscale(int ngp, int nst, const cplx *evec, int ldc, const dou
 constexpr int kSplitKB = 256;       // APW functions per atom block, padded (naa <= 256)
 constexpr int kSplitPlanes = 6;     // 3 slices x (re, im)
 constexpr int kBsplitVariants = 4;
-constexpr int kEigenVariants = 8;   // generalized eigensolver paths (exc_eigen_dev)
+constexpr int kEigenVariants = 9;   // generalized eigensolver paths (exc_eigen_dev)
 size_t exc_split_plane_elems(int ngp, int natm);   // bf16 elements of one operand's planes
 int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_bsplit_tile(int variant);
@@ -134,6 +140,30 @@ int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
 int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
+// modular split of the muffin-tin contraction (k_crt.hip)
+constexpr int kMaxMod = 14;          // moduli available: 256, 255, 253, 251, 247, 241, 239, 233, ...
+constexpr int kCrtLoDigits = 6;      // mixed-radix digits of the low part (their product < 2^53)
+constexpr int kCrtMaxUsed = 12;      // beyond 12 moduli beta > 46: row integers (and re +- im) no
+                                     // longer exact in FP64, the high part of the reconstruction
+                                     // no longer below 2^53
+constexpr int kCrtVariants = 5;      // tile / staging variants of the residue GEMM
+struct CrtTab {                      // passed by value to the kernels (kernel-argument segment)
+  int n, beta, nlo, pad_;
+  int m[kMaxMod];
+  float finv[kMaxMod];
+  double dminv[kMaxMod];
+  int w[kMaxMod][kMaxMod];           // M_j mod m_l, M_j = m_0 ... m_(j-1)
+  int v[kMaxMod];                    // M_l^-1 mod m_l
+  double mk;                         // M_nlo
+};
+void exc_crt_tables(int n, CrtTab *t);
+inline int exc_crt_kpad(int ktot) { return (ktot + 63) / 64 * 64; }
+int exc_crt_waves(int variant, int *wr, int *wc);
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, int npa, hipStream_t s);
+int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
+                        const CrtTab &t, int npa, signed char *res, int variant, hipStream_t s);
+int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
+                         const signed char *res, int variant, hipStream_t s);
 // generalized eigensolve helpers (k_eigen.hip)
 struct eig_kdesc {   // one k-point of a grouped solve: nmat n, device arrays
   const cplx *h, *o;

@@ -32,6 +32,12 @@ struct exc_ctx {
   // staging for the host-pointer entry points: device buffers and a pinned host buffer
   void *stage;
   size_t stage_bytes;
+  // apwalm of the last host-array exc_match, kept on the device for the setup calls that
+  // pass apwalm = NULL (match and the setup of one k-point without a second PCIe copy)
+  cplx *apw_res;
+  size_t apw_res_bytes;
+  int apw_res_ngp, apw_res_ngkmax;
+  bool apw_res_valid;
   void *hstage;
   size_t hstage_bytes;
   std::vector<void *> owned;
@@ -63,6 +69,20 @@ struct exc_ctx {
   mt_item *d_oitems;        // overlap tiles of the addition-theorem kernel
   size_t oitems_cap;
   int batch_noitems;
+  // modular split (EXC_BATCH_SPLIT_CRT): moduli count, GEMM variant, residue planes, row scale
+  // exponents, residues of every (tile, modulus), the tile and (tile, modulus) work lists
+  int crt_nmod, crt_variant;
+  void *crt_q;
+  size_t crt_q_bytes;
+  int *crt_x;
+  size_t crt_x_bytes;
+  signed char *crt_res;
+  size_t crt_res_bytes;
+  mt_item *d_citems;
+  size_t citems_cap;
+  int2 *d_cwork;
+  size_t cwork_cap;
+  int batch_ncitems, batch_ncwork, batch_maxcnp;
   std::vector<char> batch_sig;   // the batch the descriptors / graph were built for
   int batch_nitems, batch_nblocks, batch_maxngp, batch_nk;
   hipGraph_t graph;
@@ -80,6 +100,9 @@ struct exc_ctx {
     size_t zcb;
     double *e;              // n-length workspace (tridiagonal off-diagonal, ifail)
     size_t eb;
+    void *ws;               // the rocBLAS handle's workspace, owned here (rocblas_set_workspace)
+    size_t wsb;
+    bool ws_set;
     hipEvent_t done;
   };
   static constexpr int kEigStreams = 4;

@@ -1,0 +1,498 @@
+// Modular split of the muffin-tin contraction on gfx950 (EXC_BATCH_SPLIT_CRT).
+//
+// H_MT = conj(A) Yt^T (and O_MT = conj(A) A^T) from exact integer products instead of FP64
+// MFMA, the other terms of the setup staying FP64:
+//
+//  scale : per row i of an operand X (N x K complex, K = natm naa) the exponent e_i with
+//          |X_i|_2 < 2^e_i and s_i = beta - e_i, beta the largest with 2^(2 beta + 2) <= P,
+//          P = m_0 ... m_(n-1) the product of the pairwise coprime moduli (256, 255, 253, ...).
+//  split : X'_ik = rint(Re X_ik 2^s_i) + i rint(Im X_ik 2^s_i), exact integers below 2^beta,
+//          and per modulus m the symmetric residues of Re X', Im X' and of Re X' -+ Im X'
+//          (Gauss's third product) as int8 planes, layout [m][plane][K/16][row][16 bytes]:
+//          one 16-byte k-chunk of 64 consecutive rows is one 1 KiB wave-wide DMA.
+//  gemm  : per modulus and 128 x 128 tile, P1 = ar br^T, P2 = ai bi^T, P3 = as bs^T by
+//          v_mfma_i32_32x32x32_i8 with exact int32 accumulation (|P| <= K 128^2 < 2^31 / 3),
+//          Re = P1 + P2, Im = P3 - P1 + P2, reduced modulo m to int8 residues.
+//  merge : per tile element Garner's mixed-radix digits of Re', Im' in [-P/2, P/2) (exact:
+//          |Re'|, |Im'| <= |A'_i| |B'_j| < P/2), the FP64 value correctly rounded as
+//          fma(x_hi, M_6, x_lo), times 2^-(s_i + t_j); then the interstitial term and the
+//          Hermitian mirror exactly as the FP64 epilogue (hmlistln.f90:74-84,
+//          olpistln.f90:69-82, hamiltonoverlapocopy_UL.f90:15-27).
+// Integer sums do not depend on their order, so every variant of every kernel gives the
+// same bits; the only rounding between the inputs and H_MT is that of the operands to
+// beta-bit integers and the final conversion.
+#include "setup_dev.h"
+#include "mt_dma.h"
+
+namespace {
+
+using setup_dev::GTab;
+using setup_dev::lookup_g;
+using setup_dev::stage_barrier;
+using setup_dev::wait_vm_lgkm0;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+constexpr int kTile = 128;       // tile edge of the gemm and merge kernels; rows padded to it
+constexpr int kResTile = 2 * kTile * kTile;   // residue bytes of one (tile, modulus): Re and Im
+
+// symmetric residue of an exact integer held in a double, |x| < 2^52
+__device__ __forceinline__ int dres(double x, int m, double minv) {
+  if (m == 256) return (int)(x - 256.0 * floor((x + 128.0) * (1.0 / 256.0)));
+  return (int)(x - (double)m * rint(x * minv));
+}
+
+// symmetric residue of a small integer, |x| < 2^22 (float quotient exact to well within
+// the 1/(2m) margin of the rounding)
+__device__ __forceinline__ int fres(int x, int m, float finv) {
+  if (m == 256) return (int)(signed char)(x & 255);
+  return x - m * (int)rintf((float)x * finv);
+}
+
+// ---------------------------------------------------------------------------------------
+// scale: one workgroup per (k, operand, 64 rows); wave w sums |x|^2 over its quarter of K,
+// the quarters are added in wave order (a fixed order: the exponents do not depend on timing)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict__ ks, int ktot, int beta) {
+  const kdesc k = ks[blockIdx.z >> 1];
+  const int op = blockIdx.z & 1;              // 0: A (apwalm), 1: Yt
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane;
+  __shared__ double part[4][64];
+  if (blockIdx.x * 64 >= k.cnp) return;       // uniform over the workgroup
+  const cplx *src = op ? k.yt : k.apwalm;
+  const int ld = op ? k.ldy : k.ldg;
+  const int kq = (ktot + 3) / 4, k0 = wave * kq, k1 = min(ktot, k0 + kq);
+  double acc = 0.0;
+  if (row < k.ngp) {
+    for (int c = k0; c < k1; ++c) {
+      const cplx v = src[row + (size_t)c * ld];
+      acc = fma(v.re, v.re, acc);
+      acc = fma(v.im, v.im, acc);
+    }
+  }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0) {
+    const double s2 = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    int e = 0;
+    if (s2 > 0.0) frexp(sqrt(s2), &e);       // |X_i| < 2^e
+    (op ? k.xy : k.xa)[row] = row < k.ngp ? beta - e : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// split: 4 waves; wave w of workgroup (bx, by) converts rows 64 bx .. +63 of k-chunk
+// 4 by + w (16 values of k) into the residue planes of every modulus
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t,
+                                                        int npa) {
+  const kdesc k = ks[blockIdx.z >> 1];
+  const int op = blockIdx.z & 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane;
+  const int kc = blockIdx.y * 4 + wave;
+  if (blockIdx.x * 64 >= k.cnp || kc * 16 >= kp) return;
+  const int np = op ? 3 : npa;                 // planes: re, im, re -+ im (and re + im of A for O)
+  const cplx *src = op ? k.yt : k.apwalm;
+  const int ld = op ? k.ldy : k.ldg;
+  const int s = row < k.ngp ? (op ? k.xy : k.xa)[row] : 0;
+  double xr[16], xi[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = kc * 16 + q;
+    cplx v = {0.0, 0.0};
+    if (row < k.ngp && c < ktot) v = src[row + (size_t)c * ld];
+    xr[q] = rint(ldexp(v.re, s));
+    xi[q] = rint(ldexp(v.im, s));
+  }
+  signed char *base = (signed char *)(op ? k.qy : k.qa);
+  const size_t pstride = (size_t)kp * k.cnp;            // bytes of one plane
+  const size_t off = ((size_t)kc * k.cnp + row) * 16;
+  for (int l = 0; l < t.n; ++l) {
+    const int m = t.m[l];
+    const double mi = t.dminv[l];
+    for (int p = 0; p < np; ++p) {
+      unsigned w4[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        unsigned u = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int q = 4 * g + b;
+          // plane 2 is the left operand's re - im for A (conj(a)) and re + im for Yt;
+          // plane 3 (A only, for O) is re + im
+          const double x = p == 0 ? xr[q] : p == 1 ? xi[q] : (p == 2 && op == 0) ? xr[q] - xi[q] : xr[q] + xi[q];
+          u |= (unsigned)(dres(x, m, mi) & 255) << (8 * b);
+        }
+        w4[g] = u;
+      }
+      *(uint4 *)(base + (size_t)(l * np + p) * pstride + off) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// gemm: one (tile, modulus) per workgroup on an XCD-mapped grid (workgroup b runs on XCD b % 8
+// and takes the next work of that XCD's contiguous eighth of the list); WR x WC waves with
+// (128 / WR) x (128 / WC) wave tiles of 32 x 32 MFMA blocks; LDS stages of BK bytes of k
+// (all three planes of both panels) filled by DMA, NST stages in flight.
+// ---------------------------------------------------------------------------------------
+template <int BK, int NST, int WR, int WC>
+struct GemmCfg {
+  static constexpr int kWaves = WR * WC, kThreads = 64 * kWaves;
+  static constexpr int KC = BK / 16;                          // 16-byte k-chunks per stage
+  static constexpr int PB = KC * kTile * 16;                  // bytes of one plane of one panel
+  static constexpr int ABYTES = 3 * PB;
+  static constexpr int NL = 2 * 3 * KC * (kTile / 64);        // wave-wide 1 KiB DMAs per stage
+  static constexpr int L = NL / kWaves;
+  static constexpr int MI = kTile / WR / 32, NJ = kTile / WC / 32;
+  static_assert(NL % kWaves == 0 && BK % 32 == 0, "DMA split / MFMA k-step");
+  struct __attribute__((aligned(16))) Stage {
+    unsigned char a[ABYTES];
+    unsigned char b[ABYTES];
+  };
+};
+
+template <int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_issue(typename GemmCfg<BK, NST, WR, WC>::Stage *st, const unsigned char *pa,
+                                           const unsigned char *pb, const int (&pla)[3], const int (&plb)[3],
+                                           size_t pstride, size_t chunk_stride, int row0, int col0, int kc0) {
+  using C = GemmCfg<BK, NST, WR, WC>;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+  for (int q = 0; q < C::L; ++q) {
+    const int j = wave + C::kWaves * q;               // 0 .. NL-1: (panel, plane, chunk, 64-row half)
+    const bool isb = j >= C::NL / 2;
+    const int jj = isb ? j - C::NL / 2 : j;
+    const int h = jj & 1, kc = (jj >> 1) % C::KC, p = (jj >> 1) / C::KC;
+    const unsigned char *src = (isb ? pb + (size_t)plb[p] * pstride : pa + (size_t)pla[p] * pstride) +
+                               (size_t)(kc0 + kc) * chunk_stride + (size_t)((isb ? col0 : row0) + 64 * h + lane) * 16;
+    unsigned char *dst = (isb ? st->b : st->a) + (p * C::KC + kc) * (kTile * 16) + h * 1024;
+    __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)dst, 16, 0, 0);
+  }
+}
+
+template <int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_compute(const typename GemmCfg<BK, NST, WR, WC>::Stage *st,
+                                             v16i (&acc)[3][GemmCfg<BK, NST, WR, WC>::MI][GemmCfg<BK, NST, WR, WC>::NJ],
+                                             int wr, int wc) {
+  using C = GemmCfg<BK, NST, WR, WC>;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 31, hk = lane >> 5;
+#pragma unroll
+  for (int s = 0; s < BK / 32; ++s) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      v4i af[C::MI], bf[C::NJ];
+#pragma unroll
+      for (int i = 0; i < C::MI; ++i)
+        af[i] = *(const v4i *)(st->a + ((p * C::KC + 2 * s + hk) * kTile + wr * (32 * C::MI) + 32 * i + r) * 16);
+#pragma unroll
+      for (int j = 0; j < C::NJ; ++j)
+        bf[j] = *(const v4i *)(st->b + ((p * C::KC + 2 * s + hk) * kTile + wc * (32 * C::NJ) + 32 * j + r) * 16);
+#pragma unroll
+      for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+        for (int j = 0; j < C::NJ; ++j) acc[p][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[p][i][j], 0, 0, 0);
+    }
+  }
+}
+
+template <int BK, int NST, int WR, int WC, int MINW>
+__global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdesc *__restrict__ ks,
+                                                                   const mt_item *__restrict__ items,
+                                                                   const int2 *__restrict__ work, int nwork, int kp,
+                                                                   CrtTab t, int npa, signed char *__restrict__ res) {
+  using C = GemmCfg<BK, NST, WR, WC>;
+  using St = typename C::Stage;
+  EXC_DMA_STAGES(St, NST)
+  const int chunk = (nwork + 7) / 8;
+  const int x = blockIdx.x % 8, qb = blockIdx.x / 8;
+  const int wi = x * chunk + qb;
+  if (qb >= chunk || wi >= nwork) return;
+  const int2 wk = work[wi];                      // (item, modulus)
+  const mt_item it = items[wk.x];
+  const kdesc k = ks[it.k];
+  const int l = wk.y;
+  const size_t pstride = (size_t)kp * k.cnp, chunk_stride = (size_t)k.cnp * 16;
+  // left operand conj(A): planes re, im, re - im; right operand Yt (H) or A (O): re, im, re + im
+  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(l * npa) * pstride;
+  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(l * npa) * pstride
+                                     : (const unsigned char *)k.qy + (size_t)(l * 3) * pstride;
+  const int pla[3] = {0, 1, 2};
+  const int plb[3] = {0, 1, it.which ? 3 : 2};
+  const int row0 = it.ti * kTile, col0 = it.tj * kTile;
+  const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+  v16i acc[3][C::MI][C::NJ];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+      for (int j = 0; j < C::NJ; ++j) acc[p][i][j] = v16i{};
+  St *st[4] = {&s0_, &s1_, (St *)&s2_, (St *)&s3_};
+  const int nsteps = kp / BK;
+#pragma unroll
+  for (int u = 0; u < NST - 1; ++u)
+    if (u < nsteps) gemm_issue<BK, NST, WR, WC>(st[u], pa, pb, pla, plb, pstride, chunk_stride, row0, col0, u * C::KC);
+  for (int base = 0; base < nsteps; base += NST) {
+#pragma unroll
+    for (int u = 0; u < NST; ++u) {
+      const int step = base + u;
+      if (step < nsteps) {
+        // stage `step` has landed once at most the DMAs of the NST-2 later stages are out
+        if (step + NST - 2 < nsteps)
+          wait_vm_lgkm0<(NST - 2) * C::L>();
+        else
+          wait_vm_lgkm0<0>();
+        stage_barrier();
+        if (step + NST - 1 < nsteps)
+          gemm_issue<BK, NST, WR, WC>(st[(u + NST - 1) % NST], pa, pb, pla, plb, pstride, chunk_stride, row0, col0,
+                                      (step + NST - 1) * C::KC);
+        gemm_compute<BK, NST, WR, WC>(st[u], acc, wr, wc);
+      }
+    }
+  }
+  // Re = P1 + P2, Im = P3 - P1 + P2 modulo m; residues of one 32 x 32 block and lane are 16
+  // contiguous bytes: res[work][wave][block][re, im][lane][16]
+  const int m = t.m[l];
+  const double mi = t.dminv[l];
+  const int lane = threadIdx.x & 63;
+  signed char *out = res + ((size_t)wk.x * t.n + l) * kResTile;
+#pragma unroll
+  for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+    for (int j = 0; j < C::NJ; ++j) {
+      unsigned ur[4] = {0u, 0u, 0u, 0u}, ui[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int p1 = acc[0][i][j][v], p2 = acc[1][i][j][v], p3 = acc[2][i][j][v];
+        const int re = dres((double)(p1 + p2), m, mi);
+        const int im = dres((double)((p3 - p1) + p2), m, mi);
+        ur[v >> 2] |= (unsigned)(re & 255) << (8 * (v & 3));
+        ui[v >> 2] |= (unsigned)(im & 255) << (8 * (v & 3));
+      }
+      const int blk = wave * (C::MI * C::NJ) + i * C::NJ + j;
+      *(uint4 *)(out + ((size_t)(2 * blk) * 64 + lane) * 16) = make_uint4(ur[0], ur[1], ur[2], ur[3]);
+      *(uint4 *)(out + ((size_t)(2 * blk + 1) * 64 + lane) * 16) = make_uint4(ui[0], ui[1], ui[2], ui[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// merge: one tile per workgroup (256 threads).  Thread handles the 32 x 32 blocks' lanes of
+// one gemm layout: block index b = 0 .. 15 (wave-major as the gemm wrote them) decoded with
+// that variant's wave grid (wr, wc, i, j), lane, and the 16 values v of the lane.
+// ---------------------------------------------------------------------------------------
+template <int NM>
+__device__ __forceinline__ void garner(const int (&r)[NM], const CrtTab &t, long long &hi, long long &lo) {
+  int d[NM];
+  d[0] = r[0];
+#pragma unroll
+  for (int l = 1; l < NM; ++l) {
+    int acc = r[l];
+#pragma unroll
+    for (int j = 0; j < l; ++j) acc -= d[j] * t.w[l][j];
+    const int u = fres(acc, t.m[l], t.finv[l]);
+    d[l] = fres(u * t.v[l], t.m[l], t.finv[l]);
+  }
+  constexpr int NLO = NM < kCrtLoDigits ? NM : kCrtLoDigits;
+  lo = 0;
+#pragma unroll
+  for (int j = NLO - 1; j >= 0; --j) lo = lo * t.m[j] + d[j];
+  hi = 0;
+#pragma unroll
+  for (int j = NM - 1; j >= NLO; --j) hi = hi * t.m[j] + d[j];
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                        CrtTab t, const signed char *__restrict__ res, int wr_n,
+                                                        int wc_n, GTab gt) {
+  const mt_item it = items[blockIdx.x];
+  const kdesc k = ks[it.k];
+  const int ngp = k.ngp;
+  const int row0 = it.ti * kTile, col0 = it.tj * kTile;
+  __shared__ int gr[kTile][3], gc[kTile][3];
+  __shared__ double vr[kTile][3], vc[kTile][3];
+  __shared__ int er[kTile], ec[kTile];
+  for (int e = threadIdx.x; e < 2 * kTile; e += 256) {
+    const bool isr = e < kTile;
+    const int q = isr ? e : e - kTile;
+    const int idx = (isr ? row0 : col0) + q;
+    int g0 = 0, g1 = 0, g2 = 0, ex = 0;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (idx < ngp) {
+      const int ig = k.igpig[idx] - 1;
+      g0 = gt.ivg[3 * ig]; g1 = gt.ivg[3 * ig + 1]; g2 = gt.ivg[3 * ig + 2];
+      v0 = k.vgpc[3 * idx]; v1 = k.vgpc[3 * idx + 1]; v2 = k.vgpc[3 * idx + 2];
+      ex = isr ? k.xa[idx] : (it.which ? k.xa[idx] : k.xy[idx]);
+    }
+    int(&g)[kTile][3] = isr ? gr : gc;
+    double(&vv)[kTile][3] = isr ? vr : vc;
+    g[q][0] = g0; g[q][1] = g1; g[q][2] = g2;
+    vv[q][0] = v0; vv[q][1] = v1; vv[q][2] = v2;
+    (isr ? er : ec)[q] = ex;
+  }
+  __syncthreads();
+  const int mi_n = kTile / wr_n / 32, nj_n = kTile / wc_n / 32, per_wave = mi_n * nj_n;
+  const signed char *base = res + (size_t)blockIdx.x * NM * kResTile;
+  cplx *out = it.which ? k.o : k.h;
+  for (int c = threadIdx.x; c < 16 * 64; c += 256) {
+    const int blk = c >> 6, lane = c & 63;
+    const int wave = blk / per_wave, bi = (blk % per_wave) / nj_n, bj = blk % nj_n;
+    const int wr = wave / wc_n, wc = wave % wc_n;
+    const int cl = wc * (32 * nj_n) + 32 * bj + (lane & 31);
+    const int col = col0 + cl;
+    if (col >= ngp) continue;
+    uint4 rr[NM], ri[NM];
+#pragma unroll
+    for (int l = 0; l < NM; ++l) {
+      rr[l] = *(const uint4 *)(base + (size_t)l * kResTile + ((size_t)(2 * blk) * 64 + lane) * 16);
+      ri[l] = *(const uint4 *)(base + (size_t)l * kResTile + ((size_t)(2 * blk + 1) * 64 + lane) * 16);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int rl = wr * (32 * mi_n) + 32 * bi + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+      const int row = row0 + rl;
+      if (row > col) continue;
+      int dr[NM], di[NM];
+#pragma unroll
+      for (int l = 0; l < NM; ++l) {
+        const unsigned wr4 = (v >> 2) == 0 ? rr[l].x : (v >> 2) == 1 ? rr[l].y : (v >> 2) == 2 ? rr[l].z : rr[l].w;
+        const unsigned wi4 = (v >> 2) == 0 ? ri[l].x : (v >> 2) == 1 ? ri[l].y : (v >> 2) == 2 ? ri[l].z : ri[l].w;
+        dr[l] = (int)(signed char)((wr4 >> (8 * (v & 3))) & 255);
+        di[l] = (int)(signed char)((wi4 >> (8 * (v & 3))) & 255);
+      }
+      long long hr, lr, hi, li;
+      garner<NM>(dr, t, hr, lr);
+      garner<NM>(di, t, hi, li);
+      const int sc = -(er[rl] + ec[cl]);
+      double re = ldexp(fma((double)hr, t.mk, (double)lr), sc);
+      double im = ldexp(fma((double)hi, t.mk, (double)li), sc);
+      const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
+      if (ig) {
+        const cplx th = gt.cfunig[ig - 1];
+        if (it.which) {
+          re += th.re;
+          im += th.im;
+        } else {
+          const cplx vv = gt.veffig[ig - 1];
+          const double t1 = 0.5 * (vr[rl][0] * vc[cl][0] + vr[rl][1] * vc[cl][1] + vr[rl][2] * vc[cl][2]);
+          re += vv.re + t1 * th.re;
+          im += vv.im + t1 * th.im;
+        }
+      }
+      out[row + (size_t)col * k.ldh] = cplx{re, im};
+      if (row < col) out[col + (size_t)row * k.ldh] = cplx{re, -im};
+    }
+  }
+}
+
+template <int NM>
+int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
+                   const signed char *res, int wr, int wc, hipStream_t s) {
+  hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, t, res, wr, wc,
+                     setup_dev::make_gtab(d));
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+template <int BK, int NST, int WR, int WC, int MINW>
+int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t, int npa,
+                  signed char *res, hipStream_t s) {
+  const int chunk = (nwork + 7) / 8;
+  hipLaunchKernelGGL((crt_gemm_kernel<BK, NST, WR, WC, MINW>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks, items,
+                     work, nwork, kp, t, npa, res);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+}  // namespace
+
+void exc_crt_tables(int n, CrtTab *t) {
+  static const int kMods[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
+  *t = CrtTab{};
+  t->n = n;
+  for (int l = 0; l < n; ++l) {
+    t->m[l] = kMods[l];
+    t->dminv[l] = 1.0 / kMods[l];
+    t->finv[l] = 1.0f / (float)kMods[l];
+  }
+  // beta: the largest b with 2^(2b + 2) <= P, from log2 P with a margin against its rounding
+  double lp = 0.0;
+  for (int l = 0; l < n; ++l) lp += log2((double)kMods[l]);
+  t->beta = (int)floor((lp - 2.0 - 1e-9) / 2.0);
+  // Garner tables: w[l][j] = M_j mod m_l, v[l] = M_l^-1 mod m_l (M_j = m_0 ... m_(j-1))
+  for (int l = 1; l < n; ++l) {
+    long long Ml = 1;
+    for (int j = 0; j < l; ++j) {
+      t->w[l][j] = (int)(Ml % kMods[l]);
+      Ml = (Ml * kMods[j]) % kMods[l];
+    }
+    int inv = 0;
+    for (int x = 1; x < kMods[l]; ++x)
+      if ((Ml * x) % kMods[l] == 1) {
+        inv = x;
+        break;
+      }
+    t->v[l] = inv;
+  }
+  t->nlo = n < kCrtLoDigits ? n : kCrtLoDigits;
+  double mk = 1.0;
+  for (int j = 0; j < t->nlo; ++j) mk *= kMods[j];   // < 2^53: exact
+  t->mk = mk;
+}
+
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, int npa, hipStream_t s) {
+  if (nk <= 0 || maxnp <= 0) return EXC_OK;
+  const int ktot = d.natm * d.naa, kp = exc_crt_kpad(ktot);
+  hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, 2 * nk), dim3(256), 0, s, ks, ktot, t.beta);
+  EXC_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t, npa);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+// variants: (k bytes per LDS stage, stages, wave grid, minimum waves per SIMD)
+//   0: 32, 3, 2 x 2 (64 x 64 wave tiles), 1    1: 64, 2, 2 x 2, 1
+//   2: 32, 3, 2 x 4 (64 x 32 wave tiles), 2    3: 64, 2, 2 x 4, 2    4: 32, 4, 2 x 4, 2
+int exc_crt_waves(int variant, int *wr, int *wc) {
+  switch (variant) {
+    case 0: case 1: *wr = 2; *wc = 2; return EXC_OK;
+    case 2: case 3: case 4: *wr = 2; *wc = 4; return EXC_OK;
+    default: return EXC_ERR_ARG;
+  }
+}
+
+int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
+                        const CrtTab &t, int npa, signed char *res, int variant, hipStream_t s) {
+  if (nwork <= 0) return EXC_OK;
+  const int kp = exc_crt_kpad(d.natm * d.naa);
+  switch (variant) {
+    case 0: return launch_gemm_t<32, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 1: return launch_gemm_t<64, 2, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 2: return launch_gemm_t<32, 3, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 3: return launch_gemm_t<64, 2, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 4: return launch_gemm_t<32, 4, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    default: return EXC_ERR_ARG;
+  }
+}
+
+int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
+                         const signed char *res, int variant, hipStream_t s) {
+  if (nitems <= 0) return EXC_OK;
+  int wr = 0, wc = 0;
+  if (exc_crt_waves(variant, &wr, &wc)) return EXC_ERR_ARG;
+  switch (t.n) {
+    case 8: return launch_merge_t<8>(d, ks, items, nitems, t, res, wr, wc, s);
+    case 9: return launch_merge_t<9>(d, ks, items, nitems, t, res, wr, wc, s);
+    case 10: return launch_merge_t<10>(d, ks, items, nitems, t, res, wr, wc, s);
+    case 11: return launch_merge_t<11>(d, ks, items, nitems, t, res, wr, wc, s);
+    case 12: return launch_merge_t<12>(d, ks, items, nitems, t, res, wr, wc, s);
+    default: return EXC_ERR_ARG;
+  }
+}

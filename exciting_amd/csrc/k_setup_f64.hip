@@ -23,16 +23,18 @@ __global__ __launch_bounds__(64 * WR * WC) void mt_kernel(int ngp, int ktot, int
                                                           int ldg, const cplx *__restrict__ yt, int ldy,
                                                           const int *__restrict__ igpig, const double *__restrict__ vgpc,
                                                           GTab gt, cplx *__restrict__ h, cplx *__restrict__ o, int ldh,
-                                                          const int *__restrict__ kperm) {
+                                                          const int *__restrict__ kperm, int nsel, int which0) {
   using T = MtTile<BM, BK, WR, WC>;
   __shared__ typename T::Shared sh;
-  // work item: (tile, H or O); H and O of one tile are adjacent
+  // work item: (tile, H or O); with both (nsel 2) H and O of one tile are adjacent, else
+  // every item is of matrix which0
   const int ntri = ntile * (ntile + 1) / 2;
   const int item = blockIdx.x;
-  if (item >= 2 * ntri) return;
+  if (item >= nsel * ntri) return;
   int ti, tj;
-  tri_decode(item >> 1, ti, tj);
-  T::run(sh, ngp, ktot, apwalm, ldg, yt, ldy, igpig, vgpc, gt, h, o, ldh, kperm, ti, tj, item & 1);
+  tri_decode(item / nsel, ti, tj);
+  T::run(sh, ngp, ktot, apwalm, ldg, yt, ldy, igpig, vgpc, gt, h, o, ldh, kperm, ti, tj,
+         nsel == 2 ? (item & 1) : which0);
 }
 
 template <int BM, int BN, int BK, int WR, int WC>
@@ -44,13 +46,17 @@ int launch_ystage_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx
   return EXC_OK;
 }
 
+// the matrices to compute: H when h is given, O when o is given
+inline int sel_count(const cplx *h, const cplx *o) { return (h ? 1 : 0) + (o ? 1 : 0); }
+
 template <int BM, int BK, int WR, int WC>
 int launch_mt_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy, const int *igpig,
                 const double *vgpc, cplx *h, cplx *o, int ldh, hipStream_t s) {
   const int ntile = (ngp + BM - 1) / BM;
-  const int nitem = ntile * (ntile + 1);
+  const int nsel = sel_count(h, o);
+  const int nitem = nsel * ntile * (ntile + 1) / 2;
   hipLaunchKernelGGL((mt_kernel<BM, BK, WR, WC>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp, d.natm * d.naa, ntile,
-                     apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm);
+                     apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm, nsel, h ? 0 : 1);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -85,27 +91,29 @@ __global__ __launch_bounds__(64 * WR * WC) void mt_dma_kernel(int ngp, int ktot,
                                                               const int *__restrict__ igpig,
                                                               const double *__restrict__ vgpc, GTab gt,
                                                               cplx *__restrict__ h, cplx *__restrict__ o, int ldh,
-                                                              const int *__restrict__ kperm) {
+                                                              const int *__restrict__ kperm, int nsel, int which0) {
   using T = MtTileDMA<BM, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   __shared__ typename T::Tables tb;
   EXC_DMA_STAGES(St, NST)
   const int ntri = ntile * (ntile + 1) / 2;
   const int item = blockIdx.x;
-  if (item >= 2 * ntri) return;
+  if (item >= nsel * ntri) return;
   int ti, tj;
-  tri_decode(item >> 1, ti, tj);
+  tri_decode(item / nsel, ti, tj);
   T::run(tb, EXC_DMA_STAGE_ARGS(St), ngp, ktot, apwalm, ldg, yt, ldy, igpig, vgpc, gt, h, o, ldh, kperm, ti, tj,
-         item & 1);
+         nsel == 2 ? (item & 1) : which0);
 }
 
 template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_mt_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy, const int *igpig,
                     const double *vgpc, cplx *h, cplx *o, int ldh, hipStream_t s) {
   const int ntile = (ngp + BM - 1) / BM;
-  const int nitem = ntile * (ntile + 1);
+  const int nsel = sel_count(h, o);
+  const int nitem = nsel * ntile * (ntile + 1) / 2;
   hipLaunchKernelGGL((mt_dma_kernel<BM, BK, NST, WR, WC, G3>), dim3(nitem), dim3(64 * WR * WC), 0, s, ngp,
-                     d.natm * d.naa, ntile, apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm);
+                     d.natm * d.naa, ntile, apwalm, ldg, yt, ldy, igpig, vgpc, make_gtab(d), h, o, ldh, d.kperm, nsel,
+                     h ? 0 : 1);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -136,7 +144,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
 
 int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy, const int *igpig,
                   const double *vgpc, cplx *h, cplx *o, int ldh, int variant, hipStream_t s) {
-  if (ngp <= 0) return EXC_OK;
+  if (ngp <= 0 || (!h && !o)) return EXC_OK;
   switch (variant) {
     case 0: return launch_mt_t<128, 16, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);
     case 1: return launch_mt_t<64, 16, 2, 2>(d, ngp, apwalm, ldg, yt, ldy, igpig, vgpc, h, o, ldh, s);

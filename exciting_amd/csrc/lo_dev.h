@@ -78,14 +78,19 @@ __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, in
         orr = fma(a.re, w, orr);
         oi = fma(-a.im, w, oi);
       }
-      // (row, lo): O primary, H = conj(H(lo, row)) — contiguous over the workgroup's rows
+      // (row, lo), contiguous over the workgroup's rows: O primary, H = conj(H(lo, row));
+      // (lo, row): H primary (loa . A^T), O = conj(O(row, lo)).  h or o is null when only
+      // the other matrix is set up (a uniform branch).
       const size_t rl = (size_t)i + (nlo0 + q) * ldh;
-      h[rl] = cplx{hr[q], -hi[q]};
-      o[rl] = cplx{orr, oi};
-      // (lo, row): H primary (loa . A^T), O = conj(O(row, lo))
       const size_t lr = (nlo0 + q) + (size_t)i * ldh;
-      h[lr] = cplx{hr[q], hi[q]};
-      o[lr] = cplx{orr, -oi};
+      if (h) {
+        h[rl] = cplx{hr[q], -hi[q]};
+        h[lr] = cplx{hr[q], hi[q]};
+      }
+      if (o) {
+        o[rl] = cplx{orr, oi};
+        o[lr] = cplx{orr, -oi};
+      }
     }
   }
 }
@@ -111,8 +116,8 @@ __device__ __forceinline__ void lo_lo_body(int r, int c, int ngp, int nlotot, in
     }
   }
   const size_t idx = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;
-  h[idx] = hv;
-  o[idx] = ov;
+  if (h) h[idx] = hv;
+  if (o) o[idx] = ov;
 }
 
 }  // namespace lo_dev

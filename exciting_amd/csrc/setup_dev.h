@@ -21,6 +21,7 @@ struct GTab {
   const int *ivgig;    // box
   const cplx *cfunig;  // [ngrtot]
   const cplx *veffig;  // [ngvec]
+  unsigned *miss;      // count of basis differences found outside the G set (exc_ctx_status)
   int ngvec;
   int lo0, lo1, lo2, n0, n1, n2;
 };
@@ -31,19 +32,26 @@ inline GTab make_gtab(const exc_dev &d) {
   t.ivgig = d.ivgig;
   t.cfunig = d.cfunig;
   t.veffig = d.veffig;
+  t.miss = d.miss;
   t.ngvec = d.ngvec;
   t.lo0 = d.box_lo[0]; t.lo1 = d.box_lo[1]; t.lo2 = d.box_lo[2];
   t.n0 = d.box_n[0]; t.n1 = d.box_n[1]; t.n2 = d.box_n[2];
   return t;
 }
 
-// 1-based index of G = (d0,d1,d2) in the G set, 0 if outside the box or beyond ngvec
-// (the "ig > 0 .and. ig <= ngvec" test of hmlistln.f90:77)
+// 1-based index of G = (d0,d1,d2) = G_i - G_j of two basis vectors in the G set, 0 if outside
+// the box or beyond ngvec (the "ig > 0 .and. ig <= ngvec" test of hmlistln.f90:77).  The
+// reference never reaches that case: it stops before any setup when 2 gkmax > gmaxvr
+// (init1.f90:340-347), which is what keeps every basis difference inside the first ngvec
+// G-vectors.  A miss is therefore counted, and exc_ctx_status reports it as an error.
 __device__ __forceinline__ int lookup_g(const GTab &t, int d0, int d1, int d2) {
   const int a = d0 - t.lo0, b = d1 - t.lo1, c = d2 - t.lo2;
-  if ((unsigned)a >= (unsigned)t.n0 || (unsigned)b >= (unsigned)t.n1 || (unsigned)c >= (unsigned)t.n2) return 0;
-  const int ig = t.ivgig[a + t.n0 * (b + t.n1 * c)];
-  return (ig > 0 && ig <= t.ngvec) ? ig : 0;
+  int ig = 0;
+  if ((unsigned)a < (unsigned)t.n0 && (unsigned)b < (unsigned)t.n1 && (unsigned)c < (unsigned)t.n2)
+    ig = t.ivgig[a + t.n0 * (b + t.n1 * c)];
+  if (ig > 0 && ig <= t.ngvec) return ig;
+  if (t.miss) atomicAdd(t.miss, 1u);
+  return 0;
 }
 
 // upper-triangular tile index t -> (ti, tj), ti <= tj, enumerated column by column

@@ -1,6 +1,6 @@
 ! Test program for the Fortran driver: reads one k-point's inputs from a stream file,
-! runs match and hamiltonandoverlapsetup through the GPU driver as seceqn.f90:57 /
-! seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
+! runs match and hamiltonsetup + overlapsetup (or hamiltonandoverlapsetup) through the GPU
+! driver as seceqnfv.f90:73-77 would, and writes apwalm, H and O to a stream file.  With a third
 ! file of eigenpairs it also runs forcek_gpu and appends forceibs(3, natmtot), then
 ! gendmatmt_gpu for the same states and appends dm(wfsize, wfsize, natmtot) and genrhoir_gpu's
 ! rhoir(ngrtot) (and, with radial data, genrhomt_gpu's rhomt(lmmaxvr, nrmtmax, natmtot), then, if
@@ -105,7 +105,17 @@ program exc_driver_main
   apwalm = (0.d0, 0.d0)
   call match(ngp, gpc, tpgpc, sfacgp, apwalm)
   call newsystem(system, .false., nmat)
-  call hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)
+  ! EXC_DRIVER_SETUP selects the calls: "split" (default) hamiltonsetup then overlapsetup as
+  ! seceqnfv.f90:76-77 makes them, "fused" hamiltonandoverlapsetup; "copy" the split calls
+  ! with apwalm copied from the host again (exc_resident_apwalm = .false.)
+  call get_environment_variable("EXC_DRIVER_SETUP", fnev)
+  if (trim(fnev) == "fused") then
+    call hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)
+  else
+    if (trim(fnev) == "copy") exc_resident_apwalm = .false.
+    call hamiltonsetup(system, ngp, apwalm, igpig, vgpc)
+    call overlapsetup(system, ngp, apwalm, igpig, vgpc)
+  end if
 
   open(newunit=u, file=trim(fout), access="stream", form="unformatted", status="replace")
   write(u) nmat

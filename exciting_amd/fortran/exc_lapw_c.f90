@@ -103,15 +103,36 @@ module exc_lapw_c
       complex(c_double_complex), intent(inout) :: apwalm(*)
     end function
 
+    ! apwalm is passed by address (c_loc of its first element), or c_null_ptr for the device
+    ! copy of the last exc_match call
     integer(c_int) function exc_hamiltonandoverlapsetup(ctx, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o) &
         bind(c, name="exc_hamiltonandoverlapsetup")
       import :: c_int, c_ptr, c_double, c_double_complex
-      type(c_ptr), value :: ctx
+      type(c_ptr), value :: ctx, apwalm
       integer(c_int), value :: ngp, ngkmax, ldh
-      complex(c_double_complex), intent(in) :: apwalm(*)
       integer(c_int), intent(in) :: igpig(*)
       real(c_double), intent(in) :: vgpc(*)
       complex(c_double_complex), intent(inout) :: h(*), o(*)
+    end function
+
+    integer(c_int) function exc_hamiltonsetup(ctx, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h) &
+        bind(c, name="exc_hamiltonsetup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx, apwalm
+      integer(c_int), value :: ngp, ngkmax, ldh
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: h(*)
+    end function
+
+    integer(c_int) function exc_overlapsetup(ctx, ngp, ngkmax, apwalm, igpig, vgpc, ldh, o) &
+        bind(c, name="exc_overlapsetup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx, apwalm
+      integer(c_int), value :: ngp, ngkmax, ldh
+      integer(c_int), intent(in) :: igpig(*)
+      real(c_double), intent(in) :: vgpc(*)
+      complex(c_double_complex), intent(inout) :: o(*)
     end function
 
     integer(c_int) function exc_ho_vjp(ctx, ngp, ngkmax, apwalm, igpig, vgpc, sfacgp, nst, ldc, evec, wh, wo, &
@@ -181,6 +202,11 @@ module exc_lapw_c
     end function
 
     integer(c_int) function exc_ctx_nlotot(ctx) bind(c, name="exc_ctx_nlotot")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function exc_ctx_status(ctx) bind(c, name="exc_ctx_status")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function

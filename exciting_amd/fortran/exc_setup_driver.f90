@@ -1,6 +1,8 @@
 ! Fortran driver of the GPU setup with the reference's call signatures.
 !
 !   match(ngp, gpc, tpgpc, sfacgp, apwalm)                        match.f90:13
+!   hamiltonsetup(system, ngp, apwalm, igpig, vgpc)               hamiltonsetup.f90:8
+!   overlapsetup(system, ngp, apwalm, igpig, vgpc)                overlapsetup.f90:8
 !   hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)     hamiltonandoverlapsetup.f90:8
 !   solvewithgpu(system, nstfv, evecfv, evalfv)                   modfvsystem.f90:585 (solvewithlapack)
 !   gendmatmt_gpu(ngp, apwalm, nmatmax, nstfv, evecfv, occ, wkpt, dm [, evecsv])
@@ -21,6 +23,12 @@
 ! iteration by exc_gpu_init, which keeps the device context in this module.  Error
 ! handling follows the reference: a singular matching matrix is reported and the run
 ! continues (match.f90:166-175); any other failure stops the program.
+!
+! The setup calls receive apwalm as the reference's do; when it is the array the last match
+! call filled (same address and ngp) and exc_resident_apwalm is .true. (the default), the
+! library takes its own device copy of those coefficients instead of copying the host array
+! again.  seceqnfv.f90:70-77 leaves apwalm unchanged between match and the setup; a caller
+! that changes it in between sets exc_resident_apwalm = .false.
 module exc_setup_gpu
   use, intrinsic :: iso_c_binding
   use exc_lapw_c
@@ -31,8 +39,13 @@ module exc_setup_gpu
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: ngkmax_ = 0
   integer, save :: lmaxapw_ = 0, natmtot_ = 0
+  ! the array and ngp of the last match call (for the device-resident apwalm)
+  type(c_ptr), save :: last_apwalm = c_null_ptr
+  integer, save :: last_ngp = -1
+  logical, public, save :: exc_resident_apwalm = .true.
 
-  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, exc_gpu_nlotot, forcek_gpu, &
+  public :: exc_gpu_init, exc_gpu_finalize, match, hamiltonandoverlapsetup, hamiltonsetup, overlapsetup, &
+            exc_gpu_nlotot, forcek_gpu, &
             solvewithgpu, gendmatmt_gpu, exc_gpu_dmat_wfsize, hmlint_gpu, set_radial_gpu, gencfun_gpu, &
             genveffig_gpu, genrhomt_gpu, genrhoir_gpu, rhovalk_gpu
 
@@ -90,10 +103,13 @@ contains
     integer, intent(in) :: ngp
     real(8), intent(in) :: gpc(ngkmax_), tpgpc(2, ngkmax_)
     complex(8), intent(in) :: sfacgp(ngkmax_, *)
-    complex(8), intent(inout) :: apwalm(ngkmax_, *)
+    complex(8), intent(inout), target :: apwalm(ngkmax_, *)
     integer(c_int) :: rc, l, ias
+    last_apwalm = c_null_ptr
     rc = exc_match(ctx, int(ngp, c_int), int(ngkmax_, c_int), gpc, tpgpc, sfacgp, apwalm)
     call check(rc, "match")
+    last_apwalm = c_loc(apwalm(1, 1))
+    last_ngp = ngp
     if (rc > 0) then
       l = mod(rc - 1, lmaxapw_ + 1)
       ias = (rc - 1) / (lmaxapw_ + 1) + 1
@@ -111,15 +127,48 @@ contains
   subroutine hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc)
     type(evsystem), intent(inout) :: system
     integer, intent(in) :: ngp
-    complex(8), intent(in) :: apwalm(ngkmax_, *)
+    complex(8), intent(in), target :: apwalm(ngkmax_, *)
     integer, intent(in) :: igpig(ngkmax_)
     real(8), intent(in) :: vgpc(3, ngkmax_)
     integer(c_int) :: ldh
     ldh = int(system%hamilton%rank, c_int)
     if (system%overlap%rank /= ldh) error stop "hamiltonandoverlapsetup: H and O ranks differ"
-    call check(exc_hamiltonandoverlapsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apwalm, igpig, vgpc, &
-                                           ldh, system%hamilton%za, system%overlap%za), "hamiltonandoverlapsetup")
+    call check(exc_hamiltonandoverlapsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apw_arg(apwalm, ngp), &
+                                           igpig, vgpc, ldh, system%hamilton%za, system%overlap%za), &
+               "hamiltonandoverlapsetup")
   end subroutine hamiltonandoverlapsetup
+
+  ! hamiltonsetup.f90:8 — H of one k-point into system%hamilton%za (rank = nmat), both
+  ! triangles, overwritten; seceqnfv.f90:76 calls it right before overlapsetup
+  subroutine hamiltonsetup(system, ngp, apwalm, igpig, vgpc)
+    type(evsystem), intent(inout) :: system
+    integer, intent(in) :: ngp
+    complex(8), intent(in), target :: apwalm(ngkmax_, *)
+    integer, intent(in) :: igpig(ngkmax_)
+    real(8), intent(in) :: vgpc(3, ngkmax_)
+    call check(exc_hamiltonsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apw_arg(apwalm, ngp), igpig, vgpc, &
+                                 int(system%hamilton%rank, c_int), system%hamilton%za), "hamiltonsetup")
+  end subroutine hamiltonsetup
+
+  ! overlapsetup.f90:8 — O of one k-point into system%overlap%za, both triangles, overwritten
+  subroutine overlapsetup(system, ngp, apwalm, igpig, vgpc)
+    type(evsystem), intent(inout) :: system
+    integer, intent(in) :: ngp
+    complex(8), intent(in), target :: apwalm(ngkmax_, *)
+    integer, intent(in) :: igpig(ngkmax_)
+    real(8), intent(in) :: vgpc(3, ngkmax_)
+    call check(exc_overlapsetup(ctx, int(ngp, c_int), int(ngkmax_, c_int), apw_arg(apwalm, ngp), igpig, vgpc, &
+                                int(system%overlap%rank, c_int), system%overlap%za), "overlapsetup")
+  end subroutine overlapsetup
+
+  ! the apwalm argument of a setup call: c_null_ptr (the library's device copy) when the array
+  ! is the one the last match call filled for this ngp, else its address
+  type(c_ptr) function apw_arg(apwalm, ngp)
+    complex(8), intent(in), target :: apwalm(ngkmax_, *)
+    integer, intent(in) :: ngp
+    apw_arg = c_loc(apwalm(1, 1))
+    if (exc_resident_apwalm .and. ngp == last_ngp .and. c_associated(apw_arg, last_apwalm)) apw_arg = c_null_ptr
+  end function apw_arg
 
   ! forcek.f90:13 — the IBS force contribution of one k-point (spin-unpolarised,
   ! first-variational states), added to forceibs(3, natmtot) with the k-point weight:

@@ -50,6 +50,7 @@ EXC_BATCH_MATCH = 1
 EXC_BATCH_GRAPH = 2
 EXC_BATCH_SPLIT_BF16 = 4
 EXC_BATCH_OLP_LEGENDRE = 8
+EXC_BATCH_SPLIT_CRT = 16
 
 _lib = None
 
@@ -70,10 +71,17 @@ SIGNATURES = {
     "exc_match_dev": [_P, _I, _I, _P, _P, _P, _P],
     "exc_hamiltonandoverlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
     "exc_hamiltonandoverlapsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
+    "exc_hamiltonsetup": [_P, _I, _I, _P, _P, _P, _I, _P],
+    "exc_hamiltonsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P],
+    "exc_overlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P],
+    "exc_overlapsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P],
     "exc_ctx_nlotot": [_P],
+    "exc_ctx_status": [_P],
     "exc_ctx_set_timing": [_P, _I],
     "exc_ctx_read_timing": [_P, _P, _P],
     "exc_ctx_set_variant": [_P, _I, _I],
+    "exc_ctx_set_moduli": [_P, _I],
+    "exc_crt_moduli": [_I, _P, _P],
     "exc_ctx_num_variants": [_I],
     "exc_setup_batched_dev": [_P, _I, _P, _I],
     "exc_ho_vjp_dev": [_P, _I, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P],
@@ -100,7 +108,7 @@ SIGNATURES = {
 
 # stage order of exc_ctx_read_timing
 STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat", "hmlint",
-          "scf")
+          "scf", "crtprep", "crtgemm", "crtmerge")
 
 
 def load_library(path: os.PathLike | None = None):
@@ -208,6 +216,11 @@ class Context:
     @property
     def nlotot(self):
         return self.lib.exc_ctx_nlotot(self.h)
+
+    def status(self):
+        """exc_ctx_status: raises ExcError if a setup since the last call met a basis difference
+        G_i - G_j outside the first ngvec G-vectors (the reference's 2 gkmax > gmaxvr stop)."""
+        self._call("exc_ctx_status", self.h)
 
     def close(self):
         if getattr(self, "h", None):
@@ -443,6 +456,10 @@ class Context:
     def match_dev(self, ngp, ngkmax, gpc, tpgpc, sfacgp, apwalm):
         return self._call("exc_match_dev", self.h, ngp, ngkmax, _ptr(gpc), _ptr(tpgpc), _ptr(sfacgp), _ptr(apwalm),
                           ok_positive=True)
+
+    def set_moduli(self, n):
+        """Number of moduli of the modular split (EXC_BATCH_SPLIT_CRT)."""
+        self._call("exc_ctx_set_moduli", self.h, int(n))
 
     def set_timing(self, on=True):
         self._call("exc_ctx_set_timing", self.h, 1 if on else 0)

@@ -9,6 +9,8 @@
  *                                  (== hamiltonsetup.f90:8 followed by overlapsetup.f90:8,
  *                                   including hmlistln.f90:13, olpistln.f90:13 and
  *                                   hamiltonoverlapocopy_UL.f90:6)
+ *   exc_hamiltonsetup              src_eigensystem/hamiltonsetup.f90:8
+ *   exc_overlapsetup               src_eigensystem/overlapsetup.f90:8
  *
  * Module state that the reference routines read implicitly (mt_hscf, oalo, ololo, idxlo,
  * ivg, ivgig, cfunig, veffig, apwfr-derived boundary values ...) is uploaded once per SCF
@@ -28,7 +30,8 @@
  *  - Return value: 0 = success; > 0 mirrors a LAPACK info (exc_match: 1 + the 0-based
  *    index (l + (lmaxapw+1)*ias) of the first singular matching matrix, like zgesv's
  *    info in match.f90:163-175 — the coefficients are still written, as the reference
- *    continues); < 0 = EXC_ERR_*.
+ *    continues: for a singular block they are the unsolved right-hand sides, which is what
+ *    zgesv leaves behind with info > 0); < 0 = EXC_ERR_*.
  *  - One context serves one host thread at a time (the reference calls the setup from one
  *    thread per MPI rank, scf_cycle.f90:241-276).  Calls on one context are serialised on
  *    its stream.
@@ -135,13 +138,30 @@ int exc_match_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *gpc, const do
 
 /* hamiltonandoverlapsetup.f90:8 (= hamiltonsetup + overlapsetup) — H and O of one k-point.
  *   apwalm(ngkmax, apwordmax, lmmaxapw, natmtot), igpig(ngkmax), vgpc(3, ngkmax),
- *   H(ldh, nmat), O(ldh, nmat) complex with nmat = ngp + nlotot <= ldh. */
+ *   H(ldh, nmat), O(ldh, nmat) complex with nmat = ngp + nlotot <= ldh.
+ * Host-array entries: apwalm = NULL takes the coefficients of the context's last exc_match
+ * call, still on the device (same ngp and ngkmax; EXC_ERR_STATE otherwise), so match and the
+ * setup of one k-point move apwalm over PCIe once; the caller must not have changed its copy. */
 int exc_hamiltonandoverlapsetup(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
                                 const int *igpig, const double *vgpc, int ldh, double *h,
                                 double *o);
 int exc_hamiltonandoverlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
                                     const int *igpig, const double *vgpc, int ldh, double *h,
                                     double *o);
+/* hamiltonsetup.f90:8, Subroutine hamiltonsetup(system, ngp, apwalm, igpig, vgpc) — H alone
+ * (muffin-tin APW-APW, APW-lo and lo-lo blocks, hmlistln.f90:13) — and overlapsetup.f90:8,
+ * Subroutine overlapsetup(system, ngp, apwalm, igpig, vgpc) — O alone (olpistln.f90:13), the
+ * two calls seceqnfv.f90:76-77 makes one after the other.  Same arguments as above with the
+ * one output matrix; each writes its matrix completely, both triangles (what the reference
+ * holds after overlapsetup's hamiltonoverlapocopy_UL, overlapsetup.f90:164-166). */
+int exc_hamiltonsetup(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                      const double *vgpc, int ldh, double *h);
+int exc_hamiltonsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
+                          const int *igpig, const double *vgpc, int ldh, double *h);
+int exc_overlapsetup(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                     const double *vgpc, int ldh, double *o);
+int exc_overlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
+                         const int *igpig, const double *vgpc, int ldh, double *o);
 
 /* ---- grouped setup of many k-points (no reference counterpart: the reference loops over
  * k in scf_cycle.f90:241-276 and calls match + hamiltonsetup + overlapsetup per k). ----
@@ -166,6 +186,13 @@ typedef struct exc_kpoint_dev {
 #define EXC_BATCH_SPLIT_BF16 4 /* muffin-tin contraction from split-BF16 slices with FP32 MFMA   */
                                /* accumulation and per-(row, atom) power-of-two scaling (3 slices,*/
                                /* 6 slice pairs); Y stage, interstitial, lo blocks stay FP64      */
+#define EXC_BATCH_SPLIT_CRT 16  /* muffin-tin contraction from exact integer products: each row of  */
+                                /* A and Yt scaled by a power of two to integers below 2^beta (row */
+                                /* 2-norm), their residues modulo n pairwise coprime moduli <= 256  */
+                                /* multiplied by int8 MFMA with int32 accumulation (Gauss's three  */
+                                /* products per modulus), the results reconstructed by the Chinese */
+                                /* remainder theorem and rounded once to FP64; n from             */
+                                /* exc_ctx_set_moduli.  Y stage, interstitial, lo blocks stay FP64 */
 #define EXC_BATCH_OLP_LEGENDRE 8 /* needs EXC_BATCH_MATCH: the muffin-tin part of O from the      */
                                  /* radial factors of match by the spherical-harmonic addition   */
                                  /* theorem, sum_a conj(S_a) S_a' sum_l P_l(u.u') sum_io r r',   */
@@ -217,7 +244,10 @@ int exc_eigen(exc_ctx *ctx, int nmat, int ldh, double *h, double *o, int nev, do
  * solved as one strided batch, one launch per step for all of them, the tridiagonal divide
  * and conquer per k-point on worker streams (h, o left unchanged); variant 5 back-transforms
  * the eigenvectors per k-point instead of as one batch; variant 7 finds the tridiagonal
- * eigenpairs by bisection and inverse iteration as ZHEGVX does (slow).  Solver variants 0-4
+ * eigenpairs by bisection and inverse iteration as ZHEGVX does (slow); variant 8 is variant 6
+ * with the per-k divide and conquer on one worker stream.  Every rocBLAS handle of the solve
+ * gets a workspace sized before anything is queued, so no workspace is allocated or freed
+ * while a solve is in flight.  Solver variants 0-4
  * (exc_ctx_set_variant, EXC_STAGE_EIGEN) instead run the single-k solves one after another
  * on a worker stream forked from and joined back into the context's stream (h, o
  * overwritten).  Returns 0, or the first failing k-point's status. */
@@ -318,6 +348,15 @@ int exc_ctx_genveffig(exc_ctx *ctx, const double *veffir, double *veffig);
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
+/* Deferred argument check of the setup calls, the counterpart of init1.f90:340-347, where the
+ * reference stops when 2 gkmax > gmaxvr: that guard keeps every difference G_i - G_j of two
+ * basis vectors of a k-point inside the first ngvec G-vectors (hmlistln.f90:77 and
+ * olpistln.f90 look them up there).  The setup kernels count every pair they find outside
+ * (that term would be dropped); exc_ctx_status waits for the context's stream and returns
+ * EXC_ERR_ARG (and clears the count) if any setup, grouped setup or adjoint since the last
+ * call met one, EXC_OK otherwise.  The host-array entry points call it before they return. */
+int exc_ctx_status(exc_ctx *ctx);
+
 /* Per-stage kernel timers, the counterpart of the reference's timing accumulators
  * (mod_timing.F90: timematch, time_hmlaan, time_olpaan, time_hmlalon, time_hmlistln, ...).
  * When enabled, every launch of a stage is bracketed by HIP events on the context's stream.
@@ -339,11 +378,22 @@ int exc_ctx_nlotot(const exc_ctx *ctx);
 #define EXC_STAGE_DMAT 11    /* exc_gendmatmt_dev (muffin-tin density matrix)                 */
 #define EXC_STAGE_HMLINT 12  /* exc_ctx_hmlint (muffin-tin radial integrals and Gaunt sums)   */
 #define EXC_STAGE_SCF 13     /* exc_ctx_gencfun, exc_ctx_genveffig (Theta, V(G), FFTs)        */
-#define EXC_NSTAGE 14
+#define EXC_STAGE_CRTPREP 14 /* row scales and residue planes (EXC_BATCH_SPLIT_CRT)           */
+#define EXC_STAGE_CRTGEMM 15 /* residue products, every tile and modulus                      */
+#define EXC_STAGE_CRTMERGE 16 /* reconstruction, interstitial and mirror                      */
+#define EXC_NSTAGE 17
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
+/* Number of moduli of EXC_BATCH_SPLIT_CRT (8 .. 12, default 11).  With n moduli the row
+ * integers have beta bits (the largest with 2^(2 beta + 2) <= their product): 34 bits at n = 9,
+ * 38 at 10, 42 at 11, 46 at 12; the residue products cost n int8 GEMMs per real product.  No
+ * reference counterpart (the reference computes in FP64). */
+int exc_ctx_set_moduli(exc_ctx *ctx, int n);
+/* The first n moduli and beta of the modular split (for a caller's error estimate). */
+int exc_crt_moduli(int n, int *moduli, int *beta);
+
 /* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,
- * EXC_STAGE_BSPLIT), as chosen by
+ * EXC_STAGE_BSPLIT, EXC_STAGE_CRTGEMM), as chosen by
  * the autotuner (exciting_amd/autotune.py); -1 restores the built-in choice by basis size.
  * exc_ctx_num_variants returns how many variants a stage has.  No reference counterpart. */
 int exc_ctx_set_variant(exc_ctx *ctx, int stage, int variant);

@@ -68,7 +68,12 @@ def match(cr, kp, ngkmax=None):
                 lms = np.arange(l * l, (l + 1) ** 2)
                 # b(io, igp, m)
                 b = djl[l, :, :, None] * (zt2[None, :, None] * np.conj(ylm[lms, :]).T[None, :, :])
-                sol = np.linalg.solve(d, b.reshape(cr.apwordmax, -1)).reshape(b.shape)
+                try:
+                    sol = np.linalg.solve(d, b.reshape(cr.apwordmax, -1)).reshape(b.shape)
+                except np.linalg.LinAlgError:
+                    # zgesv with info > 0 (an exactly zero pivot) leaves the right-hand sides
+                    # unsolved; match.f90:166-185 reports it and stores them (match.f90:163-175)
+                    sol = b
                 apw[:ngp, :, l * l:(l + 1) ** 2, ias] = np.transpose(sol, (1, 0, 2))
     return apw
 
@@ -131,12 +136,24 @@ def _mt_sum(cr, left, right):
     return out
 
 
-def hamiltonsetup(cr, kp, apwalm, H):
-    """hamiltonsetup.f90:57-174: per atom zm = h_a apwi, H += apwi^H zm; APW-lo; lo-lo; interstitial."""
+def _mt_sum_per_atom(cr, left, right):
+    """sum_a left_a^H right_a as the reference forms it: one zgemm per atom over all of the
+    atom's APW functions, full matrices (hamiltonsetup.f90:90-103, overlapsetup.f90:95-108).
+    The same sum as _mt_sum in another order; used by the CPU baseline, where its one
+    K = naa product per atom is what the reference's BLAS runs."""
+    out = left[0].conj().T @ right[0]
+    for a in range(1, cr.natm):
+        out += left[a].conj().T @ right[a]
+    return out
+
+
+def hamiltonsetup(cr, kp, apwalm, H, per_atom=False):
+    """hamiltonsetup.f90:57-174: per atom zm = h_a apwi, H += apwi^H zm; APW-lo; lo-lo; interstitial.
+    per_atom: the muffin-tin sum in the reference's per-atom zgemm order (_mt_sum_per_atom)."""
     ngp = kp.ngp
     apwis = [_apwi(cr, apwalm, ngp, ias) for ias in range(cr.natm)]
     zms = [cr.haa[:, :, ias] @ apwis[ias] for ias in range(cr.natm)]
-    H[:ngp, :ngp] += _mt_sum(cr, apwis, zms)
+    H[:ngp, :ngp] += (_mt_sum_per_atom if per_atom else _mt_sum)(cr, apwis, zms)
     for ias in range(cr.natm):
         is_ = cr.spidx[ias]
         apwi = apwis[ias]
@@ -157,12 +174,12 @@ def hamiltonsetup(cr, kp, apwalm, H):
     return H
 
 
-def overlapsetup(cr, kp, apwalm, O, H=None):
+def overlapsetup(cr, kp, apwalm, O, H=None, per_atom=False):
     """overlapsetup.f90:50-166: O += apwi^H (conj apwi2)^H; APW-lo; lo-lo; interstitial; copy_UL."""
     ngp = kp.ngp
     apwis = [_apwi(cr, apwalm, ngp, ias) for ias in range(cr.natm)]
     # zm = conj(apwi)^T (apwi2, h1 = 0); O += apwi^H zm^H = conj(A) A^T
-    O[:ngp, :ngp] += _mt_sum(cr, apwis, apwis)
+    O[:ngp, :ngp] += (_mt_sum_per_atom if per_atom else _mt_sum)(cr, apwis, apwis)
     for ias in range(cr.natm):
         is_ = cr.spidx[ias]
         if cr.nlorb[is_] != 0:
@@ -199,15 +216,17 @@ def copy_ul(M):
     return M
 
 
-def setup(cr, kp, apwalm=None):
-    """match (if apwalm is None) + hamiltonsetup + overlapsetup, as seceqn/seceqnfv call them."""
+def setup(cr, kp, apwalm=None, per_atom=False):
+    """match (if apwalm is None) + hamiltonsetup + overlapsetup, as seceqn/seceqnfv call them.
+    per_atom: the muffin-tin sums as one zgemm per atom, the reference's own order (the CPU
+    baseline); the default sums in l-blocks (the checker, smaller rounding error)."""
     if apwalm is None:
         apwalm = match(cr, kp)
     nmat = kp.ngp + cr.nlotot
     H = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
     O = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
-    hamiltonsetup(cr, kp, apwalm, H)
-    overlapsetup(cr, kp, apwalm, O, H)
+    hamiltonsetup(cr, kp, apwalm, H, per_atom=per_atom)
+    overlapsetup(cr, kp, apwalm, O, H, per_atom=per_atom)
     return H, O, apwalm
 
 

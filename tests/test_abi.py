@@ -76,7 +76,8 @@ def test_argument_errors_without_gpu():
     assert lib.exc_eigen_batched_dev(None, 1, None, None, None, None, 1, None, None, None) == lapw.EXC_ERR_ARG
     assert lib.exc_ctx_destroy(None) == 0
     assert lib.exc_ctx_num_variants(2) >= 2
-    assert lib.exc_ctx_num_variants(10) == 8          # eigensolve paths 0-7
+    assert lib.exc_ctx_num_variants(10) == 9          # eigensolve paths 0-8
+    assert lib.exc_ctx_num_variants(15) == 5          # residue GEMM variants of the modular split
 
 
 def test_fortran_driver_built():

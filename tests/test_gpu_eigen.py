@@ -80,7 +80,7 @@ def test_eigen_reports_indefinite_overlap():
     assert ei.value.code == n + 1          # leading minor of order 1 not positive definite
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(9))
 def test_eigen_solver_variants(variant):
     """Every solver path (0 zhegvdx, 1 zhegvx, 2 zhegvd, 3 zhegv, 4 level-3 reduction +
     zheevd; 5-7 are the grouped multi-k paths, the same as 4 for one k-point) to the same
@@ -182,7 +182,7 @@ def _random_pair(rng, n, lower_garbage):
     return H, O, np.asfortranarray(Hs), np.asfortranarray(Os)
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(9))
 def test_eigen_reads_upper_triangles_only(variant):
     """Like ZHEGVX(uplo 'U'): the lower triangles and the imaginary diagonal of H, O are not
     part of the input."""
@@ -204,7 +204,7 @@ def _to_dev(M):
     return torch.from_numpy(np.ascontiguousarray(M.T)).cuda()   # (nmat, ld) row-major = Fortran (ld, nmat)
 
 
-@pytest.mark.parametrize("variant", [5, 6, 7])
+@pytest.mark.parametrize("variant", [5, 6, 7, 8])
 def test_eigen_grouped_ragged(variant):
     """The grouped multi-k solve (every k-point padded to the largest nmat, one strided-batched
     launch per step; variants 6 and 7 also back-transform the eigenvectors as one batch, 64
@@ -236,7 +236,7 @@ def test_eigen_grouped_ragged(variant):
         _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"grouped v{variant} k{i} n={n}")
 
 
-@pytest.mark.parametrize("variant", [5, 6, 7])
+@pytest.mark.parametrize("variant", [5, 6, 7, 8])
 def test_eigen_grouped_after_setup(variant):
     """The grouped paths after the grouped setup of workload 1 (6 k-points, nmat 1182-1282):
     each k-point against LAPACK on the same GPU-built H and O."""
@@ -265,7 +265,7 @@ def test_eigen_grouped_after_setup(variant):
                f"grouped v{variant} after setup k{i} n={b.nmat[i]}")
 
 
-@pytest.mark.parametrize("variant", [5, 6, 7])
+@pytest.mark.parametrize("variant", [5, 6, 7, 8])
 def test_eigen_grouped_degenerate(variant):
     """Exactly degenerate spectra (eigenvalues in pairs and triples, as symmetry makes them):
     H = L Q diag(lam) Q^H L^H with O = L L^H, so the generalized eigenvalues are lam.  The
@@ -303,3 +303,33 @@ def test_eigen_grouped_degenerate(variant):
     for i, n in enumerate(sizes):
         H, O, e_ref = refs[i]
         _check(H, O, evs[i].cpu().numpy(), vecs[i].cpu().numpy().T, e_ref, f"degenerate v{variant} k{i} n={n}")
+
+
+def test_eigen_grouped_concurrent_equals_serialized():
+    """Variant 6 runs the k-points' tridiagonal divide and conquer concurrently on four worker
+    streams; variant 8 makes the same calls on the same data one after another on one stream.
+    Their eigenpairs must agree bit for bit (two concurrent runs included), on the grouped
+    setup of workload 1 (8 k-points, nmat 1182-1282)."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 8)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    nev = 200
+    out = {}
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE)
+        torch.cuda.synchronize()
+        for run, variant in enumerate((8, 6, 6)):
+            ctx.set_variant("eigen", variant)
+            evs = [torch.full((nev,), float("nan"), dtype=torch.float64, device="cuda") for _ in range(b.nk)]
+            vecs = [torch.full((nev, n), float("nan"), dtype=torch.complex128, device="cuda") for n in b.nmat]
+            ctx.eigen_batched_dev(b.nmat, b.H, b.O, nev, evs, vecs)
+            torch.cuda.synchronize()
+            out[run] = (evs, vecs)
+            print(f"grouped eigensolve variant {variant} (run {run}) done", flush=True)
+    for run in (1, 2):
+        for i in range(b.nk):
+            assert torch.equal(out[run][0][i], out[0][0][i]), f"run {run} eigenvalues k{i}"
+            assert torch.equal(out[run][1][i], out[0][1][i]), f"run {run} eigenvectors k{i}"

@@ -1,8 +1,11 @@
 """The Fortran driver (amdflang, iso_c_binding) with the reference signatures, on the GPU.
 
-Runs exciting_amd/_lib/exc_driver_main: match(ngp, gpc, tpgpc, sfacgp, apwalm) and
-hamiltonandoverlapsetup(system, ngp, apwalm, igpig, vgpc) for one k-point, and checks
-apwalm, H and O against the oracle (relative Frobenius error <= 1e-12).
+Runs exciting_amd/_lib/exc_driver_main: match(ngp, gpc, tpgpc, sfacgp, apwalm) followed by
+hamiltonsetup(system, ngp, apwalm, igpig, vgpc) and overlapsetup(...) as seceqnfv.f90:76-77
+calls them (apwalm taken from the library's device copy of match's result, or copied from the
+host again), or by the fused hamiltonandoverlapsetup, for one k-point, and checks apwalm, H
+and O against the oracle (relative Frobenius error <= 1e-12); the three call sequences run the
+same kernels and must give the same matrices bit for bit.
 """
 import os
 import subprocess
@@ -27,18 +30,25 @@ def test_fortran_driver(tmp_path, which):
         pytest.fail("Fortran driver not built: run python -m exciting_amd.build")
     w = small("b") if which == "small-b" else workload(0)
     cr, kp = w.crystal, w.kpoints[0]
-    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    fin = tmp_path / "in.bin"
     write_input(fin, cr, kp)
-    print(f"running Fortran driver on {which}", flush=True)
-    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
-    print(r.stdout, r.stderr)
-    assert r.returncode == 0
-    apw, H, O = read_output(fout, cr, kp.igpig.shape[0])
     H_ref, O_ref, A_ref = R.setup(cr, kp)
     n = kp.ngp
-    assert np.abs(apw[:n] - A_ref[:n]).max() <= 1e-12 * np.abs(A_ref[:n]).max()
-    assert np.linalg.norm(H - H_ref) <= 1e-12 * np.linalg.norm(H_ref)
-    assert np.linalg.norm(O - O_ref) <= 1e-12 * np.linalg.norm(O_ref)
+    res = {}
+    for mode in ("split", "copy", "fused"):
+        fout = tmp_path / f"out_{mode}.bin"
+        print(f"running Fortran driver on {which}, setup calls {mode}", flush=True)
+        r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
+                           env=dict(os.environ, EXC_DRIVER_SETUP=mode))
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+        apw, H, O = read_output(fout, cr, kp.igpig.shape[0])
+        assert np.abs(apw[:n] - A_ref[:n]).max() <= 1e-12 * np.abs(A_ref[:n]).max()
+        assert np.linalg.norm(H - H_ref) <= 1e-12 * np.linalg.norm(H_ref)
+        assert np.linalg.norm(O - O_ref) <= 1e-12 * np.linalg.norm(O_ref)
+        res[mode] = (H, O)
+    for mode in ("copy", "fused"):
+        assert np.array_equal(res[mode][0], res["split"][0]) and np.array_equal(res[mode][1], res["split"][1]), mode
 
 
 def test_fortran_forcek(tmp_path):

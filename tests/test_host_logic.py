@@ -64,3 +64,29 @@ def test_upper_triangle_tile_decode():
         return t - j * (j + 1) // 2, j
     seen = [decode(t) for t in range(40 * 41 // 2)]
     assert seen == [(i, j) for j in range(40) for i in range(j + 1)]
+
+
+def test_autotune_apply_pins_table_choice(tmp_path):
+    class Ctx:
+        def __init__(self):
+            self.pinned = {}
+
+        def set_variant(self, st, v):
+            self.pinned[st] = v
+
+    cr = workload(1, 2).crystal
+    ngps = [1230, 1174]
+    p = tmp_path / "tune.json"
+    assert autotune.apply(Ctx(), cr, ngps, p) == {}
+    autotune.save(p, autotune.key(cr, ngps), {"bmt": 19, "ystage": 12})
+    c = Ctx()
+    assert autotune.apply(c, cr, ngps, p) == {"bmt": 19, "ystage": 12}
+    assert c.pinned == {"bmt": 19, "ystage": 12}
+    assert autotune.key(cr, ngps) == "N1280-naa242-natm8"
+    # the committed table is well-formed: every entry names known stages and valid variants
+    if autotune.TABLE.exists():
+        from exciting_amd import lapw
+        for k, v in json.loads(autotune.TABLE.read_text()).items():
+            assert k.startswith("N") and "-naa" in k and "-natm" in k
+            for st, var in v.items():
+                assert st in lapw.STAGES and isinstance(var, int) and var >= 0

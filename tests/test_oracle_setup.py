@@ -254,3 +254,17 @@ def test_overlap_addition_theorem(key):
             O += np.conj(S)[:, None] * S[None, :] * t
         err = np.linalg.norm(O - O_mt) / np.linalg.norm(O_mt)
         assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("key", ["b", "d"])
+def test_per_atom_order_equals_l_blocks(key):
+    """The CPU baseline's per-atom zgemm order (the reference's, hamiltonsetup.f90:90-103) and
+    the checker's l-block order sum the same terms: H and O agree to rounding."""
+    w = small(key)
+    cr, kp = w.crystal, w.kpoints[0]
+    A = R.match(cr, kp)
+    H0, O0, _ = R.setup(cr, kp, apwalm=A)
+    H1, O1, _ = R.setup(cr, kp, apwalm=A, per_atom=True)
+    assert np.linalg.norm(H1 - H0) <= 1e-13 * np.linalg.norm(H0)
+    assert np.linalg.norm(O1 - O0) <= 1e-13 * np.linalg.norm(O0)
+    assert not np.array_equal(H1, H0) or cr.natm == 1 or cr.lmaxmat == 0   # the two orders are different code
