@@ -558,7 +558,8 @@ def main():
         if args.path in ("perk", "host"):
             tuned = autotune.tune(ctx, ["mt", "ystage"], step_perk, reps=2, log=log)
         else:
-            tuned = autotune.tune(ctx, ["bmt", "ystage"], lambda: step_batched(False), reps=2, log=log)
+            tuned = autotune.tune(ctx, ["bmt", "ystage"] + (["olp"] if olpflag else []), lambda: step_batched(False),
+                                  reps=2, log=log)
     if args.path == "batched":
         # stage breakdown of one eager (uncaptured) grouped pass, then time the graph replays
         step_batched(False)

@@ -72,6 +72,7 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->bsplit_variant = -1;
   c->crt_variant = -1;
   c->crt_nmod = 11;
+  c->olp_variant = -1;
   c->eig_variant = -1;
   c->gnt_lmaxvr = -1;
   c->hml_lmaxvr = -1;
@@ -192,6 +193,10 @@ int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
       if (variant < -1 || variant >= kCrtVariants) return EXC_ERR_ARG;
       c->crt_variant = variant;
       return EXC_OK;
+    case EXC_STAGE_OLP:
+      if (variant < -1 || variant >= kOlpVariants) return EXC_ERR_ARG;
+      c->olp_variant = variant;
+      return EXC_OK;
     default:
       return EXC_ERR_ARG;
   }
@@ -205,6 +210,7 @@ int exc_ctx_num_variants(int stage) {
     case EXC_STAGE_BSPLIT: return kBsplitVariants;
     case EXC_STAGE_EIGEN: return kEigenVariants;
     case EXC_STAGE_CRTGEMM: return kCrtVariants;
+    case EXC_STAGE_OLP: return kOlpVariants;
     default: return 0;
   }
 }
@@ -599,8 +605,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const bool split = flags & EXC_BATCH_SPLIT_BF16;
   const bool crt = flags & EXC_BATCH_SPLIT_CRT;
   if (split && crt) return EXC_ERR_ARG;
-  // exact int32 sums of three residue products of K terms: 3 K 128^2 < 2^31
-  if (crt && c->d.natm * c->d.naa > 43000) return EXC_ERR_UNSUPPORTED;
+  // exact int32 sums of K residue products: K 128^2 < 2^31
+  if (crt && c->d.natm * c->d.naa > 131000) return EXC_ERR_UNSUPPORTED;
   // the addition-theorem overlap needs the radial factors that only the grouped match writes
   const bool olp = flags & EXC_BATCH_OLP_LEGENDRE;
   if (olp && !(flags & EXC_BATCH_MATCH)) return EXC_ERR_ARG;
@@ -620,7 +626,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // LDS filled by DMA)
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 19);
   const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
-  const int cv = c->crt_variant >= 0 ? c->crt_variant : 2;
+  const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
+  const int ov = c->olp_variant >= 0 ? c->olp_variant : 0;
   CrtTab ct;
   if (crt) exc_crt_tables(c->crt_nmod, &ct);
   const int npa = olp ? 3 : 4;                               // residue planes of A (re + im for O)
@@ -628,7 +635,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // batch signature: descriptors, flags and variants
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 6 * sizeof(int));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
-  int tail[6] = {nk, flags, bv, yv, crt ? cv : -1, crt ? c->crt_nmod : -1};
+  int tail[6] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? c->crt_nmod : -1) + 100 * ov};
   memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
   const bool same = (sig == c->batch_sig);
   if (same && (flags & EXC_BATCH_GRAPH) && c->graph_exec) {
@@ -662,9 +669,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     }
     if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot ? rtot : 1)))) return rc;
     size_t qtot = 0, xtot = 0;
+    int cbt = 0, cwr = 0, cwc = 0;
+    if (crt && exc_crt_shape(cv, &cbt, &cwr, &cwc)) return EXC_ERR_ARG;
     if (crt) {
       for (int k = 0; k < nk; ++k) {
-        const size_t cnp = (size_t)(ks[k].ngp + 127) / 128 * 128;
+        const size_t cnp = (size_t)(ks[k].ngp + 255) / 256 * 256;
         qtot += (size_t)(npa + 3) * ct.n * kp * cnp;
         xtot += 2 * cnp;
       }
@@ -719,8 +728,9 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       e.xa = e.xy = nullptr;
       if (crt) {
         // planes [modulus][plane][K/16][cnp][16] of A then of Yt; tiles (k, ti <= tj, H | O)
-        // column by column, and the (tile, modulus) work modulus-major within the k-point
-        const int cnp = (q.ngp + 127) / 128 * 128;
+        // in 4 x 4 blocks of tiles, and the (tile, modulus, product) work modulus- and
+        // product-major within the k-point, the tiles in that block order
+        const int cnp = (q.ngp + 255) / 256 * 256;
         e.cnp = cnp;
         e.qa = (char *)c->crt_q + qoff;
         e.qy = (char *)c->crt_q + qoff + (size_t)npa * ct.n * kp * cnp;
@@ -729,14 +739,15 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         e.xy = c->crt_x + xoff + cnp;
         xoff += 2 * (size_t)cnp;
         maxcnp = cnp > maxcnp ? cnp : maxcnp;
-        const int nt = cnp / 128, first = (int)citems.size();
-        for (int tj = 0; tj < nt; ++tj)
-          for (int ti = 0; ti <= tj; ++ti) {
-            citems.push_back({k, ti, tj, 0});
-            if (!olp) citems.push_back({k, ti, tj, 1});
-          }
+        const int nt = (q.ngp + cbt - 1) / cbt, first = (int)citems.size(), S = 4;
+        for (int which = 0; which < (olp ? 1 : 2); ++which)
+          for (int bj = 0; bj < nt; bj += S)
+            for (int bi = 0; bi <= bj; bi += S)
+              for (int tj = bj; tj < std::min(bj + S, nt); ++tj)
+                for (int ti = bi; ti < std::min(bi + S, tj + 1); ++ti) citems.push_back({k, ti, tj, which});
         for (int l = 0; l < ct.n; ++l)
-          for (int i = first; i < (int)citems.size(); ++i) cwork.push_back(make_int2(i, l));
+          for (int p = 0; p < 3; ++p)
+            for (int i = first; i < (int)citems.size(); ++i) cwork.push_back(make_int2(i, 4 * l + p));
       }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
       const int nt = (q.ngp + T - 1) / T;
@@ -765,7 +776,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     c->batch_ncwork = (int)cwork.size();
     c->batch_maxcnp = maxcnp;
     if (crt) {
-      const size_t rbytes = (size_t)citems.size() * ct.n * 2 * 128 * 128;
+      const size_t rbytes = (size_t)citems.size() * ct.n * 3 * cbt * cbt;
       if ((rc = grow((void **)&c->crt_res, &c->crt_res_bytes, rbytes ? rbytes : 1))) return rc;
       if ((rc = grow((void **)&c->d_citems, &c->citems_cap, sizeof(mt_item) * (citems.empty() ? 1 : citems.size()))))
         return rc;
@@ -816,7 +827,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (fork) {
       if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return EXC_ERR_HIP;
       if (hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) return EXC_ERR_HIP;
-      if (olp && (r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, s2))) return r;
+      if (olp && (r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, ov, s2))) return r;
       if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, s2))) return r;
       if (hipEventRecord(c->ev_join, c->side) != hipSuccess) return EXC_ERR_HIP;
     }
@@ -858,7 +869,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (fork) return r;
     if (olp) {
       StageTimer t(c, EXC_STAGE_OLP);
-      if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, c->stream))) return r;
+      if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
     }
     {
       StageTimer t(c, EXC_STAGE_LO);

@@ -47,6 +47,11 @@ int pin_workspace(exc_ctx::EigWorker &W, F &&calls) {
 // Variant 4 runs in two phases: phase 1 factors O (the caller checks its info on the host
 // before phase 2, so a non-positive-definite O never feeds the later steps); the other
 // variants do all their work in phase 2.
+// a call that ran, or (in a size query) recorded its workspace
+inline bool call_ok(rocblas_status st) {
+  return st == rocblas_status_success || st == rocblas_status_size_unchanged || st == rocblas_status_size_increased;
+}
+
 // The rocBLAS / rocSOLVER calls of one solve (phase 1: the Cholesky factor of variant 4;
 // phase 2: the rest), used both to run them and, in a size query, to size the handle's
 // workspace; in_a: eigenvectors returned in A (all n columns) rather than in Z.
@@ -75,23 +80,21 @@ rocblas_status eig_calls(exc_ctx::EigWorker &W, int v, int phase, int nmat, int 
     default: {
       const rocblas_double_complex one = {1.0, 0.0};
       rocblas_status st = rocsolver_ztrtri(W.blas, up, rocblas_diagonal_non_unit, nmat, B, ldh, d_ints + 2);   // B = U^-1
-      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // Z = H U^-1
+      if (call_ok(st))                                                                  // Z = H U^-1
         st = rocblas_ztrmm(W.blas, rocblas_side_right, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
                            nmat, &one, B, ldh, A, ldh, Z, nmat);
-      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // H = U^-H Z
+      if (call_ok(st))                                                                  // H = U^-H Z
         st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_conjugate_transpose,
                            rocblas_diagonal_non_unit, nmat, nmat, &one, B, ldh, Z, nmat, A, ldh);
-      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // C Y = Y diag(w)
+      if (call_ok(st))                                                                  // C Y = Y diag(w)
         st = rocsolver_zheevd(W.blas, ev, up, nmat, A, ldh, W.w, W.e, d_ints + 3);
-      if (st == rocblas_status_success || st == rocblas_status_size_unchanged)          // Z = U^-1 Y(:, 1:nev)
+      if (call_ok(st))                                                                  // Z = U^-1 Y(:, 1:nev)
         st = rocblas_ztrmm(W.blas, rocblas_side_left, up, rocblas_operation_none, rocblas_diagonal_non_unit, nmat,
                            nev, &one, B, ldh, A, ldh, Z, nmat);
       return st;
     }
   }
 }
-
-inline bool call_ok(rocblas_status st) { return st == rocblas_status_success || st == rocblas_status_size_unchanged; }
 
 int eig_enqueue(exc_ctx::EigWorker &W, int v, int phase, int nmat, int ldh, double *h, double *o, int nev,
                 double *eval, double *evec, int ldv, int *d_ints) {

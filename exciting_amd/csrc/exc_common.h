@@ -88,7 +88,7 @@ struct kdesc {
   // radial factors of apwalm for the overlap by the addition theorem (EXC_BATCH_OLP_LEGENDRE),
   // written by the grouped match: [(atom*ngp + row)*2(lmaxmat+1) + 2l + io]
   double *rf;
-  // modular split (EXC_BATCH_SPLIT_CRT): rows padded to cnp (a multiple of 128), int8 residue
+  // modular split (EXC_BATCH_SPLIT_CRT): rows padded to cnp (a multiple of 256), int8 residue
   // planes of A and Yt, layout [modulus][plane][K/16][cnp][16], and per-row scale exponents
   int cnp, pad_;
   void *qa, *qy;
@@ -137,7 +137,8 @@ int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
 int exc_bsplit_tile(int variant);
 int exc_bsplit_max_blocks(int variant);
 int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
-int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s);
+constexpr int kOlpVariants = 2;
+int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
 // modular split of the muffin-tin contraction (k_crt.hip)
@@ -158,7 +159,7 @@ struct CrtTab {                      // passed by value to the kernels (kernel-a
 };
 void exc_crt_tables(int n, CrtTab *t);
 inline int exc_crt_kpad(int ktot) { return (ktot + 63) / 64 * 64; }
-int exc_crt_waves(int variant, int *wr, int *wc);
+int exc_crt_shape(int variant, int *bt, int *wr, int *wc);   // tile edge and wave grid of a GEMM variant
 int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, int npa, hipStream_t s);
 int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
                         const CrtTab &t, int npa, signed char *res, int variant, hipStream_t s);

@@ -72,6 +72,7 @@ struct exc_ctx {
   // modular split (EXC_BATCH_SPLIT_CRT): moduli count, GEMM variant, residue planes, row scale
   // exponents, residues of every (tile, modulus), the tile and (tile, modulus) work lists
   int crt_nmod, crt_variant;
+  int olp_variant;          // thread blocking of the addition-theorem overlap (-1: built-in)
   void *crt_q;
   size_t crt_q_bytes;
   int *crt_x;

@@ -10,10 +10,11 @@
 //          and per modulus m the symmetric residues of Re X', Im X' and of Re X' -+ Im X'
 //          (Gauss's third product) as int8 planes, layout [m][plane][K/16][row][16 bytes]:
 //          one 16-byte k-chunk of 64 consecutive rows is one 1 KiB wave-wide DMA.
-//  gemm  : per modulus and 128 x 128 tile, P1 = ar br^T, P2 = ai bi^T, P3 = as bs^T by
-//          v_mfma_i32_32x32x32_i8 with exact int32 accumulation (|P| <= K 128^2 < 2^31 / 3),
-//          Re = P1 + P2, Im = P3 - P1 + P2, reduced modulo m to int8 residues.
-//  merge : per tile element Garner's mixed-radix digits of Re', Im' in [-P/2, P/2) (exact:
+//  gemm  : per modulus, tile and product, P1 = ar br^T, P2 = ai bi^T or P3 = as bs^T by
+//          v_mfma_i32_32x32x32_i8 with exact int32 accumulation (|P| <= K 128^2 < 2^31),
+//          reduced modulo m to int8 residues.
+//  merge : per tile element Re = P1 + P2, Im = P3 - P1 + P2 modulo each m, then
+//          Garner's mixed-radix digits of Re', Im' in [-P/2, P/2) (exact:
 //          |Re'|, |Im'| <= |A'_i| |B'_j| < P/2), the FP64 value correctly rounded as
 //          fma(x_hi, M_6, x_lo), times 2^-(s_i + t_j); then the interstitial term and the
 //          Hermitian mirror exactly as the FP64 epilogue (hmlistln.f90:74-84,
@@ -35,8 +36,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kTile = 128;       // tile edge of the gemm and merge kernels; rows padded to it
-constexpr int kResTile = 2 * kTile * kTile;   // residue bytes of one (tile, modulus): Re and Im
+constexpr int kMaxTile = 256;    // largest tile edge of the gemm and merge kernels; rows padded to it
 
 // symmetric residue of an exact integer held in a double, |x| < 2^52
 __device__ __forceinline__ int dres(double x, int m, double minv) {
@@ -111,134 +111,140 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict_
   signed char *base = (signed char *)(op ? k.qy : k.qa);
   const size_t pstride = (size_t)kp * k.cnp;            // bytes of one plane
   const size_t off = ((size_t)kc * k.cnp + row) * 16;
+#pragma unroll 1
   for (int l = 0; l < t.n; ++l) {
     const int m = t.m[l];
     const double mi = t.dminv[l];
-    for (int p = 0; p < np; ++p) {
-      unsigned w4[4];
+    const float fi = t.finv[l];
+    // residues of re and im from the integers; those of re -+ im from theirs ((a - b) mod m =
+    // (a mod m - b mod m) mod m): plane 2 is re - im for A (the conjugated left operand) and
+    // re + im for Yt, plane 3 (A only, for O's right operand) re + im
+    unsigned w[4][4];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        unsigned u = 0;
+    for (int g = 0; g < 4; ++g) {
+      unsigned u0 = 0, u1 = 0, u2 = 0, u3 = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int q = 4 * g + b;
-          // plane 2 is the left operand's re - im for A (conj(a)) and re + im for Yt;
-          // plane 3 (A only, for O) is re + im
-          const double x = p == 0 ? xr[q] : p == 1 ? xi[q] : (p == 2 && op == 0) ? xr[q] - xi[q] : xr[q] + xi[q];
-          u |= (unsigned)(dres(x, m, mi) & 255) << (8 * b);
-        }
-        w4[g] = u;
+      for (int b = 0; b < 4; ++b) {
+        const int q = 4 * g + b;
+        const int rr = dres(xr[q], m, mi), ri = dres(xi[q], m, mi);
+        u0 |= (unsigned)(rr & 255) << (8 * b);
+        u1 |= (unsigned)(ri & 255) << (8 * b);
+        u2 |= (unsigned)(fres(op == 0 ? rr - ri : rr + ri, m, fi) & 255) << (8 * b);
+        u3 |= (unsigned)(fres(rr + ri, m, fi) & 255) << (8 * b);
       }
-      *(uint4 *)(base + (size_t)(l * np + p) * pstride + off) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+      w[0][g] = u0;
+      w[1][g] = u1;
+      w[2][g] = u2;
+      w[3][g] = u3;
     }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < np)
+        *(uint4 *)(base + (size_t)(l * np + p) * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
   }
 }
 
 // ---------------------------------------------------------------------------------------
-// gemm: one (tile, modulus) per workgroup on an XCD-mapped grid (workgroup b runs on XCD b % 8
-// and takes the next work of that XCD's contiguous eighth of the list); WR x WC waves with
-// (128 / WR) x (128 / WC) wave tiles of 32 x 32 MFMA blocks; LDS stages of BK bytes of k
-// (all three planes of both panels) filled by DMA, NST stages in flight.
+// gemm: one real residue product (tile, modulus, product p) per workgroup on an XCD-mapped
+// grid (workgroup b runs on XCD b % 8 and takes the next work of that XCD's contiguous eighth
+// of the list; the list keeps the tiles of one (k-point, modulus, product) in 4 x 4 blocks, so
+// the workgroups an XCD runs together share their row and column panels in its L2).  BT x BT
+// tiles, WR x WC waves with (BT / WR) x (BT / WC) wave tiles of 32 x 32 MFMA blocks, LDS
+// stages of BK bytes of k of one plane of both panels, filled by DMA, NST stages in flight.
+//   p = 0: P1 = ar br^T, p = 1: P2 = ai bi^T, p = 2: P3 = as bs^T (as = ar - ai of conj(A),
+//   bs = br + bi), each reduced modulo m to int8 residues; the merge combines them.
 // ---------------------------------------------------------------------------------------
-template <int BK, int NST, int WR, int WC>
+template <int BT, int BK, int NST, int WR, int WC>
 struct GemmCfg {
   static constexpr int kWaves = WR * WC, kThreads = 64 * kWaves;
   static constexpr int KC = BK / 16;                          // 16-byte k-chunks per stage
-  static constexpr int PB = KC * kTile * 16;                  // bytes of one plane of one panel
-  static constexpr int ABYTES = 3 * PB;
-  static constexpr int NL = 2 * 3 * KC * (kTile / 64);        // wave-wide 1 KiB DMAs per stage
+  static constexpr int PB = KC * BT * 16;                     // bytes of one panel per stage
+  static constexpr int NL = 2 * KC * (BT / 64);               // wave-wide 1 KiB DMAs per stage
   static constexpr int L = NL / kWaves;
-  static constexpr int MI = kTile / WR / 32, NJ = kTile / WC / 32;
-  static_assert(NL % kWaves == 0 && BK % 32 == 0, "DMA split / MFMA k-step");
+  static constexpr int MI = BT / WR / 32, NJ = BT / WC / 32;
+  static_assert(NL % kWaves == 0 && BK % 32 == 0 && BT % 64 == 0, "DMA split / MFMA k-step");
   struct __attribute__((aligned(16))) Stage {
-    unsigned char a[ABYTES];
-    unsigned char b[ABYTES];
+    unsigned char a[PB];
+    unsigned char b[PB];
   };
 };
 
-template <int BK, int NST, int WR, int WC>
-__device__ __forceinline__ void gemm_issue(typename GemmCfg<BK, NST, WR, WC>::Stage *st, const unsigned char *pa,
-                                           const unsigned char *pb, const int (&pla)[3], const int (&plb)[3],
-                                           size_t pstride, size_t chunk_stride, int row0, int col0, int kc0) {
-  using C = GemmCfg<BK, NST, WR, WC>;
+template <int BT, int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_issue(typename GemmCfg<BT, BK, NST, WR, WC>::Stage *st, const unsigned char *pa,
+                                           const unsigned char *pb, size_t chunk_stride, int row0, int col0,
+                                           int kc0) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
   for (int q = 0; q < C::L; ++q) {
-    const int j = wave + C::kWaves * q;               // 0 .. NL-1: (panel, plane, chunk, 64-row half)
+    const int j = wave + C::kWaves * q;               // 0 .. NL-1: (panel, chunk, 64-row quarter)
     const bool isb = j >= C::NL / 2;
     const int jj = isb ? j - C::NL / 2 : j;
-    const int h = jj & 1, kc = (jj >> 1) % C::KC, p = (jj >> 1) / C::KC;
-    const unsigned char *src = (isb ? pb + (size_t)plb[p] * pstride : pa + (size_t)pla[p] * pstride) +
-                               (size_t)(kc0 + kc) * chunk_stride + (size_t)((isb ? col0 : row0) + 64 * h + lane) * 16;
-    unsigned char *dst = (isb ? st->b : st->a) + (p * C::KC + kc) * (kTile * 16) + h * 1024;
+    const int h = jj % (BT / 64), kc = jj / (BT / 64);
+    const unsigned char *src = (isb ? pb : pa) + (size_t)(kc0 + kc) * chunk_stride +
+                               (size_t)((isb ? col0 : row0) + 64 * h + lane) * 16;
+    unsigned char *dst = (isb ? st->b : st->a) + kc * (BT * 16) + h * 1024;
     __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)dst, 16, 0, 0);
   }
 }
 
-template <int BK, int NST, int WR, int WC>
-__device__ __forceinline__ void gemm_compute(const typename GemmCfg<BK, NST, WR, WC>::Stage *st,
-                                             v16i (&acc)[3][GemmCfg<BK, NST, WR, WC>::MI][GemmCfg<BK, NST, WR, WC>::NJ],
+template <int BT, int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_compute(const typename GemmCfg<BT, BK, NST, WR, WC>::Stage *st,
+                                             v16i (&acc)[GemmCfg<BT, BK, NST, WR, WC>::MI][GemmCfg<BT, BK, NST, WR, WC>::NJ],
                                              int wr, int wc) {
-  using C = GemmCfg<BK, NST, WR, WC>;
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
   const int lane = threadIdx.x & 63;
   const int r = lane & 31, hk = lane >> 5;
 #pragma unroll
   for (int s = 0; s < BK / 32; ++s) {
+    v4i af[C::MI], bf[C::NJ];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      v4i af[C::MI], bf[C::NJ];
+    for (int i = 0; i < C::MI; ++i)
+      af[i] = *(const v4i *)(st->a + ((2 * s + hk) * BT + wr * (32 * C::MI) + 32 * i + r) * 16);
 #pragma unroll
-      for (int i = 0; i < C::MI; ++i)
-        af[i] = *(const v4i *)(st->a + ((p * C::KC + 2 * s + hk) * kTile + wr * (32 * C::MI) + 32 * i + r) * 16);
+    for (int j = 0; j < C::NJ; ++j)
+      bf[j] = *(const v4i *)(st->b + ((2 * s + hk) * BT + wc * (32 * C::NJ) + 32 * j + r) * 16);
 #pragma unroll
-      for (int j = 0; j < C::NJ; ++j)
-        bf[j] = *(const v4i *)(st->b + ((p * C::KC + 2 * s + hk) * kTile + wc * (32 * C::NJ) + 32 * j + r) * 16);
+    for (int i = 0; i < C::MI; ++i)
 #pragma unroll
-      for (int i = 0; i < C::MI; ++i)
-#pragma unroll
-        for (int j = 0; j < C::NJ; ++j) acc[p][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[p][i][j], 0, 0, 0);
-    }
+      for (int j = 0; j < C::NJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
   }
 }
 
-template <int BK, int NST, int WR, int WC, int MINW>
+template <int BT, int BK, int NST, int WR, int WC, int MINW>
 __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdesc *__restrict__ ks,
-                                                                   const mt_item *__restrict__ items,
-                                                                   const int2 *__restrict__ work, int nwork, int kp,
-                                                                   CrtTab t, int npa, signed char *__restrict__ res) {
-  using C = GemmCfg<BK, NST, WR, WC>;
+                                                                      const mt_item *__restrict__ items,
+                                                                      const int2 *__restrict__ work, int nwork, int kp,
+                                                                      CrtTab t, int npa, signed char *__restrict__ res) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
   using St = typename C::Stage;
   EXC_DMA_STAGES(St, NST)
   const int chunk = (nwork + 7) / 8;
   const int x = blockIdx.x % 8, qb = blockIdx.x / 8;
   const int wi = x * chunk + qb;
   if (qb >= chunk || wi >= nwork) return;
-  const int2 wk = work[wi];                      // (item, modulus)
+  const int2 wk = work[wi];                      // (item, 4 * modulus + product)
   const mt_item it = items[wk.x];
   const kdesc k = ks[it.k];
-  const int l = wk.y;
+  const int l = wk.y >> 2, p = wk.y & 3;
   const size_t pstride = (size_t)kp * k.cnp, chunk_stride = (size_t)k.cnp * 16;
   // left operand conj(A): planes re, im, re - im; right operand Yt (H) or A (O): re, im, re + im
-  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(l * npa) * pstride;
-  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(l * npa) * pstride
-                                     : (const unsigned char *)k.qy + (size_t)(l * 3) * pstride;
-  const int pla[3] = {0, 1, 2};
-  const int plb[3] = {0, 1, it.which ? 3 : 2};
-  const int row0 = it.ti * kTile, col0 = it.tj * kTile;
+  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(l * npa + p) * pstride;
+  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(l * npa + (p == 2 ? 3 : p)) * pstride
+                                     : (const unsigned char *)k.qy + (size_t)(l * 3 + p) * pstride;
+  const int row0 = it.ti * BT, col0 = it.tj * BT;
   const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
-  v16i acc[3][C::MI][C::NJ];
+  v16i acc[C::MI][C::NJ];
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+  for (int i = 0; i < C::MI; ++i)
 #pragma unroll
-    for (int i = 0; i < C::MI; ++i)
-#pragma unroll
-      for (int j = 0; j < C::NJ; ++j) acc[p][i][j] = v16i{};
+    for (int j = 0; j < C::NJ; ++j) acc[i][j] = v16i{};
   St *st[4] = {&s0_, &s1_, (St *)&s2_, (St *)&s3_};
   const int nsteps = kp / BK;
 #pragma unroll
   for (int u = 0; u < NST - 1; ++u)
-    if (u < nsteps) gemm_issue<BK, NST, WR, WC>(st[u], pa, pb, pla, plb, pstride, chunk_stride, row0, col0, u * C::KC);
+    if (u < nsteps) gemm_issue<BT, BK, NST, WR, WC>(st[u], pa, pb, chunk_stride, row0, col0, u * C::KC);
   for (int base = 0; base < nsteps; base += NST) {
 #pragma unroll
     for (int u = 0; u < NST; ++u) {
@@ -251,34 +257,27 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
           wait_vm_lgkm0<0>();
         stage_barrier();
         if (step + NST - 1 < nsteps)
-          gemm_issue<BK, NST, WR, WC>(st[(u + NST - 1) % NST], pa, pb, pla, plb, pstride, chunk_stride, row0, col0,
-                                      (step + NST - 1) * C::KC);
-        gemm_compute<BK, NST, WR, WC>(st[u], acc, wr, wc);
+          gemm_issue<BT, BK, NST, WR, WC>(st[(u + NST - 1) % NST], pa, pb, chunk_stride, row0, col0,
+                                          (step + NST - 1) * C::KC);
+        gemm_compute<BT, BK, NST, WR, WC>(st[u], acc, wr, wc);
       }
     }
   }
-  // Re = P1 + P2, Im = P3 - P1 + P2 modulo m; residues of one 32 x 32 block and lane are 16
-  // contiguous bytes: res[work][wave][block][re, im][lane][16]
+  // residues modulo m of one 32 x 32 block and lane are 16 contiguous bytes:
+  // res[item][modulus][product][wave][block][lane][16]
   const int m = t.m[l];
   const double mi = t.dminv[l];
   const int lane = threadIdx.x & 63;
-  signed char *out = res + ((size_t)wk.x * t.n + l) * kResTile;
+  signed char *out = res + (((size_t)wk.x * t.n + l) * 3 + p) * (size_t)(BT * BT);
 #pragma unroll
   for (int i = 0; i < C::MI; ++i)
 #pragma unroll
     for (int j = 0; j < C::NJ; ++j) {
-      unsigned ur[4] = {0u, 0u, 0u, 0u}, ui[4] = {0u, 0u, 0u, 0u};
+      unsigned u4[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int p1 = acc[0][i][j][v], p2 = acc[1][i][j][v], p3 = acc[2][i][j][v];
-        const int re = dres((double)(p1 + p2), m, mi);
-        const int im = dres((double)((p3 - p1) + p2), m, mi);
-        ur[v >> 2] |= (unsigned)(re & 255) << (8 * (v & 3));
-        ui[v >> 2] |= (unsigned)(im & 255) << (8 * (v & 3));
-      }
+      for (int v = 0; v < 16; ++v) u4[v >> 2] |= (unsigned)(dres((double)acc[i][j][v], m, mi) & 255) << (8 * (v & 3));
       const int blk = wave * (C::MI * C::NJ) + i * C::NJ + j;
-      *(uint4 *)(out + ((size_t)(2 * blk) * 64 + lane) * 16) = make_uint4(ur[0], ur[1], ur[2], ur[3]);
-      *(uint4 *)(out + ((size_t)(2 * blk + 1) * 64 + lane) * 16) = make_uint4(ui[0], ui[1], ui[2], ui[3]);
+      *(uint4 *)(out + ((size_t)blk * 64 + lane) * 16) = make_uint4(u4[0], u4[1], u4[2], u4[3]);
     }
 }
 
@@ -310,18 +309,18 @@ __device__ __forceinline__ void garner(const int (&r)[NM], const CrtTab &t, long
 
 template <int NM>
 __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
-                                                        CrtTab t, const signed char *__restrict__ res, int wr_n,
+                                                        CrtTab t, const signed char *__restrict__ res, int bt, int wr_n,
                                                         int wc_n, GTab gt) {
   const mt_item it = items[blockIdx.x];
   const kdesc k = ks[it.k];
   const int ngp = k.ngp;
-  const int row0 = it.ti * kTile, col0 = it.tj * kTile;
-  __shared__ int gr[kTile][3], gc[kTile][3];
-  __shared__ double vr[kTile][3], vc[kTile][3];
-  __shared__ int er[kTile], ec[kTile];
-  for (int e = threadIdx.x; e < 2 * kTile; e += 256) {
-    const bool isr = e < kTile;
-    const int q = isr ? e : e - kTile;
+  const int row0 = it.ti * bt, col0 = it.tj * bt;
+  __shared__ int gr[kMaxTile][3], gc[kMaxTile][3];
+  __shared__ double vr[kMaxTile][3], vc[kMaxTile][3];
+  __shared__ int er[kMaxTile], ec[kMaxTile];
+  for (int e = threadIdx.x; e < 2 * bt; e += 256) {
+    const bool isr = e < bt;
+    const int q = isr ? e : e - bt;
     const int idx = (isr ? row0 : col0) + q;
     int g0 = 0, g1 = 0, g2 = 0, ex = 0;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
@@ -331,28 +330,32 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
       v0 = k.vgpc[3 * idx]; v1 = k.vgpc[3 * idx + 1]; v2 = k.vgpc[3 * idx + 2];
       ex = isr ? k.xa[idx] : (it.which ? k.xa[idx] : k.xy[idx]);
     }
-    int(&g)[kTile][3] = isr ? gr : gc;
-    double(&vv)[kTile][3] = isr ? vr : vc;
+    int(&g)[kMaxTile][3] = isr ? gr : gc;
+    double(&vv)[kMaxTile][3] = isr ? vr : vc;
     g[q][0] = g0; g[q][1] = g1; g[q][2] = g2;
     vv[q][0] = v0; vv[q][1] = v1; vv[q][2] = v2;
     (isr ? er : ec)[q] = ex;
   }
   __syncthreads();
-  const int mi_n = kTile / wr_n / 32, nj_n = kTile / wc_n / 32, per_wave = mi_n * nj_n;
-  const signed char *base = res + (size_t)blockIdx.x * NM * kResTile;
+  const int mi_n = bt / wr_n / 32, nj_n = bt / wc_n / 32, per_wave = mi_n * nj_n;
+  const int nblk = (bt / 32) * (bt / 32);
+  const size_t ptile = (size_t)bt * bt;                     // residue bytes of one (modulus, product)
+  const signed char *base = res + (size_t)blockIdx.x * NM * 3 * ptile;
   cplx *out = it.which ? k.o : k.h;
-  for (int c = threadIdx.x; c < 16 * 64; c += 256) {
+  for (int c = threadIdx.x; c < nblk * 64; c += 256) {
     const int blk = c >> 6, lane = c & 63;
     const int wave = blk / per_wave, bi = (blk % per_wave) / nj_n, bj = blk % nj_n;
     const int wr = wave / wc_n, wc = wave % wc_n;
     const int cl = wc * (32 * nj_n) + 32 * bj + (lane & 31);
     const int col = col0 + cl;
     if (col >= ngp) continue;
-    uint4 rr[NM], ri[NM];
+    uint4 q1[NM], q2[NM], q3[NM];
 #pragma unroll
     for (int l = 0; l < NM; ++l) {
-      rr[l] = *(const uint4 *)(base + (size_t)l * kResTile + ((size_t)(2 * blk) * 64 + lane) * 16);
-      ri[l] = *(const uint4 *)(base + (size_t)l * kResTile + ((size_t)(2 * blk + 1) * 64 + lane) * 16);
+      const signed char *b = base + (size_t)l * 3 * ptile + ((size_t)blk * 64 + lane) * 16;
+      q1[l] = *(const uint4 *)b;
+      q2[l] = *(const uint4 *)(b + ptile);
+      q3[l] = *(const uint4 *)(b + 2 * ptile);
     }
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -362,10 +365,15 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
       int dr[NM], di[NM];
 #pragma unroll
       for (int l = 0; l < NM; ++l) {
-        const unsigned wr4 = (v >> 2) == 0 ? rr[l].x : (v >> 2) == 1 ? rr[l].y : (v >> 2) == 2 ? rr[l].z : rr[l].w;
-        const unsigned wi4 = (v >> 2) == 0 ? ri[l].x : (v >> 2) == 1 ? ri[l].y : (v >> 2) == 2 ? ri[l].z : ri[l].w;
-        dr[l] = (int)(signed char)((wr4 >> (8 * (v & 3))) & 255);
-        di[l] = (int)(signed char)((wi4 >> (8 * (v & 3))) & 255);
+        const int g = v >> 2, sh = 8 * (v & 3);
+        const unsigned w1 = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
+        const unsigned w2 = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
+        const unsigned w3 = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
+        const int p1 = (signed char)((w1 >> sh) & 255), p2 = (signed char)((w2 >> sh) & 255);
+        const int p3 = (signed char)((w3 >> sh) & 255);
+        // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo m)
+        dr[l] = fres(p1 + p2, t.m[l], t.finv[l]);
+        di[l] = fres(p3 - p1 + p2, t.m[l], t.finv[l]);
       }
       long long hr, lr, hi, li;
       garner<NM>(dr, t, hr, lr);
@@ -394,19 +402,19 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
 
 template <int NM>
 int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
-                   const signed char *res, int wr, int wc, hipStream_t s) {
-  hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, t, res, wr, wc,
+                   const signed char *res, int bt, int wr, int wc, hipStream_t s) {
+  hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, t, res, bt, wr, wc,
                      setup_dev::make_gtab(d));
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
 
-template <int BK, int NST, int WR, int WC, int MINW>
+template <int BT, int BK, int NST, int WR, int WC, int MINW>
 int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t, int npa,
                   signed char *res, hipStream_t s) {
   const int chunk = (nwork + 7) / 8;
-  hipLaunchKernelGGL((crt_gemm_kernel<BK, NST, WR, WC, MINW>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks, items,
-                     work, nwork, kp, t, npa, res);
+  hipLaunchKernelGGL((crt_gemm_kernel<BT, BK, NST, WR, WC, MINW>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks,
+                     items, work, nwork, kp, t, npa, res);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -457,13 +465,14 @@ int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, co
   return EXC_OK;
 }
 
-// variants: (k bytes per LDS stage, stages, wave grid, minimum waves per SIMD)
-//   0: 32, 3, 2 x 2 (64 x 64 wave tiles), 1    1: 64, 2, 2 x 2, 1
-//   2: 32, 3, 2 x 4 (64 x 32 wave tiles), 2    3: 64, 2, 2 x 4, 2    4: 32, 4, 2 x 4, 2
-int exc_crt_waves(int variant, int *wr, int *wc) {
+// variants: (tile edge, k bytes per LDS stage, stages, wave grid, minimum waves per SIMD)
+//   0: 256, 64, 3, 2 x 4 (128 x 64 wave tiles), 1    1: 256, 32, 4, 2 x 4, 1
+//   2: 128, 64, 3, 2 x 2 (64 x 64 wave tiles), 2     3: 256, 64, 2, 2 x 4, 1
+//   4: 128, 32, 4, 2 x 2, 2
+int exc_crt_shape(int variant, int *bt, int *wr, int *wc) {
   switch (variant) {
-    case 0: case 1: *wr = 2; *wc = 2; return EXC_OK;
-    case 2: case 3: case 4: *wr = 2; *wc = 4; return EXC_OK;
+    case 0: case 1: case 3: *bt = 256; *wr = 2; *wc = 4; return EXC_OK;
+    case 2: case 4: *bt = 128; *wr = 2; *wc = 2; return EXC_OK;
     default: return EXC_ERR_ARG;
   }
 }
@@ -473,11 +482,11 @@ int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items,
   if (nwork <= 0) return EXC_OK;
   const int kp = exc_crt_kpad(d.natm * d.naa);
   switch (variant) {
-    case 0: return launch_gemm_t<32, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 1: return launch_gemm_t<64, 2, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 2: return launch_gemm_t<32, 3, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 3: return launch_gemm_t<64, 2, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 4: return launch_gemm_t<32, 4, 2, 4, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 0: return launch_gemm_t<256, 64, 3, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 1: return launch_gemm_t<256, 32, 4, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 2: return launch_gemm_t<128, 64, 3, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 3: return launch_gemm_t<256, 64, 2, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 4: return launch_gemm_t<128, 32, 4, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
     default: return EXC_ERR_ARG;
   }
 }
@@ -485,14 +494,14 @@ int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items,
 int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
                          const signed char *res, int variant, hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
-  int wr = 0, wc = 0;
-  if (exc_crt_waves(variant, &wr, &wc)) return EXC_ERR_ARG;
+  int bt = 0, wr = 0, wc = 0;
+  if (exc_crt_shape(variant, &bt, &wr, &wc)) return EXC_ERR_ARG;
   switch (t.n) {
-    case 8: return launch_merge_t<8>(d, ks, items, nitems, t, res, wr, wc, s);
-    case 9: return launch_merge_t<9>(d, ks, items, nitems, t, res, wr, wc, s);
-    case 10: return launch_merge_t<10>(d, ks, items, nitems, t, res, wr, wc, s);
-    case 11: return launch_merge_t<11>(d, ks, items, nitems, t, res, wr, wc, s);
-    case 12: return launch_merge_t<12>(d, ks, items, nitems, t, res, wr, wc, s);
+    case 8: return launch_merge_t<8>(d, ks, items, nitems, t, res, bt, wr, wc, s);
+    case 9: return launch_merge_t<9>(d, ks, items, nitems, t, res, bt, wr, wc, s);
+    case 10: return launch_merge_t<10>(d, ks, items, nitems, t, res, bt, wr, wc, s);
+    case 11: return launch_merge_t<11>(d, ks, items, nitems, t, res, bt, wr, wc, s);
+    case 12: return launch_merge_t<12>(d, ks, items, nitems, t, res, bt, wr, wc, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -12,10 +12,11 @@
 // same matrix as the reference's conj(A) A^T contraction (olpaa, overlapsetup), a different
 // rounding: about 440 FP64 FMAs per element instead of 8 natm naa flops, on the VALU.
 //
-// Workgroup: one 64x64 tile of the upper triangle (512 threads, 4x2 elements each); per
-// atom the rows' and columns' radial factors and structure factors are brought to LDS by
-// DMA, the next atom's while the current one is evaluated (two stages).  Epilogue as the MFMA
-// tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
+// Workgroup: one 64x64 tile of the upper triangle, 4 x CJ elements per thread (rows tr + 16 i,
+// columns tc + (64 / CJ) j; variant 0: CJ = 2, 512 threads; variant 1: CJ = 4, 256 threads,
+// half the LDS reads per FMA); per atom the rows' and columns' radial factors and structure
+// factors are brought to LDS by DMA, the next atom's while the current one is evaluated (two
+// stages).  Epilogue as the MFMA tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
 #include "setup_dev.h"
 #include "mt_dma.h"
 
@@ -25,7 +26,6 @@ using namespace setup_dev;
 
 constexpr int kOT = 64;        // tile edge
 constexpr int kNLmax = 16;     // lmaxmat + 1 supported
-constexpr int kThreads = 512;  // 8 waves; thread (tr, tc) owns rows tr + 16 i (i < 4), cols tc + 32 j (j < 2)
 
 struct d2 {
   double a, b;
@@ -58,12 +58,13 @@ struct __attribute__((aligned(16))) OlpStage {
 
 // DMA of atom `at` into a stage: 2 nl + 2 wave-wide loads of 64 x 16 bytes, rows past N
 // clamped to row N-1 (their products are never stored)
+template <int NTHR>
 __device__ __forceinline__ void olp_issue(OlpStage *st, const kdesc &k, int at, int nl, int row0, int col0) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ngp = k.ngp;
   const d2 *rfa = (const d2 *)(k.rf + (size_t)at * ngp * 2 * nl);
-  for (int j = wave; j < 2 * nl + 2; j += kThreads / 64) {
+  for (int j = wave; j < 2 * nl + 2; j += NTHR / 64) {
     const void *src;
     void *dst;
     if (j < 2 * nl) {
@@ -83,8 +84,10 @@ __device__ __forceinline__ void olp_issue(OlpStage *st, const kdesc &k, int at, 
   }
 }
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void olp_kernel(
+template <int CJ>
+__global__ __launch_bounds__(64 * 64 / (4 * CJ)) __attribute__((amdgpu_waves_per_eu(CJ == 2 ? 4 : 2))) void olp_kernel(
     const kdesc *__restrict__ ks, const mt_item *__restrict__ items, int nitems, int natm, int nl, GTab gt) {
+  constexpr int kThreads = 64 * 64 / (4 * CJ), CS = 64 / CJ;   // threads; column stride of a thread's elements
   __shared__ OlpStage st0;
   __shared__ OlpStage st1;
   __shared__ double ur[kOT][3], uc[kOT][3];
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   const kdesc k = ks[w.k];
   const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
   const int tid = threadIdx.x, tr = tid & 15, tc = tid >> 4;
-  olp_issue(&st0, k, 0, nl, row0, col0);
+  olp_issue<kThreads>(&st0, k, 0, nl, row0, col0);
   for (int e = tid; e < 2 * kOT; e += kThreads) {
     const bool isr = e < kOT;
     const int q = isr ? e : e - kOT;
@@ -116,12 +119,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     g[q][0] = g0; g[q][1] = g1; g[q][2] = g2;
   }
   __syncthreads();
-  double x[4][2], are[4][2], aim[4][2];
+  double x[4][CJ], are[4][CJ], aim[4][CJ];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int a = tr + 16 * i, b = tc + 32 * j;
+    for (int j = 0; j < CJ; ++j) {
+      const int a = tr + 16 * i, b = tc + CS * j;
       x[i][j] = ur[a][0] * uc[b][0] + ur[a][1] * uc[b][1] + ur[a][2] * uc[b][2];
       are[i][j] = aim[i][j] = 0.0;
     }
@@ -129,24 +132,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   auto atom = [&](const OlpStage *cur, OlpStage *nxt, int at) {
     wait_vm_lgkm0<0>();
     stage_barrier();                                  // atom `at` landed; nxt is no longer read
-    if (at + 1 < natm) olp_issue(nxt, k, at + 1, nl, row0, col0);
-    double b1[4][2], b2[4][2];
+    if (at + 1 < natm) olp_issue<kThreads>(nxt, k, at + 1, nl, row0, col0);
+    double b1[4][CJ], b2[4][CJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b1[i][j] = b2[i][j] = 0.0;
+      for (int j = 0; j < CJ; ++j) b1[i][j] = b2[i][j] = 0.0;
 #pragma unroll 1
     for (int l = nl - 1; l >= 0; --l) {
       const double al = kClenshaw.al[l], bl = kClenshaw.bl[l];
-      dv2 p[4], q[2];
+      dv2 p[4], q[CJ];
 #pragma unroll
       for (int i = 0; i < 4; ++i) p[i] = *reinterpret_cast<const dv2 *>(&cur->rr[(tr + 16 * i) * nl + l]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) q[j] = *reinterpret_cast<const dv2 *>(&cur->rc[(tc + 32 * j) * nl + l]);
+      for (int j = 0; j < CJ; ++j) q[j] = *reinterpret_cast<const dv2 *>(&cur->rc[(tc + CS * j) * nl + l]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < CJ; ++j) {
           const double a = p[i].x * q[j].x + p[i].y * q[j].y;
           const double b0 = a + al * x[i][j] * b1[i][j] + bl * b2[i][j];
           b2[i][j] = b1[i][j];
@@ -156,8 +159,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const cplx s = cur->sr[tr + 16 * i], t = cur->sc[tc + 32 * j];
+      for (int j = 0; j < CJ; ++j) {
+        const cplx s = cur->sr[tr + 16 * i], t = cur->sc[tc + CS * j];
         are[i][j] += (s.re * t.re + s.im * t.im) * b1[i][j];     // conj(S(G)) S(G')
         aim[i][j] += (s.re * t.im - s.im * t.re) * b1[i][j];
       }
@@ -167,8 +170,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     if (at + 1 < natm) atom(&st1, &st0, at + 1);
   }
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int cl = tc + 32 * j, c = col0 + cl;
+  for (int j = 0; j < CJ; ++j) {
+    const int cl = tc + CS * j, c = col0 + cl;
     if (c >= ngp) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -191,11 +194,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
 int exc_olp_tile() { return kOT; }
 
-int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
+int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
   if (d.lmaxmat + 1 > kNLmax || d.apwordmax != 2) return EXC_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(olp_kernel, dim3(nitems), dim3(kThreads), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
-                     make_gtab(d));
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL(olp_kernel<2>, dim3(nitems), dim3(512), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
+                         make_gtab(d));
+      break;
+    case 1:
+      hipLaunchKernelGGL(olp_kernel<4>, dim3(nitems), dim3(256), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
+                         make_gtab(d));
+      break;
+    default:
+      return EXC_ERR_ARG;
+  }
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

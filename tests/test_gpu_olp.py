@@ -83,3 +83,27 @@ def test_olp_legendre_needs_match():
     w = small("a")
     with pytest.raises(RuntimeError):
         _run(w, lapw.EXC_BATCH_OLP_LEGENDRE)
+
+
+def test_olp_variants_bit_identical():
+    """The overlap kernel's thread blockings (4 x 2 and 4 x 4 elements per thread) run the same
+    Clenshaw recurrence per element in the same order: O bit for bit."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 3)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE
+    out = {}
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        for v in range(ctx.num_variants("olp")):
+            ctx.set_variant("olp", v)
+            for t in b.O:
+                t.fill_(float("nan"))
+            ctx.setup_batched_dev(b, flags)
+            torch.cuda.synchronize()
+            out[v] = [t.clone() for t in b.O]
+    for v in out:
+        for i in range(b.nk):
+            assert torch.equal(out[v][i], out[0][i]), f"overlap variant {v} k{i}"
