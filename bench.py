@@ -128,10 +128,14 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
     if kind == "crt" and args.moduli:
         ctx.set_moduli(args.moduli)
     nmod = args.moduli or 11
+    st = "crtgemm" if kind == "crt" else "bsplit"
     if args.tune:
         from exciting_amd import autotune
-        tuned = autotune.tune(ctx, ["crtgemm" if kind == "crt" else "bsplit"], lambda: ctx.setup_batched_dev(batch, flags),
-                              reps=2, log=log)
+        tuned = autotune.tune(ctx, [st], lambda: ctx.setup_batched_dev(batch, flags), reps=2, log=log)
+        if args.tune_out:
+            autotune.save(args.tune_out, autotune.key(cr, [kp.ngp for kp in w.kpoints]), tuned)
+    elif not args.set and st in (args.table or {}):
+        tuned = {st: args.table[st]}           # pinned from the tuned table by main()
     ctx.setup_batched_dev(batch, flags)          # eager pass: stage breakdown
     torch.cuda.synchronize()
     ctx.read_timing()
@@ -480,6 +484,10 @@ def main():
     ap.add_argument("--nk", type=int, default=None, help="limit the number of k-points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune tile variants before timing")
+    ap.add_argument("--tune-out", default=None, metavar="JSON",
+                    help="with --tune: merge the chosen variants into this table under the batch's shape key")
+    ap.add_argument("--no-table", action="store_true",
+                    help="ignore the committed tuned-variant table (exciting_amd/tuned.json); built-in defaults")
     ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
                     help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision measurements")
@@ -549,6 +557,14 @@ def main():
             ctx.hamiltonandoverlapsetup(kp, h_apw[i], out=h_ho[i])
 
     tuned = None
+    args.table = None
+    if args.path == "batched" and not args.tune and not args.set and not args.no_table:
+        # the committed table's variants for this batch shape (built-in defaults where it has none)
+        from exciting_amd import autotune
+        args.table = autotune.apply(ctx, cr, [kp.ngp for kp in w.kpoints]) or None
+        if args.table:
+            log(f"tuned table {autotune.key(cr, [kp.ngp for kp in w.kpoints])}: {args.table}")
+            tuned = {k: v for k, v in args.table.items() if k in ("bmt", "ystage", "olp")} or None
     for kv in args.set:
         st, v = kv.split("=")
         ctx.set_variant(st, int(v))
@@ -560,6 +576,8 @@ def main():
         else:
             tuned = autotune.tune(ctx, ["bmt", "ystage"] + (["olp"] if olpflag else []), lambda: step_batched(False),
                                   reps=2, log=log)
+            if args.tune_out:
+                autotune.save(args.tune_out, autotune.key(cr, [kp.ngp for kp in w.kpoints]), tuned)
     if args.path == "batched":
         # stage breakdown of one eager (uncaptured) grouped pass, then time the graph replays
         step_batched(False)

@@ -75,11 +75,12 @@ def tune(ctx, stages, run: Callable[[], None], reps: int = 3, log=None) -> Dict[
 
 
 def save(path: os.PathLike, key, chosen: Dict[str, int]):
+    """Merge `chosen` into the table's entry for `key` (stages it does not name are kept)."""
     table = {}
     if os.path.exists(path):
         with open(path) as f:
             table = json.load(f)
-    table[str(key)] = chosen
+    table[str(key)] = {**table.get(str(key), {}), **chosen}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
 
