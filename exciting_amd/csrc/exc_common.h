@@ -147,7 +147,7 @@ constexpr int kCrtLoDigits = 6;      // mixed-radix digits of the low part (thei
 constexpr int kCrtMaxUsed = 12;      // beyond 12 moduli beta > 46: row integers (and re +- im) no
                                      // longer exact in FP64, the high part of the reconstruction
                                      // no longer below 2^53
-constexpr int kCrtVariants = 5;      // tile / staging variants of the residue GEMM
+constexpr int kCrtVariants = 8;      // tile / staging variants of the residue GEMM
 struct CrtTab {                      // passed by value to the kernels (kernel-argument segment)
   int n, beta, nlo, pad_;
   int m[kMaxMod];

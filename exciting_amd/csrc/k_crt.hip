@@ -282,9 +282,9 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
 }
 
 // ---------------------------------------------------------------------------------------
-// merge: one tile per workgroup (256 threads).  Thread handles the 32 x 32 blocks' lanes of
-// one gemm layout: block index b = 0 .. 15 (wave-major as the gemm wrote them) decoded with
-// that variant's wave grid (wr, wc, i, j), lane, and the 16 values v of the lane.
+// merge: one tile per workgroup (4 waves).  Wave w takes the 32 x 32 blocks w, w + 4, ...
+// of the gemm layout (wave-major as the gemm wrote them, decoded with that variant's wave
+// grid), one lane per gemm lane with its 16 values.
 // ---------------------------------------------------------------------------------------
 template <int NM>
 __device__ __forceinline__ void garner(const int (&r)[NM], const CrtTab &t, long long &hi, long long &lo) {
@@ -342,13 +342,20 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
   const size_t ptile = (size_t)bt * bt;                     // residue bytes of one (modulus, product)
   const signed char *base = res + (size_t)blockIdx.x * NM * 3 * ptile;
   cplx *out = it.which ? k.o : k.h;
-  for (int c = threadIdx.x; c < nblk * 64; c += 256) {
-    const int blk = c >> 6, lane = c & 63;
-    const int wave = blk / per_wave, bi = (blk % per_wave) / nj_n, bj = blk % nj_n;
-    const int wr = wave / wc_n, wc = wave % wc_n;
-    const int cl = wc * (32 * nj_n) + 32 * bj + (lane & 31);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // Element (row, col) of a 32 x 32 block sits in lane (col mod 32) of the gemm layout, so the
+  // conjugate store at (col, row) is coalesced along the lanes; the store at (row, col) goes
+  // through this wave's LDS slice, 16 rows of the block at a time: stored [col][row] with one
+  // pad element per column, read back with 16 lanes along a column (256 contiguous bytes)
+  __shared__ cplx tb[4][32 * 17];
+  cplx *tw = tb[wave];
+  for (int b4 = 0; b4 < nblk; b4 += 4) {                    // the same trip count in every wave
+    const int blk = b4 + wave;
+    const int wv = blk / per_wave, bi = (blk % per_wave) / nj_n, bj = blk % nj_n;
+    const int wr = wv / wc_n, wc = wv % wc_n;
+    const int rb = wr * (32 * mi_n) + 32 * bi, cb = wc * (32 * nj_n) + 32 * bj;   // block origin in the tile
+    const int cl = cb + (lane & 31);
     const int col = col0 + cl;
-    if (col >= ngp) continue;
     uint4 q1[NM], q2[NM], q3[NM];
 #pragma unroll
     for (int l = 0; l < NM; ++l) {
@@ -358,44 +365,61 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
       q3[l] = *(const uint4 *)(b + 2 * ptile);
     }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int rl = wr * (32 * mi_n) + 32 * bi + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-      const int row = row0 + rl;
-      if (row > col) continue;
-      int dr[NM], di[NM];
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int l = 0; l < NM; ++l) {
-        const int g = v >> 2, sh = 8 * (v & 3);
-        const unsigned w1 = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
-        const unsigned w2 = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
-        const unsigned w3 = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
-        const int p1 = (signed char)((w1 >> sh) & 255), p2 = (signed char)((w2 >> sh) & 255);
-        const int p3 = (signed char)((w3 >> sh) & 255);
-        // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo m)
-        dr[l] = fres(p1 + p2, t.m[l], t.finv[l]);
-        di[l] = fres(p3 - p1 + p2, t.m[l], t.finv[l]);
-      }
-      long long hr, lr, hi, li;
-      garner<NM>(dr, t, hr, lr);
-      garner<NM>(di, t, hi, li);
-      const int sc = -(er[rl] + ec[cl]);
-      double re = ldexp(fma((double)hr, t.mk, (double)lr), sc);
-      double im = ldexp(fma((double)hi, t.mk, (double)li), sc);
-      const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
-      if (ig) {
-        const cplx th = gt.cfunig[ig - 1];
-        if (it.which) {
-          re += th.re;
-          im += th.im;
-        } else {
-          const cplx vv = gt.veffig[ig - 1];
-          const double t1 = 0.5 * (vr[rl][0] * vc[cl][0] + vr[rl][1] * vc[cl][1] + vr[rl][2] * vc[cl][2]);
-          re += vv.re + t1 * th.re;
-          im += vv.im + t1 * th.im;
+      for (int vv = 0; vv < 8; ++vv) {
+        const int v = 8 * h + vv;
+        const int rh = 8 * (vv >> 2) + 4 * (lane >> 5) + (v & 3);   // row within the half
+        const int rl = rb + 16 * h + rh;
+        const int row = row0 + rl;
+        cplx val = {0.0, 0.0};
+        if (col < ngp && row <= col) {
+          int dr[NM], di[NM];
+#pragma unroll
+          for (int l = 0; l < NM; ++l) {
+            const int g = v >> 2, sh = 8 * (v & 3);
+            const unsigned w1 = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
+            const unsigned w2 = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
+            const unsigned w3 = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
+            const int p1 = (signed char)((w1 >> sh) & 255), p2 = (signed char)((w2 >> sh) & 255);
+            const int p3 = (signed char)((w3 >> sh) & 255);
+            // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo m)
+            dr[l] = fres(p1 + p2, t.m[l], t.finv[l]);
+            di[l] = fres(p3 - p1 + p2, t.m[l], t.finv[l]);
+          }
+          long long hr, lr, hi, li;
+          garner<NM>(dr, t, hr, lr);
+          garner<NM>(di, t, hi, li);
+          const int sc = -(er[rl] + ec[cl]);
+          double re = ldexp(fma((double)hr, t.mk, (double)lr), sc);
+          double im = ldexp(fma((double)hi, t.mk, (double)li), sc);
+          const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
+          if (ig) {
+            const cplx th = gt.cfunig[ig - 1];
+            if (it.which) {
+              re += th.re;
+              im += th.im;
+            } else {
+              const cplx vv2 = gt.veffig[ig - 1];
+              const double t1 = 0.5 * (vr[rl][0] * vc[cl][0] + vr[rl][1] * vc[cl][1] + vr[rl][2] * vc[cl][2]);
+              re += vv2.re + t1 * th.re;
+              im += vv2.im + t1 * th.im;
+            }
+          }
+          val = cplx{re, im};
+          if (row < col) out[col + (size_t)row * k.ldh] = cplx{re, -im};
         }
+        tw[(lane & 31) * 17 + rh] = val;
       }
-      out[row + (size_t)col * k.ldh] = cplx{re, im};
-      if (row < col) out[col + (size_t)row * k.ldh] = cplx{re, -im};
+      __syncthreads();
+      // 16 rows x 32 columns of this half: lane -> row (lane & 15), columns (lane >> 4) + 4 i
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int cc = (lane >> 4) + 4 * i, r = lane & 15;
+        const int grow = row0 + rb + 16 * h + r, gcol = col0 + cb + cc;
+        if (gcol < ngp && grow <= gcol) out[grow + (size_t)gcol * k.ldh] = tw[cc * 17 + r];
+      }
+      __syncthreads();                                       // the slice is rewritten next
     }
   }
 }
@@ -412,6 +436,7 @@ int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
 template <int BT, int BK, int NST, int WR, int WC, int MINW>
 int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t, int npa,
                   signed char *res, hipStream_t s) {
+  if (kp % BK) return EXC_ERR_UNSUPPORTED;      // the k loop runs whole stages of BK bytes
   const int chunk = (nwork + 7) / 8;
   hipLaunchKernelGGL((crt_gemm_kernel<BT, BK, NST, WR, WC, MINW>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks,
                      items, work, nwork, kp, t, npa, res);
@@ -469,9 +494,12 @@ int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, co
 //   0: 256, 64, 3, 2 x 4 (128 x 64 wave tiles), 1    1: 256, 32, 4, 2 x 4, 1
 //   2: 128, 64, 3, 2 x 2 (64 x 64 wave tiles), 2     3: 256, 64, 2, 2 x 4, 1
 //   4: 128, 32, 4, 2 x 2, 2
+//   5: 256, 64, 3, 2 x 2 (128 x 128 wave tiles: two thirds of the LDS reads per MFMA), 1
+//   6: 256, 32, 4, 2 x 2, 1                           7: 256, 64, 4, 2 x 2, 1
 int exc_crt_shape(int variant, int *bt, int *wr, int *wc) {
   switch (variant) {
     case 0: case 1: case 3: *bt = 256; *wr = 2; *wc = 4; return EXC_OK;
+    case 5: case 6: case 7: *bt = 256; *wr = 2; *wc = 2; return EXC_OK;
     case 2: case 4: *bt = 128; *wr = 2; *wc = 2; return EXC_OK;
     default: return EXC_ERR_ARG;
   }
@@ -487,6 +515,9 @@ int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items,
     case 2: return launch_gemm_t<128, 64, 3, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
     case 3: return launch_gemm_t<256, 64, 2, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
     case 4: return launch_gemm_t<128, 32, 4, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 5: return launch_gemm_t<256, 64, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 6: return launch_gemm_t<256, 32, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 7: return launch_gemm_t<256, 64, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
     default: return EXC_ERR_ARG;
   }
 }
