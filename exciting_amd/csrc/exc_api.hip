@@ -2,9 +2,48 @@
 // uploads, match, the per-k and grouped setup, the adjoint and the stage timers.  The entry
 // points of the eigensolve, the density matrix and the per-SCF producers are in
 // exc_api_eigen.hip, exc_api_density.hip and exc_api_scf.hip.
+#include <mutex>
+
 #include "exc_ctx.h"
 
 using namespace exc_impl;
+
+// Fixed device workspace of exc_hamiltonandoverlapsetup_noalloc_dev (the Y stage's
+// Yt = A h^T, ngp x natm naa complex), for callers that cannot allocate.  In host code
+// `scratch` names a host-side stand-in, not device memory: exc_ctx_create takes its device
+// address once per device, with that device current (hipGetSymbolAddress), and only that
+// address is handed to kernels.  Every GPU the library runs on holds its own copy until the
+// process ends.
+static constexpr size_t kScratchBytes = (size_t)1 << 30;   // 1 GiB: ngp * natm * naa <= 64 Mi complex
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch[kMaxScratchDevices];   // device address of `scratch` per device (null: not taken yet)
+
+int register_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return EXC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> g(g_scratch_lock);
+  if (g_scratch[device]) return EXC_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "exc_lapw: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+            hipGetErrorString(e));
+    return EXC_ERR_HIP;
+  }
+  g_scratch[device] = p;
+  return EXC_OK;
+}
+
+void *scratch_of(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_lock);
+  return device >= 0 && device < kMaxScratchDevices ? g_scratch[device] : nullptr;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -25,6 +64,10 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
     return EXC_ERR_HIP;
   }
   c->stream = nullptr;
+  if (int rc = register_scratch(c->device)) {
+    delete c;
+    return rc;
+  }
   memset(&c->d, 0, sizeof(c->d));
   exc_dev &d = c->d;
   d.natm = m.natmtot;
@@ -483,34 +526,41 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
 namespace {
 
 // H (h given), O (o given) or both of one k-point from device arrays
+// ws: the Y stage's workspace; NULL takes the context's grow-only buffer.  With a caller's
+// workspace nothing is allocated and no stage timer runs (a timer may create its events).
 int setup_dev_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig, const double *vgpc, int ldh,
-                  double *h, double *o) {
+                  double *h, double *o, cplx *ws = nullptr) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   const int nmat = ngp + c->d.nlotot;
   if (ngp < 0 || ngkmax < ngp || ldh < nmat || (!h && !o) || (ngp > 0 && (!apwalm || !igpig || !vgpc)))
     return EXC_ERR_ARG;
   int rc = use_device(c);
   if (rc) return rc;
+  const bool untimed = ws != nullptr;   // a StageTimer of stage -1 records nothing
   if (ngp > 0) {
     // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA;
     // the MT tile with Gauss's three real products per complex product)
     const int yv = c->y_variant >= 0 ? c->y_variant : 5;
     const int mv = c->mt_variant >= 0 ? c->mt_variant : 8;
+    cplx *yt = ws;
     if (h) {   // the Y stage feeds only H
-      const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
-      if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
-      StageTimer t(c, EXC_STAGE_YSTAGE);
-      rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, yv, c->stream);
+      if (!ws) {
+        const size_t need = sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm;
+        if ((rc = grow((void **)&c->yt, &c->yt_bytes, need))) return rc;
+        yt = c->yt;
+      }
+      StageTimer t(c, untimed ? -1 : EXC_STAGE_YSTAGE);
+      rc = exc_launch_ystage(c->d, ngp, (const cplx *)apwalm, ngkmax, yt, ngp, yv, c->stream);
     }
     if (rc) return rc;
     {
-      StageTimer t(c, EXC_STAGE_MT);
-      rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, c->yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o, ldh,
-                         mv, c->stream);
+      StageTimer t(c, untimed ? -1 : EXC_STAGE_MT);
+      rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o, ldh, mv,
+                         c->stream);
     }
     if (rc) return rc;
   }
-  StageTimer t(c, EXC_STAGE_LO);
+  StageTimer t(c, untimed ? -1 : EXC_STAGE_LO);
   return exc_launch_lo(c->d, ngp, (const cplx *)apwalm, ngkmax, (cplx *)h, (cplx *)o, ldh, c->stream);
 }
 
@@ -564,6 +614,23 @@ int exc_hamiltonandoverlapsetup_dev(exc_ctx *c, int ngp, int ngkmax, const doubl
                                     const double *vgpc, int ldh, double *h, double *o) {
   if (!h || !o) return EXC_ERR_ARG;
   return setup_dev_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o);
+}
+
+int exc_hamiltonandoverlapsetup_noalloc_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,
+                                            const double *vgpc, int ldh, double *h, double *o) {
+  if (!h || !o) return EXC_ERR_ARG;
+  if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  const size_t need = sizeof(cplx) * (size_t)std::max(ngp, 0) * c->d.naa * c->d.natm;
+  if (need > kScratchBytes) {
+    fprintf(stderr,
+            "exc_lapw: exc_hamiltonandoverlapsetup_noalloc_dev: ngp %d needs %zu bytes of workspace, the fixed limit "
+            "is %zu (ngp * natm * naa <= %zu)\n",
+            ngp, need, kScratchBytes, kScratchBytes / sizeof(cplx));
+    return EXC_ERR_ARG;
+  }
+  cplx *ws = (cplx *)scratch_of(c->device);
+  if (!ws) return EXC_ERR_STATE;
+  return setup_dev_sel(c, ngp, ngkmax, apwalm, igpig, vgpc, ldh, h, o, ws);
 }
 
 int exc_hamiltonsetup_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const int *igpig,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "exc_match_dev": [_P, _I, _I, _P, _P, _P, _P],
     "exc_hamiltonandoverlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
     "exc_hamiltonandoverlapsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
+    "exc_hamiltonandoverlapsetup_noalloc_dev": [_P, _I, _I, _P, _P, _P, _I, _P, _P],
     "exc_hamiltonsetup": [_P, _I, _I, _P, _P, _P, _I, _P],
     "exc_hamiltonsetup_dev": [_P, _I, _I, _P, _P, _P, _I, _P],
     "exc_overlapsetup": [_P, _I, _I, _P, _P, _P, _I, _P],

@@ -148,6 +148,17 @@ int exc_hamiltonandoverlapsetup(exc_ctx *ctx, int ngp, int ngkmax, const double 
 int exc_hamiltonandoverlapsetup_dev(exc_ctx *ctx, int ngp, int ngkmax, const double *apwalm,
                                     const int *igpig, const double *vgpc, int ldh, double *h,
                                     double *o);
+/* No counterpart in the reference: exc_hamiltonandoverlapsetup_dev for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels, variants and results; the Y stage's workspace is a fixed
+ * device array of the library (1 GiB on each GPU, held until the process ends) instead of
+ * the context's grow-only buffer, and no stage timer runs.  ngp * natm * naa above 64 Mi
+ * (complex elements of that workspace) gives EXC_ERR_ARG before anything is queued.  Every
+ * call on one GPU shares the array: calls that overlap on one GPU, from any context or
+ * stream, race. */
+int exc_hamiltonandoverlapsetup_noalloc_dev(exc_ctx *ctx, int ngp, int ngkmax,
+                                            const double *apwalm, const int *igpig,
+                                            const double *vgpc, int ldh, double *h, double *o);
 /* hamiltonsetup.f90:8, Subroutine hamiltonsetup(system, ngp, apwalm, igpig, vgpc) — H alone
  * (muffin-tin APW-APW, APW-lo and lo-lo blocks, hmlistln.f90:13) — and overlapsetup.f90:8,
  * Subroutine overlapsetup(system, ngp, apwalm, igpig, vgpc) — O alone (olpistln.f90:13), the
