@@ -169,7 +169,14 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
         roof = {"bound": "mfma", "kernel": f"crt_gemm_kernel (residue products, {nmod} moduli, int8 MFMA)",
                 "achieved": ops / (ms * 1e-3) / 1e12, "peak": PEAK_INT8 / 1e12, "unit": "TOP/s",
                 "frac": ops / (ms * 1e-3) / PEAK_INT8, "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12,
-                "per_launch_ops": ops}
+                "per_launch_ops": ops, "traffic": None}
+        # HBM traffic of the residue GEMM from the committed rocprofv3 summary of the default command
+        prof = HERE / "profiles" / f"setup_kernels_workload{args.config}.json"
+        if args.nk is None and not args.tune and not args.set and not args.moduli and olpflag and prof.exists():
+            rec = json.loads(prof.read_text()).get("crt_gemm_kernel")
+            if rec:
+                roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
+                roof["traffic_source"] = str(prof.relative_to(HERE))
     else:
         ms = eager["bsplit"][0]
         ex = share * bsplit_alg_flops(cr, ngps)
@@ -564,7 +571,7 @@ def main():
         args.table = autotune.apply(ctx, cr, [kp.ngp for kp in w.kpoints]) or None
         if args.table:
             log(f"tuned table {autotune.key(cr, [kp.ngp for kp in w.kpoints])}: {args.table}")
-            tuned = {k: v for k, v in args.table.items() if k in ("bmt", "ystage", "olp")} or None
+            tuned = {k: v for k, v in args.table.items() if k in ("bmt", "ystage", "olp", "graph")} or None
     for kv in args.set:
         st, v = kv.split("=")
         ctx.set_variant(st, int(v))
@@ -576,6 +583,7 @@ def main():
         else:
             tuned = autotune.tune(ctx, ["bmt", "ystage"] + (["olp"] if olpflag else []), lambda: step_batched(False),
                                   reps=2, log=log)
+            tuned.update(autotune.tune(ctx, ["graph"], lambda: step_batched(True), reps=3, log=log))
             if args.tune_out:
                 autotune.save(args.tune_out, autotune.key(cr, [kp.ngp for kp in w.kpoints]), tuned)
     if args.path == "batched":

@@ -116,6 +116,7 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->crt_variant = -1;
   c->crt_nmod = 11;
   c->olp_variant = -1;
+  c->fork_variant = -1;
   c->eig_variant = -1;
   c->gnt_lmaxvr = -1;
   c->hml_lmaxvr = -1;
@@ -240,6 +241,10 @@ int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
       if (variant < -1 || variant >= kOlpVariants) return EXC_ERR_ARG;
       c->olp_variant = variant;
       return EXC_OK;
+    case EXC_STAGE_GRAPH:
+      if (variant < -1 || variant >= 2) return EXC_ERR_ARG;
+      c->fork_variant = variant;
+      return EXC_OK;
     default:
       return EXC_ERR_ARG;
   }
@@ -254,6 +259,7 @@ int exc_ctx_num_variants(int stage) {
     case EXC_STAGE_EIGEN: return kEigenVariants;
     case EXC_STAGE_CRTGEMM: return kCrtVariants;
     case EXC_STAGE_OLP: return kOlpVariants;
+    case EXC_STAGE_GRAPH: return 2;   // the overlap and lo kernels on a side branch, or in line
     default: return 0;
   }
 }
@@ -695,14 +701,15 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
   const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
   const int ov = c->olp_variant >= 0 ? c->olp_variant : 0;
+  const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
   CrtTab ct;
   if (crt) exc_crt_tables(c->crt_nmod, &ct);
   const int npa = olp ? 3 : 4;                               // residue planes of A (re + im for O)
   const int kp = exc_crt_kpad(c->d.natm * c->d.naa);
   // batch signature: descriptors, flags and variants
-  std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + 6 * sizeof(int));
+  int tail[7] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? c->crt_nmod : -1) + 100 * ov, fv};
+  std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + sizeof(tail));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
-  int tail[6] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? c->crt_nmod : -1) + 100 * ov};
   memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
   const bool same = (sig == c->batch_sig);
   if (same && (flags & EXC_BATCH_GRAPH) && c->graph_exec) {
@@ -875,8 +882,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // structure factors, and write parts of H and O the MT stage does not) form a branch of
   // their own beside the Y and MT stages; with a persistent MT kernel (static work split,
   // all workgroups assumed resident) the branches join before it.  Eager calls keep one
-  // stream so the stage timers stay exact
-  const bool fork = capture;
+  // stream so the stage timers stay exact.  With every stage filling the GPU the branches
+  // only compete (workload 3: 111 ms per replay with the branch, 105 ms as one chain; workload
+  // 1: 4.79 against 4.60 ms), so the default graph is one chain (variant 1 of
+  // EXC_STAGE_GRAPH) and the branch is variant 0
+  const bool fork = capture && fv == 0;
   const bool join_early = split || exc_bmt_persistent(bv);
   if (fork && !c->side) {
     EXC_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
@@ -935,11 +945,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (fork && !join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
     if (fork) return r;
     if (olp) {
-      StageTimer t(c, EXC_STAGE_OLP);
+      StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
       if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
     }
     {
-      StageTimer t(c, EXC_STAGE_LO);
+      StageTimer t(c, capture ? -1 : EXC_STAGE_LO);
       if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, c->stream))) return r;
     }
     return r;

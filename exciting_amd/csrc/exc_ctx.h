@@ -73,6 +73,7 @@ struct exc_ctx {
   // exponents, residues of every (tile, modulus), the tile and (tile, modulus) work lists
   int crt_nmod, crt_variant;
   int olp_variant;          // thread blocking of the addition-theorem overlap (-1: built-in)
+  int fork_variant;         // grouped graph: 0 overlap + lo on a side branch, 1 one chain (-1: 1)
   void *crt_q;
   size_t crt_q_bytes;
   int *crt_x;

@@ -87,6 +87,73 @@ __global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict_
 // split: 4 waves; wave w of workgroup (bx, by) converts rows 64 bx .. +63 of k-chunk
 // 4 by + w (16 values of k) into the residue planes of every modulus
 // ---------------------------------------------------------------------------------------
+// the low bytes of four ints as one word (byte 0 from a)
+__device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
+  return __builtin_amdgcn_perm((unsigned)b, (unsigned)a, 0x0c0c0400u) |
+         __builtin_amdgcn_perm((unsigned)d, (unsigned)c, 0x04000c0cu);
+}
+
+// symmetric residue of s, |s| <= 2h = m - 1
+__device__ __forceinline__ int fold(int s, int m, int h) { return s > h ? s - m : (s < -h ? s + m : s); }
+
+// Residue planes of one modulus for 16 values of k of one row.  OP 0: A (plane 2 = re - im,
+// the conjugated left operand; NP = 4 adds plane 3 = re + im for O's right operand), OP 1: Yt
+// (plane 2 = re + im).  m = 256 (l = 0) takes the low bytes of x mod 256 (the two's-complement
+// bytes of the symmetric residues); an odd m the exact x - m rint(x/m) (|x/m| is never within
+// 1/(2m) of a half-integer, far above the rounding of x/m).
+template <int OP, int NP>
+__device__ __forceinline__ void split_planes(const double (&xr)[16], const double (&xi)[16], const CrtTab &t,
+                                             signed char *base, size_t pstride, size_t off) {
+  {
+    unsigned w[NP][4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      int br[4], bi[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const double a = xr[4 * g + b], c = xi[4 * g + b];
+        br[b] = (int)fma(-256.0, floor(a * (1.0 / 256.0)), a);
+        bi[b] = (int)fma(-256.0, floor(c * (1.0 / 256.0)), c);
+      }
+      w[0][g] = pack4(br[0], br[1], br[2], br[3]);
+      w[1][g] = pack4(bi[0], bi[1], bi[2], bi[3]);
+      if (OP == 0)
+        w[2][g] = pack4(br[0] - bi[0], br[1] - bi[1], br[2] - bi[2], br[3] - bi[3]);
+      else
+        w[2][g] = pack4(br[0] + bi[0], br[1] + bi[1], br[2] + bi[2], br[3] + bi[3]);
+      if (NP == 4) w[NP - 1][g] = pack4(br[0] + bi[0], br[1] + bi[1], br[2] + bi[2], br[3] + bi[3]);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      *(uint4 *)(base + (size_t)p * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+  }
+#pragma unroll 1
+  for (int l = 1; l < t.n; ++l) {
+    const int m = t.m[l], h = (m - 1) >> 1;
+    const double dm = (double)m, mi = t.dminv[l];
+    unsigned w[NP][4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      int rr[4], ri[4], r2[4], r3[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const double a = xr[4 * g + b], c = xi[4 * g + b];
+        rr[b] = (int)fma(-dm, rint(a * mi), a);
+        ri[b] = (int)fma(-dm, rint(c * mi), c);
+        r2[b] = fold(OP == 0 ? rr[b] - ri[b] : rr[b] + ri[b], m, h);
+        if (NP == 4) r3[b] = fold(rr[b] + ri[b], m, h);
+      }
+      w[0][g] = pack4(rr[0], rr[1], rr[2], rr[3]);
+      w[1][g] = pack4(ri[0], ri[1], ri[2], ri[3]);
+      w[2][g] = pack4(r2[0], r2[1], r2[2], r2[3]);
+      if (NP == 4) w[NP - 1][g] = pack4(r3[0], r3[1], r3[2], r3[3]);
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      *(uint4 *)(base + (size_t)(l * NP + p) * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+  }
+}
+
 __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t,
                                                         int npa) {
   const kdesc k = ks[blockIdx.z >> 1];
@@ -95,7 +162,6 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict_
   const int row = blockIdx.x * 64 + lane;
   const int kc = blockIdx.y * 4 + wave;
   if (blockIdx.x * 64 >= k.cnp || kc * 16 >= kp) return;
-  const int np = op ? 3 : npa;                 // planes: re, im, re -+ im (and re + im of A for O)
   const cplx *src = op ? k.yt : k.apwalm;
   const int ld = op ? k.ldy : k.ldg;
   const int s = row < k.ngp ? (op ? k.xy : k.xa)[row] : 0;
@@ -111,37 +177,12 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict_
   signed char *base = (signed char *)(op ? k.qy : k.qa);
   const size_t pstride = (size_t)kp * k.cnp;            // bytes of one plane
   const size_t off = ((size_t)kc * k.cnp + row) * 16;
-#pragma unroll 1
-  for (int l = 0; l < t.n; ++l) {
-    const int m = t.m[l];
-    const double mi = t.dminv[l];
-    const float fi = t.finv[l];
-    // residues of re and im from the integers; those of re -+ im from theirs ((a - b) mod m =
-    // (a mod m - b mod m) mod m): plane 2 is re - im for A (the conjugated left operand) and
-    // re + im for Yt, plane 3 (A only, for O's right operand) re + im
-    unsigned w[4][4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      unsigned u0 = 0, u1 = 0, u2 = 0, u3 = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int q = 4 * g + b;
-        const int rr = dres(xr[q], m, mi), ri = dres(xi[q], m, mi);
-        u0 |= (unsigned)(rr & 255) << (8 * b);
-        u1 |= (unsigned)(ri & 255) << (8 * b);
-        u2 |= (unsigned)(fres(op == 0 ? rr - ri : rr + ri, m, fi) & 255) << (8 * b);
-        u3 |= (unsigned)(fres(rr + ri, m, fi) & 255) << (8 * b);
-      }
-      w[0][g] = u0;
-      w[1][g] = u1;
-      w[2][g] = u2;
-      w[3][g] = u3;
-    }
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-      if (p < np)
-        *(uint4 *)(base + (size_t)(l * np + p) * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
-  }
+  if (op)
+    split_planes<1, 3>(xr, xi, t, base, pstride, off);
+  else if (npa == 3)
+    split_planes<0, 3>(xr, xi, t, base, pstride, off);
+  else
+    split_planes<0, 4>(xr, xi, t, base, pstride, off);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -286,31 +327,93 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
 // of the gemm layout (wave-major as the gemm wrote them, decoded with that variant's wave
 // grid), one lane per gemm lane with its 16 values.
 // ---------------------------------------------------------------------------------------
-template <int NM>
-__device__ __forceinline__ void garner(const int (&r)[NM], const CrtTab &t, long long &hi, long long &lo) {
-  int d[NM];
-  d[0] = r[0];
-#pragma unroll
-  for (int l = 1; l < NM; ++l) {
-    int acc = r[l];
-#pragma unroll
-    for (int j = 0; j < l; ++j) acc -= d[j] * t.w[l][j];
-    const int u = fres(acc, t.m[l], t.finv[l]);
-    d[l] = fres(u * t.v[l], t.m[l], t.finv[l]);
+// The moduli and Garner's tables are fixed at compile time for each modulus count (the
+// moduli list is constant; only how many of them is chosen), so every reduction below
+// multiplies by immediates: w[l][j] = M_j mod m_l, v[l] = M_l^-1 mod m_l, M_j = m_0 ... m_(j-1).
+constexpr int kModList[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
+
+constexpr int inv_mod(int a, int m) {   // a^-1 mod m, gcd(a, m) = 1 (extended Euclid)
+  int r0 = m, r1 = a % m, t0 = 0, t1 = 1;
+  while (r1) {
+    const int q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
+    r0 = r1; r1 = r2; t0 = t1; t1 = t2;
   }
-  constexpr int NLO = NM < kCrtLoDigits ? NM : kCrtLoDigits;
-  lo = 0;
-#pragma unroll
-  for (int j = NLO - 1; j >= 0; --j) lo = lo * t.m[j] + d[j];
-  hi = 0;
-#pragma unroll
-  for (int j = NM - 1; j >= NLO; --j) hi = hi * t.m[j] + d[j];
+  return t0 < 0 ? t0 + m : t0;
 }
 
 template <int NM>
+struct GarnerTab {
+  int w[NM][NM];    // w[l][j] = (M_j mod m_l) v[l] mod m_l
+  int v[NM];
+  double mk;        // M_6 (or P for fewer moduli): x = x_hi M_6 + x_lo
+  constexpr GarnerTab() : w(), v(), mk(1.0) {
+    for (int l = 1; l < NM; ++l) {
+      long long ml = 1;
+      int mj[NM] = {};
+      for (int j = 0; j < l; ++j) {
+        mj[j] = (int)(ml % kModList[l]);
+        ml = ml * kModList[j] % kModList[l];
+      }
+      v[l] = inv_mod((int)ml, kModList[l]);
+      for (int j = 0; j < l; ++j) w[l][j] = mj[j] * v[l] % kModList[l];
+    }
+    for (int j = 0; j < (NM < kCrtLoDigits ? NM : kCrtLoDigits); ++j) mk *= kModList[j];
+  }
+};
+
+// symmetric residue modulo m = kModList[L] of |x| < 2^22 (L = 0: the low byte); the float
+// quotient is within 2^-21 |x| / m of x / m, far inside the 1/(2m) margin of the rounding
+template <int L>
+__device__ __forceinline__ int cres(int x) {
+  constexpr int m = kModList[L];
+  if constexpr (L == 0) {
+    return (int)(signed char)(x & 255);
+  } else {
+    constexpr float finv = 1.0f / (float)m;
+    return x - __mul24((int)rintf((float)x * finv), m);
+  }
+}
+
+// Garner's digit d_l = (r_l - sum_j d_j M_j) M_l^-1 mod m_l, formed as
+// r_l v_l - sum_j d_j (M_j v_l mod m_l) and reduced once: r need not be reduced itself
+// (|r| <= 384, |d_j| <= 128, so the sum stays below 2^22 for up to 12 moduli)
+template <int NM, int L>
+__device__ __forceinline__ void garner_digit(int (&d)[NM], const int (&r)[NM]) {
+  constexpr GarnerTab<NM> G{};
+  if constexpr (L == 0) {
+    d[0] = cres<0>(r[0]);
+  } else {
+    int acc = __mul24(r[L], G.v[L]);
+#pragma unroll
+    for (int j = 0; j < L; ++j) acc -= __mul24(d[j], G.w[L][j]);
+    d[L] = cres<L>(acc);
+  }
+  if constexpr (L + 1 < NM) garner_digit<NM, L + 1>(d, r);
+}
+
+// x in [-P/2, P/2) from values r_l congruent to it modulo each m_l: Garner's mixed-radix
+// digits, then x as the correctly rounded fma(x_hi, M_6, x_lo), the two parts exact in FP64
+// (|x_lo| < M_6 / 2 < 2^52)
+template <int NM>
+__device__ __forceinline__ double garner(const int (&r)[NM]) {
+  constexpr GarnerTab<NM> G{};
+  int d[NM];
+  garner_digit<NM, 0>(d, r);
+  constexpr int NLO = NM < kCrtLoDigits ? NM : kCrtLoDigits;
+  double lo = (double)d[NLO - 1];
+#pragma unroll
+  for (int j = NLO - 2; j >= 0; --j) lo = fma(lo, (double)kModList[j], (double)d[j]);
+  double hi = 0.0;
+#pragma unroll
+  for (int j = NM - 1; j >= NLO; --j) hi = fma(hi, (double)kModList[j], (double)d[j]);
+  return fma(hi, G.mk, lo);
+}
+
+
+template <int NM>
 __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
-                                                        CrtTab t, const signed char *__restrict__ res, int bt, int wr_n,
-                                                        int wc_n, GTab gt) {
+                                                        const signed char *__restrict__ res, int bt, int wr_n, int wc_n,
+                                                        GTab gt) {
   const mt_item it = items[blockIdx.x];
   const kdesc k = ks[it.k];
   const int ngp = k.ngp;
@@ -374,25 +477,26 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
         const int row = row0 + rl;
         cplx val = {0.0, 0.0};
         if (col < ngp && row <= col) {
+          // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo each m)
+          const int g = v >> 2, sh = 8 * (v & 3);
+          unsigned w1[NM], w2[NM], w3[NM];
+#pragma unroll
+          for (int l = 0; l < NM; ++l) {
+            w1[l] = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
+            w2[l] = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
+            w3[l] = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
+          }
           int dr[NM], di[NM];
 #pragma unroll
           for (int l = 0; l < NM; ++l) {
-            const int g = v >> 2, sh = 8 * (v & 3);
-            const unsigned w1 = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
-            const unsigned w2 = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
-            const unsigned w3 = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
-            const int p1 = (signed char)((w1 >> sh) & 255), p2 = (signed char)((w2 >> sh) & 255);
-            const int p3 = (signed char)((w3 >> sh) & 255);
-            // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo m)
-            dr[l] = fres(p1 + p2, t.m[l], t.finv[l]);
-            di[l] = fres(p3 - p1 + p2, t.m[l], t.finv[l]);
+            const int p1 = (signed char)(w1[l] >> sh), p2 = (signed char)(w2[l] >> sh);
+            const int p3 = (signed char)(w3[l] >> sh);
+            dr[l] = p1 + p2;                 // congruent to Re modulo m_l
+            di[l] = p3 - p1 + p2;            // congruent to Im
           }
-          long long hr, lr, hi, li;
-          garner<NM>(dr, t, hr, lr);
-          garner<NM>(di, t, hi, li);
           const int sc = -(er[rl] + ec[cl]);
-          double re = ldexp(fma((double)hr, t.mk, (double)lr), sc);
-          double im = ldexp(fma((double)hi, t.mk, (double)li), sc);
+          double re = ldexp(garner<NM>(dr), sc);
+          double im = ldexp(garner<NM>(di), sc);
           const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
           if (ig) {
             const cplx th = gt.cfunig[ig - 1];
@@ -427,7 +531,7 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
 template <int NM>
 int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
                    const signed char *res, int bt, int wr, int wc, hipStream_t s) {
-  hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, t, res, bt, wr, wc,
+  hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, res, bt, wr, wc,
                      setup_dev::make_gtab(d));
   EXC_CHECK(hipGetLastError());
   return EXC_OK;

@@ -404,7 +404,9 @@ int exc_ctx_set_moduli(exc_ctx *ctx, int n);
 int exc_crt_moduli(int n, int *moduli, int *beta);
 
 /* Tile-shape variant of a stage's kernel (EXC_STAGE_YSTAGE, EXC_STAGE_MT, EXC_STAGE_BMT,
- * EXC_STAGE_BSPLIT, EXC_STAGE_CRTGEMM, EXC_STAGE_OLP), as chosen by
+ * EXC_STAGE_BSPLIT, EXC_STAGE_CRTGEMM, EXC_STAGE_OLP), and for EXC_STAGE_GRAPH the shape of
+ * the grouped hipGraph (0: the overlap and lo kernels on a branch of their own beside the
+ * Y and muffin-tin stages, 1: all stages in one chain), as chosen by
  * the autotuner (exciting_amd/autotune.py); -1 restores the built-in choice by basis size.
  * exc_ctx_num_variants returns how many variants a stage has.  No reference counterpart. */
 int exc_ctx_set_variant(exc_ctx *ctx, int stage, int variant);

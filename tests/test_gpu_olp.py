@@ -59,12 +59,16 @@ def test_olp_legendre_against_oracle(which):
         assert np.array_equal(O[il], np.conj(O.T[il]))
 
 
-def test_olp_legendre_graph_replays():
+@pytest.mark.parametrize("shape", [0, 1])
+def test_olp_legendre_graph_replays(shape):
+    """Graph replays equal the eager launch bit for bit, with the overlap and lo kernels on a
+    branch of their own (graph variant 0) or in line (1)."""
     import torch
     from exciting_amd import lapw
     w = workload(1, 4)
     flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE
     ctx, s, b, eager = _run(w, flags)
+    ctx.set_variant("graph", shape)
     with torch.cuda.stream(s):
         for rep in range(3):
             for t in b.H + b.O:
@@ -73,7 +77,7 @@ def test_olp_legendre_graph_replays():
             (x @ x).sum().item()
             ctx.setup_batched_dev(b, flags | lapw.EXC_BATCH_GRAPH)
             torch.cuda.synchronize()
-            print(f"graph replay {rep}", flush=True)
+            print(f"graph variant {shape} replay {rep}", flush=True)
             for i in range(b.nk):
                 assert torch.equal(b.H[i].cpu(), eager[i][1]) and torch.equal(b.O[i].cpu(), eager[i][2])
 
