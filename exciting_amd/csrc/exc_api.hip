@@ -700,7 +700,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 19);
   const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
   const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
-  const int ov = c->olp_variant >= 0 ? c->olp_variant : 0;
+  const int ov = c->olp_variant >= 0 ? c->olp_variant : 2;   // rows along lanes, column factors as scalars
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
   CrtTab ct;
   if (crt) exc_crt_tables(c->crt_nmod, &ct);
@@ -741,7 +741,12 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if ((rc = grow((void **)&c->planes, &c->planes_bytes, 2 * (ptot ? ptot : 1)))) return rc;
       if ((rc = grow((void **)&c->expo, &c->expo_bytes, sizeof(int) * (etot ? etot : 1)))) return rc;
     }
-    if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot ? rtot : 1)))) return rc;
+    // radial factors and their transposed copy (overlap variant 2: N rounded up to 64 rows per
+    // atom and l)
+    size_t rttot = 0;
+    for (int k = 0; k < nk; ++k) rttot += (size_t)(ks[k].ngp + 63) / 64 * 64 * c->d.natm * nl2;
+    if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot + rttot ? rtot + rttot : 1))))
+      return rc;
     size_t qtot = 0, xtot = 0;
     int cbt = 0, cwr = 0, cwc = 0;
     if (crt && exc_crt_shape(cv, &cbt, &cwr, &cwc)) return EXC_ERR_ARG;
@@ -762,7 +767,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     std::vector<mt_item> items, oitems;
     const int T = split ? exc_bsplit_tile(bv) : exc_bmt_tile(bv);
     const int TO = exc_olp_tile();
-    size_t yoff = 0, poff = 0, eoff = 0, roff = 0;
+    size_t yoff = 0, poff = 0, eoff = 0, roff = 0, rtoff = 0;
     int maxngp = 0;
     for (int k = 0; k < nk; ++k) {
       const exc_kpoint_dev &q = ks[k];
@@ -792,10 +797,12 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         e.ey = c->expo + eoff + (size_t)q.ngp * c->d.natm;
         eoff += 2 * (size_t)q.ngp * c->d.natm;
       }
-      e.rf = nullptr;
+      e.rf = e.rft = nullptr;
       if (olp) {
         e.rf = c->rfbuf + roff;
+        e.rft = c->rfbuf + rtot + rtoff;
         roff += (size_t)q.ngp * c->d.natm * nl2;
+        rtoff += (size_t)(q.ngp + 63) / 64 * 64 * c->d.natm * nl2;
       }
       e.cnp = 0;
       e.qa = e.qy = nullptr;
@@ -904,7 +911,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (fork) {
       if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return EXC_ERR_HIP;
       if (hipStreamWaitEvent(c->side, c->ev_fork, 0) != hipSuccess) return EXC_ERR_HIP;
-      if (olp && (r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, ov, s2))) return r;
+      if (olp && (r = exc_launch_olp(c->d, nk, c->batch_maxngp, c->d_ks, c->d_oitems, c->batch_noitems, ov, s2))) return r;
       if ((r = exc_launch_blo(c->d, nk, c->batch_maxngp, c->d_ks, s2))) return r;
       if (hipEventRecord(c->ev_join, c->side) != hipSuccess) return EXC_ERR_HIP;
     }
@@ -946,7 +953,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (fork) return r;
     if (olp) {
       StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
-      if ((r = exc_launch_olp(c->d, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
+      if ((r = exc_launch_olp(c->d, nk, c->batch_maxngp, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
     }
     {
       StageTimer t(c, capture ? -1 : EXC_STAGE_LO);

@@ -88,6 +88,7 @@ struct kdesc {
   // radial factors of apwalm for the overlap by the addition theorem (EXC_BATCH_OLP_LEGENDRE),
   // written by the grouped match: [(atom*ngp + row)*2(lmaxmat+1) + 2l + io]
   double *rf;
+  double *rft;               // the same per (atom, l) with rows fastest, times sqrt(gamma_l) (overlap variant 2)
   // modular split (EXC_BATCH_SPLIT_CRT): rows padded to cnp (a multiple of 256), int8 residue
   // planes of A and Yt, layout [modulus][plane][K/16][cnp][16], and per-row scale exponents
   int cnp, pad_;
@@ -137,8 +138,9 @@ int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
 int exc_bsplit_tile(int variant);
 int exc_bsplit_max_blocks(int variant);
 int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
-constexpr int kOlpVariants = 2;
-int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s);
+constexpr int kOlpVariants = 3;
+int exc_launch_olp(const exc_dev &d, int nk, int maxngp, const kdesc *ks, const mt_item *items, int nitems, int variant,
+                   hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
 // modular split of the muffin-tin contraction (k_crt.hip)

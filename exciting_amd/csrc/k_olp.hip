@@ -12,11 +12,12 @@
 // same matrix as the reference's conj(A) A^T contraction (olpaa, overlapsetup), a different
 // rounding: about 440 FP64 FMAs per element instead of 8 natm naa flops, on the VALU.
 //
-// Workgroup: one 64x64 tile of the upper triangle, 4 x CJ elements per thread (rows tr + 16 i,
-// columns tc + (64 / CJ) j; variant 0: CJ = 2, 512 threads; variant 1: CJ = 4, 256 threads,
-// half the LDS reads per FMA); per atom the rows' and columns' radial factors and structure
-// factors are brought to LDS by DMA, the next atom's while the current one is evaluated (two
-// stages).  Epilogue as the MFMA tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
+// Variants 0 and 1 — workgroup: one 64x64 tile of the upper triangle, 4 x CJ elements per
+// thread (rows tr + 16 i, columns tc + (64 / CJ) j; variant 0: CJ = 2, 512 threads; variant
+// 1: CJ = 4, 256 threads, half the LDS reads per FMA); per atom the rows' and columns' radial
+// factors and structure factors are brought to LDS by DMA, the next atom's while the current
+// one is evaluated (two stages).  Variant 2 (the default, below) keeps the atom loop out of
+// LDS.  Epilogue as the MFMA tile: + Theta(G_i - G_j), store (i, j) and conj at (j, i).
 #include "setup_dev.h"
 #include "mt_dma.h"
 
@@ -190,11 +191,193 @@ __global__ __launch_bounds__(64 * 64 / (4 * CJ)) __attribute__((amdgpu_waves_per
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Variant 2: lanes along rows.  One 64 x 64 tile per workgroup of 8 waves; lane = row, wave w
+// takes the 8 columns col0 + 8w .. +7, the same in all its lanes, so the columns' radial and
+// structure factors are wave-uniform scalars: they are read through the scalar cache
+// (constant-address-space loads: the factors are written by earlier kernels and only read
+// here) and the atom loop touches no LDS (the LDS reads of variants 0 and 1 bound them).  The
+// l-sum runs Clenshaw's recurrence on b~_l = gamma_l b_l with gamma_0 = 1,
+// gamma_(l+1) = gamma_l (2l+1)/(l+1), which makes the coefficient of x b~_(l+1) one:
+//   b~_l = gamma_l c_l + x b~_(l+1) + beta~_l b~_(l+2),  beta~_l = -(l+1)^2 / ((2l+1)(2l+3)),
+// and b~_0 = b_0 = sum_l c_l P_l(x): four FP64 operations per element and l instead of five
+// (rounded differently from variants 0 and 1).  The radial factors are first copied to
+// rft[((atom nl + l) N + G) 2 + io] times sqrt(gamma_l) (olp_rft_kernel): one l of 64 rows is
+// then 1 KiB contiguous, and the 8 columns of a wave 128 contiguous bytes.
+// ---------------------------------------------------------------------------------------
+struct ScaledClenshawTab {
+  double gamma[kNLmax], beta[kNLmax];
+};
+constexpr ScaledClenshawTab make_scaled_clenshaw() {
+  ScaledClenshawTab t{};
+  double g = 1.0;
+  for (int l = 0; l < kNLmax; ++l) {
+    t.gamma[l] = g;
+    g = g * (double)(2 * l + 1) / (double)(l + 1);
+    t.beta[l] = -(double)((l + 1) * (l + 1)) / (double)((2 * l + 1) * (2 * l + 3));
+  }
+  return t;
+}
+__constant__ ScaledClenshawTab kScaled = make_scaled_clenshaw();
+
+typedef const __attribute__((address_space(4))) double *cdbl_ptr;   // scalar-cache (constant) reads
+
+// rows of rft per (atom, l): N rounded up to the 64 of a tile, the padding zero, so the
+// columns of a tile never need clamping
+__device__ __forceinline__ int rft_ld(int ngp) { return (ngp + kOT - 1) / kOT * kOT; }
+
+// rf (per atom and row, l and io fastest) -> rft (per atom and l, rows fastest), times sqrt(gamma_l)
+__global__ __launch_bounds__(256) void olp_rft_kernel(const kdesc *__restrict__ ks, int nl) {
+  const kdesc k = ks[blockIdx.z];
+  const int row = blockIdx.x * 256 + threadIdx.x, at = blockIdx.y, ld = rft_ld(k.ngp);
+  if (row >= ld) return;
+  const d2 *src = (const d2 *)(k.rf + ((size_t)at * k.ngp + row) * 2 * nl);
+  d2 *dst = (d2 *)k.rft;
+  for (int l = 0; l < nl; ++l) {
+    const double g = sqrt(kScaled.gamma[l]);
+    const d2 v = row < k.ngp ? src[l] : d2{0.0, 0.0};
+    dst[((size_t)at * nl + l) * ld + row] = d2{v.a * g, v.b * g};
+  }
+}
+
+template <int NL>
+__global__ __launch_bounds__(512) void olp_rows_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                       int nitems, int natm, GTab gt) {
+  constexpr int E = 8;                                  // columns per wave
+  __shared__ double uc[kOT][3];
+  __shared__ int gc[kOT][3];
+  __shared__ cplx tr[8][E * 65];                        // mirror transpose, per wave [column][row] (+1 pad)
+  const int item = blockIdx.x;
+  if (item >= nitems) return;
+  const mt_item w = items[item];
+  const kdesc k = ks[w.k];
+  const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int row = row0 + lane, rowc = min(row, ngp - 1);
+  const int cw = col0 + E * wave;                       // first column of this wave
+  if (threadIdx.x < kOT) {
+    const int q = threadIdx.x, idx = col0 + q;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    int g0 = 0, g1 = 0, g2 = 0;
+    if (idx < ngp) {
+      double st, ct, sp, cp;
+      sincos(k.tpgpc[2 * idx], &st, &ct);
+      sincos(k.tpgpc[2 * idx + 1], &sp, &cp);
+      u0 = st * cp; u1 = st * sp; u2 = ct;
+      const int ig = k.igpig[idx] - 1;
+      g0 = gt.ivg[3 * ig]; g1 = gt.ivg[3 * ig + 1]; g2 = gt.ivg[3 * ig + 2];
+    }
+    uc[q][0] = u0; uc[q][1] = u1; uc[q][2] = u2;
+    gc[q][0] = g0; gc[q][1] = g1; gc[q][2] = g2;
+  }
+  double ur0, ur1, ur2;
+  {
+    double st, ct, sp, cp;
+    sincos(k.tpgpc[2 * rowc], &st, &ct);
+    sincos(k.tpgpc[2 * rowc + 1], &sp, &cp);
+    ur0 = st * cp; ur1 = st * sp; ur2 = ct;
+  }
+  __syncthreads();
+  double x[E], are[E], aim[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = E * wave + e;
+    x[e] = ur0 * uc[q][0] + ur1 * uc[q][1] + ur2 * uc[q][2];
+    are[e] = aim[e] = 0.0;
+  }
+  // columns past N read the zero padding of rft (their products are never stored)
+  const int ld = rft_ld(ngp);
+  const d2 *rft = (const d2 *)k.rft;
+#pragma unroll 1
+  for (int at = 0; at < natm; ++at) {
+    const cplx sr = k.sfacgp[rowc + (size_t)k.ldg * at];
+    double b1[E], b2[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) b1[e] = b2[e] = 0.0;
+    // the columns' factors of one l in SGPRs, those of the next l loaded while this one is
+    // used; the scheduling barriers keep the scalar loads from being hoisted further (the
+    // registers would spill) and let the rows' vector loads move ahead (mask: VMEM reads)
+    auto col = [&](int l, d2 (&c)[E]) {
+      cdbl_ptr pc = (cdbl_ptr)(rft + ((size_t)at * NL + l) * ld + cw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) c[e] = d2{pc[2 * e], pc[2 * e + 1]};
+    };
+    d2 cur[E], nxt[E];
+    col(NL - 1, cur);
+#pragma unroll
+    for (int l = NL - 1; l >= 0; --l) {
+      const d2 r = rft[((size_t)at * NL + l) * ld + rowc];
+      if (l > 0) col(l - 1, nxt);
+      const double bt = kScaled.beta[l];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double a = fma(r.b, cur[e].b, r.a * cur[e].a);
+        const double b0 = fma(bt, b2[e], fma(x[e], b1[e], a));
+        b2[e] = b1[e];
+        b1[e] = b0;
+      }
+      __builtin_amdgcn_sched_barrier(0x0020);
+#pragma unroll
+      for (int e = 0; e < E; ++e) cur[e] = nxt[e];
+    }
+    // structure factors of the columns (rows ngp..ngkmax-1 of sfacgp are the caller's padding
+    // and may be read: their products are never stored)
+    cdbl_ptr ps = (cdbl_ptr)(k.sfacgp + (size_t)k.ldg * at);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int ce = cw + e < k.ldg ? cw + e : 0;
+      const cplx sc = {ps[2 * ce], ps[2 * ce + 1]};
+      const double t1 = sc.re * b1[e], t2 = sc.im * b1[e];   // conj(S(G)) S(G') sum_l P_l c_l
+      are[e] = fma(sr.re, t1, fma(sr.im, t2, are[e]));
+      aim[e] = fma(sr.re, t2, fma(-sr.im, t1, aim[e]));
+    }
+  }
+  // + Theta(G_i - G_j); store (i, j) (lanes along the column: coalesced) and conj at (j, i)
+  // through this wave's LDS slice (8 lanes along each row of the mirror)
+  int gr0 = 0, gr1 = 0, gr2 = 0;
+  {
+    const int ig = k.igpig[rowc] - 1;
+    gr0 = gt.ivg[3 * ig]; gr1 = gt.ivg[3 * ig + 1]; gr2 = gt.ivg[3 * ig + 2];
+  }
+  cplx *tw = tr[wave];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = E * wave + e, c = col0 + q;
+    double re = are[e], im = aim[e];
+    if (c < ngp && row <= c) {
+      const int ig = lookup_g(gt, gr0 - gc[q][0], gr1 - gc[q][1], gr2 - gc[q][2]);
+      if (ig) {
+        const cplx th = gt.cfunig[ig - 1];
+        re += th.re;
+        im += th.im;
+      }
+      k.o[row + (size_t)c * k.ldh] = cplx{re, im};
+    }
+    tw[e * 65 + lane] = cplx{re, -im};
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int e = lane & 7, r = (lane >> 3) + 8 * i;     // row r of the tile, column e of the wave
+    const int gr = row0 + r, c = cw + e;
+    if (c < ngp && gr < c) k.o[c + (size_t)gr * k.ldh] = tw[e * 65 + r];
+  }
+}
+
+template <int NL>
+void launch_rows(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
+  hipLaunchKernelGGL(olp_rows_kernel<NL>, dim3(nitems), dim3(512), 0, s, ks, items, nitems, d.natm, make_gtab(d));
+}
+
 }  // namespace
 
 int exc_olp_tile() { return kOT; }
 
-int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s) {
+int exc_launch_olp(const exc_dev &d, int nk, int maxngp, const kdesc *ks, const mt_item *items, int nitems, int variant,
+                   hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
   if (d.lmaxmat + 1 > kNLmax || d.apwordmax != 2) return EXC_ERR_UNSUPPORTED;
   switch (variant) {
@@ -205,6 +388,26 @@ int exc_launch_olp(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
     case 1:
       hipLaunchKernelGGL(olp_kernel<4>, dim3(nitems), dim3(256), 0, s, ks, items, nitems, d.natm, d.lmaxmat + 1,
                          make_gtab(d));
+      break;
+    case 2:
+      hipLaunchKernelGGL(olp_rft_kernel, dim3((maxngp + kOT - 1 + 255) / 256, d.natm, nk), dim3(256), 0, s, ks,
+                         d.lmaxmat + 1);
+      EXC_CHECK(hipGetLastError());
+      switch (d.lmaxmat + 1) {
+        case 2: launch_rows<2>(d, ks, items, nitems, s); break;
+        case 3: launch_rows<3>(d, ks, items, nitems, s); break;
+        case 4: launch_rows<4>(d, ks, items, nitems, s); break;
+        case 5: launch_rows<5>(d, ks, items, nitems, s); break;
+        case 6: launch_rows<6>(d, ks, items, nitems, s); break;
+        case 7: launch_rows<7>(d, ks, items, nitems, s); break;
+        case 8: launch_rows<8>(d, ks, items, nitems, s); break;
+        case 9: launch_rows<9>(d, ks, items, nitems, s); break;
+        case 10: launch_rows<10>(d, ks, items, nitems, s); break;
+        case 11: launch_rows<11>(d, ks, items, nitems, s); break;
+        case 12: launch_rows<12>(d, ks, items, nitems, s); break;
+        case 13: launch_rows<13>(d, ks, items, nitems, s); break;
+        default: return EXC_ERR_UNSUPPORTED;
+      }
       break;
     default:
       return EXC_ERR_ARG;
