@@ -104,11 +104,11 @@ def lowest_eigs(H, O, nev=200):
 
 
 def crt_alg_ops(cr, ngps, nmod, share):
-    """Algorithmic INT8 MFMA ops of the residue products: per modulus three real products
-    (Gauss's form) over the upper triangle (N (N + 1) / 2 elements) and K = natm * maxaa, two
-    ops per multiply-add; share = 0.5 when O comes from the addition theorem (H only)."""
+    """Algorithmic INT8 MFMA ops of the residue products: per modulus two real products (the
+    two maps of Z_m[i]) over the upper triangle (N (N + 1) / 2 elements) and K = natm * maxaa,
+    two ops per multiply-add; share = 0.5 when O comes from the addition theorem (H only)."""
     K = cr.natm * cr.maxaa
-    return sum(2.0 * 3 * nmod * K * n * (n + 1) / 2 for n in ngps) * (2 * share)
+    return sum(2.0 * 2 * nmod * K * n * (n + 1) / 2 for n in ngps) * (2 * share)
 
 
 def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt"):
@@ -125,9 +125,9 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
         st, v = kv.split("=")
         ctx.set_variant(st, int(v))
         tuned = dict(tuned or {}, **{st: int(v)})
-    if kind == "crt" and args.moduli:
-        ctx.set_moduli(args.moduli)
-    nmod = args.moduli or 11
+    nmod = args.moduli
+    if kind == "crt":
+        ctx.set_moduli(nmod)
     st = "crtgemm" if kind == "crt" else "bsplit"
     if args.tune:
         from exciting_amd import autotune
@@ -172,7 +172,7 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
                 "per_launch_ops": ops, "traffic": None}
         # HBM traffic of the residue GEMM from the committed rocprofv3 summary of the default command
         prof = HERE / "profiles" / f"setup_kernels_workload{args.config}.json"
-        if args.nk is None and not args.tune and not args.set and not args.moduli and olpflag and prof.exists():
+        if args.nk is None and not args.tune and not args.set and args.moduli == 11 and olpflag and prof.exists():
             rec = json.loads(prof.read_text()).get("crt_gemm_kernel")
             if rec:
                 roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
@@ -499,7 +499,7 @@ def main():
                     help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision measurements")
     ap.add_argument("--no-bf16", action="store_true", help="skip the three-slice BF16 split measurement")
-    ap.add_argument("--moduli", type=int, default=None, help="moduli of the modular split (8..12, default 11)")
+    ap.add_argument("--moduli", type=int, default=11, help="moduli of the modular split (8..12, default 11)")
     ap.add_argument("--no-hmlint", action="store_true", help="skip the muffin-tin integrals (hmlint) measurement")
     ap.add_argument("--eigen", type=int, default=200, metavar="NEV",
                     help="grouped path: also time setup + device eigensolve of the lowest NEV states "

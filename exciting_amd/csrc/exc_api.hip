@@ -704,7 +704,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
   CrtTab ct;
   if (crt) exc_crt_tables(c->crt_nmod, &ct);
-  const int npa = olp ? 3 : 4;                               // residue planes of A (re + im for O)
+  constexpr int kCrtPlanes = 2;                             // residue planes per operand and modulus
   const int kp = exc_crt_kpad(c->d.natm * c->d.naa);
   // batch signature: descriptors, flags and variants
   int tail[7] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? c->crt_nmod : -1) + 100 * ov, fv};
@@ -753,7 +753,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (crt) {
       for (int k = 0; k < nk; ++k) {
         const size_t cnp = (size_t)(ks[k].ngp + 255) / 256 * 256;
-        qtot += (size_t)(npa + 3) * ct.n * kp * cnp;
+        qtot += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
         xtot += 2 * cnp;
       }
       if ((rc = grow(&c->crt_q, &c->crt_q_bytes, qtot ? qtot : 1))) return rc;
@@ -814,8 +814,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         const int cnp = (q.ngp + 255) / 256 * 256;
         e.cnp = cnp;
         e.qa = (char *)c->crt_q + qoff;
-        e.qy = (char *)c->crt_q + qoff + (size_t)npa * ct.n * kp * cnp;
-        qoff += (size_t)(npa + 3) * ct.n * kp * cnp;
+        e.qy = (char *)c->crt_q + qoff + (size_t)kCrtPlanes * ct.n * kp * cnp;
+        qoff += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
         e.xa = c->crt_x + xoff;
         e.xy = c->crt_x + xoff + cnp;
         xoff += 2 * (size_t)cnp;
@@ -827,7 +827,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
               for (int tj = bj; tj < std::min(bj + S, nt); ++tj)
                 for (int ti = bi; ti < std::min(bi + S, tj + 1); ++ti) citems.push_back({k, ti, tj, which});
         for (int l = 0; l < ct.n; ++l)
-          for (int p = 0; p < 3; ++p)
+          for (int p = 0; p < 2; ++p)
             for (int i = first; i < (int)citems.size(); ++i) cwork.push_back(make_int2(i, 4 * l + p));
       }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
@@ -857,7 +857,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     c->batch_ncwork = (int)cwork.size();
     c->batch_maxcnp = maxcnp;
     if (crt) {
-      const size_t rbytes = (size_t)citems.size() * ct.n * 3 * cbt * cbt;
+      const size_t rbytes = (size_t)citems.size() * ct.n * 2 * cbt * cbt;
       if ((rc = grow((void **)&c->crt_res, &c->crt_res_bytes, rbytes ? rbytes : 1))) return rc;
       if ((rc = grow((void **)&c->d_citems, &c->citems_cap, sizeof(mt_item) * (citems.empty() ? 1 : citems.size()))))
         return rc;
@@ -924,11 +924,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       // APW-APW tiles: no conflict with the side branch
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTPREP);
-        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, npa, c->stream))) return r;
+        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, c->stream))) return r;
       }
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTGEMM);
-        if ((r = exc_launch_crt_gemm(c->d, c->d_ks, c->d_citems, c->d_cwork, c->batch_ncwork, ct, npa, c->crt_res, cv,
+        if ((r = exc_launch_crt_gemm(c->d, c->d_ks, c->d_citems, c->d_cwork, c->batch_ncwork, ct, c->crt_res, cv,
                                      c->stream)))
           return r;
       }

@@ -144,27 +144,23 @@ int exc_launch_olp(const exc_dev &d, int nk, int maxngp, const kdesc *ks, const 
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
                       hipStream_t s);
 // modular split of the muffin-tin contraction (k_crt.hip)
-constexpr int kMaxMod = 14;          // moduli available: 256, 255, 253, 251, 247, 241, 239, 233, ...
+constexpr int kMaxMod = 14;          // moduli available: 241, 233, 229, 221, 205, 197, 193, ...
 constexpr int kCrtLoDigits = 6;      // mixed-radix digits of the low part (their product < 2^53)
-constexpr int kCrtMaxUsed = 12;      // beyond 12 moduli beta > 46: row integers (and re +- im) no
-                                     // longer exact in FP64, the high part of the reconstruction
-                                     // no longer below 2^53
+constexpr int kCrtMaxUsed = 12;      // beyond 12 moduli beta > 44: re -+ iota im of the row
+                                     // integers no longer exact in FP64
 constexpr int kCrtVariants = 8;      // tile / staging variants of the residue GEMM
 struct CrtTab {                      // passed by value to the kernels (kernel-argument segment)
-  int n, beta, nlo, pad_;
-  int m[kMaxMod];
-  float finv[kMaxMod];
+  int n, beta;
+  int m[kMaxMod];                    // odd, pairwise coprime, every prime factor 1 mod 4
+  int iota[kMaxMod];                 // a square root of -1 modulo m
   double dminv[kMaxMod];
-  int w[kMaxMod][kMaxMod];           // M_j mod m_l, M_j = m_0 ... m_(j-1)
-  int v[kMaxMod];                    // M_l^-1 mod m_l
-  double mk;                         // M_nlo
 };
 void exc_crt_tables(int n, CrtTab *t);
 inline int exc_crt_kpad(int ktot) { return (ktot + 63) / 64 * 64; }
 int exc_crt_shape(int variant, int *bt, int *wr, int *wc);   // tile edge and wave grid of a GEMM variant
-int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, int npa, hipStream_t s);
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, hipStream_t s);
 int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
-                        const CrtTab &t, int npa, signed char *res, int variant, hipStream_t s);
+                        const CrtTab &t, signed char *res, int variant, hipStream_t s);
 int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
                          const signed char *res, int variant, hipStream_t s);
 // generalized eigensolve helpers (k_eigen.hip)

@@ -5,16 +5,21 @@
 //
 //  scale : per row i of an operand X (N x K complex, K = natm naa) the exponent e_i with
 //          |X_i|_2 < 2^e_i and s_i = beta - e_i, beta the largest with 2^(2 beta + 2) <= P,
-//          P = m_0 ... m_(n-1) the product of the pairwise coprime moduli (256, 255, 253, ...).
-//  split : X'_ik = rint(Re X_ik 2^s_i) + i rint(Im X_ik 2^s_i), exact integers below 2^beta,
-//          and per modulus m the symmetric residues of Re X', Im X' and of Re X' -+ Im X'
-//          (Gauss's third product) as int8 planes, layout [m][plane][K/16][row][16 bytes]:
-//          one 16-byte k-chunk of 64 consecutive rows is one 1 KiB wave-wide DMA.
-//  gemm  : per modulus, tile and product, P1 = ar br^T, P2 = ai bi^T or P3 = as bs^T by
-//          v_mfma_i32_32x32x32_i8 with exact int32 accumulation (|P| <= K 128^2 < 2^31),
-//          reduced modulo m to int8 residues.
-//  merge : per tile element Re = P1 + P2, Im = P3 - P1 + P2 modulo each m, then
-//          Garner's mixed-radix digits of Re', Im' in [-P/2, P/2) (exact:
+//          P = m_0 ... m_(n-1) the product of the pairwise coprime moduli (241, 233, 229, ...).
+//  split : X'_ik = rint(Re X_ik 2^s_i) + i rint(Im X_ik 2^s_i), exact integers below 2^beta.
+//          Every modulus m is odd with prime factors 1 mod 4, so it has a square root iota of
+//          -1 and Z_m[i] splits into two copies of Z_m by the ring maps x + iy -> x +- iota y.
+//          Per modulus the two planes of an operand are those maps, as symmetric int8
+//          residues: Re X' - iota Im X' and Re X' + iota Im X' for the conjugated left operand,
+//          Re X' + iota Im X' and Re X' - iota Im X' for the right one; layout
+//          [m][plane][K/16][row][16 bytes]: one 16-byte k-chunk of 64 consecutive rows is one
+//          1 KiB wave-wide DMA.
+//  gemm  : per modulus, tile and map, U = u_a u_b^T or V = v_a v_b^T by
+//          v_mfma_i32_32x32x32_i8 with exact int32 accumulation (|P| <= K 120^2 < 2^31),
+//          reduced modulo m to int8 residues: two real products per modulus, where Gauss's
+//          form needs three.
+//  merge : per tile element Re = (U + V)/2, Im = (U - V)/(2 iota) modulo each m, then
+//          Garner's mixed-radix digits of Re', Im' in [-(P-1)/2, (P-1)/2] (exact:
 //          |Re'|, |Im'| <= |A'_i| |B'_j| < P/2), the FP64 value correctly rounded as
 //          fma(x_hi, M_6, x_lo), times 2^-(s_i + t_j); then the interstitial term and the
 //          Hermitian mirror exactly as the FP64 epilogue (hmlistln.f90:74-84,
@@ -38,18 +43,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int kMaxTile = 256;    // largest tile edge of the gemm and merge kernels; rows padded to it
 
-// symmetric residue of an exact integer held in a double, |x| < 2^52
-__device__ __forceinline__ int dres(double x, int m, double minv) {
-  if (m == 256) return (int)(x - 256.0 * floor((x + 128.0) * (1.0 / 256.0)));
-  return (int)(x - (double)m * rint(x * minv));
-}
-
-// symmetric residue of a small integer, |x| < 2^22 (float quotient exact to well within
-// the 1/(2m) margin of the rounding)
-__device__ __forceinline__ int fres(int x, int m, float finv) {
-  if (m == 256) return (int)(signed char)(x & 255);
-  return x - m * (int)rintf((float)x * finv);
-}
+// symmetric residue of an exact integer held in a double, |x| < 2^52, modulo an odd m
+// (|x/m| is never within 1/(2m) of a half-integer, far above the rounding of x/m)
+__device__ __forceinline__ int dres(double x, int m, double minv) { return (int)fma(-(double)m, rint(x * minv), x); }
 
 // ---------------------------------------------------------------------------------------
 // scale: one workgroup per (k, operand, 64 rows); wave w sums |x|^2 over its quarter of K,
@@ -93,69 +89,39 @@ __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
          __builtin_amdgcn_perm((unsigned)d, (unsigned)c, 0x04000c0cu);
 }
 
-// symmetric residue of s, |s| <= 2h = m - 1
-__device__ __forceinline__ int fold(int s, int m, int h) { return s > h ? s - m : (s < -h ? s + m : s); }
-
-// Residue planes of one modulus for 16 values of k of one row.  OP 0: A (plane 2 = re - im,
-// the conjugated left operand; NP = 4 adds plane 3 = re + im for O's right operand), OP 1: Yt
-// (plane 2 = re + im).  m = 256 (l = 0) takes the low bytes of x mod 256 (the two's-complement
-// bytes of the symmetric residues); an odd m the exact x - m rint(x/m) (|x/m| is never within
-// 1/(2m) of a half-integer, far above the rounding of x/m).
-template <int OP, int NP>
+// The two residue planes of one modulus for 16 values of k of one row: the maps
+// Re X' + sg iota Im X' and Re X' - sg iota Im X' (sg = -1 for the conjugated left operand A,
+// +1 for the right operand Yt), formed exactly in FP64 from Re X' and the residue of Im X'
+// (|Re X'| + iota (m-1)/2 < 2^45: the quotient by m is then far more accurate than the
+// 1/(2m) margin of its rounding) and reduced to their symmetric residues.
+template <int SG>
 __device__ __forceinline__ void split_planes(const double (&xr)[16], const double (&xi)[16], const CrtTab &t,
                                              signed char *base, size_t pstride, size_t off) {
-  {
-    unsigned w[NP][4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      int br[4], bi[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const double a = xr[4 * g + b], c = xi[4 * g + b];
-        br[b] = (int)fma(-256.0, floor(a * (1.0 / 256.0)), a);
-        bi[b] = (int)fma(-256.0, floor(c * (1.0 / 256.0)), c);
-      }
-      w[0][g] = pack4(br[0], br[1], br[2], br[3]);
-      w[1][g] = pack4(bi[0], bi[1], bi[2], bi[3]);
-      if (OP == 0)
-        w[2][g] = pack4(br[0] - bi[0], br[1] - bi[1], br[2] - bi[2], br[3] - bi[3]);
-      else
-        w[2][g] = pack4(br[0] + bi[0], br[1] + bi[1], br[2] + bi[2], br[3] + bi[3]);
-      if (NP == 4) w[NP - 1][g] = pack4(br[0] + bi[0], br[1] + bi[1], br[2] + bi[2], br[3] + bi[3]);
-    }
-#pragma unroll
-    for (int p = 0; p < NP; ++p)
-      *(uint4 *)(base + (size_t)p * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
-  }
 #pragma unroll 1
-  for (int l = 1; l < t.n; ++l) {
-    const int m = t.m[l], h = (m - 1) >> 1;
-    const double dm = (double)m, mi = t.dminv[l];
-    unsigned w[NP][4];
+  for (int l = 0; l < t.n; ++l) {
+    const int m = t.m[l];
+    const double mi = t.dminv[l], io = (double)(SG * t.iota[l]);
+    unsigned w[2][4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      int rr[4], ri[4], r2[4], r3[4];
+      int ru[4], rv[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const double a = xr[4 * g + b], c = xi[4 * g + b];
-        rr[b] = (int)fma(-dm, rint(a * mi), a);
-        ri[b] = (int)fma(-dm, rint(c * mi), c);
-        r2[b] = fold(OP == 0 ? rr[b] - ri[b] : rr[b] + ri[b], m, h);
-        if (NP == 4) r3[b] = fold(rr[b] + ri[b], m, h);
+        const double ci = fma(-(double)m, rint(c * mi), c);     // Im X' mod m, exact
+        ru[b] = dres(fma(io, ci, a), m, mi);
+        rv[b] = dres(fma(-io, ci, a), m, mi);
       }
-      w[0][g] = pack4(rr[0], rr[1], rr[2], rr[3]);
-      w[1][g] = pack4(ri[0], ri[1], ri[2], ri[3]);
-      w[2][g] = pack4(r2[0], r2[1], r2[2], r2[3]);
-      if (NP == 4) w[NP - 1][g] = pack4(r3[0], r3[1], r3[2], r3[3]);
+      w[0][g] = pack4(ru[0], ru[1], ru[2], ru[3]);
+      w[1][g] = pack4(rv[0], rv[1], rv[2], rv[3]);
     }
 #pragma unroll
-    for (int p = 0; p < NP; ++p)
-      *(uint4 *)(base + (size_t)(l * NP + p) * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+    for (int p = 0; p < 2; ++p)
+      *(uint4 *)(base + (size_t)(2 * l + p) * pstride + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
   }
 }
 
-__global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t,
-                                                        int npa) {
+__global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t) {
   const kdesc k = ks[blockIdx.z >> 1];
   const int op = blockIdx.z & 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -178,11 +144,9 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict_
   const size_t pstride = (size_t)kp * k.cnp;            // bytes of one plane
   const size_t off = ((size_t)kc * k.cnp + row) * 16;
   if (op)
-    split_planes<1, 3>(xr, xi, t, base, pstride, off);
-  else if (npa == 3)
-    split_planes<0, 3>(xr, xi, t, base, pstride, off);
+    split_planes<1>(xr, xi, t, base, pstride, off);
   else
-    split_planes<0, 4>(xr, xi, t, base, pstride, off);
+    split_planes<-1>(xr, xi, t, base, pstride, off);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -192,8 +156,8 @@ __global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict_
 // the workgroups an XCD runs together share their row and column panels in its L2).  BT x BT
 // tiles, WR x WC waves with (BT / WR) x (BT / WC) wave tiles of 32 x 32 MFMA blocks, LDS
 // stages of BK bytes of k of one plane of both panels, filled by DMA, NST stages in flight.
-//   p = 0: P1 = ar br^T, p = 1: P2 = ai bi^T, p = 2: P3 = as bs^T (as = ar - ai of conj(A),
-//   bs = br + bi), each reduced modulo m to int8 residues; the merge combines them.
+//   p = 0: U = u_a u_b^T, p = 1: V = v_a v_b^T (the two maps of Z_m[i]), each reduced modulo
+//   m to int8 residues; the merge combines them.
 // ---------------------------------------------------------------------------------------
 template <int BT, int BK, int NST, int WR, int WC>
 struct GemmCfg {
@@ -257,7 +221,7 @@ template <int BT, int BK, int NST, int WR, int WC, int MINW>
 __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdesc *__restrict__ ks,
                                                                       const mt_item *__restrict__ items,
                                                                       const int2 *__restrict__ work, int nwork, int kp,
-                                                                      CrtTab t, int npa, signed char *__restrict__ res) {
+                                                                      CrtTab t, signed char *__restrict__ res) {
   using C = GemmCfg<BT, BK, NST, WR, WC>;
   using St = typename C::Stage;
   EXC_DMA_STAGES(St, NST)
@@ -270,10 +234,11 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
   const kdesc k = ks[it.k];
   const int l = wk.y >> 2, p = wk.y & 3;
   const size_t pstride = (size_t)kp * k.cnp, chunk_stride = (size_t)k.cnp * 16;
-  // left operand conj(A): planes re, im, re - im; right operand Yt (H) or A (O): re, im, re + im
-  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(l * npa + p) * pstride;
-  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(l * npa + (p == 2 ? 3 : p)) * pstride
-                                     : (const unsigned char *)k.qy + (size_t)(l * 3 + p) * pstride;
+  // left operand conj(A): its planes u, v; right operand Yt (H): its planes u, v; or A (O),
+  // whose maps as a right operand are the left planes swapped
+  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(2 * l + p) * pstride;
+  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(2 * l + 1 - p) * pstride
+                                     : (const unsigned char *)k.qy + (size_t)(2 * l + p) * pstride;
   const int row0 = it.ti * BT, col0 = it.tj * BT;
   const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
   v16i acc[C::MI][C::NJ];
@@ -309,7 +274,7 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
   const int m = t.m[l];
   const double mi = t.dminv[l];
   const int lane = threadIdx.x & 63;
-  signed char *out = res + (((size_t)wk.x * t.n + l) * 3 + p) * (size_t)(BT * BT);
+  signed char *out = res + (((size_t)wk.x * t.n + l) * 2 + p) * (size_t)(BT * BT);
 #pragma unroll
   for (int i = 0; i < C::MI; ++i)
 #pragma unroll
@@ -329,11 +294,12 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
 // ---------------------------------------------------------------------------------------
 // The moduli and Garner's tables are fixed at compile time for each modulus count (the
 // moduli list is constant; only how many of them is chosen), so every reduction below
-// multiplies by immediates: w[l][j] = M_j mod m_l, v[l] = M_l^-1 mod m_l, M_j = m_0 ... m_(j-1).
-constexpr int kModList[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
+// multiplies by immediates.  Moduli: odd, pairwise coprime, every prime factor 1 mod 4
+// (221 = 13 17, 205 = 5 41), the largest first.
+constexpr int kModList[kMaxMod] = {241, 233, 229, 221, 205, 197, 193, 181, 173, 157, 149, 137, 113, 109};
 
 constexpr int inv_mod(int a, int m) {   // a^-1 mod m, gcd(a, m) = 1 (extended Euclid)
-  int r0 = m, r1 = a % m, t0 = 0, t1 = 1;
+  int r0 = m, r1 = ((a % m) + m) % m, t0 = 0, t1 = 1;
   while (r1) {
     const int q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
     r0 = r1; r1 = r2; t0 = t1; t1 = t2;
@@ -341,64 +307,67 @@ constexpr int inv_mod(int a, int m) {   // a^-1 mod m, gcd(a, m) = 1 (extended E
   return t0 < 0 ? t0 + m : t0;
 }
 
+constexpr int iota_of(int m) {          // the smallest square root of -1 modulo m
+  for (int x = 1; x < m; ++x)
+    if ((x * x + 1) % m == 0) return x;
+  return 0;
+}
+
+// Garner's mixed-radix digits d_l = (r_l - sum_j d_j M_j) M_l^-1 mod m_l (M_j = m_0 ... m_(j-1))
+// of Re' and Im', from U and V: r_l = (U + V) 2^-1 for Re, (U - V) (2 iota_l)^-1 for Im, so
+// the digit is s_l c_l - sum_j d_j w[l][j] reduced once, with s_l = U +- V unreduced,
+// c_l = (2^-1 or (2 iota_l)^-1) M_l^-1 mod m_l and w[l][j] = (M_j mod m_l) M_l^-1 mod m_l.
 template <int NM>
 struct GarnerTab {
-  int w[NM][NM];    // w[l][j] = (M_j mod m_l) v[l] mod m_l
-  int v[NM];
+  int w[NM][NM];
+  int cre[NM], cim[NM];
   double mk;        // M_6 (or P for fewer moduli): x = x_hi M_6 + x_lo
-  constexpr GarnerTab() : w(), v(), mk(1.0) {
-    for (int l = 1; l < NM; ++l) {
+  constexpr GarnerTab() : w(), cre(), cim(), mk(1.0) {
+    for (int l = 0; l < NM; ++l) {
+      const int m = kModList[l];
       long long ml = 1;
       int mj[NM] = {};
       for (int j = 0; j < l; ++j) {
-        mj[j] = (int)(ml % kModList[l]);
-        ml = ml * kModList[j] % kModList[l];
+        mj[j] = (int)(ml % m);
+        ml = ml * kModList[j] % m;
       }
-      v[l] = inv_mod((int)ml, kModList[l]);
-      for (int j = 0; j < l; ++j) w[l][j] = mj[j] * v[l] % kModList[l];
+      const int v = inv_mod((int)ml, m);
+      for (int j = 0; j < l; ++j) w[l][j] = mj[j] * v % m;
+      cre[l] = inv_mod(2, m) * v % m;
+      cim[l] = inv_mod(2 * iota_of(m) % m, m) * v % m;
     }
     for (int j = 0; j < (NM < kCrtLoDigits ? NM : kCrtLoDigits); ++j) mk *= kModList[j];
   }
 };
 
-// symmetric residue modulo m = kModList[L] of |x| < 2^22 (L = 0: the low byte); the float
-// quotient is within 2^-21 |x| / m of x / m, far inside the 1/(2m) margin of the rounding
+// symmetric residue modulo m = kModList[L] of |x| < 2^22; the float quotient is within
+// 2^-21 |x| / m of x / m, far inside the 1/(2m) margin of the rounding
 template <int L>
 __device__ __forceinline__ int cres(int x) {
   constexpr int m = kModList[L];
-  if constexpr (L == 0) {
-    return (int)(signed char)(x & 255);
-  } else {
-    constexpr float finv = 1.0f / (float)m;
-    return x - __mul24((int)rintf((float)x * finv), m);
-  }
+  constexpr float finv = 1.0f / (float)m;
+  return x - __mul24((int)rintf((float)x * finv), m);
 }
 
-// Garner's digit d_l = (r_l - sum_j d_j M_j) M_l^-1 mod m_l, formed as
-// r_l v_l - sum_j d_j (M_j v_l mod m_l) and reduced once: r need not be reduced itself
-// (|r| <= 384, |d_j| <= 128, so the sum stays below 2^22 for up to 12 moduli)
-template <int NM, int L>
-__device__ __forceinline__ void garner_digit(int (&d)[NM], const int (&r)[NM]) {
+// |s_l c_l| <= 240 240 and |d_j w| <= 120 240: the sum stays below 2^22 for up to 12 moduli
+template <int NM, int L, bool IM>
+__device__ __forceinline__ void garner_digit(int (&d)[NM], const int (&s)[NM]) {
   constexpr GarnerTab<NM> G{};
-  if constexpr (L == 0) {
-    d[0] = cres<0>(r[0]);
-  } else {
-    int acc = __mul24(r[L], G.v[L]);
+  int acc = __mul24(s[L], IM ? G.cim[L] : G.cre[L]);
 #pragma unroll
-    for (int j = 0; j < L; ++j) acc -= __mul24(d[j], G.w[L][j]);
-    d[L] = cres<L>(acc);
-  }
-  if constexpr (L + 1 < NM) garner_digit<NM, L + 1>(d, r);
+  for (int j = 0; j < L; ++j) acc -= __mul24(d[j], G.w[L][j]);
+  d[L] = cres<L>(acc);
+  if constexpr (L + 1 < NM) garner_digit<NM, L + 1, IM>(d, s);
 }
 
-// x in [-P/2, P/2) from values r_l congruent to it modulo each m_l: Garner's mixed-radix
-// digits, then x as the correctly rounded fma(x_hi, M_6, x_lo), the two parts exact in FP64
-// (|x_lo| < M_6 / 2 < 2^52)
-template <int NM>
-__device__ __forceinline__ double garner(const int (&r)[NM]) {
+// Re' (IM false) or Im' in [-(P-1)/2, (P-1)/2] from s_l = U + V (or U - V) modulo each m_l:
+// Garner's digits, then the correctly rounded fma(x_hi, M_6, x_lo), the two parts exact in
+// FP64 (|x_lo| < M_6 / 2 < 2^52)
+template <int NM, bool IM>
+__device__ __forceinline__ double garner(const int (&s)[NM]) {
   constexpr GarnerTab<NM> G{};
   int d[NM];
-  garner_digit<NM, 0>(d, r);
+  garner_digit<NM, 0, IM>(d, s);
   constexpr int NLO = NM < kCrtLoDigits ? NM : kCrtLoDigits;
   double lo = (double)d[NLO - 1];
 #pragma unroll
@@ -408,7 +377,6 @@ __device__ __forceinline__ double garner(const int (&r)[NM]) {
   for (int j = NM - 1; j >= NLO; --j) hi = fma(hi, (double)kModList[j], (double)d[j]);
   return fma(hi, G.mk, lo);
 }
-
 
 template <int NM>
 __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
@@ -443,7 +411,7 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
   const int mi_n = bt / wr_n / 32, nj_n = bt / wc_n / 32, per_wave = mi_n * nj_n;
   const int nblk = (bt / 32) * (bt / 32);
   const size_t ptile = (size_t)bt * bt;                     // residue bytes of one (modulus, product)
-  const signed char *base = res + (size_t)blockIdx.x * NM * 3 * ptile;
+  const signed char *base = res + (size_t)blockIdx.x * NM * 2 * ptile;
   cplx *out = it.which ? k.o : k.h;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Element (row, col) of a 32 x 32 block sits in lane (col mod 32) of the gemm layout, so the
@@ -459,13 +427,12 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
     const int rb = wr * (32 * mi_n) + 32 * bi, cb = wc * (32 * nj_n) + 32 * bj;   // block origin in the tile
     const int cl = cb + (lane & 31);
     const int col = col0 + cl;
-    uint4 q1[NM], q2[NM], q3[NM];
+    uint4 q1[NM], q2[NM];
 #pragma unroll
     for (int l = 0; l < NM; ++l) {
-      const signed char *b = base + (size_t)l * 3 * ptile + ((size_t)blk * 64 + lane) * 16;
+      const signed char *b = base + (size_t)l * 2 * ptile + ((size_t)blk * 64 + lane) * 16;
       q1[l] = *(const uint4 *)b;
       q2[l] = *(const uint4 *)(b + ptile);
-      q3[l] = *(const uint4 *)(b + 2 * ptile);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -477,26 +444,20 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
         const int row = row0 + rl;
         cplx val = {0.0, 0.0};
         if (col < ngp && row <= col) {
-          // conj(a) b: Re = P1 + P2, Im = P3 - P1 + P2 (modulo each m)
+          // conj(a) b: Re = (U + V)/2, Im = (U - V)/(2 iota) modulo each m
           const int g = v >> 2, sh = 8 * (v & 3);
-          unsigned w1[NM], w2[NM], w3[NM];
+          int sr[NM], si[NM];
 #pragma unroll
           for (int l = 0; l < NM; ++l) {
-            w1[l] = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
-            w2[l] = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
-            w3[l] = g == 0 ? q3[l].x : g == 1 ? q3[l].y : g == 2 ? q3[l].z : q3[l].w;
-          }
-          int dr[NM], di[NM];
-#pragma unroll
-          for (int l = 0; l < NM; ++l) {
-            const int p1 = (signed char)(w1[l] >> sh), p2 = (signed char)(w2[l] >> sh);
-            const int p3 = (signed char)(w3[l] >> sh);
-            dr[l] = p1 + p2;                 // congruent to Re modulo m_l
-            di[l] = p3 - p1 + p2;            // congruent to Im
+            const unsigned w1 = g == 0 ? q1[l].x : g == 1 ? q1[l].y : g == 2 ? q1[l].z : q1[l].w;
+            const unsigned w2 = g == 0 ? q2[l].x : g == 1 ? q2[l].y : g == 2 ? q2[l].z : q2[l].w;
+            const int u = (signed char)(w1 >> sh), vv = (signed char)(w2 >> sh);
+            sr[l] = u + vv;
+            si[l] = u - vv;
           }
           const int sc = -(er[rl] + ec[cl]);
-          double re = ldexp(garner<NM>(dr), sc);
-          double im = ldexp(garner<NM>(di), sc);
+          double re = ldexp(garner<NM, false>(sr), sc);
+          double im = ldexp(garner<NM, true>(si), sc);
           const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
           if (ig) {
             const cplx th = gt.cfunig[ig - 1];
@@ -538,12 +499,12 @@ int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int 
 }
 
 template <int BT, int BK, int NST, int WR, int WC, int MINW>
-int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t, int npa,
+int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t,
                   signed char *res, hipStream_t s) {
   if (kp % BK) return EXC_ERR_UNSUPPORTED;      // the k loop runs whole stages of BK bytes
   const int chunk = (nwork + 7) / 8;
   hipLaunchKernelGGL((crt_gemm_kernel<BT, BK, NST, WR, WC, MINW>), dim3(8 * chunk), dim3(64 * WR * WC), 0, s, ks,
-                     items, work, nwork, kp, t, npa, res);
+                     items, work, nwork, kp, t, res);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -551,45 +512,25 @@ int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int n
 }  // namespace
 
 void exc_crt_tables(int n, CrtTab *t) {
-  static const int kMods[kMaxMod] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
   *t = CrtTab{};
   t->n = n;
+  double lp = 0.0;
   for (int l = 0; l < n; ++l) {
-    t->m[l] = kMods[l];
-    t->dminv[l] = 1.0 / kMods[l];
-    t->finv[l] = 1.0f / (float)kMods[l];
+    t->m[l] = kModList[l];
+    t->iota[l] = iota_of(kModList[l]);
+    t->dminv[l] = 1.0 / kModList[l];
+    lp += log2((double)kModList[l]);
   }
   // beta: the largest b with 2^(2b + 2) <= P, from log2 P with a margin against its rounding
-  double lp = 0.0;
-  for (int l = 0; l < n; ++l) lp += log2((double)kMods[l]);
   t->beta = (int)floor((lp - 2.0 - 1e-9) / 2.0);
-  // Garner tables: w[l][j] = M_j mod m_l, v[l] = M_l^-1 mod m_l (M_j = m_0 ... m_(j-1))
-  for (int l = 1; l < n; ++l) {
-    long long Ml = 1;
-    for (int j = 0; j < l; ++j) {
-      t->w[l][j] = (int)(Ml % kMods[l]);
-      Ml = (Ml * kMods[j]) % kMods[l];
-    }
-    int inv = 0;
-    for (int x = 1; x < kMods[l]; ++x)
-      if ((Ml * x) % kMods[l] == 1) {
-        inv = x;
-        break;
-      }
-    t->v[l] = inv;
-  }
-  t->nlo = n < kCrtLoDigits ? n : kCrtLoDigits;
-  double mk = 1.0;
-  for (int j = 0; j < t->nlo; ++j) mk *= kMods[j];   // < 2^53: exact
-  t->mk = mk;
 }
 
-int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, int npa, hipStream_t s) {
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, hipStream_t s) {
   if (nk <= 0 || maxnp <= 0) return EXC_OK;
   const int ktot = d.natm * d.naa, kp = exc_crt_kpad(ktot);
   hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, 2 * nk), dim3(256), 0, s, ks, ktot, t.beta);
   EXC_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t, npa);
+  hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -610,18 +551,18 @@ int exc_crt_shape(int variant, int *bt, int *wr, int *wc) {
 }
 
 int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
-                        const CrtTab &t, int npa, signed char *res, int variant, hipStream_t s) {
+                        const CrtTab &t, signed char *res, int variant, hipStream_t s) {
   if (nwork <= 0) return EXC_OK;
   const int kp = exc_crt_kpad(d.natm * d.naa);
   switch (variant) {
-    case 0: return launch_gemm_t<256, 64, 3, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 1: return launch_gemm_t<256, 32, 4, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 2: return launch_gemm_t<128, 64, 3, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 3: return launch_gemm_t<256, 64, 2, 2, 4, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 4: return launch_gemm_t<128, 32, 4, 2, 2, 2>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 5: return launch_gemm_t<256, 64, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 6: return launch_gemm_t<256, 32, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
-    case 7: return launch_gemm_t<256, 64, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, npa, res, s);
+    case 0: return launch_gemm_t<256, 64, 3, 2, 4, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 1: return launch_gemm_t<256, 32, 4, 2, 4, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 2: return launch_gemm_t<128, 64, 3, 2, 2, 2>(ks, items, work, nwork, kp, t, res, s);
+    case 3: return launch_gemm_t<256, 64, 2, 2, 4, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 4: return launch_gemm_t<128, 32, 4, 2, 2, 2>(ks, items, work, nwork, kp, t, res, s);
+    case 5: return launch_gemm_t<256, 64, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 6: return launch_gemm_t<256, 32, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 7: return launch_gemm_t<256, 64, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
     default: return EXC_ERR_ARG;
   }
 }

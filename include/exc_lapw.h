@@ -199,11 +199,12 @@ typedef struct exc_kpoint_dev {
                                /* 6 slice pairs); Y stage, interstitial, lo blocks stay FP64      */
 #define EXC_BATCH_SPLIT_CRT 16  /* muffin-tin contraction from exact integer products: each row of  */
                                 /* A and Yt scaled by a power of two to integers below 2^beta (row */
-                                /* 2-norm), their residues modulo n pairwise coprime moduli <= 256  */
-                                /* multiplied by int8 MFMA with int32 accumulation (Gauss's three  */
-                                /* products per modulus), the results reconstructed by the Chinese */
-                                /* remainder theorem and rounded once to FP64; n from             */
-                                /* exc_ctx_set_moduli.  Y stage, interstitial, lo blocks stay FP64 */
+                                /* 2-norm), their residues modulo n pairwise coprime odd moduli    */
+                                /* < 256 in which -1 is a square, so each complex product splits   */
+                                /* into two real int8 MFMA products with int32 accumulation, the   */
+                                /* results reconstructed by the Chinese remainder theorem and      */
+                                /* rounded once to FP64; n from exc_ctx_set_moduli.  Y stage,      */
+                                /* interstitial, lo blocks stay FP64                               */
 #define EXC_BATCH_OLP_LEGENDRE 8 /* needs EXC_BATCH_MATCH: the muffin-tin part of O from the      */
                                  /* radial factors of match by the spherical-harmonic addition   */
                                  /* theorem, sum_a conj(S_a) S_a' sum_l P_l(u.u') sum_io r r',   */
@@ -396,8 +397,8 @@ int exc_ctx_status(exc_ctx *ctx);
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
 /* Number of moduli of EXC_BATCH_SPLIT_CRT (8 .. 12, default 11).  With n moduli the row
- * integers have beta bits (the largest with 2^(2 beta + 2) <= their product): 34 bits at n = 9,
- * 38 at 10, 42 at 11, 46 at 12; the residue products cost n int8 GEMMs per real product.  No
+ * integers have beta bits (the largest with 2^(2 beta + 2) <= their product): 29 bits at n = 8,
+ * 33 at 9, 37 at 10, 40 at 11, 44 at 12; the residue products cost 2 n int8 GEMMs.  No
  * reference counterpart (the reference computes in FP64). */
 int exc_ctx_set_moduli(exc_ctx *ctx, int n);
 /* The first n moduli and beta of the modular split (for a caller's error estimate). */

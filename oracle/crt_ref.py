@@ -9,14 +9,15 @@ restates that arithmetic step by step so that a GPU result can be compared with 
 * ``row_scale`` — per row i of an operand X (N x K complex) the exponent e_i with
   |X_i|_2 < 2^e_i (frexp), and the scale exponent s_i = beta - e_i;
 * ``integers`` — X' = rint(Re X 2^s_i) + i rint(Im X 2^s_i), exact integers below 2^beta;
-* ``residues`` — symmetric residues of integers modulo m: [-128, 127] for m = 256,
-  [-(m-1)/2, (m-1)/2] for odd m;
-* ``product`` — conj(A) B^T from the integers: for each modulus the three real residue
-  products of Gauss's form, P1 = ar br^T, P2 = ai bi^T, P3 = (ar - ai)(br + bi)^T (exact:
-  residues <= 128 in magnitude, sums below 2^53, so float64 GEMMs are exact), combined to
-  Re = P1 + P2, Im = P3 - P1 + P2 and reduced modulo m; the integers Re', Im' in [-P/2, P/2)
-  reconstructed from their residues by Garner's mixed-radix algorithm; the FP64 value the
-  correctly rounded Re' (Im'), times 2^-(s_i + t_j).
+* ``residues`` — symmetric residues of integers modulo an odd m, in [-(m-1)/2, (m-1)/2];
+* ``product`` — conj(A) B^T from the integers.  Every modulus m has a square root iota of -1
+  (its prime factors are all 1 mod 4), so Z_m[i] splits into two copies of Z_m through
+  x + iy -> x + iota y and x - iota y, both ring maps.  Per modulus the planes
+  u_a = ar - iota ai, v_a = ar + iota ai (the maps of conj(a)) and u_b = br + iota bi,
+  v_b = br - iota bi give two real residue products U = u_a u_b^T, V = v_a v_b^T, the maps
+  of conj(A) B^T, and Re = (U + V)/2, Im = (U - V)/(2 iota) modulo m; the integers Re', Im' in
+  [-P/2, P/2) reconstructed from their residues by Garner's mixed-radix algorithm; the FP64
+  value the correctly rounded Re' (Im'), times 2^-(s_i + t_j).
 
 With 2^(2 beta + 2) <= P = prod m and every row's 2-norm below 2^beta, |Re'|, |Im'| <= |A'_i|
 |B'_j| < P / 2, so the reconstruction is exact.  ``big_product`` forms Re', Im' with Python
@@ -30,9 +31,18 @@ import math
 
 import numpy as np
 
-# pairwise coprime moduli, 256 first (its residues are two's-complement low bytes)
-MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199)
+# pairwise coprime odd moduli whose prime factors are all 1 mod 4 (so -1 is a square modulo
+# each), the largest first: 221 = 13 17, 205 = 5 41
+MODULI = (241, 233, 229, 221, 205, 197, 193, 181, 173, 157, 149, 137, 113, 109)
 LO_DIGITS = 6          # digits of the low part of the reconstruction: prod(MODULI[:6]) < 2^53
+
+
+def iota(m):
+    """The smallest square root of -1 modulo m."""
+    for x in range(1, m):
+        if (x * x + 1) % m == 0:
+            return x
+    raise ValueError(f"-1 is not a square modulo {m}")
 
 
 def moduli(n):
@@ -64,9 +74,7 @@ def integers(X, s):
 
 
 def residues(x, m):
-    """Symmetric residues of the integer-valued float64 array x modulo m."""
-    if m == 256:
-        return x - 256.0 * np.floor((x + 128.0) / 256.0)
+    """Symmetric residues of the integer-valued float64 array x modulo the odd m."""
     return x - m * np.rint(x / m)
 
 
@@ -85,7 +93,7 @@ def _garner_tables(mods):
 
 
 def _sym(x, m):
-    """Symmetric residue of int64 x: [-128, 127] for m = 256, [-(m-1)/2, (m-1)/2] for odd m."""
+    """Symmetric residue of int64 x modulo the odd m: [-(m-1)/2, (m-1)/2]."""
     r = np.mod(x, m)
     return np.where(r > (m - 1) // 2, r - m, r)
 
@@ -118,20 +126,32 @@ def to_float(hi, lo, Mk):
     return ex.astype(np.float64)
 
 
+def planes(xr, xi, m, left):
+    """The two residue planes of an operand modulo m: for the left operand conj(a) the maps
+    ar - iota ai and ar + iota ai, for the right operand b the maps br + iota bi and br - iota bi."""
+    t = iota(m)
+    s = -1.0 if left else 1.0
+    ri = residues(xi, m)                       # keeps xr +- t ri small: exact in float64
+    return residues(xr + s * t * ri, m), residues(xr - s * t * ri, m)
+
+
 def _int_products(ar, ai, br, bi, mods, conj):
-    """Residues modulo each m of Re and Im of op(a) b^T, op = conj or identity (Gauss's form)."""
+    """Residues modulo each m of Re and Im of op(a) b^T, op = conj or identity: two real
+    products per modulus through the split of Z_m[i]."""
     out_re, out_im = [], []
     for m in mods:
-        rar, rai, rbr, rbi = (residues(x, m) for x in (ar, ai, br, bi))
-        ras = residues(ar - ai if conj else ar + ai, m)
-        rbs = residues(br + bi, m)
-        p1 = rar @ rbr.T
-        p2 = rai @ rbi.T
-        p3 = ras @ rbs.T
-        re = p1 + p2 if conj else p1 - p2
-        im = p3 - p1 + p2 if conj else p3 - p1 - p2
-        out_re.append(_sym(re.astype(np.int64), m))
-        out_im.append(_sym(im.astype(np.int64), m))
+        t = iota(m)
+        if conj:
+            ua, va = planes(ar, ai, m, True)
+        else:                                   # the maps of a itself: ar + iota ai, ar - iota ai
+            ua, va = planes(ar, ai, m, False)
+        ub, vb = planes(br, bi, m, False)
+        U = (ua @ ub.T).astype(np.int64)        # exact: |residue| <= 120, sums below 2^53
+        V = (va @ vb.T).astype(np.int64)
+        inv2 = pow(2, -1, m)
+        inv2i = pow(2 * t, -1, m)
+        out_re.append(_sym(_sym(U + V, m) * inv2, m))
+        out_im.append(_sym(_sym(U - V, m) * inv2i, m))
     return out_re, out_im
 
 

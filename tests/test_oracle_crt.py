@@ -1,10 +1,12 @@
 """Pins the modular emulation of the split contraction (oracle/crt_ref.py) by routes independent
 of it.
 
+* moduli: pairwise coprime, odd, -1 a square modulo each (iota^2 = -1), and the split of
+  Z_m[i] a ring map: the maps of a product are the products of the maps;
 * residues: against Python's own modulo of the same integers, in the symmetric ranges;
-* reconstruction: Garner's digits give back random integers anywhere in [-P/2, P/2), for every
-  modulus count, and the float conversion equals float() of the exact integer (correct rounding,
-  ties to even included);
+* reconstruction: Garner's digits give back random integers anywhere in [-(P-1)/2, (P-1)/2],
+  for every modulus count, and the float conversion equals float() of the exact integer
+  (correct rounding, ties to even included);
 * products: Re', Im' reconstructed from the residue products equal the exact Python-integer
   products of the same integer matrices (conjugated and plain), with row norms at the 2^beta
   bound; a product past the bound (one modulus too few) is detected as different;
@@ -32,6 +34,23 @@ def test_moduli_coprime_and_beta():
         b = Q.beta(n)
         assert (1 << (2 * b + 2)) <= P < (1 << (2 * b + 4))
     assert math.prod(Q.MODULI[:Q.LO_DIGITS]) < 2 ** 53
+    for m in Q.MODULI:
+        assert m % 2 == 1 and m <= 255
+        t = Q.iota(m)
+        assert (t * t + 1) % m == 0
+
+
+def test_gaussian_split_is_a_ring_map():
+    """(x + iy) -> x + iota y modulo m maps sums to sums and products to products, for both
+    square roots iota and -iota: the basis of the two-product form."""
+    rng = np.random.default_rng(2)
+    for m in Q.MODULI:
+        t = Q.iota(m)
+        for s in (t, m - t):
+            for _ in range(50):
+                a, b, c, d = (int(v) for v in rng.integers(-10 ** 6, 10 ** 6, 4))
+                re, im = a * c - b * d, a * d + b * c          # (a + ib)(c + id)
+                assert (re + s * im - (a + s * b) * (c + s * d)) % m == 0
 
 
 def test_residues_symmetric():
@@ -39,7 +58,7 @@ def test_residues_symmetric():
     x = np.concatenate([rng.integers(-2 ** 46, 2 ** 46, 5000), np.arange(-600, 600)]).astype(np.float64)
     for m in Q.MODULI:
         r = Q.residues(x, m)
-        lo, hi = (-128, 127) if m == 256 else (-(m - 1) // 2, (m - 1) // 2)
+        lo, hi = -(m - 1) // 2, (m - 1) // 2
         assert r.min() >= lo and r.max() <= hi
         ref = np.array([((int(v) - lo) % m) + lo for v in x], dtype=np.float64)
         assert np.array_equal(r, ref)
@@ -52,7 +71,7 @@ def test_reconstruct_round_trip(n):
     rng = np.random.default_rng(n)
     xs = [int(v) for v in rng.integers(-2 ** 62, 2 ** 62, 300)]
     xs += [(int(a) << 40) + int(b) for a, b in zip(rng.integers(-2 ** 40, 2 ** 40, 300), rng.integers(0, 2 ** 40, 300))]
-    xs = [x % P - P // 2 for x in xs] + [-P // 2, P // 2 - 1, 0, 1, -1, P // 3]
+    xs = [x % P - (P - 1) // 2 for x in xs] + [-(P - 1) // 2, (P - 1) // 2, 0, 1, -1, P // 3]
     res = [np.array([x % m for x in xs], dtype=np.int64) for m in mods]
     hi, lo, Mk = Q.reconstruct(res, mods)
     got = [int(h) * Mk + int(l) for h, l in zip(hi, lo)]
