@@ -496,7 +496,7 @@ def main():
     ap.add_argument("--no-table", action="store_true",
                     help="ignore the committed tuned-variant table (exciting_amd/tuned.json); built-in defaults")
     ap.add_argument("--set", action="append", default=[], metavar="STAGE=VARIANT",
-                    help="pin a stage's tile variant (e.g. bmt=10, ystage=8); repeatable")
+                    help="pin a stage's tile variant (e.g. bmt=1, ystage=2); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision measurements")
     ap.add_argument("--no-bf16", action="store_true", help="skip the three-slice BF16 split measurement")
     ap.add_argument("--moduli", type=int, default=11, help="moduli of the modular split (8..12, default 11)")
@@ -632,10 +632,10 @@ def main():
         mt_ms, mt_cnt = tim["mt"]
         fl = sum(mt_flops(cr, kp.ngp) for kp in w.kpoints) * args.steps
     achieved = fl / (mt_ms * 1e-3) if mt_ms > 0 else 0.0
-    # the grouped default MT kernel (variant 19) forms each complex product from three real
-    # MFMA products (Gauss's form) instead of four: it executes 6 of the 8 algorithmic flops
-    bmt_v = (tuned or {}).get("bmt", 19)
-    gauss = args.path == "batched" and bmt_v == 19
+    # the grouped MT variants 0, 2, 3 (0 is the default) form each complex product from three
+    # real MFMA products (Gauss's form) instead of four: they execute 6 of the 8 algorithmic flops
+    bmt_v = (tuned or {}).get("bmt", 0)
+    gauss = args.path == "batched" and bmt_v in (0, 2, 3)
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64, "traffic": None,
             "kernel": ("bmt_dma_grid_kernel (grouped, all k)" if args.path == "batched" else "mt_dma_kernel (per k)")

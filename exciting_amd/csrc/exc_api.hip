@@ -546,8 +546,8 @@ int setup_dev_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const i
   if (ngp > 0) {
     // defaults from the autotuner's sweep on MI355X (64x64 MT and Y tiles, LDS filled by DMA;
     // the MT tile with Gauss's three real products per complex product)
-    const int yv = c->y_variant >= 0 ? c->y_variant : 5;
-    const int mv = c->mt_variant >= 0 ? c->mt_variant : 8;
+    const int yv = c->y_variant >= 0 ? c->y_variant : 3;
+    const int mv = c->mt_variant >= 0 ? c->mt_variant : 0;
     cplx *yt = ws;
     if (h) {   // the Y stage feeds only H
       if (!ws) {
@@ -697,8 +697,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // defaults: the fastest variants measured on MI355X for N = 1150..5000 (grouped MT: 64x64
   // tile, 8 waves, two LDS stages filled by DMA, one item per workgroup; split: 64x64 tile,
   // LDS filled by DMA)
-  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 19);
-  const int yv = c->y_variant >= 0 ? c->y_variant : 12;    // 64x128 Y tiles, DMA-staged, 3 real products
+  const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 0);
+  const int yv = c->y_variant >= 0 ? c->y_variant : 0;     // 64x128 Y tiles, DMA-staged, 3 real products
   const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
   const int ov = c->olp_variant >= 0 ? c->olp_variant : 2;   // rows along lanes, column factors as scalars
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
@@ -832,13 +832,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       }
       maxngp = q.ngp > maxngp ? q.ngp : maxngp;
       const int nt = (q.ngp + T - 1) / T;
-      const bool fused = !split && exc_bmt_fused(bv);
-      // with the addition-theorem overlap the MT work list holds the H tiles only (a fused
-      // variant still computes both; its O is then overwritten)
+      // with the addition-theorem overlap the MT work list holds the H tiles only
       for (int tj = 0; tj < nt; ++tj)
         for (int ti = 0; ti <= tj; ++ti) {
           items.push_back({k, ti, tj, 0});
-          if (!fused && !olp) items.push_back({k, ti, tj, 1});
+          if (!olp) items.push_back({k, ti, tj, 1});
         }
       if (olp) {
         const int no = (q.ngp + TO - 1) / TO;
@@ -875,7 +873,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (!items.empty())
       EXC_CHECK(hipMemcpy(c->d_items, items.data(), sizeof(mt_item) * items.size(), hipMemcpyHostToDevice));
     c->batch_nitems = (int)items.size();
-    int nb = c->ncu * (split ? exc_bsplit_max_blocks(bv) : exc_bmt_max_blocks(bv));
+    int nb = c->ncu * (split ? exc_bsplit_max_blocks(bv) : 1);
     nb = nb < c->batch_nitems ? nb : c->batch_nitems;
     if (nb >= 8) nb -= nb % 8;
     c->batch_nblocks = nb > 0 ? nb : 1;
@@ -886,15 +884,15 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const bool capture = flags & EXC_BATCH_GRAPH;
   if (capture && c->stream == nullptr) return EXC_ERR_ARG;   // the null stream cannot be captured
   // inside a graph the overlap and lo kernels (they read only A, the radial factors and the
-  // structure factors, and write parts of H and O the MT stage does not) form a branch of
-  // their own beside the Y and MT stages; with a persistent MT kernel (static work split,
-  // all workgroups assumed resident) the branches join before it.  Eager calls keep one
+  // structure factors, and write parts of H and O the MT stage does not) can form a branch of
+  // their own beside the Y and MT stages; with a persistent split-BF16 kernel (static work
+  // split, all workgroups assumed resident) the branches join before it.  Eager calls keep one
   // stream so the stage timers stay exact.  With every stage filling the GPU the branches
   // only compete (workload 3: 111 ms per replay with the branch, 105 ms as one chain; workload
   // 1: 4.79 against 4.60 ms), so the default graph is one chain (variant 1 of
   // EXC_STAGE_GRAPH) and the branch is variant 0
   const bool fork = capture && fv == 0;
-  const bool join_early = split || exc_bmt_persistent(bv);
+  const bool join_early = split;
   if (fork && !c->side) {
     EXC_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     EXC_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -947,7 +945,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     } else {
       if (fork && join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
       StageTimer t(c, capture ? -1 : EXC_STAGE_BMT);
-      if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->batch_nblocks, c->stream))) return r;
+      if ((r = exc_launch_bmt(c->d, c->d_ks, c->d_items, c->batch_nitems, bv, c->stream))) return r;
     }
     if (fork && !join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
     if (fork) return r;

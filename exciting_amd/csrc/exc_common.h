@@ -109,18 +109,14 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
 int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy,
                   const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
                   hipStream_t s);
-constexpr int kYstageVariants = 14;
-constexpr int kMtVariants = 9;
-constexpr int kBmtVariants = 22;
+constexpr int kYstageVariants = 4;   // per-k and grouped Y stage (the same shapes)
+constexpr int kMtVariants = 3;       // per-k MT
+constexpr int kBmtVariants = 4;      // grouped MT
 // Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.
 int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, int variant, hipStream_t s);
 int exc_bmt_tile(int variant);   // square tile edge of a grouped-MT variant
-bool exc_bmt_fused(int variant);  // the variant computes H and O of a tile pair in one item
-bool exc_bmt_persistent(int variant);   // persistent grid with a static split of the work list
-int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
-                   hipStream_t s);
-int exc_bmt_max_blocks(int variant);   // resident workgroups per CU of a grouped-MT variant
+int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s);
 int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s);
 // adjoint (vector-Jacobian product) of the setup for rank-nst cotangents
 size_t exc_vjp_work_elems(const exc_dev &d, int ngp, int nst);

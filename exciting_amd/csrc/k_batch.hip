@@ -1,10 +1,10 @@
 // Grouped launches over all k-points of a batch (each k with its own basis size N_k).
 //
-// match, Y stage and the lo blocks are grids with a k dimension; the muffin-tin stage is a
-// persistent kernel: a fixed number of workgroups walks a work list of (k, tile row,
-// tile column, H or O) items, so the uneven tile counts of the k-points are load-balanced
-// over the whole chip instead of leaving CUs idle at the end of each k-point.  The device
-// code is the same as the per-k path (match_dev.h, setup_dev.h, lo_dev.h).
+// match, Y stage and the lo blocks are grids with a k dimension; the muffin-tin stage runs a
+// work list of (k, tile row, tile column, H or O) items over all k-points, one item per
+// workgroup, so the uneven tile counts of the k-points are balanced over the whole chip
+// instead of leaving CUs idle at the end of each k-point.  The device code is the same as
+// the per-k path (match_dev.h, setup_dev.h, mt_dma.h, lo_dev.h).
 #include "lo_dev.h"
 #include "match_dev.h"
 #include "setup_dev.h"
@@ -37,29 +37,6 @@ int launch_bmatch_l(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipSt
   return EXC_OK;
 }
 
-template <int BM, int BN, int BK, int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void bystage_kernel(const kdesc *__restrict__ ks, int natm, int naa,
-                                                               const cplx *__restrict__ haa,
-                                                               const int *__restrict__ fperm) {
-  __shared__ Smem<BM, BN, BK> sm;
-  const int kk = blockIdx.z / natm, ias = blockIdx.z % natm;
-  const kdesc k = ks[kk];
-  const int row0 = blockIdx.x * BM;
-  if (row0 >= k.ngp) return;
-  YTile<BM, BN, BK, WR, WC>::run(sm, k.ngp, naa, k.apwalm + (size_t)k.ldg * naa * ias, k.ldg,
-                                 haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0,
-                                 blockIdx.y * BN, fperm);
-}
-
-template <int BM, int BN, int BK, int WR, int WC>
-int launch_bystage_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
-  dim3 grid((maxngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm * nk);
-  hipLaunchKernelGGL((bystage_kernel<BM, BN, BK, WR, WC>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa, d.haa,
-                     d.fperm);
-  EXC_CHECK(hipGetLastError());
-  return EXC_OK;
-}
-
 template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *__restrict__ ks, int natm, int naa,
                                                                    const cplx *__restrict__ haa,
@@ -84,67 +61,10 @@ int launch_bystage_dma_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, 
   return EXC_OK;
 }
 
-template <int BM, int BK, int WR, int WC, bool DB>
-__global__ __launch_bounds__(64 * WR * WC) void bmt_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
-                                                           int nitems, int ktot, GTab gt,
-                                                           const int *__restrict__ kperm) {
-  using T = MtTile<BM, BK, WR, WC, DB>;
-  __shared__ typename T::Shared sh;
-  // Workgroup b runs on XCD b % 8 (round-robin dispatch).  Each XCD walks its own
-  // contiguous eighth of the work list, so the k-points it touches stay in its L2.
-  const int nx = gridDim.x >= 8 ? 8 : 1;
-  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
-  const int chunk = (nitems + nx - 1) / nx;
-  const int end = min(nitems, (x + 1) * chunk);
-  for (int it = x * chunk + lb; it < end; it += per) {
-    const mt_item w = items[it];
-    const kdesc k = ks[w.k];
-    __syncthreads();   // the previous item's epilogue has finished reading the tile tables
-    T::run(sh, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h, k.o, k.ldh, kperm, w.ti, w.tj,
-           w.which);
-  }
-}
-
-template <int BM, int BK, int WR, int WC, bool DB = false>
-int launch_bmt_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((bmt_kernel<BM, BK, WR, WC, DB>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
-                     d.natm * d.naa, make_gtab(d), d.kperm);
-  EXC_CHECK(hipGetLastError());
-  return EXC_OK;
-}
-
-template <int BM, int BK, int WR, int WC, bool DB = false>
-int max_blocks_t() {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_kernel<BM, BK, WR, WC, DB>, 64 * WR * WC, 0) != hipSuccess)
-    return 1;
-  return n > 0 ? n : 1;
-}
-
-// MT items with LDS filled by DMA, NST stages in flight
-template <int BM, int BK, int NST, int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void bmt_dma_kernel(const kdesc *__restrict__ ks,
-                                                               const mt_item *__restrict__ items, int nitems, int ktot,
-                                                               GTab gt, const int *__restrict__ kperm) {
-  using T = MtTileDMA<BM, BK, NST, WR, WC>;
-  using St = typename T::Stage;
-  __shared__ typename T::Tables tb;
-  EXC_DMA_STAGES(St, NST)
-  const int nx = gridDim.x >= 8 ? 8 : 1;
-  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
-  const int chunk = (nitems + nx - 1) / nx;
-  const int end = min(nitems, (x + 1) * chunk);
-  for (int it = x * chunk + lb; it < end; it += per) {
-    const mt_item w = items[it];
-    const kdesc k = ks[w.k];
-    T::run(tb, EXC_DMA_STAGE_ARGS(St), k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h,
-           k.o, k.ldh, kperm, w.ti, w.tj, w.which);
-  }
-}
-
-// the same items on a plain grid, one item per workgroup: workgroup b runs on XCD b % 8 and
-// takes the next item of that XCD's contiguous eighth of the list (L2 locality as in the
-// persistent kernel), while the dispatcher balances the load dynamically
+// MT items on a plain grid, one item per workgroup, LDS filled by DMA with NST stages in
+// flight: workgroup b runs on XCD b % 8 and takes the next item of that XCD's contiguous
+// eighth of the list (the k-points an XCD works on stay in its L2), while the dispatcher
+// balances the load dynamically
 template <int BM, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void bmt_dma_grid_kernel(const kdesc *__restrict__ ks,
                                                                     const mt_item *__restrict__ items, int nitems,
@@ -171,58 +91,6 @@ int launch_bmt_dma_grid_t(const exc_dev &d, const kdesc *ks, const mt_item *item
                      items, nitems, d.natm * d.naa, make_gtab(d), d.kperm);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
-}
-
-template <int BM, int BK, int NST, int WR, int WC>
-int launch_bmt_dma_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((bmt_dma_kernel<BM, BK, NST, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
-                     d.natm * d.naa, make_gtab(d), d.kperm);
-  EXC_CHECK(hipGetLastError());
-  return EXC_OK;
-}
-
-template <int BM, int BK, int NST, int WR, int WC>
-int max_blocks_dma_t() {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_dma_kernel<BM, BK, NST, WR, WC>, 64 * WR * WC, 0) !=
-      hipSuccess)
-    return 1;
-  return n > 0 ? n : 1;
-}
-
-// fused H+O items: one work item per (k, tile row, tile column)
-template <int BM, int BK, int WR, int WC>
-__global__ __launch_bounds__(64 * WR * WC) void bmt_ho_kernel(const kdesc *__restrict__ ks,
-                                                              const mt_item *__restrict__ items, int nitems, int ktot,
-                                                              GTab gt, const int *__restrict__ kperm) {
-  using T = MtTileHO<BM, BK, WR, WC>;
-  __shared__ typename T::Shared sh;
-  const int nx = gridDim.x >= 8 ? 8 : 1;
-  const int x = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
-  const int chunk = (nitems + nx - 1) / nx;
-  const int end = min(nitems, (x + 1) * chunk);
-  for (int it = x * chunk + lb; it < end; it += per) {
-    const mt_item w = items[it];
-    const kdesc k = ks[w.k];
-    __syncthreads();
-    T::run(sh, k.ngp, ktot, k.apwalm, k.ldg, k.yt, k.ldy, k.igpig, k.vgpc, gt, k.h, k.o, k.ldh, kperm, w.ti, w.tj);
-  }
-}
-
-template <int BM, int BK, int WR, int WC>
-int launch_bmt_ho_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((bmt_ho_kernel<BM, BK, WR, WC>), dim3(nblocks), dim3(64 * WR * WC), 0, s, ks, items, nitems,
-                     d.natm * d.naa, make_gtab(d), d.kperm);
-  EXC_CHECK(hipGetLastError());
-  return EXC_OK;
-}
-
-template <int BM, int BK, int WR, int WC>
-int max_blocks_ho_t() {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bmt_ho_kernel<BM, BK, WR, WC>, 64 * WR * WC, 0) != hipSuccess)
-    return 1;
-  return n > 0 ? n : 1;
 }
 
 __global__ __launch_bounds__(lo_dev::kRows * lo_dev::kSlots) void blo_apw_kernel(const kdesc *__restrict__ ks, int naa,
@@ -268,101 +136,30 @@ int exc_launch_bmatch(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hip
   }
 }
 
+// Y-stage variants: as the per-k Y stage's (exc_launch_ystage); 0 is the grouped default
 int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, int variant, hipStream_t s) {
   if (nk <= 0 || maxngp <= 0) return EXC_OK;
   switch (variant) {
-    case 0: return launch_bystage_t<128, 128, 16, 2, 4>(d, nk, maxngp, ks, s);
-    case 1: return launch_bystage_t<64, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
-    case 2: return launch_bystage_t<128, 64, 16, 2, 2>(d, nk, maxngp, ks, s);
-    case 3: return launch_bystage_t<64, 64, 32, 2, 2>(d, nk, maxngp, ks, s);
-    case 4: return launch_bystage_dma_t<64, 64, 16, 2, 2, 2>(d, nk, maxngp, ks, s);
-    case 5: return launch_bystage_dma_t<64, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
-    case 6: return launch_bystage_dma_t<128, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
-    case 7: return launch_bystage_dma_t<64, 64, 8, 4, 2, 2>(d, nk, maxngp, ks, s);
-    case 8: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4>(d, nk, maxngp, ks, s);
-    case 9: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
-    case 10: return launch_bystage_dma_t<64, 64, 32, 2, 2, 4>(d, nk, maxngp, ks, s);
-    case 11: return launch_bystage_dma_t<64, 128, 8, 3, 2, 4>(d, nk, maxngp, ks, s);
-    case 12: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, nk, maxngp, ks, s);   // Gauss's 3 products
-    case 13: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, nk, maxngp, ks, s);
+    case 0: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4, true>(d, nk, maxngp, ks, s);   // Gauss's 3 products
+    case 1: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, nk, maxngp, ks, s);
+    case 2: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 3: return launch_bystage_dma_t<64, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
     default: return EXC_ERR_ARG;
   }
 }
 
-int exc_bmt_tile(int variant) {
-  switch (variant) {
-    case 0: return 32;
-    case 1: return 64;
-    case 2: return 128;
-    case 3: return 32;
-    case 4: return 32;
-    case 5: return 64;
-    case 6: return 64;
-    case 7: return 64;
-    case 8: return 64;
-    case 9: return 32;
-    case 10: case 11: case 12: case 13: case 14: return 64;
-    case 15: case 16: case 17: return 128;
-    case 18: case 19: case 20: case 21: return 64;
-    default: return 0;
-  }
-}
+int exc_bmt_tile(int variant) { return variant >= 0 && variant < kBmtVariants ? 64 : 0; }
 
-bool exc_bmt_fused(int variant) { return variant >= 7 && variant <= 9; }
-
-bool exc_bmt_persistent(int variant) { return variant < 18 || variant > 21; }
-
-int exc_bmt_max_blocks(int variant) {
-  switch (variant) {
-    case 0: return max_blocks_t<32, 16, 2, 2>();
-    case 1: return max_blocks_t<64, 16, 2, 2>();
-    case 2: return max_blocks_t<128, 16, 2, 4>();
-    case 3: return max_blocks_t<32, 32, 2, 2>();
-    case 4: return max_blocks_t<32, 16, 2, 2, true>();
-    case 5: return max_blocks_t<64, 16, 2, 2, true>();
-    case 6: return max_blocks_t<64, 16, 2, 4, true>();
-    case 7: return max_blocks_ho_t<64, 16, 2, 4>();
-    case 8: return max_blocks_ho_t<64, 8, 2, 4>();
-    case 9: return max_blocks_ho_t<32, 16, 2, 2>();
-    case 10: return max_blocks_dma_t<64, 16, 2, 2, 4>();
-    case 11: return max_blocks_dma_t<64, 8, 4, 2, 4>();
-    case 12: return max_blocks_dma_t<64, 16, 3, 2, 4>();
-    case 13: return max_blocks_dma_t<64, 8, 3, 2, 2>();
-    case 14: return max_blocks_dma_t<64, 16, 2, 2, 2>();
-    case 15: return max_blocks_dma_t<128, 16, 2, 2, 4>();
-    case 16: return max_blocks_dma_t<128, 8, 2, 2, 4>();
-    case 17: return max_blocks_dma_t<128, 8, 3, 2, 4>();
-    case 18: case 19: case 20: case 21: return 1;   // plain grid: the block count is the item count
-    default: return 1;
-  }
-}
-
-int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
-                   hipStream_t s) {
+// grouped MT variants: 64 x 64 tile, 8 waves; 0: BK 16, two DMA stages, Gauss's three real
+// products per complex product (the default); 1: the same with four products; 2: BK 8, four
+// stages, three products; 3: BK 16, three stages, three products
+int exc_launch_bmt(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
   switch (variant) {
-    case 0: return launch_bmt_t<32, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 1: return launch_bmt_t<64, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 2: return launch_bmt_t<128, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 3: return launch_bmt_t<32, 32, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 4: return launch_bmt_t<32, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
-    case 5: return launch_bmt_t<64, 16, 2, 2, true>(d, ks, items, nitems, nblocks, s);
-    case 6: return launch_bmt_t<64, 16, 2, 4, true>(d, ks, items, nitems, nblocks, s);
-    case 7: return launch_bmt_ho_t<64, 16, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 8: return launch_bmt_ho_t<64, 8, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 9: return launch_bmt_ho_t<32, 16, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 10: return launch_bmt_dma_t<64, 16, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 11: return launch_bmt_dma_t<64, 8, 4, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 12: return launch_bmt_dma_t<64, 16, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 13: return launch_bmt_dma_t<64, 8, 3, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 14: return launch_bmt_dma_t<64, 16, 2, 2, 2>(d, ks, items, nitems, nblocks, s);
-    case 15: return launch_bmt_dma_t<128, 16, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 16: return launch_bmt_dma_t<128, 8, 2, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 17: return launch_bmt_dma_t<128, 8, 3, 2, 4>(d, ks, items, nitems, nblocks, s);
-    case 18: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4>(d, ks, items, nitems, s);
-    case 19: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4, true>(d, ks, items, nitems, s);   // Gauss's 3 products
-    case 20: return launch_bmt_dma_grid_t<64, 8, 4, 2, 4, true>(d, ks, items, nitems, s);
-    case 21: return launch_bmt_dma_grid_t<64, 16, 3, 2, 4, true>(d, ks, items, nitems, s);
+    case 0: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4, true>(d, ks, items, nitems, s);
+    case 1: return launch_bmt_dma_grid_t<64, 16, 2, 2, 4>(d, ks, items, nitems, s);
+    case 2: return launch_bmt_dma_grid_t<64, 8, 4, 2, 4, true>(d, ks, items, nitems, s);
+    case 3: return launch_bmt_dma_grid_t<64, 16, 3, 2, 4, true>(d, ks, items, nitems, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -1,6 +1,7 @@
 // Complex FP64 MFMA tiles of the setup with their LDS filled by DMA (global_load_lds_dwordx4):
-// the muffin-tin tile (MtTileDMA, same product and epilogue as MtTile) and the Y-stage tile
-// (YTileDMA, as YTile), without staging registers and with NST stages in flight.
+// the muffin-tin tile (MtTileDMA: conj(A) Yt^T or conj(A) A^T, then the interstitial term and
+// the Hermitian mirror) and the Y-stage tile (YTileDMA: A_a h_a^T), without staging registers
+// and with NST stages in flight.
 //
 // One wave-wide DMA moves one k-column of a 64-row panel: lane l reads the 16-byte complex
 // element of row row0 + 64p + l and the LDS receives it at (stage base) + 16 l.  The
@@ -13,8 +14,8 @@
 // of the current stage from the DMA into the others and waits for that DMA only where this
 // code asks for it (vmcnt <= the loads issued for the later stages).
 //
-// MFMA sequence per element identical to zf64::Core::compute: the results are bit-identical
-// to the register-staged kernels.
+// MFMA sequence per element identical to zf64::Core::compute (the adjoint's tiles) for the
+// four-product form: all four-product variants are bit-identical to each other.
 #pragma once
 #include <type_traits>
 #include "setup_dev.h"

@@ -154,75 +154,6 @@ struct Core {
 
   // Full K loop with two LDS stages: the next stage is stored while the current one is
   // multiplied, one barrier per stage.
-  __device__ __forceinline__ void run2(Smem<BM, BN, BK> (&sm)[2], const cplx *__restrict__ A, int lda, int row0,
-                                       int mrows, const cplx *__restrict__ B, int ldb, int col0, int nrows, int ktot,
-                                       const int *__restrict__ kperm) {
-    const int wave = threadIdx.x >> 6;
-    const int wr = wave / WC, wc = wave % WC;
-    zero();
-    cplx ra[LA], rb[LB];
-    gload(A, lda, row0, mrows, B, ldb, col0, nrows, 0, ktot, kperm, ra, rb);
-    __syncthreads();
-    sstore(sm[0], ra, rb);
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = 0; k0 < ktot; k0 += BK) {
-      const bool more = k0 + BK < ktot;
-      if (more) gload(A, lda, row0, mrows, B, ldb, col0, nrows, k0 + BK, ktot, kperm, ra, rb);
-      compute(sm[cur], wr, wc);
-      if (more) sstore(sm[cur ^ 1], ra, rb);
-      __syncthreads();
-      cur ^= 1;
-    }
-  }
-
-  // H and O of one tile pair in one pass: B columns [0, BN/2) of the tile are columns
-  // col0.. of Y, columns [BN/2, BN) are columns col0.. of A itself, so the A row panel is
-  // read once for both products.
-  __device__ __forceinline__ static void gload_ho(const cplx *__restrict__ A, int lda, int row0, int mrows,
-                                                  const cplx *__restrict__ Y, int ldy, int col0, int ncols, int k0,
-                                                  int ktot, const int *__restrict__ kperm, cplx (&ra)[LA],
-                                                  cplx (&rb)[LB]) {
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int r = 0; r < LA; ++r) {
-      const int e = tid + kThreads * r;
-      const int row = e % BM, k = e / BM;
-      const int gr = row0 + row, gk = k0 + k;
-      ra[r] = (gr < mrows && gk < ktot) ? A[gr + (size_t)(kperm ? kperm[gk] : gk) * lda] : cplx{0.0, 0.0};
-    }
-#pragma unroll
-    for (int r = 0; r < LB; ++r) {
-      const int e = tid + kThreads * r;
-      const int row = e % BN, k = e / BN;
-      const bool isy = row < BN / 2;
-      const int gr = col0 + (isy ? row : row - BN / 2), gk = k0 + k;
-      const size_t kc = kperm ? kperm[gk < ktot ? gk : 0] : gk;
-      rb[r] = (gr < ncols && gk < ktot) ? (isy ? Y[gr + kc * ldy] : A[gr + kc * lda]) : cplx{0.0, 0.0};
-    }
-  }
-
-  __device__ __forceinline__ void run2_ho(Smem<BM, BN, BK> (&sm)[2], const cplx *__restrict__ A, int lda, int row0,
-                                          int mrows, const cplx *__restrict__ Y, int ldy, int col0, int ncols, int ktot,
-                                          const int *__restrict__ kperm) {
-    const int wave = threadIdx.x >> 6;
-    const int wr = wave / WC, wc = wave % WC;
-    zero();
-    cplx ra[LA], rb[LB];
-    gload_ho(A, lda, row0, mrows, Y, ldy, col0, ncols, 0, ktot, kperm, ra, rb);
-    __syncthreads();
-    sstore(sm[0], ra, rb);
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = 0; k0 < ktot; k0 += BK) {
-      const bool more = k0 + BK < ktot;
-      if (more) gload_ho(A, lda, row0, mrows, Y, ldy, col0, ncols, k0 + BK, ktot, kperm, ra, rb);
-      compute(sm[cur], wr, wc);
-      if (more) sstore(sm[cur ^ 1], ra, rb);
-      __syncthreads();
-      cur ^= 1;
-    }
-  }
 
   // Accumulator element (v) of block (i, j) sits at tile row / column (the output layout of
   // v_mfma_f64_16x16x4f64, measured on gfx950: register v of lane l holds row 4v + l/16):

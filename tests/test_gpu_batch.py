@@ -1,11 +1,12 @@
 """Grouped (batched) setup of many k-points: one launch per stage, the muffin-tin stage a
-persistent work-list kernel, optionally captured in a hipGraph.
+work list of tiles over all k-points, optionally captured in a hipGraph.
 
 Its arithmetic is the per-k path's (same device code, same summation order), so H, O and
-apwalm must equal the per-k results bit for bit, for every tile variant but the one with
-Gauss's three real products per complex product (grouped MT variant 19: other roundings,
-H and O within 1e-14 of the per-k results relative to their norm); the per-k path is itself
-checked against the oracle in test_gpu_setup.py, and one k here is re-checked."""
+apwalm must equal the per-k results bit for bit, for every four-product tile variant; the
+variants with Gauss's three real products per complex product (grouped MT 0, 2, 3 and Y 0,
+1: other roundings) give H and O within 1e-14 of the per-k results relative to their norm.
+The per-k path is itself checked against the oracle in test_gpu_setup.py, and one k here is
+re-checked."""
 import numpy as np
 import pytest
 
@@ -15,12 +16,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _perk(ctx, batch, four_product=True):
-    """The per-k results; with four_product, on the four-product per-k variants (MT 6, Y 5)."""
+    """The per-k results; with four_product, on the four-product per-k variants (MT 1, Y 3)."""
     import torch
     out = []
     if four_product:
-        ctx.set_variant("mt", 6)
-        ctx.set_variant("ystage", 5)
+        ctx.set_variant("mt", 1)
+        ctx.set_variant("ystage", 3)
     for i, d in enumerate(batch.dks):
         ctx.match_dev(d.ngp, d.ngkmax, d.gpc, d.tpgpc, d.sfacgp, batch.apw[i])
         ctx.setup_dev(d.ngp, d.ngkmax, batch.apw[i], d.igpig, d.vgpc, batch.nmat[i], batch.H[i], batch.O[i])
@@ -40,8 +41,8 @@ def _clear(batch):
         batch.O[i].zero_()
 
 
-GAUSS_BMT = (19, 20, 21)   # grouped MT variants with Gauss's three real products per complex product
-GAUSS_Y = (12, 13)  # Y variants likewise
+GAUSS_BMT = (0, 2, 3)   # grouped MT variants with Gauss's three real products per complex product
+GAUSS_Y = (0, 1)  # Y variants likewise
 
 
 def _same(batch, ref, what, rtol=0.0):
@@ -69,7 +70,7 @@ def test_batched_equals_per_k(cfg, nk):
     with torch.cuda.stream(s):
         b = lapw.DeviceBatch(w.crystal, w.kpoints)
         ref = _perk(ctx, b)
-        ctx.set_variant("ystage", 8)   # a four-product Y: MT variants bit for bit with the per-k path
+        ctx.set_variant("ystage", 2)   # a four-product Y: MT variants bit for bit with the per-k path
         for v in range(ctx.num_variants("bmt")):
             print(f"cfg {cfg} nk {nk} grouped MT variant {v}", flush=True)
             ctx.set_variant("bmt", v)
@@ -77,7 +78,7 @@ def test_batched_equals_per_k(cfg, nk):
             ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH)
             torch.cuda.synchronize()
             _same(b, ref, f"variant {v}", 1e-14 if v in GAUSS_BMT else 0.0)
-        ctx.set_variant("bmt", 18)   # the four-product MT kernel: Y variants bit for bit with the per-k path
+        ctx.set_variant("bmt", 1)   # the four-product MT kernel: Y variants bit for bit with the per-k path
         for v in range(ctx.num_variants("ystage")):
             print(f"cfg {cfg} nk {nk} grouped Y variant {v}", flush=True)
             ctx.set_variant("ystage", v)

@@ -102,18 +102,18 @@ def test_device_entry_points_match_host():
     assert np.array_equal(O.cpu().numpy().T, O_h)
 
 
-GAUSS_Y = (12, 13)   # per-k Y and MT variants with Gauss's three real products per complex product
-GAUSS_MT = (8,)
-FOUR_PRODUCT = {"mt": 6, "ystage": 5}
+GAUSS_Y = (0, 1)   # per-k Y and MT variants with Gauss's three real products per complex product
+GAUSS_MT = (0,)
+FOUR_PRODUCT = {"mt": 1, "ystage": 3}
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
 def test_per_k_variants_bit_identical(cfg):
-    """Every tile variant of the per-k Y and MT stages (register- or DMA-staged LDS, any tile
-    shape) sums each element in the same order, so H and O agree bit for bit with the
-    four-product variants MT 6 / Y 5 (checked against the oracle like the defaults in
+    """Every four-product tile variant of the per-k Y and MT stages (any tile shape and DMA
+    staging) sums each element in the same order, so H and O agree bit for bit with the
+    four-product variants MT 1 / Y 3 (checked against the oracle like the defaults in
     test_workload0 / test_workload1_two_kpoints); the variants with Gauss's three real products
-    per complex product (MT 8, Y 12) round otherwise and agree to 1e-14 relative.  Workload 0
+    per complex product (MT 0, Y 0 and 1) round otherwise and agree to 1e-14 relative.  Workload 0
     (N = 97) exercises partial tiles and a K that is not a multiple of the K-step."""
     w = workload(cfg, 1)
     cr, kp = w.crystal, w.kpoints[0]
