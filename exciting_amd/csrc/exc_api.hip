@@ -700,7 +700,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 0);
   const int yv = c->y_variant >= 0 ? c->y_variant : 0;     // 64x128 Y tiles, DMA-staged, 3 real products
   const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
-  const int ov = c->olp_variant >= 0 ? c->olp_variant : 2;   // rows along lanes, column factors as scalars
+  const int ov = c->olp_variant >= 0 ? c->olp_variant : 3;   // rows along lanes, column factors as scalars, pipelined
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
   CrtTab ct;
   if (crt) exc_crt_tables(c->crt_nmod, &ct);
@@ -741,10 +741,16 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if ((rc = grow((void **)&c->planes, &c->planes_bytes, 2 * (ptot ? ptot : 1)))) return rc;
       if ((rc = grow((void **)&c->expo, &c->expo_bytes, sizeof(int) * (etot ? etot : 1)))) return rc;
     }
-    // radial factors and their transposed copy (overlap variant 2: N rounded up to 64 rows per
-    // atom and l)
+    // radial factors and their transposed copy (overlap variants 2 and 3: N rounded up to 64
+    // rows per atom and slot, NL + 1 slots per atom: the radial factors of each l and the
+    // structure factors; variant 3 reads up to 3 slots past the last k-point's: padding)
     size_t rttot = 0;
-    for (int k = 0; k < nk; ++k) rttot += (size_t)(ks[k].ngp + 63) / 64 * 64 * c->d.natm * nl2;
+    size_t rtpad = 0;
+    for (int k = 0; k < nk; ++k) {
+      rttot += (size_t)(ks[k].ngp + 63) / 64 * 64 * c->d.natm * (nl2 + 2);
+      rtpad = std::max(rtpad, (size_t)(ks[k].ngp + 63) / 64 * 64 * 2 * 4);
+    }
+    rttot += rtpad;
     if (olp && (rc = grow((void **)&c->rfbuf, &c->rf_bytes, sizeof(double) * (rtot + rttot ? rtot + rttot : 1))))
       return rc;
     size_t qtot = 0, xtot = 0;
@@ -802,7 +808,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         e.rf = c->rfbuf + roff;
         e.rft = c->rfbuf + rtot + rtoff;
         roff += (size_t)q.ngp * c->d.natm * nl2;
-        rtoff += (size_t)(q.ngp + 63) / 64 * 64 * c->d.natm * nl2;
+        rtoff += (size_t)(q.ngp + 63) / 64 * 64 * c->d.natm * (nl2 + 2);
       }
       e.cnp = 0;
       e.qa = e.qy = nullptr;
@@ -839,9 +845,14 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
           if (!olp) items.push_back({k, ti, tj, 1});
         }
       if (olp) {
-        const int no = (q.ngp + TO - 1) / TO;
-        for (int tj = 0; tj < no; ++tj)
-          for (int ti = 0; ti <= tj; ++ti) oitems.push_back({k, ti, tj, 1});
+        // in 8 x 8 blocks of tiles: the 64 workgroups an XCD runs at a time (the overlap
+        // kernel hands each XCD a contiguous part of the list) share 8 row and 8 column
+        // panels of the radial factors in its L2
+        const int no = (q.ngp + TO - 1) / TO, S = 8;
+        for (int bj = 0; bj < no; bj += S)
+          for (int bi = 0; bi <= bj; bi += S)
+            for (int tj = bj; tj < std::min(bj + S, no); ++tj)
+              for (int ti = bi; ti < std::min(bi + S, tj + 1); ++ti) oitems.push_back({k, ti, tj, 1});
       }
     }
     if (olp) {

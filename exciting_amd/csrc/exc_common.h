@@ -134,7 +134,7 @@ int exc_launch_split(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipS
 int exc_bsplit_tile(int variant);
 int exc_bsplit_max_blocks(int variant);
 int exc_olp_tile();   // tile edge of the addition-theorem overlap kernel
-constexpr int kOlpVariants = 3;
+constexpr int kOlpVariants = 4;
 int exc_launch_olp(const exc_dev &d, int nk, int maxngp, const kdesc *ks, const mt_item *items, int nitems, int variant,
                    hipStream_t s);
 int exc_launch_bsplit(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, int variant, int nblocks,
@@ -144,7 +144,7 @@ constexpr int kMaxMod = 14;          // moduli available: 241, 233, 229, 221, 20
 constexpr int kCrtLoDigits = 6;      // mixed-radix digits of the low part (their product < 2^53)
 constexpr int kCrtMaxUsed = 12;      // beyond 12 moduli beta > 44: re -+ iota im of the row
                                      // integers no longer exact in FP64
-constexpr int kCrtVariants = 8;      // tile / staging variants of the residue GEMM
+constexpr int kCrtVariants = 11;     // tile / staging variants of the residue GEMM
 struct CrtTab {                      // passed by value to the kernels (kernel-argument segment)
   int n, beta;
   int m[kMaxMod];                    // odd, pairwise coprime, every prime factor 1 mod 4

@@ -288,6 +288,187 @@ __global__ __launch_bounds__(64 * WR * WC, MINW) void crt_gemm_kernel(const kdes
 }
 
 // ---------------------------------------------------------------------------------------
+// gemm, register-pipelined form for one wave per SIMD (4 waves, 2 x 2, 128 x 128 wave tiles:
+// 8 LDS fragment reads per 16 MFMAs instead of 6 per 8 — the LDS, read by the MFMA operands
+// and written by the DMA, bounds the 2 x 4 form).  With no second wave on the SIMD to cover
+// the LDS latency, the fragments of MFMA k-step s + 1 are read while the MFMAs of step s run
+// (two register sets), and the workgroup barrier that hands an LDS stage over to the DMA is
+// taken once per stage, at its last k-step, after its last fragments are in registers:
+//   wait for the next stage's DMA | barrier | read its first fragments | DMA into the stage
+//   just finished | MFMAs of the last k-step
+// so that the next stage's first fragments are in flight during those MFMAs.  Same
+// products, same integer sums, same output layout as crt_gemm_kernel with a 2 x 2 wave grid.
+// ---------------------------------------------------------------------------------------
+template <int BT, int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_frags(const typename GemmCfg<BT, BK, NST, WR, WC>::Stage *st, int s,
+                                           v4i (&af)[GemmCfg<BT, BK, NST, WR, WC>::MI],
+                                           v4i (&bf)[GemmCfg<BT, BK, NST, WR, WC>::NJ], int wr, int wc) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 31, hk = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < C::MI; ++i)
+    af[i] = *(const v4i *)(st->a + ((2 * s + hk) * BT + wr * (32 * C::MI) + 32 * i + r) * 16);
+#pragma unroll
+  for (int j = 0; j < C::NJ; ++j)
+    bf[j] = *(const v4i *)(st->b + ((2 * s + hk) * BT + wc * (32 * C::NJ) + 32 * j + r) * 16);
+}
+
+template <int BT, int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_mfma(const v4i (&af)[GemmCfg<BT, BK, NST, WR, WC>::MI],
+                                          const v4i (&bf)[GemmCfg<BT, BK, NST, WR, WC>::NJ],
+                                          v16i (&acc)[GemmCfg<BT, BK, NST, WR, WC>::MI][GemmCfg<BT, BK, NST, WR, WC>::NJ]) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
+#pragma unroll
+  for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+    for (int j = 0; j < C::NJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+}
+
+// the L wave-wide DMAs of one stage from per-slot sources fixed for the tile (slot q: panel,
+// 16-byte chunk, 64-row quarter), `off` bytes further along k: no address arithmetic at the
+// hand-over beyond one add per slot
+template <int BT, int BK, int NST, int WR, int WC>
+__device__ __forceinline__ void gemm_issue_pre(typename GemmCfg<BT, BK, NST, WR, WC>::Stage *st, const unsigned char *pak,
+                                               const unsigned char *pbk,
+                                               const unsigned (&voff)[GemmCfg<BT, BK, NST, WR, WC>::L],
+                                               const int (&doff)[GemmCfg<BT, BK, NST, WR, WC>::L]) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
+  static_assert((C::NL / 2) % C::kWaves == 0, "slot q of every wave in one panel");
+#pragma unroll
+  for (int q = 0; q < C::L; ++q)
+    __builtin_amdgcn_global_load_lds((const void *)((q * C::kWaves >= C::NL / 2 ? pbk : pak) + voff[q]),
+                                     (lds_void *)((unsigned char *)st + doff[q]), 16, 0, 0);
+}
+
+template <int BT, int BK, int NST, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, 1) void crt_gemm_pipe_kernel(const kdesc *__restrict__ ks,
+                                                                        const mt_item *__restrict__ items,
+                                                                        const int2 *__restrict__ work, int nwork,
+                                                                        int kp, CrtTab t, signed char *__restrict__ res) {
+  using C = GemmCfg<BT, BK, NST, WR, WC>;
+  using St = typename C::Stage;
+  constexpr int S = BK / 32;                    // MFMA k-steps per stage
+  static_assert(S % 2 == 0 && NST >= 2 && NST <= 4, "register sets alternate within a stage; LDS size");
+  EXC_DMA_STAGES(St, NST)
+  const int chunk = (nwork + 7) / 8;
+  const int x = blockIdx.x % 8, qb = blockIdx.x / 8;
+  const int wi = x * chunk + qb;
+  if (qb >= chunk || wi >= nwork) return;
+  const int2 wk = work[wi];
+  const mt_item it = items[wk.x];
+  const kdesc k = ks[it.k];
+  const int l = wk.y >> 2, p = wk.y & 3;
+  const size_t pstride = (size_t)kp * k.cnp, chunk_stride = (size_t)k.cnp * 16;
+  const unsigned char *pa = (const unsigned char *)k.qa + (size_t)(2 * l + p) * pstride;
+  const unsigned char *pb = it.which ? (const unsigned char *)k.qa + (size_t)(2 * l + 1 - p) * pstride
+                                     : (const unsigned char *)k.qy + (size_t)(2 * l + p) * pstride;
+  const int row0 = it.ti * BT, col0 = it.tj * BT;
+  const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
+  v16i acc[C::MI][C::NJ];
+#pragma unroll
+  for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+    for (int j = 0; j < C::NJ; ++j) acc[i][j] = v16i{};
+  St *st[4] = {&s0_, &s1_, (St *)&s2_, (St *)&s3_};
+  const int nsteps = kp / BK;
+  // DMA slot q of this wave: panel (A for the first half of the slots), 16-byte k-chunk kc
+  // and 64-row quarter h; its source is the panel's base for the stage plus voff[q] (below
+  // 2^32: kc < KC, rows < cnp), its destination doff[q] bytes into the stage
+  unsigned voff[C::L];
+  int doff[C::L];
+  {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+    for (int q = 0; q < C::L; ++q) {
+      const int j = wv + C::kWaves * q;
+      const bool isb = j >= C::NL / 2;
+      const int jj = isb ? j - C::NL / 2 : j;
+      const int h = jj % (BT / 64), kc = jj / (BT / 64);
+      voff[q] = (unsigned)((size_t)kc * chunk_stride + (size_t)((isb ? col0 : row0) + 64 * h + lane) * 16);
+      doff[q] = (isb ? C::PB : 0) + kc * (BT * 16) + h * 1024;
+    }
+  }
+  const size_t sstride = (size_t)C::KC * chunk_stride;       // bytes of k per stage
+  // stage j's DMA is issued at the hand-over of stage j - NST (stages 0 .. NST-2 here, NST-1
+  // at the first hand-over below); stages past the end load the last one again (never read),
+  // so that exactly NST - 2 stages' DMA after the next one is in flight at every hand-over
+  const size_t klast = (size_t)(nsteps - 1) * sstride;
+#pragma unroll
+  for (int u = 0; u < NST - 1; ++u) {
+    const size_t o = std::min((size_t)u * sstride, klast);
+    gemm_issue_pre<BT, BK, NST, WR, WC>(st[u], pa + o, pb + o, voff, doff);
+  }
+  v4i fa[2][C::MI], fb[2][C::NJ];
+  // hand-over before stage 0
+  wait_vm_lgkm0<(NST - 2) * C::L>();
+  stage_barrier();
+  gemm_frags<BT, BK, NST, WR, WC>(st[0], 0, fa[0], fb[0], wr, wc);
+  {
+    const size_t o = std::min((size_t)(NST - 1) * sstride, klast);
+    gemm_issue_pre<BT, BK, NST, WR, WC>(st[NST - 1], pa + o, pb + o, voff, doff);
+  }
+  // one stage per iteration, its LDS buffer chosen at run time, and no branch inside: the
+  // accumulators never flow through one (they would be copied between register files), and
+  // the scheduler can put one fragment read and one DMA between two MFMAs.  The last stage
+  // hands over too (its reads unused).
+  size_t koff = (size_t)(NST - 1) * sstride;    // k offset of the DMA at the next hand-over, less one stage
+#pragma unroll 1
+  for (int step = 0; step < nsteps; ++step) {
+    const int u = step % NST;
+    const int un = u + 1 < NST ? u + 1 : 0;
+    koff = koff + sstride < klast ? koff + sstride : klast;    // stage min(step + NST, nsteps - 1)
+    // (selects, not an indexed array: a run-time index into an array of LDS addresses would
+    // make it a static table)
+    St *const cs = u == 0 ? &s0_ : (u == 1 ? &s1_ : (u == 2 ? (St *)&s2_ : (St *)&s3_));
+    St *const ns = un == 0 ? &s0_ : (un == 1 ? &s1_ : (un == 2 ? (St *)&s2_ : (St *)&s3_));
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (s + 1 < S) {
+        gemm_frags<BT, BK, NST, WR, WC>(cs, s + 1, fa[(s + 1) & 1], fb[(s + 1) & 1], wr, wc);
+        gemm_mfma<BT, BK, NST, WR, WC>(fa[s & 1], fb[s & 1], acc);
+#pragma unroll
+        for (int q = 0; q < C::MI + C::NJ; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+      } else {
+        // hand-over: stage step + 1 has landed once at most the DMA of the NST - 2 stages
+        // after it is out
+        wait_vm_lgkm0<(NST - 2) * C::L>();
+        stage_barrier();
+        gemm_frags<BT, BK, NST, WR, WC>(ns, 0, fa[0], fb[0], wr, wc);
+        gemm_issue_pre<BT, BK, NST, WR, WC>(cs, pa + koff, pb + koff, voff, doff);
+        gemm_mfma<BT, BK, NST, WR, WC>(fa[s & 1], fb[s & 1], acc);
+#pragma unroll
+        for (int q = 0; q < C::MI + C::NJ; ++q) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          if (q < C::L) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0);
+        }
+      }
+    }
+  }
+  wait_vm_lgkm0<0>();                            // no DMA into this workgroup's LDS outlives it
+  const int m = t.m[l];
+  const double mi = t.dminv[l];
+  const int lane = threadIdx.x & 63;
+  signed char *out = res + (((size_t)wk.x * t.n + l) * 2 + p) * (size_t)(BT * BT);
+#pragma unroll
+  for (int i = 0; i < C::MI; ++i)
+#pragma unroll
+    for (int j = 0; j < C::NJ; ++j) {
+      unsigned u4[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int v = 0; v < 16; ++v) u4[v >> 2] |= (unsigned)(dres((double)acc[i][j][v], m, mi) & 255) << (8 * (v & 3));
+      const int blk = wave * (C::MI * C::NJ) + i * C::NJ + j;
+      *(uint4 *)(out + ((size_t)blk * 64 + lane) * 16) = make_uint4(u4[0], u4[1], u4[2], u4[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // merge: one tile per workgroup (4 waves).  Wave w takes the 32 x 32 blocks w, w + 4, ...
 // of the gemm layout (wave-major as the gemm wrote them, decoded with that variant's wave
 // grid), one lane per gemm lane with its 16 values.
@@ -509,6 +690,17 @@ int launch_gemm_t(const kdesc *ks, const mt_item *items, const int2 *work, int n
   return EXC_OK;
 }
 
+template <int BT, int BK, int NST>
+int launch_gemm_pipe_t(const kdesc *ks, const mt_item *items, const int2 *work, int nwork, int kp, const CrtTab &t,
+                       signed char *res, hipStream_t s) {
+  if (kp % BK) return EXC_ERR_UNSUPPORTED;
+  const int chunk = (nwork + 7) / 8;
+  hipLaunchKernelGGL((crt_gemm_pipe_kernel<BT, BK, NST, 2, 2>), dim3(8 * chunk), dim3(256), 0, s, ks, items, work,
+                     nwork, kp, t, res);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
 }  // namespace
 
 void exc_crt_tables(int n, CrtTab *t) {
@@ -541,10 +733,12 @@ int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, co
 //   4: 128, 32, 4, 2 x 2, 2
 //   5: 256, 64, 3, 2 x 2 (128 x 128 wave tiles: two thirds of the LDS reads per MFMA), 1
 //   6: 256, 32, 4, 2 x 2, 1                           7: 256, 64, 4, 2 x 2, 1
+//   8: 256, 64, 3, 2 x 2, register-pipelined          9: 256, 64, 2, 2 x 2, register-pipelined
+//  10: 256, 64, 4, 2 x 2, register-pipelined
 int exc_crt_shape(int variant, int *bt, int *wr, int *wc) {
   switch (variant) {
     case 0: case 1: case 3: *bt = 256; *wr = 2; *wc = 4; return EXC_OK;
-    case 5: case 6: case 7: *bt = 256; *wr = 2; *wc = 2; return EXC_OK;
+    case 5: case 6: case 7: case 8: case 9: case 10: *bt = 256; *wr = 2; *wc = 2; return EXC_OK;
     case 2: case 4: *bt = 128; *wr = 2; *wc = 2; return EXC_OK;
     default: return EXC_ERR_ARG;
   }
@@ -563,6 +757,9 @@ int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items,
     case 5: return launch_gemm_t<256, 64, 3, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
     case 6: return launch_gemm_t<256, 32, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
     case 7: return launch_gemm_t<256, 64, 4, 2, 2, 1>(ks, items, work, nwork, kp, t, res, s);
+    case 8: return launch_gemm_pipe_t<256, 64, 3>(ks, items, work, nwork, kp, t, res, s);
+    case 9: return launch_gemm_pipe_t<256, 64, 2>(ks, items, work, nwork, kp, t, res, s);
+    case 10: return launch_gemm_pipe_t<256, 64, 4>(ks, items, work, nwork, kp, t, res, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -202,8 +202,8 @@ __global__ __launch_bounds__(64 * 64 / (4 * CJ)) __attribute__((amdgpu_waves_per
 //   b~_l = gamma_l c_l + x b~_(l+1) + beta~_l b~_(l+2),  beta~_l = -(l+1)^2 / ((2l+1)(2l+3)),
 // and b~_0 = b_0 = sum_l c_l P_l(x): four FP64 operations per element and l instead of five
 // (rounded differently from variants 0 and 1).  The radial factors are first copied to
-// rft[((atom nl + l) N + G) 2 + io] times sqrt(gamma_l) (olp_rft_kernel): one l of 64 rows is
-// then 1 KiB contiguous, and the 8 columns of a wave 128 contiguous bytes.
+// rft[((atom (nl + 1) + nl - 1 - l) N + G) 2 + io] times sqrt(gamma_l) (olp_rft_kernel): one
+// l of 64 rows is then 1 KiB contiguous, and the 8 columns of a wave 128 contiguous bytes.
 // ---------------------------------------------------------------------------------------
 struct ScaledClenshawTab {
   double gamma[kNLmax], beta[kNLmax];
@@ -226,18 +226,22 @@ typedef const __attribute__((address_space(4))) double *cdbl_ptr;   // scalar-ca
 // columns of a tile never need clamping
 __device__ __forceinline__ int rft_ld(int ngp) { return (ngp + kOT - 1) / kOT * kOT; }
 
-// rf (per atom and row, l and io fastest) -> rft (per atom and l, rows fastest), times sqrt(gamma_l)
+// rf (per atom and row, l and io fastest) -> rft (per atom and slot, rows fastest): slot
+// nl - 1 - l holds the radial factors of l times sqrt(gamma_l) (l descending, the order of
+// the recurrence), slot nl the structure factor S_a(G) (re, im); rows past N are zero
 __global__ __launch_bounds__(256) void olp_rft_kernel(const kdesc *__restrict__ ks, int nl) {
   const kdesc k = ks[blockIdx.z];
   const int row = blockIdx.x * 256 + threadIdx.x, at = blockIdx.y, ld = rft_ld(k.ngp);
   if (row >= ld) return;
   const d2 *src = (const d2 *)(k.rf + ((size_t)at * k.ngp + row) * 2 * nl);
-  d2 *dst = (d2 *)k.rft;
+  d2 *dst = (d2 *)k.rft + (size_t)at * (nl + 1) * ld + row;
   for (int l = 0; l < nl; ++l) {
     const double g = sqrt(kScaled.gamma[l]);
     const d2 v = row < k.ngp ? src[l] : d2{0.0, 0.0};
-    dst[((size_t)at * nl + l) * ld + row] = d2{v.a * g, v.b * g};
+    dst[(size_t)(nl - 1 - l) * ld] = d2{v.a * g, v.b * g};
   }
+  const cplx sf = row < k.ngp ? k.sfacgp[row + (size_t)k.ldg * at] : cplx{0.0, 0.0};
+  dst[(size_t)nl * ld] = d2{sf.re, sf.im};
 }
 
 template <int NL>
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(512) void olp_rows_kernel(const kdesc *__restrict__
     // used; the scheduling barriers keep the scalar loads from being hoisted further (the
     // registers would spill) and let the rows' vector loads move ahead (mask: VMEM reads)
     auto col = [&](int l, d2 (&c)[E]) {
-      cdbl_ptr pc = (cdbl_ptr)(rft + ((size_t)at * NL + l) * ld + cw);
+      cdbl_ptr pc = (cdbl_ptr)(rft + ((size_t)at * (NL + 1) + NL - 1 - l) * ld + cw);
 #pragma unroll
       for (int e = 0; e < E; ++e) c[e] = d2{pc[2 * e], pc[2 * e + 1]};
     };
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(512) void olp_rows_kernel(const kdesc *__restrict__
     col(NL - 1, cur);
 #pragma unroll
     for (int l = NL - 1; l >= 0; --l) {
-      const d2 r = rft[((size_t)at * NL + l) * ld + rowc];
+      const d2 r = rft[((size_t)at * (NL + 1) + NL - 1 - l) * ld + rowc];
       if (l > 0) col(l - 1, nxt);
       const double bt = kScaled.beta[l];
 #pragma unroll
@@ -372,6 +376,194 @@ void launch_rows(const exc_dev &d, const kdesc *ks, const mt_item *items, int ni
   hipLaunchKernelGGL(olp_rows_kernel<NL>, dim3(nitems), dim3(512), 0, s, ks, items, nitems, d.natm, make_gtab(d));
 }
 
+// ---------------------------------------------------------------------------------------
+// Variant 3: variant 2's arithmetic (bit for bit) on a software pipeline.  Variant 2 waits for
+// every l's operands as it needs them: the rows' factors (a vector load) and the columns'
+// (scalar loads) come from L2 or further, and its waves stall on each.  Here one atom is
+// NL + 1 steps (the l-sum, l descending, then the structure factors, from rft's last slot):
+//  * the rows' operands of step s + 2 are loaded at step s (global loads, returned in order,
+//    so the wait for step s leaves the later ones in flight);
+//  * the columns' operands of step s + 1 are loaded at step s as soon as step s's have
+//    arrived (scalar loads return out of order: any wait is for all of them, so the next
+//    ones are issued only after it), with beta~ of step s + 1 from LDS;
+//  * at most 128 VGPRs (a few spilled), so that two workgroups share a CU;
+//  * workgroup b runs on XCD b % 8 and takes the next item of that XCD's contiguous eighth
+//    of the list (which holds each k-point's tiles in 8 x 8 blocks): the tiles an XCD runs
+//    together share their row and column factors in its L2.
+// ---------------------------------------------------------------------------------------
+typedef double gdv2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) gdv2 *gdv2_ptr;
+
+// forces the wait for the scalar loads of `v` here (and of every scalar load before them);
+// returns a zero the compiler cannot see through, so that loads indexed with it stay after
+// this point (instead of being hoisted out of the loop into registers held throughout)
+__device__ __forceinline__ int scalar_ready(double v) {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z) : "s"(v));
+  return z;
+}
+
+template <int NL>
+__global__ __launch_bounds__(512, 4) void olp_pipe_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
+                                                       int nitems, int natm, GTab gt) {
+  constexpr int E = 8, NS = NL + 1, D = 2;             // columns per wave; steps per atom; row prefetch distance
+  static_assert(D <= NS, "prefetch within two atoms");
+  __shared__ double uc[kOT][3];
+  __shared__ int gc[kOT][3];
+  __shared__ cplx tr[8][E * 65];
+  __shared__ double bsh[NL];                            // beta~_l, read per step (held in no register)
+  const int chunk = (nitems + 7) / 8, qb = blockIdx.x / 8;
+  const int item = (blockIdx.x % 8) * chunk + qb;
+  if (qb >= chunk || item >= nitems) return;
+  const mt_item w = items[item];
+  const kdesc k = ks[w.k];
+  const int ngp = k.ngp, row0 = w.ti * kOT, col0 = w.tj * kOT;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int row = row0 + lane, rowc = min(row, ngp - 1);
+  const int cw = col0 + E * wave;
+  if (threadIdx.x < kOT) {
+    const int q = threadIdx.x, idx = col0 + q;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    int g0 = 0, g1 = 0, g2 = 0;
+    if (idx < ngp) {
+      double st, ct, sp, cp;
+      sincos(k.tpgpc[2 * idx], &st, &ct);
+      sincos(k.tpgpc[2 * idx + 1], &sp, &cp);
+      u0 = st * cp; u1 = st * sp; u2 = ct;
+      const int ig = k.igpig[idx] - 1;
+      g0 = gt.ivg[3 * ig]; g1 = gt.ivg[3 * ig + 1]; g2 = gt.ivg[3 * ig + 2];
+    }
+    uc[q][0] = u0; uc[q][1] = u1; uc[q][2] = u2;
+    gc[q][0] = g0; gc[q][1] = g1; gc[q][2] = g2;
+  } else if (threadIdx.x < kOT + NL) {
+    bsh[threadIdx.x - kOT] = kScaled.beta[threadIdx.x - kOT];
+  }
+  double ur0, ur1, ur2;
+  {
+    double st, ct, sp, cp;
+    sincos(k.tpgpc[2 * rowc], &st, &ct);
+    sincos(k.tpgpc[2 * rowc + 1], &sp, &cp);
+    ur0 = st * cp; ur1 = st * sp; ur2 = ct;
+  }
+  __syncthreads();
+  double x[E], are[E], aim[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = E * wave + e;
+    x[e] = ur0 * uc[q][0] + ur1 * uc[q][1] + ur2 * uc[q][2];
+    are[e] = aim[e] = 0.0;
+  }
+  // rft is one stream of slots (atom-major, NL + 1 slots per atom, ld rows each), walked by two
+  // running pointers: the rows' D slots ahead, the columns' one slot ahead.  Past the last
+  // atom they read up to D slots further (the next k-point's factors, or the padding after
+  // the last one): unused.  Rows and columns past N read rft's zero padding.
+  const int ld = rft_ld(ngp);
+  const size_t sb = (size_t)ld * 16;                    // bytes per slot
+  const unsigned char *rsp = (const unsigned char *)k.rft;
+  const unsigned char *csp = (const unsigned char *)k.rft + (size_t)cw * 16;
+  const unsigned rofs = (unsigned)row * 16;
+  auto rowld = [&]() -> gdv2 {
+    const gdv2 v = *(gdv2_ptr)(rsp + rofs);
+    rsp += sb;
+    return v;
+  };
+  auto colld = [&](gdv2 (&c)[E]) {
+    // the address is wave-uniform; readfirstlane keeps it (and the loads) scalar where the
+    // compiler would otherwise fold it into the rows' per-lane pointer
+    const unsigned long long a = (unsigned long long)csp;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    cdbl_ptr pc = (cdbl_ptr)(((unsigned long long)hi << 32) | lo);
+#pragma unroll
+    for (int e = 0; e < E; ++e) c[e] = gdv2{pc[2 * e], pc[2 * e + 1]};
+    csp += sb;
+  };
+  gdv2 rr[NS + D];                                      // rr[s]: step s of this atom, rr[NS + t]: step t of the next
+#pragma unroll
+  for (int t = 0; t < D; ++t) rr[t] = rowld();
+  gdv2 cur[E], nxt[E];
+  double bcur = 0.0, bnxt = 0.0;
+  colld(cur);
+#pragma unroll 1
+  for (int at = 0; at < natm; ++at) {
+    double b1[E], b2[E];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      // the scheduling barriers (only vector-memory reads may cross them) keep the order
+      // wait for this step's columns | issue the next step's | this step's arithmetic
+      rr[s + D] = rowld();
+      __builtin_amdgcn_sched_barrier(0x0020);
+      const int z = scalar_ready(cur[0].x);
+      __builtin_amdgcn_sched_barrier(0x0020);
+      colld(nxt);
+      // beta~ of the next step (steps 2 .. NL-1 use one) with the next columns
+      if (s + 1 >= 2 && s + 1 < NL) bnxt = bsh[NL - 2 - s + z];
+      __builtin_amdgcn_sched_barrier(0x0020);
+      const gdv2 r = rr[s];
+      if (s < NL) {
+        const double bt = bcur;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double a = fma(r.y, cur[e].y, r.x * cur[e].x);
+          const double b0 = s == 0 ? a : (s == 1 ? fma(x[e], b1[e], a) : fma(bt, b2[e], fma(x[e], b1[e], a)));
+          b2[e] = s == 0 ? 0.0 : b1[e];
+          b1[e] = b0;
+        }
+      } else {
+        // conj(S(G)) S(G') sum_l P_l c_l: r = S(G) of the row, cur = S(G') of the columns
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double t1 = cur[e].x * b1[e], t2 = cur[e].y * b1[e];
+          are[e] = fma(r.x, t1, fma(r.y, t2, are[e]));
+          aim[e] = fma(r.x, t2, fma(-r.y, t1, aim[e]));
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) cur[e] = nxt[e];
+      bcur = bnxt;
+    }
+#pragma unroll
+    for (int t = 0; t < D; ++t) rr[t] = rr[NS + t];
+  }
+  // + Theta(G_i - G_j); store (i, j) and conj at (j, i) through this wave's LDS slice, as variant 2
+  int gr0 = 0, gr1 = 0, gr2 = 0;
+  {
+    const int ig = k.igpig[rowc] - 1;
+    gr0 = gt.ivg[3 * ig]; gr1 = gt.ivg[3 * ig + 1]; gr2 = gt.ivg[3 * ig + 2];
+  }
+  cplx *tw = tr[wave];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int q = E * wave + e, c = col0 + q;
+    double re = are[e], im = aim[e];
+    if (c < ngp && row <= c) {
+      const int ig = lookup_g(gt, gr0 - gc[q][0], gr1 - gc[q][1], gr2 - gc[q][2]);
+      if (ig) {
+        const cplx th = gt.cfunig[ig - 1];
+        re += th.re;
+        im += th.im;
+      }
+      k.o[row + (size_t)c * k.ldh] = cplx{re, im};
+    }
+    tw[e * 65 + lane] = cplx{re, -im};
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int e = lane & 7, r = (lane >> 3) + 8 * i;
+    const int gr = row0 + r, c = cw + e;
+    if (c < ngp && gr < c) k.o[c + (size_t)gr * k.ldh] = tw[e * 65 + r];
+  }
+}
+
+template <int NL>
+void launch_pipe(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, hipStream_t s) {
+  hipLaunchKernelGGL(olp_pipe_kernel<NL>, dim3((nitems + 7) / 8 * 8), dim3(512), 0, s, ks, items, nitems, d.natm,
+                     make_gtab(d));
+}
+
 }  // namespace
 
 int exc_olp_tile() { return kOT; }
@@ -406,6 +598,26 @@ int exc_launch_olp(const exc_dev &d, int nk, int maxngp, const kdesc *ks, const 
         case 11: launch_rows<11>(d, ks, items, nitems, s); break;
         case 12: launch_rows<12>(d, ks, items, nitems, s); break;
         case 13: launch_rows<13>(d, ks, items, nitems, s); break;
+        default: return EXC_ERR_UNSUPPORTED;
+      }
+      break;
+    case 3:
+      hipLaunchKernelGGL(olp_rft_kernel, dim3((maxngp + kOT - 1 + 255) / 256, d.natm, nk), dim3(256), 0, s, ks,
+                         d.lmaxmat + 1);
+      EXC_CHECK(hipGetLastError());
+      switch (d.lmaxmat + 1) {
+        case 2: launch_pipe<2>(d, ks, items, nitems, s); break;
+        case 3: launch_pipe<3>(d, ks, items, nitems, s); break;
+        case 4: launch_pipe<4>(d, ks, items, nitems, s); break;
+        case 5: launch_pipe<5>(d, ks, items, nitems, s); break;
+        case 6: launch_pipe<6>(d, ks, items, nitems, s); break;
+        case 7: launch_pipe<7>(d, ks, items, nitems, s); break;
+        case 8: launch_pipe<8>(d, ks, items, nitems, s); break;
+        case 9: launch_pipe<9>(d, ks, items, nitems, s); break;
+        case 10: launch_pipe<10>(d, ks, items, nitems, s); break;
+        case 11: launch_pipe<11>(d, ks, items, nitems, s); break;
+        case 12: launch_pipe<12>(d, ks, items, nitems, s); break;
+        case 13: launch_pipe<13>(d, ks, items, nitems, s); break;
         default: return EXC_ERR_UNSUPPORTED;
       }
       break;

@@ -77,8 +77,8 @@ def test_argument_errors_without_gpu():
     assert lib.exc_ctx_destroy(None) == 0
     assert lib.exc_ctx_num_variants(2) >= 2
     assert lib.exc_ctx_num_variants(10) == 9          # eigensolve paths 0-8
-    assert lib.exc_ctx_num_variants(15) == 8          # residue GEMM variants of the modular split
-    assert lib.exc_ctx_num_variants(9) == 3           # thread blockings of the addition-theorem overlap
+    assert lib.exc_ctx_num_variants(15) == 11         # residue GEMM variants of the modular split
+    assert lib.exc_ctx_num_variants(9) == 4           # thread blockings of the addition-theorem overlap
     assert lib.exc_ctx_num_variants(5) == 2           # grouped graph: overlap on a side branch or in line
 
 

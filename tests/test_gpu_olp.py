@@ -32,7 +32,7 @@ def _run(w, flags, olp_variant=None):
     return ctx, s, b, [(b.apw[i].cpu(), b.H[i].cpu(), b.O[i].cpu()) for i in range(b.nk)]
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 3])
 @pytest.mark.parametrize("which", ["small-d", "workload0", "workload1"])
 def test_olp_legendre_against_oracle(which, variant):
     from exciting_amd import lapw
@@ -96,7 +96,8 @@ def test_olp_legendre_needs_match():
 def test_olp_variants_bit_identical():
     """The overlap kernel's thread blockings (4 x 2 and 4 x 4 elements per thread) run the same
     Clenshaw recurrence per element in the same order: O bit for bit; the row-parallel
-    variant 2 (scaled recurrence) agrees to within rounding."""
+    variant 2 (scaled recurrence) agrees to within rounding, and its pipelined form (variant
+    3) bit for bit."""
     import torch
     from exciting_amd import lapw
     w = workload(1, 3)
@@ -115,6 +116,8 @@ def test_olp_variants_bit_identical():
             out[v] = [t.clone() for t in b.O]
     for i in range(b.nk):
         assert torch.equal(out[1][i], out[0][i]), f"overlap variant 1 k{i}"
+        # variant 3 runs variant 2's operations in the same order (pipelined loads only)
+        assert torch.equal(out[3][i], out[2][i]), f"overlap variant 3 k{i}"
         # variant 2 runs the recurrence scaled (another rounding): within a few units of the
         # last place of the largest element
         d = (out[2][i] - out[0][i]).abs().max().item()
