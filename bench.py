@@ -166,14 +166,18 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
     if kind == "crt":
         ms = eager["crtgemm"][0]
         ops = crt_alg_ops(cr, ngps, nmod, share)
-        roof = {"bound": "mfma", "kernel": f"crt_gemm_kernel (residue products, {nmod} moduli, int8 MFMA)",
+        # variants 8-10 are the register-pipelined kernel; the library's default by basis size
+        # (the tuned table pins one for every workload here)
+        v = (tuned or {}).get("crtgemm", 8 if max(ngps) >= 2048 else 4)
+        gname = "crt_gemm_pipe_kernel" if v in (8, 9, 10) else "crt_gemm_kernel"
+        roof = {"bound": "mfma", "kernel": f"{gname} (residue products, {nmod} moduli, int8 MFMA)",
                 "achieved": ops / (ms * 1e-3) / 1e12, "peak": PEAK_INT8 / 1e12, "unit": "TOP/s",
                 "frac": ops / (ms * 1e-3) / PEAK_INT8, "fp64_equivalent_TFLOPs": fl64 / (ms * 1e-3) / 1e12,
                 "per_launch_ops": ops, "traffic": None}
         # HBM traffic of the residue GEMM from the committed rocprofv3 summary of the default command
         prof = HERE / "profiles" / f"setup_kernels_workload{args.config}.json"
         if args.nk is None and not args.tune and not args.set and args.moduli == 11 and olpflag and prof.exists():
-            rec = json.loads(prof.read_text()).get("crt_gemm_kernel")
+            rec = json.loads(prof.read_text()).get(gname)
             if rec:
                 roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
                 roof["traffic_source"] = str(prof.relative_to(HERE))

@@ -177,6 +177,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->planes) hipFree(c->planes);
   if (c->crt_q) hipFree(c->crt_q);
   if (c->crt_x) hipFree(c->crt_x);
+  if (c->crt_parts) hipFree(c->crt_parts);
   if (c->crt_res) hipFree(c->crt_res);
   if (c->d_citems) hipFree(c->d_citems);
   if (c->d_cwork) hipFree(c->d_cwork);
@@ -699,7 +700,10 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // LDS filled by DMA)
   const int bv = split ? (c->bsplit_variant >= 0 ? c->bsplit_variant : 3) : (c->bmt_variant >= 0 ? c->bmt_variant : 0);
   const int yv = c->y_variant >= 0 ? c->y_variant : 0;     // 64x128 Y tiles, DMA-staged, 3 real products
-  const int cv = c->crt_variant >= 0 ? c->crt_variant : 0;   // 256 x 256 tiles, BK 64, 3 stages
+  // residue GEMM: the register-pipelined 256 x 256 form from N = 2048, 128 x 128 tiles below
+  int maxn = 0;
+  for (int k = 0; k < nk; ++k) maxn = std::max(maxn, ks[k].ngp);
+  const int cv = c->crt_variant >= 0 ? c->crt_variant : (maxn >= 2048 ? 8 : 4);
   const int ov = c->olp_variant >= 0 ? c->olp_variant : 3;   // rows along lanes, column factors as scalars, pipelined
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
   CrtTab ct;
@@ -756,18 +760,23 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     size_t qtot = 0, xtot = 0;
     int cbt = 0, cwr = 0, cwc = 0;
     if (crt && exc_crt_shape(cv, &cbt, &cwr, &cwc)) return EXC_ERR_ARG;
+    // squared row norms of A per atom (cnp each), when the grouped match runs here
+    const size_t nparts = do_match ? (size_t)c->d.natm : 0;
+    size_t ptot2 = 0;
     if (crt) {
       for (int k = 0; k < nk; ++k) {
         const size_t cnp = (size_t)(ks[k].ngp + 255) / 256 * 256;
         qtot += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
         xtot += 2 * cnp;
+        ptot2 += nparts * cnp;
       }
       if ((rc = grow(&c->crt_q, &c->crt_q_bytes, qtot ? qtot : 1))) return rc;
       if ((rc = grow((void **)&c->crt_x, &c->crt_x_bytes, sizeof(int) * (xtot ? xtot : 1)))) return rc;
+      if ((rc = grow((void **)&c->crt_parts, &c->crt_parts_bytes, sizeof(double) * (ptot2 ? ptot2 : 1)))) return rc;
     }
     std::vector<mt_item> citems;
     std::vector<int2> cwork;
-    size_t qoff = 0, xoff = 0;
+    size_t qoff = 0, xoff = 0, poff2 = 0;
     int maxcnp = 0;
     std::vector<kdesc> kd(nk);
     std::vector<mt_item> items, oitems;
@@ -812,6 +821,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       }
       e.cnp = 0;
       e.qa = e.qy = nullptr;
+      e.pa2 = nullptr;
       e.xa = e.xy = nullptr;
       if (crt) {
         // planes [modulus][plane][K/16][cnp][16] of A then of Yt; tiles (k, ti <= tj, H | O)
@@ -824,6 +834,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         qoff += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
         e.xa = c->crt_x + xoff;
         e.xy = c->crt_x + xoff + cnp;
+        e.pa2 = do_match ? c->crt_parts + poff2 : nullptr;
+        poff2 += nparts * cnp;
         xoff += 2 * (size_t)cnp;
         maxcnp = cnp > maxcnp ? cnp : maxcnp;
         const int nt = (q.ngp + cbt - 1) / cbt, first = (int)citems.size(), S = 4;
@@ -933,7 +945,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       // APW-APW tiles: no conflict with the side branch
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTPREP);
-        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, c->stream))) return r;
+        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, do_match, c->stream))) return r;
       }
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTGEMM);

@@ -94,6 +94,9 @@ struct kdesc {
   int cnp, pad_;
   void *qa, *qy;
   int *xa, *xy;
+  // squared row norms of A per atom, written by the grouped match, [atom][cnp]: A's
+  // exponents xa are summed from them
+  double *pa2;
 };
 
 // Work item of the grouped muffin-tin launch: (k, tile row, tile column, H=0 / O=1).
@@ -154,7 +157,8 @@ struct CrtTab {                      // passed by value to the kernels (kernel-a
 void exc_crt_tables(int n, CrtTab *t);
 inline int exc_crt_kpad(int ktot) { return (ktot + 63) / 64 * 64; }
 int exc_crt_shape(int variant, int *bt, int *wr, int *wc);   // tile edge and wave grid of a GEMM variant
-int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, hipStream_t s);
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, bool a_parts,
+                        hipStream_t s);
 int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
                         const CrtTab &t, signed char *res, int variant, hipStream_t s);
 int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,

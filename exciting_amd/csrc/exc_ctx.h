@@ -78,6 +78,8 @@ struct exc_ctx {
   size_t crt_q_bytes;
   int *crt_x;
   size_t crt_x_bytes;
+  double *crt_parts;          // partial squared row norms of A and Yt (modular split)
+  size_t crt_parts_bytes;
   signed char *crt_res;
   size_t crt_res_bytes;
   mt_item *d_citems;

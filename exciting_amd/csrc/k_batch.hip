@@ -25,7 +25,8 @@ __global__ __launch_bounds__(kMatchThreads) void bmatch_kernel(const kdesc *__re
   const int ig = blockIdx.x * kMatchThreads + threadIdx.x;
   if (ig >= k.ngp) return;
   match_dev::match_body<L>(ig, blockIdx.y, apwordmax, spidx, rmt, fpso, dinv, k.gpc, k.tpgpc, k.sfacgp, k.ldg,
-                           k.apwalm, k.ldg, k.rf, nl, k.ngp);
+                           k.apwalm, k.ldg, k.rf, nl, k.ngp,
+                           k.pa2 ? k.pa2 + (size_t)blockIdx.y * k.cnp + ig : nullptr);
 }
 
 template <int L>

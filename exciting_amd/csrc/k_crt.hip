@@ -48,12 +48,13 @@ constexpr int kMaxTile = 256;    // largest tile edge of the gemm and merge kern
 __device__ __forceinline__ int dres(double x, int m, double minv) { return (int)fma(-(double)m, rint(x * minv), x); }
 
 // ---------------------------------------------------------------------------------------
-// scale: one workgroup per (k, operand, 64 rows); wave w sums |x|^2 over its quarter of K,
-// the quarters are added in wave order (a fixed order: the exponents do not depend on timing)
+// scale: the per-row exponents s = beta - e, |X_i| < 2^e, of Yt, and of A when the caller
+// supplies apwalm (with the grouped match, crt_expo_kernel sums A's from the norms the match
+// wrote): one workgroup per (k, 64 rows); wave w sums |x|^2 over its quarter of K, the
+// quarters added in wave order (a fixed order: the exponents do not depend on timing)
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict__ ks, int ktot, int beta) {
-  const kdesc k = ks[blockIdx.z >> 1];
-  const int op = blockIdx.z & 1;              // 0: A (apwalm), 1: Yt
+__global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict__ ks, int ktot, int beta, int op) {
+  const kdesc k = ks[blockIdx.z];             // op 0: A (apwalm), 1: Yt
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = blockIdx.x * 64 + lane;
   __shared__ double part[4][64];
@@ -77,6 +78,20 @@ __global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict_
     if (s2 > 0.0) frexp(sqrt(s2), &e);       // |X_i| < 2^e
     (op ? k.xy : k.xa)[row] = row < k.ngp ? beta - e : 0;
   }
+}
+
+// A's exponents from the squared norms per (atom, row) that the grouped match wrote (in
+// closed form), summed in atom order: no pass over A.  One thread per row.
+__global__ __launch_bounds__(256) void crt_expo_kernel(const kdesc *__restrict__ ks, int natm, int beta) {
+  const kdesc k = ks[blockIdx.z];
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= k.cnp) return;
+  double s2 = 0.0;
+  if (row < k.ngp)
+    for (int a = 0; a < natm; ++a) s2 += k.pa2[(size_t)a * k.cnp + row];
+  int e = 0;
+  if (s2 > 0.0) frexp(sqrt(s2), &e);
+  k.xa[row] = row < k.ngp ? beta - e : 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -717,10 +732,20 @@ void exc_crt_tables(int n, CrtTab *t) {
   t->beta = (int)floor((lp - 2.0 - 1e-9) / 2.0);
 }
 
-int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, hipStream_t s) {
+// a_parts: the grouped match wrote A's squared norms per atom (else A takes a pass of its
+// own, as Yt always does)
+int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, bool a_parts,
+                        hipStream_t s) {
   if (nk <= 0 || maxnp <= 0) return EXC_OK;
   const int ktot = d.natm * d.naa, kp = exc_crt_kpad(ktot);
-  hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, 2 * nk), dim3(256), 0, s, ks, ktot, t.beta);
+  if (a_parts) {
+    hipLaunchKernelGGL(crt_expo_kernel, dim3((maxnp + 255) / 256, 1, nk), dim3(256), 0, s, ks, d.natm, t.beta);
+    EXC_CHECK(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, nk), dim3(256), 0, s, ks, ktot, t.beta, 0);
+    EXC_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, nk), dim3(256), 0, s, ks, ktot, t.beta, 1);
   EXC_CHECK(hipGetLastError());
   hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t);
   EXC_CHECK(hipGetLastError());

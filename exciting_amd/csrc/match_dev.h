@@ -89,7 +89,7 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
                                            const double *__restrict__ dinv, const double *__restrict__ gpc,
                                            const double *__restrict__ tpgpc, const cplx *__restrict__ sfacgp,
                                            int ldsf, cplx *__restrict__ apwalm, int ldg, double *__restrict__ rf = nullptr,
-                                           int nl = 0, int ngp = 0) {
+                                           int nl = 0, int ngp = 0, double *__restrict__ s2out = nullptr) {
 #pragma clang fp contract(off)   // see sbessel_seq
   constexpr int lmmax = (L + 1) * (L + 1);
   const double g = gpc[ig];
@@ -108,6 +108,18 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
   const size_t colstride = (size_t)ldg;
   cplx *out = apwalm + ig + (size_t)ldg * apwordmax * lmmax * ias;
 
+  if (s2out) {
+    // sum over this atom's (lm, io) of |apwalm|^2 = (4 pi/sqrt(Omega))^2 |S|^2 |Y_lm|^2 r_io^2,
+    // by sum_m |Y_lm|^2 = (2l+1)/(4 pi): O(lmax) instead of a sum over every value
+    double t = 0.0;
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+      const double r0 = di[l * 4 + 0] * jl[l] + di[l * 4 + 2] * djl[l];
+      const double r1 = di[l * 4 + 1] * jl[l] + di[l * 4 + 3] * djl[l];
+      t = fma((double)(2 * l + 1), fma(r1, r1, r0 * r0), t);
+    }
+    *s2out = fpso * fpso * (sf.re * sf.re + sf.im * sf.im) * t * 0.079577471545947667884;   // 1/(4 pi)
+  }
   double pmm = 0.28209479177387814347;  // 1/sqrt(4 pi)
 #pragma unroll
   for (int m = 0; m <= L; ++m) {

@@ -183,6 +183,31 @@ def test_crt_variants_bit_identical_and_graph():
     ctx.close()
 
 
+def test_crt_caller_apwalm_equals_match():
+    """Without EXC_BATCH_MATCH the caller's apwalm is used and A's row exponents come from a
+    pass over it (with the match they are summed from the norms the match kernel writes in
+    closed form).  On the same apwalm the two runs agree bit for bit (the exponents are the
+    same integers)."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 2)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints)
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_CRT)
+        torch.cuda.synchronize()
+        ref = [(h.clone(), o.clone()) for h, o in zip(b.H, b.O)]
+        for t in b.H + b.O:
+            t.fill_(float("nan"))
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_SPLIT_CRT)
+        torch.cuda.synchronize()
+        for i in range(b.nk):
+            print(f"workload 1 k{i}: caller apwalm against match", flush=True)
+            assert torch.equal(b.H[i], ref[i][0]) and torch.equal(b.O[i], ref[i][1]), f"k{i}"
+    ctx.close()
+
+
 def test_crt_moduli_count():
     """Fewer moduli give fewer bits and a larger error, within each count's own bound
     2^(1 - beta) sqrt(2K) |A_i| |A_j| against the FP64 oracle."""
