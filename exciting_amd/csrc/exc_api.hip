@@ -178,6 +178,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->crt_q) hipFree(c->crt_q);
   if (c->crt_x) hipFree(c->crt_x);
   if (c->crt_parts) hipFree(c->crt_parts);
+  if (c->gk_buf) hipFree(c->gk_buf);
   if (c->crt_res) hipFree(c->crt_res);
   if (c->d_citems) hipFree(c->d_citems);
   if (c->d_cwork) hipFree(c->d_cwork);

@@ -355,6 +355,90 @@ int exc_ctx_genveffig(exc_ctx *c, const double *veffir, double *veffig) {
   return EXC_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// G+k sets (gengpvec.f90, gensfacgp.f90)
+// ---------------------------------------------------------------------------------------
+namespace {
+// the gk_buf layout: [kv: 6 nk doubles][ngp: nk ints, 8-byte aligned][staging]
+size_t gk_head(int nk) { return 48 * (size_t)nk + ((4 * (size_t)nk + 255) / 256) * 256 + 256; }
+}  // namespace
+
+int exc_gengpvec_dev(exc_ctx *c, int nk, const double *bvec, const double *vpl, const double *vpc, double gkmax,
+                     int ngkmax, int *ngp, int *igpig, double *vgpl, double *vgpc, double *gpc, double *tpgpc,
+                     double *sfacgp) {
+  if (!c || nk < 0 || !bvec || ngkmax <= 0 || !(gkmax > 0.0)) return EXC_ERR_ARG;
+  if (nk == 0) return EXC_OK;
+  if (!vpl || !vpc || !ngp || !igpig || !vgpl || !vgpc || !gpc || !tpgpc) return EXC_ERR_ARG;
+  if (!c->have_atoms || !c->have_gvec) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t head = gk_head(nk);
+  if ((rc = grow(&c->gk_buf, &c->gk_buf_bytes, head))) return rc;
+  std::vector<double> kv(6 * (size_t)nk);
+  for (int k = 0; k < nk; ++k)
+    for (int d = 0; d < 3; ++d) {
+      kv[6 * k + d] = vpl[3 * k + d];
+      kv[6 * k + 3 + d] = vpc[3 * k + d];
+    }
+  double *dkv = (double *)c->gk_buf;
+  int *dngp = (int *)((char *)c->gk_buf + 48 * (size_t)nk);
+  EXC_CHECK(hipMemcpyAsync(dkv, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice, c->stream));
+  if ((rc = exc_launch_gengpvec(c->d, nk, bvec, dkv, gkmax, ngkmax, dngp, igpig, vgpl, vgpc, gpc, tpgpc,
+                                (cplx *)sfacgp, c->stream)))
+    return rc;
+  EXC_CHECK(hipMemcpyAsync(ngp, dngp, sizeof(int) * (size_t)nk, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nk; ++k)
+    if (ngp[k] > ngkmax) return EXC_ERR_ARG;
+  return EXC_OK;
+}
+
+int exc_gengpvec(exc_ctx *c, const double *bvec, const double *vpl, const double *vpc, double gkmax, int ngkmax,
+                 int *ngp, int *igpig, double *vgpl, double *vgpc, double *gpc, double *tpgpc) {
+  if (!c || !bvec || !vpl || !vpc || ngkmax <= 0 || !(gkmax > 0.0) || !ngp || !igpig || !vgpl || !vgpc || !gpc ||
+      !tpgpc)
+    return EXC_ERR_ARG;
+  if (!c->have_atoms || !c->have_gvec) return EXC_ERR_STATE;
+  int rc = use_device(c);
+  if (rc) return rc;
+  // device staging after the head: igpig, vgpl, vgpc, gpc, tpgpc (9 doubles + 1 int per row)
+  const size_t head = gk_head(1), n = (size_t)ngkmax;
+  if ((rc = grow(&c->gk_buf, &c->gk_buf_bytes, head + 72 * n + 4 * n + 256))) return rc;
+  char *st = (char *)c->gk_buf + head;
+  double *d_vgpl = (double *)st, *d_vgpc = d_vgpl + 3 * n, *d_gpc = d_vgpc + 3 * n, *d_tp = d_gpc + n;
+  int *d_ig = (int *)(d_tp + 2 * n);
+  int cnt = 0;
+  const int rc2 = exc_gengpvec_dev(c, 1, bvec, vpl, vpc, gkmax, ngkmax, &cnt, d_ig, d_vgpl, d_vgpc, d_gpc, d_tp,
+                                   nullptr);
+  if (rc2 && rc2 != EXC_ERR_ARG) return rc2;
+  *ngp = cnt;
+  EXC_CHECK(hipMemcpyAsync(igpig, d_ig, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(vgpl, d_vgpl, 24 * n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(vgpc, d_vgpc, 24 * n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(gpc, d_gpc, 8 * n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(tpgpc, d_tp, 16 * n, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return rc2;
+}
+
+int exc_gensfacgp(exc_ctx *c, int ngp, const double *vgpc, int ld, double *sfacgp) {
+  if (!c || ngp < 0 || ld < ngp || (ngp > 0 && (!vgpc || !sfacgp))) return EXC_ERR_ARG;
+  if (!c->have_atoms) return EXC_ERR_STATE;
+  if (ngp == 0) return EXC_OK;
+  int rc = use_device(c);
+  if (rc) return rc;
+  const size_t head = gk_head(1), nv = 3 * (size_t)ngp, ns = (size_t)ld * c->d.natm;
+  if ((rc = grow(&c->gk_buf, &c->gk_buf_bytes, head + 8 * nv + 16 * ns + 256))) return rc;
+  double *d_v = (double *)((char *)c->gk_buf + head);
+  cplx *d_s = (cplx *)(d_v + nv);
+  EXC_CHECK(hipMemcpyAsync(d_v, vgpc, 8 * nv, hipMemcpyHostToDevice, c->stream));
+  EXC_CHECK(hipMemcpyAsync(d_s, sfacgp, 16 * ns, hipMemcpyHostToDevice, c->stream));   // rows past ngp kept
+  if ((rc = exc_launch_gensfacgp(c->d, ngp, d_v, ld, d_s, c->stream))) return rc;
+  EXC_CHECK(hipMemcpyAsync(sfacgp, d_s, 16 * ns, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  return EXC_OK;
+}
+
 namespace {
 constexpr int kRhoirChunk = 64;   // states per batched FFT of genrhoir
 

@@ -78,8 +78,10 @@ struct exc_ctx {
   size_t crt_q_bytes;
   int *crt_x;
   size_t crt_x_bytes;
-  double *crt_parts;          // partial squared row norms of A and Yt (modular split)
+  double *crt_parts;          // squared row norms of A per atom (modular split)
   size_t crt_parts_bytes;
+  void *gk_buf;               // G+k sets: k-point coordinates, counts, and the host entries' staging
+  size_t gk_buf_bytes;
   signed char *crt_res;
   size_t crt_res_bytes;
   mt_item *d_citems;

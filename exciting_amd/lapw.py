@@ -105,6 +105,9 @@ SIGNATURES = {
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
+    "exc_gengpvec": [_P, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P],
+    "exc_gensfacgp": [_P, _I, _P, _I, _P],
+    "exc_gengpvec_dev": [_P, _I, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P, _P],
 }
 
 # stage order of exc_ctx_read_timing
@@ -371,6 +374,51 @@ class Context:
         cir = np.zeros(cr.ngrtot)
         self._call("exc_ctx_gencfun", self.h, _ptr(bvec), _ptr(cf), _ptr(cir))
         return cf, cir
+
+    def gengpvec(self, vpl, vpc, gkmax, ngkmax):
+        """exc_gengpvec with the crystal's bvec: (ngp, igpig, vgpl, vgpc, gpc, tpgpc) as host
+        arrays (Fortran layouts, rows past ngp zero)."""
+        bvec = np.asfortranarray(self.cr.bvec, dtype=np.float64)
+        vl = np.ascontiguousarray(vpl, dtype=np.float64)
+        vc = np.ascontiguousarray(vpc, dtype=np.float64)
+        ngp = C.c_int(0)
+        igpig = np.zeros(ngkmax, dtype=np.int32)
+        vgpl = np.zeros((3, ngkmax), order="F")
+        vgpc = np.zeros((3, ngkmax), order="F")
+        gpc = np.zeros(ngkmax)
+        tpgpc = np.zeros((2, ngkmax), order="F")
+        self._call("exc_gengpvec", self.h, _ptr(bvec), _ptr(vl), _ptr(vc), float(gkmax), int(ngkmax), C.byref(ngp),
+                   _ptr(igpig), _ptr(vgpl), _ptr(vgpc), _ptr(gpc), _ptr(tpgpc))
+        return ngp.value, igpig, vgpl, vgpc, gpc, tpgpc
+
+    def gensfacgp(self, ngp, vgpc, ld):
+        """exc_gensfacgp: S_a(G+k) of vgpc(3, >= ngp) as a host array (ld, natmtot)."""
+        v = np.asfortranarray(vgpc[:, :ngp], dtype=np.float64)
+        out = np.zeros((ld, self.cr.natm), dtype=np.complex128, order="F")
+        self._call("exc_gensfacgp", self.h, int(ngp), _ptr(v), int(ld), _ptr(out))
+        return out
+
+    def gengpvec_dev(self, vpl, vpc, gkmax, ngkmax, sfacgp=True):
+        """exc_gengpvec_dev for nk k-points (vpl, vpc: (nk, 3)): ngp (list) and torch device
+        tensors igpig (nk, ngkmax), vgpl, vgpc (nk, ngkmax, 3), gpc (nk, ngkmax),
+        tpgpc (nk, ngkmax, 2), sfacgp (nk, natm, ngkmax) complex or None."""
+        import torch
+        vl = np.ascontiguousarray(vpl, dtype=np.float64).reshape(-1, 3)
+        vc = np.ascontiguousarray(vpc, dtype=np.float64).reshape(-1, 3)
+        nk = vl.shape[0]
+        bvec = np.asfortranarray(self.cr.bvec, dtype=np.float64)
+        dev = torch.device("cuda")
+        igpig = torch.empty((nk, ngkmax), dtype=torch.int32, device=dev)
+        vgpl = torch.empty((nk, ngkmax, 3), dtype=torch.float64, device=dev)
+        vgpc = torch.empty((nk, ngkmax, 3), dtype=torch.float64, device=dev)
+        gpc = torch.empty((nk, ngkmax), dtype=torch.float64, device=dev)
+        tpgpc = torch.empty((nk, ngkmax, 2), dtype=torch.float64, device=dev)
+        sf = torch.empty((nk, self.cr.natm, ngkmax), dtype=torch.complex128, device=dev) if sfacgp else None
+        ngp = (C.c_int * nk)()
+        self._call("exc_gengpvec_dev", self.h, nk, _ptr(bvec), _ptr(vl), _ptr(vc), float(gkmax), int(ngkmax),
+                   C.cast(ngp, _P), _ptr(igpig), _ptr(vgpl), _ptr(vgpc), _ptr(gpc), _ptr(tpgpc),
+                   _ptr(sf) if sf is not None else None)
+        return list(ngp), igpig, vgpl, vgpc, gpc, tpgpc, sf
 
     def genveffig(self, veffir):
         """exc_ctx_genveffig: V(G) (ngvec,) from V(r) on the grid (ngrtot,)."""

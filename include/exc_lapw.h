@@ -357,6 +357,29 @@ int exc_genrhoir(exc_ctx *ctx, int ngp, const int *igpig, int nst, int ldv, cons
 int exc_ctx_gencfun(exc_ctx *ctx, const double *bvec, double *cfunig, double *cfunir);
 int exc_ctx_genveffig(exc_ctx *ctx, const double *veffir, double *veffig);
 
+/* G+k sets of k-points on the GPU (gengpvec.f90:13-94 and gensfacgp.f90:13-58; once per
+ * k-set in init1, where the reference calls them): the G vectors of exc_ctx_set_gvec's ivg
+ * (the reference's sorted order, gengvec.f90:74), G = i1 b1 + i2 b2 + i3 b3 (bvec(3,3) as in
+ * exc_ctx_gencfun), that satisfy |G + k| < gkmax, in that order.
+ * exc_gengpvec — one k-point, host arrays, the reference's arguments: vpl(3), vpc(3) the
+ *   k-point in lattice and Cartesian coordinates; out ngp, igpig(ngkmax) (1-based G index),
+ *   vgpl(3,ngkmax), vgpc(3,ngkmax), gpc(ngkmax), tpgpc(2,ngkmax) (sphcrd.f90: theta, phi),
+ *   rows past ngp zero.  More than ngkmax vectors: EXC_ERR_ARG with *ngp set to the full
+ *   count and the first ngkmax rows written (the reference stops, gengpvec.f90:53-58).
+ * exc_gensfacgp — S_a(G+k) = exp(i (G+k).r_a) of ngp given vectors vgpc(3,ngp) for every atom
+ *   (exc_ctx_set_atoms' positions) into sfacgp(ld, natmtot), host arrays.
+ * exc_gengpvec_dev — nk k-points at once, one workgroup each: vpl, vpc (3,nk) host arrays,
+ *   ngp(nk) host output; device outputs per k-point with leading dimension ngkmax:
+ *   igpig(ngkmax,nk), vgpl(3,ngkmax,nk), vgpc(3,ngkmax,nk), gpc(ngkmax,nk), tpgpc(2,ngkmax,nk),
+ *   and sfacgp(ngkmax,natmtot,nk) complex (null: not formed) — the layouts the setup entries
+ *   read.  Synchronizes the context's stream (ngp comes back to the host). */
+int exc_gengpvec(exc_ctx *ctx, const double *bvec, const double *vpl, const double *vpc, double gkmax,
+                 int ngkmax, int *ngp, int *igpig, double *vgpl, double *vgpc, double *gpc, double *tpgpc);
+int exc_gensfacgp(exc_ctx *ctx, int ngp, const double *vgpc, int ld, double *sfacgp);
+int exc_gengpvec_dev(exc_ctx *ctx, int nk, const double *bvec, const double *vpl, const double *vpc, double gkmax,
+                     int ngkmax, int *ngp, int *igpig, double *vgpl, double *vgpc, double *gpc, double *tpgpc,
+                     double *sfacgp);
+
 /* Number of local-orbital columns (nlotot) of the context, for sizing H and O. */
 int exc_ctx_nlotot(const exc_ctx *ctx);
 
