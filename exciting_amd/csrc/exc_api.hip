@@ -479,6 +479,19 @@ int exc_ctx_set_gvec(exc_ctx *c, const int *ivg, const int *ivgig, const double 
   return EXC_OK;
 }
 
+int exc_ctx_set_meffig(exc_ctx *c, const double *meffig) {
+  if (!c) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  if (meffig) {
+    if ((rc = dev_upload(c, &c->d.meffig, meffig, (size_t)c->dims.ngvec))) return rc;
+    EXC_CHECK(hipStreamSynchronize(c->stream));
+  }
+  c->d.zora = meffig ? 1 : 0;
+  c->batch_sig.clear();        // a captured graph holds the tables' addresses: capture again
+  return EXC_OK;
+}
+
 int exc_match_dev(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *tpgpc, const double *sfacgp,
                   double *apwalm) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
@@ -1005,6 +1018,7 @@ int exc_ho_vjp_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
                    double *dedtau, double *dedv) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (!c->d.ffacg) return EXC_ERR_STATE;
+  if (c->d.zora) return EXC_ERR_UNSUPPORTED;   // the adjoint differentiates the Theta-form kinetic term only
   const int nmat = ngp + c->d.nlotot;
   if (ngp < 0 || ngkmax < ngp || nst < 0 || ldc < nmat || !dedtau || !dedv) return EXC_ERR_ARG;
   if (ngp > 0 && nst > 0 && (!apwalm || !igpig || !vgpc || !sfacgp || !evec || !wh || !wo)) return EXC_ERR_ARG;

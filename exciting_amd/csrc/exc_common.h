@@ -62,6 +62,8 @@ struct exc_dev {
   int *ivgig;          // box
   cplx *cfunig;        // [ngrtot]
   cplx *veffig;        // [ngvec]
+  cplx *meffig;        // [ngvec]: the kinetic term's factor with ValenceRelativity /= 'none'
+  int zora;            // 1: H's interstitial kinetic term takes meffig instead of cfunig
   double *ffacg;       // [nspecies][ngvec]
   int *kperm;          // [natm*naa]: summation order of the MT product (column of apwalm)
   int *fperm;          // [naa]: summation order inside one atom (Y stage)

@@ -663,8 +663,8 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
             } else {
               const cplx vv2 = gt.veffig[ig - 1];
               const double t1 = 0.5 * (vr[rl][0] * vc[cl][0] + vr[rl][1] * vc[cl][1] + vr[rl][2] * vc[cl][2]);
-              re += vv2.re + t1 * th.re;
-              im += vv2.im + t1 * th.im;
+              re += vv2.re + t1 * gt.kinig[ig - 1].re;
+              im += vv2.im + t1 * gt.kinig[ig - 1].im;
             }
           }
           val = cplx{re, im};

@@ -296,8 +296,8 @@ __global__ __launch_bounds__(64 * WR * WC) void bsplit_kernel(const kdesc *__res
             } else {
               const cplx vv = gt.veffig[ig - 1];
               const double t1 = 0.5 * (sh.vr[rl][0] * sh.vc[cl][0] + sh.vr[rl][1] * sh.vc[cl][1] + sh.vr[rl][2] * sh.vc[cl][2]);
-              re += vv.re + t1 * th.re;
-              im += vv.im + t1 * th.im;
+              re += vv.re + t1 * gt.kinig[ig - 1].re;
+              im += vv.im + t1 * gt.kinig[ig - 1].im;
             }
           }
           out[r + (size_t)c * k.ldh] = cplx{re, im};
@@ -498,8 +498,8 @@ __global__ __launch_bounds__(256) void bsplit2_kernel(const kdesc *__restrict__ 
             } else {
               const cplx vv = gt.veffig[ig - 1];
               const double t1 = 0.5 * (k.vgpc[3 * r] * vc0 + k.vgpc[3 * r + 1] * vc1 + k.vgpc[3 * r + 2] * vc2);
-              re += vv.re + t1 * th.re;
-              im += vv.im + t1 * th.im;
+              re += vv.re + t1 * gt.kinig[ig - 1].re;
+              im += vv.im + t1 * gt.kinig[ig - 1].im;
             }
           }
           out[r + (size_t)c * k.ldh] = cplx{re, im};

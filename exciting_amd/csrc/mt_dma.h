@@ -338,8 +338,8 @@ struct MtTileDMA {
             } else {
               const cplx vv = gt.veffig[ig - 1];
               const double t1 = 0.5 * (tb.vr[rl][0] * tb.vc[cl][0] + tb.vr[rl][1] * tb.vc[cl][1] + tb.vr[rl][2] * tb.vc[cl][2]);
-              re += vv.re + t1 * th.re;
-              im += vv.im + t1 * th.im;
+              re += vv.re + t1 * gt.kinig[ig - 1].re;
+              im += vv.im + t1 * gt.kinig[ig - 1].im;
             }
           }
           out[r + (size_t)c * ldh] = cplx{re, im};

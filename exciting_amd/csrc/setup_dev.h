@@ -21,6 +21,8 @@ struct GTab {
   const int *ivgig;    // box
   const cplx *cfunig;  // [ngrtot]
   const cplx *veffig;  // [ngvec]
+  const cplx *kinig;   // the factor of H's interstitial kinetic term: cfunig, or meffig [ngvec]
+                       // with ValenceRelativity /= 'none' (hmlistln.f90:61-72 against :74-84)
   unsigned *miss;      // count of basis differences found outside the G set (exc_ctx_status)
   int ngvec;
   int lo0, lo1, lo2, n0, n1, n2;
@@ -32,6 +34,7 @@ inline GTab make_gtab(const exc_dev &d) {
   t.ivgig = d.ivgig;
   t.cfunig = d.cfunig;
   t.veffig = d.veffig;
+  t.kinig = d.zora ? d.meffig : d.cfunig;
   t.miss = d.miss;
   t.ngvec = d.ngvec;
   t.lo0 = d.box_lo[0]; t.lo1 = d.box_lo[1]; t.lo2 = d.box_lo[2];

@@ -105,6 +105,7 @@ SIGNATURES = {
     "exc_gendmatmt_dev": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
+    "exc_ctx_set_meffig": [_P, _P],
     "exc_gengpvec": [_P, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P],
     "exc_gensfacgp": [_P, _I, _P, _I, _P],
     "exc_gengpvec_dev": [_P, _I, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P, _P],
@@ -374,6 +375,15 @@ class Context:
         cir = np.zeros(cr.ngrtot)
         self._call("exc_ctx_gencfun", self.h, _ptr(bvec), _ptr(cf), _ptr(cir))
         return cf, cir
+
+    def set_meffig(self, meffig):
+        """exc_ctx_set_meffig: the interstitial kinetic factor of H with ValenceRelativity /=
+        'none' (ngvec complex), or None for the 'none' form (cfunig)."""
+        if meffig is None:
+            self._call("exc_ctx_set_meffig", self.h, None)
+            return
+        m = np.ascontiguousarray(meffig, dtype=np.complex128)
+        self._call("exc_ctx_set_meffig", self.h, _ptr(m))
 
     def gengpvec(self, vpl, vpc, gkmax, ngkmax):
         """exc_gengpvec with the crystal's bvec: (ngp, igpig, vgpl, vgpc, gpc, tpgpc) as host

@@ -128,6 +128,15 @@ int exc_ctx_set_mt(exc_ctx *ctx, const double *haa, const double *loa, const dou
 int exc_ctx_set_gvec(exc_ctx *ctx, const int *ivg, const int *ivgig, const double *cfunig,
                      const double *veffig, const double *ffacg);
 
+/* ValenceRelativity /= 'none' in the interstitial (hmlistln.f90:61-72): meffig(ngvec) complex,
+ * the Fourier components of the effective-mass factor (mod_potential_and_density), takes
+ * cfunig's place in H's kinetic term, H(i,j) += V(G_i - G_j) + (1/2)(G_i+k).(G_j+k)
+ * meffig(G_i - G_j); O keeps cfunig (olpistln.f90:69-82; the IORA overlap terms are not
+ * supported).  NULL restores the 'none' form.  The muffin-tin matrices are the caller's
+ * (exc_ctx_set_mt: exc_ctx_hmlint forms the 'none' ones only), and exc_ho_vjp returns
+ * EXC_ERR_UNSUPPORTED while meffig is set. */
+int exc_ctx_set_meffig(exc_ctx *ctx, const double *meffig);
+
 /* match.f90:13 — APW matching coefficients of one k-point.
  *   gpc(ngkmax), tpgpc(2, ngkmax), sfacgp(ngkmax, natmtot) complex,
  *   apwalm(ngkmax, apwordmax, lmmaxapw, natmtot) complex (output; rows > ngp untouched). */

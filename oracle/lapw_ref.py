@@ -85,8 +85,10 @@ def _apwi(cr, apwalm, ngp, ias):
     return np.transpose(a, (1, 2, 0)).reshape(naa, ngp, order="F")
 
 
-def hmlistln(cr, kp, H):
-    """Upper-triangle interstitial term H(i,j) += V(G_i-G_j) + 1/2 (G_i+k).(G_j+k) Theta(G_i-G_j), i <= j."""
+def hmlistln(cr, kp, H, meffig=None):
+    """Upper-triangle interstitial term H(i,j) += V(G_i-G_j) + 1/2 (G_i+k).(G_j+k) Theta(G_i-G_j), i <= j
+    (hmlistln.f90:74-84); with meffig (ValenceRelativity /= 'none', hmlistln.f90:61-72) the
+    kinetic term takes meffig(G_i-G_j) instead of Theta."""
     ngp = kp.ngp
     ivk = cr.ivg[:, kp.igpig[:ngp] - 1]
     iv = ivk[:, :, None] - ivk[:, None, :]
@@ -99,7 +101,8 @@ def hmlistln(cr, kp, H):
     ok = (ig > 0) & (ig <= cr.ngvec) & np.triu(np.ones((ngp, ngp), dtype=bool))
     t1 = 0.5 * (kp.vgpc[:, :ngp].T @ kp.vgpc[:, :ngp])
     igc = np.where(ok, ig - 1, 0)
-    zt = cr.veffig[igc] + t1 * cr.cfunig[igc]
+    kin = cr.cfunig if meffig is None else meffig
+    zt = cr.veffig[igc] + t1 * kin[igc]
     H[:ngp, :ngp] += np.where(ok, zt, 0.0)
     return ok, igc
 
@@ -147,7 +150,7 @@ def _mt_sum_per_atom(cr, left, right):
     return out
 
 
-def hamiltonsetup(cr, kp, apwalm, H, per_atom=False):
+def hamiltonsetup(cr, kp, apwalm, H, per_atom=False, meffig=None):
     """hamiltonsetup.f90:57-174: per atom zm = h_a apwi, H += apwi^H zm; APW-lo; lo-lo; interstitial.
     per_atom: the muffin-tin sum in the reference's per-atom zgemm order (_mt_sum_per_atom)."""
     ngp = kp.ngp
@@ -170,7 +173,7 @@ def hamiltonsetup(cr, kp, apwalm, H, per_atom=False):
             j3 = ngp + cr.idxlo[l3 * (l3 + 1) + l3, nl - 1, ias] - 1
             n = j3 - j1 + 1
             H[j1:j3 + 1, j1:j3 + 1] += cr.lolo[:n, :n, ias]
-    hmlistln(cr, kp, H)
+    hmlistln(cr, kp, H, meffig=meffig)
     return H
 
 
@@ -216,16 +219,17 @@ def copy_ul(M):
     return M
 
 
-def setup(cr, kp, apwalm=None, per_atom=False):
+def setup(cr, kp, apwalm=None, per_atom=False, meffig=None):
     """match (if apwalm is None) + hamiltonsetup + overlapsetup, as seceqn/seceqnfv call them.
     per_atom: the muffin-tin sums as one zgemm per atom, the reference's own order (the CPU
-    baseline); the default sums in l-blocks (the checker, smaller rounding error)."""
+    baseline); the default sums in l-blocks (the checker, smaller rounding error).  meffig:
+    H's interstitial kinetic factor with ValenceRelativity /= 'none' (hmlistln.f90:61-72)."""
     if apwalm is None:
         apwalm = match(cr, kp)
     nmat = kp.ngp + cr.nlotot
     H = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
     O = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
-    hamiltonsetup(cr, kp, apwalm, H, per_atom=per_atom)
+    hamiltonsetup(cr, kp, apwalm, H, per_atom=per_atom, meffig=meffig)
     overlapsetup(cr, kp, apwalm, O, H, per_atom=per_atom)
     return H, O, apwalm
 

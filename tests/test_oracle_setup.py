@@ -168,6 +168,28 @@ def test_free_electron_limit():
     np.testing.assert_allclose(O, np.eye(n), atol=0)
 
 
+def test_meffig_kinetic_term():
+    """ValenceRelativity /= 'none' (hmlistln.f90:61-72): meffig takes Theta's place in H's
+    kinetic term only.  meffig = delta_G0 with Theta kept gives the free-electron kinetic
+    diagonal whatever Theta is, and meffig = cfunig gives the 'none' matrix bit for bit."""
+    w = small("a")
+    cr, kp = w.crystal, w.kpoints[0]
+    import copy
+    c2 = copy.copy(cr)
+    c2.veffig = np.zeros_like(cr.veffig)
+    n = kp.ngp
+    m = np.zeros(cr.ngvec, dtype=np.complex128)
+    m[0] = 1.0
+    H = np.zeros((n, n), dtype=np.complex128)
+    R.hmlistln(c2, kp, H, meffig=m)
+    np.testing.assert_allclose(H, np.diag(0.5 * (kp.vgpc[:, :n] ** 2).sum(axis=0)), atol=1e-15)
+    H0 = np.zeros((n, n), dtype=np.complex128)
+    H1 = np.zeros((n, n), dtype=np.complex128)
+    R.hmlistln(cr, kp, H0)
+    R.hmlistln(cr, kp, H1, meffig=cr.cfunig[:cr.ngvec].copy())
+    assert np.array_equal(H0, H1)
+
+
 def test_step_function_real_space():
     """Theta(G) = (1/Omega) int_cell Theta(r) exp(-i G.r) dr, by a grid sum over the cell."""
     w = small("a")
