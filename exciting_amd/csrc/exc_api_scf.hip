@@ -326,11 +326,19 @@ int exc_ctx_gencfun(exc_ctx *c, const double *bvec, double *cfunig, double *cfun
         HIPFFT_SUCCESS)
       return EXC_ERR_HIP;
     if ((rc = exc_launch_scf_real(n, c->fz, c->cfunir, c->stream))) return rc;
+    if (c->cfdamp_t1 > 0.0 && (rc = exc_launch_cfdamp(c->d, bvec, n, c->cfdamp_t1, c->d.cfunig, c->stream)))
+      return rc;
   }
   if (cfunig) EXC_CHECK(hipMemcpyAsync(cfunig, c->d.cfunig, 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   if (cfunir) EXC_CHECK(hipMemcpyAsync(cfunir, c->cfunir, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_cfunir = true;
+  return EXC_OK;
+}
+
+int exc_ctx_set_cfdamp(exc_ctx *c, double cfdamp, double gmaxvr) {
+  if (!c || !(cfdamp >= 0.0) || !(gmaxvr > 0.0)) return EXC_ERR_ARG;
+  c->cfdamp_t1 = cfdamp / gmaxvr;
   return EXC_OK;
 }
 

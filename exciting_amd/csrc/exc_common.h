@@ -205,6 +205,7 @@ int exc_launch_hmlint(const exc_dev &d, int lmv, int nrmtmax, const int *nrmt, c
                       const double *veffmt, double *rad, const cplx *gnt, hipStream_t s);
 // Theta(G), Theta(r), V(G) (k_scf.hip)
 int exc_launch_cfunig(const exc_dev &d, const double *bvec, int ngrtot, cplx *cf, cplx *z, hipStream_t s);
+int exc_launch_cfdamp(const exc_dev &d, const double *bvec, int ngrtot, double t1, cplx *cf, hipStream_t s);
 // G+k sets (k_gvec.hip): kv holds (vpl, vpc) per k-point, 6 doubles each, in device memory
 int exc_launch_gengpvec(const exc_dev &d, int nk, const double *bvec, const double *kv, double gkmax, int ngkmax,
                         int *ngp, int *igpig, double *vgpl, double *vgpc, double *gpc, double *tpgpc, cplx *sfacgp,

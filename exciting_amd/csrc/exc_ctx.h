@@ -82,6 +82,7 @@ struct exc_ctx {
   size_t crt_parts_bytes;
   void *gk_buf;               // G+k sets: k-point coordinates, counts, and the host entries' staging
   size_t gk_buf_bytes;
+  double cfdamp_t1;           // gencfun's damping exponent cfdamp / gmaxvr (0: none)
   signed char *crt_res;
   size_t crt_res_bytes;
   mt_item *d_citems;

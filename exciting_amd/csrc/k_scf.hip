@@ -55,6 +55,19 @@ __global__ __launch_bounds__(kST) void scf_cfunig_kernel(exc_dev d, Bvec B, int 
   z[grid_index(d, d.ivg, g)] = v;
 }
 
+// gencfun.f90:108-114: Theta(G) *= exp(-(t1 |G|)^2), t1 = cfdamp / gmaxvr, after Theta(r) is
+// taken from the undamped Theta(G); |G| as scf_cfunig_kernel forms it
+__global__ __launch_bounds__(kST) void scf_cfdamp_kernel(exc_dev d, Bvec B, int ngrtot, double t1, cplx *cf) {
+  const int g = blockIdx.x * kST + threadIdx.x;
+  if (g >= ngrtot) return;
+  const int *iv = d.ivg + 3 * g;
+  double G[3];
+  for (int i = 0; i < 3; ++i) G[i] = B.b[i] * iv[0] + B.b[i + 3] * iv[1] + B.b[i + 6] * iv[2];
+  const double gc = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+  const double x = t1 * gc, t2 = exp(-(x * x));
+  cf[g] = cplx{cf[g].re * t2, cf[g].im * t2};
+}
+
 __global__ __launch_bounds__(kST) void scf_real_kernel(int n, const cplx *z, double *out) {
   const int i = blockIdx.x * kST + threadIdx.x;
   if (i < n) out[i] = z[i].re;
@@ -114,6 +127,14 @@ int exc_launch_cfunig(const exc_dev &d, const double *bvec, int ngrtot, cplx *cf
   Bvec B;
   for (int i = 0; i < 9; ++i) B.b[i] = bvec[i];
   hipLaunchKernelGGL(scf_cfunig_kernel, dim3(blocks(ngrtot)), dim3(kST), 0, s, d, B, ngrtot, cf, z);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_cfdamp(const exc_dev &d, const double *bvec, int ngrtot, double t1, cplx *cf, hipStream_t s) {
+  Bvec B;
+  for (int i = 0; i < 9; ++i) B.b[i] = bvec[i];
+  hipLaunchKernelGGL(scf_cfdamp_kernel, dim3(blocks(ngrtot)), dim3(kST), 0, s, d, B, ngrtot, t1, cf);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

@@ -52,6 +52,12 @@ module exc_lapw_c
       real(c_double), intent(in) :: oalo(*), ololo(*)
     end function
 
+    integer(c_int) function exc_ctx_set_cfdamp(ctx, cfdamp, gmaxvr) bind(c, name="exc_ctx_set_cfdamp")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cfdamp, gmaxvr
+    end function
+
     integer(c_int) function exc_ctx_gencfun(ctx, bvec, cfunig, cfunir) bind(c, name="exc_ctx_gencfun")
       import :: c_int, c_ptr, c_double, c_double_complex
       type(c_ptr), value :: ctx

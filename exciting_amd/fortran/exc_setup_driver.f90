@@ -10,7 +10,7 @@
 !   hmlint_gpu(lmaxvr, nrmt, spr, apwfr, lofr, veffmt, haa, loa, lolo, oalo, ololo)
 !                                                                 hmlint.f90:14 + olprad.f90:13
 !   set_radial_gpu(nrmt, spr, apwfr)                             match.f90:139-148 (zd by polynom)
-!   gencfun_gpu(bvec, cfunig, cfunir)                             gencfun.f90:13 (cfdamp = 0)
+!   gencfun_gpu(bvec, cfunig, cfunir [, cfdamp, gmaxvr])          gencfun.f90:13
 !   genrhomt_gpu(dm, rhomt)                                       genrhomt.f90:13
 !   genrhoir_gpu(ngp, igpig, nmatmax, nstfv, evecfv, occ, wkpt, epsocc, rhoir)
 !                                                                 genrhoir.f90:13
@@ -265,13 +265,21 @@ contains
     if (rc > 0) write(*, '("Warning(set_radial_gpu): singular matching matrix, info = ", I0)') rc
   end subroutine set_radial_gpu
 
-  ! gencfun.f90:13 (cfdamp = 0): Theta(G) (cfunig(ngrtot)) and Theta(r) (cfunir(ngrtot)) on the
-  ! GPU from the atoms of the context; bvec(3,3) as mod_lattice.  Theta(G) replaces the
-  ! context's cfunig; Theta(r) stays on the device for genveffig_gpu.
-  subroutine gencfun_gpu(bvec, cfunig, cfunir)
+  ! gencfun.f90:13: Theta(G) (cfunig(ngrtot)) and Theta(r) (cfunir(ngrtot)) on the GPU from
+  ! the atoms of the context; bvec(3,3) as mod_lattice.  With cfdamp > 0 (and gmaxvr, both as
+  ! input%groundstate) Theta(G) is damped by exp(-(cfdamp |G| / gmaxvr)^2) after Theta(r) is
+  ! taken (gencfun.f90:108-114); absent, no damping.  Theta(G) replaces the context's cfunig;
+  ! Theta(r) stays on the device for genveffig_gpu.
+  subroutine gencfun_gpu(bvec, cfunig, cfunir, cfdamp, gmaxvr)
     real(8), intent(in) :: bvec(3, 3)
     complex(8), intent(out) :: cfunig(*)
     real(8), intent(out) :: cfunir(*)
+    real(8), intent(in), optional :: cfdamp, gmaxvr
+    if (present(cfdamp) .and. present(gmaxvr)) then
+      call check(exc_ctx_set_cfdamp(ctx, cfdamp, gmaxvr), "gencfun (cfdamp)")
+    else
+      call check(exc_ctx_set_cfdamp(ctx, 0.d0, 1.d0), "gencfun (cfdamp)")
+    end if
     call check(exc_ctx_gencfun(ctx, bvec, cfunig, cfunir), "gencfun")
   end subroutine gencfun_gpu
 

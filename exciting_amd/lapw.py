@@ -106,6 +106,7 @@ SIGNATURES = {
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
     "exc_ctx_set_meffig": [_P, _P],
+    "exc_ctx_set_cfdamp": [_P, C.c_double, C.c_double],
     "exc_gengpvec": [_P, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P],
     "exc_gensfacgp": [_P, _I, _P, _I, _P],
     "exc_gengpvec_dev": [_P, _I, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P, _P],
@@ -366,6 +367,10 @@ class Context:
         wk = (C.c_double * nk)(*[float(x) for x in wkpts])
         return self._call("exc_genrhoir_batched_dev", self.h, nk, C.cast(batch.desc, _P), nst, C.cast(ldv, _P),
                           C.cast(ev, _P), C.cast(oc, _P), C.cast(wk, _P), float(epsocc), _ptr(rhoir))
+
+    def set_cfdamp(self, cfdamp, gmaxvr):
+        """exc_ctx_set_cfdamp: gencfun's damping of Theta(G) (0: none)."""
+        self._call("exc_ctx_set_cfdamp", self.h, float(cfdamp), float(gmaxvr))
 
     def gencfun(self):
         """exc_ctx_gencfun with the crystal's bvec: (cfunig(ngrtot), cfunir(ngrtot)) as host arrays."""

@@ -364,6 +364,10 @@ int exc_genrhoir(exc_ctx *ctx, int ngp, const int *igpig, int nst, int ldv, cons
  *   at the first ngvec G, which replaces the context's veffig; optional host output
  *   veffig(ngvec) complex.  Needs exc_ctx_gencfun first. */
 int exc_ctx_gencfun(exc_ctx *ctx, const double *bvec, double *cfunig, double *cfunir);
+/* input%groundstate%cfdamp and gmaxvr for exc_ctx_gencfun (gencfun.f90:108-114): with
+ * cfdamp > 0, Theta(G) is damped by exp(-(cfdamp |G| / gmaxvr)^2) after Theta(r) is taken
+ * from the undamped values, as the reference does.  0 (the default): no damping. */
+int exc_ctx_set_cfdamp(exc_ctx *ctx, double cfdamp, double gmaxvr);
 int exc_ctx_genveffig(exc_ctx *ctx, const double *veffir, double *veffig);
 
 /* G+k sets of k-points on the GPU (gengpvec.f90:13-94 and gensfacgp.f90:13-58; once per

@@ -450,10 +450,11 @@ def _fft3(z, n, sgn):
     return out.ravel(order="F")
 
 
-def gencfun(cr, fft=_fft3):
-    """gencfun.f90:52-96 (cfdamp = 0): Theta(G) = delta_G0 - sum_a e^(-i G.r_a) Theta_s(G) with
+def gencfun(cr, fft=_fft3, cfdamp=0.0, gmaxvr=None):
+    """gencfun.f90:52-114: Theta(G) = delta_G0 - sum_a e^(-i G.r_a) Theta_s(G) with
     Theta_s(G) = (4 pi/Omega)(sin GR - GR cos GR)/G^3, (4 pi/Omega) R^3/3 at G = 0, for every G
-    of the grid, and Theta(r) = the backward transform (real part)."""
+    of the grid, and Theta(r) = the backward transform (real part); with cfdamp > 0 Theta(G) is
+    then multiplied by exp(-(cfdamp/gmaxvr |G|)^2) (Theta(r) stays undamped)."""
     t1 = 4.0 * np.pi / cr.omega
     cf = np.zeros(cr.ngrtot, dtype=np.complex128)
     cf[0] = 1.0
@@ -467,7 +468,11 @@ def gencfun(cr, fft=_fft3):
         cf -= (np.cos(ph) + 1j * np.sin(ph)) * ff
     z = np.zeros(cr.ngrtot, dtype=np.complex128)
     z[igfft(cr)] = cf
-    return cf, fft(z, fft_grid(cr), 1).real
+    cir = fft(z, fft_grid(cr), 1).real
+    if cfdamp > 0.0:
+        t1 = cfdamp / gmaxvr
+        cf = cf * np.exp(-(t1 * cr.gc) ** 2)
+    return cf, cir
 
 
 def genveffig(cr, veffir, cfunir, fft=_fft3):

@@ -42,6 +42,36 @@ def test_gencfun_genveffig_against_oracle(which):
     assert max(e) <= 1e-12
 
 
+def test_gencfun_damped_against_oracle():
+    """exc_ctx_set_cfdamp: the damped Theta(G) within 1e-12 of the oracle's, Theta(r) the
+    undamped one, the setup reads the damped table; cfdamp 0 restores the undamped values."""
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 1)
+    cr = w.crystal
+    ctx = lapw.Context(cr)
+    ctx.set_cfdamp(0.6, cr.gmaxvr)
+    cf, cir = ctx.gencfun()
+    cf_ref, cir_ref = R.gencfun(cr, cfdamp=0.6, gmaxvr=cr.gmaxvr)
+    e = [np.abs(cf - cf_ref).max(), np.abs(cir - cir_ref).max()]
+    print(f"damped: Theta(G) {e[0]:.2e}, Theta(r) {e[1]:.2e}", flush=True)
+    assert max(e) <= 1e-12
+    cr2 = dataclasses.replace(cr, cfunig=cf_ref)
+    kp = w.kpoints[0]
+    H_ref, O_ref, A = R.setup(cr2, kp)
+    H, O = ctx.hamiltonandoverlapsetup(kp, A)
+    eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
+    eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
+    print(f"setup on the damped Theta: H {eh:.2e}, O {eo:.2e}", flush=True)
+    assert eh <= 1e-12 and eo <= 1e-12
+    ctx.set_cfdamp(0.0, cr.gmaxvr)
+    cf0, _ = ctx.gencfun()
+    assert np.abs(cf0 - R.gencfun(cr)[0]).max() <= 1e-12
+    with pytest.raises(lapw.ExcError):
+        ctx.set_cfdamp(-1.0, cr.gmaxvr)
+    ctx.close()
+
+
 def test_setup_reads_the_new_tables():
     from exciting_amd import lapw
     from oracle import lapw_ref as R

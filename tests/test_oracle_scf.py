@@ -64,6 +64,29 @@ def test_gencfun_properties():
     assert np.abs(back - cf)[paired].max() <= 1e-13
 
 
+def test_gencfun_damping():
+    """cfdamp > 0 (gencfun.f90:108-114): Theta(G) is multiplied by a Gaussian in |G| whose
+    width is gmaxvr / cfdamp, after Theta(r) is taken: Theta(r) and Theta(0) unchanged, and
+    log(damped / undamped) against |G|^2 from the integer vectors (bvec ivg, not the crystal's
+    gc) is a line through the origin of slope -(cfdamp / gmaxvr)^2 — the ratio at |G| = gmaxvr
+    is exp(-cfdamp^2)."""
+    cr = small("d").crystal
+    cf0, cir0 = R.gencfun(cr)
+    cfdamp = 0.7
+    cf, cir = R.gencfun(cr, cfdamp=cfdamp, gmaxvr=cr.gmaxvr)
+    assert np.array_equal(cir, cir0) and cf[0] == cf0[0]
+    g2 = np.sum((cr.bvec @ cr.ivg.astype(float)) ** 2, axis=0)
+    ok = np.abs(cf0) > 1e-8
+    lr = np.log(np.abs(cf[ok] / cf0[ok]))
+    slope = np.sum(lr * g2[ok]) / np.sum(g2[ok] ** 2)
+    assert abs(slope + (cfdamp / cr.gmaxvr) ** 2) <= 1e-12
+    assert np.abs(lr - slope * g2[ok]).max() <= 1e-10
+    assert np.abs(np.angle(cf[ok] / cf0[ok])).max() <= 1e-14       # a real positive factor
+    # the ratio where |G| is nearest gmaxvr
+    i = np.argmin(np.abs(np.sqrt(g2) - cr.gmaxvr) + ~ok)
+    assert abs(abs(cf[i] / cf0[i]) - np.exp(-(cfdamp * np.sqrt(g2[i]) / cr.gmaxvr) ** 2)) <= 1e-13
+
+
 def test_genveffig_plane_wave():
     cr = small("d").crystal
     n = R.fft_grid(cr)
