@@ -135,6 +135,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   else hipDeviceSynchronize();
   for (void *p : c->owned) hipFree(p);
   if (c->yt) hipFree(c->yt);
+  if (c->apwo) hipFree(c->apwo);
   for (int i = 0; i <= exc_ctx::kEigStreams; ++i) {
     exc_ctx::EigWorker &w = c->ew[i];
     if (w.blas) rocblas_destroy_handle(w.blas);
@@ -492,6 +493,53 @@ int exc_ctx_set_meffig(exc_ctx *c, const double *meffig) {
   return EXC_OK;
 }
 
+int exc_ctx_set_iora(exc_ctx *c, const double *h1aa, const double *h1loa, const double *h1lolo,
+                     const double *m2effig) {
+  if (!c) return EXC_ERR_ARG;
+  if (!c->have_atoms) return EXC_ERR_STATE;
+  const int na = c->dims.natmtot, ao = c->dims.apwordmax, la = c->dims.lmaxapw, nl = c->d.lmaxmat + 1;
+  const int nlo = c->dims.nlomax;
+  if (nlo > 0 && (!h1loa != !h1lolo)) return EXC_ERR_ARG;
+  int rc = use_device(c);
+  if (rc) return rc;
+  int iora = 0;
+  if (h1aa) {
+    // I + h1aa(:,:,l,a) = L L^T per atom and l <= lmaxmat (the l-range of overlapsetup.f90:60-68)
+    std::vector<double> lf((size_t)na * nl * 3);
+    for (int ias = 0; ias < na; ++ias)
+      for (int l = 0; l < nl; ++l) {
+        const double *h = h1aa + (size_t)ao * ao * (l + (size_t)(la + 1) * ias);   // h(io1 + ao io2)
+        const double m00 = 1.0 + h[0], m11 = 1.0 + h[3], m10 = h[1], m01 = h[2];
+        if (!std::isfinite(m00) || !std::isfinite(m11) || !std::isfinite(m10) || !std::isfinite(m01))
+          return EXC_ERR_ARG;
+        // olprad makes h1aa symmetric (the same radial integrand for (io1, io2) and (io2, io1))
+        if (!(fabs(m10 - m01) <= 1e-12 * std::max(1.0, std::max(fabs(m10), fabs(m01))))) return EXC_ERR_ARG;
+        if (!(m00 > 0.0)) return EXC_ERR_UNSUPPORTED;
+        const double l00 = sqrt(m00), l10 = 0.5 * (m10 + m01) / l00, dd = m11 - l10 * l10;
+        if (!(dd > 0.0)) return EXC_ERR_UNSUPPORTED;   // I + h1aa not positive definite
+        double *q = lf.data() + ((size_t)ias * nl + l) * 3;
+        q[0] = l00;
+        q[1] = l10;
+        q[2] = sqrt(dd);
+      }
+    if ((rc = dev_upload(c, &c->d.olpl, lf.data(), lf.size()))) return rc;
+    iora |= 1;
+  }
+  if (nlo > 0 && h1loa) {
+    if ((rc = dev_upload(c, &c->d.h1loa, h1loa, (size_t)ao * nlo * na))) return rc;
+    if ((rc = dev_upload(c, &c->d.h1lolo, h1lolo, (size_t)nlo * nlo * na))) return rc;
+    iora |= 2;
+  }
+  if (m2effig) {
+    if ((rc = dev_upload(c, &c->d.m2effig, m2effig, (size_t)c->dims.ngvec))) return rc;
+    iora |= 4;
+  }
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  c->d.iora = iora;
+  c->batch_sig.clear();
+  return EXC_OK;
+}
+
 int exc_match_dev(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *tpgpc, const double *sfacgp,
                   double *apwalm) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
@@ -576,8 +624,21 @@ int setup_dev_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const i
     if (rc) return rc;
     {
       StageTimer t(c, untimed ? -1 : EXC_STAGE_MT);
-      rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o, ldh, mv,
-                         c->stream);
+      if ((c->d.iora & 1) && o) {
+        // 'iora*': O's tiles take the operand A L, H's keep A (two launches)
+        if (ws) return EXC_ERR_UNSUPPORTED;
+        if ((rc = grow((void **)&c->apwo, &c->apwo_bytes, sizeof(cplx) * (size_t)ngp * c->d.naa * c->d.natm)))
+          return rc;
+        rc = exc_launch_iora_apw(c->d, ngp, (const cplx *)apwalm, ngkmax, c->apwo, ngp, c->stream);
+        if (!rc && h)
+          rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, yt, ngp, igpig, vgpc, (cplx *)h, nullptr, ldh,
+                             mv, c->stream);
+        if (!rc)
+          rc = exc_launch_mt(c->d, ngp, c->apwo, ngp, yt, ngp, igpig, vgpc, nullptr, (cplx *)o, ldh, mv, c->stream);
+      } else {
+        rc = exc_launch_mt(c->d, ngp, (const cplx *)apwalm, ngkmax, yt, ngp, igpig, vgpc, (cplx *)h, (cplx *)o, ldh,
+                           mv, c->stream);
+      }
     }
     if (rc) return rc;
   }
@@ -641,6 +702,7 @@ int exc_hamiltonandoverlapsetup_noalloc_dev(exc_ctx *c, int ngp, int ngkmax, con
                                             const double *vgpc, int ldh, double *h, double *o) {
   if (!h || !o) return EXC_ERR_ARG;
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
+  if (c->d.iora & 1) return EXC_ERR_UNSUPPORTED;   // the 'iora*' APW-APW overlap needs a second operand buffer
   const size_t need = sizeof(cplx) * (size_t)std::max(ngp, 0) * c->d.naa * c->d.natm;
   if (need > kScratchBytes) {
     fprintf(stderr,
@@ -699,6 +761,9 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const bool olp = flags & EXC_BATCH_OLP_LEGENDRE;
   if (olp && !(flags & EXC_BATCH_MATCH)) return EXC_ERR_ARG;
   if (split && c->d.naa > kSplitKB) return EXC_ERR_UNSUPPORTED;
+  // 'iora*' overlap terms: the APW-APW factor I + h1aa in the addition-theorem form only; the
+  // split products' O tiles take none of the terms
+  if (((c->d.iora & 1) || ((split || crt) && c->d.iora)) && !olp) return EXC_ERR_UNSUPPORTED;
   if (nk == 0) return EXC_OK;
   int rc = use_device(c);
   if (rc) return rc;
@@ -1018,7 +1083,8 @@ int exc_ho_vjp_dev(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
                    double *dedtau, double *dedv) {
   if (!ready(c)) return c ? EXC_ERR_STATE : EXC_ERR_ARG;
   if (!c->d.ffacg) return EXC_ERR_STATE;
-  if (c->d.zora) return EXC_ERR_UNSUPPORTED;   // the adjoint differentiates the Theta-form kinetic term only
+  // the adjoint differentiates the Theta-form kinetic term and the 'none' overlap only
+  if (c->d.zora || c->d.iora) return EXC_ERR_UNSUPPORTED;
   const int nmat = ngp + c->d.nlotot;
   if (ngp < 0 || ngkmax < ngp || nst < 0 || ldc < nmat || !dedtau || !dedv) return EXC_ERR_ARG;
   if (ngp > 0 && nst > 0 && (!apwalm || !igpig || !vgpc || !sfacgp || !evec || !wh || !wo)) return EXC_ERR_ARG;

@@ -64,6 +64,13 @@ struct exc_dev {
   cplx *veffig;        // [ngvec]
   cplx *meffig;        // [ngvec]: the kinetic term's factor with ValenceRelativity /= 'none'
   int zora;            // 1: H's interstitial kinetic term takes meffig instead of cfunig
+  // ValenceRelativity 'iora*' overlap terms (exc_ctx_set_iora); iora bits: 1 the APW-APW
+  // factor I + h1aa, 2 h1loa / h1lolo, 4 the interstitial term with m2effig
+  int iora;
+  double *olpl;        // [natm][lmaxmat+1][3]: lower Cholesky factor (L00, L10, L11) of I + h1aa(:,:,l,a)
+  double *h1loa;       // [natm][nlomax][apwordmax], added to oalo
+  double *h1lolo;      // [natm][nlomax][nlomax], added to ololo
+  cplx *m2effig;       // [ngvec]
   double *ffacg;       // [nspecies][ngvec]
   int *kperm;          // [natm*naa]: summation order of the MT product (column of apwalm)
   int *fperm;          // [naa]: summation order inside one atom (Y stage)
@@ -220,3 +227,4 @@ int exc_launch_rhoir_accum(int N, int nb, const cplx *z, const double *occ, int 
                            double omega, double *rho, hipStream_t s);
 int exc_launch_lo(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *h, cplx *o, int ldh,
                   hipStream_t s);
+int exc_launch_iora_apw(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *out, int ldo, hipStream_t s);

@@ -29,6 +29,8 @@ struct exc_ctx {
   // grow-only workspaces
   cplx *yt;
   size_t yt_bytes;
+  cplx *apwo;                 // per-k 'iora*' overlap operand A L (exc_launch_iora_apw)
+  size_t apwo_bytes;
   // staging for the host-pointer entry points: device buffers and a pinned host buffer
   void *stage;
   size_t stage_bytes;

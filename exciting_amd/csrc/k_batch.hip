@@ -20,20 +20,21 @@ template <int L>
 __global__ __launch_bounds__(kMatchThreads) void bmatch_kernel(const kdesc *__restrict__ ks, int apwordmax,
                                                                 const int *__restrict__ spidx,
                                                                 const double *__restrict__ rmt, double fpso,
-                                                                const double *__restrict__ dinv, int nl) {
+                                                                const double *__restrict__ dinv, int nl,
+                                                                const double *__restrict__ olpl) {
   const kdesc k = ks[blockIdx.z];
   const int ig = blockIdx.x * kMatchThreads + threadIdx.x;
   if (ig >= k.ngp) return;
   match_dev::match_body<L>(ig, blockIdx.y, apwordmax, spidx, rmt, fpso, dinv, k.gpc, k.tpgpc, k.sfacgp, k.ldg,
                            k.apwalm, k.ldg, k.rf, nl, k.ngp,
-                           k.pa2 ? k.pa2 + (size_t)blockIdx.y * k.cnp + ig : nullptr);
+                           k.pa2 ? k.pa2 + (size_t)blockIdx.y * k.cnp + ig : nullptr, olpl);
 }
 
 template <int L>
 int launch_bmatch_l(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
   dim3 grid((maxngp + kMatchThreads - 1) / kMatchThreads, d.natm, nk);
   hipLaunchKernelGGL(bmatch_kernel<L>, grid, dim3(kMatchThreads), 0, s, ks, d.apwordmax, d.spidx, d.rmt,
-                     4.0 * M_PI / sqrt(d.omega), d.dinv, d.lmaxmat + 1);
+                     4.0 * M_PI / sqrt(d.omega), d.dinv, d.lmaxmat + 1, (d.iora & 1) ? d.olpl : nullptr);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -99,21 +100,23 @@ __global__ __launch_bounds__(lo_dev::kRows * lo_dev::kSlots) void blo_apw_kernel
                                                                                   int nlotot,
                                                                                   const lo_desc *__restrict__ lod,
                                                                                   const cplx *__restrict__ loa,
-                                                                                  const double *__restrict__ oalo) {
+                                                                                  const double *__restrict__ oalo,
+                                                                                  const double *__restrict__ h1loa) {
   __shared__ lo_dev::ApwLoShared sh;
   const kdesc k = ks[blockIdx.z];
   if ((int)blockIdx.x * lo_dev::kRows >= k.ngp) return;
   lo_dev::lo_apw_body(sh, blockIdx.x, blockIdx.y, k.ngp, naa, maxnlo, nlomax, apwordmax, nlotot, lod, k.apwalm, k.ldg,
-                      loa, oalo, k.h, k.o, k.ldh);
+                      loa, oalo, h1loa, k.h, k.o, k.ldh);
 }
 
 __global__ __launch_bounds__(lo_dev::kLoThreads) void blo_lo_kernel(const kdesc *__restrict__ ks, int nlotot, int maxnlo,
                                                                      int nlomax, const lo_desc *__restrict__ lod,
                                                                      const cplx *__restrict__ lolo,
-                                                                     const double *__restrict__ ololo) {
+                                                                     const double *__restrict__ ololo,
+                                                                     const double *__restrict__ h1lolo) {
   const kdesc k = ks[blockIdx.z];
   lo_dev::lo_lo_body(blockIdx.x * lo_dev::kLoThreads + threadIdx.x, blockIdx.y, k.ngp, nlotot, maxnlo, nlomax, lod,
-                     lolo, ololo, k.h, k.o, k.ldh);
+                     lolo, ololo, h1lolo, k.h, k.o, k.ldh);
 }
 
 }  // namespace
@@ -171,12 +174,12 @@ int exc_launch_blo(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStr
   if (maxngp > 0) {
     dim3 grid((maxngp + lo_dev::kRows - 1) / lo_dev::kRows, d.natm, nk);
     hipLaunchKernelGGL(blo_apw_kernel, grid, dim3(lo_dev::kRows * lo_dev::kSlots), 0, s, ks, d.naa, d.maxnlo, d.nlomax,
-                       d.apwordmax, d.nlotot, d.lod, d.loa, d.oalo);
+                       d.apwordmax, d.nlotot, d.lod, d.loa, d.oalo, (d.iora & 2) ? d.h1loa : nullptr);
     EXC_CHECK(hipGetLastError());
   }
   dim3 grid2((d.nlotot + lo_dev::kLoThreads - 1) / lo_dev::kLoThreads, d.nlotot, nk);
   hipLaunchKernelGGL(blo_lo_kernel, grid2, dim3(lo_dev::kLoThreads), 0, s, ks, d.nlotot, d.maxnlo, d.nlomax, d.lod,
-                     d.lolo, d.ololo);
+                     d.lolo, d.ololo, (d.iora & 2) ? d.h1lolo : nullptr);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }

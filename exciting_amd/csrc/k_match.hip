@@ -27,7 +27,38 @@ int launch_match_l(const exc_dev &d, int ngp, const double *gpc, const double *t
   return EXC_OK;
 }
 
+// The 'iora*' overlap operand of the per-k MFMA path: B = A L per atom and l, where
+// I + h1aa(:,:,l,a) = L L^T (olpl), so that sum_f conj(B(i,f)) B(j,f) is the reference's
+// sum_f conj(A(i,f)) ((I + h1aa)^T A^T)(f,j) (overlapsetup.f90:81-94 with h1aa symmetric).
+// One thread per (row, lm, atom): B(:, io = 0) = L00 A0 + L10 A1, B(:, 1) = L11 A1.
+__global__ __launch_bounds__(256) void iora_apw_kernel(int ngp, const cplx *__restrict__ A, int ldg, int lmmax, int nl,
+                                                       const double *__restrict__ olpl, cplx *__restrict__ B,
+                                                       int ldb) {
+  const int g = blockIdx.x * 256 + threadIdx.x, lm = blockIdx.y, ias = blockIdx.z;
+  if (g >= ngp) return;
+  int l = (int)sqrt((double)lm);
+  while (l * l > lm) --l;
+  while ((l + 1) * (l + 1) <= lm) ++l;
+  const size_t c0 = (size_t)2 * (lm + (size_t)lmmax * ias);
+  const cplx a0 = A[g + c0 * ldg], a1 = A[g + (c0 + 1) * ldg];
+  const double *lf = olpl + ((size_t)ias * nl + l) * 3;
+  const double l00 = lf[0], l10 = lf[1], l11 = lf[2];
+  B[g + c0 * ldb] = cplx{l00 * a0.re + l10 * a1.re, l00 * a0.im + l10 * a1.im};
+  B[g + (c0 + 1) * ldb] = cplx{l11 * a1.re, l11 * a1.im};
+}
+
 }  // namespace
+
+int exc_launch_iora_apw(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *out, int ldo, hipStream_t s) {
+  if (ngp <= 0) return EXC_OK;
+  if (d.apwordmax != 2 || !(d.iora & 1) || !d.olpl) return EXC_ERR_STATE;
+  const int nl = d.lmaxmat + 1;
+  hipLaunchKernelGGL(iora_apw_kernel, dim3((ngp + 255) / 256, nl * nl, d.natm), dim3(256), 0, s, ngp, apwalm, ldg,
+                     nl * nl, nl, d.olpl, out, ldo);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
 int exc_launch_match(const exc_dev &d, int ngp, const double *gpc, const double *tpgpc,
                      const cplx *sfacgp, int ldsf, cplx *apwalm, int ldg, hipStream_t s) {
   if (ngp <= 0) return EXC_OK;

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64 * 64 / (4 * CJ)) __attribute__((amdgpu_waves_per
       double re = are[i][j], im = aim[i][j];
       const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
       if (ig) {
-        const cplx th = gt.cfunig[ig - 1];
+        const cplx th = olp_theta(gt, ig, k.vgpc + 3 * r, k.vgpc + 3 * c);
         re += th.re;
         im += th.im;
       }
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(512) void olp_rows_kernel(const kdesc *__restrict__
     if (c < ngp && row <= c) {
       const int ig = lookup_g(gt, gr0 - gc[q][0], gr1 - gc[q][1], gr2 - gc[q][2]);
       if (ig) {
-        const cplx th = gt.cfunig[ig - 1];
+        const cplx th = olp_theta(gt, ig, k.vgpc + 3 * row, k.vgpc + 3 * c);
         re += th.re;
         im += th.im;
       }
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(512, 4) void olp_pipe_kernel(const kdesc *__restric
     if (c < ngp && row <= c) {
       const int ig = lookup_g(gt, gr0 - gc[q][0], gr1 - gc[q][1], gr2 - gc[q][2]);
       if (ig) {
-        const cplx th = gt.cfunig[ig - 1];
+        const cplx th = olp_theta(gt, ig, k.vgpc + 3 * row, k.vgpc + 3 * c);
         re += th.re;
         im += th.im;
       }

@@ -30,8 +30,8 @@ struct ApwLoShared {
 __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, int ngp, int naa, int maxnlo, int nlomax,
                                             int apwordmax, int nlotot, const lo_desc *__restrict__ lod,
                                             const cplx *__restrict__ apwalm, int ldg, const cplx *__restrict__ loa,
-                                            const double *__restrict__ oalo, cplx *__restrict__ h,
-                                            cplx *__restrict__ o, int ldh) {
+                                            const double *__restrict__ oalo, const double *__restrict__ h1loa,
+                                            cplx *__restrict__ h, cplx *__restrict__ o, int ldh) {
   const int tid = threadIdx.x;
   if (tid == 0) {
     int first = -1, cnt = 0;
@@ -74,7 +74,9 @@ __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, in
       double orr = 0.0, oi = 0.0;
       for (int io = 0; io < apwordmax; ++io) {
         const cplx a = A[i + (size_t)(io + apwordmax * dq.lm) * ldg];
-        const double w = oalo[io + apwordmax * (dq.ilo + nlomax * ias)];
+        // oalo + h1loa ('iora*', overlapsetup.f90:126; h1loa is zero otherwise)
+        const size_t wi = io + apwordmax * (dq.ilo + nlomax * ias);
+        const double w = h1loa ? oalo[wi] + h1loa[wi] : oalo[wi];
         orr = fma(a.re, w, orr);
         oi = fma(-a.im, w, oi);
       }
@@ -97,8 +99,8 @@ __device__ __forceinline__ void lo_apw_body(ApwLoShared &sh, int bx, int ias, in
 
 __device__ __forceinline__ void lo_lo_body(int r, int c, int ngp, int nlotot, int maxnlo, int nlomax,
                                            const lo_desc *__restrict__ lod, const cplx *__restrict__ lolo,
-                                           const double *__restrict__ ololo, cplx *__restrict__ h,
-                                           cplx *__restrict__ o, int ldh) {
+                                           const double *__restrict__ ololo, const double *__restrict__ h1lolo,
+                                           cplx *__restrict__ h, cplx *__restrict__ o, int ldh) {
   if (r >= nlotot) return;
   const lo_desc dr = lod[r], dc = lod[c];
   cplx hv = {0.0, 0.0}, ov = {0.0, 0.0};
@@ -111,8 +113,10 @@ __device__ __forceinline__ void lo_lo_body(int r, int c, int ngp, int nlotot, in
       hv = cplx{t.re, -t.im};
     }
     if (dr.l == dc.l && dr.m == dc.m) {
-      const double *oo = ololo + (size_t)nlomax * nlomax * dr.ias;
-      ov.re = (r <= c) ? oo[dr.ilo + nlomax * dc.ilo] : oo[dc.ilo + nlomax * dr.ilo];
+      // ololo + h1lolo ('iora*', overlapsetup.f90:144)
+      const size_t base = (size_t)nlomax * nlomax * dr.ias;
+      const size_t oi = base + ((r <= c) ? dr.ilo + nlomax * dc.ilo : dc.ilo + nlomax * dr.ilo);
+      ov.re = h1lolo ? ololo[oi] + h1lolo[oi] : ololo[oi];
     }
   }
   const size_t idx = (size_t)(ngp + r) + (size_t)(ngp + c) * ldh;

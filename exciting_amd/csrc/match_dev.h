@@ -89,7 +89,8 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
                                            const double *__restrict__ dinv, const double *__restrict__ gpc,
                                            const double *__restrict__ tpgpc, const cplx *__restrict__ sfacgp,
                                            int ldsf, cplx *__restrict__ apwalm, int ldg, double *__restrict__ rf = nullptr,
-                                           int nl = 0, int ngp = 0, double *__restrict__ s2out = nullptr) {
+                                           int nl = 0, int ngp = 0, double *__restrict__ s2out = nullptr,
+                                           const double *__restrict__ olpl = nullptr) {
 #pragma clang fp contract(off)   // see sbessel_seq
   constexpr int lmmax = (L + 1) * (L + 1);
   const double g = gpc[ig];
@@ -164,10 +165,19 @@ __device__ __forceinline__ void match_body(int ig, int ias, int apwordmax, const
       if (rf && m == 0 && l < nl) {
         // radial factors for the overlap by the addition theorem, weighted by
         // (4 pi / sqrt(Omega)) sqrt((2l+1) / 4 pi): rf[(ias*ngp + ig)*2nl + 2l + io]
+        // With the 'iora*' overlap factor I + h1aa = L L^T (olpl, lower L per atom and l) the
+        // pair (r0, r1) is replaced by (r0, r1) L, so that the sum over io of the two rows'
+        // products is r M r'^T (overlapsetup.f90:81-94)
         const double wl = fpso * sqrt((double)(2 * l + 1) * 0.079577471545947667884);   // 1/(4 pi)
         double *q = rf + ((size_t)ias * ngp + ig) * 2 * nl + 2 * l;
-        q[0] = wl * r0;
-        q[1] = wl * r1;
+        double s0 = r0, s1 = r1;
+        if (olpl) {
+          const double *lf = olpl + ((size_t)ias * nl + l) * 3;
+          s0 = lf[0] * r0 + lf[1] * r1;
+          s1 = lf[2] * r1;
+        }
+        q[0] = wl * s0;
+        q[1] = wl * s1;
       }
       const int lmp = l * (l + 1) + m;
       out[colstride * (0 + 2 * lmp)] = {cp.re * r0, cp.im * r0};

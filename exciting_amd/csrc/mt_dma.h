@@ -331,8 +331,8 @@ struct MtTileDMA {
           double re = cr[i][j][v], im = ci[i][j][v];
           const int ig = lookup_g(gt, tb.gr[rl][0] - tb.gc[cl][0], tb.gr[rl][1] - tb.gc[cl][1], tb.gr[rl][2] - tb.gc[cl][2]);
           if (ig) {
-            const cplx th = gt.cfunig[ig - 1];
             if (which) {
+              const cplx th = olp_theta(gt, ig, tb.vr[rl], tb.vc[cl]);
               re += th.re;
               im += th.im;
             } else {

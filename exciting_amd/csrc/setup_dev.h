@@ -23,6 +23,8 @@ struct GTab {
   const cplx *veffig;  // [ngvec]
   const cplx *kinig;   // the factor of H's interstitial kinetic term: cfunig, or meffig [ngvec]
                        // with ValenceRelativity /= 'none' (hmlistln.f90:61-72 against :74-84)
+  const cplx *m2eff;   // 'iora*': O's interstitial term a2h (G_i+k).(G_j+k) m2effig [ngvec], or null
+  double a2h;          // 1/2 a2 = 1/4 alpha^2 (olpistln.f90:46-47, 60)
   unsigned *miss;      // count of basis differences found outside the G set (exc_ctx_status)
   int ngvec;
   int lo0, lo1, lo2, n0, n1, n2;
@@ -35,6 +37,9 @@ inline GTab make_gtab(const exc_dev &d) {
   t.cfunig = d.cfunig;
   t.veffig = d.veffig;
   t.kinig = d.zora ? d.meffig : d.cfunig;
+  t.m2eff = (d.iora & 4) ? d.m2effig : nullptr;
+  const double alpha = 1.0 / 137.03599911, a2 = 0.5 * (alpha * alpha);
+  t.a2h = 0.5 * a2;
   t.miss = d.miss;
   t.ngvec = d.ngvec;
   t.lo0 = d.box_lo[0]; t.lo1 = d.box_lo[1]; t.lo2 = d.box_lo[2];
@@ -55,6 +60,17 @@ __device__ __forceinline__ int lookup_g(const GTab &t, int d0, int d1, int d2) {
   if (ig > 0 && ig <= t.ngvec) return ig;
   if (t.miss) atomicAdd(t.miss, 1u);
   return 0;
+}
+
+// O's interstitial value at G = G_i - G_j (index ig > 0): Theta(G) (olpistln.f90:69-82), and
+// with 'iora*' 1/2 a2 (G_i+k).(G_j+k) m2effig(G) + Theta(G) (olpistln.f90:53-62); vi, vj are
+// the two G+k vectors, read only in that case
+__device__ __forceinline__ cplx olp_theta(const GTab &t, int ig, const double *vi, const double *vj) {
+  const cplx th = t.cfunig[ig - 1];
+  if (!t.m2eff) return th;
+  const double t1 = t.a2h * (vi[0] * vj[0] + vi[1] * vj[1] + vi[2] * vj[2]);
+  const cplx m = t.m2eff[ig - 1];
+  return cplx{t1 * m.re + th.re, t1 * m.im + th.im};
 }
 
 // upper-triangular tile index t -> (ti, tj), ti <= tj, enumerated column by column

@@ -52,6 +52,19 @@ module exc_lapw_c
       real(c_double), intent(in) :: oalo(*), ololo(*)
     end function
 
+    integer(c_int) function exc_ctx_set_meffig(ctx, meffig) bind(c, name="exc_ctx_set_meffig")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: meffig(*)
+    end function
+
+    integer(c_int) function exc_ctx_set_iora(ctx, h1aa, h1loa, h1lolo, m2effig) bind(c, name="exc_ctx_set_iora")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: h1aa(*), h1loa(*), h1lolo(*)
+      complex(c_double_complex), intent(in) :: m2effig(*)
+    end function
+
     integer(c_int) function exc_ctx_set_cfdamp(ctx, cfdamp, gmaxvr) bind(c, name="exc_ctx_set_cfdamp")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

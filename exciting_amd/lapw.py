@@ -106,6 +106,7 @@ SIGNATURES = {
     "exc_gendmatmt": [_P, _I, _I, _P, _I, _I, _P, _P, _P, C.c_double, _P],
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
     "exc_ctx_set_meffig": [_P, _P],
+    "exc_ctx_set_iora": [_P, _P, _P, _P, _P],
     "exc_ctx_set_cfdamp": [_P, C.c_double, C.c_double],
     "exc_gengpvec": [_P, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P],
     "exc_gensfacgp": [_P, _I, _P, _I, _P],
@@ -389,6 +390,21 @@ class Context:
             return
         m = np.ascontiguousarray(meffig, dtype=np.complex128)
         self._call("exc_ctx_set_meffig", self.h, _ptr(m))
+
+    def set_iora(self, h1aa=None, h1loa=None, h1lolo=None, m2effig=None):
+        """exc_ctx_set_iora: the 'iora*' overlap terms (Fortran layouts: h1aa(apwordmax,
+        apwordmax, 0:lmaxapw, natmtot), h1loa(apwordmax, nlomax, natmtot), h1lolo(nlomax,
+        nlomax, natmtot), m2effig(ngvec) complex); None turns a term off."""
+        keep = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.asfortranarray(x, dtype=dt)
+            keep.append(a)
+            return _ptr(a)
+        self._call("exc_ctx_set_iora", self.h, arr(h1aa, np.float64), arr(h1loa, np.float64),
+                   arr(h1lolo, np.float64), arr(m2effig, np.complex128))
 
     def gengpvec(self, vpl, vpc, gkmax, ngkmax):
         """exc_gengpvec with the crystal's bvec: (ngp, igpig, vgpl, vgpc, gpc, tpgpc) as host

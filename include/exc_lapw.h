@@ -131,11 +131,29 @@ int exc_ctx_set_gvec(exc_ctx *ctx, const int *ivg, const int *ivgig, const doubl
 /* ValenceRelativity /= 'none' in the interstitial (hmlistln.f90:61-72): meffig(ngvec) complex,
  * the Fourier components of the effective-mass factor (mod_potential_and_density), takes
  * cfunig's place in H's kinetic term, H(i,j) += V(G_i - G_j) + (1/2)(G_i+k).(G_j+k)
- * meffig(G_i - G_j); O keeps cfunig (olpistln.f90:69-82; the IORA overlap terms are not
- * supported).  NULL restores the 'none' form.  The muffin-tin matrices are the caller's
- * (exc_ctx_set_mt: exc_ctx_hmlint forms the 'none' ones only), and exc_ho_vjp returns
- * EXC_ERR_UNSUPPORTED while meffig is set. */
+ * meffig(G_i - G_j); O keeps cfunig (olpistln.f90:69-82; the 'iora*' overlap terms are
+ * exc_ctx_set_iora's).  NULL restores the 'none' form.  The muffin-tin matrices are the
+ * caller's (exc_ctx_set_mt: exc_ctx_hmlint forms the 'none' ones only), and exc_ho_vjp
+ * returns EXC_ERR_UNSUPPORTED while meffig is set. */
 int exc_ctx_set_meffig(exc_ctx *ctx, const double *meffig);
+
+/* ValenceRelativity 'iora*' overlap terms (seceqnfv.f90:74 h1on; overlapsetup.f90:8):
+ *   h1aa(apwordmax, apwordmax, 0:lmaxapw, natmtot) real (olprad.f90:75): the APW-APW block
+ *     sum_a conj(A_a) (I + h1aa)^T A_a^T over l <= lmaxmat (overlapsetup.f90:81-94);
+ *   h1loa(apwordmax, nlomax, natmtot), h1lolo(nlomax, nlomax, natmtot) real: added to oalo
+ *     and ololo in the APW-lo and lo-lo blocks (overlapsetup.f90:126, 144);
+ *   m2effig(ngvec) complex (genmeffig.f90): O's interstitial element becomes
+ *     1/2 a2 (G_i+k).(G_j+k) m2effig(G_i - G_j) + cfunig(G_i - G_j), a2 = alpha^2 / 2
+ *     (olpistln.f90:53-62).
+ * Each may be NULL (that term off); all NULL restores the 'none' overlap.  h1aa must be
+ * symmetric in (io1, io2) (to 1e-12 relative; EXC_ERR_ARG) and I + h1aa positive definite
+ * (EXC_ERR_UNSUPPORTED otherwise): the library factors it as L L^T.  H is unchanged (its
+ * 'iora*' muffin-tin and kinetic parts are the caller's haa and exc_ctx_set_meffig).  With
+ * h1aa set, exc_setup_batched_dev needs EXC_BATCH_OLP_LEGENDRE and the no-allocation entry
+ * returns EXC_ERR_UNSUPPORTED; with any term set, the split flags need
+ * EXC_BATCH_OLP_LEGENDRE and exc_ho_vjp returns EXC_ERR_UNSUPPORTED. */
+int exc_ctx_set_iora(exc_ctx *ctx, const double *h1aa, const double *h1loa, const double *h1lolo,
+                     const double *m2effig);
 
 /* match.f90:13 — APW matching coefficients of one k-point.
  *   gpc(ngkmax), tpgpc(2, ngkmax), sfacgp(ngkmax, natmtot) complex,

@@ -5,9 +5,9 @@ Each function restates one reference routine, in numpy, on the arrays of
 
 * ``match``             — match.f90:83-187 (APW matching coefficients)
 * ``hamiltonsetup``     — src_eigensystem/hamiltonsetup.f90:57-174
-* ``overlapsetup``      — src_eigensystem/overlapsetup.f90:50-166
+* ``overlapsetup``      — src_eigensystem/overlapsetup.f90:50-166 (with the 'iora*' h1 terms)
 * ``hmlistln``          — src_eigensystem/hmlistln.f90:74-84 (ValenceRelativity='none')
-* ``olpistln``          — src_eigensystem/olpistln.f90:69-82 (h1 = 0)
+* ``olpistln``          — src_eigensystem/olpistln.f90:53-82 (with the 'iora*' m2effig term)
 * ``copy_ul``           — src_eigensystem/hamiltonoverlapocopy_UL.f90:6-30
 * ``eigen``             — the FP64 generalized eigensolve (ZHEGVX range 'I',
                           modfvsystem.f90:691-693) via LAPACK zhegv
@@ -107,7 +107,14 @@ def hmlistln(cr, kp, H, meffig=None):
     return ok, igc
 
 
-def olpistln(cr, kp, O):
+# a2 = alpha^2 / 2 of the 'iora*' overlap's interstitial term (olpistln.f90:46-47)
+IORA_A2 = 0.5 * (1.0 / 137.03599911) ** 2
+
+
+def olpistln(cr, kp, O, m2effig=None):
+    """Upper-triangle interstitial term O(i,j) += Theta(G_i-G_j), i <= j (olpistln.f90:69-82); with
+    m2effig ('iora*', olpistln.f90:53-62) the element is 1/2 a2 (G_i+k).(G_j+k) m2effig(G_i-G_j)
+    + Theta(G_i-G_j)."""
     ngp = kp.ngp
     ivk = cr.ivg[:, kp.igpig[:ngp] - 1]
     iv = ivk[:, :, None] - ivk[:, None, :]
@@ -117,7 +124,12 @@ def olpistln(cr, kp, O):
     ig = np.zeros((ngp, ngp), dtype=np.int64)
     ig[inb] = cr.ivgig[idx[0][inb], idx[1][inb], idx[2][inb]]
     ok = (ig > 0) & (ig <= cr.ngvec) & np.triu(np.ones((ngp, ngp), dtype=bool))
-    O[:ngp, :ngp] += np.where(ok, cr.cfunig[np.where(ok, ig - 1, 0)], 0.0)
+    igc = np.where(ok, ig - 1, 0)
+    zt = cr.cfunig[igc]
+    if m2effig is not None:
+        t1 = 0.5 * IORA_A2 * (kp.vgpc[:, :ngp].T @ kp.vgpc[:, :ngp])
+        zt = t1 * m2effig[igc] + zt
+    O[:ngp, :ngp] += np.where(ok, zt, 0.0)
 
 
 def _l_blocks(cr):
@@ -177,12 +189,30 @@ def hamiltonsetup(cr, kp, apwalm, H, per_atom=False, meffig=None):
     return H
 
 
-def overlapsetup(cr, kp, apwalm, O, H=None, per_atom=False):
-    """overlapsetup.f90:50-166: O += apwi^H (conj apwi2)^H; APW-lo; lo-lo; interstitial; copy_UL."""
+def _h1_right(cr, h1aa, apwi, ias):
+    """zm^H of overlapsetup.f90:81-94 for one atom: row f2 = (l, m, io2) of apwi plus
+    sum_io1 h1aa(io1, io2, l) times row (l, m, io1), for l <= lmaxmat."""
+    ao = cr.apwordmax
+    out = apwi.copy()
+    for l in range(cr.lmaxmat + 1):
+        for mp in range(2 * l + 1):
+            f0 = ao * (l * l + mp)
+            out[f0:f0 + ao] += h1aa[:ao, :ao, l, ias].T @ apwi[f0:f0 + ao]
+    return out
+
+
+def overlapsetup(cr, kp, apwalm, O, H=None, per_atom=False, iora=None):
+    """overlapsetup.f90:50-166: O += apwi^H zm^H; APW-lo; lo-lo; interstitial; copy_UL.
+    iora: ValenceRelativity 'iora*' terms, a dict with any of h1aa (apwordmax, apwordmax,
+    lmaxapw+1, natm), h1loa (apwordmax, nlomax, natm), h1lolo (nlomax, nlomax, natm) and
+    m2effig (ngvec); the h1 arrays are zero and m2effig unused otherwise."""
+    iora = iora or {}
+    h1aa, h1loa, h1lolo = iora.get("h1aa"), iora.get("h1loa"), iora.get("h1lolo")
     ngp = kp.ngp
     apwis = [_apwi(cr, apwalm, ngp, ias) for ias in range(cr.natm)]
-    # zm = conj(apwi)^T (apwi2, h1 = 0); O += apwi^H zm^H = conj(A) A^T
-    O[:ngp, :ngp] += (_mt_sum_per_atom if per_atom else _mt_sum)(cr, apwis, apwis)
+    # zm = conj(apwi)^T (I + h1aa); O += apwi^H zm^H = conj(A) (I + h1aa)^T A^T
+    rights = apwis if h1aa is None else [_h1_right(cr, h1aa, apwis[ias], ias) for ias in range(cr.natm)]
+    O[:ngp, :ngp] += (_mt_sum_per_atom if per_atom else _mt_sum)(cr, apwis, rights)
     for ias in range(cr.natm):
         is_ = cr.spidx[ias]
         if cr.nlorb[is_] != 0:
@@ -192,7 +222,8 @@ def overlapsetup(cr, kp, apwalm, O, H=None, per_atom=False):
                 j1 = ngp + cr.idxlo[lm1, ilo, ias] - 1
                 j2 = ngp + cr.idxlo[lm2, ilo, ias] - 1
                 for io in range(cr.apwordmax):
-                    O[:ngp, j1:j2 + 1] += np.conj(apwalm[:ngp, io, lm1:lm2 + 1, ias] * cr.oalo[io, ilo, ias])
+                    w = cr.oalo[io, ilo, ias] + (0.0 if h1loa is None else h1loa[io, ilo, ias])
+                    O[:ngp, j1:j2 + 1] += np.conj(apwalm[:ngp, io, lm1:lm2 + 1, ias] * w)
                 for j in range(j1, j2 + 1):
                     O[j, :ngp] = np.conj(O[:ngp, j])
             for ilo1 in range(cr.nlorb[is_]):
@@ -202,9 +233,10 @@ def overlapsetup(cr, kp, apwalm, O, H=None, per_atom=False):
                         lm1 = l * l
                         j1 = ngp + cr.idxlo[lm1, ilo1, ias] - 1
                         j2 = ngp + cr.idxlo[lm1, ilo2, ias] - 1
+                        w = cr.ololo[ilo1, ilo2, ias] + (0.0 if h1lolo is None else h1lolo[ilo1, ilo2, ias])
                         for mm in range(2 * l + 1):
-                            O[j1 + mm, j2 + mm] += cr.ololo[ilo1, ilo2, ias]
-    olpistln(cr, kp, O)
+                            O[j1 + mm, j2 + mm] += w
+    olpistln(cr, kp, O, m2effig=iora.get("m2effig"))
     if H is not None:
         copy_ul(H)
     copy_ul(O)
@@ -219,18 +251,19 @@ def copy_ul(M):
     return M
 
 
-def setup(cr, kp, apwalm=None, per_atom=False, meffig=None):
+def setup(cr, kp, apwalm=None, per_atom=False, meffig=None, iora=None):
     """match (if apwalm is None) + hamiltonsetup + overlapsetup, as seceqn/seceqnfv call them.
     per_atom: the muffin-tin sums as one zgemm per atom, the reference's own order (the CPU
     baseline); the default sums in l-blocks (the checker, smaller rounding error).  meffig:
-    H's interstitial kinetic factor with ValenceRelativity /= 'none' (hmlistln.f90:61-72)."""
+    H's interstitial kinetic factor with ValenceRelativity /= 'none' (hmlistln.f90:61-72); iora:
+    the 'iora*' overlap terms (overlapsetup)."""
     if apwalm is None:
         apwalm = match(cr, kp)
     nmat = kp.ngp + cr.nlotot
     H = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
     O = np.zeros((nmat, nmat), dtype=np.complex128, order="F")
     hamiltonsetup(cr, kp, apwalm, H, per_atom=per_atom, meffig=meffig)
-    overlapsetup(cr, kp, apwalm, O, H, per_atom=per_atom)
+    overlapsetup(cr, kp, apwalm, O, H, per_atom=per_atom, iora=iora)
     return H, O, apwalm
 
 

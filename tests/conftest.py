@@ -32,3 +32,20 @@ def small(key):
                   cell=(9.0, 9.5, 10.0)),
     }
     return synth.make_small(**specs[key])
+
+
+def iora_terms(cr, seed, scale=1e-2):
+    """Seeded 'iora*' overlap terms for a crystal: h1aa symmetric in (io1, io2) with I + h1aa
+    positive definite, h1loa and h1lolo (h1lolo symmetric, as olprad makes it), and a Hermitian
+    m2effig (m2effig(-G) = conj m2effig(G)): cfunig times 1 + 0.3 cos(0.9 |G|)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ao, na = cr.apwordmax, cr.natm
+    h = rng.uniform(-scale, scale, (ao, ao, cr.lmaxapw + 1, na))
+    h1aa = 0.5 * (h + h.transpose(1, 0, 2, 3))
+    nlo = cr.oalo.shape[1]
+    h1loa = rng.uniform(-scale, scale, (ao, nlo, na))
+    q = rng.uniform(-scale, scale, (nlo, nlo, na))
+    h1lolo = 0.5 * (q + q.transpose(1, 0, 2))
+    m2effig = cr.cfunig[:cr.ngvec] * (1.0 + 0.3 * np.cos(0.9 * cr.gc[:cr.ngvec]))
+    return dict(h1aa=h1aa, h1loa=h1loa, h1lolo=h1lolo, m2effig=m2effig)

@@ -190,6 +190,65 @@ def test_meffig_kinetic_term():
     assert np.array_equal(H0, H1)
 
 
+def test_iora_overlap_terms():
+    """The oracle's 'iora*' overlap (overlapsetup.f90:81-94, 126, 144; olpistln.f90:53-62),
+    pinned term by term against forms written independently of it:
+    * all terms zero: the 'none' O bit for bit;
+    * h1aa with only (io1, io2) = (0, 1) set to e: O gains e sum_{a,lm} conj(A(i, io 1, lm, a))
+      A(j, io 0, lm, a) — an einsum over apwalm (a transposed h1aa would pair io 0 with io 1
+      the other way round);
+    * h1aa = (c - 1) I: the muffin-tin APW part times c;
+    * h1loa = -oalo, h1lolo = -ololo: the APW-lo and lo-lo blocks vanish;
+    * m2effig alone: O - O_none = a2 times the kinetic part hmlistln forms with meffig = m2effig
+      and V = 0, with a2 = alpha^2 / 2, alpha = 1/137.03599911."""
+    import copy
+    from conftest import iora_terms
+    w = small("b")
+    cr, kp = w.crystal, w.kpoints[0]
+    n, nmat = kp.ngp, kp.ngp + cr.nlotot
+    t = iora_terms(cr, 5)
+    _, O0, A = R.setup(cr, kp)
+    zero = {k: np.zeros_like(v) for k, v in t.items()}
+    _, Oz, _ = R.setup(cr, kp, A, iora=zero)
+    assert np.array_equal(Oz, O0)
+    # h1aa(0, 1, l, a) = e
+    e = 1e-3
+    h = np.zeros_like(t["h1aa"])
+    h[0, 1] = e
+    _, O1, _ = R.setup(cr, kp, A, iora=dict(h1aa=h))
+    nl2 = (cr.lmaxmat + 1) ** 2
+    ref = e * np.einsum("ila,jla->ij", np.conj(A[:n, 1, :nl2, :]), A[:n, 0, :nl2, :])
+    dO = (O1 - O0)[:n, :n]
+    iu = np.triu_indices(n)
+    assert np.abs(dO[iu] - ref[iu]).max() <= 1e-14 * np.abs(O0).max()
+    # h1aa = (c - 1) I
+    c = 1.25
+    h = np.zeros_like(t["h1aa"])
+    h[0, 0] = h[1, 1] = c - 1.0
+    _, Oc, _ = R.setup(cr, kp, A, iora=dict(h1aa=h))
+    mt = np.zeros((n, n), dtype=complex)
+    mt += sum(R._apwi(cr, A, n, a).conj().T @ R._apwi(cr, A, n, a) for a in range(cr.natm))
+    d = (Oc - O0)[:n, :n]
+    assert np.abs(d[iu] - (c - 1.0) * mt[iu]).max() <= 1e-13 * np.abs(mt).max()
+    # h1loa = -oalo, h1lolo = -ololo
+    _, Ol, _ = R.setup(cr, kp, A, iora=dict(h1loa=-cr.oalo, h1lolo=-cr.ololo))
+    assert np.abs(Ol[:n, n:]).max() == 0.0 and np.abs(Ol[n:, n:]).max() == 0.0
+    assert np.array_equal(Ol[:n, :n], O0[:n, :n])
+    assert np.abs(O0[:n, n:]).max() > 1e-2 and np.abs(O0[n:, n:]).max() > 0.1
+    # m2effig
+    m = t["m2effig"]
+    _, Om, _ = R.setup(cr, kp, A, iora=dict(m2effig=m))
+    c2 = copy.copy(cr)
+    c2.veffig = np.zeros_like(cr.veffig)
+    K = np.zeros((n, n), dtype=np.complex128)
+    R.hmlistln(c2, kp, K, meffig=m)
+    a2 = 0.5 / 137.03599911 ** 2
+    d = (Om - O0)[:n, :n]
+    # the difference of two O of order 1: rounding of O itself, far below the term (~1e-4)
+    assert np.abs(d[iu] - a2 * K[iu]).max() <= 1e-15 * np.abs(O0).max()
+    assert np.abs(K).max() > 1.0
+
+
 def test_step_function_real_space():
     """Theta(G) = (1/Omega) int_cell Theta(r) exp(-i G.r) dr, by a grid sum over the cell."""
     w = small("a")
