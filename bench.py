@@ -113,7 +113,8 @@ def crt_alg_ops(cr, ngps, nmod, share):
 
 def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt"):
     """The split-precision leg: the muffin-tin contraction from exact integer residue products
-    (kind 'crt', EXC_BATCH_SPLIT_CRT, default 11 moduli) or from three BF16 slices (kind
+    (kind 'crt', EXC_BATCH_SPLIT_CRT: the library's form and moduli count unless --moduli
+    pins the count) or from three BF16 slices (kind
     'bf16', EXC_BATCH_SPLIT_BF16); everything else as the FP64 leg.  Eigenvalue error: the
     lowest 200 of k-point 0 against the FP64 GPU path's (e0)."""
     import torch
@@ -125,9 +126,8 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
         st, v = kv.split("=")
         ctx.set_variant(st, int(v))
         tuned = dict(tuned or {}, **{st: int(v)})
-    nmod = args.moduli
     if kind == "crt":
-        ctx.set_moduli(nmod)
+        ctx.set_moduli(args.moduli or 0)
     st = "crtgemm" if kind == "crt" else "bsplit"
     if args.tune:
         from exciting_amd import autotune
@@ -163,7 +163,9 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
     share = 0.5 if olpflag else 1.0           # H only when O comes from the addition theorem
     ngps = [kp.ngp for kp in w.kpoints]
     fl64 = share * sum(mt_flops(cr, n) for n in ngps)
+    form = nmod = sigma = None
     if kind == "crt":
+        form, nmod, sigma = ctx.crt_form()
         ms = eager["crtgemm"][0]
         ops = crt_alg_ops(cr, ngps, nmod, share)
         # variants 8-10 are the register-pipelined kernel; the library's default by basis size
@@ -176,7 +178,7 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
                 "per_launch_ops": ops, "traffic": None}
         # HBM traffic of the residue GEMM from the committed rocprofv3 summary of the default command
         prof = HERE / "profiles" / f"setup_kernels_workload{args.config}.json"
-        if args.nk is None and not args.tune and not args.set and args.moduli == 11 and olpflag and prof.exists():
+        if args.nk is None and not args.tune and not args.set and args.moduli is None and olpflag and prof.exists():
             rec = json.loads(prof.read_text()).get(gname)
             if rec:
                 roof["traffic"] = rec["fetch_bytes"] + rec["write_bytes"]
@@ -190,7 +192,8 @@ def run_split(args, ctx, batch, cr, w, fp64_value, e0, log, olpflag=0, kind="crt
     log(f"  {kind} split: {value:.1f} k-points/s ({value / fp64_value:.2f}x FP64), lowest-200 eig err {err:.2e} Ha")
     return {"value": value, "unit": "k-points/s", "speedup_vs_fp64": value / fp64_value,
             "ms_per_step": 1000.0 * dt / args.steps, "eig_err_lowest200_Ha_k0": err, "lowest_eig_Ha_k0": float(e0[0]),
-            "meets_1e-8_Ha": err <= 1e-8, "moduli": nmod if kind == "crt" else None, "roofline": roof,
+            "meets_1e-8_Ha": err <= 1e-8, "moduli": nmod, "form": {0: "A,Yt", 1: "gram"}.get(form),
+            "gram_sigma": sigma if form == 1 else None, "roofline": roof,
             "stages_ms": {k: v for k, (v, c) in eager.items() if c}, "variants": tuned}
 
 
@@ -503,7 +506,8 @@ def main():
                     help="pin a stage's tile variant (e.g. bmt=1, ystage=2); repeatable")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision measurements")
     ap.add_argument("--no-bf16", action="store_true", help="skip the three-slice BF16 split measurement")
-    ap.add_argument("--moduli", type=int, default=11, help="moduli of the modular split (8..12, default 11)")
+    ap.add_argument("--moduli", type=int, default=None,
+                    help="moduli of the modular split (8..12; default: the library's, 10 with the Gram form, else 11)")
     ap.add_argument("--no-hmlint", action="store_true", help="skip the muffin-tin integrals (hmlint) measurement")
     ap.add_argument("--eigen", type=int, default=200, metavar="NEV",
                     help="grouped path: also time setup + device eigensolve of the lowest NEV states "

@@ -114,9 +114,10 @@ int exc_ctx_create(const exc_dims *dims, exc_ctx **out) {
   c->bmt_variant = -1;
   c->bsplit_variant = -1;
   c->crt_variant = -1;
-  c->crt_nmod = 11;
+  c->crt_nmod = 0;                 // built-in: 10 moduli with the Gram form, 11 without
   c->olp_variant = -1;
   c->fork_variant = -1;
+  c->crt_form = -1;
   c->eig_variant = -1;
   c->gnt_lmaxvr = -1;
   c->hml_lmaxvr = -1;
@@ -179,6 +180,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->crt_q) hipFree(c->crt_q);
   if (c->crt_x) hipFree(c->crt_x);
   if (c->crt_parts) hipFree(c->crt_parts);
+  if (c->gram_buf) hipFree(c->gram_buf);
   if (c->gk_buf) hipFree(c->gk_buf);
   if (c->crt_res) hipFree(c->crt_res);
   if (c->d_citems) hipFree(c->d_citems);
@@ -248,6 +250,10 @@ int exc_ctx_set_variant(exc_ctx *c, int stage, int variant) {
       if (variant < -1 || variant >= 2) return EXC_ERR_ARG;
       c->fork_variant = variant;
       return EXC_OK;
+    case EXC_STAGE_CRTFORM:
+      if (variant < -1 || variant >= 2) return EXC_ERR_ARG;
+      c->crt_form = variant;
+      return EXC_OK;
     default:
       return EXC_ERR_ARG;
   }
@@ -263,13 +269,22 @@ int exc_ctx_num_variants(int stage) {
     case EXC_STAGE_CRTGEMM: return kCrtVariants;
     case EXC_STAGE_OLP: return kOlpVariants;
     case EXC_STAGE_GRAPH: return 2;   // the overlap and lo kernels on a side branch, or in line
+    case EXC_STAGE_CRTFORM: return 2; // the modular split's operands: A and Yt, or the Gram form
     default: return 0;
   }
 }
 
 int exc_ctx_set_moduli(exc_ctx *c, int n) {
-  if (!c || n < 8 || n > kCrtMaxUsed) return EXC_ERR_ARG;
+  if (!c || (n != 0 && (n < 8 || n > kCrtMaxUsed))) return EXC_ERR_ARG;
   c->crt_nmod = n;
+  return EXC_OK;
+}
+
+int exc_ctx_crt_form(const exc_ctx *c, int *form, int *nmod, double *sigma) {
+  if (!c || !form || !nmod || !sigma) return EXC_ERR_ARG;
+  *form = c->crt_last_form;
+  *nmod = c->crt_last_nmod;
+  *sigma = c->crt_last_form == 1 ? c->gram_sigma : 0.0;
   return EXC_OK;
 }
 
@@ -450,6 +465,7 @@ int exc_ctx_set_mt(exc_ctx *c, const double *haa, const double *loa, const doubl
   }
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_mt = true;
+  c->gram_valid = false;
   return EXC_OK;
 }
 
@@ -688,6 +704,58 @@ int setup_host_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
   return exc_ctx_status(c);
 }
 
+// The Gram form's table (exc_launch_crt_gram_table) from the eigen-decomposition of every
+// h_a (rocSOLVER, strided batch), sigma = -min lambda (times 1 + 1e-6) when some lambda < 0,
+// else 0.  Once per change of h_a; failed decompositions leave the form unused.
+int gram_build(exc_ctx *c) {
+  if (c->gram_valid) return EXC_OK;
+  const int na = c->d.natm, n = c->d.naa;
+  const size_t nn = (size_t)n * n * na;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_u = al(sizeof(cplx) * nn), o_l = o_u + al(sizeof(cplx) * nn), o_e = o_l + al(8 * (size_t)n * na);
+  const size_t o_i = o_e + al(8 * (size_t)n * na), o_s = o_i + al(4 * (size_t)na);
+  int rc = grow(&c->gram_buf, &c->gram_bytes, o_s + 8);
+  if (rc) return rc;
+  char *b = (char *)c->gram_buf;
+  cplx *T = (cplx *)b, *U = (cplx *)(b + o_u);
+  double *lam = (double *)(b + o_l), *E = (double *)(b + o_e), *dsig = (double *)(b + o_s);
+  int *info = (int *)(b + o_i);
+  EXC_CHECK(hipMemcpyAsync(U, c->d.haa, sizeof(cplx) * nn, hipMemcpyDeviceToDevice, c->stream));
+  exc_ctx::EigWorker &W = c->ew[0];
+  W.s = c->stream;
+  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
+    W.blas = nullptr;
+    return EXC_ERR_HIP;
+  }
+  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
+  if (rocsolver_zheevd_strided_batched(W.blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                       (rocblas_double_complex *)U, n, (rocblas_stride)n * n, lam, n, E, n, info,
+                                       na) != rocblas_status_success)
+    return EXC_ERR_HIP;
+  std::vector<double> hl((size_t)n * na);
+  std::vector<int> hi(na);
+  EXC_CHECK(hipMemcpyAsync(hl.data(), lam, 8 * hl.size(), hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipMemcpyAsync(hi.data(), info, 4 * (size_t)na, hipMemcpyDeviceToHost, c->stream));
+  EXC_CHECK(hipStreamSynchronize(c->stream));
+  bool ok = true;
+  double lmin = 0.0, lmax = 0.0;
+  for (int a = 0; a < na; ++a) ok = ok && hi[a] == 0;
+  for (double v : hl) {
+    ok = ok && std::isfinite(v);
+    lmin = std::min(lmin, v);
+    lmax = std::max(lmax, fabs(v));
+  }
+  c->gram_sigma = ok ? (lmin < 0.0 ? -lmin * (1.0 + 1e-6) : 0.0) : HUGE_VAL;
+  c->gram_lmax = lmax;
+  if (ok) {
+    if ((rc = exc_launch_crt_gram_table(c->d, U, lam, c->gram_sigma, T, c->stream))) return rc;
+    EXC_CHECK(hipMemcpyAsync(dsig, &c->gram_sigma, 8, hipMemcpyHostToDevice, c->stream));
+    EXC_CHECK(hipStreamSynchronize(c->stream));
+  }
+  c->gram_valid = true;
+  return EXC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -785,12 +853,35 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   const int cv = c->crt_variant >= 0 ? c->crt_variant : (maxn >= 2048 ? 8 : 4);
   const int ov = c->olp_variant >= 0 ? c->olp_variant : 3;   // rows along lanes, column factors as scalars, pipelined
   const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
+  // the modular split's Gram form: needs O_APW from the addition theorem, equal to A's Gram
+  // matrix (no 'iora*' h1aa); built in by default unless the shift sigma exceeds half of
+  // h_a's largest |eigenvalue| (the form's rounding grows with sigma)
+  bool gram = false;
+  if (crt && olp && !(c->d.iora & 1) && c->crt_form != 0) {
+    if ((rc = gram_build(c))) return rc;
+    gram = std::isfinite(c->gram_sigma) && (c->crt_form == 1 || c->gram_sigma <= 0.5 * c->gram_lmax);
+  }
+  const char *gram_b = (const char *)c->gram_buf;
+  const cplx *gram_t = gram ? (const cplx *)gram_b : nullptr;
+  const double *gram_s = nullptr;
+  if (gram) {
+    const size_t nn = (size_t)c->d.naa * c->d.naa * c->d.natm, nl = (size_t)c->d.naa * c->d.natm;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    gram_s = (const double *)(gram_b + 2 * al(sizeof(cplx) * nn) + 2 * al(8 * nl) + al(4 * (size_t)c->d.natm));
+  }
+  // moduli: the caller's count, else 10 with the Gram form (its rounding error is far below
+  // the (A, Yt) form's: 5.7e-10 Ha at 10 moduli against 2.6e-9 at 11 on workload 3) and 11
+  const int nmod = c->crt_nmod > 0 ? c->crt_nmod : (gram ? 10 : 11);
   CrtTab ct;
-  if (crt) exc_crt_tables(c->crt_nmod, &ct);
+  if (crt) {
+    exc_crt_tables(nmod, &ct);
+    c->crt_last_form = gram ? 1 : 0;
+    c->crt_last_nmod = nmod;
+  }
   constexpr int kCrtPlanes = 2;                             // residue planes per operand and modulus
   const int kp = exc_crt_kpad(c->d.natm * c->d.naa);
   // batch signature: descriptors, flags and variants
-  int tail[7] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? c->crt_nmod : -1) + 100 * ov, fv};
+  int tail[8] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? nmod : -1) + 100 * ov, fv, gram};
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + sizeof(tail));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
   memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
@@ -913,7 +1004,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         qoff += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
         e.xa = c->crt_x + xoff;
         e.xy = c->crt_x + xoff + cnp;
-        e.pa2 = do_match ? c->crt_parts + poff2 : nullptr;
+        e.pa2 = (do_match && !gram) ? c->crt_parts + poff2 : nullptr;
         poff2 += nparts * cnp;
         xoff += 2 * (size_t)cnp;
         maxcnp = cnp > maxcnp ? cnp : maxcnp;
@@ -922,7 +1013,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
           for (int bj = 0; bj < nt; bj += S)
             for (int bi = 0; bi <= bj; bi += S)
               for (int tj = bj; tj < std::min(bj + S, nt); ++tj)
-                for (int ti = bi; ti < std::min(bi + S, tj + 1); ++ti) citems.push_back({k, ti, tj, which});
+                for (int ti = bi; ti < std::min(bi + S, tj + 1); ++ti)
+                  citems.push_back({k, ti, tj, gram ? 2 : which});
         for (int l = 0; l < ct.n; ++l)
           for (int p = 0; p < 2; ++p)
             for (int i = first; i < (int)citems.size(); ++i) cwork.push_back(make_int2(i, 4 * l + p));
@@ -1016,15 +1108,24 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       if (hipEventRecord(c->ev_join, c->side) != hipSuccess) return EXC_ERR_HIP;
     }
     {
+      // the Gram form's E = A conj(U) (Lambda + sigma)^(1/2) is the Y stage on its table
       StageTimer t(c, capture ? -1 : EXC_STAGE_YSTAGE);
-      if ((r = exc_launch_bystage(c->d, nk, c->batch_maxngp, c->d_ks, yv, c->stream))) return r;
+      exc_dev dy = c->d;
+      if (gram) dy.haa = (cplx *)gram_t;
+      if ((r = exc_launch_bystage(dy, nk, c->batch_maxngp, c->d_ks, yv, c->stream))) return r;
+    }
+    // the Gram form's merge reads O's APW block: the overlap runs before it
+    const bool olp_first = gram && !fork;
+    if (olp_first) {
+      StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
+      if ((r = exc_launch_olp(c->d, nk, c->batch_maxngp, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
     }
     if (crt) {
       // the residue GEMM and the merge write only H's (and without the addition theorem O's)
       // APW-APW tiles: no conflict with the side branch
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTPREP);
-        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, do_match, c->stream))) return r;
+        if ((r = exc_launch_crt_prep(c->d, nk, c->batch_maxcnp, c->d_ks, ct, do_match, gram, c->stream))) return r;
       }
       {
         StageTimer t(c, capture ? -1 : EXC_STAGE_CRTGEMM);
@@ -1032,8 +1133,10 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
                                      c->stream)))
           return r;
       }
+      if (gram && fork && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
       StageTimer t(c, capture ? -1 : EXC_STAGE_CRTMERGE);
-      if ((r = exc_launch_crt_merge(c->d, c->d_ks, c->d_citems, c->batch_ncitems, ct, c->crt_res, cv, c->stream)))
+      if ((r = exc_launch_crt_merge(c->d, c->d_ks, c->d_citems, c->batch_ncitems, ct, c->crt_res, cv, gram_s,
+                                    c->stream)))
         return r;
     } else if (split) {
       {
@@ -1051,7 +1154,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     }
     if (fork && !join_early && hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) return EXC_ERR_HIP;
     if (fork) return r;
-    if (olp) {
+    if (olp && !olp_first) {
       StageTimer t(c, capture ? -1 : EXC_STAGE_OLP);
       if ((r = exc_launch_olp(c->d, nk, c->batch_maxngp, c->d_ks, c->d_oitems, c->batch_noitems, ov, c->stream))) return r;
     }

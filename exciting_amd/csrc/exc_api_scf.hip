@@ -156,6 +156,7 @@ int exc_ctx_hmlint(exc_ctx *c, int lmaxvr, int nrmtmax, const int *nrmt, const d
   if (nl && ololo) EXC_CHECK(hipMemcpyAsync(ololo, c->d.ololo, 8 * nl * nl * na, hipMemcpyDeviceToHost, c->stream));
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->have_mt = true;
+  c->gram_valid = false;
   c->hml_lmaxvr = lmaxvr;
   c->hml_nrmtmax = nrmtmax;
   return EXC_OK;

@@ -167,11 +167,13 @@ void exc_crt_tables(int n, CrtTab *t);
 inline int exc_crt_kpad(int ktot) { return (ktot + 63) / 64 * 64; }
 int exc_crt_shape(int variant, int *bt, int *wr, int *wc);   // tile edge and wave grid of a GEMM variant
 int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, bool a_parts,
-                        hipStream_t s);
+                        bool gram, hipStream_t s);
 int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items, const int2 *work, int nwork,
                         const CrtTab &t, signed char *res, int variant, hipStream_t s);
 int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
-                         const signed char *res, int variant, hipStream_t s);
+                         const signed char *res, int variant, const double *gsig, hipStream_t s);
+int exc_launch_crt_gram_table(const exc_dev &d, const cplx *U, const double *lam, double sigma, cplx *T,
+                              hipStream_t s);
 // generalized eigensolve helpers (k_eigen.hip)
 struct eig_kdesc {   // one k-point of a grouped solve: nmat n, device arrays
   const cplx *h, *o;

@@ -85,6 +85,14 @@ struct exc_ctx {
   void *gk_buf;               // G+k sets: k-point coordinates, counts, and the host entries' staging
   size_t gk_buf_bytes;
   double cfdamp_t1;           // gencfun's damping exponent cfdamp / gmaxvr (0: none)
+  // Gram form of the modular split: the table conj(U) (Lambda + sigma)^(1/2) per atom, the
+  // eigenvectors and values of h_a, sigma (host and device); valid until h_a changes
+  int crt_form;               // -1 built-in (Gram where it applies), 0 A and Yt, 1 Gram
+  int crt_last_form, crt_last_nmod;   // what the last modular-split setup used
+  void *gram_buf;
+  size_t gram_bytes;
+  bool gram_valid;
+  double gram_sigma, gram_lmax;
   signed char *crt_res;
   size_t crt_res_bytes;
   mt_item *d_citems;

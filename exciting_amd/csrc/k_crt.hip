@@ -27,6 +27,14 @@
 // Integer sums do not depend on their order, so every variant of every kernel gives the
 // same bits; the only rounding between the inputs and H_MT is that of the operands to
 // beta-bit integers and the final conversion.
+//
+// Gram form (the default where it applies): with h_a = U_a Lambda_a U_a^H and a shift
+// sigma >= -min lambda, E_a = A_a conj(U_a) (Lambda_a + sigma)^(1/2) (the Y stage on that
+// table instead of h_a) gives conj(E) E^T = conj(A) (h + sigma) A^T = H_MT + sigma O_APW, so
+// H_MT = conj(E) E^T - sigma O_APW with O_APW the FP64 addition-theorem overlap (O minus
+// its interstitial term).  One operand is split instead of two, and the right operand's
+// planes are the left's swapped, as for O = conj(A) A^T; merge items of this form carry
+// which = 2.
 #include "setup_dev.h"
 #include "mt_dma.h"
 
@@ -54,14 +62,14 @@ __device__ __forceinline__ int dres(double x, int m, double minv) { return (int)
 // quarters added in wave order (a fixed order: the exponents do not depend on timing)
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict__ ks, int ktot, int beta, int op) {
-  const kdesc k = ks[blockIdx.z];             // op 0: A (apwalm), 1: Yt
+  const kdesc k = ks[blockIdx.z];             // op 0: A (apwalm), 1: Yt, 2: E (in Yt's place) as the left operand
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = blockIdx.x * 64 + lane;
   __shared__ double part[4][64];
   if (blockIdx.x * 64 >= k.cnp) return;       // uniform over the workgroup
   const cplx *src = op ? k.yt : k.apwalm;
   const int ld = op ? k.ldy : k.ldg;
-  const int kq = (ktot + 3) / 4, k0 = wave * kq, k1 = min(ktot, k0 + kq);
+  const int kq = (ktot + 3) / 4, k0 = wave * kq, k1 = min(ktot, k0 + kq);   // op 2 writes xa
   double acc = 0.0;
   if (row < k.ngp) {
     for (int c = k0; c < k1; ++c) {
@@ -76,7 +84,7 @@ __global__ __launch_bounds__(256) void crt_scale_kernel(const kdesc *__restrict_
     const double s2 = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
     int e = 0;
     if (s2 > 0.0) frexp(sqrt(s2), &e);       // |X_i| < 2^e
-    (op ? k.xy : k.xa)[row] = row < k.ngp ? beta - e : 0;
+    (op == 1 ? k.xy : k.xa)[row] = row < k.ngp ? beta - e : 0;
   }
 }
 
@@ -136,15 +144,17 @@ __device__ __forceinline__ void split_planes(const double (&xr)[16], const doubl
   }
 }
 
-__global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t) {
-  const kdesc k = ks[blockIdx.z >> 1];
-  const int op = blockIdx.z & 1;
+// gram: one operand per k-point (grid z = k), E from Yt's buffer into the left planes
+__global__ __launch_bounds__(256) void crt_split_kernel(const kdesc *__restrict__ ks, int ktot, int kp, CrtTab t,
+                                                        int gram) {
+  const kdesc k = ks[gram ? blockIdx.z : blockIdx.z >> 1];
+  const int op = gram ? 0 : blockIdx.z & 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = blockIdx.x * 64 + lane;
   const int kc = blockIdx.y * 4 + wave;
   if (blockIdx.x * 64 >= k.cnp || kc * 16 >= kp) return;
-  const cplx *src = op ? k.yt : k.apwalm;
-  const int ld = op ? k.ldy : k.ldg;
+  const cplx *src = (op || gram) ? k.yt : k.apwalm;
+  const int ld = (op || gram) ? k.ldy : k.ldg;
   const int s = row < k.ngp ? (op ? k.xy : k.xa)[row] : 0;
   double xr[16], xi[16];
 #pragma unroll
@@ -577,7 +587,7 @@ __device__ __forceinline__ double garner(const int (&s)[NM]) {
 template <int NM>
 __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict__ ks, const mt_item *__restrict__ items,
                                                         const signed char *__restrict__ res, int bt, int wr_n, int wc_n,
-                                                        GTab gt) {
+                                                        GTab gt, const double *__restrict__ gsig) {
   const mt_item it = items[blockIdx.x];
   const kdesc k = ks[it.k];
   const int ngp = k.ngp;
@@ -608,7 +618,8 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
   const int nblk = (bt / 32) * (bt / 32);
   const size_t ptile = (size_t)bt * bt;                     // residue bytes of one (modulus, product)
   const signed char *base = res + (size_t)blockIdx.x * NM * 2 * ptile;
-  cplx *out = it.which ? k.o : k.h;
+  cplx *out = it.which == 1 ? k.o : k.h;
+  const double sig = it.which == 2 ? *gsig : 0.0;           // Gram form: H_MT = conj(E) E^T - sigma O_APW
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // Element (row, col) of a 32 x 32 block sits in lane (col mod 32) of the gemm layout, so the
   // conjugate store at (col, row) is coalesced along the lanes; the store at (row, col) goes
@@ -632,6 +643,17 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      // Gram form: this half's eight O elements, loaded together ahead of the reconstruction;
+      // read at (col, row), O's conjugate mirror (bit for bit), where the lanes are contiguous
+      cplx om[8];
+      if (it.which == 2) {
+#pragma unroll
+        for (int vv = 0; vv < 8; ++vv) {
+          const int v = 8 * h + vv;
+          const int row = row0 + rb + 16 * h + 8 * (vv >> 2) + 4 * (lane >> 5) + (v & 3);
+          om[vv] = (col < ngp && row <= col) ? k.o[col + (size_t)row * k.ldh] : cplx{0.0, 0.0};
+        }
+      }
 #pragma unroll
       for (int vv = 0; vv < 8; ++vv) {
         const int v = 8 * h + vv;
@@ -655,9 +677,19 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
           double re = ldexp(garner<NM, false>(sr), sc);
           double im = ldexp(garner<NM, true>(si), sc);
           const int ig = lookup_g(gt, gr[rl][0] - gc[cl][0], gr[rl][1] - gc[cl][1], gr[rl][2] - gc[cl][2]);
+          if (it.which == 2) {
+            // O_APW = O - its interstitial term, O as the overlap kernel wrote it
+            cplx oa = {om[vv].re, -om[vv].im};
+            if (ig) {
+              const cplx tho = setup_dev::olp_theta(gt, ig, vr[rl], vc[cl]);
+              oa = cplx{oa.re - tho.re, oa.im - tho.im};
+            }
+            re = fma(-sig, oa.re, re);
+            im = fma(-sig, oa.im, im);
+          }
           if (ig) {
             const cplx th = gt.cfunig[ig - 1];
-            if (it.which) {
+            if (it.which == 1) {
               re += th.re;
               im += th.im;
             } else {
@@ -687,9 +719,9 @@ __global__ __launch_bounds__(256) void crt_merge_kernel(const kdesc *__restrict_
 
 template <int NM>
 int launch_merge_t(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
-                   const signed char *res, int bt, int wr, int wc, hipStream_t s) {
+                   const signed char *res, int bt, int wr, int wc, const double *gsig, hipStream_t s) {
   hipLaunchKernelGGL((crt_merge_kernel<NM>), dim3(nitems), dim3(256), 0, s, ks, items, res, bt, wr, wc,
-                     setup_dev::make_gtab(d));
+                     setup_dev::make_gtab(d), gsig);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -733,11 +765,18 @@ void exc_crt_tables(int n, CrtTab *t) {
 }
 
 // a_parts: the grouped match wrote A's squared norms per atom (else A takes a pass of its
-// own, as Yt always does)
+// own, as Yt always does); gram: E in Yt's buffer is the only operand
 int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, const CrtTab &t, bool a_parts,
-                        hipStream_t s) {
+                        bool gram, hipStream_t s) {
   if (nk <= 0 || maxnp <= 0) return EXC_OK;
   const int ktot = d.natm * d.naa, kp = exc_crt_kpad(ktot);
+  if (gram) {
+    hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, nk), dim3(256), 0, s, ks, ktot, t.beta, 2);
+    EXC_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, nk), dim3(256), 0, s, ks, ktot, kp, t, 1);
+    EXC_CHECK(hipGetLastError());
+    return EXC_OK;
+  }
   if (a_parts) {
     hipLaunchKernelGGL(crt_expo_kernel, dim3((maxnp + 255) / 256, 1, nk), dim3(256), 0, s, ks, d.natm, t.beta);
     EXC_CHECK(hipGetLastError());
@@ -747,7 +786,7 @@ int exc_launch_crt_prep(const exc_dev &d, int nk, int maxnp, const kdesc *ks, co
   }
   hipLaunchKernelGGL(crt_scale_kernel, dim3(maxnp / 64, 1, nk), dim3(256), 0, s, ks, ktot, t.beta, 1);
   EXC_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t);
+  hipLaunchKernelGGL(crt_split_kernel, dim3(maxnp / 64, kp / 64, 2 * nk), dim3(256), 0, s, ks, ktot, kp, t, 0);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -790,16 +829,39 @@ int exc_launch_crt_gemm(const exc_dev &d, const kdesc *ks, const mt_item *items,
 }
 
 int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items, int nitems, const CrtTab &t,
-                         const signed char *res, int variant, hipStream_t s) {
+                         const signed char *res, int variant, const double *gsig, hipStream_t s) {
   if (nitems <= 0) return EXC_OK;
   int bt = 0, wr = 0, wc = 0;
   if (exc_crt_shape(variant, &bt, &wr, &wc)) return EXC_ERR_ARG;
   switch (t.n) {
-    case 8: return launch_merge_t<8>(d, ks, items, nitems, t, res, bt, wr, wc, s);
-    case 9: return launch_merge_t<9>(d, ks, items, nitems, t, res, bt, wr, wc, s);
-    case 10: return launch_merge_t<10>(d, ks, items, nitems, t, res, bt, wr, wc, s);
-    case 11: return launch_merge_t<11>(d, ks, items, nitems, t, res, bt, wr, wc, s);
-    case 12: return launch_merge_t<12>(d, ks, items, nitems, t, res, bt, wr, wc, s);
+    case 8: return launch_merge_t<8>(d, ks, items, nitems, t, res, bt, wr, wc, gsig, s);
+    case 9: return launch_merge_t<9>(d, ks, items, nitems, t, res, bt, wr, wc, gsig, s);
+    case 10: return launch_merge_t<10>(d, ks, items, nitems, t, res, bt, wr, wc, gsig, s);
+    case 11: return launch_merge_t<11>(d, ks, items, nitems, t, res, bt, wr, wc, gsig, s);
+    case 12: return launch_merge_t<12>(d, ks, items, nitems, t, res, bt, wr, wc, gsig, s);
     default: return EXC_ERR_ARG;
   }
+}
+
+namespace {
+// T_a(r, f) = conj(U_a(f, r)) sqrt(lambda_r + sigma), column-major per atom: the table the Y
+// stage reads in h_a's place (it forms A_a T_a^T = A_a conj(U_a) (Lambda_a + sigma)^(1/2))
+__global__ __launch_bounds__(256) void crt_gram_table_kernel(int naa, const cplx *__restrict__ U,
+                                                             const double *__restrict__ lam, double sigma,
+                                                             cplx *__restrict__ T) {
+  const int r = blockIdx.x * 256 + threadIdx.x, f = blockIdx.y, a = blockIdx.z;
+  if (r >= naa) return;
+  const size_t o = (size_t)a * naa * naa;
+  const cplx u = U[o + f + (size_t)naa * r];
+  const double w = sqrt(fmax(lam[(size_t)a * naa + r] + sigma, 0.0));
+  T[o + r + (size_t)naa * f] = cplx{u.re * w, -u.im * w};
+}
+}  // namespace
+
+int exc_launch_crt_gram_table(const exc_dev &d, const cplx *U, const double *lam, double sigma, cplx *T,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(crt_gram_table_kernel, dim3((d.naa + 255) / 256, d.naa, d.natm), dim3(256), 0, s, d.naa, U, lam,
+                     sigma, T);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
 }

@@ -107,6 +107,7 @@ SIGNATURES = {
     "exc_gendmatmt_batched_dev": [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
     "exc_ctx_set_meffig": [_P, _P],
     "exc_ctx_set_iora": [_P, _P, _P, _P, _P],
+    "exc_ctx_crt_form": [_P, _P, _P, _P],
     "exc_ctx_set_cfdamp": [_P, C.c_double, C.c_double],
     "exc_gengpvec": [_P, _P, _P, _P, C.c_double, _I, _P, _P, _P, _P, _P, _P],
     "exc_gensfacgp": [_P, _I, _P, _I, _P],
@@ -116,6 +117,8 @@ SIGNATURES = {
 # stage order of exc_ctx_read_timing
 STAGES = ("match", "ystage", "mt", "lo", "bmt", "graph", "split", "bsplit", "vjp", "olp", "eigen", "dmat", "hmlint",
           "scf", "crtprep", "crtgemm", "crtmerge")
+# stages that take a variant but have no timer of their own (EXC_STAGE_CRTFORM)
+VARIANT_STAGES = STAGES + ("crtform",)
 
 
 def load_library(path: os.PathLike | None = None):
@@ -538,18 +541,24 @@ class Context:
                           ok_positive=True)
 
     def set_moduli(self, n):
-        """Number of moduli of the modular split (EXC_BATCH_SPLIT_CRT)."""
+        """Number of moduli of the modular split (EXC_BATCH_SPLIT_CRT); 0: the built-in count."""
         self._call("exc_ctx_set_moduli", self.h, int(n))
+
+    def crt_form(self):
+        """(form, moduli, sigma) of the last modular-split setup: form 0 A and Yt, 1 Gram."""
+        f, n, s = C.c_int(), C.c_int(), C.c_double()
+        self._call("exc_ctx_crt_form", self.h, C.byref(f), C.byref(n), C.byref(s))
+        return f.value, n.value, s.value
 
     def set_timing(self, on=True):
         self._call("exc_ctx_set_timing", self.h, 1 if on else 0)
 
     def set_variant(self, stage, variant):
         """Pin the tile variant of a stage ('ystage' or 'mt'); -1 = built-in choice."""
-        self._call("exc_ctx_set_variant", self.h, STAGES.index(stage), int(variant))
+        self._call("exc_ctx_set_variant", self.h, VARIANT_STAGES.index(stage), int(variant))
 
     def num_variants(self, stage):
-        return self.lib.exc_ctx_num_variants(STAGES.index(stage))
+        return self.lib.exc_ctx_num_variants(VARIANT_STAGES.index(stage))
 
     def read_timing(self):
         """{stage: (total_ms, launches)} recorded since the last read (synchronises the stream)."""

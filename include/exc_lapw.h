@@ -231,7 +231,10 @@ typedef struct exc_kpoint_dev {
                                 /* into two real int8 MFMA products with int32 accumulation, the   */
                                 /* results reconstructed by the Chinese remainder theorem and      */
                                 /* rounded once to FP64; n from exc_ctx_set_moduli.  Y stage,      */
-                                /* interstitial, lo blocks stay FP64                               */
+                                /* interstitial, lo blocks stay FP64.  With                        */
+                                /* EXC_BATCH_OLP_LEGENDRE (and no 'iora*' h1aa) the Gram form:     */
+                                /* E = A conj(U) (Lambda + sigma)^(1/2) from h_a = U Lambda U^H,   */
+                                /* H_MT = conj(E) E^T - sigma O_APW, one operand to split          */
 #define EXC_BATCH_OLP_LEGENDRE 8 /* needs EXC_BATCH_MATCH: the muffin-tin part of O from the      */
                                  /* radial factors of match by the spherical-harmonic addition   */
                                  /* theorem, sum_a conj(S_a) S_a' sum_l P_l(u.u') sum_io r r',   */
@@ -447,14 +450,22 @@ int exc_ctx_status(exc_ctx *ctx);
 #define EXC_STAGE_CRTPREP 14 /* row scales and residue planes (EXC_BATCH_SPLIT_CRT)           */
 #define EXC_STAGE_CRTGEMM 15 /* residue products, every tile and modulus                      */
 #define EXC_STAGE_CRTMERGE 16 /* reconstruction, interstitial and mirror                      */
+#define EXC_STAGE_CRTFORM 17 /* the modular split's operands: 0 A and Yt, 1 the Gram form      */
 #define EXC_NSTAGE 17
 int exc_ctx_set_timing(exc_ctx *ctx, int enable);
 
-/* Number of moduli of EXC_BATCH_SPLIT_CRT (8 .. 12, default 11).  With n moduli the row
- * integers have beta bits (the largest with 2^(2 beta + 2) <= their product): 29 bits at n = 8,
- * 33 at 9, 37 at 10, 40 at 11, 44 at 12; the residue products cost 2 n int8 GEMMs.  No
- * reference counterpart (the reference computes in FP64). */
+/* Number of moduli of EXC_BATCH_SPLIT_CRT (8 .. 12; 0, the default: 10 with the Gram form,
+ * 11 without).  With n moduli the row integers have beta bits (the largest with
+ * 2^(2 beta + 2) <= their product): 29 bits at n = 8, 33 at 9, 37 at 10, 40 at 11, 44 at 12;
+ * the residue products cost 2 n int8 GEMMs.  No reference counterpart (the reference
+ * computes in FP64). */
 int exc_ctx_set_moduli(exc_ctx *ctx, int n);
+/* What the last EXC_BATCH_SPLIT_CRT setup used: form 0 (A and Yt) or 1 (Gram), the number of
+ * moduli, and the Gram form's shift sigma (0 for form 0).  The form is chosen per call:
+ * EXC_STAGE_CRTFORM pins it (variant 0 or 1; the Gram form still needs
+ * EXC_BATCH_OLP_LEGENDRE and no h1aa), the built-in choice takes the Gram form unless sigma
+ * exceeds half of the largest |eigenvalue| of the h_a.  No reference counterpart. */
+int exc_ctx_crt_form(const exc_ctx *ctx, int *form, int *nmod, double *sigma);
 /* The first n moduli and beta of the modular split (for a caller's error estimate). */
 int exc_crt_moduli(int n, int *moduli, int *beta);
 

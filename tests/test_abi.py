@@ -80,6 +80,7 @@ def test_argument_errors_without_gpu():
     assert lib.exc_ctx_num_variants(15) == 11         # residue GEMM variants of the modular split
     assert lib.exc_ctx_num_variants(9) == 4           # thread blockings of the addition-theorem overlap
     assert lib.exc_ctx_num_variants(5) == 2           # grouped graph: overlap on a side branch or in line
+    assert lib.exc_ctx_num_variants(17) == 2          # modular split: A and Yt, or the Gram form
 
 
 def test_fortran_driver_built():

@@ -14,7 +14,10 @@ The integer products are exact, so the GPU result is fixed by the scheme itself:
 * the lowest 200 eigenvalues within 1e-8 Ha of the FP64 oracle's, on workload 1 (2 k-points)
   and on workload 3's first k-point (the committed oracle probes): the contract's precision bar;
 * every GEMM variant gives the same H and O bit for bit (integer sums do not depend on order),
-  and graph replays equal the eager launch.
+  and graph replays equal the eager launch;
+* the Gram form (E = A conj(U) (Lambda + sigma)^(1/2), H_MT = conj(E) E^T - sigma O_APW), the
+  default with the addition-theorem overlap: against the emulation on the host's E to the
+  same kind of bound, and both forms' eigenvalues within 1e-8 Ha.
 """
 import os
 
@@ -100,6 +103,86 @@ def test_crt_against_emulation(which):
     ctx.close()
 
 
+def _gram_e(cr, A, n, sigma):
+    """E = A conj(U) (Lambda + sigma)^(1/2) per atom from h_a = U Lambda U^H (numpy's eigh; the
+    library's rocSOLVER vectors differ by phases and within degenerate spaces, conj(E) E^T
+    does not)."""
+    out = []
+    for a in range(cr.natm):
+        lam, U = np.linalg.eigh(cr.haa[:, :, a])
+        out.append(A[:n, :, :, a].reshape(n, -1, order="F") @ (np.conj(U) * np.sqrt(np.maximum(lam + sigma, 0.0))))
+    return np.concatenate(out, axis=1)
+
+
+def test_crt_gram_form_against_emulation():
+    """The Gram form (the default with the addition-theorem overlap): the library reports form 1,
+    10 moduli and sigma = -min lambda(h_a) (1 + 1e-6); H's APW block equals the emulation's
+    conj(E) E^T (oracle/crt_ref.py on the host's E) - sigma O_APW + the interstitial term to
+    the scheme's own bound (each rounded row of E within sqrt(2K)/2 units of the exact one, a
+    unit below 2^(1 - beta) |E_i|), plus the roundings of the FP64 terms."""
+    from exciting_amd import lapw
+    from oracle import crt_ref as Q
+    from oracle import lapw_ref as R
+    w = workload(1, 2)
+    cr = w.crystal
+    ctx, b = _run(w, 2, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_CRT | lapw.EXC_BATCH_OLP_LEGENDRE)
+    form, nmod, sigma = ctx.crt_form()
+    lmin = min(np.linalg.eigvalsh(cr.haa[:, :, a])[0] for a in range(cr.natm))
+    print(f"form {form}, {nmod} moduli, sigma {sigma:.6f} (min lambda {lmin:.6f})", flush=True)
+    assert form == 1 and nmod == 10
+    assert abs(sigma - max(0.0, -lmin) * (1 + 1e-6)) <= 1e-12 * max(1.0, abs(lmin))
+    beta = Q.beta(nmod)
+    for i in range(2):
+        kp = w.kpoints[i]
+        n = kp.ngp
+        A4 = b.apw[i].cpu().numpy().transpose(3, 2, 1, 0)
+        H = b.H[i].cpu().numpy().T
+        O = b.O[i].cpu().numpy().T
+        E = _gram_e(cr, A4, n, sigma)
+        Ge, _, _ = Q.product(E, E, nmod)
+        Oi = np.zeros((n, n), dtype=complex)
+        R.olpistln(cr, kp, Oi)
+        Hi = np.zeros((n + cr.nlotot,) * 2, dtype=complex)
+        R.hmlistln(cr, kp, Hi)
+        Hi = Hi[:n, :n]
+        iu = np.triu_indices(n)
+        Oapw = O[:n, :n][iu] - Oi[iu]
+        ref = Ge[iu] - sigma * Oapw + Hi[iu]
+        ne = np.linalg.norm(E, axis=1)
+        tol = (2.0 ** (2 - beta) * np.sqrt(2 * E.shape[1]) * np.outer(ne, ne)[iu]
+               + 8 * EPS * (np.abs(Ge[iu]) + sigma * np.abs(O[:n, :n][iu]) + np.abs(Hi[iu])))
+        d = np.abs(H[:n, :n][iu] - ref)
+        print(f"Gram form k{i}: H max |d|/bound {np.max(d / tol):.2e}", flush=True)
+        assert (d <= tol).all()
+    ctx.close()
+
+
+@pytest.mark.parametrize("form,nmod", [(0, 0), (1, 0), (1, 11), (0, 10)])
+def test_crt_forms_eigenvalues(form, nmod):
+    """Both forms of the modular split, at the built-in and at a pinned moduli count: the
+    reported form and count, and the lowest 200 eigenvalues of workload 1's first k-point
+    within 1e-8 Ha of the FP64 oracle's."""
+    import torch
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 1)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(w.crystal, stream=s)
+    ctx.set_variant("crtform", form)
+    ctx.set_moduli(nmod)
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(w.crystal, w.kpoints[:1])
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_CRT | lapw.EXC_BATCH_OLP_LEGENDRE)
+        torch.cuda.synchronize()
+    f, n, sg = ctx.crt_form()
+    assert f == form and n == (nmod or (10 if form else 11)) and (sg > 0.0) == (form == 1)
+    H_ref, O_ref, _ = R.setup(w.crystal, w.kpoints[0])
+    de = np.abs(_eig(b.H[0].cpu().numpy().T, b.O[0].cpu().numpy().T) - _eig(H_ref, O_ref)).max()
+    print(f"form {f}, {n} moduli: lowest 200 eigenvalues vs FP64 oracle {de:.2e} Ha", flush=True)
+    assert de <= 1e-8
+    ctx.close()
+
+
 def _eig(H, O, nev=200):
     import scipy.linalg
     m = min(nev, H.shape[0])
@@ -138,8 +221,9 @@ def test_crt_eigenvalues_workload3():
     eo = np.linalg.norm(O @ X - ref["ox"]) / np.linalg.norm(ref["ox"])
     e = _eig(H, O, NEV)
     de = np.abs(e - ref["eig"]).max()
-    print(f"workload 3 k0, modular split (11 moduli): H {eh:.2e} O {eo:.2e} (probes), lowest {NEV} eigenvalues "
-          f"{de:.2e} Ha", flush=True)
+    form, nmod, _ = ctx.crt_form()
+    print(f"workload 3 k0, modular split (form {form}, {nmod} moduli): H {eh:.2e} O {eo:.2e} (probes), lowest "
+          f"{NEV} eigenvalues {de:.2e} Ha", flush=True)
     assert eo <= 1e-12 and eh <= 1e-9
     assert de <= 1e-8
     ctx.close()

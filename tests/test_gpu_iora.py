@@ -83,6 +83,8 @@ def test_iora_grouped_against_oracle():
                 m.fill_(float("nan"))
             ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | fl)
             torch.cuda.synchronize()
+            if fl & lapw.EXC_BATCH_SPLIT_CRT:
+                assert ctx.crt_form()[0] == 0          # h1aa: the (A, Yt) form
             for i, (H_ref, O_ref, _) in enumerate(refs):
                 eh = _rel(b.H[i].cpu().numpy().T, H_ref)
                 eo = _rel(b.O[i].cpu().numpy().T, O_ref)
@@ -145,4 +147,33 @@ def test_iora_refusals_and_clear():
         for i in range(b.nk):
             assert torch.equal(b.H[i].cpu(), ref[i][0]) and torch.equal(b.O[i].cpu(), ref[i][1])
     ctx.status()
+    ctx.close()
+
+
+def test_iora_interstitial_term_with_gram_split():
+    """m2effig alone leaves the modular split in its Gram form, whose merge takes O's APW block
+    as O minus O's interstitial element — with the m2effig term: H equals the 'none' run's
+    to 1e-13 (the recovered block differs from the FP64 one by its rounding), O gains the
+    term."""
+    import torch
+    from exciting_amd import lapw
+    w = workload(1, 1)
+    cr = w.crystal
+    t = iora_terms(cr, 24)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(cr, stream=s)
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_OLP_LEGENDRE | lapw.EXC_BATCH_SPLIT_CRT
+    out = []
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(cr, w.kpoints)
+        for m in (None, t["m2effig"]):
+            ctx.set_iora(m2effig=m)
+            ctx.setup_batched_dev(b, flags)
+            torch.cuda.synchronize()
+            assert ctx.crt_form()[0] == 1
+            out.append((b.H[0].cpu().numpy().T.copy(), b.O[0].cpu().numpy().T.copy()))
+    dh = _rel(out[1][0], out[0][0])
+    do = _rel(out[1][1], out[0][1])
+    print(f"Gram split with m2effig: H vs 'none' {dh:.2e}, O vs 'none' {do:.2e}", flush=True)
+    assert dh <= 1e-13 and do > 1e-8
     ctx.close()
