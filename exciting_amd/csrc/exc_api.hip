@@ -852,7 +852,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   for (int k = 0; k < nk; ++k) maxn = std::max(maxn, ks[k].ngp);
   const int cv = c->crt_variant >= 0 ? c->crt_variant : (maxn >= 2048 ? 8 : 4);
   const int ov = c->olp_variant >= 0 ? c->olp_variant : 3;   // rows along lanes, column factors as scalars, pipelined
-  const int fv = c->fork_variant >= 0 ? c->fork_variant : 1;   // one chain: faster on workloads 1-4
+  int fv = c->fork_variant;
   // the modular split's Gram form: needs O_APW from the addition theorem, equal to A's Gram
   // matrix (no 'iora*' h1aa); built in by default unless the shift sigma exceeds half of
   // h_a's largest |eigenvalue| (the form's rounding grows with sigma)
@@ -861,6 +861,10 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if ((rc = gram_build(c))) return rc;
     gram = std::isfinite(c->gram_sigma) && (c->crt_form == 1 || c->gram_sigma <= 0.5 * c->gram_lmax);
   }
+  // graph shape: one chain (faster for the FP64 path on workloads 1-4: the overlap kernel only
+  // competes with the MT kernel), the overlap and lo on a side branch for the Gram form, whose
+  // residue GEMM leaves the vector units idle (+1% at workload 3; the merge waits for the branch)
+  if (fv < 0) fv = gram ? 0 : 1;
   const char *gram_b = (const char *)c->gram_buf;
   const cplx *gram_t = gram ? (const cplx *)gram_b : nullptr;
   const double *gram_s = nullptr;
@@ -936,7 +940,7 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
     if (crt) {
       for (int k = 0; k < nk; ++k) {
         const size_t cnp = (size_t)(ks[k].ngp + 255) / 256 * 256;
-        qtot += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
+        qtot += (size_t)(gram ? 1 : 2) * kCrtPlanes * ct.n * kp * cnp;   // the Gram form splits E only
         xtot += 2 * cnp;
         ptot2 += nparts * cnp;
       }
@@ -1000,8 +1004,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
         const int cnp = (q.ngp + 255) / 256 * 256;
         e.cnp = cnp;
         e.qa = (char *)c->crt_q + qoff;
-        e.qy = (char *)c->crt_q + qoff + (size_t)kCrtPlanes * ct.n * kp * cnp;
-        qoff += (size_t)2 * kCrtPlanes * ct.n * kp * cnp;
+        e.qy = gram ? nullptr : (char *)c->crt_q + qoff + (size_t)kCrtPlanes * ct.n * kp * cnp;
+        qoff += (size_t)(gram ? 1 : 2) * kCrtPlanes * ct.n * kp * cnp;
         e.xa = c->crt_x + xoff;
         e.xy = c->crt_x + xoff + cnp;
         e.pa2 = (do_match && !gram) ? c->crt_parts + poff2 : nullptr;
