@@ -181,6 +181,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->crt_x) hipFree(c->crt_x);
   if (c->crt_parts) hipFree(c->crt_parts);
   if (c->gram_buf) hipFree(c->gram_buf);
+  if (c->gram_blas) rocblas_destroy_handle(c->gram_blas);
   if (c->gk_buf) hipFree(c->gk_buf);
   if (c->crt_res) hipFree(c->crt_res);
   if (c->d_citems) hipFree(c->d_citems);
@@ -721,23 +722,24 @@ int gram_build(exc_ctx *c) {
   double *lam = (double *)(b + o_l), *E = (double *)(b + o_e), *dsig = (double *)(b + o_s);
   int *info = (int *)(b + o_i);
   EXC_CHECK(hipMemcpyAsync(U, c->d.haa, sizeof(cplx) * nn, hipMemcpyDeviceToDevice, c->stream));
-  exc_ctx::EigWorker &W = c->ew[0];
-  W.s = c->stream;
-  if (!W.blas && rocblas_create_handle(&W.blas) != rocblas_status_success) {
-    W.blas = nullptr;
+  // a handle of its own with the library-managed workspace (the eigensolve workers' handles
+  // hold fixed workspaces sized for their own calls); the stream is drained below
+  if (!c->gram_blas && rocblas_create_handle(&c->gram_blas) != rocblas_status_success) {
+    c->gram_blas = nullptr;
     return EXC_ERR_HIP;
   }
-  if (rocblas_set_stream(W.blas, W.s) != rocblas_status_success) return EXC_ERR_HIP;
-  if (rocsolver_zheevd_strided_batched(W.blas, rocblas_evect_original, rocblas_fill_upper, n,
-                                       (rocblas_double_complex *)U, n, (rocblas_stride)n * n, lam, n, E, n, info,
-                                       na) != rocblas_status_success)
-    return EXC_ERR_HIP;
+  if (rocblas_set_stream(c->gram_blas, c->stream) != rocblas_status_success) return EXC_ERR_HIP;
+  const bool solved = rocsolver_zheevd_strided_batched(c->gram_blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                                       (rocblas_double_complex *)U, n, (rocblas_stride)n * n, lam, n,
+                                                       E, n, info, na) == rocblas_status_success;
   std::vector<double> hl((size_t)n * na);
-  std::vector<int> hi(na);
-  EXC_CHECK(hipMemcpyAsync(hl.data(), lam, 8 * hl.size(), hipMemcpyDeviceToHost, c->stream));
-  EXC_CHECK(hipMemcpyAsync(hi.data(), info, 4 * (size_t)na, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> hi(na, 1);
+  if (solved) {
+    EXC_CHECK(hipMemcpyAsync(hl.data(), lam, 8 * hl.size(), hipMemcpyDeviceToHost, c->stream));
+    EXC_CHECK(hipMemcpyAsync(hi.data(), info, 4 * (size_t)na, hipMemcpyDeviceToHost, c->stream));
+  }
   EXC_CHECK(hipStreamSynchronize(c->stream));
-  bool ok = true;
+  bool ok = solved;
   double lmin = 0.0, lmax = 0.0;
   for (int a = 0; a < na; ++a) ok = ok && hi[a] == 0;
   for (double v : hl) {

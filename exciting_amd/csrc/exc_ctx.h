@@ -91,6 +91,7 @@ struct exc_ctx {
   int crt_last_form, crt_last_nmod;   // what the last modular-split setup used
   void *gram_buf;
   size_t gram_bytes;
+  rocblas_handle gram_blas;   // its own handle: the eigensolve workers pin their workspaces
   bool gram_valid;
   double gram_sigma, gram_lmax;
   signed char *crt_res;

@@ -183,6 +183,39 @@ def test_crt_forms_eigenvalues(form, nmod):
     ctx.close()
 
 
+def test_crt_gram_after_eigensolve_and_new_h():
+    """The Gram form's eigen-decomposition runs on a rocBLAS handle of its own: after an
+    eigensolve on the same context (whose workers pin their workspaces) and a new set of h_a
+    (the context's tables uploaded again: the decomposition is redone), the split still takes
+    the Gram form and its eigenvalues stay within 1e-8 Ha of the FP64 oracle's."""
+    import torch
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 1)
+    cr = w.crystal
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(cr, stream=s)
+    H_ref, O_ref, A = R.setup(cr, w.kpoints[0])
+    H, O = ctx.hamiltonandoverlapsetup(w.kpoints[0], A)
+    ev, _ = ctx.eigen(H, O, 50)
+    assert np.abs(ev - _eig(H_ref, O_ref, 50)).max() <= 1e-8
+    flags = lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_CRT | lapw.EXC_BATCH_OLP_LEGENDRE | lapw.EXC_BATCH_GRAPH
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(cr, w.kpoints[:1])
+        for rep in range(2):
+            if rep:                                    # h_a set again: the table is rebuilt
+                hold = [np.asfortranarray(x, dtype=t) for x, t in ((cr.haa, np.complex128), (cr.loa, np.complex128),
+                        (cr.lolo, np.complex128), (cr.oalo, np.float64), (cr.ololo, np.float64))]
+                ctx._call("exc_ctx_set_mt", ctx.h, *[lapw._ptr(x) for x in hold])
+            ctx.setup_batched_dev(b, flags)
+            torch.cuda.synchronize()
+            assert ctx.crt_form()[0] == 1
+            de = np.abs(_eig(b.H[0].cpu().numpy().T, b.O[0].cpu().numpy().T) - _eig(H_ref, O_ref)).max()
+            print(f"Gram form after an eigensolve, pass {rep}: {de:.2e} Ha", flush=True)
+            assert de <= 1e-8
+    ctx.close()
+
+
 def _eig(H, O, nev=200):
     import scipy.linalg
     m = min(nev, H.shape[0])
