@@ -567,9 +567,11 @@ def main():
                  np.zeros((kp.ngp + cr.nlotot,) * 2, np.complex128, order="F")) for kp in w.kpoints]
 
     def step_host():
+        # as the Fortran driver runs them: match fills the caller's apwalm, the setup reads the
+        # library's device copy of it (the array is unchanged since), H and O come back
         for i, kp in enumerate(w.kpoints):
             ctx.match(kp, out=h_apw[i])
-            ctx.hamiltonandoverlapsetup(kp, h_apw[i], out=h_ho[i])
+            ctx.hamiltonandoverlapsetup(kp, h_apw[i], out=h_ho[i], resident=True)
 
     tuned = None
     args.table = None

@@ -161,6 +161,7 @@ int exc_ctx_destroy(exc_ctx *c) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  for (hipEvent_t e : c->copy_ev) hipEventDestroy(e);
   if (c->stage) hipFree(c->stage);
   if (c->apw_res) hipFree(c->apw_res);
   if (c->hstage) hipHostFree(c->hstage);
@@ -596,9 +597,9 @@ int exc_match(exc_ctx *c, int ngp, int ngkmax, const double *gpc, const double *
   if (ngp > 0) {
     int r2 = grow_host(&c->hstage, &c->hstage_bytes, b_apw);
     if (r2) return r2;
-    EXC_CHECK(hipMemcpyAsync(c->hstage, d_apw, b_apw, hipMemcpyDeviceToHost, c->stream));
-    EXC_CHECK(hipStreamSynchronize(c->stream));
-    copy_cols((char *)apwalm, 16 * nk, (const char *)c->hstage, 16 * nk, 16 * (size_t)ngp, ao * lmm * na);
+    if ((r2 = d2h_cols(c, (char *)apwalm, 16 * nk, (const char *)d_apw, 16 * nk, 16 * (size_t)ngp, ao * lmm * na,
+                       (char *)c->hstage)))
+      return r2;
   }
   EXC_CHECK(hipStreamSynchronize(c->stream));
   c->apw_res_valid = true;
@@ -697,11 +698,8 @@ int setup_host_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
   if (rc) return rc;
   // rows nmat+1..ldh of the caller's matrices are not part of the result: copy rows 1..nmat
   const size_t pitch = 16 * (size_t)ldh, width = 16 * (size_t)nmat;
-  if (h) EXC_CHECK(hipMemcpyAsync(hs, dh, b_h, hipMemcpyDeviceToHost, c->stream));
-  if (o) EXC_CHECK(hipMemcpyAsync(hs + (h ? b_h : 0), dout, b_h, hipMemcpyDeviceToHost, c->stream));
-  EXC_CHECK(hipStreamSynchronize(c->stream));
-  if (h) copy_cols((char *)h, pitch, hs, pitch, width, nmat);
-  if (o) copy_cols((char *)o, pitch, hs + (h ? b_h : 0), pitch, width, nmat);
+  if (h && (rc = d2h_cols(c, (char *)h, pitch, (const char *)dh, pitch, width, nmat, hs))) return rc;
+  if (o && (rc = d2h_cols(c, (char *)o, pitch, (const char *)dout, pitch, width, nmat, hs + (h ? b_h : 0)))) return rc;
   return exc_ctx_status(c);
 }
 

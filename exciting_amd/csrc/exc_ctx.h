@@ -51,6 +51,7 @@ struct exc_ctx {
   };
   std::vector<mark> marks;
   std::vector<hipEvent_t> pool;
+  std::vector<hipEvent_t> copy_ev;   // chunked device-to-host copies (d2h_cols)
   // grouped launches
   int bmt_variant;          // -1: built-in choice
   int bsplit_variant;
@@ -222,6 +223,36 @@ inline void copy_cols(char *dst, size_t dld, const char *src, size_t sld, size_t
   work(0, std::min(ncols, per));
   if (done < ncols) work(done, ncols);
   for (auto &x : th) x.join();
+}
+
+// Device columns (leading dimension sld bytes, rowbytes of each wanted) to the caller's host
+// columns (leading dimension dld) through the pinned staging buffer `stage` (ncols * sld
+// bytes), in chunks of about 16 MiB: every chunk's DMA is queued with an event after it, and
+// the host copy of a chunk runs as soon as its event has passed, while the later chunks'
+// DMAs are still in flight.  Returns after the last host copy.
+inline int d2h_cols(exc_ctx *c, char *dst, size_t dld, const char *dsrc, size_t sld, size_t rowbytes, size_t ncols,
+                    char *stage) {
+  constexpr size_t kMaxChunks = 64;
+  if (ncols == 0) return EXC_OK;
+  size_t per = std::max<size_t>(1, ((size_t)16 << 20) / sld);
+  per = std::max(per, (ncols + kMaxChunks - 1) / kMaxChunks);
+  const size_t nch = (ncols + per - 1) / per;
+  while (c->copy_ev.size() < nch) {
+    hipEvent_t e;
+    EXC_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->copy_ev.push_back(e);
+  }
+  for (size_t j = 0; j < nch; ++j) {
+    const size_t a = j * per, b = std::min(ncols, a + per);
+    EXC_CHECK(hipMemcpyAsync(stage + a * sld, dsrc + a * sld, (b - a) * sld, hipMemcpyDeviceToHost, c->stream));
+    EXC_CHECK(hipEventRecord(c->copy_ev[j], c->stream));
+  }
+  for (size_t j = 0; j < nch; ++j) {
+    const size_t a = j * per, b = std::min(ncols, a + per);
+    EXC_CHECK(hipEventSynchronize(c->copy_ev[j]));
+    copy_cols(dst + a * dld, dld, stage + a * sld, sld, rowbytes, b - a);
+  }
+  return EXC_OK;
 }
 
 // A larger buffer replaces the old one; work already queued on any stream may still read or

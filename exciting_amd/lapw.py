@@ -263,9 +263,11 @@ class Context:
                                      ok_positive=True)
         return apw
 
-    def hamiltonandoverlapsetup(self, kp, apwalm, ldh=None, out=None):
+    def hamiltonandoverlapsetup(self, kp, apwalm, ldh=None, out=None, resident=False):
         """(H, O) of k-point kp (hamiltonandoverlapsetup.f90:8), Fortran-ordered (ldh, nmat);
-        `out`: a preallocated pair of such arrays."""
+        `out`: a preallocated pair of such arrays.  resident: apwalm is the array the last
+        match filled, and the library's device copy of it is used (NULL to the entry, as the
+        Fortran driver passes it): no upload."""
         nmat = kp.ngp + self.nlotot
         ldh = nmat if ldh is None else ldh
         ngkmax = apwalm.shape[0]
@@ -277,7 +279,7 @@ class Context:
             for m in (H, O):
                 if m.shape != (ldh, nmat) or m.dtype != np.complex128 or not m.flags.f_contiguous:
                     raise ValueError("hamiltonandoverlapsetup: out must be Fortran-ordered complex128 (ldh, nmat)")
-        apw = _f(apwalm, np.complex128)
+        apw = None if resident else _f(apwalm, np.complex128)
         ig = _f(kp.igpig[:ngkmax], np.int32)
         vg = _f(kp.vgpc[:, :ngkmax], np.float64)
         self._call("exc_hamiltonandoverlapsetup", self.h, kp.ngp, ngkmax, _ptr(apw), _ptr(ig), _ptr(vg), ldh,
