@@ -719,6 +719,8 @@ int gram_build(exc_ctx *c) {
   cplx *T = (cplx *)b, *U = (cplx *)(b + o_u);
   double *lam = (double *)(b + o_l), *E = (double *)(b + o_e), *dsig = (double *)(b + o_s);
   int *info = (int *)(b + o_i);
+  c->gram_table = T;
+  c->gram_sig_dev = dsig;
   EXC_CHECK(hipMemcpyAsync(U, c->d.haa, sizeof(cplx) * nn, hipMemcpyDeviceToDevice, c->stream));
   // a handle of its own with the library-managed workspace (the eigensolve workers' handles
   // hold fixed workspaces sized for their own calls); the stream is drained below
@@ -865,14 +867,8 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   // competes with the MT kernel), the overlap and lo on a side branch for the Gram form, whose
   // residue GEMM leaves the vector units idle (+1% at workload 3; the merge waits for the branch)
   if (fv < 0) fv = gram ? 0 : 1;
-  const char *gram_b = (const char *)c->gram_buf;
-  const cplx *gram_t = gram ? (const cplx *)gram_b : nullptr;
-  const double *gram_s = nullptr;
-  if (gram) {
-    const size_t nn = (size_t)c->d.naa * c->d.naa * c->d.natm, nl = (size_t)c->d.naa * c->d.natm;
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    gram_s = (const double *)(gram_b + 2 * al(sizeof(cplx) * nn) + 2 * al(8 * nl) + al(4 * (size_t)c->d.natm));
-  }
+  const cplx *gram_t = gram ? c->gram_table : nullptr;
+  const double *gram_s = gram ? c->gram_sig_dev : nullptr;
   // moduli: the caller's count, else 10 with the Gram form (its rounding error is far below
   // the (A, Yt) form's: 5.7e-10 Ha at 10 moduli against 2.6e-9 at 11 on workload 3) and 11
   const int nmod = c->crt_nmod > 0 ? c->crt_nmod : (gram ? 10 : 11);

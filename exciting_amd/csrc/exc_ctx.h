@@ -93,6 +93,8 @@ struct exc_ctx {
   void *gram_buf;
   size_t gram_bytes;
   rocblas_handle gram_blas;   // its own handle: the eigensolve workers pin their workspaces
+  cplx *gram_table;           // in gram_buf: the Y stage's table conj(U) (Lambda + sigma)^(1/2)
+  double *gram_sig_dev;       // in gram_buf: sigma for the merge
   bool gram_valid;
   double gram_sigma, gram_lmax;
   signed char *crt_res;
