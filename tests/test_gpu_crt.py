@@ -216,6 +216,37 @@ def test_crt_gram_after_eigensolve_and_new_h():
     ctx.close()
 
 
+def test_crt_gram_form_left_for_a_large_shift():
+    """With h_a - 30 I (eigenvalues from about -31 to -7) the shift sigma would exceed half of
+    the largest |eigenvalue|: the built-in choice takes the (A, Yt) form at 11 moduli, and H
+    and O stay within the split's bar of the FP64 oracle's on the same h."""
+    import dataclasses
+    import torch
+    from exciting_amd import lapw
+    from oracle import lapw_ref as R
+    w = workload(1, 1)
+    cr = w.crystal
+    haa = cr.haa.copy()
+    for a in range(cr.natm):
+        haa[:, :, a] -= 30.0 * np.eye(haa.shape[0])
+    cr2 = dataclasses.replace(cr, haa=haa)
+    s = torch.cuda.Stream()
+    ctx = lapw.Context(cr2, stream=s)
+    with torch.cuda.stream(s):
+        b = lapw.DeviceBatch(cr2, w.kpoints[:1])
+        ctx.setup_batched_dev(b, lapw.EXC_BATCH_MATCH | lapw.EXC_BATCH_SPLIT_CRT | lapw.EXC_BATCH_OLP_LEGENDRE)
+        torch.cuda.synchronize()
+    form, nmod, _ = ctx.crt_form()
+    H_ref, O_ref, _ = R.setup(cr2, w.kpoints[0])
+    H, O = b.H[0].cpu().numpy().T, b.O[0].cpu().numpy().T
+    eh = np.linalg.norm(H - H_ref) / np.linalg.norm(H_ref)
+    eo = np.linalg.norm(O - O_ref) / np.linalg.norm(O_ref)
+    print(f"h - 30 I: form {form}, {nmod} moduli, |dH|/|H| {eh:.2e} |dO|/|O| {eo:.2e}", flush=True)
+    assert form == 0 and nmod == 11
+    assert eh <= 1e-9 and eo <= 1e-12
+    ctx.close()
+
+
 def _eig(H, O, nev=200):
     import scipy.linalg
     m = min(nev, H.shape[0])
