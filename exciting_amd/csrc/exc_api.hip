@@ -703,9 +703,13 @@ int setup_host_sel(exc_ctx *c, int ngp, int ngkmax, const double *apwalm, const 
   return exc_ctx_status(c);
 }
 
-// The Gram form's table (exc_launch_crt_gram_table) from the eigen-decomposition of every
-// h_a (rocSOLVER, strided batch), sigma = -min lambda (times 1 + 1e-6) when some lambda < 0,
-// else 0.  Once per change of h_a; failed decompositions leave the form unused.
+// The Gram form's table from every h_a (rocSOLVER, strided batches): the eigenvalues give
+// sigma = -min lambda (times 1 + 1e-6) when some lambda < 0, plus 1e-10 of the largest
+// |lambda| (h + sigma I is then positive definite beyond the eigensolver's rounding); the
+// table is the lower Cholesky factor of h + sigma I in the Y stage's summation order
+// (exc_launch_crt_gram_tri: a triangular table, whose Y stage skips the zero part), or, if
+// that factorization fails, conj(U) (Lambda + sigma)^(1/2) from the eigenvectors.  Once per
+// change of h_a; a failed eigensolve leaves the form unused.
 int gram_build(exc_ctx *c) {
   if (c->gram_valid) return EXC_OK;
   const int na = c->d.natm, n = c->d.naa;
@@ -713,12 +717,14 @@ int gram_build(exc_ctx *c) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t o_u = al(sizeof(cplx) * nn), o_l = o_u + al(sizeof(cplx) * nn), o_e = o_l + al(8 * (size_t)n * na);
   const size_t o_i = o_e + al(8 * (size_t)n * na), o_s = o_i + al(4 * (size_t)na);
-  int rc = grow(&c->gram_buf, &c->gram_bytes, o_s + 8);
+  const size_t o_m = o_s + al(8);
+  int rc = grow(&c->gram_buf, &c->gram_bytes, o_m + sizeof(cplx) * nn);
   if (rc) return rc;
   char *b = (char *)c->gram_buf;
   cplx *T = (cplx *)b, *U = (cplx *)(b + o_u);
   double *lam = (double *)(b + o_l), *E = (double *)(b + o_e), *dsig = (double *)(b + o_s);
   int *info = (int *)(b + o_i);
+  cplx *Mp = (cplx *)(b + o_m);
   c->gram_table = T;
   c->gram_sig_dev = dsig;
   EXC_CHECK(hipMemcpyAsync(U, c->d.haa, sizeof(cplx) * nn, hipMemcpyDeviceToDevice, c->stream));
@@ -747,10 +753,22 @@ int gram_build(exc_ctx *c) {
     lmin = std::min(lmin, v);
     lmax = std::max(lmax, fabs(v));
   }
-  c->gram_sigma = ok ? (lmin < 0.0 ? -lmin * (1.0 + 1e-6) : 0.0) : HUGE_VAL;
+  c->gram_sigma = ok ? (lmin < 0.0 ? -lmin * (1.0 + 1e-6) : 0.0) + 1e-10 * lmax : HUGE_VAL;
   c->gram_lmax = lmax;
+  c->gram_tri = false;
   if (ok) {
-    if ((rc = exc_launch_crt_gram_table(c->d, U, lam, c->gram_sigma, T, c->stream))) return rc;
+    if ((rc = exc_launch_crt_gram_perm(c->d, c->gram_sigma, Mp, c->stream))) return rc;
+    bool tri = rocsolver_zpotrf_strided_batched(c->gram_blas, rocblas_fill_lower, n, (rocblas_double_complex *)Mp, n,
+                                                (rocblas_stride)n * n, info, na) == rocblas_status_success;
+    if (tri) {
+      EXC_CHECK(hipMemcpyAsync(hi.data(), info, 4 * (size_t)na, hipMemcpyDeviceToHost, c->stream));
+      EXC_CHECK(hipStreamSynchronize(c->stream));
+      for (int a = 0; a < na; ++a) tri = tri && hi[a] == 0;
+    }
+    rc = tri ? exc_launch_crt_gram_tri(c->d, Mp, T, c->stream)
+             : exc_launch_crt_gram_table(c->d, U, lam, c->gram_sigma, T, c->stream);
+    if (rc) return rc;
+    c->gram_tri = tri;
     EXC_CHECK(hipMemcpyAsync(dsig, &c->gram_sigma, 8, hipMemcpyHostToDevice, c->stream));
     EXC_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -881,7 +899,12 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
   constexpr int kCrtPlanes = 2;                             // residue planes per operand and modulus
   const int kp = exc_crt_kpad(c->d.natm * c->d.naa);
   // batch signature: descriptors, flags and variants
-  int tail[8] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? nmod : -1) + 100 * ov, fv, gram};
+  // the Gram form's E stage on its triangular table: narrow column tiles skip more of it
+  // (64 x 64 three-product tiles, 5.7 against 6.7 ms at workload 3; the dense Y stage keeps
+  // 64 x 128, 8.8 against 9.1 ms)
+  const bool etri = gram && c->gram_tri;
+  const int ye = c->y_variant >= 0 ? c->y_variant : (etri ? 4 : 0);
+  int tail[10] = {nk, flags, bv, yv, crt ? cv : -1, (crt ? nmod : -1) + 100 * ov, fv, gram, etri, ye};
   std::vector<char> sig(sizeof(exc_kpoint_dev) * nk + sizeof(tail));
   memcpy(sig.data(), ks, sizeof(exc_kpoint_dev) * nk);
   memcpy(sig.data() + sizeof(exc_kpoint_dev) * nk, tail, sizeof(tail));
@@ -1111,8 +1134,11 @@ int exc_setup_batched_dev(exc_ctx *c, int nk, const exc_kpoint_dev *ks, int flag
       // the Gram form's E = A conj(U) (Lambda + sigma)^(1/2) is the Y stage on its table
       StageTimer t(c, capture ? -1 : EXC_STAGE_YSTAGE);
       exc_dev dy = c->d;
-      if (gram) dy.haa = (cplx *)gram_t;
-      if ((r = exc_launch_bystage(dy, nk, c->batch_maxngp, c->d_ks, yv, c->stream))) return r;
+      if (gram) {
+        dy.haa = (cplx *)gram_t;
+        dy.ytri = etri ? 1 : 0;
+      }
+      if ((r = exc_launch_bystage(dy, nk, c->batch_maxngp, c->d_ks, gram ? ye : yv, c->stream))) return r;
     }
     // the Gram form's merge reads O's APW block: the overlap runs before it
     const bool olp_first = gram && !fork;

@@ -67,6 +67,7 @@ struct exc_dev {
   // ValenceRelativity 'iora*' overlap terms (exc_ctx_set_iora); iora bits: 1 the APW-APW
   // factor I + h1aa, 2 h1loa / h1lolo, 4 the interstitial term with m2effig
   int iora;
+  int ytri;            // grouped Y stage: haa is the Gram form's triangular table (tile sums start at the column)
   double *olpl;        // [natm][lmaxmat+1][3]: lower Cholesky factor (L00, L10, L11) of I + h1aa(:,:,l,a)
   double *h1loa;       // [natm][nlomax][apwordmax], added to oalo
   double *h1lolo;      // [natm][nlomax][nlomax], added to ololo
@@ -121,7 +122,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
 int exc_launch_mt(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, const cplx *yt, int ldy,
                   const int *igpig, const double *vgpc, cplx *h, cplx *o, int ldh, int variant,
                   hipStream_t s);
-constexpr int kYstageVariants = 4;   // per-k and grouped Y stage (the same shapes)
+constexpr int kYstageVariants = 5;   // per-k and grouped Y stage (the same shapes)
 constexpr int kMtVariants = 3;       // per-k MT
 constexpr int kBmtVariants = 4;      // grouped MT
 // Grouped launches over nk k-points described by ks (device memory); maxngp bounds the grids.
@@ -174,6 +175,8 @@ int exc_launch_crt_merge(const exc_dev &d, const kdesc *ks, const mt_item *items
                          const signed char *res, int variant, const double *gsig, hipStream_t s);
 int exc_launch_crt_gram_table(const exc_dev &d, const cplx *U, const double *lam, double sigma, cplx *T,
                               hipStream_t s);
+int exc_launch_crt_gram_perm(const exc_dev &d, double sigma, cplx *Mp, hipStream_t s);
+int exc_launch_crt_gram_tri(const exc_dev &d, const cplx *L, cplx *T, hipStream_t s);
 // generalized eigensolve helpers (k_eigen.hip)
 struct eig_kdesc {   // one k-point of a grouped solve: nmat n, device arrays
   const cplx *h, *o;

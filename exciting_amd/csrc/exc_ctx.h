@@ -97,6 +97,7 @@ struct exc_ctx {
   double *gram_sig_dev;       // in gram_buf: sigma for the merge
   bool gram_valid;
   double gram_sigma, gram_lmax;
+  bool gram_tri;              // the table is the triangular (Cholesky) one
   signed char *crt_res;
   size_t crt_res_bytes;
   mt_item *d_citems;

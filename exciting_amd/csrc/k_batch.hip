@@ -42,7 +42,7 @@ int launch_bmatch_l(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipSt
 template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 __global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *__restrict__ ks, int natm, int naa,
                                                                    const cplx *__restrict__ haa,
-                                                                   const int *__restrict__ fperm) {
+                                                                   const int *__restrict__ fperm, int tri) {
   using T = YTileDMA<BM, BN, BK, NST, WR, WC, G3>;
   using St = typename T::Stage;
   EXC_DMA_STAGES(St, NST)
@@ -50,15 +50,19 @@ __global__ __launch_bounds__(64 * WR * WC) void bystage_dma_kernel(const kdesc *
   const kdesc k = ks[kk];
   const int row0 = blockIdx.x * BM;
   if (row0 >= k.ngp) return;
+  // tri: the table is zero at positions of fperm before its column (the Gram form's
+  // triangular factor), so this column tile's sum starts at its first column
+  const int col0 = blockIdx.y * BN;
   T::run(EXC_DMA_STAGE_ARGS(St), k.ngp, naa, k.apwalm + (size_t)k.ldg * naa * ias, k.ldg,
-         haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0, blockIdx.y * BN, fperm);
+         haa + (size_t)naa * naa * ias, k.yt + (size_t)k.ldy * naa * ias, k.ldy, row0, col0, fperm,
+         tri ? min(col0, naa) : 0);
 }
 
 template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 int launch_bystage_dma_t(const exc_dev &d, int nk, int maxngp, const kdesc *ks, hipStream_t s) {
   dim3 grid((maxngp + BM - 1) / BM, (d.naa + BN - 1) / BN, d.natm * nk);
   hipLaunchKernelGGL((bystage_dma_kernel<BM, BN, BK, NST, WR, WC, G3>), grid, dim3(64 * WR * WC), 0, s, ks, d.natm, d.naa,
-                     d.haa, d.fperm);
+                     d.haa, d.fperm, d.ytri);
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
@@ -148,6 +152,7 @@ int exc_launch_bystage(const exc_dev &d, int nk, int maxngp, const kdesc *ks, in
     case 1: return launch_bystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, nk, maxngp, ks, s);
     case 2: return launch_bystage_dma_t<64, 128, 8, 2, 2, 4>(d, nk, maxngp, ks, s);
     case 3: return launch_bystage_dma_t<64, 64, 16, 2, 2, 4>(d, nk, maxngp, ks, s);
+    case 4: return launch_bystage_dma_t<64, 64, 8, 2, 2, 4, true>(d, nk, maxngp, ks, s);   // narrow, Gauss's form
     default: return EXC_ERR_ARG;
   }
 }

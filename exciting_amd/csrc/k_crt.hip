@@ -865,3 +865,47 @@ int exc_launch_crt_gram_table(const exc_dev &d, const cplx *U, const double *lam
   EXC_CHECK(hipGetLastError());
   return EXC_OK;
 }
+
+namespace {
+// M_p = P (h + sigma I) P^T per atom: h's rows and columns in the Y stage's summation order
+__global__ __launch_bounds__(256) void crt_gram_perm_kernel(int naa, const cplx *__restrict__ h,
+                                                            const int *__restrict__ fperm, double sigma,
+                                                            cplx *__restrict__ Mp) {
+  const int a = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y, at = blockIdx.z;
+  if (a >= naa) return;
+  const size_t o = (size_t)at * naa * naa;
+  cplx v = h[o + fperm[a] + (size_t)naa * fperm[b]];
+  if (a == b) v.re += sigma;
+  Mp[o + a + (size_t)naa * b] = v;
+}
+
+// T(r, fperm[kk]) = conj(L(kk, r)) for kk >= r and 0 above, L the lower Cholesky factor of M_p:
+// the Y stage on T forms E = A_p conj(L), conj(E) E^T = conj(A) (h + sigma) A^T, and the sum of
+// a column tile starting at r0 may start at position r0 of the order
+__global__ __launch_bounds__(256) void crt_gram_tri_kernel(int naa, const cplx *__restrict__ L,
+                                                           const int *__restrict__ fperm, cplx *__restrict__ T) {
+  const int r = blockIdx.x * 256 + threadIdx.x, kk = blockIdx.y, at = blockIdx.z;
+  if (r >= naa) return;
+  const size_t o = (size_t)at * naa * naa;
+  cplx v = {0.0, 0.0};
+  if (kk >= r) {
+    const cplx l = L[o + kk + (size_t)naa * r];
+    v = cplx{l.re, -l.im};
+  }
+  T[o + r + (size_t)naa * fperm[kk]] = v;
+}
+}  // namespace
+
+int exc_launch_crt_gram_perm(const exc_dev &d, double sigma, cplx *Mp, hipStream_t s) {
+  hipLaunchKernelGGL(crt_gram_perm_kernel, dim3((d.naa + 255) / 256, d.naa, d.natm), dim3(256), 0, s, d.naa, d.haa,
+                     d.fperm, sigma, Mp);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_launch_crt_gram_tri(const exc_dev &d, const cplx *L, cplx *T, hipStream_t s) {
+  hipLaunchKernelGGL(crt_gram_tri_kernel, dim3((d.naa + 255) / 256, d.naa, d.natm), dim3(256), 0, s, d.naa, L, d.fperm,
+                     T);
+  EXC_CHECK(hipGetLastError());
+  return EXC_OK;
+}

@@ -71,7 +71,8 @@ int launch_mt_dma_t(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cons
 
 // Y-stage variants (the same shapes as the grouped Y stage's): 0: 64 x 128 tile, BK 8, Gauss's
 // three real products per complex product; 1: the same with BK 16; 2: 64 x 128, BK 8, four
-// products; 3: 64 x 64, BK 16, four DMA stages, four products (the per-k default)
+// products; 3: 64 x 64, BK 16, four DMA stages, four products (the per-k default); 4: 64 x 64,
+// BK 8, three products (narrow column tiles: more of the Gram form's triangular table skipped)
 int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cplx *yt, int ldy, int variant,
                       hipStream_t s) {
   if (ngp <= 0) return EXC_OK;
@@ -80,6 +81,7 @@ int exc_launch_ystage(const exc_dev &d, int ngp, const cplx *apwalm, int ldg, cp
     case 1: return launch_ystage_dma_t<64, 128, 16, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 2: return launch_ystage_dma_t<64, 128, 8, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
     case 3: return launch_ystage_dma_t<64, 64, 16, 2, 2, 4>(d, ngp, apwalm, ldg, yt, ldy, s);
+    case 4: return launch_ystage_dma_t<64, 64, 8, 2, 2, 4, true>(d, ngp, apwalm, ldg, yt, ldy, s);
     default: return EXC_ERR_ARG;
   }
 }

@@ -350,16 +350,17 @@ struct MtTileDMA {
 };
 
 // One (row tile, column tile) of Yt_a = A_a h_a^T (no conjugation), summed over f in the
-// order fperm; one tile per workgroup.
+// order fperm; one tile per workgroup.  kfirst: the sum starts at position kfirst of that
+// order (a table whose rows before it are zero for this tile's columns).
 template <int BM, int BN, int BK, int NST, int WR, int WC, bool G3 = false>
 struct YTileDMA {
   using C = DmaCore<BM, BN, BK, NST, WR, WC, false, G3>;
   using Stage = typename C::Stage;
   __device__ static void run(Stage *s0, Stage *s1, Stage *s2, Stage *s3, int ngp, int naa,
                              const cplx *__restrict__ A, int ldg, const cplx *__restrict__ ha, cplx *__restrict__ Y,
-                             int ldy, int row0, int col0, const int *__restrict__ fperm) {
+                             int ldy, int row0, int col0, const int *__restrict__ fperm, int kfirst = 0) {
     d4_t cr[C::MB][C::NB], ci[C::MB][C::NB];
-    C::run(s0, s1, s2, s3, A, ldg, row0, ngp, ha, naa, col0, naa, naa, fperm, cr, ci);
+    C::run(s0, s1, s2, s3, A, ldg, row0, ngp, ha, naa, col0, naa, naa - kfirst, fperm + kfirst, cr, ci);
     const int wave = threadIdx.x >> 6, wr = wave / WC, wc = wave % WC;
 #pragma unroll
     for (int i = 0; i < C::MB; ++i)

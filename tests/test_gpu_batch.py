@@ -42,7 +42,7 @@ def _clear(batch):
 
 
 GAUSS_BMT = (0, 2, 3)   # grouped MT variants with Gauss's three real products per complex product
-GAUSS_Y = (0, 1)  # Y variants likewise
+GAUSS_Y = (0, 1, 4)  # Y variants likewise
 
 
 def _same(batch, ref, what, rtol=0.0):

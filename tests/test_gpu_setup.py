@@ -102,7 +102,7 @@ def test_device_entry_points_match_host():
     assert np.array_equal(O.cpu().numpy().T, O_h)
 
 
-GAUSS_Y = (0, 1)   # per-k Y and MT variants with Gauss's three real products per complex product
+GAUSS_Y = (0, 1, 4)   # per-k Y and MT variants with Gauss's three real products per complex product
 GAUSS_MT = (0,)
 FOUR_PRODUCT = {"mt": 1, "ystage": 3}
 
